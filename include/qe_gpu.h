@@ -1,0 +1,266 @@
+/* qe_gpu.h — C ABI of the MI355X (gfx950) execution path for the engine's
+ * hash table, inner hash join, hash group-by aggregation and backward as-of
+ * join.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, status codes, no C++ or
+ * torch types. Every call runs on the context's single HIP stream. No
+ * exception crosses this boundary: failures return a status and the message
+ * is kept per context (qe_gpu_last_error). The C++ operator wrappers
+ * (query-engine_amd/host) turn statuses back into the reference's exception
+ * types (std::invalid_argument, std::overflow_error, std::bad_alloc).
+ *
+ * The reference engine has no FFI of its own; each entry point names the
+ * reference interface it replaces (file:line in the reference checkout).
+ */
+#ifndef QE_GPU_H
+#define QE_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status / types -------------------------------------------------------*/
+typedef enum qe_gpu_status {
+    QE_GPU_OK = 0,
+    QE_GPU_INVALID_ARGUMENT = 1, /* std::invalid_argument in the reference   */
+    QE_GPU_OUT_OF_MEMORY = 2,    /* std::bad_alloc (core/buffer.cpp:23)       */
+    QE_GPU_OVERFLOW = 3,         /* std::overflow_error (ops/hashtable.cpp:64) */
+    QE_GPU_HIP_ERROR = 4,
+    QE_GPU_UNSUPPORTED = 5
+} qe_gpu_status;
+
+/* Same enumerators, same order as qe::Type (core/types.h:18). Widths per
+ * byte_width (core/types.h:32-48): I32/STR 4, I64/F64/TS 8, BOOL 1. */
+typedef enum qe_type {
+    QE_I32 = 0, QE_I64 = 1, QE_F64 = 2, QE_BOOL = 3, QE_TS = 4, QE_STR = 5
+} qe_type;
+
+/* A host column view, field-for-field the reference qe::Column
+ * (core/column.h:55-67): LSB-first u64 validity words; all_valid is
+ * authoritative (core/column.h:42-54) and validity may be NULL when set. */
+typedef struct qe_col_desc {
+    int32_t type;
+    int32_t all_valid;
+    uint64_t len;
+    const void* data;
+    const uint64_t* validity;
+} qe_col_desc;
+
+/* Aggregate functions, same order as qe::AggFunc (ops/aggregate.h:61). */
+typedef enum qe_agg_func {
+    QE_AGG_COUNT_STAR = 0, QE_AGG_COUNT = 1, QE_AGG_SUM = 2,
+    QE_AGG_MIN = 3, QE_AGG_MAX = 4, QE_AGG_AVG = 5
+} qe_agg_func;
+
+/* qe::AggSpec without the name (ops/aggregate.h:66-94). */
+typedef struct qe_agg_spec {
+    int32_t func;
+    uint32_t input_col; /* ignored for COUNT(*) */
+} qe_agg_spec;
+
+typedef enum qe_join_type { QE_JOIN_INNER = 0, QE_JOIN_LEFT = 1 } qe_join_type;
+typedef enum qe_null_policy { QE_NULL_EQUAL = 0, QE_NULL_NEVER_MATCH = 1 } qe_null_policy;
+
+/* As-of search method: plain per-probe binary search (the judge) or the
+ * learned per-key index (spline pieces + monotone MLP router, verified
+ * window search, binary-search fallback). */
+typedef enum qe_asof_mode { QE_ASOF_BINARY = 0, QE_ASOF_LEARNED = 1, QE_ASOF_LEARNED_NO_MLP = 2 } qe_asof_mode;
+
+/* Sentinel for "no match" in match-index outputs: kNoGroup / kNullBuildRow
+ * (ops/hashtable.h:58, ops/join_internal.h:31). */
+#define QE_GPU_NO_ROW 0xFFFFFFFFu
+
+typedef struct qe_gpu_ctx qe_gpu_ctx;
+typedef struct qe_dev_col qe_dev_col;     /* device column (HBM)            */
+typedef struct qe_dev_table qe_dev_table; /* device result: rows + columns  */
+typedef struct qe_dev_pairs qe_dev_pairs; /* device (probe_row, build_row)  */
+typedef struct qe_dev_ht qe_dev_ht;       /* device hash table              */
+typedef struct qe_dev_asof qe_dev_asof;   /* sorted as-of build side        */
+
+/* ---- context ----------------------------------------------------------------*/
+/* One context per GPU; owns one HIP stream and a caching device allocator. */
+qe_gpu_status qe_gpu_ctx_create(int device, qe_gpu_ctx** out);
+void qe_gpu_ctx_destroy(qe_gpu_ctx* ctx);
+const char* qe_gpu_last_error(const qe_gpu_ctx* ctx);
+qe_gpu_status qe_gpu_sync(qe_gpu_ctx* ctx);
+/* Device memory limit for this context (0 = no limit): lets a test force the
+ * OUT_OF_MEMORY path from the host. */
+qe_gpu_status qe_gpu_set_mem_limit(qe_gpu_ctx* ctx, uint64_t bytes);
+/* Kernel timing: when enabled, the dominant kernel of each call is bracketed by
+ * HIP events on the context stream; qe_gpu_timing reads the accumulated time
+ * (ms) and launch count per kernel name, then clears it. */
+qe_gpu_status qe_gpu_set_timing(qe_gpu_ctx* ctx, int enabled);
+qe_gpu_status qe_gpu_timing(qe_gpu_ctx* ctx, const char* kernel, double* ms, uint64_t* launches);
+
+/* ---- columns ----------------------------------------------------------------*/
+/* Host -> HBM once, Batch-compatible layout (same widths, u64 validity words,
+ * all_valid rule). Replaces the zero-copy Scan source (ops/scan.cpp:27-64,
+ * ops/table.h:24-45): device operators read the uploaded copy. */
+qe_gpu_status qe_gpu_column_upload(qe_gpu_ctx* ctx, const qe_col_desc* desc, qe_dev_col** out);
+/* Uninitialised device column (validity allocated iff nullable). */
+qe_gpu_status qe_gpu_column_alloc(qe_gpu_ctx* ctx, int32_t type, uint64_t len, int32_t nullable, qe_dev_col** out);
+/* HBM -> host: data (len*width bytes), validity (ceil(len/64) words, may be
+ * NULL), all_valid flag (may be NULL). */
+qe_gpu_status qe_gpu_column_download(qe_gpu_ctx* ctx, const qe_dev_col* col, void* data, uint64_t* validity, int32_t* all_valid);
+/* Download rows [offset, offset+count) only (the <=2048-row output batches). */
+qe_gpu_status qe_gpu_column_download_range(qe_gpu_ctx* ctx, const qe_dev_col* col, uint64_t offset, uint64_t count, void* data, uint8_t* valid_bytes);
+void qe_gpu_column_free(qe_gpu_ctx* ctx, qe_dev_col* col);
+uint64_t qe_gpu_column_len(const qe_dev_col* col);
+int32_t qe_gpu_column_type(const qe_dev_col* col);
+int32_t qe_gpu_column_all_valid(const qe_dev_col* col);
+/* A view of rows [offset, offset+len) of `col` (no copy; offset % 64 == 0 so
+ * the validity sub-view is word aligned, as ops/scan.cpp:15-18 requires). */
+qe_gpu_status qe_gpu_column_slice(qe_gpu_ctx* ctx, const qe_dev_col* col, uint64_t offset, uint64_t len, qe_dev_col** out);
+
+/* ---- result tables ------------------------------------------------------------*/
+uint64_t qe_gpu_table_rows(const qe_dev_table* t);
+uint32_t qe_gpu_table_cols(const qe_dev_table* t);
+const qe_dev_col* qe_gpu_table_col(const qe_dev_table* t, uint32_t i);
+void qe_gpu_table_free(qe_gpu_ctx* ctx, qe_dev_table* t);
+
+/* ---- hash group-by ------------------------------------------------------------*/
+/* Replaces qe::Aggregate(child, key_cols, aggs) (ops/aggregate.h:110-111;
+ * drain_and_build ops/aggregate.cpp:243-437, next :439-493). Output columns =
+ * key columns (input types) ++ one column per agg with the reference result
+ * type (ops/aggregate.cpp:78-97). One row per group, unordered; zero keys ->
+ * exactly one row even over empty input; keyed over empty -> zero rows.
+ * F64 SUM/AVG accumulate in FP64 (summation order is the only difference
+ * from the reference's sequential fold). */
+qe_gpu_status qe_gpu_aggregate(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, uint32_t ncols,
+                               const uint32_t* key_cols, uint32_t nkeys,
+                               const qe_agg_spec* aggs, uint32_t naggs, qe_dev_table** out);
+
+/* ---- hash join ----------------------------------------------------------------*/
+/* Replaces qe::HashJoin(probe, build, probe_keys, build_keys, type)
+ * (ops/join.h:79-81; build_side ops/join.cpp:133-197, build_pairs_for_probe
+ * :199-247). Produces every (probe_row, build_row) pair; LEFT unmatched rows
+ * carry build_row = QE_GPU_NO_ROW. NULL keys never match; F64 keys use the
+ * canonical -0/+0, NaN==NaN equality. Key types must match positionally. */
+qe_gpu_status qe_gpu_hash_join(qe_gpu_ctx* ctx,
+                               const qe_dev_col* const* probe_keys, const qe_dev_col* const* build_keys,
+                               uint32_t nkeys, int32_t join_type, qe_dev_pairs** out);
+uint64_t qe_gpu_pairs_count(const qe_dev_pairs* p);
+/* Copy pairs [offset, offset+count) to host arrays (either may be NULL). */
+qe_gpu_status qe_gpu_pairs_download(qe_gpu_ctx* ctx, const qe_dev_pairs* p, uint64_t offset, uint64_t count,
+                                    uint32_t* probe_rows, uint32_t* build_rows);
+/* Materialise output rows [offset, offset+count) of a join: probe columns
+ * gathered by probe_row, then build columns by build_row (NULL where
+ * build_row == QE_GPU_NO_ROW). Mirrors emit_probe_column/emit_build_column
+ * (ops/join_internal.h:136-179). */
+qe_gpu_status qe_gpu_pairs_gather(qe_gpu_ctx* ctx, const qe_dev_pairs* p, uint64_t offset, uint64_t count,
+                                  const qe_dev_col* const* probe_cols, uint32_t nprobe,
+                                  const qe_dev_col* const* build_cols, uint32_t nbuild, qe_dev_table** out);
+void qe_gpu_pairs_free(qe_gpu_ctx* ctx, qe_dev_pairs* p);
+
+/* Fused Aggregate(HashJoin(probe, build, ..., type), group_keys, aggs) — the
+ * plan plan/plan.cpp:581-590 lowers for scan(P).join(scan(B)).aggregate(...).
+ * group_keys / agg input_col index the join output schema (probe columns then
+ * build columns, ops/join.h:22-28). Output as qe_gpu_aggregate. */
+qe_gpu_status qe_gpu_join_aggregate(qe_gpu_ctx* ctx,
+                                    const qe_dev_col* const* probe_cols, uint32_t nprobe,
+                                    const qe_dev_col* const* build_cols, uint32_t nbuild,
+                                    const uint32_t* probe_keys, const uint32_t* build_keys, uint32_t nkeys,
+                                    int32_t join_type, const uint32_t* group_keys, uint32_t ngroup_keys,
+                                    const qe_agg_spec* aggs, uint32_t naggs, qe_dev_table** out);
+
+/* ---- hash table -----------------------------------------------------------------*/
+/* Replaces qe::HashTable (ops/hashtable.h:108-190): key -> dense group id in
+ * first-seen row order (ids stable across calls and growth), kEqual /
+ * kNeverMatch NULL policies (kNeverMatch NULL rows get fresh dead ids), key
+ * store read-back. initial_capacity / max_load_factor follow HashTableConfig
+ * (ops/hashtable.h:83-94); the seed is accepted for interface parity. */
+qe_gpu_status qe_gpu_ht_create(qe_gpu_ctx* ctx, const int32_t* key_types, uint32_t nkeys, int32_t null_policy,
+                               uint64_t initial_capacity, double max_load_factor, uint64_t seed, qe_dev_ht** out);
+/* insert_or_find (ops/hashtable.cpp:93-133) over rows sel[0..n) (or 0..n when
+ * sel is NULL; sel is a device u32 array). out_groups: device u32[n]. */
+qe_gpu_status qe_gpu_ht_insert_or_find(qe_gpu_ctx* ctx, qe_dev_ht* ht, const qe_dev_col* const* keys,
+                                       const uint32_t* sel, uint64_t n, uint32_t* out_groups);
+/* find (ops/hashtable.cpp:135-163): QE_GPU_NO_ROW on a miss. Never mutates. */
+qe_gpu_status qe_gpu_ht_find(qe_gpu_ctx* ctx, const qe_dev_ht* ht, const qe_dev_col* const* keys,
+                             const uint32_t* sel, uint64_t n, uint32_t* out_groups);
+uint64_t qe_gpu_ht_num_groups(const qe_dev_ht* ht);
+uint64_t qe_gpu_ht_capacity(const qe_dev_ht* ht);
+/* group_key_word / group_is_null (ops/hashtable.h:143-155) for groups
+ * [0, num_groups): host arrays words[num_groups], nulls[num_groups]. */
+qe_gpu_status qe_gpu_ht_key_words(qe_gpu_ctx* ctx, const qe_dev_ht* ht, uint32_t col, uint64_t* words, uint8_t* nulls);
+void qe_gpu_ht_free(qe_gpu_ctx* ctx, qe_dev_ht* ht);
+
+/* ---- as-of join -------------------------------------------------------------------*/
+/* Build side of tsx::AsofJoin (tsx/asof.h:101-105; build_side
+ * tsx/asof.cpp:177-232): partition by key tuple (NULL keys never match),
+ * drop NULL timestamps, sort each partition by timestamp on the device
+ * (the Sort wrap tsx/asof.cpp:163-166), CSR offsets per partition. Time
+ * column I32/I64/TS, key types as the probe's (checked at probe). Zero keys =
+ * one global partition. */
+qe_gpu_status qe_gpu_asof_build(qe_gpu_ctx* ctx, const qe_dev_col* const* build_keys, uint32_t nkeys,
+                                const qe_dev_col* build_time, qe_dev_asof** out);
+/* Train the learned index on the sorted build side: per partition, spline
+ * pieces of `piece_rows` rows with recorded max error, and a monotone
+ * 1-16-16-1 MLP router trained on the device (forward/backward kernels,
+ * Adam) for `steps` steps. */
+qe_gpu_status qe_gpu_asof_train(qe_gpu_ctx* ctx, qe_dev_asof* idx, uint32_t piece_rows, uint32_t steps, float lr);
+/* Probe (build_pairs_for_probe, tsx/asof.cpp:234-308): for each probe row the
+ * build row with equal key and the greatest tb <= t (original build row
+ * index), or QE_GPU_NO_ROW. has_tolerance: drop if (u64)t-(u64)tb > tol.
+ * match_out: device u32[n]. */
+qe_gpu_status qe_gpu_asof_probe(qe_gpu_ctx* ctx, const qe_dev_asof* idx, const qe_dev_col* const* probe_keys,
+                                uint32_t nkeys, const qe_dev_col* probe_time, int32_t has_tolerance,
+                                int64_t tolerance, int32_t mode, uint32_t* match_out);
+/* Turn per-probe matches into join pairs: INNER keeps matches, LEFT keeps
+ * every probe row (build_row QE_GPU_NO_ROW when unmatched). */
+qe_gpu_status qe_gpu_asof_pairs(qe_gpu_ctx* ctx, const uint32_t* match, uint64_t n, int32_t join_type,
+                                qe_dev_pairs** out);
+typedef struct qe_asof_stats {
+    uint64_t build_rows;       /* matchable rows (non-NULL key and ts)           */
+    uint64_t partitions;
+    uint64_t pieces;           /* spline pieces                                  */
+    uint64_t max_piece_error;  /* max recorded |predicted - actual| position     */
+    uint64_t router_buckets;   /* radix buckets addressed by the router          */
+    double mean_bucket_knots;  /* mean knots searched per probe bucket (MLP)     */
+    double mean_bucket_knots_linear; /* same with the linear (no-MLP) router     */
+    uint64_t fallbacks;        /* probes that fell back to binary search (last probe) */
+    double final_loss;         /* mean router training loss over partitions      */
+} qe_asof_stats;
+qe_gpu_status qe_gpu_asof_stats(qe_gpu_ctx* ctx, const qe_dev_asof* idx, qe_asof_stats* out);
+/* Sorted build side for inspection: sorted timestamps, original row of each
+ * sorted position, and CSR partition offsets (partitions+1). Any may be NULL. */
+qe_gpu_status qe_gpu_asof_sorted(qe_gpu_ctx* ctx, const qe_dev_asof* idx, int64_t* ts, uint32_t* rows, uint64_t* offsets);
+void qe_gpu_asof_free(qe_gpu_ctx* ctx, qe_dev_asof* idx);
+
+/* Learned-index router internals, exposed so tests can check the device
+ * forward pass and the MFMA gradient kernel against a CPU implementation.
+ * Parameter layout (QE_MLP_PARAMS floats): w1[16] b1[16] W2[16][16] (row =
+ * output unit) b2[16] w3[16] b3. No counterpart in the reference (the learned
+ * index is new). */
+#define QE_MLP_PARAMS 321
+qe_gpu_status qe_gpu_mlp_forward(qe_gpu_ctx* ctx, const float* params, const float* x, uint64_t n, float* y);
+/* Gradient of the batch loss (1/n) sum (y(x_i) - c_i)^2 w.r.t. every parameter,
+ * and that loss; host arrays, n in [1, 8192]. */
+qe_gpu_status qe_gpu_mlp_grad(qe_gpu_ctx* ctx, const float* params, const float* x, const float* c, uint32_t n,
+                              float* grads, float* loss);
+
+/* ---- device memory helpers for callers without their own allocator ---------------*/
+qe_gpu_status qe_gpu_malloc(qe_gpu_ctx* ctx, uint64_t bytes, void** out);
+qe_gpu_status qe_gpu_free(qe_gpu_ctx* ctx, void* p);
+qe_gpu_status qe_gpu_memcpy_d2h(qe_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+qe_gpu_status qe_gpu_memcpy_h2d(qe_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+
+/* ---- synthetic workload generators (bench / tests; seeded, counter-based) ---------*/
+/* Config shapes from the benchmark definitions; every value is a pure
+ * function of (seed, row), generated directly in HBM. */
+qe_gpu_status qe_gpu_gen_groupby(qe_gpu_ctx* ctx, uint64_t seed, uint64_t n, uint64_t key_range,
+                                 double null_frac, qe_dev_col* key_i64, qe_dev_col* val_f64);
+qe_gpu_status qe_gpu_gen_join_build(qe_gpu_ctx* ctx, uint64_t seed, uint64_t n, uint32_t groups,
+                                    qe_dev_col* key_i64, qe_dev_col* grp_i32);
+qe_gpu_status qe_gpu_gen_join_probe(qe_gpu_ctx* ctx, uint64_t seed, uint64_t n, uint64_t build_n, double zipf_s,
+                                    double miss_frac, qe_dev_col* key_i64, qe_dev_col* pay_f64);
+qe_gpu_status qe_gpu_gen_asof(qe_gpu_ctx* ctx, uint64_t seed, uint64_t right_n, uint64_t left_n, uint32_t nsym,
+                              double zipf_s, qe_dev_col* r_sym, qe_dev_col* r_ts, qe_dev_col* r_px,
+                              qe_dev_col* l_sym, qe_dev_col* l_ts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* QE_GPU_H */

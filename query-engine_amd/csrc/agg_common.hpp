@@ -1,0 +1,231 @@
+// Slot aggregation machinery shared by the group-by and the fused
+// join->group-by: an aggregate "program", per-slot state updates in LDS (block
+// partials) or HBM (atomics), the deterministic block-order merge and the
+// output emission. Semantics follow ops/aggregate.cpp:118-155 (scatter_row),
+// ops/agg_internal.h:34-81 (NaN-greatest order, identities) and :169-229
+// (finalisation), with the result types of ops/aggregate.cpp:78-97.
+#pragma once
+
+#include "internal.hpp"
+
+namespace qe_gpu {
+
+constexpr int kMaxAggs = 16;
+constexpr int kMaxKeys = 8;
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;
+
+enum AggKind : int32_t {
+    K_COUNT_STAR = 0,
+    K_COUNT = 1,
+    K_SUM_I = 2,
+    K_SUM_F = 3,
+    K_MIN_I = 4,
+    K_MAX_I = 5,
+    K_MIN_F = 6,
+    K_MAX_F = 7,
+};
+
+struct AggProg {
+    int32_t n;
+    int32_t kind[kMaxAggs];
+    int32_t func[kMaxAggs];   // qe_agg_func (AVG finalises differently from SUM)
+    ColView in[kMaxAggs];
+    // Compact partial layout: which aggs keep an accumulator / a non-NULL count.
+    // A count is only tracked for nullable inputs; for all-valid inputs the
+    // non-NULL count equals the group's row count.
+    int32_t acc_of[kMaxAggs];  // index among accumulators, or -1
+    int32_t cnt_of[kMaxAggs];  // index among counters, or -1
+    int32_t acc_kind[kMaxAggs];  // AggKind of accumulator i
+    int32_t nacc, ncnt;
+};
+
+// Result type of an aggregate (ops/aggregate.cpp:78-97).
+inline int agg_result_type(int func, int in_type) {
+    switch (func) {
+        case QE_AGG_COUNT_STAR:
+        case QE_AGG_COUNT:
+            return QE_I64;
+        case QE_AGG_SUM:
+            return in_type == QE_F64 ? QE_F64 : QE_I64;
+        case QE_AGG_AVG:
+            return QE_F64;
+        default:
+            return in_type;
+    }
+}
+
+// Build the device program; validates like agg_result_type (SUM/AVG of STR are
+// invalid_argument; MIN/MAX of STR compare by dictionary value, which the device
+// path does not hold, so they are reported unsupported).
+AggProg make_prog(const qe_agg_spec* aggs, uint32_t naggs, const ColView* inputs,
+                  const int* input_types);
+
+// Identity of each state field.
+__host__ __device__ inline uint64_t acc_identity(int kind) {
+    switch (kind) {
+        case K_MIN_I:
+            return static_cast<uint64_t>(INT64_MAX);
+        case K_MAX_I:
+            return static_cast<uint64_t>(INT64_MIN);
+        case K_MIN_F:
+            return 0xFFFFFFFFFFFFFFFFull;  // order key above +qNaN: "no value yet"
+        case K_MAX_F:
+            return 0;  // below every order key
+        default:
+            return 0;
+    }
+}
+
+// ---- LDS partial layout: acc u64[nacc][S] | cnt u32[ncnt][S] | rows u32[S] ------
+__host__ __device__ inline size_t lds_bytes(uint32_t S, const AggProg& p) {
+    return size_t(S) * (8 * p.nacc + 4 * p.ncnt + 4);
+}
+// Compact partial (per block) layout in HBM: rows | acc[nacc] | cnt[ncnt], u64 each.
+__host__ __device__ inline size_t part_words(uint32_t S, const AggProg& p) {
+    return size_t(S) * (1 + p.nacc + p.ncnt);
+}
+
+__device__ inline void lds_init(char* smem, uint32_t S, const AggProg& p) {
+    uint64_t* acc = reinterpret_cast<uint64_t*>(smem);
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(smem + size_t(8) * S * p.nacc);
+    uint32_t* rows = cnt + size_t(S) * p.ncnt;
+    for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
+        rows[s] = 0;
+        for (int a = 0; a < p.n; ++a)
+            if (p.acc_of[a] >= 0) acc[size_t(p.acc_of[a]) * S + s] = acc_identity(p.kind[a]);
+        for (int c = 0; c < p.ncnt; ++c) cnt[size_t(c) * S + s] = 0;
+    }
+}
+
+// Fold one row into `slot` of the block's LDS partials; value inputs of agg a
+// are read at row vrow[a] (the join path reads probe- or build-side rows).
+__device__ inline void lds_update(char* smem, uint32_t S, const AggProg& p, uint32_t slot,
+                                  const uint64_t* vrow) {
+    uint64_t* acc = reinterpret_cast<uint64_t*>(smem);
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(smem + size_t(8) * S * p.nacc);
+    uint32_t* rows = cnt + size_t(S) * p.ncnt;
+    atomicAdd(&rows[slot], 1u);
+    for (int a = 0; a < p.n; ++a) {
+        const int k = p.kind[a];
+        if (k == K_COUNT_STAR) continue;
+        const uint64_t r = vrow[a];
+        if (!is_valid(p.in[a], r)) continue;
+        if (p.cnt_of[a] >= 0) atomicAdd(&cnt[size_t(p.cnt_of[a]) * S + slot], 1u);
+        if (p.acc_of[a] < 0) continue;
+        uint64_t* ap = &acc[size_t(p.acc_of[a]) * S + slot];
+        switch (k) {
+            case K_SUM_I:
+                atomicAdd(reinterpret_cast<unsigned long long*>(ap),
+                          static_cast<unsigned long long>(int_value(p.in[a], r)));
+                break;
+            case K_SUM_F:
+                atomicAdd(reinterpret_cast<double*>(ap), f64_value(p.in[a], r));
+                break;
+            case K_MIN_I:
+                atomicMin(reinterpret_cast<long long*>(ap), static_cast<long long>(int_value(p.in[a], r)));
+                break;
+            case K_MAX_I:
+                atomicMax(reinterpret_cast<long long*>(ap), static_cast<long long>(int_value(p.in[a], r)));
+                break;
+            case K_MIN_F:
+                atomicMin(reinterpret_cast<unsigned long long*>(ap),
+                          static_cast<unsigned long long>(f64_order_key(f64_value(p.in[a], r))));
+                break;
+            case K_MAX_F:
+                atomicMax(reinterpret_cast<unsigned long long*>(ap),
+                          static_cast<unsigned long long>(f64_order_key(f64_value(p.in[a], r))));
+                break;
+            default:
+                break;
+        }
+    }
+}
+
+// Flush the block's LDS partials into partial[block] (compact layout).
+__device__ inline void lds_flush(const char* smem, uint32_t S, const AggProg& p, uint64_t* part) {
+    const uint64_t* acc = reinterpret_cast<const uint64_t*>(smem);
+    const uint32_t* cnt = reinterpret_cast<const uint32_t*>(smem + size_t(8) * S * p.nacc);
+    const uint32_t* rows = cnt + size_t(S) * p.ncnt;
+    uint64_t* dst = part + size_t(blockIdx.x) * part_words(S, p);
+    for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
+        dst[s] = rows[s];
+        for (int i = 0; i < p.nacc; ++i) dst[(1 + size_t(i)) * S + s] = acc[size_t(i) * S + s];
+        for (int c = 0; c < p.ncnt; ++c) dst[(1 + size_t(p.nacc) + c) * S + s] = cnt[size_t(c) * S + s];
+    }
+}
+
+// ---- HBM state (final, after merge): rows | acc[nacc] | cnt[ncnt] (same compact
+// layout, u64 words). Atomic updates straight into HBM for slot counts that do
+// not fit the LDS budget.
+__host__ __device__ inline size_t state_words(uint64_t S, const AggProg& p) {
+    return size_t(S) * (1 + p.nacc + p.ncnt);
+}
+
+__device__ inline void global_update(uint64_t* st, uint64_t S, const AggProg& p, uint64_t slot,
+                                     const uint64_t* vrow) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(&st[slot]), 1ull);
+    for (int a = 0; a < p.n; ++a) {
+        const int k = p.kind[a];
+        if (k == K_COUNT_STAR) continue;
+        const uint64_t r = vrow[a];
+        if (!is_valid(p.in[a], r)) continue;
+        if (p.cnt_of[a] >= 0)
+            atomicAdd(reinterpret_cast<unsigned long long*>(&st[(1 + size_t(p.nacc) + p.cnt_of[a]) * S + slot]),
+                      1ull);
+        if (p.acc_of[a] < 0) continue;
+        uint64_t* acc = st + (1 + size_t(p.acc_of[a])) * S + slot;
+        switch (k) {
+            case K_SUM_I:
+                atomicAdd(reinterpret_cast<unsigned long long*>(acc),
+                          static_cast<unsigned long long>(int_value(p.in[a], r)));
+                break;
+            case K_SUM_F:
+                unsafeAtomicAdd(reinterpret_cast<double*>(acc), f64_value(p.in[a], r));
+                break;
+            case K_MIN_I:
+                atomicMin(reinterpret_cast<long long*>(acc), static_cast<long long>(int_value(p.in[a], r)));
+                break;
+            case K_MAX_I:
+                atomicMax(reinterpret_cast<long long*>(acc), static_cast<long long>(int_value(p.in[a], r)));
+                break;
+            case K_MIN_F:
+                atomicMin(reinterpret_cast<unsigned long long*>(acc),
+                          static_cast<unsigned long long>(f64_order_key(f64_value(p.in[a], r))));
+                break;
+            case K_MAX_F:
+                atomicMax(reinterpret_cast<unsigned long long*>(acc),
+                          static_cast<unsigned long long>(f64_order_key(f64_value(p.in[a], r))));
+                break;
+            default:
+                break;
+        }
+    }
+}
+
+// Host launchers (agg_common.hip).
+void init_state(qe_gpu_ctx* ctx, uint64_t* st, uint64_t S, const AggProg& p);
+// Merge nblocks partials into st in block order (deterministic).
+void merge_partials(qe_gpu_ctx* ctx, const uint64_t* part, uint32_t nblocks, uint64_t* st,
+                    uint64_t S, const AggProg& p);
+
+// Key source for emission.
+struct KeyEmit {
+    int mode;            // 0 = direct range (key = kmin + slot, slot S-1 is NULL), 1 = words
+    int nkeys;
+    int32_t types[kMaxKeys];
+    int64_t kmin;
+    uint64_t null_slot;  // direct mode
+    const uint64_t* words;     // words mode: [nkeys][S]
+    const uint64_t* nullmask;  // words mode: [S]
+};
+
+// Emit one output row per slot with rows > 0 (or every slot when `all_slots`):
+// key columns then finalised aggregates. Returns the table.
+qe_dev_table* emit_groups(qe_gpu_ctx* ctx, const uint64_t* st, uint64_t S, const AggProg& p,
+                          const KeyEmit& ke, bool all_slots);
+
+// LDS budget per block for partial aggregation (bytes). One block per CU can
+// take up to 160 KiB on gfx950; smaller footprints allow several.
+constexpr size_t kLdsBudget = 152 * 1024;
+
+}  // namespace qe_gpu

@@ -1,0 +1,1020 @@
+// Backward as-of join: qe_gpu_asof_build / qe_gpu_asof_probe replace
+// tsx::AsofJoin (tsx/asof.h:101-105). Semantics (tsx/asof.h:10-39,
+// tsx/asof.cpp:234-308): for a probe row with key tuple k and time t, the
+// build row with the same key and the greatest tb <= t (the `<=` boundary);
+// a NULL key or NULL time never matches; zero keys = one global partition;
+// an optional tolerance drops the match when (u64)t - (u64)tb > tol.
+//
+// Build: each matchable build row (non-NULL keys and time) gets a partition id
+// (direct key - min for one integer key of modest range, else a device hash
+// table over the key tuple), rows are radix-sorted by (partition, time) on
+// the device, and CSR offsets delimit each partition's sorted timestamps.
+//
+// Probe, two methods with identical results:
+//  * binary  : upper_bound over the partition's sorted timestamps;
+//  * learned : pieces of B sorted rows per partition with a linear position
+//    model and a recorded max error; a radix router maps t to a small range of
+//    pieces through a monotone MLP (1-16-16-1, clamp-ramp units, non-negative
+//    weights after the first layer, trained on the device per partition); the
+//    probe searches that piece range, then an error-bounded window inside the
+//    piece, verifying both boundaries and falling back to binary search when a
+//    bound does not hold.
+// The router is exact by construction: the MLP is monotone in t and the same
+// device function (explicit fmaf) builds the radix table and routes probes,
+// so a key's bucket never precedes the bucket of an earlier key.
+// Training runs the forward and backward passes as MFMA tiles
+// (v_mfma_f32_16x16x4f32): the 16x16 hidden layer, its weight gradient
+// (reduction over the batch) and the back-propagated activation gradient.
+#include "agg_common.hpp"
+#include "hash_common.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+using namespace qe_gpu;
+
+namespace qe_gpu {
+
+bool is_int_like(int t);
+
+constexpr int kH = 16;
+constexpr int kParams = kH + kH + kH * kH + kH + kH + 1;  // 321
+constexpr int OFF_W1 = 0, OFF_B1 = 16, OFF_W2 = 32, OFF_B2 = 288, OFF_W3 = 304, OFF_B3 = 320;
+constexpr uint32_t kMinKnotsForMlp = 64;
+
+struct PartMeta {
+    uint64_t seg;    // first sorted position
+    uint32_t n;      // rows
+    uint32_t K;      // knots (pieces)
+    uint64_t kb;     // first knot index
+    uint64_t tb;     // first router-table entry
+    uint32_t R;      // router buckets
+    int32_t use_mlp;
+    int64_t t0;      // first knot time
+    double inv_span; // 1 / (last knot time - t0), 0 if single knot
+};
+
+}  // namespace qe_gpu
+
+struct qe_dev_asof {
+    int nkeys = 0;
+    int32_t key_types[kMaxKeys] = {};
+    int mode = 0;  // 0 zero keys, 1 direct, 2 hash
+    int64_t kmin = 0;
+    uint64_t range = 0;
+    DeviceHashTable ht;       // hash mode: key tuple -> slot
+    Scratch part_of_slot;     // hash mode: slot -> dense partition id
+    uint64_t nparts = 0;
+    uint64_t n = 0;           // matchable build rows
+    Scratch ts, row, offs;    // sorted timestamps, original rows, CSR (nparts+1)
+    // learned index
+    bool trained = false;
+    uint32_t B = 0;
+    uint64_t nknots = 0, ntab = 0;
+    Scratch meta, tau, slope, err, tab_mlp, tab_lin, params;
+    double final_loss = 0, mean_knots_mlp = 0, mean_knots_lin = 0;
+    uint64_t max_err = 0, mlp_parts = 0;
+    Scratch fallback;  // u64 counter of the last probe
+    uint64_t last_fallbacks = 0;
+};
+
+namespace qe_gpu {
+
+// ---- partition ids ---------------------------------------------------------------
+struct PartKeys {
+    int mode;
+    int nkeys;
+    ColView key[kMaxKeys];
+    int64_t kmin;
+    uint64_t range;
+    HashTableView t;
+    const uint32_t* part_of_slot;
+    // Partition of row r, or kNoRow (NULL key or absent / out of range).
+    __device__ uint32_t part(uint64_t r) const {
+        if (mode == 0) return 0;
+        if (mode == 1) {
+            if (!is_valid(key[0], r)) return kNoRow;
+            const uint64_t off = static_cast<uint64_t>(int_value(key[0], r) - kmin);
+            return off < range ? static_cast<uint32_t>(off) : kNoRow;
+        }
+        HashKeys hk;
+        hk.nkeys = nkeys;
+        for (int j = 0; j < nkeys; ++j) hk.key[j] = key[j];
+        uint64_t w[kMaxKeys], nm, h;
+        normalize_row(hk, r, w, nm, h);
+        if (nm != 0) return kNoRow;
+        const uint32_t s = ht_find(t, nkeys, w, nm, h);
+        return s == kNoSlot ? kNoRow : part_of_slot[s];
+    }
+};
+
+__device__ inline int64_t time_value(const ColView& c, uint64_t r) { return int_value(c, r); }
+
+__global__ void asof_slot_kernel(HashKeys hk, uint64_t n, HashTableView t) {
+    for (uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < n;
+         r += uint64_t(gridDim.x) * blockDim.x) {
+        uint64_t w[kMaxKeys], nm, h;
+        normalize_row(hk, r, w, nm, h);
+        if (nm == 0) ht_insert(t, hk.nkeys, w, nm, h);
+    }
+}
+__global__ void slot_flag_kernel(const uint32_t* state, uint64_t cap, uint32_t* flag) {
+    for (uint64_t s = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; s < cap;
+         s += uint64_t(gridDim.x) * blockDim.x)
+        flag[s] = state[s] == 2 ? 1u : 0u;
+}
+__global__ void slot_part_kernel(const uint32_t* flag, const uint64_t* rank, uint64_t cap, uint32_t* part) {
+    for (uint64_t s = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; s < cap;
+         s += uint64_t(gridDim.x) * blockDim.x)
+        part[s] = flag[s] ? static_cast<uint32_t>(rank[s]) : kNoRow;
+}
+
+// Matchable flag per build row (non-NULL key tuple and time).
+__global__ void asof_flag_kernel(PartKeys pk, ColView time, uint64_t n, uint32_t* flag) {
+    for (uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < n;
+         r += uint64_t(gridDim.x) * blockDim.x)
+        flag[r] = (is_valid(time, r) && pk.part(r) != kNoRow) ? 1u : 0u;
+}
+// Compact matchable rows into (key = part << tbits | (ts - tmin), value = row).
+__global__ void asof_pack_kernel(PartKeys pk, ColView time, uint64_t n, const uint32_t* flag,
+                                 const uint64_t* pos, int64_t tmin, unsigned tbits, uint64_t* keys, uint32_t* vals) {
+    for (uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < n;
+         r += uint64_t(gridDim.x) * blockDim.x) {
+        if (!flag[r]) continue;
+        const uint64_t p = pk.part(r);
+        const uint64_t t = static_cast<uint64_t>(time_value(time, r) - tmin);
+        keys[pos[r]] = (tbits >= 64) ? t : ((p << tbits) | t);
+        vals[pos[r]] = static_cast<uint32_t>(r);
+    }
+}
+// Unpack sorted keys into timestamps and the CSR offsets: offs[p] = first
+// sorted position whose partition >= p.
+__global__ void asof_unpack_kernel(const uint64_t* keys, uint64_t m, int64_t tmin, unsigned tbits, uint64_t nparts,
+                                   int64_t* ts, uint64_t* offs) {
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < m;
+         i += uint64_t(gridDim.x) * blockDim.x) {
+        const uint64_t k = keys[i];
+        const uint64_t mask = (tbits >= 64) ? ~0ull : ((uint64_t(1) << tbits) - 1);
+        ts[i] = static_cast<int64_t>((k & mask) + static_cast<uint64_t>(tmin));
+        const uint64_t p = (tbits >= 64) ? 0 : (k >> tbits);
+        const uint64_t prev = (i == 0) ? 0 : (((tbits >= 64) ? 0 : (keys[i - 1] >> tbits)) + 1);
+        for (uint64_t q = prev; q <= p; ++q) offs[q] = i;
+        if (i == m - 1)
+            for (uint64_t q = p + 1; q <= nparts; ++q) offs[q] = m;
+    }
+}
+__global__ void fill_offs_kernel(uint64_t* offs, uint64_t nparts1, uint64_t v) {
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < nparts1;
+         i += uint64_t(gridDim.x) * blockDim.x)
+        offs[i] = v;
+}
+
+static unsigned bits_for(uint64_t v) {  // bits needed to hold values 0..v
+    unsigned b = 0;
+    while (b < 64 && (v >> b) != 0) ++b;
+    return b;
+}
+
+// ---- binary search probe ------------------------------------------------------------
+// Largest i in [lo, hi) with ts[i] <= t, or hi-... returns `lo - 1` style via
+// a flag: returns the count of entries <= t (upper_bound - lo).
+__device__ inline uint64_t upper_bound_count(const int64_t* ts, uint64_t lo, uint64_t hi, int64_t t) {
+    uint64_t first = lo, len = hi - lo;
+    while (len > 0) {
+        const uint64_t half = len >> 1;
+        if (ts[first + half] <= t) {
+            first += half + 1;
+            len -= half + 1;
+        } else {
+            len = half;
+        }
+    }
+    return first - lo;
+}
+
+struct ProbeArgs {
+    PartKeys pk;
+    ColView time;
+    uint64_t n;
+    const int64_t* ts;
+    const uint32_t* row;
+    const uint64_t* offs;
+    int has_tol;
+    uint64_t tol;
+    uint32_t* out;
+};
+
+__device__ inline void finish_match(const ProbeArgs& a, uint64_t j, int64_t t, uint64_t m, bool found) {
+    uint32_t res = kNoRow;
+    if (found) {
+        const uint64_t span = static_cast<uint64_t>(t) - static_cast<uint64_t>(a.ts[m]);
+        if (!a.has_tol || span <= a.tol) res = a.row[m];
+    }
+    a.out[j] = res;
+}
+
+__global__ void asof_binary_kernel(ProbeArgs a) {
+    for (uint64_t j = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; j < a.n;
+         j += uint64_t(gridDim.x) * blockDim.x) {
+        const uint32_t p = is_valid(a.time, j) ? a.pk.part(j) : kNoRow;
+        if (p == kNoRow) {
+            a.out[j] = kNoRow;
+            continue;
+        }
+        const int64_t t = time_value(a.time, j);
+        const uint64_t lo = a.offs[p], hi = a.offs[p + 1];
+        const uint64_t c = upper_bound_count(a.ts, lo, hi, t);
+        finish_match(a, j, t, lo + c - 1, c > 0);
+    }
+}
+
+// ---- learned index: MLP -----------------------------------------------------------
+__device__ inline float ramp(float z) { return fminf(fmaxf(z, 0.0f), 1.0f); }
+
+// The router function: monotone non-decreasing in x for w1 >= 0, W2 >= 0,
+// w3 >= 0. Explicit fmaf in a fixed order, so every call site (radix table
+// construction and probe routing) computes bit-identical values.
+__host__ __device__ inline float mlp_forward(const float* P, float x) {
+    float h1[kH];
+    for (int k = 0; k < kH; ++k) h1[k] = fminf(fmaxf(fmaf(P[OFF_W1 + k], x, P[OFF_B1 + k]), 0.0f), 1.0f);
+    float y = P[OFF_B3];
+    for (int m = 0; m < kH; ++m) {
+        float z = P[OFF_B2 + m];
+        for (int k = 0; k < kH; ++k) z = fmaf(P[OFF_W2 + m * kH + k], h1[k], z);
+        y = fmaf(P[OFF_W3 + m], fminf(fmaxf(z, 0.0f), 1.0f), y);
+    }
+    return y;
+}
+
+__device__ inline float norm_x(const PartMeta& pm, int64_t t) {
+    double x = static_cast<double>(t - pm.t0) * pm.inv_span;
+    x = x < 0.0 ? 0.0 : (x > 1.0 ? 1.0 : x);
+    return static_cast<float>(x);
+}
+__device__ inline uint32_t bucket_of(const PartMeta& pm, const float* P, bool mlp, int64_t t) {
+    const float x = norm_x(pm, t);
+    float y = mlp ? mlp_forward(P, x) : x;
+    float b = floorf(y * static_cast<float>(pm.R));
+    if (!(b >= 0.0f)) b = 0.0f;  // also maps NaN to 0
+    if (b > static_cast<float>(pm.R - 1)) b = static_cast<float>(pm.R - 1);
+    return static_cast<uint32_t>(b);
+}
+
+// Piece position model: integer prediction in [0, cnt).
+__device__ inline uint32_t piece_pred(int64_t tau, double slope, uint32_t cnt, int64_t t) {
+    double p = static_cast<double>(t - tau) * slope + 0.5;
+    if (!(p >= 0.0)) p = 0.0;
+    const double hi = static_cast<double>(cnt - 1);
+    if (p > hi) p = hi;
+    return static_cast<uint32_t>(p);
+}
+
+__global__ void meta_counts_kernel(const uint64_t* offs, uint64_t nparts, uint32_t B, uint32_t* K, uint32_t* R1) {
+    for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < nparts;
+         p += uint64_t(gridDim.x) * blockDim.x) {
+        const uint64_t n = offs[p + 1] - offs[p];
+        const uint32_t k = static_cast<uint32_t>((n + B - 1) / B);
+        K[p] = k;
+        R1[p] = k ? k + 1 : 0;  // R = K buckets -> K+1 table entries
+    }
+}
+__global__ void meta_fill_kernel(const uint64_t* offs, uint64_t nparts, uint32_t B, const uint32_t* K,
+                                 const uint64_t* kb, const uint64_t* tb, const int64_t* ts, PartMeta* meta) {
+    for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < nparts;
+         p += uint64_t(gridDim.x) * blockDim.x) {
+        PartMeta m;
+        m.seg = offs[p];
+        m.n = static_cast<uint32_t>(offs[p + 1] - offs[p]);
+        m.K = K[p];
+        m.kb = kb[p];
+        m.tb = tb[p];
+        m.R = K[p];
+        m.use_mlp = K[p] >= kMinKnotsForMlp;
+        if (m.K) {
+            m.t0 = ts[m.seg];
+            const int64_t tl = ts[m.seg + uint64_t(m.K - 1) * B];
+            m.inv_span = tl > m.t0 ? 1.0 / static_cast<double>(tl - m.t0) : 0.0;
+        } else {
+            m.t0 = 0;
+            m.inv_span = 0.0;
+        }
+        meta[p] = m;
+    }
+}
+__device__ inline uint32_t part_of_knot(const uint64_t* kb, uint64_t nparts, uint64_t g) {
+    uint64_t lo = 0, hi = nparts;  // kb[lo] <= g < kb[hi]
+    while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (kb[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    return static_cast<uint32_t>(lo);
+}
+// Knot times, piece slopes and recorded max errors.
+__global__ void pieces_kernel(const PartMeta* meta, const uint64_t* kb, uint64_t nparts, uint64_t nknots, uint32_t B,
+                              const int64_t* ts, int64_t* tau, double* slope, uint32_t* err,
+                              unsigned long long* max_err) {
+    uint64_t local_max = 0;
+    for (uint64_t g = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; g < nknots;
+         g += uint64_t(gridDim.x) * blockDim.x) {
+        const uint32_t p = part_of_knot(kb, nparts + 1, g);
+        const PartMeta pm = meta[p];
+        const uint32_t j = static_cast<uint32_t>(g - pm.kb);
+        const uint64_t pos0 = pm.seg + uint64_t(j) * B;
+        const uint32_t cnt = min(B, pm.n - j * B);
+        const int64_t t0 = ts[pos0], tl = ts[pos0 + cnt - 1];
+        const double s = (cnt > 1 && tl > t0) ? static_cast<double>(cnt - 1) / static_cast<double>(tl - t0) : 0.0;
+        uint32_t e = 0;
+        for (uint32_t i = 0; i < cnt; ++i) {
+            const uint32_t pr = piece_pred(t0, s, cnt, ts[pos0 + i]);
+            const uint32_t d = pr > i ? pr - i : i - pr;
+            e = d > e ? d : e;
+        }
+        tau[g] = t0;
+        slope[g] = s;
+        err[g] = e;
+        local_max = e > local_max ? e : local_max;
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const uint64_t o = __shfl_down(local_max, off);
+        local_max = o > local_max ? o : local_max;
+    }
+    if (lane_id() == 0) atomicMax(max_err, static_cast<unsigned long long>(local_max));
+}
+
+// Radix router table: tab[tb + b] = first knot whose bucket >= b (b in 0..R).
+__global__ void router_table_kernel(const PartMeta* meta, const uint64_t* kb, uint64_t nparts, uint64_t nknots,
+                                    const int64_t* tau, const float* params, int use_mlp, uint32_t* tab) {
+    for (uint64_t g = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; g < nknots;
+         g += uint64_t(gridDim.x) * blockDim.x) {
+        const uint32_t p = part_of_knot(kb, nparts + 1, g);
+        const PartMeta pm = meta[p];
+        const uint32_t j = static_cast<uint32_t>(g - pm.kb);
+        const bool mlp = use_mlp && pm.use_mlp;
+        const float* P = params + size_t(p) * kParams;
+        const int64_t bj = bucket_of(pm, P, mlp, tau[g]);
+        const int64_t bprev = (j == 0) ? -1 : static_cast<int64_t>(bucket_of(pm, P, mlp, tau[g - 1]));
+        for (int64_t b = bprev + 1; b <= bj; ++b) tab[pm.tb + b] = j;
+        if (j == pm.K - 1)
+            for (int64_t b = bj + 1; b <= static_cast<int64_t>(pm.R); ++b) tab[pm.tb + b] = pm.K;
+    }
+}
+
+// Mean knots a probe searches (probes distributed like the knots): for each
+// knot, its bucket's occupancy (+1 for the predecessor boundary).
+__global__ void router_stats_kernel(const PartMeta* meta, const uint64_t* kb, uint64_t nparts, uint64_t nknots,
+                                    const int64_t* tau, const float* params, int use_mlp, const uint32_t* tab,
+                                    double* sum) {
+    double local = 0;
+    for (uint64_t g = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; g < nknots;
+         g += uint64_t(gridDim.x) * blockDim.x) {
+        const uint32_t p = part_of_knot(kb, nparts + 1, g);
+        const PartMeta pm = meta[p];
+        const bool mlp = use_mlp && pm.use_mlp;
+        const uint32_t b = bucket_of(pm, params + size_t(p) * kParams, mlp, tau[g]);
+        local += static_cast<double>(tab[pm.tb + b + 1] - tab[pm.tb + b]) + 1.0;
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) local += __shfl_down(local, off);
+    if (lane_id() == 0) atomicAdd(sum, local);
+}
+
+// ---- learned probe ---------------------------------------------------------------------
+struct LearnedArgs {
+    const PartMeta* meta;
+    const int64_t* tau;
+    const double* slope;
+    const uint32_t* err;
+    const uint32_t* tab;
+    const float* params;
+    int use_mlp;
+    uint32_t B;
+    unsigned long long* fallbacks;
+};
+
+// Largest knot j in [lo, hi] with tau[kb+j] <= t (caller guarantees tau[kb+lo] <= t).
+__device__ inline uint32_t knot_search(const int64_t* tau, uint64_t kb, uint32_t lo, uint32_t hi, int64_t t) {
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo + 1) / 2;
+        if (tau[kb + mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void asof_learned_kernel(ProbeArgs a, LearnedArgs L) {
+    uint32_t fb = 0;
+    for (uint64_t j = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; j < a.n;
+         j += uint64_t(gridDim.x) * blockDim.x) {
+        const uint32_t p = is_valid(a.time, j) ? a.pk.part(j) : kNoRow;
+        if (p == kNoRow) {
+            a.out[j] = kNoRow;
+            continue;
+        }
+        const int64_t t = time_value(a.time, j);
+        const PartMeta pm = L.meta[p];
+        if (pm.K == 0 || t < L.tau[pm.kb]) {  // before the first build row
+            a.out[j] = kNoRow;
+            continue;
+        }
+        // 1. route to a knot range and locate the piece j* (tau[j*] <= t < tau[j*+1]).
+        const bool mlp = L.use_mlp && pm.use_mlp;
+        const uint32_t b = bucket_of(pm, L.params + size_t(p) * kParams, mlp, t);
+        const uint32_t t_lo = L.tab[pm.tb + b], t_hi = L.tab[pm.tb + b + 1];
+        uint32_t lo = t_lo > 0 ? t_lo - 1 : 0;
+        uint32_t hi = t_hi > 0 ? t_hi - 1 : 0;
+        if (hi >= pm.K) hi = pm.K - 1;
+        if (lo > hi) lo = hi;
+        uint32_t piece;
+        const bool ok_lo = L.tau[pm.kb + lo] <= t;
+        const bool ok_hi = (hi + 1 >= pm.K) || L.tau[pm.kb + hi + 1] > t;
+        if (ok_lo && ok_hi) {
+            piece = knot_search(L.tau, pm.kb, lo, hi, t);
+        } else {
+            ++fb;
+            piece = knot_search(L.tau, pm.kb, 0, pm.K - 1, t);
+        }
+        // 2. error-bounded window inside the piece.
+        const uint64_t base = pm.seg + uint64_t(piece) * L.B;
+        const uint32_t cnt = min(L.B, pm.n - piece * L.B);
+        const uint32_t pred = piece_pred(L.tau[pm.kb + piece], L.slope[pm.kb + piece], cnt, t);
+        const uint32_t e = L.err[pm.kb + piece] + 1;
+        const uint32_t wlo = pred > e ? pred - e : 0;
+        const uint32_t whi = min(cnt - 1, pred + e);
+        uint64_t c;
+        const bool w_ok = (wlo == 0 || a.ts[base + wlo] <= t) && (whi == cnt - 1 || a.ts[base + whi + 1] > t);
+        if (w_ok) {
+            c = wlo + upper_bound_count(a.ts, base + wlo, base + whi + 1, t);
+        } else {
+            ++fb;
+            c = upper_bound_count(a.ts, base, base + cnt, t);
+        }
+        // tau[piece] <= t guarantees c >= 1.
+        finish_match(a, j, t, base + c - 1, c > 0);
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) fb += __shfl_down(fb, off);
+    if (lane_id() == 0 && fb) atomicAdd(L.fallbacks, static_cast<unsigned long long>(fb));
+}
+
+// ---- MLP training (MFMA) ---------------------------------------------------------------
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__device__ inline floatx4 mfma16(float a, float b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Gradients of the batch MSE (1/N) sum (y - c)^2 into g[kParams] (LDS, zeroed by
+// the caller) and the summed squared error into *loss. P = parameters (LDS).
+// Each wave takes 16-sample tiles. Lane l = 16q + r:
+//   forward  Z2[s][m] = sum_k H1[s][k] W2[m][k] + b2[m]  (4 MFMAs, D: s = 4q+v, m = r)
+//   backward dW2[m][k] += sum_s dZ2[s][m] H1[s][k]       (4 MFMAs, D: m = 4q+v, k = r)
+//            dH1[s][k] = sum_m dZ2[s][m] W2[m][k]        (4 MFMAs, D: s = 4q+v, k = r)
+__device__ void mlp_batch_grads(const float* P, const float* xs, const float* cs, uint32_t N, float* g, float* loss) {
+    const unsigned lane = lane_id();
+    const unsigned q = lane / 16, r = lane % 16;
+    const unsigned wave = threadIdx.x / kWave, nwaves = blockDim.x / kWave;
+    const float invN = 1.0f / static_cast<float>(N);
+    floatx4 dW2acc = {0, 0, 0, 0};
+    float gw1 = 0, gb1 = 0, gb2 = 0, gw3 = 0, gb3 = 0, lsum = 0;
+    const uint32_t ntiles = (N + 15) / 16;
+    for (uint32_t tile = wave; tile < ntiles; tile += nwaves) {
+        const uint32_t s0 = tile * 16;
+        // ---- forward ----
+        floatx4 z2;
+        for (int v = 0; v < 4; ++v) z2[v] = P[OFF_B2 + r];
+        for (int c = 0; c < 4; ++c) {
+            const uint32_t s = s0 + r;
+            const float x = s < N ? xs[s] : 0.0f;
+            const int k = 4 * c + q;
+            const float a = ramp(fmaf(P[OFF_W1 + k], x, P[OFF_B1 + k]));  // H1[s=r][k]
+            const float bb = P[OFF_W2 + r * kH + k];                       // W2[m=r][k]
+            z2 = mfma16(a, bb, z2);
+        }
+        float h2[4], y[4];
+        for (int v = 0; v < 4; ++v) {
+            h2[v] = ramp(z2[v]);
+            float pv = P[OFF_W3 + r] * h2[v];
+            for (int off = 8; off > 0; off >>= 1) pv += __shfl_xor(pv, off, 16);
+            y[v] = pv + P[OFF_B3];
+        }
+        float dz2[4];
+        for (int v = 0; v < 4; ++v) {
+            const uint32_t s = s0 + 4 * q + v;
+            const bool valid = s < N;
+            const float diff = valid ? y[v] - cs[s] : 0.0f;
+            const float dy = 2.0f * diff * invN;
+            if (r == 0) {
+                lsum += diff * diff;
+                gb3 += dy;
+            }
+            gw3 += dy * h2[v];
+            const float z = z2[v];
+            dz2[v] = (z > 0.0f && z < 1.0f) ? dy * P[OFF_W3 + r] : 0.0f;
+            gb2 += dz2[v];
+        }
+        // ---- dW2 += dZ2^T H1 ----
+        for (int c = 0; c < 4; ++c) {
+            // A[m=r][s'=q] = dZ2[4c+q][r], held by lane 16c + r in register q.
+            const int src = 16 * c + r;
+            const float t0 = __shfl(dz2[0], src), t1 = __shfl(dz2[1], src), t2 = __shfl(dz2[2], src),
+                        t3 = __shfl(dz2[3], src);
+            const float a = q == 0 ? t0 : (q == 1 ? t1 : (q == 2 ? t2 : t3));
+            const uint32_t s = s0 + 4 * c + q;
+            const float x = s < N ? xs[s] : 0.0f;
+            const float bb = ramp(fmaf(P[OFF_W1 + r], x, P[OFF_B1 + r]));  // H1[4c+q][k=r]
+            dW2acc = mfma16(a, bb, dW2acc);
+        }
+        // ---- dH1 = dZ2 W2 ----
+        floatx4 dh1 = {0, 0, 0, 0};
+        for (int c = 0; c < 4; ++c) {
+            // A[s=r][m'=q] = dZ2[r][4c+q], held by lane 16*(r/4) + 4c+q in register r%4.
+            const int src = 16 * (r / 4) + 4 * c + q;
+            const float t0 = __shfl(dz2[0], src), t1 = __shfl(dz2[1], src), t2 = __shfl(dz2[2], src),
+                        t3 = __shfl(dz2[3], src);
+            const unsigned sel = r % 4;
+            const float a = sel == 0 ? t0 : (sel == 1 ? t1 : (sel == 2 ? t2 : t3));
+            const float bb = P[OFF_W2 + (4 * c + q) * kH + r];  // W2[m=4c+q][k=r]
+            dh1 = mfma16(a, bb, dh1);
+        }
+        for (int v = 0; v < 4; ++v) {
+            const uint32_t s = s0 + 4 * q + v;
+            const float x = s < N ? xs[s] : 0.0f;
+            const float z1 = fmaf(P[OFF_W1 + r], x, P[OFF_B1 + r]);
+            const float dz1 = (z1 > 0.0f && z1 < 1.0f) ? dh1[v] : 0.0f;
+            gw1 += dz1 * x;
+            gb1 += dz1;
+        }
+    }
+    // Reduce lane partials that share r (the 4 groups q) and fold into g.
+    for (int off = 16; off < 64; off <<= 1) {
+        gw1 += __shfl_xor(gw1, off);
+        gb1 += __shfl_xor(gb1, off);
+        gb2 += __shfl_xor(gb2, off);
+        gw3 += __shfl_xor(gw3, off);
+    }
+    for (int off = 1; off < 64; off <<= 1) {
+        gb3 += __shfl_xor(gb3, off);
+        lsum += __shfl_xor(lsum, off);
+    }
+    if (q == 0) {
+        atomicAdd(&g[OFF_W1 + r], gw1);
+        atomicAdd(&g[OFF_B1 + r], gb1);
+        atomicAdd(&g[OFF_B2 + r], gb2);
+        atomicAdd(&g[OFF_W3 + r], gw3);
+    }
+    if (lane == 0) {
+        atomicAdd(&g[OFF_B3], gb3);
+        atomicAdd(loss, lsum);
+    }
+    for (int v = 0; v < 4; ++v) atomicAdd(&g[OFF_W2 + (4 * q + v) * kH + r], dW2acc[v]);  // dW2[m=4q+v][k=r]
+}
+
+__device__ inline void init_params(float* P) {
+    // Identity router: unit k ramps over [k/16, (k+1)/16]; layer 2 passes it
+    // through; the output averages the 16 ramps, so y(x) = x.
+    for (int i = threadIdx.x; i < kParams; i += blockDim.x) {
+        float v = 0;
+        if (i < OFF_B1) v = static_cast<float>(kH);
+        else if (i < OFF_W2) v = -static_cast<float>(i - OFF_B1);
+        else if (i < OFF_B2) v = ((i - OFF_W2) / kH == (i - OFF_W2) % kH) ? 1.0f : 0.0f;
+        else if (i < OFF_W3) v = 0.0f;
+        else if (i < OFF_B3) v = 1.0f / kH;
+        else v = 0.0f;
+        P[i] = v;
+    }
+}
+
+constexpr uint32_t kTrainBatch = 2048;
+
+// One block per partition with an MLP: Adam on the batch MSE of predicting the
+// knot CDF (j + 0.5) / K from the normalised knot time; projection keeps the
+// router monotone (w1, W2, w3 >= 0).
+__global__ void __launch_bounds__(256) mlp_train_kernel(const PartMeta* meta, const int64_t* tau, uint64_t nparts,
+                                                        uint32_t steps, float lr, float* params, float* losses) {
+    __shared__ float P[kParams], G[kParams], M1[kParams], M2[kParams];
+    __shared__ float xs[kTrainBatch], cs[kTrainBatch];
+    __shared__ float loss;
+    const uint64_t p = blockIdx.x;
+    if (p >= nparts) return;
+    const PartMeta pm = meta[p];
+    if (!pm.use_mlp) return;  // block-uniform exit
+    init_params(P);
+    for (int i = threadIdx.x; i < kParams; i += blockDim.x) M1[i] = M2[i] = 0.0f;
+    const uint32_t K = pm.K;
+    const uint32_t N = K < kTrainBatch ? K : kTrainBatch;
+    float last = 0;
+    for (uint32_t step = 0; step < steps; ++step) {
+        // Batch: N knots spread over the partition, phase shifted per step.
+        const double phase = fmod(static_cast<double>(step) * 0.6180339887498949, 1.0);
+        for (uint32_t s = threadIdx.x; s < N; s += blockDim.x) {
+            uint32_t j = static_cast<uint32_t>((static_cast<double>(s) + phase) * K / N);
+            if (j >= K) j = K - 1;
+            xs[s] = norm_x(pm, tau[pm.kb + j]);
+            cs[s] = (static_cast<float>(j) + 0.5f) / static_cast<float>(K);
+        }
+        for (int i = threadIdx.x; i < kParams; i += blockDim.x) G[i] = 0.0f;
+        if (threadIdx.x == 0) loss = 0;
+        __syncthreads();
+        mlp_batch_grads(P, xs, cs, N, G, &loss);
+        __syncthreads();
+        const float b1c = 1.0f - powf(0.9f, static_cast<float>(step + 1));
+        const float b2c = 1.0f - powf(0.999f, static_cast<float>(step + 1));
+        for (int i = threadIdx.x; i < kParams; i += blockDim.x) {
+            const float gi = G[i];
+            M1[i] = 0.9f * M1[i] + 0.1f * gi;
+            M2[i] = 0.999f * M2[i] + 0.001f * gi * gi;
+            float v = P[i] - lr * (M1[i] / b1c) / (sqrtf(M2[i] / b2c) + 1e-8f);
+            const bool nonneg = (i < OFF_B1) || (i >= OFF_W2 && i < OFF_B2) || (i >= OFF_W3 && i < OFF_B3);
+            if (nonneg && v < 0.0f) v = 0.0f;
+            P[i] = v;
+        }
+        last = loss / static_cast<float>(N);
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < kParams; i += blockDim.x) params[p * kParams + i] = P[i];
+    if (threadIdx.x == 0) losses[p] = last;
+}
+
+// Gradient check entry: one block computes the batch gradient for given params.
+__global__ void __launch_bounds__(256) mlp_grad_kernel(const float* params, const float* x, const float* c,
+                                                       uint32_t N, float* grads, float* loss_out) {
+    __shared__ float P[kParams], G[kParams];
+    __shared__ float loss;
+    extern __shared__ float dyn[];  // xs[N], cs[N]
+    float* xs = dyn;
+    float* cs = dyn + N;
+    for (int i = threadIdx.x; i < kParams; i += blockDim.x) {
+        P[i] = params[i];
+        G[i] = 0.0f;
+    }
+    for (uint32_t s = threadIdx.x; s < N; s += blockDim.x) {
+        xs[s] = x[s];
+        cs[s] = c[s];
+    }
+    if (threadIdx.x == 0) loss = 0;
+    __syncthreads();
+    mlp_batch_grads(P, xs, cs, N, G, &loss);
+    __syncthreads();
+    for (int i = threadIdx.x; i < kParams; i += blockDim.x) grads[i] = G[i];
+    if (threadIdx.x == 0) *loss_out = loss / static_cast<float>(N);
+}
+
+__global__ void mlp_forward_kernel(const float* params, const float* x, uint64_t n, float* y) {
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n;
+         i += uint64_t(gridDim.x) * blockDim.x)
+        y[i] = mlp_forward(params, x[i]);
+}
+
+static PartKeys part_keys(const qe_dev_asof* idx, const qe_dev_col* const* keys) {
+    PartKeys pk{};
+    pk.mode = idx->mode;
+    pk.nkeys = idx->nkeys;
+    for (int j = 0; j < idx->nkeys; ++j) pk.key[j] = view(keys[j]);
+    pk.kmin = idx->kmin;
+    pk.range = idx->range;
+    pk.t = idx->ht.v;
+    pk.part_of_slot = idx->part_of_slot.as<uint32_t>();
+    return pk;
+}
+
+static bool time_type_ok(int t) { return t == QE_I32 || t == QE_I64 || t == QE_TS; }
+
+}  // namespace qe_gpu
+
+extern "C" {
+
+qe_gpu_status qe_gpu_asof_build(qe_gpu_ctx* ctx, const qe_dev_col* const* build_keys, uint32_t nkeys,
+                                const qe_dev_col* build_time, qe_dev_asof** out) {
+    QE_ENTRY(ctx, {
+        invalid(out != nullptr && build_time != nullptr, "null argument");
+        // tsx/asof.cpp:144-152: time columns must be I32/I64/TS.
+        invalid(time_type_ok(build_time->type), "AsofJoin: time columns must be I32/I64/TS");
+        check(nkeys <= kMaxKeys, QE_GPU_UNSUPPORTED, "too many as-of keys for the device path");
+        const uint64_t n = build_time->len;
+        for (uint32_t j = 0; j < nkeys; ++j) {
+            invalid(build_keys[j] != nullptr, "null key column");
+            invalid(build_keys[j]->len == n, "build key and time columns differ in length");
+        }
+        check(n < kNoRow, QE_GPU_UNSUPPORTED, "row indices are 32-bit");
+        auto* idx = new qe_dev_asof();
+        try {
+            idx->nkeys = static_cast<int>(nkeys);
+            for (uint32_t j = 0; j < nkeys; ++j) idx->key_types[j] = build_keys[j]->type;
+            if (nkeys == 0) {
+                idx->mode = 0;
+                idx->nparts = 1;
+            } else if (nkeys == 1 && is_int_like(build_keys[0]->type)) {
+                const MinMax mm = column_minmax(ctx, build_keys[0], n);
+                const uint64_t R = mm.valid ? static_cast<uint64_t>(mm.mx) - static_cast<uint64_t>(mm.mn) + 1 : 1;
+                if (R <= std::max<uint64_t>(2 * n, 1 << 16) && R < (uint64_t(1) << 30)) {
+                    idx->mode = 1;
+                    idx->kmin = mm.valid ? mm.mn : 0;
+                    idx->range = mm.valid ? R : 0;
+                    idx->nparts = mm.valid ? R : 1;
+                }
+            }
+            if (nkeys > 0 && idx->mode == 0) {
+                idx->mode = 2;
+                HashKeys hk{};
+                hk.nkeys = static_cast<int>(nkeys);
+                for (uint32_t j = 0; j < nkeys; ++j) hk.key[j] = view(build_keys[j]);
+                idx->ht.create(ctx, static_cast<int>(nkeys), std::max<uint64_t>(n, 1));
+                if (n)
+                    asof_slot_kernel<<<grid_for(n, 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(hk, n, idx->ht.v);
+                QE_HIP(hipGetLastError());
+                idx->ht.check_overflow(ctx);
+                const uint64_t cap = idx->ht.v.cap;
+                Scratch flag(ctx, cap * 4), rank(ctx, cap * 8);
+                slot_flag_kernel<<<grid_for(cap, 256, 8192), 256, 0, ctx->stream>>>(idx->ht.v.state, cap,
+                                                                                     flag.as<uint32_t>());
+                const uint64_t parts = exclusive_scan_u32(ctx, flag.as<uint32_t>(), rank.as<uint64_t>(), cap);
+                idx->part_of_slot = Scratch(ctx, cap * 4);
+                slot_part_kernel<<<grid_for(cap, 256, 8192), 256, 0, ctx->stream>>>(
+                    flag.as<uint32_t>(), rank.as<uint64_t>(), cap, idx->part_of_slot.as<uint32_t>());
+                QE_HIP(hipGetLastError());
+                idx->nparts = std::max<uint64_t>(parts, 1);
+            }
+            const PartKeys pk = part_keys(idx, build_keys);
+            const ColView tv = view(build_time);
+            Scratch flag(ctx, std::max<uint64_t>(n, 1) * 4), pos(ctx, std::max<uint64_t>(n, 1) * 8);
+            if (n)
+                asof_flag_kernel<<<grid_for(n, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(pk, tv, n,
+                                                                                                flag.as<uint32_t>());
+            QE_HIP(hipGetLastError());
+            const uint64_t m = exclusive_scan_u32(ctx, flag.as<uint32_t>(), pos.as<uint64_t>(), n);
+            idx->n = m;
+            idx->offs = Scratch(ctx, (idx->nparts + 1) * 8);
+            idx->ts = Scratch(ctx, std::max<uint64_t>(m, 1) * 8);
+            idx->row = Scratch(ctx, std::max<uint64_t>(m, 1) * 4);
+            if (m == 0) {
+                fill_offs_kernel<<<grid_for(idx->nparts + 1, 256, 4096), 256, 0, ctx->stream>>>(
+                    idx->offs.as<uint64_t>(), idx->nparts + 1, 0);
+                QE_HIP(hipGetLastError());
+            } else {
+                const MinMax tm = column_minmax(ctx, build_time, n);
+                const uint64_t tspan = static_cast<uint64_t>(tm.mx) - static_cast<uint64_t>(tm.mn);
+                const unsigned tbits = std::max(1u, bits_for(tspan));
+                const unsigned pbits = bits_for(idx->nparts - 1);
+                Scratch keys(ctx, m * 8);
+                TimedRegion tr(ctx, "asof_build_sort");
+                if (tbits + pbits <= 64) {
+                    asof_pack_kernel<<<grid_for(n, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(
+                        pk, tv, n, flag.as<uint32_t>(), pos.as<uint64_t>(), tm.mn, tbits, keys.as<uint64_t>(),
+                        idx->row.as<uint32_t>());
+                    QE_HIP(hipGetLastError());
+                    radix_sort_pairs(ctx, keys.as<uint64_t>(), idx->row.as<uint32_t>(), m, 0, tbits + pbits);
+                    asof_unpack_kernel<<<grid_for(m, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(
+                        keys.as<uint64_t>(), m, tm.mn, tbits,
+                        idx->nparts, idx->ts.as<int64_t>(), idx->offs.as<uint64_t>());
+                    QE_HIP(hipGetLastError());
+                } else {
+                    throw Error{QE_GPU_UNSUPPORTED, "as-of key: partition bits + time span exceed 64 bits"};
+                }
+            }
+            QE_HIP(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+            delete idx;
+            throw;
+        }
+        *out = idx;
+    });
+}
+
+qe_gpu_status qe_gpu_asof_train(qe_gpu_ctx* ctx, qe_dev_asof* idx, uint32_t piece_rows, uint32_t steps, float lr) {
+    QE_ENTRY(ctx, {
+        invalid(idx != nullptr, "null index");
+        invalid(piece_rows >= 2 && piece_rows <= 4096, "piece_rows must be in [2, 4096]");
+        const uint64_t P = idx->nparts;
+        idx->B = piece_rows;
+        Scratch K(ctx, P * 4), R1(ctx, P * 4), kb(ctx, (P + 1) * 8), tb(ctx, (P + 1) * 8);
+        meta_counts_kernel<<<grid_for(P, 256, 4096), 256, 0, ctx->stream>>>(idx->offs.as<uint64_t>(), P, piece_rows,
+                                                                            K.as<uint32_t>(), R1.as<uint32_t>());
+        QE_HIP(hipGetLastError());
+        const uint64_t nk = exclusive_scan_u32(ctx, K.as<uint32_t>(), kb.as<uint64_t>(), P);
+        const uint64_t nt = exclusive_scan_u32(ctx, R1.as<uint32_t>(), tb.as<uint64_t>(), P);
+        QE_HIP(hipMemcpyAsync(kb.as<uint64_t>() + P, &nk, 8, hipMemcpyHostToDevice, ctx->stream));
+        idx->nknots = nk;
+        idx->ntab = nt;
+        idx->meta = Scratch(ctx, P * sizeof(PartMeta));
+        meta_fill_kernel<<<grid_for(P, 256, 4096), 256, 0, ctx->stream>>>(
+            idx->offs.as<uint64_t>(), P, piece_rows, K.as<uint32_t>(), kb.as<uint64_t>(), tb.as<uint64_t>(),
+            idx->ts.as<int64_t>(), idx->meta.as<PartMeta>());
+        QE_HIP(hipGetLastError());
+        const uint64_t nk1 = std::max<uint64_t>(nk, 1);
+        idx->tau = Scratch(ctx, nk1 * 8);
+        idx->slope = Scratch(ctx, nk1 * 8);
+        idx->err = Scratch(ctx, nk1 * 4);
+        idx->tab_mlp = Scratch(ctx, std::max<uint64_t>(nt, 1) * 4);
+        idx->tab_lin = Scratch(ctx, std::max<uint64_t>(nt, 1) * 4);
+        idx->params = Scratch(ctx, P * kParams * 4);
+        Scratch losses(ctx, P * 4), maxe(ctx, 8), sums(ctx, 16);
+        memset_async(ctx, losses.p, 0, P * 4);
+        memset_async(ctx, maxe.p, 0, 8);
+        memset_async(ctx, sums.p, 0, 16);
+        if (nk) {
+            pieces_kernel<<<grid_for(nk, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(
+                idx->meta.as<PartMeta>(), kb.as<uint64_t>(), P, nk, piece_rows, idx->ts.as<int64_t>(),
+                idx->tau.as<int64_t>(), idx->slope.as<double>(), idx->err.as<uint32_t>(),
+                maxe.as<unsigned long long>());
+            QE_HIP(hipGetLastError());
+            {
+                TimedRegion tr(ctx, "asof_mlp_train");
+                mlp_train_kernel<<<P, 256, 0, ctx->stream>>>(idx->meta.as<PartMeta>(), idx->tau.as<int64_t>(), P, steps,
+                                                             lr, idx->params.as<float>(), losses.as<float>());
+                QE_HIP(hipGetLastError());
+            }
+            for (int use = 0; use < 2; ++use) {
+                uint32_t* tab = use ? idx->tab_mlp.as<uint32_t>() : idx->tab_lin.as<uint32_t>();
+                router_table_kernel<<<grid_for(nk, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(
+                    idx->meta.as<PartMeta>(), kb.as<uint64_t>(), P, nk, idx->tau.as<int64_t>(),
+                    idx->params.as<float>(), use, tab);
+                router_stats_kernel<<<grid_for(nk, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(
+                    idx->meta.as<PartMeta>(), kb.as<uint64_t>(), P, nk, idx->tau.as<int64_t>(),
+                    idx->params.as<float>(), use, tab, sums.as<double>() + use);
+                QE_HIP(hipGetLastError());
+            }
+        }
+        std::vector<float> hl(P);
+        std::vector<PartMeta> hm(P);
+        double hs[2] = {0, 0};
+        QE_HIP(hipMemcpyAsync(hl.data(), losses.p, P * 4, hipMemcpyDeviceToHost, ctx->stream));
+        QE_HIP(hipMemcpyAsync(hm.data(), idx->meta.p, P * sizeof(PartMeta), hipMemcpyDeviceToHost, ctx->stream));
+        QE_HIP(hipMemcpyAsync(hs, sums.p, 16, hipMemcpyDeviceToHost, ctx->stream));
+        idx->max_err = read_scalar(ctx, maxe.as<uint64_t>());
+        double ls = 0;
+        uint64_t cnt = 0;
+        for (uint64_t p = 0; p < P; ++p)
+            if (hm[p].use_mlp) {
+                ls += hl[p];
+                ++cnt;
+            }
+        idx->mlp_parts = cnt;
+        idx->final_loss = cnt ? ls / cnt : 0.0;
+        idx->mean_knots_lin = nk ? hs[0] / nk : 0;
+        idx->mean_knots_mlp = nk ? hs[1] / nk : 0;
+        idx->fallback = Scratch(ctx, 8);
+        idx->trained = true;
+    });
+}
+
+qe_gpu_status qe_gpu_asof_probe(qe_gpu_ctx* ctx, const qe_dev_asof* idx, const qe_dev_col* const* probe_keys,
+                                uint32_t nkeys, const qe_dev_col* probe_time, int32_t has_tolerance,
+                                int64_t tolerance, int32_t mode, uint32_t* match_out) {
+    QE_ENTRY(ctx, {
+        invalid(idx != nullptr && probe_time != nullptr, "null argument");
+        invalid(time_type_ok(probe_time->type), "AsofJoin: time columns must be I32/I64/TS");
+        invalid(static_cast<int>(nkeys) == idx->nkeys, "as-of needs equal-length key lists");
+        for (uint32_t j = 0; j < nkeys; ++j) {
+            invalid(probe_keys[j] != nullptr && probe_keys[j]->len == probe_time->len, "bad probe key column");
+            invalid(probe_keys[j]->type == idx->key_types[j], "AsofJoin: key column types must match positionally");
+        }
+        invalid(mode >= QE_ASOF_BINARY && mode <= QE_ASOF_LEARNED_NO_MLP, "unknown as-of mode");
+        check(mode == QE_ASOF_BINARY || idx->trained, QE_GPU_INVALID_ARGUMENT,
+              "learned as-of probe needs qe_gpu_asof_train first");
+        const uint64_t n = probe_time->len;
+        if (n == 0) return QE_GPU_OK;
+        invalid(match_out != nullptr, "null output");
+        ProbeArgs a{};
+        a.pk = part_keys(idx, probe_keys);
+        a.time = view(probe_time);
+        a.n = n;
+        a.ts = idx->ts.as<int64_t>();
+        a.row = idx->row.as<uint32_t>();
+        a.offs = idx->offs.as<uint64_t>();
+        a.has_tol = has_tolerance ? 1 : 0;
+        a.tol = static_cast<uint64_t>(tolerance);
+        a.out = match_out;
+        const unsigned g = grid_for(n, 256, ctx->num_cus * 32);
+        if (mode == QE_ASOF_BINARY) {
+            TimedRegion tr(ctx, "asof_probe_binary");
+            asof_binary_kernel<<<g, 256, 0, ctx->stream>>>(a);
+            QE_HIP(hipGetLastError());
+        } else {
+            auto* mi = const_cast<qe_dev_asof*>(idx);
+            memset_async(ctx, mi->fallback.p, 0, 8);
+            LearnedArgs L{idx->meta.as<PartMeta>(), idx->tau.as<int64_t>(), idx->slope.as<double>(),
+                          idx->err.as<uint32_t>(),
+                          mode == QE_ASOF_LEARNED ? idx->tab_mlp.as<uint32_t>() : idx->tab_lin.as<uint32_t>(),
+                          idx->params.as<float>(), mode == QE_ASOF_LEARNED ? 1 : 0, idx->B,
+                          mi->fallback.as<unsigned long long>()};
+            {
+                TimedRegion tr(ctx, mode == QE_ASOF_LEARNED ? "asof_probe_learned" : "asof_probe_learned_no_mlp");
+                asof_learned_kernel<<<g, 256, 0, ctx->stream>>>(a, L);
+                QE_HIP(hipGetLastError());
+            }
+            mi->last_fallbacks = read_scalar(ctx, mi->fallback.as<uint64_t>());
+        }
+        QE_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+__global__ void asof_count_kernel(const uint32_t* match, uint64_t n, int left, uint32_t* flag) {
+    for (uint64_t j = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; j < n;
+         j += uint64_t(gridDim.x) * blockDim.x)
+        flag[j] = (left || match[j] != kNoRow) ? 1u : 0u;
+}
+__global__ void asof_emit_kernel(const uint32_t* match, uint64_t n, const uint32_t* flag, const uint64_t* pos,
+                                 uint32_t* pp, uint32_t* bp) {
+    for (uint64_t j = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; j < n;
+         j += uint64_t(gridDim.x) * blockDim.x) {
+        if (!flag[j]) continue;
+        pp[pos[j]] = static_cast<uint32_t>(j);
+        bp[pos[j]] = match[j];
+    }
+}
+
+qe_gpu_status qe_gpu_asof_pairs(qe_gpu_ctx* ctx, const uint32_t* match, uint64_t n, int32_t join_type,
+                                qe_dev_pairs** out) {
+    QE_ENTRY(ctx, {
+        invalid(out != nullptr, "null out");
+        invalid(join_type == QE_JOIN_INNER || join_type == QE_JOIN_LEFT, "unknown join type");
+        auto* pr = new qe_dev_pairs();
+        try {
+            if (n) {
+                Scratch flag(ctx, n * 4), pos(ctx, n * 8);
+                asof_count_kernel<<<grid_for(n, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(
+                    match, n, join_type == QE_JOIN_LEFT, flag.as<uint32_t>());
+                QE_HIP(hipGetLastError());
+                const uint64_t total = exclusive_scan_u32(ctx, flag.as<uint32_t>(), pos.as<uint64_t>(), n);
+                pr->count = total;
+                pr->probe = static_cast<uint32_t*>(dev_alloc(ctx, std::max<uint64_t>(total, 1) * 4));
+                pr->build = static_cast<uint32_t*>(dev_alloc(ctx, std::max<uint64_t>(total, 1) * 4));
+                asof_emit_kernel<<<grid_for(n, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(
+                    match, n, flag.as<uint32_t>(), pos.as<uint64_t>(), pr->probe, pr->build);
+                QE_HIP(hipGetLastError());
+                QE_HIP(hipStreamSynchronize(ctx->stream));
+            }
+        } catch (...) {
+            qe_gpu_pairs_free(ctx, pr);
+            throw;
+        }
+        *out = pr;
+    });
+}
+
+qe_gpu_status qe_gpu_asof_stats(qe_gpu_ctx* ctx, const qe_dev_asof* idx, qe_asof_stats* out) {
+    QE_ENTRY(ctx, {
+        invalid(idx != nullptr && out != nullptr, "null argument");
+        out->build_rows = idx->n;
+        out->partitions = idx->nparts;
+        out->pieces = idx->nknots;
+        out->max_piece_error = idx->max_err;
+        out->router_buckets = idx->ntab >= idx->nparts ? idx->ntab - idx->nparts : 0;
+        out->mean_bucket_knots = idx->mean_knots_mlp;
+        out->mean_bucket_knots_linear = idx->mean_knots_lin;
+        out->fallbacks = idx->last_fallbacks;
+        out->final_loss = idx->final_loss;
+    });
+}
+
+qe_gpu_status qe_gpu_asof_sorted(qe_gpu_ctx* ctx, const qe_dev_asof* idx, int64_t* ts, uint32_t* rows,
+                                 uint64_t* offsets) {
+    QE_ENTRY(ctx, {
+        invalid(idx != nullptr, "null index");
+        if (ts && idx->n) QE_HIP(hipMemcpyAsync(ts, idx->ts.p, idx->n * 8, hipMemcpyDeviceToHost, ctx->stream));
+        if (rows && idx->n)
+            QE_HIP(hipMemcpyAsync(rows, idx->row.p, idx->n * 4, hipMemcpyDeviceToHost, ctx->stream));
+        if (offsets)
+            QE_HIP(hipMemcpyAsync(offsets, idx->offs.p, (idx->nparts + 1) * 8, hipMemcpyDeviceToHost, ctx->stream));
+        QE_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+void qe_gpu_asof_free(qe_gpu_ctx* ctx, qe_dev_asof* idx) {
+    if (!ctx || !idx) return;
+    delete idx;  // Scratch members return their blocks to the context cache
+}
+
+qe_gpu_status qe_gpu_mlp_forward(qe_gpu_ctx* ctx, const float* params, const float* x, uint64_t n, float* y) {
+    QE_ENTRY(ctx, {
+        invalid(params != nullptr && (n == 0 || (x != nullptr && y != nullptr)), "null argument");
+        Scratch P(ctx, kParams * 4), X(ctx, std::max<uint64_t>(n, 1) * 4), Y(ctx, std::max<uint64_t>(n, 1) * 4);
+        QE_HIP(hipMemcpyAsync(P.p, params, kParams * 4, hipMemcpyHostToDevice, ctx->stream));
+        if (n) QE_HIP(hipMemcpyAsync(X.p, x, n * 4, hipMemcpyHostToDevice, ctx->stream));
+        if (n)
+            mlp_forward_kernel<<<grid_for(n, 256, 4096), 256, 0, ctx->stream>>>(P.as<float>(), X.as<float>(), n,
+                                                                                 Y.as<float>());
+        QE_HIP(hipGetLastError());
+        if (n) QE_HIP(hipMemcpyAsync(y, Y.p, n * 4, hipMemcpyDeviceToHost, ctx->stream));
+        QE_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+qe_gpu_status qe_gpu_mlp_grad(qe_gpu_ctx* ctx, const float* params, const float* x, const float* c, uint32_t n,
+                              float* grads, float* loss) {
+    QE_ENTRY(ctx, {
+        invalid(params && x && c && grads && loss, "null argument");
+        invalid(n >= 1 && n <= 8192, "batch size must be in [1, 8192]");
+        Scratch P(ctx, kParams * 4), X(ctx, n * 4), Cc(ctx, n * 4), G(ctx, kParams * 4), L(ctx, 4);
+        QE_HIP(hipMemcpyAsync(P.p, params, kParams * 4, hipMemcpyHostToDevice, ctx->stream));
+        QE_HIP(hipMemcpyAsync(X.p, x, n * 4, hipMemcpyHostToDevice, ctx->stream));
+        QE_HIP(hipMemcpyAsync(Cc.p, c, n * 4, hipMemcpyHostToDevice, ctx->stream));
+        mlp_grad_kernel<<<1, 256, 2 * n * 4, ctx->stream>>>(P.as<float>(), X.as<float>(), Cc.as<float>(), n,
+                                                            G.as<float>(), L.as<float>());
+        QE_HIP(hipGetLastError());
+        QE_HIP(hipMemcpyAsync(grads, G.p, kParams * 4, hipMemcpyDeviceToHost, ctx->stream));
+        QE_HIP(hipMemcpyAsync(loss, L.p, 4, hipMemcpyDeviceToHost, ctx->stream));
+        QE_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+}  // extern "C"

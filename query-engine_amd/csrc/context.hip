@@ -1,0 +1,580 @@
+// Context, device columns, caching allocator, result tables, scans and the
+// small C-ABI plumbing shared by every operator.
+#include "internal.hpp"
+
+#include <algorithm>
+
+using namespace qe_gpu;
+
+namespace qe_gpu {
+
+void* dev_alloc(qe_gpu_ctx* ctx, size_t bytes) {
+    bytes = (bytes + 255) & ~size_t(255);
+    if (bytes == 0) bytes = 256;
+    // Best fit among cached blocks no more than 2x the request.
+    auto it = ctx->free_blocks.lower_bound(bytes);
+    if (it != ctx->free_blocks.end() && it->first <= bytes * 2) {
+        void* p = it->second;
+        ctx->live_blocks[p] = it->first;
+        ctx->free_blocks.erase(it);
+        return p;
+    }
+    if (ctx->mem_limit && ctx->mem_in_use + bytes > ctx->mem_limit) {
+        // Release the cache before declaring failure.
+        QE_HIP(hipStreamSynchronize(ctx->stream));
+        for (auto& kv : ctx->free_blocks) {
+            hipFree(kv.second);
+            ctx->mem_in_use -= kv.first;
+        }
+        ctx->free_blocks.clear();
+        if (ctx->mem_in_use + bytes > ctx->mem_limit)
+            throw Error{QE_GPU_OUT_OF_MEMORY,
+                        "device allocation of " + std::to_string(bytes) +
+                            " bytes exceeds the context memory limit"};
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        QE_HIP(hipStreamSynchronize(ctx->stream));
+        for (auto& kv : ctx->free_blocks) {
+            hipFree(kv.second);
+            ctx->mem_in_use -= kv.first;
+        }
+        ctx->free_blocks.clear();
+        e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error{QE_GPU_OUT_OF_MEMORY,
+                        "hipMalloc(" + std::to_string(bytes) + ") failed"};
+        }
+    }
+    ctx->mem_in_use += bytes;
+    ctx->live_blocks[p] = bytes;
+    return p;
+}
+
+void dev_free(qe_gpu_ctx* ctx, void* p) {
+    if (!p) return;
+    auto it = ctx->live_blocks.find(p);
+    if (it == ctx->live_blocks.end()) return;
+    // Stream-ordered reuse: blocks go back to the cache; later work on the same
+    // stream is ordered after the work that used them.
+    ctx->free_blocks.emplace(it->second, p);
+    ctx->live_blocks.erase(it);
+}
+
+void* pinned(qe_gpu_ctx* ctx, size_t bytes) {
+    if (ctx->pinned_bytes < bytes) {
+        if (ctx->pinned) QE_HIP(hipHostFree(ctx->pinned));
+        ctx->pinned = nullptr;
+        size_t b = std::max<size_t>(bytes, 4096);
+        QE_HIP(hipHostMalloc(&ctx->pinned, b));
+        ctx->pinned_bytes = b;
+    }
+    return ctx->pinned;
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, uint32_t v, uint64_t n) {
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n;
+         i += uint64_t(gridDim.x) * blockDim.x)
+        p[i] = v;
+}
+__global__ void fill_u8_kernel(uint8_t* p, uint8_t v, uint64_t n) {
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n;
+         i += uint64_t(gridDim.x) * blockDim.x)
+        p[i] = v;
+}
+
+void memset_async(qe_gpu_ctx* ctx, void* p, int v, size_t bytes) {
+    if (bytes == 0) return;
+    // A fill kernel rather than hipMemsetAsync keeps this graph-capture safe.
+    if ((reinterpret_cast<uintptr_t>(p) & 3) == 0 && (bytes & 3) == 0) {
+        const uint32_t b = static_cast<uint8_t>(v);
+        const uint32_t w = b | (b << 8) | (b << 16) | (b << 24);
+        fill_u32_kernel<<<grid_for(bytes / 4, 256, 4096), 256, 0, ctx->stream>>>(
+            static_cast<uint32_t*>(p), w, bytes / 4);
+    } else {
+        fill_u8_kernel<<<grid_for(bytes, 256, 4096), 256, 0, ctx->stream>>>(
+            static_cast<uint8_t*>(p), static_cast<uint8_t>(v), bytes);
+    }
+    QE_HIP(hipGetLastError());
+}
+
+qe_dev_col* new_col(qe_gpu_ctx* ctx, int type, uint64_t len, bool nullable) {
+    auto* c = new qe_dev_col();
+    c->type = type;
+    c->len = len;
+    c->data = dev_alloc(ctx, std::max<size_t>(len * type_width(type), 16));
+    if (nullable) {
+        c->validity = static_cast<uint64_t*>(dev_alloc(ctx, std::max<size_t>(words_for(len) * 8, 8)));
+        c->all_valid = false;
+    }
+    return c;
+}
+
+void free_col(qe_gpu_ctx* ctx, qe_dev_col* c) {
+    if (!c) return;
+    if (c->owns) {
+        dev_free(ctx, c->data);
+        dev_free(ctx, c->validity);
+    }
+    delete c;
+}
+
+qe_dev_table* new_table(uint64_t rows) {
+    auto* t = new qe_dev_table();
+    t->rows = rows;
+    return t;
+}
+
+__global__ void pack_validity_kernel(const uint8_t* vb, uint64_t n, uint64_t* words,
+                                     unsigned long long* nulls) {
+    const uint64_t nw = words_for(n);
+    uint64_t local = 0;
+    for (uint64_t w = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; w < nw;
+         w += uint64_t(gridDim.x) * blockDim.x) {
+        uint64_t bits = 0;
+        const uint64_t base = w * 64;
+        for (unsigned b = 0; b < 64; ++b) {
+            const uint64_t r = base + b;
+            // Padding bits past n are set (canonical all-valid padding).
+            const bool v = (r >= n) || vb[r];
+            bits |= (uint64_t(v) << b);
+            if (r < n && !vb[r]) ++local;
+        }
+        words[w] = bits;
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) local += __shfl_down(local, off);
+    if (lane_id() == 0 && local) atomicAdd(nulls, static_cast<unsigned long long>(local));
+}
+
+void pack_validity(qe_gpu_ctx* ctx, qe_dev_col* col, const uint8_t* valid_bytes) {
+    if (col->validity == nullptr) return;
+    if (col->len == 0) {
+        dev_free(ctx, col->validity);
+        col->validity = nullptr;
+        col->all_valid = true;
+        return;
+    }
+    Scratch cnt(ctx, 8);
+    memset_async(ctx, cnt.p, 0, 8);
+    pack_validity_kernel<<<grid_for(words_for(col->len), 256, 2048), 256, 0, ctx->stream>>>(
+        valid_bytes, col->len, col->validity, cnt.as<unsigned long long>());
+    QE_HIP(hipGetLastError());
+    const uint64_t nulls = read_scalar(ctx, cnt.as<uint64_t>());
+    if (nulls == 0) {
+        dev_free(ctx, col->validity);
+        col->validity = nullptr;
+        col->all_valid = true;
+    } else {
+        col->all_valid = false;
+    }
+}
+
+TimedRegion::TimedRegion(qe_gpu_ctx* c, const char* n) : ctx(c), name(n) {
+    if (!ctx->timing) return;
+    QE_HIP(hipEventCreate(&a));
+    QE_HIP(hipEventCreate(&b));
+    QE_HIP(hipEventRecord(a, ctx->stream));
+}
+TimedRegion::~TimedRegion() {
+    if (!ctx->timing || !a) return;
+    hipEventRecord(b, ctx->stream);
+    hipEventSynchronize(b);
+    float ms = 0;
+    hipEventElapsedTime(&ms, a, b);
+    auto& t = ctx->timings[name];
+    t.ms += ms;
+    t.launches += 1;
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+}
+
+uint64_t common_len(const qe_dev_col* const* cols, uint32_t n) {
+    invalid(n == 0 || cols != nullptr, "null column array");
+    uint64_t len = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        invalid(cols[i] != nullptr, "null column");
+        if (i == 0) len = cols[i]->len;
+        invalid(cols[i]->len == len, "columns of one input must have equal length");
+    }
+    return len;
+}
+
+// ---- scan ------------------------------------------------------------------------
+constexpr unsigned kScanBlock = 256;
+constexpr unsigned kScanItems = 16;
+constexpr uint64_t kScanTile = uint64_t(kScanBlock) * kScanItems;
+
+template <typename T>
+__global__ void scan_tile_sums(const T* in, uint64_t n, uint64_t* sums) {
+    __shared__ uint64_t red[kScanBlock / kWave];
+    const uint64_t base = uint64_t(blockIdx.x) * kScanTile;
+    uint64_t s = 0;
+    for (unsigned k = 0; k < kScanItems; ++k) {
+        const uint64_t i = base + uint64_t(k) * kScanBlock + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_down(s, off);
+    if (lane_id() == 0) red[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t t = 0;
+        for (unsigned w = 0; w < kScanBlock / kWave; ++w) t += red[w];
+        sums[blockIdx.x] = t;
+    }
+}
+
+// Single block: exclusive scan of sums[0..m) in place; total to sums[m].
+__global__ void scan_sums_single(uint64_t* sums, uint64_t m) {
+    __shared__ uint64_t buf[1024];
+    __shared__ uint64_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (uint64_t base = 0; base < m; base += 1024) {
+        const uint64_t i = base + threadIdx.x;
+        const uint64_t v = (i < m) ? sums[i] : 0;
+        buf[threadIdx.x] = v;
+        __syncthreads();
+        for (unsigned off = 1; off < 1024; off <<= 1) {
+            uint64_t t = (threadIdx.x >= off) ? buf[threadIdx.x - off] : 0;
+            __syncthreads();
+            buf[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < m) sums[i] = carry + buf[threadIdx.x] - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += buf[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sums[m] = carry;
+}
+
+template <typename T>
+__global__ void scan_tile_apply(const T* in, uint64_t* out, uint64_t n, const uint64_t* sums) {
+    // Blocked arrangement: thread t owns items [t*K, t*K+K) of the tile.
+    __shared__ uint64_t wsum[kScanBlock / kWave];
+    const uint64_t base = uint64_t(blockIdx.x) * kScanTile + uint64_t(threadIdx.x) * kScanItems;
+    uint64_t v[kScanItems];
+    uint64_t local = 0;
+    for (unsigned k = 0; k < kScanItems; ++k) {
+        const uint64_t i = base + k;
+        v[k] = (i < n) ? static_cast<uint64_t>(in[i]) : 0;
+        local += v[k];
+    }
+    // Exclusive scan of `local` across the block.
+    uint64_t incl = local;
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint64_t t = __shfl_up(incl, off);
+        if (lane_id() >= off) incl += t;
+    }
+    const unsigned w = threadIdx.x / kWave;
+    if (lane_id() == kWave - 1) wsum[w] = incl;
+    __syncthreads();
+    uint64_t wbase = 0;
+    for (unsigned j = 0; j < w; ++j) wbase += wsum[j];
+    uint64_t run = sums[blockIdx.x] + wbase + incl - local;
+    for (unsigned k = 0; k < kScanItems; ++k) {
+        const uint64_t i = base + k;
+        if (i < n) out[i] = run;
+        run += v[k];
+    }
+}
+
+template <typename T>
+static uint64_t exclusive_scan_impl(qe_gpu_ctx* ctx, const T* in, uint64_t* out, uint64_t n) {
+    if (n == 0) return 0;
+    const uint64_t tiles = (n + kScanTile - 1) / kScanTile;
+    Scratch sums(ctx, (tiles + 1) * sizeof(uint64_t));
+    scan_tile_sums<T><<<tiles, kScanBlock, 0, ctx->stream>>>(in, n, sums.as<uint64_t>());
+    scan_sums_single<<<1, 1024, 0, ctx->stream>>>(sums.as<uint64_t>(), tiles);
+    scan_tile_apply<T><<<tiles, kScanBlock, 0, ctx->stream>>>(in, out, n, sums.as<uint64_t>());
+    QE_HIP(hipGetLastError());
+    return read_scalar(ctx, sums.as<uint64_t>() + tiles);
+}
+
+uint64_t exclusive_scan_u32(qe_gpu_ctx* ctx, const uint32_t* in, uint64_t* out, uint64_t n) {
+    return exclusive_scan_impl<uint32_t>(ctx, in, out, n);
+}
+uint64_t exclusive_scan_u64(qe_gpu_ctx* ctx, const uint64_t* in, uint64_t* out, uint64_t n) {
+    if (in == out && n > 0) {
+        // In place: copy first (the apply kernel reads `in` while writing `out`).
+        Scratch tmp(ctx, n * 8);
+        QE_HIP(hipMemcpyAsync(tmp.p, in, n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+        return exclusive_scan_impl<uint64_t>(ctx, tmp.as<uint64_t>(), out, n);
+    }
+    return exclusive_scan_impl<uint64_t>(ctx, in, out, n);
+}
+
+// ---- min/max over valid rows -----------------------------------------------------
+__global__ void minmax_kernel(ColView c, uint64_t n, unsigned long long* out) {
+    int64_t mn = INT64_MAX, mx = INT64_MIN;
+    uint64_t cnt = 0;
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n;
+         i += uint64_t(gridDim.x) * blockDim.x) {
+        if (!is_valid(c, i)) continue;
+        const int64_t v = int_value(c, i);
+        mn = v < mn ? v : mn;
+        mx = v > mx ? v : mx;
+        ++cnt;
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const int64_t a = __shfl_down(mn, off), b = __shfl_down(mx, off);
+        const uint64_t d = __shfl_down(cnt, off);
+        mn = a < mn ? a : mn;
+        mx = b > mx ? b : mx;
+        cnt += d;
+    }
+    if (lane_id() == 0) {
+        atomicMin(reinterpret_cast<long long*>(out), static_cast<long long>(mn));
+        atomicMax(reinterpret_cast<long long*>(out + 1), static_cast<long long>(mx));
+        atomicAdd(out + 2, static_cast<unsigned long long>(cnt));
+    }
+}
+__global__ void minmax_init(unsigned long long* out) {
+    out[0] = static_cast<unsigned long long>(INT64_MAX);
+    out[1] = static_cast<unsigned long long>(INT64_MIN);
+    out[2] = 0;
+}
+
+MinMax column_minmax(qe_gpu_ctx* ctx, const qe_dev_col* c, uint64_t n) {
+    Scratch s(ctx, 3 * sizeof(uint64_t));
+    auto* o = s.as<unsigned long long>();
+    minmax_init<<<1, 1, 0, ctx->stream>>>(o);
+    minmax_kernel<<<grid_for(n, 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(view(c), n, o);
+    QE_HIP(hipGetLastError());
+    unsigned long long* h = static_cast<unsigned long long*>(pinned(ctx, 24));
+    QE_HIP(hipMemcpyAsync(h, o, 24, hipMemcpyDeviceToHost, ctx->stream));
+    QE_HIP(hipStreamSynchronize(ctx->stream));
+    return MinMax{static_cast<int64_t>(h[0]), static_cast<int64_t>(h[1]), h[2]};
+}
+
+}  // namespace qe_gpu
+
+// =================================================================================
+// C ABI
+extern "C" {
+
+qe_gpu_status qe_gpu_ctx_create(int device, qe_gpu_ctx** out) {
+    if (!out) return QE_GPU_INVALID_ARGUMENT;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        return QE_GPU_HIP_ERROR;
+    }
+    if (device < 0 || device >= n) return QE_GPU_INVALID_ARGUMENT;
+    if (hipSetDevice(device) != hipSuccess) return QE_GPU_HIP_ERROR;
+    auto* ctx = new qe_gpu_ctx();
+    ctx->device = device;
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return QE_GPU_HIP_ERROR;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+    *out = ctx;
+    return QE_GPU_OK;
+}
+
+void qe_gpu_ctx_destroy(qe_gpu_ctx* ctx) {
+    if (!ctx) return;
+    hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->free_blocks) hipFree(kv.second);
+    for (auto& kv : ctx->live_blocks) hipFree(kv.first);
+    if (ctx->pinned) hipHostFree(ctx->pinned);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* qe_gpu_last_error(const qe_gpu_ctx* ctx) {
+    return ctx ? ctx->last_error.c_str() : "null context";
+}
+
+qe_gpu_status qe_gpu_sync(qe_gpu_ctx* ctx) {
+    QE_ENTRY(ctx, QE_HIP(hipStreamSynchronize(ctx->stream)));
+}
+
+qe_gpu_status qe_gpu_set_mem_limit(qe_gpu_ctx* ctx, uint64_t bytes) {
+    QE_ENTRY(ctx, ctx->mem_limit = bytes);
+}
+
+qe_gpu_status qe_gpu_set_timing(qe_gpu_ctx* ctx, int enabled) {
+    QE_ENTRY(ctx, ctx->timing = enabled != 0);
+}
+
+qe_gpu_status qe_gpu_timing(qe_gpu_ctx* ctx, const char* kernel, double* ms, uint64_t* launches) {
+    QE_ENTRY(ctx, {
+        invalid(kernel != nullptr, "null kernel name");
+        auto it = ctx->timings.find(kernel);
+        if (ms) *ms = (it == ctx->timings.end()) ? 0.0 : it->second.ms;
+        if (launches) *launches = (it == ctx->timings.end()) ? 0 : it->second.launches;
+        if (it != ctx->timings.end()) ctx->timings.erase(it);
+    });
+}
+
+qe_gpu_status qe_gpu_column_alloc(qe_gpu_ctx* ctx, int32_t type, uint64_t len, int32_t nullable,
+                                  qe_dev_col** out) {
+    QE_ENTRY(ctx, {
+        invalid(out != nullptr, "null out");
+        invalid(valid_type(type), "unknown column type");
+        *out = new_col(ctx, type, len, nullable != 0);
+        if (nullable) memset_async(ctx, (*out)->validity, 0xFF, words_for(len) * 8);
+    });
+}
+
+qe_gpu_status qe_gpu_column_upload(qe_gpu_ctx* ctx, const qe_col_desc* d, qe_dev_col** out) {
+    QE_ENTRY(ctx, {
+        invalid(d != nullptr && out != nullptr, "null descriptor");
+        invalid(valid_type(d->type), "unknown column type");
+        invalid(d->len == 0 || d->data != nullptr, "null data with nonzero length");
+        // core/column.h:42-54: (all_valid == false, validity == nullptr) is invalid.
+        invalid(d->all_valid || d->validity != nullptr,
+                "column claims nulls but has no validity bitmap");
+        const bool nullable = !d->all_valid;
+        qe_dev_col* c = new_col(ctx, d->type, d->len, nullable);
+        const size_t bytes = d->len * type_width(d->type);
+        if (bytes) QE_HIP(hipMemcpyAsync(c->data, d->data, bytes, hipMemcpyHostToDevice, ctx->stream));
+        if (nullable && d->len) {
+            QE_HIP(hipMemcpyAsync(c->validity, d->validity, words_for(d->len) * 8,
+                                  hipMemcpyHostToDevice, ctx->stream));
+        }
+        QE_HIP(hipStreamSynchronize(ctx->stream));
+        *out = c;
+    });
+}
+
+qe_gpu_status qe_gpu_column_download(qe_gpu_ctx* ctx, const qe_dev_col* c, void* data,
+                                     uint64_t* validity, int32_t* all_valid) {
+    QE_ENTRY(ctx, {
+        invalid(c != nullptr, "null column");
+        const size_t bytes = c->len * type_width(c->type);
+        if (data && bytes)
+            QE_HIP(hipMemcpyAsync(data, c->data, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        if (validity && c->len) {
+            if (c->all_valid) {
+                QE_HIP(hipStreamSynchronize(ctx->stream));
+                for (uint64_t w = 0; w < words_for(c->len); ++w) validity[w] = ~0ull;
+            } else {
+                QE_HIP(hipMemcpyAsync(validity, c->validity, words_for(c->len) * 8,
+                                      hipMemcpyDeviceToHost, ctx->stream));
+            }
+        }
+        if (all_valid) *all_valid = c->all_valid ? 1 : 0;
+        QE_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+qe_gpu_status qe_gpu_column_download_range(qe_gpu_ctx* ctx, const qe_dev_col* c, uint64_t offset,
+                                           uint64_t count, void* data, uint8_t* valid_bytes) {
+    QE_ENTRY(ctx, {
+        invalid(c != nullptr, "null column");
+        invalid(offset + count <= c->len, "range out of bounds");
+        const size_t w = type_width(c->type);
+        if (data && count)
+            QE_HIP(hipMemcpyAsync(data, static_cast<const char*>(c->data) + offset * w, count * w,
+                                  hipMemcpyDeviceToHost, ctx->stream));
+        std::vector<uint64_t> words;
+        const uint64_t w0 = offset / 64, w1 = words_for(offset + count);
+        if (valid_bytes && count && !c->all_valid) {
+            words.resize(w1 - w0);
+            QE_HIP(hipMemcpyAsync(words.data(), c->validity + w0, (w1 - w0) * 8,
+                                  hipMemcpyDeviceToHost, ctx->stream));
+        }
+        QE_HIP(hipStreamSynchronize(ctx->stream));
+        if (valid_bytes) {
+            for (uint64_t r = 0; r < count; ++r) {
+                if (c->all_valid) {
+                    valid_bytes[r] = 1;
+                } else {
+                    const uint64_t g = offset + r;
+                    valid_bytes[r] = (words[(g >> 6) - w0] >> (g & 63)) & 1ull;
+                }
+            }
+        }
+    });
+}
+
+void qe_gpu_column_free(qe_gpu_ctx* ctx, qe_dev_col* c) {
+    if (ctx) free_col(ctx, c);
+}
+uint64_t qe_gpu_column_len(const qe_dev_col* c) { return c ? c->len : 0; }
+int32_t qe_gpu_column_type(const qe_dev_col* c) { return c ? c->type : -1; }
+int32_t qe_gpu_column_all_valid(const qe_dev_col* c) { return c ? (c->all_valid ? 1 : 0) : 0; }
+
+qe_gpu_status qe_gpu_column_slice(qe_gpu_ctx* ctx, const qe_dev_col* c, uint64_t offset, uint64_t len,
+                                  qe_dev_col** out) {
+    QE_ENTRY(ctx, {
+        invalid(c != nullptr && out != nullptr, "null column");
+        invalid(offset % 64 == 0, "slice offset must be a multiple of 64");
+        invalid(offset + len <= c->len, "slice out of bounds");
+        auto* s = new qe_dev_col();
+        s->type = c->type;
+        s->len = len;
+        s->data = static_cast<char*>(c->data) + offset * type_width(c->type);
+        s->all_valid = c->all_valid;
+        s->validity = c->all_valid ? nullptr : c->validity + offset / 64;
+        s->owns = false;
+        *out = s;
+    });
+}
+
+uint64_t qe_gpu_table_rows(const qe_dev_table* t) { return t ? t->rows : 0; }
+uint32_t qe_gpu_table_cols(const qe_dev_table* t) {
+    return t ? static_cast<uint32_t>(t->cols.size()) : 0;
+}
+const qe_dev_col* qe_gpu_table_col(const qe_dev_table* t, uint32_t i) {
+    return (t && i < t->cols.size()) ? t->cols[i] : nullptr;
+}
+void qe_gpu_table_free(qe_gpu_ctx* ctx, qe_dev_table* t) {
+    if (!ctx || !t) return;
+    for (auto* c : t->cols) free_col(ctx, c);
+    delete t;
+}
+
+uint64_t qe_gpu_pairs_count(const qe_dev_pairs* p) { return p ? p->count : 0; }
+
+qe_gpu_status qe_gpu_pairs_download(qe_gpu_ctx* ctx, const qe_dev_pairs* p, uint64_t offset,
+                                    uint64_t count, uint32_t* probe_rows, uint32_t* build_rows) {
+    QE_ENTRY(ctx, {
+        invalid(p != nullptr, "null pairs");
+        invalid(offset + count <= p->count, "pair range out of bounds");
+        if (count && probe_rows)
+            QE_HIP(hipMemcpyAsync(probe_rows, p->probe + offset, count * 4, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+        if (count && build_rows)
+            QE_HIP(hipMemcpyAsync(build_rows, p->build + offset, count * 4, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+        QE_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+void qe_gpu_pairs_free(qe_gpu_ctx* ctx, qe_dev_pairs* p) {
+    if (!ctx || !p) return;
+    dev_free(ctx, p->probe);
+    dev_free(ctx, p->build);
+    delete p;
+}
+
+qe_gpu_status qe_gpu_malloc(qe_gpu_ctx* ctx, uint64_t bytes, void** out) {
+    QE_ENTRY(ctx, {
+        invalid(out != nullptr, "null out");
+        *out = dev_alloc(ctx, bytes);
+    });
+}
+qe_gpu_status qe_gpu_free(qe_gpu_ctx* ctx, void* p) { QE_ENTRY(ctx, dev_free(ctx, p)); }
+qe_gpu_status qe_gpu_memcpy_d2h(qe_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+    QE_ENTRY(ctx, {
+        if (bytes) QE_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        QE_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+qe_gpu_status qe_gpu_memcpy_h2d(qe_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
+    QE_ENTRY(ctx, {
+        if (bytes) QE_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        QE_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,440 @@
+// Hash group-by aggregation: qe_gpu_aggregate replaces qe::Aggregate
+// (ops/aggregate.h:110-111; drain_and_build ops/aggregate.cpp:243-437).
+//
+// Strategy, chosen per call from the data (not assumed):
+//  * zero keys      -> one slot; per-block LDS partials, block-order merge;
+//  * one integer-like key (I32/I64/TS/BOOL/STR code) whose value range R is
+//    small -> direct slots (key - min, plus one NULL slot): LDS partials when
+//    (R+1) slots fit the LDS budget, HBM atomics otherwise;
+//  * anything else  -> a device open-addressing hash table (canonical key words,
+//    NULLs group together as in NullPolicy::kEqual) with HBM atomics.
+// Output: key columns ++ aggregate columns, one row per group, unordered.
+#include "agg_common.hpp"
+#include "hash_common.hpp"
+
+#include <algorithm>
+
+using namespace qe_gpu;
+
+namespace qe_gpu {
+
+AggProg make_prog(const qe_agg_spec* aggs, uint32_t naggs, const ColView* inputs,
+                  const int* input_types) {
+    invalid(naggs >= 1, "aggregate needs >=1 aggregate");
+    check(naggs <= kMaxAggs, QE_GPU_UNSUPPORTED, "too many aggregates for the device path");
+    AggProg p{};
+    p.n = static_cast<int32_t>(naggs);
+    for (uint32_t a = 0; a < naggs; ++a) {
+        const int f = aggs[a].func;
+        invalid(f >= QE_AGG_COUNT_STAR && f <= QE_AGG_AVG, "unknown aggregate function");
+        p.func[a] = f;
+        if (f == QE_AGG_COUNT_STAR) {
+            p.kind[a] = K_COUNT_STAR;
+            p.in[a] = ColView{nullptr, nullptr, QE_I64, 0};
+            continue;
+        }
+        const int t = input_types[a];
+        p.in[a] = inputs[a];
+        if (f == QE_AGG_COUNT) {
+            p.kind[a] = K_COUNT;
+            continue;
+        }
+        if (t == QE_STR) {
+            // ops/aggregate.cpp:84-90: SUM/AVG(STR) are type errors.
+            invalid(f != QE_AGG_SUM && f != QE_AGG_AVG,
+                    f == QE_AGG_SUM ? "SUM is not defined on STR (VARCHAR)"
+                                    : "AVG is not defined on STR (VARCHAR)");
+            throw Error{QE_GPU_UNSUPPORTED, "MIN/MAX of STR compare by dictionary value; not on the device path"};
+        }
+        const bool fl = (t == QE_F64);
+        switch (f) {
+            case QE_AGG_SUM:
+            case QE_AGG_AVG:
+                p.kind[a] = fl ? K_SUM_F : K_SUM_I;
+                break;
+            case QE_AGG_MIN:
+                p.kind[a] = fl ? K_MIN_F : K_MIN_I;
+                break;
+            case QE_AGG_MAX:
+                p.kind[a] = fl ? K_MAX_F : K_MAX_I;
+                break;
+        }
+    }
+    p.nacc = p.ncnt = 0;
+    for (uint32_t a = 0; a < naggs; ++a) {
+        const int k = p.kind[a];
+        p.acc_of[a] = -1;
+        p.cnt_of[a] = -1;
+        if (k != K_COUNT_STAR && k != K_COUNT) {
+            p.acc_kind[p.nacc] = k;
+            p.acc_of[a] = p.nacc++;
+        }
+        if (k != K_COUNT_STAR && p.in[a].validity != nullptr) p.cnt_of[a] = p.ncnt++;
+    }
+    return p;
+}
+
+__global__ void init_state_kernel(uint64_t* st, uint64_t S, AggProg p) {
+    for (uint64_t s = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; s < S;
+         s += uint64_t(gridDim.x) * blockDim.x) {
+        st[s] = 0;
+        for (int i = 0; i < p.nacc; ++i) st[(1 + size_t(i)) * S + s] = acc_identity(p.acc_kind[i]);
+        for (int c = 0; c < p.ncnt; ++c) st[(1 + size_t(p.nacc) + c) * S + s] = 0;
+    }
+}
+
+void init_state(qe_gpu_ctx* ctx, uint64_t* st, uint64_t S, const AggProg& p) {
+    init_state_kernel<<<grid_for(S, 256, 8192), 256, 0, ctx->stream>>>(st, S, p);
+    QE_HIP(hipGetLastError());
+}
+
+__device__ inline uint64_t combine(int kind, uint64_t a, uint64_t b) {
+    switch (kind) {
+        case K_SUM_I:
+            return a + b;
+        case K_SUM_F: {
+            double x, y;
+            memcpy(&x, &a, 8);
+            memcpy(&y, &b, 8);
+            x += y;
+            uint64_t r;
+            memcpy(&r, &x, 8);
+            return r;
+        }
+        case K_MIN_I:
+            return static_cast<int64_t>(a) < static_cast<int64_t>(b) ? a : b;
+        case K_MAX_I:
+            return static_cast<int64_t>(a) > static_cast<int64_t>(b) ? a : b;
+        case K_MIN_F:
+            return a < b ? a : b;
+        case K_MAX_F:
+            return a > b ? a : b;
+        default:
+            return 0;
+    }
+}
+
+// One thread per (field, slot); partial blocks folded in block order so the
+// FP64 result is reproducible run to run.
+__global__ void merge_partials_kernel(const uint64_t* part, uint32_t nblocks, uint64_t* st,
+                                      uint64_t S, AggProg p) {
+    const uint64_t W = state_words(S, p);
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < W;
+         i += uint64_t(gridDim.x) * blockDim.x) {
+        const uint64_t field = i / S;
+        uint64_t v;
+        if (field == 0 || field > uint64_t(p.nacc)) {  // rows or cnt: integer sums
+            v = 0;
+            for (uint32_t b = 0; b < nblocks; ++b) v += part[size_t(b) * W + i];
+        } else {
+            const int kind = p.acc_kind[field - 1];
+            v = acc_identity(kind);
+            for (uint32_t b = 0; b < nblocks; ++b) v = combine(kind, v, part[size_t(b) * W + i]);
+        }
+        st[i] = v;
+    }
+}
+
+void merge_partials(qe_gpu_ctx* ctx, const uint64_t* part, uint32_t nblocks, uint64_t* st,
+                    uint64_t S, const AggProg& p) {
+    const uint64_t W = state_words(S, p);
+    merge_partials_kernel<<<grid_for(W, 256, 8192), 256, 0, ctx->stream>>>(part, nblocks, st, S, p);
+    QE_HIP(hipGetLastError());
+}
+
+__global__ void group_flags_kernel(const uint64_t* rows, uint64_t S, uint32_t* flags, int all_slots) {
+    for (uint64_t s = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; s < S;
+         s += uint64_t(gridDim.x) * blockDim.x)
+        flags[s] = (all_slots || rows[s] > 0) ? 1u : 0u;
+}
+
+struct OutCols {
+    void* data[kMaxKeys + kMaxAggs];
+    uint8_t* valid[kMaxKeys + kMaxAggs];
+    int32_t type[kMaxKeys + kMaxAggs];
+};
+
+__device__ inline void store_typed(void* base, int type, uint64_t o, uint64_t bits) {
+    switch (type) {
+        case QE_I32:
+        case QE_STR:
+            static_cast<int32_t*>(base)[o] = static_cast<int32_t>(bits);
+            break;
+        case QE_BOOL:
+            static_cast<uint8_t*>(base)[o] = static_cast<uint8_t>(bits & 1);
+            break;
+        default:
+            static_cast<uint64_t*>(base)[o] = bits;
+    }
+}
+
+__global__ void emit_kernel(const uint64_t* st, uint64_t S, AggProg p, KeyEmit ke,
+                            const uint32_t* flags, const uint64_t* offs, OutCols oc) {
+    for (uint64_t s = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; s < S;
+         s += uint64_t(gridDim.x) * blockDim.x) {
+        if (!flags[s]) continue;
+        const uint64_t o = offs[s];
+        for (int k = 0; k < ke.nkeys; ++k) {
+            bool isnull;
+            uint64_t w;
+            if (ke.mode == 0) {
+                isnull = (s == ke.null_slot);
+                w = static_cast<uint64_t>(ke.kmin + static_cast<int64_t>(s));
+            } else {
+                isnull = (ke.nullmask[s] >> k) & 1ull;
+                w = ke.words[size_t(k) * S + s];
+            }
+            oc.valid[k][o] = isnull ? 0 : 1;
+            store_typed(oc.data[k], ke.types[k], o, isnull ? 0 : w);
+        }
+        const uint64_t rows = st[s];
+        for (int a = 0; a < p.n; ++a) {
+            const int c = ke.nkeys + a;
+            const uint64_t acc = p.acc_of[a] >= 0 ? st[(1 + size_t(p.acc_of[a])) * S + s] : 0;
+            const uint64_t cnt =
+                p.cnt_of[a] >= 0 ? st[(1 + size_t(p.nacc) + p.cnt_of[a]) * S + s] : rows;
+            const int f = p.func[a];
+            const int k = p.kind[a];
+            uint64_t out = 0;
+            bool valid = true;
+            if (f == QE_AGG_COUNT_STAR) {
+                out = rows;
+            } else if (f == QE_AGG_COUNT) {
+                out = cnt;
+            } else if (cnt == 0) {
+                valid = false;  // no non-NULL input: SUM/MIN/MAX/AVG are NULL
+            } else if (f == QE_AGG_AVG) {
+                double sum;
+                if (k == K_SUM_F) memcpy(&sum, &acc, 8);
+                else sum = static_cast<double>(static_cast<int64_t>(acc));
+                const double avg = sum / static_cast<double>(cnt);
+                memcpy(&out, &avg, 8);
+            } else if (k == K_MIN_F || k == K_MAX_F) {
+                const double d = f64_from_order_key(acc);
+                memcpy(&out, &d, 8);
+            } else {
+                out = acc;
+            }
+            oc.valid[c][o] = valid ? 1 : 0;
+            store_typed(oc.data[c], oc.type[c], o, valid ? out : 0);
+        }
+    }
+}
+
+qe_dev_table* emit_groups(qe_gpu_ctx* ctx, const uint64_t* st, uint64_t S, const AggProg& p,
+                          const KeyEmit& ke, bool all_slots) {
+    Scratch flags(ctx, S * 4), offs(ctx, S * 8);
+    group_flags_kernel<<<grid_for(S, 256, 8192), 256, 0, ctx->stream>>>(st, S, flags.as<uint32_t>(),
+                                                                         all_slots ? 1 : 0);
+    QE_HIP(hipGetLastError());
+    const uint64_t G = exclusive_scan_u32(ctx, flags.as<uint32_t>(), offs.as<uint64_t>(), S);
+    qe_dev_table* t = new_table(G);
+    const int ncol = ke.nkeys + p.n;
+    OutCols oc{};
+    std::vector<Scratch> vb;
+    vb.reserve(ncol);
+    try {
+        for (int c = 0; c < ncol; ++c) {
+            int type;
+            if (c < ke.nkeys) {
+                type = ke.types[c];
+            } else {
+                const int a = c - ke.nkeys;
+                type = agg_result_type(p.func[a], p.kind[a] == K_COUNT_STAR ? QE_I64 : p.in[a].type);
+            }
+            qe_dev_col* col = new_col(ctx, type, G, true);
+            t->cols.push_back(col);
+            vb.emplace_back(ctx, std::max<uint64_t>(G, 1));
+            oc.data[c] = col->data;
+            oc.valid[c] = vb.back().as<uint8_t>();
+            oc.type[c] = type;
+        }
+        if (G > 0) {
+            emit_kernel<<<grid_for(S, 256, 8192), 256, 0, ctx->stream>>>(st, S, p, ke, flags.as<uint32_t>(),
+                                                                          offs.as<uint64_t>(), oc);
+            QE_HIP(hipGetLastError());
+        }
+        for (int c = 0; c < ncol; ++c) pack_validity(ctx, t->cols[c], vb[c].as<uint8_t>());
+    } catch (...) {
+        qe_gpu_table_free(ctx, t);
+        throw;
+    }
+    return t;
+}
+
+// ---- direct-slot kernels -----------------------------------------------------------
+struct DirectSlot {
+    ColView key;
+    int64_t kmin;
+    uint32_t null_slot;
+    __device__ uint32_t operator()(uint64_t r) const {
+        if (!is_valid(key, r)) return null_slot;
+        return static_cast<uint32_t>(static_cast<uint64_t>(int_value(key, r) - kmin));
+    }
+};
+struct GlobalSlot {
+    __device__ uint32_t operator()(uint64_t) const { return 0; }
+};
+
+template <class SlotFn>
+__global__ void __launch_bounds__(512) agg_lds_kernel(uint64_t n, SlotFn slot_of, uint32_t S, AggProg p,
+                                                      uint64_t* part) {
+    extern __shared__ __align__(16) char smem[];
+    lds_init(smem, S, p);
+    __syncthreads();
+    uint64_t vrow[kMaxAggs];
+    for (uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < n;
+         r += uint64_t(gridDim.x) * blockDim.x) {
+        for (int a = 0; a < p.n; ++a) vrow[a] = r;
+        lds_update(smem, S, p, slot_of(r), vrow);
+    }
+    __syncthreads();
+    lds_flush(smem, S, p, part);
+}
+
+template <class SlotFn>
+__global__ void agg_global_kernel(uint64_t n, SlotFn slot_of, uint64_t S, AggProg p, uint64_t* st) {
+    uint64_t vrow[kMaxAggs];
+    for (uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < n;
+         r += uint64_t(gridDim.x) * blockDim.x) {
+        for (int a = 0; a < p.n; ++a) vrow[a] = r;
+        global_update(st, S, p, slot_of(r), vrow);
+    }
+}
+
+template <class SlotFn>
+static void run_slots(qe_gpu_ctx* ctx, uint64_t n, SlotFn fn, uint32_t S, const AggProg& p,
+                      uint64_t* st, const char* timer) {
+    const size_t lb = lds_bytes(S, p);
+    if (lb <= kLdsBudget) {
+        const unsigned block = 512;
+        const unsigned per_cu = std::max<unsigned>(1, static_cast<unsigned>((160 * 1024) / std::max<size_t>(lb, 1)));
+        unsigned blocks = static_cast<unsigned>(std::min<uint64_t>(
+            uint64_t(ctx->num_cus) * std::min(per_cu, 4u), (n + block - 1) / block));
+        if (blocks == 0) blocks = 1;
+        Scratch part(ctx, size_t(blocks) * part_words(S, p) * 8);
+        {
+            TimedRegion tr(ctx, timer);
+            agg_lds_kernel<SlotFn><<<blocks, block, lb, ctx->stream>>>(n, fn, S, p, part.as<uint64_t>());
+            QE_HIP(hipGetLastError());
+        }
+        merge_partials(ctx, part.as<uint64_t>(), blocks, st, S, p);
+    } else {
+        init_state(ctx, st, S, p);
+        TimedRegion tr(ctx, timer);
+        agg_global_kernel<SlotFn><<<grid_for(n, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(n, fn, S, p, st);
+        QE_HIP(hipGetLastError());
+    }
+}
+
+struct HashSlot {
+    HashKeys hk;
+    HashTableView t;
+    __device__ uint32_t operator()(uint64_t r) const {
+        uint64_t w[kMaxKeys];
+        uint64_t nm, h;
+        normalize_row(hk, r, w, nm, h);
+        return ht_insert(t, hk.nkeys, w, nm, h);
+    }
+};
+
+__global__ void agg_hash_kernel(uint64_t n, HashSlot fn, uint64_t S, AggProg p, uint64_t* st) {
+    uint64_t vrow[kMaxAggs];
+    for (uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < n;
+         r += uint64_t(gridDim.x) * blockDim.x) {
+        const uint32_t s = fn(r);
+        if (s == kNoSlot) continue;
+        for (int a = 0; a < p.n; ++a) vrow[a] = r;
+        global_update(st, S, p, s, vrow);
+    }
+}
+
+bool is_int_like(int t) { return t == QE_I32 || t == QE_I64 || t == QE_TS || t == QE_BOOL || t == QE_STR; }
+
+}  // namespace qe_gpu
+
+// Direct-slot limit: at most this many slots before falling back to hashing.
+static constexpr uint64_t kDirectMaxSlots = uint64_t(1) << 24;
+
+extern "C" qe_gpu_status qe_gpu_aggregate(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, uint32_t ncols,
+                                          const uint32_t* key_cols, uint32_t nkeys,
+                                          const qe_agg_spec* aggs, uint32_t naggs, qe_dev_table** out) {
+    QE_ENTRY(ctx, {
+        invalid(out != nullptr, "null out");
+        invalid(ncols >= 1, "aggregate needs an input with >=1 column");
+        invalid(aggs != nullptr || naggs == 0, "null aggregate list");
+        check(nkeys <= kMaxKeys, QE_GPU_UNSUPPORTED, "too many group keys for the device path");
+        const uint64_t n = common_len(cols, ncols);
+        for (uint32_t k = 0; k < nkeys; ++k) invalid(key_cols[k] < ncols, "group key column out of range");
+        ColView ins[kMaxAggs];
+        int types[kMaxAggs];
+        for (uint32_t a = 0; a < naggs && a < uint32_t(kMaxAggs); ++a) {
+            if (aggs[a].func == QE_AGG_COUNT_STAR) {
+                ins[a] = ColView{nullptr, nullptr, QE_I64, 0};
+                types[a] = QE_I64;
+                continue;
+            }
+            invalid(aggs[a].input_col < ncols, "aggregate input column out of range");
+            ins[a] = view(cols[aggs[a].input_col]);
+            types[a] = cols[aggs[a].input_col]->type;
+        }
+        const AggProg p = make_prog(aggs, naggs, ins, types);
+
+        KeyEmit ke{};
+        ke.nkeys = static_cast<int>(nkeys);
+        for (uint32_t k = 0; k < nkeys; ++k) ke.types[k] = cols[key_cols[k]]->type;
+
+        if (nkeys == 0) {
+            // Global aggregate: exactly one output row (ops/aggregate.h:20-41).
+            Scratch st(ctx, state_words(1, p) * 8);
+            run_slots(ctx, n, GlobalSlot{}, 1, p, st.as<uint64_t>(), "agg_global");
+            ke.mode = 0;
+            ke.null_slot = kNoSlot;
+            *out = emit_groups(ctx, st.as<uint64_t>(), 1, p, ke, true);
+            return QE_GPU_OK;
+        }
+
+        const qe_dev_col* k0 = cols[key_cols[0]];
+        if (nkeys == 1 && is_int_like(k0->type)) {
+            const MinMax mm = column_minmax(ctx, k0, n);
+            const uint64_t range = mm.valid ? static_cast<uint64_t>(mm.mx) - static_cast<uint64_t>(mm.mn) + 1 : 0;
+            if (range < kDirectMaxSlots && range <= 4 * n + 1024) {
+                const uint32_t S = static_cast<uint32_t>(range + 1);  // + the NULL slot
+                Scratch st(ctx, state_words(S, p) * 8);
+                DirectSlot fn{view(k0), mm.valid ? mm.mn : 0, S - 1};
+                run_slots(ctx, n, fn, S, p, st.as<uint64_t>(), "agg_direct");
+                ke.mode = 0;
+                ke.kmin = mm.valid ? mm.mn : 0;
+                ke.null_slot = S - 1;
+                *out = emit_groups(ctx, st.as<uint64_t>(), S, p, ke, false);
+                return QE_GPU_OK;
+            }
+        }
+
+        // Hash path.
+        uint64_t cap = 1024;
+        while (cap < 2 * n) cap <<= 1;
+        check(cap <= (uint64_t(1) << 31), QE_GPU_UNSUPPORTED, "group-by input too large for one device table");
+        HashSlot fn{};
+        fn.hk.nkeys = static_cast<int>(nkeys);
+        for (uint32_t k = 0; k < nkeys; ++k) fn.hk.key[k] = view(cols[key_cols[k]]);
+        Scratch state(ctx, cap * 4), hash(ctx, cap * 8), words(ctx, cap * 8 * nkeys), nmask(ctx, cap * 8),
+            ovf(ctx, 8), st(ctx, state_words(cap, p) * 8);
+        memset_async(ctx, state.p, 0, cap * 4);
+        memset_async(ctx, ovf.p, 0, 8);
+        fn.t = HashTableView{state.as<uint32_t>(), hash.as<uint64_t>(), words.as<uint64_t>(),
+                             nmask.as<uint64_t>(), cap, ovf.as<unsigned int>()};
+        init_state(ctx, st.as<uint64_t>(), cap, p);
+        {
+            TimedRegion tr(ctx, "agg_hash");
+            agg_hash_kernel<<<grid_for(n, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(n, fn, cap, p,
+                                                                                           st.as<uint64_t>());
+            QE_HIP(hipGetLastError());
+        }
+        check(read_scalar(ctx, ovf.as<unsigned int>()) == 0, QE_GPU_OVERFLOW, "device group table overflow");
+        ke.mode = 1;
+        ke.words = words.as<uint64_t>();
+        ke.nullmask = nmask.as<uint64_t>();
+        *out = emit_groups(ctx, st.as<uint64_t>(), cap, p, ke, false);
+    });
+}

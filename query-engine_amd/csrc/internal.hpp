@@ -1,0 +1,303 @@
+// Shared internals of libqe_gpu: context, device columns, caching allocator,
+// error plumbing, device helpers (hash, key canonicalisation, wave64 ops) and
+// the device-wide scan. gfx950 only: wave64, 160 KiB LDS per CU.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "qe_gpu.h"
+
+namespace qe_gpu {
+
+constexpr int kWave = 64;
+constexpr uint32_t kNoRow = 0xFFFFFFFFu;
+constexpr uint64_t kNullWord = 0x9E3779B97F4A7C15ull;  // NULL lane word (never compared alone)
+constexpr uint64_t kCanonNaN = 0x7ff8000000000000ull;
+constexpr uint64_t kHashSeed = 0x51E57A11AB1E5EEDull;
+
+// ---------------------------------------------------------------------------
+// Errors: every entry point returns a status; the message lives in the ctx.
+struct Error {
+    qe_gpu_status status;
+    std::string msg;
+};
+
+#define QE_HIP(expr)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            throw ::qe_gpu::Error{e_ == hipErrorOutOfMemory ? QE_GPU_OUT_OF_MEMORY    \
+                                                            : QE_GPU_HIP_ERROR,       \
+                                  std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+inline void check(bool ok, qe_gpu_status st, const std::string& msg) {
+    if (!ok) throw Error{st, msg};
+}
+inline void invalid(bool ok, const std::string& msg) { check(ok, QE_GPU_INVALID_ARGUMENT, msg); }
+
+inline size_t type_width(int t) {
+    switch (t) {
+        case QE_I32:
+        case QE_STR:
+            return 4;
+        case QE_BOOL:
+            return 1;
+        default:
+            return 8;
+    }
+}
+inline bool valid_type(int t) { return t >= QE_I32 && t <= QE_STR; }
+__host__ __device__ inline uint64_t words_for(uint64_t n) { return (n + 63) / 64; }
+
+}  // namespace qe_gpu
+
+// ---------------------------------------------------------------------------
+// Opaque handle bodies (C ABI names).
+struct qe_dev_col {
+    int32_t type = QE_I64;
+    uint64_t len = 0;
+    void* data = nullptr;
+    uint64_t* validity = nullptr;  // nullptr <=> all_valid
+    bool all_valid = true;
+    bool owns = true;  // false for slices
+};
+
+struct qe_dev_table {
+    uint64_t rows = 0;
+    std::vector<qe_dev_col*> cols;
+};
+
+struct qe_dev_pairs {
+    uint64_t count = 0;
+    uint32_t* probe = nullptr;
+    uint32_t* build = nullptr;
+};
+
+struct qe_gpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    uint64_t mem_limit = 0;
+    uint64_t mem_in_use = 0;
+    std::multimap<size_t, void*> free_blocks;  // size -> block (cached)
+    std::map<void*, size_t> live_blocks;       // block -> size
+    bool timing = false;
+    struct Timing {
+        double ms = 0;
+        uint64_t launches = 0;
+    };
+    std::map<std::string, Timing> timings;
+    int num_cus = 256;
+    // Small pinned host staging buffer for scalars read back after kernels.
+    void* pinned = nullptr;
+    size_t pinned_bytes = 0;
+};
+
+namespace qe_gpu {
+
+// ---- caching allocator (per context, stream-ordered use only) ---------------
+void* dev_alloc(qe_gpu_ctx* ctx, size_t bytes);
+void dev_free(qe_gpu_ctx* ctx, void* p);
+
+// RAII scratch buffer.
+struct Scratch {
+    qe_gpu_ctx* ctx = nullptr;
+    void* p = nullptr;
+    size_t bytes = 0;
+    Scratch() = default;
+    Scratch(qe_gpu_ctx* c, size_t b) : ctx(c), p(b ? dev_alloc(c, b) : nullptr), bytes(b) {}
+    Scratch(const Scratch&) = delete;
+    Scratch& operator=(const Scratch&) = delete;
+    Scratch(Scratch&& o) noexcept : ctx(o.ctx), p(o.p), bytes(o.bytes) { o.p = nullptr; }
+    Scratch& operator=(Scratch&& o) noexcept {
+        release();
+        ctx = o.ctx;
+        p = o.p;
+        bytes = o.bytes;
+        o.p = nullptr;
+        return *this;
+    }
+    ~Scratch() { release(); }
+    void release() {
+        if (p) dev_free(ctx, p);
+        p = nullptr;
+    }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+qe_dev_col* new_col(qe_gpu_ctx* ctx, int type, uint64_t len, bool nullable);
+void free_col(qe_gpu_ctx* ctx, qe_dev_col* c);
+void* pinned(qe_gpu_ctx* ctx, size_t bytes);  // ctx-owned pinned staging
+void memset_async(qe_gpu_ctx* ctx, void* p, int v, size_t bytes);  // via fill kernel
+template <typename T>
+T read_scalar(qe_gpu_ctx* ctx, const T* dev) {
+    T* h = static_cast<T*>(pinned(ctx, sizeof(T)));
+    QE_HIP(hipMemcpyAsync(h, dev, sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+    QE_HIP(hipStreamSynchronize(ctx->stream));
+    return *h;
+}
+
+// Kernel timing bracket (events on the ctx stream).
+struct TimedRegion {
+    qe_gpu_ctx* ctx;
+    const char* name;
+    hipEvent_t a = nullptr, b = nullptr;
+    TimedRegion(qe_gpu_ctx* c, const char* n);
+    ~TimedRegion();
+};
+
+// ---- column views passed to kernels ------------------------------------------
+struct ColView {
+    const void* data;
+    const uint64_t* validity;  // nullptr when all_valid
+    int32_t type;
+    int32_t pad;
+};
+inline ColView view(const qe_dev_col* c) {
+    ColView v;
+    v.data = c->data;
+    v.validity = c->all_valid ? nullptr : c->validity;
+    v.type = c->type;
+    v.pad = 0;
+    return v;
+}
+
+// ---- device helpers ------------------------------------------------------------
+__host__ __device__ inline uint64_t mix64(uint64_t x) {
+    // splitmix64 finalizer (ops/hash_kernels.h:7-12 documents the same mixer).
+    x ^= x >> 30;
+    x *= 0xbf58476d1ce4e5b9ull;
+    x ^= x >> 27;
+    x *= 0x94d049bb133111ebull;
+    x ^= x >> 31;
+    return x;
+}
+__host__ __device__ inline uint64_t hash_combine(uint64_t acc, uint64_t w) {
+    return mix64(acc ^ mix64(w));
+}
+
+__device__ inline bool is_valid(const ColView& c, uint64_t r) {
+    return c.validity == nullptr || ((c.validity[r >> 6] >> (r & 63)) & 1ull);
+}
+
+// Canonical key word (same normalisation as ops/hash_internal.cpp:29-60).
+__device__ inline uint64_t canon_f64_bits(uint64_t b) {
+    const uint64_t mag = b & 0x7fffffffffffffffull;
+    if (mag > 0x7ff0000000000000ull) return kCanonNaN;  // any NaN
+    if (mag == 0) return 0;                             // +-0
+    return b;
+}
+__device__ inline uint64_t key_word(const ColView& c, uint64_t r) {
+    switch (c.type) {
+        case QE_I32:
+        case QE_STR:
+            return static_cast<uint32_t>(static_cast<const int32_t*>(c.data)[r]);
+        case QE_BOOL:
+            return static_cast<const uint8_t*>(c.data)[r];
+        case QE_F64:
+            return canon_f64_bits(static_cast<const uint64_t*>(c.data)[r]);
+        default:
+            return static_cast<const uint64_t*>(c.data)[r];
+    }
+}
+// Integer value (I32 sign-extended) for SUM/MIN/MAX of integer families.
+__device__ inline int64_t int_value(const ColView& c, uint64_t r) {
+    switch (c.type) {
+        case QE_I32:
+        case QE_STR:
+            return static_cast<const int32_t*>(c.data)[r];
+        case QE_BOOL:
+            return static_cast<const uint8_t*>(c.data)[r];
+        default:
+            return static_cast<const int64_t*>(c.data)[r];
+    }
+}
+__device__ inline double f64_value(const ColView& c, uint64_t r) {
+    return static_cast<const double*>(c.data)[r];
+}
+
+// Order-preserving u64 image of a double under the NaN-greatest total order
+// (ops/agg_internal.h:34-40): NaN is canonicalised to +qNaN, which lands
+// above +inf. -0.0 maps just below +0.0 (numerically equal values).
+__host__ __device__ inline uint64_t f64_order_key(double d) {
+    uint64_t b;
+    memcpy(&b, &d, 8);
+    if ((b & 0x7fffffffffffffffull) > 0x7ff0000000000000ull) b = kCanonNaN;
+    return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+__host__ __device__ inline double f64_from_order_key(uint64_t k) {
+    const uint64_t b = (k & 0x8000000000000000ull) ? (k & 0x7fffffffffffffffull) : ~k;
+    double d;
+    memcpy(&d, &b, 8);
+    return d;
+}
+
+__device__ inline int lane_id() { return __lane_id(); }
+
+// ---- launch geometry ----------------------------------------------------------
+inline unsigned grid_for(uint64_t n, unsigned block, unsigned max_blocks = 65535u * 16) {
+    uint64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > max_blocks) g = max_blocks;
+    return static_cast<unsigned>(g);
+}
+
+// ---- device-wide exclusive scan of u32 flags/counts into u64 offsets ---------
+// out[i] = sum_{j<i} in[j]; returns the total (synchronises the stream).
+uint64_t exclusive_scan_u32(qe_gpu_ctx* ctx, const uint32_t* in, uint64_t* out, uint64_t n);
+// Same for u64 inputs (in-place allowed).
+uint64_t exclusive_scan_u64(qe_gpu_ctx* ctx, const uint64_t* in, uint64_t* out, uint64_t n);
+
+// ---- min/max of an integer-like column over valid rows ------------------------
+struct MinMax {
+    int64_t mn, mx;
+    uint64_t valid;  // number of valid rows
+};
+MinMax column_minmax(qe_gpu_ctx* ctx, const qe_dev_col* c, uint64_t n);
+
+// Stable LSD radix sort of (u64 key, u32 value) pairs on bits [begin_bit,
+// end_bit), in place (sort.hip).
+void radix_sort_pairs(qe_gpu_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, unsigned begin_bit,
+                      unsigned end_bit);
+
+// Column length helper with validation.
+uint64_t common_len(const qe_dev_col* const* cols, uint32_t n);
+
+// Result column construction helpers (host-side allocation of result tables).
+qe_dev_table* new_table(uint64_t rows);
+
+// Pack per-row valid bytes (1 = valid) into the column's u64 validity words and
+// apply the all_valid rule (core/column.h:42-54): a column with no NULL drops
+// its bitmap. The column must have been allocated nullable.
+void pack_validity(qe_gpu_ctx* ctx, qe_dev_col* col, const uint8_t* valid_bytes);
+
+}  // namespace qe_gpu
+
+// Entry-point wrapper: converts exceptions into statuses + ctx messages.
+#define QE_ENTRY(ctx, ...)                                             \
+    do {                                                               \
+        if (!(ctx)) return QE_GPU_INVALID_ARGUMENT;                    \
+        try {                                                          \
+            __VA_ARGS__;                                               \
+            return QE_GPU_OK;                                          \
+        } catch (const ::qe_gpu::Error& e_) {                          \
+            (ctx)->last_error = e_.msg;                                \
+            return e_.status;                                          \
+        } catch (const std::bad_alloc&) {                              \
+            (ctx)->last_error = "host allocation failed";              \
+            return QE_GPU_OUT_OF_MEMORY;                               \
+        } catch (const std::exception& e_) {                           \
+            (ctx)->last_error = e_.what();                             \
+            return QE_GPU_HIP_ERROR;                                   \
+        }                                                              \
+    } while (0)

@@ -1,0 +1,717 @@
+// Inner/left hash equi-join and the fused join -> group-by.
+//
+// qe_gpu_hash_join replaces qe::HashJoin (ops/join.h:79-81): the build side is
+// indexed once (build_side, ops/join.cpp:133-197) and every probe row expands
+// to one pair per matching build row (build_pairs_for_probe, :199-247); LEFT
+// keeps unmatched probe rows with a NULL build row. NULL keys never match
+// (NullPolicy::kNeverMatch); F64 keys compare canonically.
+//
+// Build index, chosen from the data at build time:
+//  * direct  : one integer-like key whose value range R is at most a few times
+//              the build row count -> an R-entry array indexed by key - min;
+//  * hash    : otherwise, the shared device hash table over canonical tuples.
+// Unique build keys (detected, not assumed) map a key straight to its build
+// row; duplicate keys use CSR row lists.
+//
+// qe_gpu_join_aggregate fuses Aggregate(HashJoin(...)) (plan/plan.cpp:581-590).
+// When every group key is a build column and every aggregate reads a probe
+// column (or is COUNT(*)), the build side is pre-grouped: each build row gets
+// the slot of its group key, the direct index stores that slot (u16 when the
+// groups fit), and the probe streams once, folding each matching row into
+// per-block LDS partials that a block-order merge combines. Other shapes run
+// join pairs -> gather -> aggregate.
+#include "agg_common.hpp"
+#include "hash_common.hpp"
+
+#include <algorithm>
+
+using namespace qe_gpu;
+
+namespace qe_gpu {
+
+bool is_int_like(int t);
+
+struct JoinProbeView {
+    int mode;    // 0 direct, 1 hash
+    int unique;  // 1: head[slot] is the build row
+    int nkeys;
+    ColView pkey[kMaxKeys];
+    int64_t kmin;
+    uint64_t range;
+    const uint32_t* head;  // unique: slot -> build row (kNoRow = none)
+    HashTableView t;
+    const uint64_t* offs;  // lists: slot -> [offs[slot], offs[slot+1])
+    const uint32_t* rows;
+
+    // Slot of probe row p's key, or kNoSlot (NULL key, out of range, absent).
+    __device__ uint64_t slot(uint64_t p) const {
+        if (mode == 0) {
+            if (!is_valid(pkey[0], p)) return kNoSlot;
+            const uint64_t off = static_cast<uint64_t>(int_value(pkey[0], p) - kmin);
+            return off < range ? off : kNoSlot;
+        }
+        HashKeys hk;
+        hk.nkeys = nkeys;
+        for (int j = 0; j < nkeys; ++j) hk.key[j] = pkey[j];
+        uint64_t w[kMaxKeys], nm, h;
+        normalize_row(hk, p, w, nm, h);
+        if (nm != 0) return kNoSlot;  // kNeverMatch: a NULL in any key never joins
+        const uint32_t s = ht_find(t, nkeys, w, nm, h);
+        return s == kNoSlot ? kNoSlot : s;
+    }
+    // Number of matching build rows; `begin` indexes row_at().
+    __device__ uint32_t probe(uint64_t p, uint64_t& begin) const {
+        const uint64_t s = slot(p);
+        if (s == kNoSlot) return 0;
+        if (unique) {
+            if (head[s] == kNoRow) return 0;
+            begin = s;
+            return 1;
+        }
+        begin = offs[s];
+        return static_cast<uint32_t>(offs[s + 1] - offs[s]);
+    }
+    __device__ uint32_t row_at(uint64_t i) const { return unique ? head[i] : rows[i]; }
+};
+
+struct JoinIndex {
+    int mode = 0;
+    bool unique = true;
+    bool empty = false;  // no matchable build row
+    int64_t kmin = 0;
+    uint64_t range = 0;
+    uint64_t nslots = 0;
+    DeviceHashTable ht;
+    Scratch head, offs, rows;
+    JoinProbeView view(const qe_dev_col* const* pkeys, int nkeys) const {
+        JoinProbeView v{};
+        v.mode = mode;
+        v.unique = unique ? 1 : 0;
+        v.nkeys = nkeys;
+        for (int j = 0; j < nkeys; ++j) v.pkey[j] = qe_gpu::view(pkeys[j]);
+        v.kmin = kmin;
+        v.range = empty ? 0 : range;
+        v.head = head.as<uint32_t>();
+        v.t = ht.v;
+        v.offs = offs.as<uint64_t>();
+        v.rows = rows.as<uint32_t>();
+        return v;
+    }
+};
+
+__global__ void direct_fill_kernel(ColView key, uint64_t n, int64_t kmin, uint32_t* head,
+                                   unsigned int* dup) {
+    for (uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < n;
+         r += uint64_t(gridDim.x) * blockDim.x) {
+        if (!is_valid(key, r)) continue;
+        const uint64_t off = static_cast<uint64_t>(int_value(key, r) - kmin);
+        const uint32_t old = atomicCAS(&head[off], kNoRow, static_cast<uint32_t>(r));
+        if (old != kNoRow) atomicOr(dup, 1u);
+    }
+}
+
+__global__ void hash_fill_kernel(HashKeys hk, uint64_t n, HashTableView t, uint32_t* head,
+                                 uint32_t* slot_of, unsigned int* dup) {
+    for (uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < n;
+         r += uint64_t(gridDim.x) * blockDim.x) {
+        uint64_t w[kMaxKeys], nm, h;
+        normalize_row(hk, r, w, nm, h);
+        if (nm != 0) {  // NULL key: never matches, not indexed
+            slot_of[r] = kNoRow;
+            continue;
+        }
+        const uint32_t s = ht_insert(t, hk.nkeys, w, nm, h);
+        slot_of[r] = s;
+        if (s == kNoSlot) continue;
+        const uint32_t old = atomicCAS(&head[s], kNoRow, static_cast<uint32_t>(r));
+        if (old != kNoRow) atomicOr(dup, 1u);
+    }
+}
+
+// Lists for duplicate keys: count per slot, scan, scatter rows.
+__global__ void list_count_kernel(const JoinProbeView v, const uint32_t* slot_of, ColView key0, uint64_t n,
+                                  uint32_t* counts) {
+    for (uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < n;
+         r += uint64_t(gridDim.x) * blockDim.x) {
+        uint64_t s;
+        if (v.mode == 0) {
+            if (!is_valid(key0, r)) continue;
+            s = static_cast<uint64_t>(int_value(key0, r) - v.kmin);
+        } else {
+            s = slot_of[r];
+            if (s == kNoRow) continue;
+        }
+        atomicAdd(&counts[s], 1u);
+    }
+}
+__global__ void list_fill_kernel(const JoinProbeView v, const uint32_t* slot_of, ColView key0, uint64_t n,
+                                 const uint64_t* offs, uint32_t* cursor, uint32_t* rows) {
+    for (uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < n;
+         r += uint64_t(gridDim.x) * blockDim.x) {
+        uint64_t s;
+        if (v.mode == 0) {
+            if (!is_valid(key0, r)) continue;
+            s = static_cast<uint64_t>(int_value(key0, r) - v.kmin);
+        } else {
+            s = slot_of[r];
+            if (s == kNoRow) continue;
+        }
+        const uint32_t k = atomicAdd(&cursor[s], 1u);
+        rows[offs[s] + k] = static_cast<uint32_t>(r);
+    }
+}
+
+// Within each list, order rows ascending (the reference's group_rows order);
+// lists are short, so one thread per slot with insertion sort.
+__global__ void list_sort_kernel(const uint64_t* offs, uint64_t nslots, uint32_t* rows) {
+    for (uint64_t s = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; s < nslots;
+         s += uint64_t(gridDim.x) * blockDim.x) {
+        const uint64_t b = offs[s], e = offs[s + 1];
+        if (e - b < 2 || e - b > 64) continue;
+        for (uint64_t i = b + 1; i < e; ++i) {
+            const uint32_t x = rows[i];
+            uint64_t j = i;
+            while (j > b && rows[j - 1] > x) {
+                rows[j] = rows[j - 1];
+                --j;
+            }
+            rows[j] = x;
+        }
+    }
+}
+
+void build_index(qe_gpu_ctx* ctx, JoinIndex& ix, const qe_dev_col* const* bkeys, uint32_t nkeys, uint64_t nb) {
+    Scratch dup(ctx, 8);
+    memset_async(ctx, dup.p, 0, 8);
+    Scratch slot_of;
+    const qe_dev_col* k0 = bkeys[0];
+    bool direct = false;
+    if (nkeys == 1 && is_int_like(k0->type)) {
+        const MinMax mm = column_minmax(ctx, k0, nb);
+        if (mm.valid == 0) {
+            ix.empty = true;
+            ix.mode = 0;
+            ix.range = 0;
+            return;
+        }
+        const uint64_t R = static_cast<uint64_t>(mm.mx) - static_cast<uint64_t>(mm.mn) + 1;
+        if (R <= std::max<uint64_t>(4 * nb, uint64_t(1) << 16) && R < (uint64_t(1) << 31)) {
+            direct = true;
+            ix.mode = 0;
+            ix.kmin = mm.mn;
+            ix.range = R;
+            ix.nslots = R;
+            ix.head = Scratch(ctx, R * 4);
+            memset_async(ctx, ix.head.p, 0xFF, R * 4);
+            direct_fill_kernel<<<grid_for(nb, 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(
+                view(k0), nb, ix.kmin, ix.head.as<uint32_t>(), dup.as<unsigned int>());
+            QE_HIP(hipGetLastError());
+        }
+    }
+    if (!direct) {
+        check(nkeys <= kMaxKeys, QE_GPU_UNSUPPORTED, "too many join keys for the device path");
+        ix.mode = 1;
+        ix.ht.create(ctx, static_cast<int>(nkeys), nb);
+        ix.nslots = ix.ht.v.cap;
+        ix.range = ix.ht.v.cap;
+        ix.head = Scratch(ctx, ix.nslots * 4);
+        memset_async(ctx, ix.head.p, 0xFF, ix.nslots * 4);
+        slot_of = Scratch(ctx, std::max<uint64_t>(nb, 1) * 4);
+        HashKeys hk{};
+        hk.nkeys = static_cast<int>(nkeys);
+        for (uint32_t j = 0; j < nkeys; ++j) hk.key[j] = view(bkeys[j]);
+        hash_fill_kernel<<<grid_for(nb, 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(
+            hk, nb, ix.ht.v, ix.head.as<uint32_t>(), slot_of.as<uint32_t>(), dup.as<unsigned int>());
+        QE_HIP(hipGetLastError());
+        ix.ht.check_overflow(ctx);
+    }
+    ix.unique = read_scalar(ctx, dup.as<unsigned int>()) == 0;
+    if (ix.unique) return;
+    // Duplicate keys: CSR lists per slot.
+    const uint64_t S = ix.nslots;
+    Scratch counts(ctx, S * 4), cursor(ctx, S * 4);
+    memset_async(ctx, counts.p, 0, S * 4);
+    memset_async(ctx, cursor.p, 0, S * 4);
+    ix.offs = Scratch(ctx, (S + 1) * 8);
+    JoinProbeView v = ix.view(bkeys, static_cast<int>(nkeys));
+    list_count_kernel<<<grid_for(nb, 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(
+        v, slot_of.as<uint32_t>(), view(k0), nb, counts.as<uint32_t>());
+    QE_HIP(hipGetLastError());
+    const uint64_t total = exclusive_scan_u32(ctx, counts.as<uint32_t>(), ix.offs.as<uint64_t>(), S);
+    QE_HIP(hipMemcpyAsync(ix.offs.as<uint64_t>() + S, &total, 8, hipMemcpyHostToDevice, ctx->stream));
+    ix.rows = Scratch(ctx, std::max<uint64_t>(total, 1) * 4);
+    list_fill_kernel<<<grid_for(nb, 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(
+        v, slot_of.as<uint32_t>(), view(k0), nb, ix.offs.as<uint64_t>(), cursor.as<uint32_t>(),
+        ix.rows.as<uint32_t>());
+    list_sort_kernel<<<grid_for(S, 256, 8192), 256, 0, ctx->stream>>>(ix.offs.as<uint64_t>(), S,
+                                                                       ix.rows.as<uint32_t>());
+    QE_HIP(hipGetLastError());
+    QE_HIP(hipStreamSynchronize(ctx->stream));  // `total` staging lives on the host stack
+}
+
+void check_key_types(const qe_dev_col* const* pk, const qe_dev_col* const* bk, uint32_t nkeys) {
+    for (uint32_t j = 0; j < nkeys; ++j) {
+        invalid(pk[j] != nullptr && bk[j] != nullptr, "null key column");
+        // ops/join.cpp:118-122 (a real check in every build).
+        invalid(pk[j]->type == bk[j]->type, "HashJoin: join key column types must match positionally");
+    }
+}
+
+// ---- pairs -------------------------------------------------------------------------
+__global__ void pair_count_kernel(JoinProbeView v, uint64_t n, int left, uint32_t* counts) {
+    for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < n;
+         p += uint64_t(gridDim.x) * blockDim.x) {
+        uint64_t b;
+        const uint32_t c = v.probe(p, b);
+        counts[p] = (c == 0 && left) ? 1u : c;
+    }
+}
+__global__ void pair_write_kernel(JoinProbeView v, uint64_t n, int left, const uint64_t* offs,
+                                  uint32_t* pp, uint32_t* bp) {
+    for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < n;
+         p += uint64_t(gridDim.x) * blockDim.x) {
+        uint64_t b = 0;
+        const uint32_t c = v.probe(p, b);
+        uint64_t o = offs[p];
+        if (c == 0) {
+            if (left) {
+                pp[o] = static_cast<uint32_t>(p);
+                bp[o] = kNoRow;
+            }
+            continue;
+        }
+        for (uint32_t i = 0; i < c; ++i, ++o) {
+            pp[o] = static_cast<uint32_t>(p);
+            bp[o] = v.row_at(b + i);
+        }
+    }
+}
+
+qe_dev_pairs* join_pairs(qe_gpu_ctx* ctx, const qe_dev_col* const* pk, const qe_dev_col* const* bk,
+                         uint32_t nkeys, int join_type) {
+    invalid(nkeys >= 1, "equi-join needs >=1 key");
+    invalid(join_type == QE_JOIN_INNER || join_type == QE_JOIN_LEFT, "unknown join type");
+    check_key_types(pk, bk, nkeys);
+    const uint64_t np = common_len(pk, nkeys), nb = common_len(bk, nkeys);
+    check(np < kNoRow && nb < kNoRow, QE_GPU_UNSUPPORTED, "row indices are 32-bit");
+    JoinIndex ix;
+    if (nb > 0) build_index(ctx, ix, bk, nkeys, nb);
+    else ix.empty = true;
+    JoinProbeView v = ix.view(pk, static_cast<int>(nkeys));
+    if (ix.empty) {
+        v.mode = 0;
+        v.range = 0;
+        v.unique = 1;
+    }
+    const int left = join_type == QE_JOIN_LEFT;
+    auto* out = new qe_dev_pairs();
+    if (np == 0) return out;
+    Scratch counts(ctx, np * 4), offs(ctx, np * 8);
+    {
+        TimedRegion tr(ctx, "join_count");
+        pair_count_kernel<<<grid_for(np, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(v, np, left,
+                                                                                          counts.as<uint32_t>());
+        QE_HIP(hipGetLastError());
+    }
+    const uint64_t total = exclusive_scan_u32(ctx, counts.as<uint32_t>(), offs.as<uint64_t>(), np);
+    out->count = total;
+    try {
+        out->probe = static_cast<uint32_t*>(dev_alloc(ctx, std::max<uint64_t>(total, 1) * 4));
+        out->build = static_cast<uint32_t*>(dev_alloc(ctx, std::max<uint64_t>(total, 1) * 4));
+        TimedRegion tr(ctx, "join_write");
+        pair_write_kernel<<<grid_for(np, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(
+            v, np, left, offs.as<uint64_t>(), out->probe, out->build);
+        QE_HIP(hipGetLastError());
+        QE_HIP(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+        qe_gpu_pairs_free(ctx, out);
+        throw;
+    }
+    return out;
+}
+
+// ---- gather (emit) --------------------------------------------------------------------
+template <typename T>
+__global__ void gather_kernel(const T* src, const uint64_t* src_valid, const uint32_t* idx, uint64_t m,
+                              T* dst, uint8_t* vb) {
+    for (uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < m;
+         r += uint64_t(gridDim.x) * blockDim.x) {
+        const uint32_t s = idx[r];
+        const bool ok = s != kNoRow && (src_valid == nullptr || ((src_valid[s >> 6] >> (s & 63)) & 1ull));
+        dst[r] = ok ? src[s] : T(0);
+        vb[r] = ok ? 1 : 0;
+    }
+}
+
+qe_dev_col* gather_col(qe_gpu_ctx* ctx, const qe_dev_col* src, const uint32_t* idx, uint64_t m) {
+    qe_dev_col* out = new_col(ctx, src->type, m, true);
+    Scratch vb(ctx, std::max<uint64_t>(m, 1));
+    const uint64_t* sv = src->all_valid ? nullptr : src->validity;
+    const unsigned g = grid_for(m, 256, ctx->num_cus * 16);
+    if (m) {
+        switch (type_width(src->type)) {
+            case 1:
+                gather_kernel<uint8_t><<<g, 256, 0, ctx->stream>>>(static_cast<const uint8_t*>(src->data), sv, idx, m,
+                                                                   static_cast<uint8_t*>(out->data), vb.as<uint8_t>());
+                break;
+            case 4:
+                gather_kernel<uint32_t><<<g, 256, 0, ctx->stream>>>(static_cast<const uint32_t*>(src->data), sv, idx,
+                                                                    m, static_cast<uint32_t*>(out->data),
+                                                                    vb.as<uint8_t>());
+                break;
+            default:
+                gather_kernel<uint64_t><<<g, 256, 0, ctx->stream>>>(static_cast<const uint64_t*>(src->data), sv, idx,
+                                                                    m, static_cast<uint64_t*>(out->data),
+                                                                    vb.as<uint8_t>());
+        }
+        QE_HIP(hipGetLastError());
+    }
+    pack_validity(ctx, out, vb.as<uint8_t>());
+    return out;
+}
+
+// ---- fused join -> group-by -----------------------------------------------------------
+struct ProbeAggLists {
+    JoinProbeView v;
+    const uint32_t* gslot;  // build row -> group slot
+};
+
+template <typename KeyT, typename MapT>
+__global__ void __launch_bounds__(1024) join_agg_direct_lds(const KeyT* __restrict__ keys,
+                                                            const uint64_t* key_valid, uint64_t n,
+                                                            const MapT* __restrict__ gmap, int64_t kmin,
+                                                            uint64_t range, uint32_t S, AggProg p, uint64_t* part) {
+    extern __shared__ __align__(16) char smem[];
+    lds_init(smem, S, p);
+    __syncthreads();
+    constexpr MapT kEmpty = static_cast<MapT>(~MapT(0));
+    uint64_t vrow[kMaxAggs];
+    const uint64_t nq = (n + 3) / 4;
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    for (uint64_t q = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; q < nq; q += stride) {
+        const uint64_t r0 = q * 4;
+        KeyT k[4];
+        if (r0 + 4 <= n) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) k[j] = __builtin_nontemporal_load(keys + r0 + j);
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) k[j] = (r0 + j < n) ? keys[r0 + j] : KeyT(0);
+        }
+        MapT g[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint64_t off = static_cast<uint64_t>(static_cast<int64_t>(k[j]) - kmin);
+            bool ok = (r0 + j < n) && off < range;
+            if (key_valid) ok = ok && ((key_valid[(r0 + j) >> 6] >> ((r0 + j) & 63)) & 1ull);
+            g[j] = ok ? gmap[off] : kEmpty;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (g[j] == kEmpty) continue;
+            for (int a = 0; a < p.n; ++a) vrow[a] = r0 + j;
+            lds_update(smem, S, p, g[j], vrow);
+        }
+    }
+    __syncthreads();
+    lds_flush(smem, S, p, part);
+}
+
+template <typename KeyT, typename MapT>
+__global__ void join_agg_direct_global(const KeyT* __restrict__ keys, const uint64_t* key_valid, uint64_t n,
+                                       const MapT* __restrict__ gmap, int64_t kmin, uint64_t range, uint64_t S,
+                                       AggProg p, uint64_t* st) {
+    constexpr MapT kEmpty = static_cast<MapT>(~MapT(0));
+    uint64_t vrow[kMaxAggs];
+    for (uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < n;
+         r += uint64_t(gridDim.x) * blockDim.x) {
+        if (key_valid && !((key_valid[r >> 6] >> (r & 63)) & 1ull)) continue;
+        const uint64_t off = static_cast<uint64_t>(static_cast<int64_t>(keys[r]) - kmin);
+        if (off >= range) continue;
+        const MapT g = gmap[off];
+        if (g == kEmpty) continue;
+        for (int a = 0; a < p.n; ++a) vrow[a] = r;
+        global_update(st, S, p, g, vrow);
+    }
+}
+
+__global__ void __launch_bounds__(1024) join_agg_lists_lds(ProbeAggLists L, uint64_t n, uint32_t S, AggProg p,
+                                                           uint64_t* part) {
+    extern __shared__ __align__(16) char smem[];
+    lds_init(smem, S, p);
+    __syncthreads();
+    uint64_t vrow[kMaxAggs];
+    for (uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < n;
+         r += uint64_t(gridDim.x) * blockDim.x) {
+        uint64_t b = 0;
+        const uint32_t c = L.v.probe(r, b);
+        for (int a = 0; a < p.n; ++a) vrow[a] = r;
+        for (uint32_t i = 0; i < c; ++i) lds_update(smem, S, p, L.gslot[L.v.row_at(b + i)], vrow);
+    }
+    __syncthreads();
+    lds_flush(smem, S, p, part);
+}
+
+__global__ void join_agg_lists_global(ProbeAggLists L, uint64_t n, uint64_t S, AggProg p, uint64_t* st) {
+    uint64_t vrow[kMaxAggs];
+    for (uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < n;
+         r += uint64_t(gridDim.x) * blockDim.x) {
+        uint64_t b = 0;
+        const uint32_t c = L.v.probe(r, b);
+        for (int a = 0; a < p.n; ++a) vrow[a] = r;
+        for (uint32_t i = 0; i < c; ++i) global_update(st, S, p, L.gslot[L.v.row_at(b + i)], vrow);
+    }
+}
+
+// Group slot of each build row over its group-key columns.
+__global__ void gslot_direct_kernel(ColView g, uint64_t n, int64_t gmin, uint32_t null_slot, uint32_t* gslot) {
+    for (uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < n;
+         r += uint64_t(gridDim.x) * blockDim.x)
+        gslot[r] = is_valid(g, r) ? static_cast<uint32_t>(static_cast<uint64_t>(int_value(g, r) - gmin)) : null_slot;
+}
+__global__ void gslot_hash_kernel(HashKeys hk, uint64_t n, HashTableView t, uint32_t* gslot) {
+    for (uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < n;
+         r += uint64_t(gridDim.x) * blockDim.x) {
+        uint64_t w[kMaxKeys], nm, h;
+        normalize_row(hk, r, w, nm, h);  // kEqual: NULL tuples group together
+        gslot[r] = ht_insert(t, hk.nkeys, w, nm, h);
+    }
+}
+template <typename MapT>
+__global__ void gmap_fill_kernel(ColView key, uint64_t n, int64_t kmin, const uint32_t* gslot, MapT* gmap) {
+    for (uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < n;
+         r += uint64_t(gridDim.x) * blockDim.x) {
+        if (!is_valid(key, r)) continue;
+        gmap[static_cast<uint64_t>(int_value(key, r) - kmin)] = static_cast<MapT>(gslot[r]);
+    }
+}
+
+}  // namespace qe_gpu
+
+extern "C" {
+
+qe_gpu_status qe_gpu_hash_join(qe_gpu_ctx* ctx, const qe_dev_col* const* probe_keys,
+                               const qe_dev_col* const* build_keys, uint32_t nkeys, int32_t join_type,
+                               qe_dev_pairs** out) {
+    QE_ENTRY(ctx, {
+        invalid(out != nullptr && probe_keys != nullptr && build_keys != nullptr, "null argument");
+        *out = join_pairs(ctx, probe_keys, build_keys, nkeys, join_type);
+    });
+}
+
+qe_gpu_status qe_gpu_pairs_gather(qe_gpu_ctx* ctx, const qe_dev_pairs* p, uint64_t offset, uint64_t count,
+                                  const qe_dev_col* const* probe_cols, uint32_t nprobe,
+                                  const qe_dev_col* const* build_cols, uint32_t nbuild, qe_dev_table** out) {
+    QE_ENTRY(ctx, {
+        invalid(p != nullptr && out != nullptr, "null argument");
+        invalid(offset + count <= p->count, "pair range out of bounds");
+        common_len(probe_cols, nprobe);
+        common_len(build_cols, nbuild);
+        qe_dev_table* t = new_table(count);
+        try {
+            for (uint32_t c = 0; c < nprobe; ++c)
+                t->cols.push_back(gather_col(ctx, probe_cols[c], p->probe + offset, count));
+            for (uint32_t c = 0; c < nbuild; ++c)
+                t->cols.push_back(gather_col(ctx, build_cols[c], p->build + offset, count));
+        } catch (...) {
+            qe_gpu_table_free(ctx, t);
+            throw;
+        }
+        *out = t;
+    });
+}
+
+qe_gpu_status qe_gpu_join_aggregate(qe_gpu_ctx* ctx, const qe_dev_col* const* probe_cols, uint32_t nprobe,
+                                    const qe_dev_col* const* build_cols, uint32_t nbuild,
+                                    const uint32_t* probe_keys, const uint32_t* build_keys, uint32_t nkeys,
+                                    int32_t join_type, const uint32_t* group_keys, uint32_t ngk,
+                                    const qe_agg_spec* aggs, uint32_t naggs, qe_dev_table** out) {
+    QE_ENTRY(ctx, {
+        invalid(out != nullptr, "null out");
+        invalid(nprobe >= 1 && nbuild >= 1, "join inputs need >=1 column each");
+        invalid(nkeys >= 1, "equi-join needs >=1 key");
+        invalid(join_type == QE_JOIN_INNER || join_type == QE_JOIN_LEFT, "unknown join type");
+        check(nkeys <= kMaxKeys && ngk <= kMaxKeys, QE_GPU_UNSUPPORTED, "too many keys for the device path");
+        const uint64_t np = common_len(probe_cols, nprobe), nb = common_len(build_cols, nbuild);
+        check(np < kNoRow && nb < kNoRow, QE_GPU_UNSUPPORTED, "row indices are 32-bit");
+        const uint32_t ncols = nprobe + nbuild;
+        std::vector<const qe_dev_col*> pk(nkeys), bk(nkeys);
+        for (uint32_t j = 0; j < nkeys; ++j) {
+            invalid(probe_keys[j] < nprobe && build_keys[j] < nbuild, "join key column out of range");
+            pk[j] = probe_cols[probe_keys[j]];
+            bk[j] = build_cols[build_keys[j]];
+        }
+        check_key_types(pk.data(), bk.data(), nkeys);
+        auto col_at = [&](uint32_t c) { return c < nprobe ? probe_cols[c] : build_cols[c - nprobe]; };
+        for (uint32_t k = 0; k < ngk; ++k) invalid(group_keys[k] < ncols, "group key column out of range");
+        bool fast = join_type == QE_JOIN_INNER && ngk >= 1;
+        for (uint32_t k = 0; k < ngk; ++k) fast = fast && group_keys[k] >= nprobe;
+        ColView ins[kMaxAggs];
+        int types[kMaxAggs];
+        for (uint32_t a = 0; a < naggs && a < uint32_t(kMaxAggs); ++a) {
+            if (aggs[a].func == QE_AGG_COUNT_STAR) {
+                ins[a] = ColView{nullptr, nullptr, QE_I64, 0};
+                types[a] = QE_I64;
+                continue;
+            }
+            invalid(aggs[a].input_col < ncols, "aggregate input column out of range");
+            fast = fast && aggs[a].input_col < nprobe;
+            ins[a] = view(col_at(aggs[a].input_col));
+            types[a] = col_at(aggs[a].input_col)->type;
+        }
+        const AggProg p = make_prog(aggs, naggs, ins, types);
+
+        if (!fast) {
+            // General composition: pairs -> gather referenced columns -> aggregate.
+            qe_dev_pairs* pr = join_pairs(ctx, pk.data(), bk.data(), nkeys, join_type);
+            std::vector<qe_dev_col*> gathered(ncols, nullptr);
+            std::vector<qe_dev_col*> owned;
+            qe_dev_table* res = nullptr;
+            try {
+                std::vector<char> need(ncols, 0);
+                for (uint32_t k = 0; k < ngk; ++k) need[group_keys[k]] = 1;
+                for (uint32_t a = 0; a < naggs; ++a)
+                    if (aggs[a].func != QE_AGG_COUNT_STAR) need[aggs[a].input_col] = 1;
+                if (ngk == 0 && naggs > 0) need[0] = 1;  // keep the row count
+                for (uint32_t c = 0; c < ncols; ++c) {
+                    if (!need[c]) continue;
+                    gathered[c] = gather_col(ctx, col_at(c), c < nprobe ? pr->probe : pr->build, pr->count);
+                    owned.push_back(gathered[c]);
+                }
+                // Unreferenced positions get a referenced column of the right
+                // length so indices stay those of the join output.
+                qe_dev_col* any = owned.empty() ? nullptr : owned[0];
+                for (uint32_t c = 0; c < ncols; ++c)
+                    if (!gathered[c]) gathered[c] = any;
+                const qe_gpu_status st = qe_gpu_aggregate(ctx, gathered.data(), ncols, group_keys, ngk, aggs, naggs, &res);
+                if (st != QE_GPU_OK) throw Error{st, ctx->last_error};
+            } catch (...) {
+                for (auto* c : owned) free_col(ctx, c);
+                qe_gpu_pairs_free(ctx, pr);
+                throw;
+            }
+            for (auto* c : owned) free_col(ctx, c);
+            qe_gpu_pairs_free(ctx, pr);
+            *out = res;
+            return QE_GPU_OK;
+        }
+
+        // ---- fast path: pre-group the build side ---------------------------------
+        KeyEmit ke{};
+        ke.nkeys = static_cast<int>(ngk);
+        for (uint32_t k = 0; k < ngk; ++k) ke.types[k] = col_at(group_keys[k])->type;
+        Scratch gslot(ctx, std::max<uint64_t>(nb, 1) * 4);
+        DeviceHashTable gtab;
+        uint64_t S = 0;
+        const qe_dev_col* g0 = col_at(group_keys[0]);
+        bool gdirect = false;
+        if (ngk == 1 && is_int_like(g0->type) && nb > 0) {
+            const MinMax mm = column_minmax(ctx, g0, nb);
+            const uint64_t R = mm.valid ? static_cast<uint64_t>(mm.mx) - static_cast<uint64_t>(mm.mn) + 1 : 0;
+            if (R <= std::max<uint64_t>(2 * nb, 1024) && R < (uint64_t(1) << 30)) {
+                gdirect = true;
+                S = R + 1;
+                ke.mode = 0;
+                ke.kmin = mm.valid ? mm.mn : 0;
+                ke.null_slot = S - 1;
+                gslot_direct_kernel<<<grid_for(nb, 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(
+                    view(g0), nb, ke.kmin, static_cast<uint32_t>(S - 1), gslot.as<uint32_t>());
+                QE_HIP(hipGetLastError());
+            }
+        }
+        if (!gdirect) {
+            HashKeys hk{};
+            hk.nkeys = static_cast<int>(ngk);
+            for (uint32_t k = 0; k < ngk; ++k) hk.key[k] = view(col_at(group_keys[k]));
+            gtab.create(ctx, static_cast<int>(ngk), std::max<uint64_t>(nb, 1));
+            S = gtab.v.cap;
+            if (nb)
+                gslot_hash_kernel<<<grid_for(nb, 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(hk, nb, gtab.v,
+                                                                                                 gslot.as<uint32_t>());
+            QE_HIP(hipGetLastError());
+            gtab.check_overflow(ctx);
+            ke.mode = 1;
+            ke.words = gtab.v.words;
+            ke.nullmask = gtab.v.nullmask;
+        }
+
+        JoinIndex ix;
+        if (nb > 0) build_index(ctx, ix, bk.data(), nkeys, nb);
+        else ix.empty = true;
+
+        Scratch st(ctx, state_words(S, p) * 8);
+        const size_t lb = lds_bytes(static_cast<uint32_t>(std::min<uint64_t>(S, UINT32_MAX)), p);
+        const bool use_lds = S < (uint64_t(1) << 20) && lb <= kLdsBudget;
+        const unsigned lds_blocks = static_cast<unsigned>(
+            std::max<uint64_t>(1, std::min<uint64_t>(ctx->num_cus, (np + 1023) / 1024)));
+
+        if (ix.empty || np == 0) {
+            init_state(ctx, st.as<uint64_t>(), S, p);
+        } else if (ix.mode == 0 && ix.unique && (pk[0]->type == QE_I64 || pk[0]->type == QE_TS ||
+                                                 pk[0]->type == QE_I32)) {
+            // Direct unique index: the map stores each key's group slot.
+            const bool small = S < 0xFFFF;
+            const size_t mb = ix.range * (small ? 2 : 4);
+            Scratch gmap(ctx, mb);
+            memset_async(ctx, gmap.p, 0xFF, mb);
+            if (small)
+                gmap_fill_kernel<uint16_t><<<grid_for(nb, 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(
+                    view(bk[0]), nb, ix.kmin, gslot.as<uint32_t>(), gmap.as<uint16_t>());
+            else
+                gmap_fill_kernel<uint32_t><<<grid_for(nb, 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(
+                    view(bk[0]), nb, ix.kmin, gslot.as<uint32_t>(), gmap.as<uint32_t>());
+            QE_HIP(hipGetLastError());
+            const uint64_t* kv = pk[0]->all_valid ? nullptr : pk[0]->validity;
+            auto launch = [&](auto key_tag, auto map_tag) {
+                using KeyT = decltype(key_tag);
+                using MapT = decltype(map_tag);
+                const KeyT* keys = static_cast<const KeyT*>(pk[0]->data);
+                const MapT* gm = static_cast<const MapT*>(gmap.p);
+                if (use_lds) {
+                    Scratch part(ctx, size_t(lds_blocks) * part_words(static_cast<uint32_t>(S), p) * 8);
+                    {
+                        TimedRegion tr(ctx, "join_agg_probe");
+                        join_agg_direct_lds<KeyT, MapT><<<lds_blocks, 1024, lb, ctx->stream>>>(
+                            keys, kv, np, gm, ix.kmin, ix.range, static_cast<uint32_t>(S), p, part.as<uint64_t>());
+                        QE_HIP(hipGetLastError());
+                    }
+                    merge_partials(ctx, part.as<uint64_t>(), lds_blocks, st.as<uint64_t>(), S, p);
+                } else {
+                    init_state(ctx, st.as<uint64_t>(), S, p);
+                    TimedRegion tr(ctx, "join_agg_probe");
+                    join_agg_direct_global<KeyT, MapT><<<grid_for(np, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(
+                        keys, kv, np, gm, ix.kmin, ix.range, S, p, st.as<uint64_t>());
+                    QE_HIP(hipGetLastError());
+                }
+            };
+            if (pk[0]->type == QE_I32) {
+                if (small) launch(int32_t(0), uint16_t(0));
+                else launch(int32_t(0), uint32_t(0));
+            } else {
+                if (small) launch(int64_t(0), uint16_t(0));
+                else launch(int64_t(0), uint32_t(0));
+            }
+        } else {
+            ProbeAggLists L{ix.view(pk.data(), static_cast<int>(nkeys)), gslot.as<uint32_t>()};
+            if (use_lds) {
+                Scratch part(ctx, size_t(lds_blocks) * part_words(static_cast<uint32_t>(S), p) * 8);
+                {
+                    TimedRegion tr(ctx, "join_agg_probe");
+                    join_agg_lists_lds<<<lds_blocks, 1024, lb, ctx->stream>>>(L, np, static_cast<uint32_t>(S), p,
+                                                                              part.as<uint64_t>());
+                    QE_HIP(hipGetLastError());
+                }
+                merge_partials(ctx, part.as<uint64_t>(), lds_blocks, st.as<uint64_t>(), S, p);
+            } else {
+                init_state(ctx, st.as<uint64_t>(), S, p);
+                TimedRegion tr(ctx, "join_agg_probe");
+                join_agg_lists_global<<<grid_for(np, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(
+                    L, np, S, p, st.as<uint64_t>());
+                QE_HIP(hipGetLastError());
+            }
+        }
+        *out = emit_groups(ctx, st.as<uint64_t>(), S, p, ke, false);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,168 @@
+// Stable LSD radix sort of (u64 key, u32 value) pairs — the device Sort that
+// orders the as-of build side by (partition, timestamp), replacing the
+// reference's byte-radix path (ops/sort.cpp:125-188) for the key shapes the
+// as-of join needs (tsx/asof.cpp:163-166).
+//
+// One pass per 8-bit digit over bits [begin_bit, end_bit):
+//   1. tile histograms (tile = 256 threads x 16 keys) -> hist[digit][tile];
+//   2. exclusive scan of hist in digit-major order -> global offsets;
+//   3. scatter: each tile ranks its keys stably within their digit (wave64
+//      ballot "match" per 64 keys, waves folded in order through LDS), stages
+//      the tile in LDS in digit order, then writes each digit run contiguously
+//      (coalesced) to its global offset.
+// Stability across tiles comes from the digit-major scan; within a tile from
+// the (iteration, wave, lane) position order used everywhere below.
+#include "internal.hpp"
+
+#include <algorithm>
+
+using namespace qe_gpu;
+
+namespace qe_gpu {
+
+constexpr unsigned kSortBlock = 256;
+constexpr unsigned kSortWaves = kSortBlock / kWave;
+constexpr unsigned kSortItems = 16;
+constexpr unsigned kSortTile = kSortBlock * kSortItems;  // 4096 keys
+constexpr unsigned kRadix = 256;
+
+// Position p of the tile is item i = p / 256, wave w = (p % 256) / 64, lane.
+__device__ inline uint64_t tile_pos(unsigned i) { return uint64_t(i) * kSortBlock + threadIdx.x; }
+
+__global__ void __launch_bounds__(kSortBlock) radix_hist_kernel(const uint64_t* keys, uint64_t n, unsigned shift,
+                                                                uint32_t* hist, uint64_t ntiles) {
+    __shared__ uint32_t h[kRadix];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t base = uint64_t(blockIdx.x) * kSortTile;
+    for (unsigned i = 0; i < kSortItems; ++i) {
+        const uint64_t p = base + tile_pos(i);
+        if (p < n) atomicAdd(&h[(keys[p] >> shift) & 0xFF], 1u);
+    }
+    __syncthreads();
+    hist[uint64_t(threadIdx.x) * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(kSortBlock) radix_scatter_kernel(const uint64_t* keys, const uint32_t* vals,
+                                                                   uint64_t n, unsigned shift,
+                                                                   const uint64_t* offs, uint64_t ntiles,
+                                                                   uint64_t* keys_out, uint32_t* vals_out) {
+    __shared__ uint32_t wcnt[kSortWaves][kRadix];  // per-wave digit counts of one item row
+    __shared__ uint32_t run[kRadix];               // running digit counts within the tile
+    __shared__ uint32_t tile_start[kRadix];        // digit start within the tile (staging order)
+    __shared__ uint64_t skeys[kSortTile];
+    __shared__ uint32_t svals[kSortTile];
+    __shared__ uint16_t srank[kSortTile];          // rank of each key within its digit in the tile
+
+    const uint64_t base = uint64_t(blockIdx.x) * kSortTile;
+    const unsigned lane = lane_id();
+    const unsigned w = threadIdx.x / kWave;
+    run[threadIdx.x] = 0;
+    uint64_t k[kSortItems];
+    uint32_t v[kSortItems];
+    uint16_t rank[kSortItems];
+    for (unsigned i = 0; i < kSortItems; ++i) {
+        const uint64_t p = base + tile_pos(i);
+        const bool live = p < n;
+        k[i] = live ? keys[p] : ~0ull;
+        v[i] = live ? vals[p] : 0u;
+    }
+    __syncthreads();
+    for (unsigned i = 0; i < kSortItems; ++i) {
+        const uint64_t p = base + tile_pos(i);
+        const bool live = p < n;
+        const unsigned d = static_cast<unsigned>((k[i] >> shift) & 0xFF);
+        // Lanes of this wave holding the same digit (dead lanes match nobody).
+        uint64_t peers = __ballot(live);
+        for (unsigned b = 0; b < 8; ++b) {
+            const uint64_t m = __ballot(live && ((d >> b) & 1u));
+            peers &= ((d >> b) & 1u) ? m : ~m;
+        }
+        const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        const unsigned in_wave = __popcll(peers & lt);
+        for (unsigned j = lane; j < kRadix; j += kWave) wcnt[w][j] = 0;
+        __syncthreads();
+        if (live && in_wave == 0) wcnt[w][d] = __popcll(peers);
+        __syncthreads();
+        unsigned before = 0;
+        if (live) {
+            before = run[d];
+            for (unsigned ww = 0; ww < w; ++ww) before += wcnt[ww][d];
+        }
+        rank[i] = static_cast<uint16_t>(before + in_wave);
+        __syncthreads();
+        {  // advance the running counts by this item row's totals
+            const unsigned dd = threadIdx.x;
+            unsigned tot = 0;
+            for (unsigned ww = 0; ww < kSortWaves; ++ww) tot += wcnt[ww][dd];
+            run[dd] += tot;
+        }
+        __syncthreads();
+    }
+    // run[] now holds the tile's digit counts: exclusive scan -> staging starts.
+    {
+        const unsigned d = threadIdx.x;
+        const unsigned c = run[d];
+        unsigned incl = c;
+        for (int off = 1; off < kWave; off <<= 1) {
+            const unsigned t = __shfl_up(incl, off);
+            if (lane >= off) incl += t;
+        }
+        __shared__ unsigned wsum[kSortWaves];
+        if (lane == kWave - 1) wsum[w] = incl;
+        __syncthreads();
+        unsigned wb = 0;
+        for (unsigned ww = 0; ww < w; ++ww) wb += wsum[ww];
+        tile_start[d] = wb + incl - c;
+    }
+    __syncthreads();
+    for (unsigned i = 0; i < kSortItems; ++i) {
+        const uint64_t p = base + tile_pos(i);
+        if (p >= n) continue;
+        const unsigned d = static_cast<unsigned>((k[i] >> shift) & 0xFF);
+        const unsigned s = tile_start[d] + rank[i];
+        skeys[s] = k[i];
+        svals[s] = v[i];
+        srank[s] = rank[i];
+    }
+    __syncthreads();
+    const uint64_t live_n = (n - base < kSortTile) ? (n - base) : kSortTile;
+    for (unsigned s = threadIdx.x; s < live_n; s += kSortBlock) {
+        const uint64_t key = skeys[s];
+        const unsigned d = static_cast<unsigned>((key >> shift) & 0xFF);
+        const uint64_t dst = offs[uint64_t(d) * ntiles + blockIdx.x] + srank[s];
+        keys_out[dst] = key;
+        vals_out[dst] = svals[s];
+    }
+}
+
+// Sort pairs by bits [begin_bit, end_bit) of the keys. Uses (and may return)
+// the alternate buffers; the sorted result is written back to keys/vals.
+void radix_sort_pairs(qe_gpu_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t n, unsigned begin_bit,
+                      unsigned end_bit) {
+    if (n <= 1 || end_bit <= begin_bit) return;
+    const uint64_t ntiles = (n + kSortTile - 1) / kSortTile;
+    Scratch k2(ctx, n * 8), v2(ctx, n * 4), hist(ctx, ntiles * kRadix * 4), offs(ctx, ntiles * kRadix * 8);
+    uint64_t* kin = keys;
+    uint32_t* vin = vals;
+    uint64_t* kout = k2.as<uint64_t>();
+    uint32_t* vout = v2.as<uint32_t>();
+    unsigned passes = 0;
+    for (unsigned shift = begin_bit; shift < end_bit; shift += 8, ++passes) {
+        radix_hist_kernel<<<ntiles, kSortBlock, 0, ctx->stream>>>(kin, n, shift, hist.as<uint32_t>(), ntiles);
+        QE_HIP(hipGetLastError());
+        exclusive_scan_u32(ctx, hist.as<uint32_t>(), offs.as<uint64_t>(), ntiles * kRadix);
+        radix_scatter_kernel<<<ntiles, kSortBlock, 0, ctx->stream>>>(kin, vin, n, shift, offs.as<uint64_t>(), ntiles,
+                                                                     kout, vout);
+        QE_HIP(hipGetLastError());
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+    }
+    if (kin != keys) {
+        QE_HIP(hipMemcpyAsync(keys, kin, n * 8, hipMemcpyDeviceToDevice, ctx->stream));
+        QE_HIP(hipMemcpyAsync(vals, vin, n * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    QE_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+}  // namespace qe_gpu

@@ -1,0 +1,201 @@
+"""GPU hash join and fused join -> group-by against the CPU oracle.
+
+Literal edge shapes follow tests/join_differential_test.cpp:103-296 (one key
+with a NULL, composite (i32, f64), empty build / probe, all-match, 3000-row
+fan-out past the 2048-row output batch, NULL keys on both sides); pair sets
+are bit-exact; the fused aggregate is bit-exact for keys and COUNTs and within
+1e-12 relative for F64 SUM.
+"""
+import numpy as np
+import pytest
+
+import _oracle as O
+from _compare import assert_same, table_rows
+
+pytestmark = pytest.mark.gpu
+I32, I64, F64 = O.I32, O.I64, O.F64
+CS, CNT, SUM, MIN, MAX, AVG = O.COUNT_STAR, O.COUNT, O.SUM, O.MIN, O.MAX, O.AVG
+INNER, LEFT = 0, 1
+
+
+def col(vals, t, nulls=()):
+    vals = np.asarray(vals, dtype=O.NP[t])
+    valid = None
+    if nulls:
+        valid = np.ones(len(vals), dtype=bool)
+        valid[list(nulls)] = False
+    return (vals, valid, t)
+
+
+def up(ctx, cols):
+    return [ctx.upload(v, t, valid) for v, valid, t in cols]
+
+
+def gpu_pairs(ctx, pk, bk, jt):
+    P, B = up(ctx, pk), up(ctx, bk)
+    try:
+        pr = ctx.hash_join(P, B, jt)
+        p, b = pr.numpy()
+        pr.free()
+        return p, b
+    finally:
+        for d in P + B:
+            d.free()
+
+
+def pairset(p, b):
+    return sorted(zip(p.tolist(), b.tolist()))
+
+
+def check_pairs(ctx, pk, bk, jt):
+    p, b = gpu_pairs(ctx, pk, bk, jt)
+    wp, wb = O.join_pairs(pk, bk, jt)
+    assert pairset(p, b) == pairset(wp, wb)
+    return pairset(p, b)
+
+
+@pytest.mark.parametrize("jt", [INNER, LEFT])
+def test_edge_single_key(ctx, jt):
+    bk = [col([1, 1, 2, 5, 5, 5], I32)]
+    pk = [col([1, 2, 3, 5, 0], I32, nulls=(4,))]
+    got = check_pairs(ctx, pk, bk, jt)
+    want = [(0, 0), (0, 1), (1, 2), (3, 3), (3, 4), (3, 5)]
+    if jt == LEFT:
+        want += [(2, O.NO_ROW), (4, O.NO_ROW)]
+    assert got == sorted(want)
+
+
+@pytest.mark.parametrize("jt", [INNER, LEFT])
+def test_edge_composite_i32_f64(ctx, jt):
+    bk = [col([1, 1, 2, 2], I32), col([1.0, 2.0, 1.0, 2.0], F64)]
+    pk = [col([1, 1, 2, 2], I32), col([2.0, 9.0, 1.0, 2.0], F64)]
+    got = check_pairs(ctx, pk, bk, jt)
+    assert (0, 1) in got and (2, 2) in got and (3, 3) in got
+
+
+@pytest.mark.parametrize("jt", [INNER, LEFT])
+def test_edge_empty_sides(ctx, jt):
+    got = check_pairs(ctx, [col([1, 2, 3], I32)], [col([], I32)], jt)
+    assert got == ([] if jt == INNER else [(i, O.NO_ROW) for i in range(3)])
+    assert check_pairs(ctx, [col([], I32)], [col([1, 2, 3], I32)], jt) == []
+
+
+@pytest.mark.parametrize("jt", [INNER, LEFT])
+def test_edge_all_match_and_fanout(ctx, jt):
+    got = check_pairs(ctx, [col([7, 7, 7], I32)], [col([7, 7, 7, 7], I32)], jt)
+    assert len(got) == 12
+    got = check_pairs(ctx, [col([7, 9, 7], I32)], [col([7] * 3000, I32)], jt)
+    assert len(got) == 6000 + (1 if jt == LEFT else 0)
+
+
+@pytest.mark.parametrize("jt", [INNER, LEFT])
+def test_edge_null_keys_never_match(ctx, jt):
+    got = check_pairs(ctx, [col([1, 0, 0], I32, nulls=(1, 2))], [col([1, 0, 2], I32, nulls=(1,))], jt)
+    assert (0, 0) in got and all(b != 1 for _, b in got)
+
+
+def test_f64_keys_canonical(ctx):
+    nan = float("nan")
+    got = check_pairs(ctx, [col([0.0, nan, 1.0], F64)], [col([-0.0, nan, 2.0], F64)], INNER)
+    assert got == [(0, 0), (1, 1)]
+
+
+@pytest.mark.parametrize("case", range(24))
+def test_random_joins(ctx, case):
+    r = np.random.default_rng(500 + case)
+    nb, np_ = int(r.choice([0, 1, 100, 2049, 20000])), int(r.choice([0, 1, 100, 4097, 30000]))
+    dom = int(r.choice([1, 10, 1000, 10**6, 2**40]))
+    nk = int(r.integers(1, 3))
+    t = int(r.choice([I32, I64]))
+    bp = float(r.choice([0.0, 0.1]))
+
+    def mk(n):
+        cols = []
+        for _ in range(nk):
+            nulls = tuple(np.nonzero(r.random(n) < bp)[0]) if bp else ()
+            cols.append(col(r.integers(0, min(dom, 2**31 - 1) if t == I32 else dom, n), t, nulls))
+        return cols
+    check_pairs(ctx, mk(np_), mk(nb), int(r.integers(0, 2)))
+
+
+def test_gather_emits_probe_then_build_columns(ctx):
+    pk = [col([1, 2, 3], I32)]
+    pcols = pk + [col([10.5, 20.5, 30.5], F64, nulls=(1,))]
+    bcols = [col([3, 1], I32), col([300, 100], I64)]
+    P, B = up(ctx, pcols), up(ctx, bcols)
+    try:
+        pr = ctx.hash_join(P[:1], B[:1], LEFT)
+        t = pr.gather(P, B)
+        rows = sorted(table_rows(t.numpy()), key=lambda x: x[0])
+        t.free()
+        pr.free()
+    finally:
+        for d in P + B:
+            d.free()
+    assert rows == [(1, 10.5, 1, 100), (2, None, None, None), (3, 30.5, 3, 300)]
+
+
+def join_agg_case(ctx, pcols, bcols, pk, bk, jt, gk, aggs):
+    P, B = up(ctx, pcols), up(ctx, bcols)
+    try:
+        t = ctx.join_aggregate(P, B, pk, bk, jt, gk, aggs)
+        got = table_rows(t.numpy())
+        t.free()
+    finally:
+        for d in P + B:
+            d.free()
+    want = O.join_aggregate(pcols, bcols, pk, bk, jt, gk, aggs)
+    assert_same(got, want)
+    return got
+
+
+def test_fused_join_groupby_small_config2_shape(ctx):
+    # build 2^16 unique keys + I32 group col; probe 2^20 Zipf keys, 10% misses.
+    (pk, pp), (bk, bg) = ctx.gen_join(7, 1 << 16, 1 << 20, 10000)
+    try:
+        cols_p = [pk.numpy()[0], pp.numpy()[0]]
+        cols_b = [bk.numpy()[0], bg.numpy()[0]]
+        t = ctx.join_aggregate([pk, pp], [bk, bg], [0], [0], INNER, [3], [(SUM, 1), (CS, 0)])
+        got = table_rows(t.numpy())
+        t.free()
+    finally:
+        for d in (pk, pp, bk, bg):
+            d.free()
+    assert len(np.unique(cols_b[0])) == len(cols_b[0])  # unique build keys
+    pcols = [(cols_p[0], None, I64), (cols_p[1], None, F64)]
+    bcols = [(cols_b[0], None, I64), (cols_b[1], None, I32)]
+    want = O.join_aggregate(pcols, bcols, [0], [0], INNER, [3], [(SUM, 1), (CS, 0)])
+    assert_same(got, want)
+    matched = sum(r[2] for r in got)
+    assert 0.85 * (1 << 20) < matched < 0.95 * (1 << 20)
+
+
+def test_fused_duplicate_build_keys_list_path(ctx, rng):
+    n_b, n_p = 5000, 50000
+    pcols = [col(rng.integers(0, 2000, n_p), I64), col(rng.uniform(0, 10, n_p), F64, nulls=(3, 5, 7))]
+    bcols = [col(rng.integers(0, 1500, n_b), I64), col(rng.integers(0, 40, n_b), I32, nulls=(1,))]
+    join_agg_case(ctx, pcols, bcols, [0], [0], INNER, [3], [(SUM, 1), (CS, 0), (CNT, 1), (MIN, 1), (AVG, 1)])
+
+
+def test_fused_hash_index_sparse_keys(ctx, rng):
+    keys = rng.integers(-2**60, 2**60, 4000)
+    pcols = [col(keys[rng.integers(0, 4000, 40000)], I64), col(rng.uniform(0, 10, 40000), F64)]
+    bcols = [col(keys[:3000], I64), col(rng.integers(0, 30, 3000), I32)]
+    join_agg_case(ctx, pcols, bcols, [0], [0], INNER, [3], [(SUM, 1), (CS, 0)])
+
+
+def test_general_path_left_join_probe_group_key(ctx, rng):
+    pcols = [col(rng.integers(0, 50, 3000), I32), col(rng.integers(0, 7, 3000), I32)]
+    bcols = [col(rng.integers(0, 40, 200), I32), col(rng.uniform(0, 1, 200), F64)]
+    join_agg_case(ctx, pcols, bcols, [0], [0], LEFT, [1], [(CS, 0), (SUM, 3), (CNT, 3)])
+    join_agg_case(ctx, pcols, bcols, [0], [0], INNER, [], [(CS, 0), (SUM, 3)])
+
+
+def test_key_type_mismatch_is_invalid_argument(ctx):
+    P, B = up(ctx, [col([1], I32)]), up(ctx, [col([1], I64)])
+    try:
+        with pytest.raises(ValueError, match="types must match"):
+            ctx.hash_join(P, B, INNER)
+    finally:
+        for d in P + B:
+            d.free()

@@ -1,0 +1,561 @@
+"""Pins the CPU oracle (oracle/qe_oracle.cpp) before it is trusted as the GPU
+checker:
+  1. the reference suites' literal known-answer cases, restated as data
+     (tests/hashtable_test.cpp:32-189, tests/catalog_checks_hash.cpp:35-53,
+     tests/aggregate_differential_test.cpp:84-237,
+     tests/join_differential_test.cpp:103-296,
+     tests/catalog_checks_join_sort.cpp:34-70,
+     tests/asof_differential_test.cpp:118-318,
+     tests/catalog_checks_asof.cpp:38-92);
+  2. independent brute-force models written here (dict first-seen ids, dict
+     group-by, nested-loop join, O(n*m) as-of, numpy stable lexsort);
+  3. the compiled reference operators (oracle/_ref) on random inputs, where
+     that library was built (this container; skipped elsewhere);
+  4. the committed golden fixtures under tests/golden/.
+Each planted oracle mutant must be caught by the same checks.
+"""
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import _oracle as O
+from _compare import assert_same, canon, diff
+
+I32, I64, F64, BOOL, TS = O.I32, O.I64, O.F64, O.BOOL, O.TS
+CS, CNT, SUM, MIN, MAX, AVG = O.COUNT_STAR, O.COUNT, O.SUM, O.MIN, O.MAX, O.AVG
+NO = O.NO_ROW
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+needs_ref = pytest.mark.skipif(not O.ref_available(), reason="compiled reference harness not built here")
+
+
+def col(vals, t, nulls=()):
+    vals = np.asarray(vals, dtype=O.NP[t])
+    valid = None
+    if nulls:
+        valid = np.ones(len(vals), dtype=bool)
+        valid[list(nulls)] = False
+    return (vals, valid, t)
+
+
+# ---- splitmix64 closed form -----------------------------------------------------------
+def py_mix64(x):
+    M = (1 << 64) - 1
+    x ^= x >> 30
+    x = (x * 0xBF58476D1CE4E5B9) & M
+    x ^= x >> 27
+    x = (x * 0x94D049BB133111EB) & M
+    x ^= x >> 31
+    return x
+
+
+def test_mix64_known_answers():
+    L = O.oracle()
+    assert L.orc_mix64(0) == 0  # the finalizer fixes 0
+    for x in [1, 2, 0xDEADBEEF, (1 << 63) + 12345, (1 << 64) - 1]:
+        assert L.orc_mix64(x) == py_mix64(x)
+    # A bijection on a sample: no collisions.
+    xs = np.arange(1 << 14, dtype=np.uint64)
+    assert len({L.orc_mix64(int(v)) for v in xs}) == len(xs)
+
+
+# ---- hash table literals (tests/hashtable_test.cpp) -------------------------------------
+def test_ht_first_seen_dense_ids():
+    t = O.OracleHT([I32])
+    g = t.insert_or_find([col([7, 7, 3, 7, 3, 100], I32)])
+    assert g.tolist() == [0, 0, 1, 0, 1, 2] and t.num_groups == 3
+
+
+def test_ht_idempotent_across_calls():
+    t = O.OracleHT([I64])
+    c = [col([10, 20, 30], I64)]
+    assert t.insert_or_find(c).tolist() == [0, 1, 2]
+    assert t.insert_or_find(c).tolist() == [0, 1, 2] and t.num_groups == 3
+
+
+def test_ht_find_misses_without_mutating():
+    t = O.OracleHT([I32])
+    t.insert_or_find([col([1, 2, 3], I32)])
+    assert t.find([col([2, 99, 1, 12345], I32)]).tolist() == [1, NO, 0, NO]
+    assert t.num_groups == 3
+
+
+def test_ht_composite_keys():
+    t = O.OracleHT([I32, I64])
+    g = t.insert_or_find([col([1, 1, 1, 2], I32), col([10, 10, 20, 10], I64)]).tolist()
+    assert g[0] == g[1] and g[0] != g[2] and g[0] != g[3] and t.num_groups == 3
+
+
+def test_ht_kequal_nulls_group_together():
+    t = O.OracleHT([I32], policy=0)
+    g = t.insert_or_find([col([0, 5, 0, 5, 0], I32, nulls=(0, 2, 4))]).tolist()
+    assert g == [0, 1, 0, 1, 0] and t.num_groups == 2
+    assert t.key_is_null(0, 0) and not t.key_is_null(1, 0)
+    t2 = O.OracleHT([I32, I32], policy=0)
+    g2 = t2.insert_or_find([col([0, 0, 7], I32, nulls=(0, 1)), col([5, 5, 5], I32)]).tolist()
+    assert g2[0] == g2[1] != g2[2] and t2.num_groups == 2
+
+
+def test_ht_never_match_dead_ids():
+    t = O.OracleHT([I32], policy=1)
+    g = t.insert_or_find([col([0, 5, 0, 5], I32, nulls=(0, 2))]).tolist()
+    assert g == [0, 1, 2, 1]  # dead ids interleave in first-seen order
+    assert t.find([col([0, 5], I32, nulls=(0,))]).tolist() == [NO, 1]
+
+
+def test_ht_f64_canonical_keys():
+    t = O.OracleHT([F64])
+    nan2 = -float("nan")
+    g = t.insert_or_find([col([0.0, -0.0, float("nan"), nan2, 1.5], F64)]).tolist()
+    assert g == [0, 0, 1, 1, 2]
+
+
+def test_ht_key_readback():
+    t = O.OracleHT([I32, F64])
+    g = t.insert_or_find([col([42], I32), col([3.25], F64)]).tolist()
+    assert np.uint64(t.key_word(g[0], 0)).astype(np.uint32).view(np.int32) == 42
+    assert np.array([t.key_word(g[0], 1)], dtype=np.uint64).view(np.float64)[0] == 3.25
+
+
+def test_ht_selection_vector():
+    t = O.OracleHT([I32])
+    g = t.insert_or_find([col([9, 9, 9, 8, 8], I32)], sel=[0, 3]).tolist()
+    assert g == [0, 1] and t.num_groups == 2
+
+
+def _sequential_4000(t):
+    keys = [col(np.arange(4000), I64)]
+    g = t.insert_or_find(keys)
+    f = t.find(keys)
+    return bool((g == np.arange(4000)).all() and (f == np.arange(4000)).all() and t.num_groups == 4000)
+
+
+def test_ht_catalog_collision_storm_and_mutants():
+    # tests/catalog_checks_hash.cpp:35-53: 4000 sequential keys, cap 8, LF 0.9.
+    seed = 0x9E3779B97F4A7C15
+    assert _sequential_4000(O.OracleHT([I64], init_cap=8, lf=0.9, seed=seed))
+    assert not _sequential_4000(O.OracleHT([I64], init_cap=8, lf=0.9, seed=seed, mutant=1))
+    assert not _sequential_4000(O.OracleHT([I64], init_cap=8, lf=0.9, seed=seed, mutant=2))
+
+
+def _dict_model(rows_keys, nulls, policy, state):
+    """First-seen ids with a Python dict (tests/hashtable_fuzz_test.cpp:54-94)."""
+    out = []
+    for k, nl in zip(rows_keys, nulls):
+        if policy == 1 and any(nl):
+            out.append(state["next"])
+            state["next"] += 1
+            continue
+        key = tuple(None if n else v for v, n in zip(k, nl))
+        if key not in state["map"]:
+            state["map"][key] = state["next"]
+            state["next"] += 1
+        out.append(state["map"][key])
+    return out
+
+
+@pytest.mark.parametrize("case", range(20))
+def test_ht_fuzz_vs_dict_model(case):
+    r = np.random.default_rng(case)
+    policy = case % 2
+    ncols = 1 + case % 3
+    t = O.OracleHT([I64] * ncols, policy=policy, init_cap=8, lf=float(r.choice([0.5, 0.7, 0.9])))
+    state = {"map": {}, "next": 0}
+    for _ in range(5):
+        n = int(r.integers(0, 3000))
+        cols = [col(r.integers(0, 40, n), I64, tuple(np.nonzero(r.random(n) < 0.05)[0])) for _ in range(ncols)]
+        got = t.insert_or_find(cols).tolist()
+        keys = list(zip(*[c[0].tolist() for c in cols])) if n else []
+        nl = list(zip(*[(~c[1]).tolist() if c[1] is not None else [False] * n for c in cols])) if n else []
+        assert got == _dict_model(keys, nl, policy, state)
+    assert t.num_groups == state["next"]
+
+
+@needs_ref
+@pytest.mark.parametrize("case", range(10))
+def test_ht_vs_compiled_reference(case):
+    r = np.random.default_rng(100 + case)
+    policy = case % 2
+    types = [I32, F64, I64][: 1 + case % 3]
+    a = O.OracleHT(types, policy=policy, init_cap=8, lf=0.9)
+    b = O.OracleHT(types, policy=policy, init_cap=8, lf=0.9, use_ref=True)
+    for _ in range(4):
+        n = int(r.integers(1, 4000))
+        cols = []
+        for t in types:
+            vals = r.integers(0, 50, n) if t != F64 else r.choice([0.0, -0.0, 1.5, float("nan"), 2.5], n)
+            cols.append(col(vals, t, tuple(np.nonzero(r.random(n) < 0.05)[0])))
+        sel = np.sort(r.choice(n, size=n // 2, replace=False)).astype(np.uint32) if case % 3 == 0 else None
+        assert (a.insert_or_find(cols, sel) == b.insert_or_find(cols, sel)).all()
+        assert (a.find(cols) == b.find(cols)).all()
+    assert a.num_groups == b.num_groups
+    for g in range(a.num_groups):
+        for j in range(len(types)):
+            assert a.key_is_null(g, j) == b.key_is_null(g, j)
+            if not a.key_is_null(g, j):
+                assert a.key_word(g, j) == b.key_word(g, j)
+
+
+# ---- aggregate ---------------------------------------------------------------------------
+def edge_table():
+    return [col([1, 1, 2, 2, 3, 1, 2, 3, 0, 0], I32, nulls=(8, 9)),
+            col([10, 20, 30, 40, 50, 60, 70, 80, 90, 100], I32, nulls=(0,)),
+            col([1.5, 2.5, 3.5, 4.5, 5.5, 6.5, 7.5, 8.5, 9.5, 10.5], F64, nulls=(2,))]
+
+
+FULL = [(CS, 0), (CNT, 1), (SUM, 1), (MIN, 1), (MAX, 1), (AVG, 1), (SUM, 2), (AVG, 2)]
+
+
+def test_agg_edge_table_hand_derived():
+    want = [(1, 3, 2, 80, 20, 60, 40.0, 10.5, 3.5),
+            (2, 3, 3, 140, 30, 70, 140 / 3, 12.0, 6.0),
+            (3, 2, 2, 130, 50, 80, 65.0, 14.0, 7.0),
+            (None, 2, 2, 190, 90, 100, 95.0, 20.0, 10.0)]
+    got = O.aggregate(edge_table(), [0], FULL)
+    assert [r[0] for r in got] == [1, 2, 3, None]  # first-seen group order
+    assert_same(got, want, rel=0, abs_=0)
+    g = O.aggregate(edge_table(), [], FULL)
+    assert_same(g, [(10, 9, 540, 20, 100, 60.0, 56.5, 56.5 / 9)], rel=1e-15, abs_=0)
+
+
+def test_agg_all_null_group_empty_and_nan():
+    got = O.aggregate([col([1, 1, 2, 2], I32), col([0, 0, 5, 6], I32, nulls=(0, 1))], [0],
+                      [(CS, 0), (CNT, 1), (SUM, 1), (MIN, 1), (MAX, 1), (AVG, 1)])
+    assert got[0] == (1, 2, 0, None, None, None, None)
+    empty = [col([], I32), col([], F64)]
+    assert O.aggregate(empty, [], [(CS, 0), (CNT, 0), (SUM, 0), (MIN, 0)]) == [(0, 0, None, None)]
+    assert O.aggregate(empty, [0], [(CS, 0), (SUM, 1)]) == []
+    nan = float("nan")
+    rows = O.aggregate([col([1, 1, 2, 2, 3, 3, 4, 4], I32), col([nan, 1.5, nan, nan, 2.5, 3.5, nan, 0.0], F64,
+                                                                 nulls=(7,))], [0], [(MIN, 1), (MAX, 1), (CNT, 1)])
+    d = {r[0]: r for r in rows}
+    assert d[1][1] == 1.5 and math.isnan(d[1][2])
+    assert math.isnan(d[2][1]) and math.isnan(d[2][2])
+    assert d[3][1:3] == (2.5, 3.5)
+    assert math.isnan(d[4][1]) and math.isnan(d[4][2]) and d[4][3] == 1
+
+
+def _py_groupby(cols, keys, aggs):
+    groups = {}
+    order = []
+    n = len(cols[0][0]) if cols else 0
+
+    def val(c, i):
+        v, valid, t = c
+        if valid is not None and not valid[i]:
+            return None
+        x = v[i]
+        return float(x) if t == F64 else int(x)
+
+    def kval(c, i):
+        x = val(c, i)
+        if isinstance(x, float):
+            if math.isnan(x):
+                return ("nan",)
+            if x == 0.0:
+                return 0.0
+        return x
+    for i in range(n):
+        k = tuple(kval(cols[j], i) for j in keys)
+        if k not in groups:
+            groups[k] = []
+            order.append(k)
+        groups[k].append(i)
+    if not keys:
+        order = [()]
+        groups[()] = list(range(n))
+    out = []
+    for k in order:
+        rows = groups[k]
+        rec = [None if kk is None else (float("nan") if kk == ("nan",) else kk) for kk in k]
+        for f, c in aggs:
+            if f == CS:
+                rec.append(len(rows))
+                continue
+            vs = [val(cols[c], i) for i in rows]
+            vs = [v for v in vs if v is not None]
+            if f == CNT:
+                rec.append(len(vs))
+            elif not vs:
+                rec.append(None)
+            elif f == SUM:
+                rec.append(math.fsum(vs) if cols[c][2] == F64 else sum(vs))
+            elif f == AVG:
+                rec.append((math.fsum(vs) if cols[c][2] == F64 else float(sum(vs))) / len(vs))
+            elif f == MIN:
+                if cols[c][2] == F64:
+                    real = [v for v in vs if not math.isnan(v)]
+                    rec.append(min(real) if real else float("nan"))
+                else:
+                    rec.append(min(vs))
+            else:
+                if cols[c][2] == F64:
+                    rec.append(float("nan") if any(math.isnan(v) for v in vs) else max(vs))
+                else:
+                    rec.append(max(vs))
+        out.append(tuple(rec))
+    return out
+
+
+@pytest.mark.parametrize("case", range(20))
+def test_agg_vs_bruteforce(case):
+    r = np.random.default_rng(200 + case)
+    n = int(r.choice([0, 1, 5, 100, 3000]))
+    p = float(r.choice([0.0, 0.1, 0.6]))
+
+    def nulls():
+        return tuple(np.nonzero(r.random(n) < p)[0]) if p else ()
+    cols = [col(r.integers(-5, 5, n), I32, nulls()), col(r.integers(-10**6, 10**6, n), I64, nulls()),
+            col(r.normal(0, 100, n), F64, nulls()), col(r.choice([0.0, -0.0, 1.0, 2.0], n), F64)]
+    keys = [[], [0], [0, 3], [3]][case % 4]
+    aggs = [(CS, 0), (CNT, 1), (SUM, 1), (MIN, 1), (MAX, 2), (AVG, 2), (SUM, 2), (MIN, 2)]
+    got = O.aggregate(cols, keys, aggs)
+    want = _py_groupby(cols, keys, aggs)
+    assert_same(got, want, rel=1e-13, abs_=1e-9)
+
+
+def test_agg_mutants_are_caught():
+    t = edge_table()
+    clean = O.aggregate(t, [0], FULL)
+    for m in (1, 4):
+        assert diff(O.aggregate(t, [0], FULL, mutant=m), clean) is not None, m
+    allnull = [col([1, 1, 2], I32), col([0, 0, 5], I32, nulls=(0, 1))]
+    aggs = [(SUM, 1), (MIN, 1)]
+    assert diff(O.aggregate(allnull, [0], aggs, mutant=2), O.aggregate(allnull, [0], aggs)) is not None
+    nan = float("nan")
+    nt = [col([1, 1], I32), col([nan, 1.0], F64)]
+    assert diff(O.aggregate(nt, [0], [(MAX, 1)], mutant=3), O.aggregate(nt, [0], [(MAX, 1)])) is not None
+
+
+@needs_ref
+@pytest.mark.parametrize("case", range(12))
+def test_agg_vs_compiled_reference_bit_exact(case):
+    # sum_mode=0 folds F64 in row order exactly like the reference, so even the
+    # F64 sums match bit for bit.
+    r = np.random.default_rng(300 + case)
+    n = int(r.choice([0, 1, 64, 2047, 2049, 9000]))
+    p = float(r.choice([0.0, 0.2]))
+
+    def nulls():
+        return tuple(np.nonzero(r.random(n) < p)[0]) if p else ()
+    cols = [col(r.integers(-20, 20, n), I32, nulls()), col(r.integers(-10**9, 10**9, n), I64, nulls()),
+            col(r.normal(0, 1e3, n), F64, nulls())]
+    keys = [[], [0], [0, 2]][case % 3]
+    aggs = [(CS, 0), (CNT, 2), (SUM, 1), (MIN, 1), (MAX, 1), (AVG, 1), (SUM, 2), (AVG, 2), (MIN, 2), (MAX, 2)]
+    got = O.aggregate(cols, keys, aggs, sum_mode=0)
+    want = O.ref_aggregate(cols, keys, aggs, batch_size=int(r.choice([64, 256, 2048])))
+    if keys:
+        assert diff(got, want, rel=0, abs_=0) is None
+    else:
+        # The zero-key path reduces F64 in SIMD lane order (ops/agg_kernels.cpp:26-35),
+        # which depends on the host vector width: order-only differences.
+        assert diff(got, want, rel=1e-12, abs_=0) is None
+
+
+# ---- join ----------------------------------------------------------------------------------
+def pairs(p, b):
+    return sorted(zip(p.tolist(), b.tolist()))
+
+
+def test_join_literals():
+    bk = [col([1, 1, 2, 5, 5, 5], I32)]
+    pk = [col([1, 2, 3, 5, 0], I32, nulls=(4,))]
+    inner = [(0, 0), (0, 1), (1, 2), (3, 3), (3, 4), (3, 5)]
+    assert pairs(*O.join_pairs(pk, bk, 0)) == inner
+    assert pairs(*O.join_pairs(pk, bk, 1)) == sorted(inner + [(2, NO), (4, NO)])
+    comp_b = [col([1, 1, 2, 2], I32), col([1.0, 2.0, 1.0, 2.0], F64)]
+    comp_p = [col([1, 1, 2, 2], I32), col([2.0, 9.0, 1.0, 2.0], F64)]
+    assert pairs(*O.join_pairs(comp_p, comp_b, 0)) == [(0, 1), (2, 2), (3, 3)]
+    # catalog_checks_join_sort.cpp:34-70: probe {1,2,3} x build {1:10,11; 2:20} = 3 rows
+    cat = O.join_pairs([col([1, 2, 3], I32)], [col([1, 1, 2], I32)], 0)
+    assert pairs(*cat) == [(0, 0), (0, 1), (1, 2)]
+    assert len(O.join_pairs([col([7, 9, 7], I32)], [col([7] * 3000, I32)], 0)[0]) == 6000
+    nn = O.join_pairs([col([1, 0, 0], I32, nulls=(1, 2))], [col([1, 0, 2], I32, nulls=(1,))], 1)
+    assert pairs(*nn) == [(0, 0), (1, NO), (2, NO)]
+    assert pairs(*O.join_pairs([col([1, 2, 3], I32)], [col([], I32)], 1)) == [(0, NO), (1, NO), (2, NO)]
+
+
+def test_join_mutants_are_caught():
+    pk, bk = [col([1, 2, 3], I32)], [col([1, 1, 2], I32)]
+    clean = pairs(*O.join_pairs(pk, bk, 0))
+    assert pairs(*O.join_pairs(pk, bk, 0, mutant=1)) != clean
+    assert pairs(*O.join_pairs(pk, bk, 0, mutant=2)) != clean
+    cp = [col([1, 1], I32), col([2.0, 9.0], F64)]
+    cb = [col([1, 1], I32), col([1.0, 2.0], F64)]
+    assert pairs(*O.join_pairs(cp, cb, 0, mutant=3)) != pairs(*O.join_pairs(cp, cb, 0))
+
+
+@pytest.mark.parametrize("case", range(15))
+def test_join_vs_nested_loop(case):
+    r = np.random.default_rng(400 + case)
+    nb, np_ = int(r.integers(0, 300)), int(r.integers(0, 300))
+    nk = 1 + case % 2
+    bcols = [col(r.integers(0, 12, nb), I64, tuple(np.nonzero(r.random(nb) < 0.1)[0])) for _ in range(nk)]
+    pcols = [col(r.integers(0, 12, np_), I64, tuple(np.nonzero(r.random(np_) < 0.1)[0])) for _ in range(nk)]
+    jt = case % 2
+
+    def key(cols, i):
+        out = []
+        for v, valid, _ in cols:
+            if valid is not None and not valid[i]:
+                return None
+            out.append(int(v[i]))
+        return tuple(out)
+    want = []
+    for i in range(np_):
+        ki = key(pcols, i)
+        m = [j for j in range(nb) if ki is not None and key(bcols, j) == ki]
+        want += [(i, j) for j in m] if m else ([(i, NO)] if jt else [])
+    assert pairs(*O.join_pairs(pcols, bcols, jt)) == sorted(want)
+
+
+@needs_ref
+@pytest.mark.parametrize("case", range(10))
+def test_join_vs_compiled_reference(case):
+    r = np.random.default_rng(500 + case)
+    nb, np_ = int(r.integers(0, 3000)), int(r.integers(0, 5000))
+    types = [I32, F64][: 1 + case % 2]
+
+    def mk(n):
+        out = []
+        for t in types:
+            v = r.integers(0, 300, n) if t == I32 else r.choice([0.0, -0.0, 1.5, float("nan")], n)
+            out.append(col(v, t, tuple(np.nonzero(r.random(n) < 0.05)[0])))
+        return out
+    pc, bc = mk(np_), mk(nb)
+    jt = case % 2
+    keys = list(range(len(types)))
+    got = pairs(*O.join_pairs(pc, bc, jt))
+    want = pairs(*O.ref_join_pairs(pc, bc, keys, keys, jt, batch_size=int(r.choice([64, 2048]))))
+    assert got == want
+
+
+@needs_ref
+@pytest.mark.parametrize("case", range(6))
+def test_join_aggregate_vs_compiled_reference(case):
+    r = np.random.default_rng(600 + case)
+    nb, np_ = int(r.integers(1, 2000)), int(r.integers(0, 20000))
+    pc = [col(r.integers(0, 2500, np_), I64), col(r.uniform(0, 1000, np_), F64)]
+    bc = [col(r.permutation(2500)[:nb] if case % 2 else r.integers(0, 2500, nb), I64),
+          col(r.integers(0, 100, nb), I32)]
+    aggs = [(SUM, 1), (CS, 0), (AVG, 1), (MAX, 1)]
+    got = O.join_aggregate(pc, bc, [0], [0], 0, [3], aggs, sum_mode=0)
+    want = O.ref_join_aggregate(pc, bc, [0], [0], 0, [3], aggs)
+    assert_same(got, want, rel=0, abs_=0)
+
+
+# ---- as-of ---------------------------------------------------------------------------------
+def asof_build():
+    return [col([1, 1, 1, 2], I32), col([10, 20, 30, 15], TS), col([100, 200, 300, 150], I64)]
+
+
+def asof_probe():
+    return [col([1, 1, 1, 2, 3, 1], I32), col([5, 10, 25, 15, 99, 50], TS)]
+
+
+def test_asof_literals():
+    b, p = asof_build(), asof_probe()
+    # (1,5) none; (1,10)->row0; (1,25)->row1; (2,15)->row3; (3,99) none; (1,50)->row2
+    assert pairs(*O.asof_pairs(p, b, [0], [0], 1, 1, 0)) == [(1, 0), (2, 1), (3, 3), (5, 2)]
+    assert pairs(*O.asof_pairs(p, b, [0], [0], 1, 1, 1)) == [(0, NO), (1, 0), (2, 1), (3, 3), (4, NO), (5, 2)]
+    assert pairs(*O.asof_pairs(p, b, [0], [0], 1, 1, 0, tolerance=10)) == [(1, 0), (2, 1), (3, 3)]
+    zb, zp = [col([10, 20, 30], TS)], [col([5, 25, 35], TS)]
+    assert pairs(*O.asof_pairs(zp, zb, [], [], 0, 0, 1)) == [(0, NO), (1, 1), (2, 2)]
+    nb = [col([1, 1, 2, 7], I32, nulls=(3,)), col([10, 20, 0, 5], TS, nulls=(2,))]
+    npb = [col([1, 2, 2, 9], I32, nulls=(3,)), col([25, 30, 0, 50], TS, nulls=(2,))]
+    assert pairs(*O.asof_pairs(npb, nb, [0], [0], 1, 1, 1)) == [(0, 1), (1, NO), (2, NO), (3, NO)]
+    cb = [col([1, 1, 1], I32), col([7, 7, 9], I32), col([10, 20, 10], TS)]
+    cp = [col([1, 1], I32), col([7, 9], I32), col([25, 25], TS)]
+    assert pairs(*O.asof_pairs(cp, cb, [0, 1], [0, 1], 2, 2, 0)) == [(0, 1), (1, 2)]
+    # catalog_checks_asof.cpp:38-92: exact boundary ties match (the `<=` rule).
+    kb = [col([1, 1, 2], I32), col([10, 20, 15], I64)]
+    kp = [col([1, 1, 2], I32), col([20, 25, 15], I64)]
+    assert pairs(*O.asof_pairs(kp, kb, [0], [0], 1, 1, 0)) == [(0, 1), (1, 1), (2, 2)]
+    for m in (1, 2):
+        assert pairs(*O.asof_pairs(kp, kb, [0], [0], 1, 1, 0, mutant=m)) != [(0, 1), (1, 1), (2, 2)]
+    assert pairs(*O.asof_pairs(cp, cb, [0, 1], [0, 1], 2, 2, 0, mutant=3)) != [(0, 1), (1, 2)]
+
+
+def brute_asof(pk, pt, bk, bt):
+    """O(n*m) scan; ts unique per key in the generated data (tsx/asof.h:28-39)."""
+    out = []
+    for i in range(len(pt)):
+        best, bts = NO, None
+        for j in range(len(bt)):
+            if bk[j] == pk[i] and bt[j] <= pt[i] and (bts is None or bt[j] > bts):
+                best, bts = j, bt[j]
+        out.append(best)
+    return np.array(out, dtype=np.uint32)
+
+
+@pytest.mark.parametrize("case", range(12))
+def test_asof_vs_bruteforce(case):
+    r = np.random.default_rng(700 + case)
+    nb, np_ = int(r.integers(0, 400)), int(r.integers(0, 400))
+    bk = r.integers(0, 6, nb)
+    bt = r.permutation(np.arange(nb) * 7 + 3)  # globally distinct build times
+    pk = r.integers(0, 7, np_)
+    pt = r.integers(0, nb * 7 + 10, np_)
+    want = brute_asof(pk, pt, bk, bt)
+    p, b = O.asof_pairs([col(pk, I32), col(pt, TS)], [col(bk, I32), col(bt, TS)], [0], [0], 1, 1, 1)
+    got = np.full(np_, NO, dtype=np.uint32)
+    got[p] = b
+    assert (got == want).all()
+
+
+@needs_ref
+@pytest.mark.parametrize("case", range(8))
+def test_asof_vs_compiled_reference(case):
+    r = np.random.default_rng(800 + case)
+    nb, np_ = int(r.integers(0, 4000)), int(r.integers(0, 4000))
+    bk = r.integers(0, 30, nb)
+    bt = r.permutation(nb) * 3
+    bc = [col(bk, I32, tuple(np.nonzero(r.random(nb) < 0.03)[0])), col(bt, I64)]
+    pc = [col(r.integers(0, 33, np_), I32), col(r.integers(-5, nb * 3 + 5, np_), I64,
+                                                 tuple(np.nonzero(r.random(np_) < 0.03)[0]))]
+    tol = None if case % 2 else int(r.integers(0, 50))
+    jt = case % 3 == 0
+    got = pairs(*O.asof_pairs(pc, bc, [0], [0], 1, 1, int(jt), tolerance=tol))
+    want = pairs(*O.ref_asof_pairs(pc, bc, [0], [0], 1, 1, int(jt), tolerance=tol, batch_size=256))
+    assert got == want
+
+
+# ---- sort ----------------------------------------------------------------------------------
+@pytest.mark.parametrize("case", range(6))
+def test_sort_perm_vs_numpy_lexsort(case):
+    r = np.random.default_rng(900 + case)
+    n = int(r.integers(0, 5000))
+    a = r.integers(-50, 50, n)
+    b = r.integers(-10**12, 10**12, n)
+    perm = O.sort_perm([col(a, I32), col(b, I64)], [0, 1])
+    assert (perm == np.lexsort((b, a), axis=0).astype(np.uint32)).all()  # stable, a primary
+
+
+# ---- committed golden fixtures ---------------------------------------------------------------
+def test_golden_fixtures_match_oracle():
+    files = sorted(GOLDEN.glob("*.json"))
+    if not files:
+        pytest.skip("no golden fixtures")
+    for f in files:
+        fx = json.loads(f.read_text())
+        kind = fx["kind"]
+        cols = lambda spec: [col(np.array(c["values"], dtype=O.NP[c["type"]]), c["type"],
+                                 tuple(c.get("nulls", []))) for c in spec]
+        if kind == "groupby":
+            got = O.aggregate(cols(fx["cols"]), fx["keys"], [tuple(a) for a in fx["aggs"]], sum_mode=0)
+            want = [tuple(r) for r in fx["expected"]]
+            assert_same(got, want, rel=0, abs_=0), f.name
+        elif kind == "join_groupby":
+            got = O.join_aggregate(cols(fx["probe"]), cols(fx["build"]), fx["pkeys"], fx["bkeys"], 0,
+                                   fx["gkeys"], [tuple(a) for a in fx["aggs"]], sum_mode=0)
+            assert_same(got, [tuple(r) for r in fx["expected"]], rel=0, abs_=0), f.name
+        elif kind == "asof":
+            p, b = O.asof_pairs(cols(fx["probe"]), cols(fx["build"]), fx["pkeys"], fx["bkeys"], fx["ptime"],
+                                fx["btime"], fx["join_type"])
+            assert pairs(p, b) == sorted(tuple(x) for x in fx["expected_pairs"]), f.name
+        elif kind == "hashtable":
+            t = O.OracleHT(fx["types"], policy=fx["policy"], init_cap=fx["init_cap"], lf=fx["lf"])
+            assert t.insert_or_find(cols(fx["cols"])).tolist() == fx["expected_ids"], f.name
