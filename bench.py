@@ -144,6 +144,161 @@ def cpu_baseline_join(qe, ctx, cols, args):
     }
 
 
+def asof_workload(qe, ctx, args):
+    """Right 400*2^20 rows (sym Zipf(1.1) over 5000, bursty per-sym microsecond
+    times, price), left 100*2^20 probes; device sort + learned-index training
+    (build phase, reported separately); probe phase timed per method."""
+    rn, ln = args.asof_right, args.asof_left
+    log(f"generating as-of workload: right {rn} left {ln}")
+    (rs, rt, rp), (ls, lt) = ctx.gen_asof(20260616, rn, ln, 5000, 1.1)
+    t0 = time.perf_counter()
+    idx = ctx.asof_build([rs], rt)
+    ctx.sync()
+    t1 = time.perf_counter()
+    idx.train(args.piece_rows, args.train_steps, 0.01)
+    ctx.sync()
+    t2 = time.perf_counter()
+    out = qe.DevBuffer(ctx, 4 * ln)
+    res = {"right_rows": rn, "left_rows": ln, "build_sort_s": t1 - t0, "train_s": t2 - t1,
+           "piece_rows": args.piece_rows, "train_steps": args.train_steps}
+    results = {}
+    for name, mode, timer in (("binary", qe.ASOF_BINARY, "asof_probe_binary"),
+                              ("learned", qe.ASOF_LEARNED, "asof_probe_learned"),
+                              ("learned_no_mlp", qe.ASOF_LEARNED_NO_MLP, "asof_probe_learned_no_mlp")):
+        for _ in range(2):
+            idx.probe_into([ls], lt, out, mode=mode)
+        ctx.set_timing(True)
+        ctx.timing(timer)
+        for _ in range(args.asof_reps):
+            idx.probe_into([ls], lt, out, mode=mode)
+        ms, n = ctx.timing(timer)
+        ctx.set_timing(False)
+        results[name] = out.numpy(np.uint32, ln)
+        res[f"{name}_ms"] = ms / max(n, 1)
+        res[f"{name}_probes_per_s"] = ln / (ms / max(n, 1) / 1e3)
+    st = idx.stats()
+    res["learned_equals_binary"] = bool((results["binary"] == results["learned"]).all()
+                                        and (results["binary"] == results["learned_no_mlp"]).all())
+    res["speedup_learned_vs_binary"] = res["binary_ms"] / res["learned_ms"]
+    res["speedup_mlp_vs_no_mlp"] = res["learned_no_mlp_ms"] / res["learned_ms"]
+    res["matched_frac"] = float((results["binary"] != qe.NO_ROW).mean())
+    probe_bytes = ln * 16  # left sym 4 + ts 8 + match index 4
+    res["learned_achieved_gbs"] = probe_bytes / (res["learned_ms"] / 1e3) / 1e9
+    res["binary_achieved_gbs"] = probe_bytes / (res["binary_ms"] / 1e3) / 1e9
+    res["stats"] = {k: (float(v) if isinstance(v, float) else int(v)) for k, v in st.items()}
+    if not args.skip_cpu:
+        res["cpu_baseline"] = cpu_baseline_asof(qe, ctx, idx, ls, lt, results["binary"], st, args)
+    out.free()
+    idx.free()
+    for c in (rs, rt, rp, ls, lt):
+        c.free()
+    return res
+
+
+def cpu_baseline_asof(qe, ctx, idx, ls, lt, gpu_match, st, args):
+    """Parallel CPU upper_bound probe over the same sorted right side (the
+    reference has no parallel as-of); bounded probe sample."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import _oracle as O
+    L = O.oracle()
+    n = int(st["build_rows"])
+    parts = int(st["partitions"])
+    ts, rows, offs = idx.sorted(n, parts)
+    sample = min(1 << args.cpu_asof_log2, ls.len)
+    lsn, _ = ls.numpy()
+    ltn, _ = lt.numpy()
+    lsn, ltn = np.ascontiguousarray(lsn[:sample]), np.ascontiguousarray(ltn[:sample])
+    out = np.empty(sample, np.uint32)
+    threads = int(L.orc_bl_hw_threads())
+    t0 = time.perf_counter()
+    L.orc_bl_asof_upper_bound(ts.ctypes.data, offs.ctypes.data, parts, lsn.ctypes.data, ltn.ctypes.data, sample,
+                              threads, out.ctypes.data)
+    t1 = time.perf_counter()
+    mapped = np.where(out == qe.NO_ROW, qe.NO_ROW, rows[np.minimum(out, max(n - 1, 0))])
+    return {"value": sample / (t1 - t0), "unit": "probe rows/s", "cores": threads, "kind": "port",
+            "sample": f"first {sample} probes against the full sorted right side; std::upper_bound per probe "
+                      f"(oracle/qe_baseline.cpp)",
+            "gpu_parity_on_sample": bool((mapped == gpu_match[:sample]).all())}
+
+
+def batched_workload(qe, ctx, args):
+    """64 independent GROUP BY queries over 4096 morsels (64 per query), morsel
+    rows log-uniform [2^11, 2^21], groups per query log-uniform [16, 10^7]
+    (capped by rows), I32/I64 keys, F64/I64 values with 2% NULLs;
+    COUNT(*), SUM, AVG per group."""
+    r = np.random.default_rng(20260617)
+    nq, per = args.batch_queries, args.batch_morsels // args.batch_queries
+    lo, hi = np.log(2 ** 11), np.log(2 ** args.batch_max_log2)
+    queries, total_rows = [], 0
+    log(f"generating batched workload: {nq} queries x {per} morsels")
+    for q in range(nq):
+        sizes = np.exp(r.uniform(lo, hi, per)).astype(np.int64)
+        offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+        n = int(offs[-1])
+        groups = int(min(np.exp(r.uniform(np.log(16), np.log(1e7))), n))
+        kt = qe.I32 if r.random() < 0.5 else qe.I64
+        vt = qe.F64 if r.random() < 0.5 else qe.I64
+        k = ctx.gen_uniform(20260617, 2 * q, kt, n, 0, groups)
+        v = ctx.gen_uniform(20260617, 2 * q + 1, vt, n, 0, 1000, 0.02)
+        queries.append((k, v, offs, [(qe.COUNT_STAR, 0), (qe.SUM, 1), (qe.AVG, 1)], groups))
+        total_rows += n
+    plan = ctx.batch(queries)
+    res = {"queries": nq, "morsels": nq * per, "rows": total_rows}
+    t0 = time.perf_counter()
+    tune = plan.autotune()
+    res["autotune_s"] = time.perf_counter() - t0
+    res["tuning"] = {"tile_rows": tune[0], "blocks_per_cu": tune[1], "lds_slots": tune[2]}
+
+    def timed(mode, reps):
+        plan.run(mode, tuning=tune, collect=False)
+        ctx.sync()
+        t = time.perf_counter()
+        for _ in range(reps):
+            plan.run(mode, tuning=tune, collect=False)
+        ctx.sync()
+        return (time.perf_counter() - t) / reps
+
+    res["persistent_s"] = timed(qe.BATCH_PERSISTENT, 3)
+    res["graph_s"] = timed(qe.BATCH_GRAPH, 3)
+    res["per_batch_s"] = timed(qe.BATCH_PER_BATCH, 1)
+    res["speedup_persistent_vs_per_batch"] = res["per_batch_s"] / res["persistent_s"]
+    res["speedup_graph_vs_per_batch"] = res["per_batch_s"] / res["graph_s"]
+    res["rows_per_s_graph"] = total_rows / res["graph_s"]
+    # parity: the three modes produce identical groups on one query sample
+    outs = {m: plan.run(m, tuning=tune) for m in (qe.BATCH_PERSISTENT, qe.BATCH_GRAPH)}
+    same = True
+    for a, b in zip(outs[qe.BATCH_PERSISTENT], outs[qe.BATCH_GRAPH]):
+        ka = a.numpy()
+        kb = b.numpy()
+        same &= a.rows == b.rows and np.array_equal(np.sort(ka[0][0]), np.sort(kb[0][0]))
+        a.free()
+        b.free()
+    res["graph_equals_persistent"] = bool(same)
+    plan.free()
+    for k, v, *_ in queries:
+        k.free()
+        v.free()
+    return res
+
+
+def groupby_1m(qe, ctx, args):
+    n = 1 << 20
+    k, v = ctx.gen_groupby(20260615, n, 1024, 0.05)
+    aggs = [(qe.COUNT_STAR, 0), (qe.SUM, 1), (qe.AVG, 1)]
+    for _ in range(3):
+        ctx.aggregate([k, v], [0], aggs).free()
+    ctx.sync()
+    t = time.perf_counter()
+    reps = 20
+    for _ in range(reps):
+        ctx.aggregate([k, v], [0], aggs).free()
+    ctx.sync()
+    s = (time.perf_counter() - t) / reps
+    k.free()
+    v.free()
+    return {"rows": n, "ms_per_query": s * 1e3, "rows_per_s": n / s}
+
+
 def ctx_slice(qe, ctx, col, n):
     import ctypes as C
     out = C.c_void_p()
@@ -160,6 +315,17 @@ def main():
     ap.add_argument("--probe-log2", type=int, default=28)
     ap.add_argument("--cpu-probe-log2", type=int, default=24)
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--asof-right", type=int, default=400 << 20)
+    ap.add_argument("--asof-left", type=int, default=100 << 20)
+    ap.add_argument("--asof-reps", type=int, default=5)
+    ap.add_argument("--piece-rows", type=int, default=64)
+    ap.add_argument("--train-steps", type=int, default=150)
+    ap.add_argument("--cpu-asof-log2", type=int, default=22)
+    ap.add_argument("--batch-queries", type=int, default=64)
+    ap.add_argument("--batch-morsels", type=int, default=4096)
+    ap.add_argument("--batch-max-log2", type=int, default=21)
+    ap.add_argument("--only", choices=["join", "all"], default="all",
+                    help="join: headline workload only")
     ap.add_argument("--quick", action="store_true", help="small sizes (smoke)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -168,6 +334,8 @@ def main():
         sys.exit(2)
     if args.quick:
         args.build_log2, args.probe_log2, args.cpu_probe_log2 = 16, 20, 18
+        args.asof_right, args.asof_left, args.cpu_asof_log2 = 1 << 22, 1 << 20, 18
+        args.batch_queries, args.batch_morsels, args.batch_max_log2 = 8, 64, 16
     qe = load_pkg()
     ctx = qe.Context(0)
     res, cols = join_groupby_workload(qe, ctx, args)
@@ -176,6 +344,20 @@ def main():
     if not args.skip_cpu:
         cpu = cpu_baseline_join(qe, ctx, cols, args)
         log(f"cpu baseline {cpu['value']:.3e} rows/s on {cpu['cores']} threads, parity {cpu['gpu_parity_on_sample']}")
+    for c in cols.values():
+        c.free()
+    secondary = {}
+    if args.only == "all":
+        secondary["groupby_1m"] = groupby_1m(qe, ctx, args)
+        log(f"group-by 1M: {secondary['groupby_1m']['ms_per_query']:.3f} ms")
+        secondary["asof"] = asof_workload(qe, ctx, args)
+        a = secondary["asof"]
+        log(f"as-of: binary {a['binary_ms']:.3f} ms, learned {a['learned_ms']:.3f} ms "
+            f"(x{a['speedup_learned_vs_binary']:.2f}), identical={a['learned_equals_binary']}")
+        secondary["batched"] = batched_workload(qe, ctx, args)
+        b = secondary["batched"]
+        log(f"batched: graph {b['graph_s'] * 1e3:.2f} ms, per-batch {b['per_batch_s'] * 1e3:.1f} ms "
+            f"(x{b['speedup_graph_vs_per_batch']:.1f})")
     kach = res["kernel_achieved_gbs"]
     line = {
         "metric": "hash join + group-by probe throughput (rows/s) and fraction of HBM peak",
@@ -207,6 +389,7 @@ def main():
                           "frac": res["step_achieved_gbs"] / HBM_PEAK_GBS,
                           "algorithmic_bytes_per_step": res["step_bytes"]},
         "cpu_baseline": cpu,
+        "secondary": secondary,
     }
     print(json.dumps(line), flush=True)
     ctx.close()

@@ -241,6 +241,48 @@ qe_gpu_status qe_gpu_mlp_forward(qe_gpu_ctx* ctx, const float* params, const flo
 qe_gpu_status qe_gpu_mlp_grad(qe_gpu_ctx* ctx, const float* params, const float* x, const float* c, uint32_t n,
                               float* grads, float* loss);
 
+/* ---- batched group-by (many independent queries over uneven morsels) -------------*/
+/* One GROUP BY key query of the batch: its key and value columns, the row
+ * offsets of its morsels (host array, nmorsels+1 entries, the morsels of
+ * exec/morsel_scan.cpp:39-76 laid end to end), its aggregates (input_col 0 =
+ * key, 1 = value) and an optional upper bound on distinct groups (0 = rows).
+ * Replaces running one Aggregate per query through the morsel-parallel driver
+ * (exec/parallel.cpp:431-599). */
+typedef struct qe_batch_query {
+    const qe_dev_col* key;
+    const qe_dev_col* value;
+    uint32_t nmorsels;
+    const uint64_t* morsel_offsets;
+    uint32_t naggs;
+    const qe_agg_spec* aggs;
+    uint64_t group_bound;
+} qe_batch_query;
+/* Execution shape: rows per work tile, persistent blocks per CU, LDS slot
+ * cache entries per block (0 or a power of two). */
+typedef struct qe_batch_tuning {
+    uint32_t tile_rows;
+    uint32_t blocks_per_cu;
+    uint32_t lds_slots;
+} qe_batch_tuning;
+typedef enum qe_batch_mode {
+    QE_BATCH_PERSISTENT = 0, /* one persistent launch over a device tile queue      */
+    QE_BATCH_PER_BATCH = 1,  /* one launch per 2048-row batch (the baseline shape)   */
+    QE_BATCH_GRAPH = 2       /* the persistent pipeline captured once in a hipGraph  */
+} qe_batch_mode;
+typedef struct qe_batch_plan qe_batch_plan;
+qe_gpu_status qe_gpu_batch_prepare(qe_gpu_ctx* ctx, const qe_batch_query* queries, uint32_t nq, qe_batch_plan** out);
+/* Runs every query; outs (nq tables, or NULL to only execute) receive key ++
+ * aggregate columns per query, as qe_gpu_aggregate. tuning NULL = defaults. */
+qe_gpu_status qe_gpu_batch_run(qe_gpu_ctx* ctx, qe_batch_plan* plan, const qe_batch_query* queries, int32_t mode,
+                               const qe_batch_tuning* tuning, qe_dev_table** outs);
+/* Times tile size x blocks per CU x LDS cache on this batch; returns the fastest. */
+qe_gpu_status qe_gpu_batch_autotune(qe_gpu_ctx* ctx, qe_batch_plan* plan, qe_batch_tuning* out);
+void qe_gpu_batch_free(qe_gpu_ctx* ctx, qe_batch_plan* plan);
+/* Uniform values in [lo, hi) (integers floored) with Bernoulli(null_frac)
+ * NULLs; a pure function of (seed, stream, row). */
+qe_gpu_status qe_gpu_gen_uniform(qe_gpu_ctx* ctx, uint64_t seed, uint64_t stream, qe_dev_col* col, double lo, double hi,
+                                 double null_frac);
+
 /* ---- device memory helpers for callers without their own allocator ---------------*/
 qe_gpu_status qe_gpu_malloc(qe_gpu_ctx* ctx, uint64_t bytes, void** out);
 qe_gpu_status qe_gpu_free(qe_gpu_ctx* ctx, void* p);

@@ -124,6 +124,11 @@ _SIGS = {
     "qe_gpu_asof_free": (None, [P, P]),
     "qe_gpu_mlp_forward": (C.c_int, [P, P, P, C.c_uint64, P]),
     "qe_gpu_mlp_grad": (C.c_int, [P, P, P, P, C.c_uint32, P, P]),
+    "qe_gpu_batch_prepare": (C.c_int, [P, P, C.c_uint32, C.POINTER(P)]),
+    "qe_gpu_batch_run": (C.c_int, [P, P, P, C.c_int32, P, P]),
+    "qe_gpu_batch_autotune": (C.c_int, [P, P, P]),
+    "qe_gpu_batch_free": (None, [P, P]),
+    "qe_gpu_gen_uniform": (C.c_int, [P, C.c_uint64, C.c_uint64, P, C.c_double, C.c_double, C.c_double]),
     "qe_gpu_malloc": (C.c_int, [P, C.c_uint64, C.POINTER(P)]),
     "qe_gpu_free": (C.c_int, [P, P]),
     "qe_gpu_memcpy_d2h": (C.c_int, [P, P, P, C.c_uint64]),
@@ -424,6 +429,14 @@ class Context:
                                           ls.h, lt.h))
         return (rs, rt, rp), (ls, lt)
 
+    def gen_uniform(self, seed, stream, type, n, lo, hi, null_frac=0.0) -> DevCol:
+        c = self.alloc(type, n, nullable=null_frac > 0)
+        self._check(lib().qe_gpu_gen_uniform(self.h, seed, stream, c.h, lo, hi, null_frac))
+        return c
+
+    def batch(self, queries) -> "BatchPlan":
+        return BatchPlan(self, queries)
+
 
 class HashTable:
     """Device HashTable (ops/hashtable.h:108-190): first-seen dense ids."""
@@ -515,6 +528,55 @@ class AsofIndex:
     def free(self):
         if self.h:
             lib().qe_gpu_asof_free(self.ctx.h, self.h)
+        self.h = None
+
+
+class BatchQuery(C.Structure):
+    _fields_ = [("key", C.c_void_p), ("value", C.c_void_p), ("nmorsels", C.c_uint32),
+                ("morsel_offsets", C.c_void_p), ("naggs", C.c_uint32), ("aggs", C.c_void_p),
+                ("group_bound", C.c_uint64)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("tile_rows", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("lds_slots", C.c_uint32)]
+
+
+BATCH_PERSISTENT, BATCH_PER_BATCH, BATCH_GRAPH = 0, 1, 2
+
+
+class BatchPlan:
+    """Prepared batch of independent GROUP BY queries (qe_gpu_batch_*)."""
+
+    def __init__(self, ctx: "Context", queries):
+        """queries: list of (key DevCol, value DevCol, morsel_offsets ndarray, aggs, group_bound)."""
+        self.ctx = ctx
+        self._keep = []
+        arr = (BatchQuery * len(queries))()
+        for i, (k, v, offs, aggs, bound) in enumerate(queries):
+            o = np.ascontiguousarray(offs, dtype=np.uint64)
+            a = _aggs(aggs)
+            self._keep += [o, a]
+            arr[i] = BatchQuery(k.h, v.h, len(o) - 1, o.ctypes.data, len(aggs), C.cast(a, C.c_void_p), bound)
+        self.arr = arr
+        self.nq = len(queries)
+        h = P()
+        ctx._check(lib().qe_gpu_batch_prepare(ctx.h, arr, self.nq, C.byref(h)))
+        self.h = h
+
+    def run(self, mode=BATCH_PERSISTENT, tuning=None, collect=True):
+        outs = (P * self.nq)() if collect else None
+        t = None if tuning is None else C.byref(BatchTuning(*tuning))
+        self.ctx._check(lib().qe_gpu_batch_run(self.ctx.h, self.h, self.arr, mode, t, outs))
+        return [Table(self.ctx, outs[i]) for i in range(self.nq)] if collect else None
+
+    def autotune(self):
+        t = BatchTuning()
+        self.ctx._check(lib().qe_gpu_batch_autotune(self.ctx.h, self.h, C.byref(t)))
+        return (t.tile_rows, t.blocks_per_cu, t.lds_slots)
+
+    def free(self):
+        if self.h:
+            lib().qe_gpu_batch_free(self.ctx.h, self.h)
         self.h = None
 
 

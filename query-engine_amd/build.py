@@ -33,7 +33,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result", "-Wno-unused-value", f"-I{INCLUDE}", f"-I{CSRC}",
 ]
-HIP_SOURCES = ["context.hip", "groupby.hip", "join.hip", "hashtable.hip", "sort.hip",
+HIP_SOURCES = ["context.hip", "groupby.hip", "join.hip", "hashtable.hip", "sort.hip", "batched.hip",
                "asof.hip", "gen.hip"]
 HOST_SOURCES = ["operators.cpp", "plan.cpp"]
 

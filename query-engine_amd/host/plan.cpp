@@ -1,0 +1,189 @@
+// Plan builder + device lowering. See qe_plan.h.
+#include "qe_plan.h"
+
+#include <sstream>
+
+namespace qe::plan {
+
+namespace {
+
+const char* type_name(Type t) {
+    switch (t) {
+        case Type::I32:
+            return "I32";
+        case Type::I64:
+            return "I64";
+        case Type::F64:
+            return "F64";
+        case Type::BOOL:
+            return "BOOL";
+        case Type::TS:
+            return "TS";
+        case Type::STR:
+            return "STR";
+    }
+    return "?";
+}
+
+std::shared_ptr<PlanNode> make(PlanKind k) {
+    auto n = std::make_shared<PlanNode>();
+    n->kind = k;
+    return n;
+}
+
+Schema concat(const Schema& a, const Schema& b) {
+    Schema s = a;
+    for (const auto& f : b.fields) s.fields.push_back(f);
+    return s;
+}
+
+}  // namespace
+
+uint32_t ColRef::resolve(const Schema& s) const {
+    if (!by_name) {
+        if (index >= s.fields.size())
+            throw std::invalid_argument("plan: column index " + std::to_string(index) + " out of range");
+        return index;
+    }
+    for (uint32_t i = 0; i < s.fields.size(); ++i)
+        if (s.fields[i].first == name) return i;
+    throw std::invalid_argument("plan: unknown column name '" + name + "'");
+}
+
+PlanBuilder scan(const Table& table) {
+    auto n = make(PlanKind::Scan);
+    n->table = &table;
+    n->out_schema = table.schema();
+    return PlanBuilder(Plan(std::move(n)));
+}
+
+PlanBuilder PlanBuilder::aggregate(std::vector<ColRef> keys, std::vector<AggSpec> aggs) const {
+    const Schema& cs = plan_.output_schema();
+    if (aggs.empty()) throw std::invalid_argument("plan: aggregate needs >=1 aggregate");
+    auto n = make(PlanKind::Aggregate);
+    for (const ColRef& k : keys) {
+        const uint32_t i = k.resolve(cs);
+        n->group_keys.push_back(i);
+        n->out_schema.fields.push_back(cs.fields[i]);
+    }
+    for (const AggSpec& a : aggs) {
+        Type in = Type::I64;
+        if (a.func != AggFunc::CountStar) {
+            if (a.input_col >= cs.fields.size())
+                throw std::invalid_argument("plan: aggregate input column out of range");
+            in = cs.fields[a.input_col].second;
+        }
+        n->out_schema.fields.emplace_back(a.out_name, agg_result_type(a.func, in));
+    }
+    n->aggs = std::move(aggs);
+    n->children.push_back(plan_);
+    return PlanBuilder(Plan(std::move(n)));
+}
+
+PlanBuilder PlanBuilder::join(const Plan& build, std::vector<ColRef> left_keys, std::vector<ColRef> right_keys,
+                              JoinType type) const {
+    const Schema& ls = plan_.output_schema();
+    const Schema& rs = build.output_schema();
+    if (left_keys.size() != right_keys.size() || left_keys.empty())
+        throw std::invalid_argument("plan: join needs an equal, non-empty number of left/right keys");
+    auto n = make(PlanKind::Join);
+    for (size_t i = 0; i < left_keys.size(); ++i) {
+        const uint32_t li = left_keys[i].resolve(ls), ri = right_keys[i].resolve(rs);
+        if (ls.fields[li].second != rs.fields[ri].second)
+            throw std::invalid_argument("plan: join key " + std::to_string(i) + " type mismatch (left " +
+                                        type_name(ls.fields[li].second) + " vs right " +
+                                        type_name(rs.fields[ri].second) + ")");
+        n->left_keys.push_back(li);
+        n->right_keys.push_back(ri);
+    }
+    n->join_type = type;
+    n->out_schema = concat(ls, rs);
+    n->children = {plan_, build};
+    return PlanBuilder(Plan(std::move(n)));
+}
+
+PlanBuilder PlanBuilder::asof_join(const Plan& build, std::vector<ColRef> left_keys, std::vector<ColRef> right_keys,
+                                   ColRef left_time, ColRef right_time, tsx::AsofType type,
+                                   std::optional<int64_t> tolerance) const {
+    const Schema& ls = plan_.output_schema();
+    const Schema& rs = build.output_schema();
+    if (left_keys.size() != right_keys.size())
+        throw std::invalid_argument("plan: asof_join needs an equal number of left/right keys");
+    auto n = make(PlanKind::AsofJoin);
+    for (size_t i = 0; i < left_keys.size(); ++i) {
+        const uint32_t li = left_keys[i].resolve(ls), ri = right_keys[i].resolve(rs);
+        if (ls.fields[li].second != rs.fields[ri].second)
+            throw std::invalid_argument("plan: asof_join key " + std::to_string(i) + " type mismatch");
+        n->left_keys.push_back(li);
+        n->right_keys.push_back(ri);
+    }
+    n->left_time = left_time.resolve(ls);
+    n->right_time = right_time.resolve(rs);
+    auto is_time = [](Type t) { return t == Type::TS || t == Type::I32 || t == Type::I64; };
+    if (!is_time(ls.fields[n->left_time].second) || !is_time(rs.fields[n->right_time].second))
+        throw std::invalid_argument("plan: asof_join timestamp columns must be TS/I32/I64");
+    n->asof_type = type;
+    n->tolerance = tolerance;
+    n->out_schema = concat(ls, rs);
+    n->children = {plan_, build};
+    return PlanBuilder(Plan(std::move(n)));
+}
+
+std::unique_ptr<Operator> Plan::lower(const LowerOptions& opt) const {
+    const PlanNode& n = *node_;
+    switch (n.kind) {
+        case PlanKind::Scan:
+            return std::make_unique<Scan>(*n.table, opt.batch_size);
+        case PlanKind::Aggregate: {
+            const Plan& c = n.children[0];
+            if (opt.fuse_join_aggregate && c.kind() == PlanKind::Join) {
+                const PlanNode& j = c.node();
+                return std::make_unique<GpuJoinAggregate>(j.children[0].lower(opt), j.children[1].lower(opt),
+                                                          j.left_keys, j.right_keys, j.join_type, n.group_keys,
+                                                          n.aggs);
+            }
+            return std::make_unique<GpuAggregate>(c.lower(opt), n.group_keys, n.aggs);
+        }
+        case PlanKind::Join:
+            return std::make_unique<GpuHashJoin>(n.children[0].lower(opt), n.children[1].lower(opt), n.left_keys,
+                                                 n.right_keys, n.join_type);
+        case PlanKind::AsofJoin:
+            return std::make_unique<GpuAsofJoin>(n.children[0].lower(opt), n.children[1].lower(opt), n.left_keys,
+                                                 n.right_keys, n.left_time, n.right_time, n.asof_type, n.tolerance,
+                                                 opt.asof_method);
+    }
+    throw std::logic_error("plan: unhandled node kind");
+}
+
+namespace {
+void print(const Plan& p, int depth, std::ostringstream& os) {
+    const PlanNode& n = p.node();
+    os << std::string(depth * 2, ' ');
+    switch (n.kind) {
+        case PlanKind::Scan:
+            os << "Scan(" << n.out_schema.fields.size() << " cols)";
+            break;
+        case PlanKind::Aggregate:
+            os << "Aggregate(keys:" << n.group_keys.size() << " aggs:" << n.aggs.size() << ")";
+            break;
+        case PlanKind::Join:
+            os << "Join(" << (n.join_type == JoinType::Inner ? "INNER" : "LEFT") << " keys:" << n.left_keys.size()
+               << ")";
+            break;
+        case PlanKind::AsofJoin:
+            os << "AsofJoin(" << (n.asof_type == tsx::AsofType::Inner ? "INNER" : "LEFT")
+               << " keys:" << n.left_keys.size() << ")";
+            break;
+    }
+    os << "\n";
+    for (const Plan& c : n.children) print(c, depth + 1, os);
+}
+}  // namespace
+
+std::string Plan::to_string() const {
+    std::ostringstream os;
+    print(*this, 0, os);
+    return os.str();
+}
+
+}  // namespace qe::plan

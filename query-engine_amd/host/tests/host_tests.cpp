@@ -1,0 +1,291 @@
+// Drop-in tests of the host operator layer: the device operators behind the
+// reference's Operator contract and plan builder, driven the way the
+// reference's own suites drive HashJoin / Aggregate / AsofJoin / HashTable
+// (tests/join_differential_test.cpp, tests/aggregate_differential_test.cpp,
+// tests/asof_differential_test.cpp, tests/hashtable_test.cpp), with
+// hand-derived expected results. Needs a GPU; run from tests/test_gpu_host.py.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "qe_host.h"
+#include "qe_plan.h"
+
+using namespace qe;
+
+static int g_fail = 0, g_pass = 0;
+#define CHECK(cond)                                                            \
+    do {                                                                       \
+        if (cond) {                                                            \
+            ++g_pass;                                                          \
+        } else {                                                               \
+            ++g_fail;                                                          \
+            std::printf("  FAIL %s:%d: %s\n", __FILE__, __LINE__, #cond);     \
+        }                                                                      \
+    } while (0)
+
+template <typename T>
+OwnedColumn mk(Type t, const std::vector<T>& v, std::vector<size_t> nulls = {}) {
+    OwnedColumn c = OwnedColumn::make(t, v.size());
+    for (size_t i = 0; i < v.size(); ++i) std::memcpy(c.mutable_data() + i * byte_width(t), &v[i], sizeof(T));
+    for (size_t i : nulls) c.set_null(i);
+    return c;
+}
+
+// Drain an operator into rows of optional doubles (integers widened exactly for
+// the small values used here).
+using Row = std::vector<std::optional<double>>;
+std::vector<Row> drain(Operator& op, size_t* batches = nullptr) {
+    std::vector<Row> rows;
+    size_t nb = 0;
+    op.open();
+    while (auto b = op.next()) {
+        ++nb;
+        if (b->row_count > 2048) ++g_fail;  // <= 2048-row output batches
+        for (size_t k = 0; k < b->row_count; ++k) {
+            Row r;
+            for (const Column& c : b->cols) {
+                const bool valid = c.all_valid || ((c.validity[k >> 6] >> (k & 63)) & 1ull);
+                if (!valid) {
+                    r.push_back(std::nullopt);
+                    continue;
+                }
+                double v = 0;
+                switch (c.type) {
+                    case Type::I32:
+                    case Type::STR:
+                        v = reinterpret_cast<const int32_t*>(c.data)[k];
+                        break;
+                    case Type::BOOL:
+                        v = reinterpret_cast<const uint8_t*>(c.data)[k];
+                        break;
+                    case Type::F64:
+                        v = reinterpret_cast<const double*>(c.data)[k];
+                        break;
+                    default:
+                        v = static_cast<double>(reinterpret_cast<const int64_t*>(c.data)[k]);
+                }
+                r.push_back(v);
+            }
+            rows.push_back(r);
+        }
+    }
+    op.close();
+    if (batches) *batches = nb;
+    std::sort(rows.begin(), rows.end());
+    return rows;
+}
+
+Schema sch(std::vector<std::pair<std::string, Type>> f) { return Schema{std::move(f)}; }
+
+void test_join_groupby_plan() {
+    // build: k {1,2,3,4} -> g {10,10,20,20}; probe: k {1,1,2,3,5} pay {1,2,3,4,5}
+    Table build(sch({{"k", Type::I64}, {"g", Type::I32}}),
+                [] {
+                    std::vector<OwnedColumn> c;
+                    c.push_back(mk<int64_t>(Type::I64, {1, 2, 3, 4}));
+                    c.push_back(mk<int32_t>(Type::I32, {10, 10, 20, 20}));
+                    return c;
+                }());
+    Table probe(sch({{"k", Type::I64}, {"pay", Type::F64}}),
+                [] {
+                    std::vector<OwnedColumn> c;
+                    c.push_back(mk<int64_t>(Type::I64, {1, 1, 2, 3, 5}));
+                    c.push_back(mk<double>(Type::F64, {1.0, 2.0, 3.0, 4.0, 5.0}));
+                    return c;
+                }());
+    for (bool fuse : {true, false}) {
+        plan::LowerOptions opt;
+        opt.fuse_join_aggregate = fuse;
+        auto op = plan::scan(probe)
+                      .join(plan::scan(build), {"k"}, {"k"}, JoinType::Inner)
+                      .aggregate({"g"}, {AggSpec::sum(1, "s"), AggSpec::count_star("n")})
+                      .build(opt);
+        const auto rows = drain(*op);
+        // g=10: pays 1,2,3 -> 6, 3 rows; g=20: pay 4 -> 4, 1 row.
+        CHECK(rows.size() == 2);
+        CHECK(rows.size() == 2 && rows[0] == (Row{10.0, 6.0, 3.0}) && rows[1] == (Row{20.0, 4.0, 1.0}));
+    }
+}
+
+void test_join_fanout_crosses_output_batches() {
+    std::vector<int32_t> bk(3000, 7);
+    std::vector<int64_t> bv(3000);
+    for (int i = 0; i < 3000; ++i) bv[i] = i;
+    Table build(sch({{"c0", Type::I32}, {"c1", Type::I64}}), [&] {
+        std::vector<OwnedColumn> c;
+        c.push_back(mk<int32_t>(Type::I32, bk));
+        c.push_back(mk<int64_t>(Type::I64, bv));
+        return c;
+    }());
+    Table probe(sch({{"c0", Type::I32}, {"c1", Type::I64}}), [] {
+        std::vector<OwnedColumn> c;
+        c.push_back(mk<int32_t>(Type::I32, {7, 9, 7}));
+        c.push_back(mk<int64_t>(Type::I64, {100, 200, 300}));
+        return c;
+    }());
+    for (JoinType jt : {JoinType::Inner, JoinType::Left}) {
+        GpuHashJoin j(std::make_unique<Scan>(probe), std::make_unique<Scan>(build), {0}, {0}, jt);
+        size_t nb = 0;
+        const auto rows = drain(j, &nb);
+        CHECK(rows.size() == (jt == JoinType::Inner ? 6000u : 6001u));
+        CHECK(nb >= 3);
+        CHECK(j.output_schema().fields.size() == 4);
+        if (jt == JoinType::Left) {
+            // The unmatched probe row (9) carries NULL build columns.
+            const auto it = std::find_if(rows.begin(), rows.end(), [](const Row& r) { return r[0] == 9.0; });
+            CHECK(it != rows.end() && !(*it)[2].has_value() && !(*it)[3].has_value());
+        }
+    }
+}
+
+void test_aggregate_edges() {
+    Table t(sch({{"c0", Type::I32}, {"c1", Type::I32}}), [] {
+        std::vector<OwnedColumn> c;
+        c.push_back(mk<int32_t>(Type::I32, {1, 1, 2, 2}));
+        c.push_back(mk<int32_t>(Type::I32, {0, 0, 5, 6}, {0, 1}));
+        return c;
+    }());
+    GpuAggregate a(std::make_unique<Scan>(t), {0},
+                   {AggSpec::count_star("cs"), AggSpec::count(1, "c"), AggSpec::sum(1, "s"), AggSpec::avg(1, "a")});
+    const auto rows = drain(a);
+    CHECK(rows.size() == 2);
+    CHECK(rows.size() == 2 && rows[0] == (Row{1.0, 2.0, 0.0, std::nullopt, std::nullopt}));
+    CHECK(rows.size() == 2 && rows[1] == (Row{2.0, 2.0, 2.0, 11.0, 5.5}));
+    Table e(sch({{"c0", Type::I32}}), [] {
+        std::vector<OwnedColumn> c;
+        c.push_back(mk<int32_t>(Type::I32, {}));
+        return c;
+    }());
+    GpuAggregate g(std::make_unique<Scan>(e), {}, {AggSpec::count_star("cs"), AggSpec::sum(0, "s")});
+    const auto gr = drain(g);
+    CHECK(gr.size() == 1 && gr[0] == (Row{0.0, std::nullopt}));
+    GpuAggregate k(std::make_unique<Scan>(e), {0}, {AggSpec::count_star("cs")});
+    CHECK(drain(k).empty());
+}
+
+void test_asof_plan() {
+    Table build(sch({{"c0", Type::I32}, {"c1", Type::TS}, {"c2", Type::I64}}), [] {
+        std::vector<OwnedColumn> c;
+        c.push_back(mk<int32_t>(Type::I32, {1, 1, 1, 2}));
+        c.push_back(mk<int64_t>(Type::TS, {10, 20, 30, 15}));
+        c.push_back(mk<int64_t>(Type::I64, {100, 200, 300, 150}));
+        return c;
+    }());
+    Table probe(sch({{"c0", Type::I32}, {"c1", Type::TS}}), [] {
+        std::vector<OwnedColumn> c;
+        c.push_back(mk<int32_t>(Type::I32, {1, 1, 1, 2, 3, 1}));
+        c.push_back(mk<int64_t>(Type::TS, {5, 10, 25, 15, 99, 50}));
+        return c;
+    }());
+    for (auto method : {tsx::AsofMethod::Binary, tsx::AsofMethod::Learned}) {
+        plan::LowerOptions opt;
+        opt.asof_method = method;
+        auto op = plan::scan(probe)
+                      .asof_join(plan::scan(build), {0}, {0}, 1, 1, tsx::AsofType::Inner)
+                      .build(opt);
+        const auto rows = drain(*op);
+        CHECK(rows.size() == 4);
+        // (1,10)->100, (1,25)->200, (2,15)->150, (1,50)->300
+        std::vector<double> pay;
+        for (const auto& r : rows) pay.push_back(*r[4]);
+        std::sort(pay.begin(), pay.end());
+        CHECK((pay == std::vector<double>{100, 150, 200, 300}));
+    }
+}
+
+void test_exceptions() {
+    Table a(sch({{"k", Type::I32}}), [] {
+        std::vector<OwnedColumn> c;
+        c.push_back(mk<int32_t>(Type::I32, {1}));
+        return c;
+    }());
+    Table b(sch({{"k", Type::I64}, {"f", Type::F64}}), [] {
+        std::vector<OwnedColumn> c;
+        c.push_back(mk<int64_t>(Type::I64, {1}));
+        c.push_back(mk<double>(Type::F64, {1.0}));
+        return c;
+    }());
+    bool threw = false;
+    try {
+        GpuHashJoin j(std::make_unique<Scan>(a), std::make_unique<Scan>(b), {0}, {0}, JoinType::Inner);
+    } catch (const std::invalid_argument&) {
+        threw = true;
+    }
+    CHECK(threw);
+    threw = false;
+    try {
+        Scan s(a, 100);
+    } catch (const std::invalid_argument&) {
+        threw = true;
+    }
+    CHECK(threw);
+    threw = false;
+    try {
+        plan::scan(b).asof_join(plan::scan(b), {}, {}, "f", "f", tsx::AsofType::Inner);
+    } catch (const std::invalid_argument&) {
+        threw = true;
+    }
+    CHECK(threw);
+    threw = false;
+    try {
+        plan::scan(a).join(plan::scan(b), {"k"}, {"k"}, JoinType::Inner);
+    } catch (const std::invalid_argument&) {
+        threw = true;
+    }
+    CHECK(threw);
+}
+
+void test_hash_table_api() {
+    GpuHashTable t({Type::I32});
+    OwnedColumn c = mk<int32_t>(Type::I32, {7, 7, 3, 7, 3, 100});
+    Column v = c.view();
+    KeyColumns kc{&v, 1, nullptr};
+    uint32_t g[6];
+    t.insert_or_find(kc, 6, g);
+    CHECK(g[0] == 0 && g[1] == 0 && g[2] == 1 && g[3] == 0 && g[4] == 1 && g[5] == 2);
+    CHECK(t.num_groups() == 3);
+    OwnedColumn p = mk<int32_t>(Type::I32, {3, 99});
+    Column pv = p.view();
+    KeyColumns pk{&pv, 1, nullptr};
+    uint32_t f[2];
+    t.find(pk, 2, f);
+    CHECK(f[0] == 1 && f[1] == kNoGroup);
+    CHECK(static_cast<int32_t>(t.group_key_word(2, 0)) == 100);
+    GpuHashTable n({Type::I32}, NullPolicy::kNeverMatch);
+    OwnedColumn nc = mk<int32_t>(Type::I32, {0, 5, 0, 5}, {0, 2});
+    Column nv = nc.view();
+    KeyColumns nk{&nv, 1, nullptr};
+    uint32_t ng[4];
+    n.insert_or_find(nk, 4, ng);
+    CHECK(ng[0] == 0 && ng[1] == 1 && ng[2] == 2 && ng[3] == 1);
+}
+
+int main() {
+    const std::vector<std::pair<const char*, std::function<void()>>> tests = {
+        {"join_groupby_plan", test_join_groupby_plan},
+        {"join_fanout_crosses_output_batches", test_join_fanout_crosses_output_batches},
+        {"aggregate_edges", test_aggregate_edges},
+        {"asof_plan", test_asof_plan},
+        {"exceptions", test_exceptions},
+        {"hash_table_api", test_hash_table_api},
+    };
+    for (const auto& [name, fn] : tests) {
+        std::printf("[host] %s\n", name);
+        std::fflush(stdout);
+        try {
+            fn();
+        } catch (const std::exception& e) {
+            ++g_fail;
+            std::printf("  FAIL exception: %s\n", e.what());
+        }
+    }
+    std::printf("[host] %d checks passed, %d failed\n", g_pass, g_fail);
+    return g_fail == 0 ? 0 : 1;
+}

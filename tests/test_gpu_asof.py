@@ -1,0 +1,191 @@
+"""GPU as-of join: binary search and the learned index against the oracle.
+
+Literal edge shapes follow tests/asof_differential_test.cpp:118-318 and
+tests/catalog_checks_asof.cpp:38-92; match indices are bit-exact; the learned
+probe must return exactly the binary-search matches (the judge) with and
+without the MLP router; the device MLP forward pass and MFMA gradient kernel
+are checked against the numpy oracle.
+"""
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import _oracle as O
+
+sys.path.insert(0, str(Path(__file__).resolve().parent.parent / "oracle"))
+import mlp_oracle as M  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+I32, I64, TS = O.I32, O.I64, O.TS
+NO = O.NO_ROW
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def col(vals, t, nulls=()):
+    vals = np.asarray(vals, dtype=O.NP[t])
+    valid = None
+    if nulls:
+        valid = np.ones(len(vals), dtype=bool)
+        valid[list(nulls)] = False
+    return (vals, valid, t)
+
+
+def oracle_matches(pcols, bcols, pk, bk, pt, bt, tol=None):
+    p, b = O.asof_pairs(pcols, bcols, pk, bk, pt, bt, 1, tolerance=tol)
+    out = np.full(len(pcols[pt][0]), NO, dtype=np.uint32)
+    out[p] = b
+    return out
+
+
+def gpu_matches(ctx, qe, pcols, bcols, pk, bk, pt, bt, tol=None, modes=(0,), train=None):
+    P = [ctx.upload(v, t, valid) for v, valid, t in pcols]
+    B = [ctx.upload(v, t, valid) for v, valid, t in bcols]
+    try:
+        idx = ctx.asof_build([B[i] for i in bk], B[bt])
+        if train is not None:
+            idx.train(*train)
+        res = [idx.probe([P[i] for i in pk], P[pt], tolerance=tol, mode=m) for m in modes]
+        stats = idx.stats()
+        idx.free()
+        return res, stats
+    finally:
+        for d in P + B:
+            d.free()
+
+
+def check_all(ctx, qe, pcols, bcols, pk, bk, pt, bt, tol=None, piece_rows=8):
+    want = oracle_matches(pcols, bcols, pk, bk, pt, bt, tol)
+    res, st = gpu_matches(ctx, qe, pcols, bcols, pk, bk, pt, bt, tol, modes=(0, 1, 2), train=(piece_rows, 50, 0.01))
+    for m, got in zip((0, 1, 2), res):
+        assert got.tolist() == want.tolist(), f"mode {m}"
+    return want, st
+
+
+def test_literal_gap_tie_nearest(ctx, qe):
+    b = [col([1, 1, 1, 2], I32), col([10, 20, 30, 15], TS), col([100, 200, 300, 150], I64)]
+    p = [col([1, 1, 1, 2, 3, 1], I32), col([5, 10, 25, 15, 99, 50], TS)]
+    want, _ = check_all(ctx, qe, p, b, [0], [0], 1, 1, piece_rows=2)
+    assert want.tolist() == [NO, 0, 1, 3, NO, 2]
+    w2, _ = check_all(ctx, qe, p, b, [0], [0], 1, 1, tol=10, piece_rows=2)
+    assert w2.tolist() == [NO, 0, 1, 3, NO, NO]
+
+
+def test_literal_zero_keys_nulls_composite(ctx, qe):
+    w, _ = check_all(ctx, qe, [col([5, 25, 35], TS)], [col([10, 20, 30], TS)], [], [], 0, 0, piece_rows=2)
+    assert w.tolist() == [NO, 1, 2]
+    nb = [col([1, 1, 2, 7], I32, nulls=(3,)), col([10, 20, 0, 5], TS, nulls=(2,))]
+    npb = [col([1, 2, 2, 9], I32, nulls=(3,)), col([25, 30, 0, 50], TS, nulls=(2,))]
+    w, _ = check_all(ctx, qe, npb, nb, [0], [0], 1, 1, piece_rows=2)
+    assert w.tolist() == [1, NO, NO, NO]
+    cb = [col([1, 1, 1], I32), col([7, 7, 9], I32), col([10, 20, 10], TS)]
+    cp = [col([1, 1], I32), col([7, 9], I32), col([25, 25], TS)]
+    w, _ = check_all(ctx, qe, cp, cb, [0, 1], [0, 1], 2, 2, piece_rows=2)
+    assert w.tolist() == [1, 2]
+
+
+def test_catalog_boundary_ties(ctx, qe):
+    kb = [col([1, 1, 2], I32), col([10, 20, 15], I64)]
+    kp = [col([1, 1, 2], I32), col([20, 25, 15], I64)]
+    w, _ = check_all(ctx, qe, kp, kb, [0], [0], 1, 1, piece_rows=2)
+    assert w.tolist() == [1, 1, 2]
+
+
+def test_empty_sides(ctx, qe):
+    w, _ = check_all(ctx, qe, [col([1, 2], I32), col([5, 6], TS)], [col([], I32), col([], TS)], [0], [0], 1, 1)
+    assert w.tolist() == [NO, NO]
+    w, _ = check_all(ctx, qe, [col([], I32), col([], TS)], [col([1], I32), col([1], TS)], [0], [0], 1, 1)
+    assert w.tolist() == []
+
+
+@pytest.mark.parametrize("case", range(10))
+def test_random_unsorted_inputs(ctx, qe, case):
+    r = np.random.default_rng(4000 + case)
+    nb, np_ = int(r.choice([1, 100, 5000, 60000])), int(r.choice([1, 100, 5000, 30000]))
+    nsym = int(r.choice([1, 7, 300]))
+    bsym = r.integers(0, nsym, nb)
+    bts = r.permutation(nb * 5)[:nb] * 3 - 1000  # distinct, unsorted, some negative
+    psym = r.integers(0, nsym + 2, np_)
+    pts = r.integers(-2000, nb * 15 + 10, np_)
+    bn = tuple(np.nonzero(r.random(nb) < 0.02)[0])
+    pn = tuple(np.nonzero(r.random(np_) < 0.02)[0])
+    tcol = int(r.choice([TS, I64]))
+    bcols = [col(bsym, I32, bn), col(bts, tcol)]
+    pcols = [col(psym, I32), col(pts, tcol, pn)]
+    tol = None if case % 3 else int(r.integers(0, 40))
+    check_all(ctx, qe, pcols, bcols, [0], [0], 1, 1, tol=tol, piece_rows=int(r.choice([4, 16, 64])))
+
+
+def test_hash_partitioned_keys(ctx, qe):
+    r = np.random.default_rng(77)
+    nb, np_ = 20000, 20000
+    keys = r.integers(-2**60, 2**60, 50)
+    bcols = [col(keys[r.integers(0, 50, nb)], I64), col(r.permutation(nb) * 2, TS)]
+    pcols = [col(keys[r.integers(0, 50, np_)], I64), col(r.integers(0, 2 * nb, np_), TS)]
+    check_all(ctx, qe, pcols, bcols, [0], [0], 1, 1, piece_rows=32)
+
+
+def test_golden_fixture(ctx, qe):
+    fx = json.loads((GOLDEN / "asof_left_3000.json").read_text())
+    mk = lambda spec: [col(c["values"], c["type"], tuple(c.get("nulls", []))) for c in spec]
+    res, _ = gpu_matches(ctx, qe, mk(fx["probe"]), mk(fx["build"]), fx["pkeys"], fx["bkeys"], fx["ptime"],
+                         fx["btime"], modes=(0, 1), train=(16, 50, 0.01))
+    want = np.full(len(fx["probe"][0]["values"]), NO, dtype=np.uint32)
+    for p, b in fx["expected_pairs"]:
+        want[p] = b
+    for got in res:
+        assert got.tolist() == want.tolist()
+
+
+def test_generated_tick_shape_learned_equals_binary(ctx, qe):
+    # Scaled config shape: Zipf(1.1) symbols, bursty exponential gaps.
+    (rs, rt, rp), (ls, lt) = ctx.gen_asof(99, 1 << 22, 1 << 20, 500, 1.1)
+    try:
+        idx = ctx.asof_build([rs], rt)
+        idx.train(64, 150, 0.01)
+        b = idx.probe([ls], lt, mode=qe.ASOF_BINARY)
+        l1 = idx.probe([ls], lt, mode=qe.ASOF_LEARNED)
+        st1 = idx.stats()
+        l2 = idx.probe([ls], lt, mode=qe.ASOF_LEARNED_NO_MLP)
+        idx.free()
+        rts, _ = rt.numpy()
+        rsy, _ = rs.numpy()
+        lsy, _ = ls.numpy()
+        lts, _ = lt.numpy()
+    finally:
+        for c in (rs, rt, rp, ls, lt):
+            c.free()
+    assert (b == l1).all() and (b == l2).all()
+    assert st1["fallbacks"] == 0
+    # Spot check a sample against numpy searchsorted (per-symbol sorted runs).
+    order = np.lexsort((rts, rsy))
+    for j in np.random.default_rng(5).choice(len(lts), 2000, replace=False):
+        seg = order[rsy[order] == lsy[j]]
+        k = np.searchsorted(rts[seg], lts[j], side="right")
+        assert b[j] == (NO if k == 0 else seg[k - 1])
+    # The router gains over the same structure without the MLP.
+    assert st1["mean_bucket_knots"] < st1["mean_bucket_knots_linear"]
+
+
+def test_mlp_forward_matches_cpu(ctx):
+    r = np.random.default_rng(8)
+    p = M.identity_params() + np.abs(r.normal(scale=0.1, size=M.N_PARAMS))
+    x = np.linspace(0, 1, 4097).astype(np.float32)
+    y = ctx.mlp_forward(p.astype(np.float32), x)
+    assert np.allclose(y, M.forward(p.astype(np.float32).astype(np.float64), x), rtol=1e-5, atol=1e-5)
+    assert (np.diff(y) >= 0).all()  # monotone in x, bit for bit on the device
+
+
+@pytest.mark.parametrize("n", [1, 15, 16, 17, 300, 2048])
+def test_mlp_gradient_matches_cpu(ctx, n):
+    r = np.random.default_rng(n)
+    p = (M.identity_params() + r.normal(scale=0.05, size=M.N_PARAMS)).astype(np.float32)
+    x = r.uniform(0, 1, n).astype(np.float32)
+    c = np.sqrt(x).astype(np.float32)
+    g, loss = ctx.mlp_grad(p, x, c)
+    wl, wg = M.loss_and_grad(p.astype(np.float64), x.astype(np.float64), c.astype(np.float64))
+    assert abs(loss - wl) <= 1e-5 * max(1.0, wl)
+    scale = np.abs(wg).max() + 1e-12
+    assert np.abs(g - wg).max() <= 2e-4 * scale, np.argmax(np.abs(g - wg))

@@ -1,0 +1,89 @@
+"""Batched group-by (qe_gpu_batch_*): independent queries over uneven morsels
+through the persistent kernel, the one-launch-per-2048-row-batch baseline and
+the hipGraph-captured pipeline, each checked against the oracle per query;
+graph replays are checked against the eager result every time, with eager
+work in between."""
+import numpy as np
+import pytest
+
+import _oracle as O
+from _compare import assert_same, table_rows
+
+pytestmark = pytest.mark.gpu
+I32, I64, F64 = O.I32, O.I64, O.F64
+CS, CNT, SUM, MIN, MAX, AVG = O.COUNT_STAR, O.COUNT, O.SUM, O.MIN, O.MAX, O.AVG
+
+
+def make_batch(ctx, qe, seed, nq=6, morsels=8, max_rows=40000):
+    r = np.random.default_rng(seed)
+    queries, hostcols = [], []
+    for q in range(nq):
+        sizes = np.exp(r.uniform(np.log(100), np.log(max_rows), morsels)).astype(np.int64)
+        offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+        n = int(offs[-1])
+        groups = int(np.exp(r.uniform(np.log(4), np.log(max(5, n // 2)))))
+        ktype = I32 if q % 2 == 0 else I64
+        vtype = F64 if q % 3 != 2 else I64
+        k = ctx.gen_uniform(seed, 2 * q, ktype, n, 0, groups, 0.0)
+        v = ctx.gen_uniform(seed, 2 * q + 1, vtype, n, -1000, 1000, 0.02)
+        aggs = [(CS, 0), (SUM, 1), (AVG, 1), (CNT, 1), (MIN, 1), (MAX, 1)]
+        queries.append((k, v, offs, aggs, groups))
+        kv, _ = k.numpy()
+        vv, vvalid = v.numpy()
+        hostcols.append(([(kv, None, ktype), (vv, vvalid, vtype)], aggs))
+    return queries, hostcols
+
+
+def check_outputs(tables, hostcols):
+    for t, (cols, aggs) in zip(tables, hostcols):
+        got = table_rows(t.numpy())
+        t.free()
+        assert_same(got, O.aggregate(cols, [0], aggs))
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
+def test_modes_match_oracle(ctx, qe, mode):
+    queries, hostcols = make_batch(ctx, qe, 10 + mode)
+    plan = ctx.batch(queries)
+    try:
+        check_outputs(plan.run(mode, tuning=(4096, 2, 256)), hostcols)
+        check_outputs(plan.run(mode, tuning=(1024, 1, 0)), hostcols)
+    finally:
+        plan.free()
+        for k, v, *_ in queries:
+            k.free()
+            v.free()
+
+
+def test_graph_replays_match_eager(ctx, qe):
+    queries, hostcols = make_batch(ctx, qe, 21, nq=4)
+    plan = ctx.batch(queries)
+    try:
+        eager = [table_rows(t.numpy()) for t in plan.run(qe.BATCH_PERSISTENT, tuning=(4096, 2, 256))]
+        for rep in range(3):
+            got = plan.run(qe.BATCH_GRAPH, tuning=(4096, 2, 256))
+            for t, e in zip(got, eager):
+                assert_same(table_rows(t.numpy()), e)
+                t.free()
+            # eager work and a synchronisation between replays
+            plan.run(qe.BATCH_PERSISTENT, tuning=(2048, 1, 0), collect=False)
+            ctx.sync()
+    finally:
+        plan.free()
+        for k, v, *_ in queries:
+            k.free()
+            v.free()
+
+
+def test_autotune_returns_a_valid_shape(ctx, qe):
+    queries, hostcols = make_batch(ctx, qe, 31, nq=3, morsels=4, max_rows=20000)
+    plan = ctx.batch(queries)
+    try:
+        tile, bpc, C = plan.autotune()
+        assert tile >= 256 and bpc >= 1 and (C == 0 or C & (C - 1) == 0)
+        check_outputs(plan.run(qe.BATCH_PERSISTENT, tuning=(tile, bpc, C)), hostcols)
+    finally:
+        plan.free()
+        for k, v, *_ in queries:
+            k.free()
+            v.free()
