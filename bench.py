@@ -281,6 +281,20 @@ def batched_workload(qe, ctx, args):
     return res
 
 
+def copy_ceiling(qe, ctx, nbytes=1 << 30, reps=5):
+    """Measured HBM ceiling: a plain 16-byte streaming copy (read + write)."""
+    import ctypes as C
+    src, dst = qe.DevBuffer(ctx, nbytes), qe.DevBuffer(ctx, nbytes)
+    ms = C.c_double()
+    best = 1e30
+    for _ in range(reps + 1):
+        ctx._check(qe.lib().qe_gpu_stream_copy(ctx.h, dst.p, src.p, nbytes, C.byref(ms)))
+        best = min(best, ms.value)
+    src.free()
+    dst.free()
+    return {"bytes_moved": 2 * nbytes, "best_ms": best, "gbs": 2 * nbytes / (best / 1e3) / 1e9}
+
+
 def groupby_1m(qe, ctx, args):
     n = 1 << 20
     k, v = ctx.gen_groupby(20260615, n, 1024, 0.05)
@@ -327,6 +341,7 @@ def main():
     ap.add_argument("--only", choices=["join", "all"], default="all",
                     help="join: headline workload only")
     ap.add_argument("--quick", action="store_true", help="small sizes (smoke)")
+    ap.add_argument("--calibrate-copy", action="store_true", help="only run one 1 GiB stream copy")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X: this build has no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -338,6 +353,12 @@ def main():
         args.batch_queries, args.batch_morsels, args.batch_max_log2 = 8, 64, 16
     qe = load_pkg()
     ctx = qe.Context(0)
+    if args.calibrate_copy:
+        # PMC calibration run: one copy of known size (1 GiB read + 1 GiB write).
+        print(json.dumps({"copy": copy_ceiling(qe, ctx, reps=0)}), flush=True)
+        return
+    ceiling = copy_ceiling(qe, ctx)
+    log(f"stream copy ceiling {ceiling['gbs']:.0f} GB/s")
     res, cols = join_groupby_workload(qe, ctx, args)
     log(f"join+groupby: {res['ms_per_step']:.3f} ms/step, kernel {res['kernel_ms']:.3f} ms")
     cpu = None
@@ -384,6 +405,7 @@ def main():
             "frac": (kach / HBM_PEAK_GBS) if kach else None, "traffic": None,
             "kernel": res["kernel"], "kernel_ms": res["kernel_ms"],
             "algorithmic_bytes_per_launch": res["probe_bytes"],
+            "measured_copy_ceiling_gbs": ceiling["gbs"],
         },
         "step_roofline": {"achieved": res["step_achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": res["step_achieved_gbs"] / HBM_PEAK_GBS,

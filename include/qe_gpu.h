@@ -187,6 +187,14 @@ uint64_t qe_gpu_ht_capacity(const qe_dev_ht* ht);
 qe_gpu_status qe_gpu_ht_key_words(qe_gpu_ctx* ctx, const qe_dev_ht* ht, uint32_t col, uint64_t* words, uint8_t* nulls);
 void qe_gpu_ht_free(qe_gpu_ctx* ctx, qe_dev_ht* ht);
 
+/* ---- sort ---------------------------------------------------------------------------*/
+/* Stable sort permutation over integer-family key columns (I32/I64/TS/BOOL),
+ * keys[0] most significant, per-key desc / nulls_first flags (either array may
+ * be NULL = ASC / NULLS LAST). Replaces Sort's radix path (ops/sort.cpp:125-188,
+ * Sort ctor ops/sort.h:84). perm_out: device u32[n]. */
+qe_gpu_status qe_gpu_sort_perm(qe_gpu_ctx* ctx, const qe_dev_col* const* keys, uint32_t nkeys, const int32_t* desc,
+                               const int32_t* nulls_first, uint32_t* perm_out);
+
 /* ---- as-of join -------------------------------------------------------------------*/
 /* Build side of tsx::AsofJoin (tsx/asof.h:101-105; build_side
  * tsx/asof.cpp:177-232): partition by key tuple (NULL keys never match),
@@ -288,6 +296,11 @@ qe_gpu_status qe_gpu_malloc(qe_gpu_ctx* ctx, uint64_t bytes, void** out);
 qe_gpu_status qe_gpu_free(qe_gpu_ctx* ctx, void* p);
 qe_gpu_status qe_gpu_memcpy_d2h(qe_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes);
 qe_gpu_status qe_gpu_memcpy_h2d(qe_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes);
+/* Plain streaming copy kernel (16-byte loads/stores, bytes % 16 == 0): the
+ * measured HBM ceiling and the calibration for PMC byte counters. Returns the
+ * kernel time in ms (HIP events on the context stream). No counterpart in the
+ * reference (measurement support). */
+qe_gpu_status qe_gpu_stream_copy(qe_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes, double* ms);
 
 /* ---- synthetic workload generators (bench / tests; seeded, counter-based) ---------*/
 /* Config shapes from the benchmark definitions; every value is a pure

@@ -115,6 +115,7 @@ _SIGS = {
     "qe_gpu_ht_capacity": (C.c_uint64, [P]),
     "qe_gpu_ht_key_words": (C.c_int, [P, P, C.c_uint32, P, P]),
     "qe_gpu_ht_free": (None, [P, P]),
+    "qe_gpu_sort_perm": (C.c_int, [P, C.POINTER(P), C.c_uint32, P, P, P]),
     "qe_gpu_asof_build": (C.c_int, [P, C.POINTER(P), C.c_uint32, P, C.POINTER(P)]),
     "qe_gpu_asof_train": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.c_float]),
     "qe_gpu_asof_probe": (C.c_int, [P, P, C.POINTER(P), C.c_uint32, P, C.c_int32, C.c_int64, C.c_int32, P]),
@@ -133,6 +134,7 @@ _SIGS = {
     "qe_gpu_free": (C.c_int, [P, P]),
     "qe_gpu_memcpy_d2h": (C.c_int, [P, P, P, C.c_uint64]),
     "qe_gpu_memcpy_h2d": (C.c_int, [P, P, P, C.c_uint64]),
+    "qe_gpu_stream_copy": (C.c_int, [P, P, P, C.c_uint64, C.POINTER(C.c_double)]),
     "qe_gpu_gen_groupby": (C.c_int, [P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, P, P]),
     "qe_gpu_gen_join_build": (C.c_int, [P, C.c_uint64, C.c_uint64, C.c_uint32, P, P]),
     "qe_gpu_gen_join_probe": (C.c_int, [P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, C.c_double, P, P]),
@@ -378,6 +380,17 @@ class Context:
             _u32(build_keys), len(probe_keys), join_type, _u32(group_keys), len(group_keys), _aggs(aggs),
             len(aggs), C.byref(out)))
         return Table(self, out)
+
+    def sort_perm(self, keys, desc=None, nulls_first=None) -> np.ndarray:
+        n = keys[0].len
+        out = DevBuffer(self, 4 * max(n, 1))
+        d = None if desc is None else (C.c_int32 * len(keys))(*desc)
+        nf = None if nulls_first is None else (C.c_int32 * len(keys))(*nulls_first)
+        try:
+            self._check(lib().qe_gpu_sort_perm(self.h, _cols(keys), len(keys), d, nf, out.p))
+            return out.numpy(np.uint32, n)
+        finally:
+            out.free()
 
     # ---- hash table --------------------------------------------------------
     def hash_table(self, key_types, null_policy=0, initial_capacity=1024, max_load_factor=0.7,

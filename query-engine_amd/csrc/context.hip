@@ -570,6 +570,35 @@ qe_gpu_status qe_gpu_memcpy_d2h(qe_gpu_ctx* ctx, void* dst, const void* src, uin
         QE_HIP(hipStreamSynchronize(ctx->stream));
     });
 }
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(256) stream_copy_kernel(const u32x4* __restrict__ src, u32x4* __restrict__ dst,
+                                                           uint64_t n) {
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n;
+         i += uint64_t(gridDim.x) * blockDim.x)
+        dst[i] = __builtin_nontemporal_load(src + i);
+}
+
+qe_gpu_status qe_gpu_stream_copy(qe_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes, double* ms) {
+    QE_ENTRY(ctx, {
+        invalid(dst && src && bytes % 16 == 0, "stream copy needs 16-byte multiples");
+        hipEvent_t a, b;
+        QE_HIP(hipEventCreate(&a));
+        QE_HIP(hipEventCreate(&b));
+        const uint64_t n = bytes / 16;
+        QE_HIP(hipEventRecord(a, ctx->stream));
+        stream_copy_kernel<<<grid_for(n, 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(
+            static_cast<const u32x4*>(src), static_cast<u32x4*>(dst), n);
+        QE_HIP(hipGetLastError());
+        QE_HIP(hipEventRecord(b, ctx->stream));
+        QE_HIP(hipEventSynchronize(b));
+        float t = 0;
+        QE_HIP(hipEventElapsedTime(&t, a, b));
+        hipEventDestroy(a);
+        hipEventDestroy(b);
+        if (ms) *ms = t;
+    });
+}
+
 qe_gpu_status qe_gpu_memcpy_h2d(qe_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
     QE_ENTRY(ctx, {
         if (bytes) QE_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));

@@ -165,4 +165,88 @@ void radix_sort_pairs(qe_gpu_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t 
     QE_HIP(hipStreamSynchronize(ctx->stream));
 }
 
+// ---- general stable sort permutation (Sort's radix path) ------------------------------
+__global__ void iota_kernel(uint32_t* p, uint64_t n) {
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n;
+         i += uint64_t(gridDim.x) * blockDim.x)
+        p[i] = static_cast<uint32_t>(i);
+}
+
+// Order-preserving unsigned image of key column c at row perm[i] (sign bit
+// flipped; complemented for DESC), minus `base` so fewer digits suffice. NULL
+// rows get 0 here and are placed by a separate flag pass.
+__global__ void encode_key_kernel(ColView c, const uint32_t* perm, uint64_t n, int desc, uint64_t base,
+                                  uint64_t* out) {
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n;
+         i += uint64_t(gridDim.x) * blockDim.x) {
+        const uint32_t r = perm[i];
+        uint64_t u = 0;
+        if (is_valid(c, r)) {
+            const int64_t v = int_value(c, r);
+            u = static_cast<uint64_t>(v) ^ 0x8000000000000000ull;
+            if (desc) u = ~u;
+            u -= base;
+        }
+        out[i] = u;
+    }
+}
+__global__ void null_flag_kernel(ColView c, const uint32_t* perm, uint64_t n, int nulls_first, uint64_t* out) {
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n;
+         i += uint64_t(gridDim.x) * blockDim.x) {
+        const bool isnull = !is_valid(c, perm[i]);
+        out[i] = (isnull == static_cast<bool>(nulls_first)) ? 0u : 1u;
+    }
+}
+
 }  // namespace qe_gpu
+
+// Stable sort permutation over integer-family keys (I32/I64/TS/BOOL), keys[0]
+// most significant, per-key ASC/DESC and NULLS FIRST/LAST (null placement is
+// absolute, ops/sort.h:35-40). Replaces Sort::build_perm_radix
+// (ops/sort.cpp:125-188): LSD over keys from the last to the first, each key
+// a value pass then a NULL-placement pass, all stable.
+extern "C" qe_gpu_status qe_gpu_sort_perm(qe_gpu_ctx* ctx, const qe_dev_col* const* keys, uint32_t nkeys,
+                                          const int32_t* desc, const int32_t* nulls_first, uint32_t* perm_out) {
+    QE_ENTRY(ctx, {
+        invalid(nkeys >= 1 && keys != nullptr && perm_out != nullptr, "sort needs >=1 key");
+        const uint64_t n = common_len(keys, nkeys);
+        for (uint32_t j = 0; j < nkeys; ++j) {
+            const int t = keys[j]->type;
+            check(t == QE_I32 || t == QE_I64 || t == QE_TS || t == QE_BOOL, QE_GPU_UNSUPPORTED,
+                  "device sort supports integer-family keys");
+        }
+        check(n < kNoRow, QE_GPU_UNSUPPORTED, "row indices are 32-bit");
+        if (n == 0) return QE_GPU_OK;
+        const unsigned g = grid_for(n, 256, ctx->num_cus * 16);
+        iota_kernel<<<g, 256, 0, ctx->stream>>>(perm_out, n);
+        Scratch k64(ctx, n * 8);
+        for (uint32_t jj = nkeys; jj-- > 0;) {
+            const qe_dev_col* c = keys[jj];
+            const MinMax mm = column_minmax(ctx, c, n);
+            const int d = desc ? desc[jj] : 0;
+            uint64_t lo = 0, hi = 0;
+            if (mm.valid) {
+                uint64_t a = static_cast<uint64_t>(mm.mn) ^ 0x8000000000000000ull;
+                uint64_t b = static_cast<uint64_t>(mm.mx) ^ 0x8000000000000000ull;
+                if (d) {
+                    a = ~a;
+                    b = ~b;
+                }
+                lo = std::min(a, b);
+                hi = std::max(a, b);
+            }
+            unsigned bits = 0;
+            while (bits < 64 && ((hi - lo) >> bits) != 0) ++bits;
+            encode_key_kernel<<<g, 256, 0, ctx->stream>>>(view(c), perm_out, n, d, lo, k64.as<uint64_t>());
+            QE_HIP(hipGetLastError());
+            radix_sort_pairs(ctx, k64.as<uint64_t>(), perm_out, n, 0, (bits + 7) / 8 * 8);
+            if (!c->all_valid && mm.valid < n) {
+                null_flag_kernel<<<g, 256, 0, ctx->stream>>>(view(c), perm_out, n, nulls_first ? nulls_first[jj] : 0,
+                                                             k64.as<uint64_t>());
+                QE_HIP(hipGetLastError());
+                radix_sort_pairs(ctx, k64.as<uint64_t>(), perm_out, n, 0, 8);
+            }
+        }
+        QE_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
