@@ -161,10 +161,12 @@ def test_generated_tick_shape_learned_equals_binary(ctx, qe):
     assert st1["fallbacks"] == 0
     # Spot check a sample against numpy searchsorted (per-symbol sorted runs).
     order = np.lexsort((rts, rsy))
+    ssym, sts = rsy[order], rts[order]
     for j in np.random.default_rng(5).choice(len(lts), 2000, replace=False):
-        seg = order[rsy[order] == lsy[j]]
-        k = np.searchsorted(rts[seg], lts[j], side="right")
-        assert b[j] == (NO if k == 0 else seg[k - 1])
+        lo = np.searchsorted(ssym, lsy[j], side="left")
+        hi = np.searchsorted(ssym, lsy[j], side="right")
+        k = np.searchsorted(sts[lo:hi], lts[j], side="right")
+        assert b[j] == (NO if k == 0 else order[lo + k - 1])
     # The router gains over the same structure without the MLP.
     assert st1["mean_bucket_knots"] < st1["mean_bucket_knots_linear"]
 
