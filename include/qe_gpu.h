@@ -266,7 +266,8 @@ typedef struct qe_batch_query {
     uint64_t group_bound;
 } qe_batch_query;
 /* Execution shape: rows per work tile, persistent blocks per CU, LDS slot
- * cache entries per block (0 or a power of two). */
+ * cache entries per block (0 or a power of two, at most 512: 80 bytes each
+ * within a 64 KiB block budget). */
 typedef struct qe_batch_tuning {
     uint32_t tile_rows;
     uint32_t blocks_per_cu;

@@ -442,7 +442,7 @@ qe_gpu_status qe_gpu_batch_autotune(qe_gpu_ctx* ctx, qe_batch_plan* plan, qe_bat
         qe_batch_tuning bt{8192, 4, 256};
         for (uint32_t tile : {4096u, 16384u, 65536u})
             for (uint32_t bpc : {2u, 4u, 8u})
-                for (uint32_t C : {0u, 256u, 1024u}) {
+                for (uint32_t C : {0u, 256u, 512u}) {
                     qe_batch_tuning tn{tile, bpc, C};
                     run_batch(ctx, plan->bp, QE_BATCH_PERSISTENT, tn);  // warm
                     QE_HIP(hipStreamSynchronize(ctx->stream));

@@ -87,27 +87,16 @@ static qe_dev_col* upload_view(const Column& c) {
     return out;
 }
 
-static std::map<const Table*, std::weak_ptr<DeviceColumns>>& table_cache() {
-    static std::map<const Table*, std::weak_ptr<DeviceColumns>> m;
-    return m;
-}
-static std::map<const Table*, std::shared_ptr<DeviceColumns>>& table_pins() {
-    static std::map<const Table*, std::shared_ptr<DeviceColumns>> m;
-    return m;
-}
-
 std::shared_ptr<DeviceColumns> upload_table(const Table& t) {
-    auto& pins = table_pins();
-    auto it = pins.find(&t);
-    if (it != pins.end()) return it->second;
+    if (t.device_) return t.device_;
     auto dc = std::make_shared<DeviceColumns>();
     dc->rows = t.num_rows();
     for (size_t i = 0; i < t.schema().fields.size(); ++i) dc->cols.push_back(upload_view(t.column(i).view()));
-    pins[&t] = dc;  // uploaded once per Table (inputs stay resident in HBM)
+    t.device_ = dc;  // uploaded once per Table (inputs stay resident in HBM)
     return dc;
 }
 
-void drop_device_copy(const Table& t) { table_pins().erase(&t); }
+void drop_device_copy(const Table& t) { t.device_.reset(); }
 
 // Drain any operator to host columns (honouring selection vectors), then upload.
 std::shared_ptr<DeviceColumns> device_input(Operator& op) {

@@ -106,17 +106,26 @@ class OwnedBatch {
 };
 
 // In-memory table (ops/table.h:24-45): owns equal-length columns.
+struct DeviceColumns;
+
 class Table {
    public:
     Table(Schema schema, std::vector<OwnedColumn> cols);
+    Table(const Table&) = delete;
+    Table& operator=(const Table&) = delete;
     const Schema& schema() const { return schema_; }
     size_t num_rows() const { return rows_; }
     const OwnedColumn& column(size_t i) const { return cols_[i]; }
 
    private:
+    friend std::shared_ptr<DeviceColumns> upload_table(const Table& t);
+    friend void drop_device_copy(const Table& t);
     Schema schema_;
     std::vector<OwnedColumn> cols_;
     size_t rows_ = 0;
+    // HBM copy of the columns, made on first device use and released with the
+    // Table (or by drop_device_copy); operators holding it keep it alive.
+    mutable std::shared_ptr<DeviceColumns> device_;
 };
 
 // ---- device plumbing -------------------------------------------------------------------
@@ -127,7 +136,7 @@ void throw_status(qe_gpu_status st, qe_gpu_ctx* ctx);
 qe_gpu_ctx* device_context();
 
 // Device-resident copies of host columns; a Table's columns are uploaded once
-// and cached by address (the Table must outlive the cache entry).
+// and kept with the Table.
 struct DeviceColumns {
     std::vector<qe_dev_col*> cols;
     size_t rows = 0;
