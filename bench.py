@@ -51,8 +51,9 @@ def join_groupby_workload(qe, ctx, args):
     probe_n = 1 << args.probe_log2
     groups = 10000
     log(f"generating join workload: build {build_n} probe {probe_n}")
-    (pk, pp), (bk, bg) = ctx.gen_join(20260615, build_n, probe_n, groups, 1.0, 0.1)
+    (pk, pp), (bk, bg) = ctx.gen_join(20260615, build_n, probe_n, groups, args.zipf, args.miss)
     aggs = [(qe.SUM, 1), (qe.COUNT_STAR, 0)]
+    ctx.set_tuning(qe.TUNE_JOIN_PAIRS, args.join_pairs)
 
     def step():
         t = ctx.join_aggregate([pk, pp], [bk, bg], [0], [0], qe.INNER, [3], aggs)
@@ -89,7 +90,7 @@ def join_groupby_workload(qe, ctx, args):
         "ms_per_step": ms_step,
         "ms_per_step_with_events": (t1 - t0) * 1e3 / args.steps,
         "probe_rows": probe_n, "build_rows": build_n, "groups_out": int(nrows),
-        "kernel": "join_agg_direct_lds (streaming probe)",
+        "kernel": "join_agg_stream (direct-map fused probe)",
         "kernel_ms": kernel_ms,
         "kernel_launches": launches,
         "probe_bytes": probe_bytes, "step_bytes": step_bytes,
@@ -150,12 +151,12 @@ def asof_workload(qe, ctx, args):
     (build phase, reported separately); probe phase timed per method."""
     rn, ln = args.asof_right, args.asof_left
     log(f"generating as-of workload: right {rn} left {ln}")
-    (rs, rt, rp), (ls, lt) = ctx.gen_asof(20260616, rn, ln, 5000, 1.1)
+    (rs, rt, rp), (ls, lt) = ctx.gen_asof(20260616, rn, ln, 5000, 1.1, sessions=args.asof_sessions)
     t0 = time.perf_counter()
     idx = ctx.asof_build([rs], rt)
     ctx.sync()
     t1 = time.perf_counter()
-    idx.train(args.piece_rows, args.train_steps, 0.01)
+    idx.train(args.piece_rows, args.train_steps, args.train_lr)
     ctx.sync()
     t2 = time.perf_counter()
     out = qe.DevBuffer(ctx, 4 * ln)
@@ -329,17 +330,24 @@ def main():
     ap.add_argument("--probe-log2", type=int, default=28)
     ap.add_argument("--cpu-probe-log2", type=int, default=24)
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--zipf", type=float, default=1.0, help="Zipf exponent of the probe keys")
+    ap.add_argument("--miss", type=float, default=0.1, help="fraction of probe keys outside the build domain")
+    ap.add_argument("--join-pairs", type=int, default=4, choices=[2, 4],
+                    help="row pairs per thread per tile in the streaming probe")
     ap.add_argument("--asof-right", type=int, default=400 << 20)
     ap.add_argument("--asof-left", type=int, default=100 << 20)
     ap.add_argument("--asof-reps", type=int, default=5)
     ap.add_argument("--piece-rows", type=int, default=64)
-    ap.add_argument("--train-steps", type=int, default=150)
+    ap.add_argument("--train-steps", type=int, default=300)
+    ap.add_argument("--train-lr", type=float, default=0.003)
+    ap.add_argument("--asof-sessions", type=int, default=0,
+                    help="0: bursty exponential gaps (the tick shape); >0: trading days with U-shaped activity")
     ap.add_argument("--cpu-asof-log2", type=int, default=22)
     ap.add_argument("--batch-queries", type=int, default=64)
     ap.add_argument("--batch-morsels", type=int, default=4096)
     ap.add_argument("--batch-max-log2", type=int, default=21)
-    ap.add_argument("--only", choices=["join", "all"], default="all",
-                    help="join: headline workload only")
+    ap.add_argument("--only", choices=["join", "asof", "batched", "all"], default="all",
+                    help="join: headline workload only; asof / batched: that secondary workload alone")
     ap.add_argument("--quick", action="store_true", help="small sizes (smoke)")
     ap.add_argument("--calibrate-copy", action="store_true", help="only run one 1 GiB stream copy")
     args = ap.parse_args()
@@ -356,6 +364,11 @@ def main():
     if args.calibrate_copy:
         # PMC calibration run: one copy of known size (1 GiB read + 1 GiB write).
         print(json.dumps({"copy": copy_ceiling(qe, ctx, reps=0)}), flush=True)
+        return
+    if args.only in ("asof", "batched"):
+        # One secondary workload on its own (diagnostics; not the headline line).
+        fn = asof_workload if args.only == "asof" else batched_workload
+        print(json.dumps({args.only: fn(qe, ctx, args)}), flush=True)
         return
     ceiling = copy_ceiling(qe, ctx)
     log(f"stream copy ceiling {ceiling['gbs']:.0f} GB/s")
@@ -395,8 +408,8 @@ def main():
         "data": "synthetic (seeded, generated in HBM)",
         "config": {
             "workload": "inner hash join + group-by: build 2^%d rows (unique I64 key permutation, I32 group in "
-                        "[0,10000)), probe 2^%d rows (Zipf(1.0) keys, 10%% misses, F64 payload), SUM/COUNT per "
-                        "group" % (args.build_log2, args.probe_log2),
+                        "[0,10000)), probe 2^%d rows (Zipf(%g) keys, %g%% misses, F64 payload), SUM/COUNT per "
+                        "group" % (args.build_log2, args.probe_log2, args.zipf, 100 * args.miss),
             "build_rows": res["build_rows"], "probe_rows": res["probe_rows"], "groups": 10000,
             "parallelism": "single GPU",
         },

@@ -93,6 +93,12 @@ qe_gpu_status qe_gpu_set_mem_limit(qe_gpu_ctx* ctx, uint64_t bytes);
  * (ms) and launch count per kernel name, then clears it. */
 qe_gpu_status qe_gpu_set_timing(qe_gpu_ctx* ctx, int enabled);
 qe_gpu_status qe_gpu_timing(qe_gpu_ctx* ctx, const char* kernel, double* ms, uint64_t* launches);
+/* Execution-shape knobs (results never depend on them). No counterpart in the
+ * reference (its kernels have no launch shape); used by the autotuner and the
+ * benchmark. QE_TUNE_JOIN_PAIRS: row pairs per thread per tile of the
+ * streaming join probe, 2 or 4 (default 4). */
+typedef enum qe_tune_key { QE_TUNE_JOIN_PAIRS = 0 } qe_tune_key;
+qe_gpu_status qe_gpu_set_tuning(qe_gpu_ctx* ctx, int32_t key, int64_t value);
 
 /* ---- columns ----------------------------------------------------------------*/
 /* Host -> HBM once, Batch-compatible layout (same widths, u64 validity words,
@@ -226,10 +232,12 @@ typedef struct qe_asof_stats {
     uint64_t pieces;           /* spline pieces                                  */
     uint64_t max_piece_error;  /* max recorded |predicted - actual| position     */
     uint64_t router_buckets;   /* radix buckets addressed by the router          */
-    double mean_bucket_knots;  /* mean knots searched per probe bucket (MLP)     */
-    double mean_bucket_knots_linear; /* same with the linear (no-MLP) router     */
+    double mean_bucket_knots;  /* mean knots searched per probe bucket (learned router: MLP where it helps) */
+    double mean_bucket_knots_linear; /* same with the linear (no-MLP) router everywhere */
     uint64_t fallbacks;        /* probes that fell back to binary search (last probe) */
-    double final_loss;         /* mean router training loss over partitions      */
+    double final_loss;         /* mean router training loss over trained partitions */
+    uint64_t mlp_candidates;   /* partitions with a trained router               */
+    uint64_t mlp_partitions;   /* partitions whose learned probe uses the MLP    */
 } qe_asof_stats;
 qe_gpu_status qe_gpu_asof_stats(qe_gpu_ctx* ctx, const qe_dev_asof* idx, qe_asof_stats* out);
 /* Sorted build side for inspection: sorted timestamps, original row of each
@@ -312,8 +320,13 @@ qe_gpu_status qe_gpu_gen_join_build(qe_gpu_ctx* ctx, uint64_t seed, uint64_t n, 
                                     qe_dev_col* key_i64, qe_dev_col* grp_i32);
 qe_gpu_status qe_gpu_gen_join_probe(qe_gpu_ctx* ctx, uint64_t seed, uint64_t n, uint64_t build_n, double zipf_s,
                                     double miss_frac, qe_dev_col* key_i64, qe_dev_col* pay_f64);
+/* As-of tick shape: sym Zipf(zipf_s) over nsym, per-sym strictly increasing
+ * microsecond times. sessions == 0: exponential gaps (mean 1 ms) with 10x
+ * bursts in 512-row regimes; sessions > 0: that many trading days with a
+ * U-shaped intraday intensity and overnight gaps. Left: Zipf syms, times
+ * uniform over the sym's right-side span +-5%. */
 qe_gpu_status qe_gpu_gen_asof(qe_gpu_ctx* ctx, uint64_t seed, uint64_t right_n, uint64_t left_n, uint32_t nsym,
-                              double zipf_s, qe_dev_col* r_sym, qe_dev_col* r_ts, qe_dev_col* r_px,
+                              double zipf_s, uint32_t sessions, qe_dev_col* r_sym, qe_dev_col* r_ts, qe_dev_col* r_px,
                               qe_dev_col* l_sym, qe_dev_col* l_ts);
 
 #ifdef __cplusplus

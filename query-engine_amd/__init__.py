@@ -70,7 +70,8 @@ class AsofStats(C.Structure):
     _fields_ = [("build_rows", C.c_uint64), ("partitions", C.c_uint64), ("pieces", C.c_uint64),
                 ("max_piece_error", C.c_uint64), ("router_buckets", C.c_uint64),
                 ("mean_bucket_knots", C.c_double), ("mean_bucket_knots_linear", C.c_double),
-                ("fallbacks", C.c_uint64), ("final_loss", C.c_double)]
+                ("fallbacks", C.c_uint64), ("final_loss", C.c_double),
+                ("mlp_candidates", C.c_uint64), ("mlp_partitions", C.c_uint64)]
 
 
 P = C.c_void_p
@@ -81,6 +82,7 @@ _SIGS = {
     "qe_gpu_sync": (C.c_int, [P]),
     "qe_gpu_set_mem_limit": (C.c_int, [P, C.c_uint64]),
     "qe_gpu_set_timing": (C.c_int, [P, C.c_int]),
+    "qe_gpu_set_tuning": (C.c_int, [P, C.c_int32, C.c_int64]),
     "qe_gpu_timing": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "qe_gpu_column_upload": (C.c_int, [P, C.POINTER(ColDesc), C.POINTER(P)]),
     "qe_gpu_column_alloc": (C.c_int, [P, C.c_int32, C.c_uint64, C.c_int32, C.POINTER(P)]),
@@ -138,7 +140,7 @@ _SIGS = {
     "qe_gpu_gen_groupby": (C.c_int, [P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, P, P]),
     "qe_gpu_gen_join_build": (C.c_int, [P, C.c_uint64, C.c_uint64, C.c_uint32, P, P]),
     "qe_gpu_gen_join_probe": (C.c_int, [P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, C.c_double, P, P]),
-    "qe_gpu_gen_asof": (C.c_int, [P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint32, C.c_double,
+    "qe_gpu_gen_asof": (C.c_int, [P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint32, C.c_double, C.c_uint32,
                                   P, P, P, P, P]),
 }
 
@@ -351,6 +353,9 @@ class Context:
     def set_timing(self, on: bool):
         self._check(lib().qe_gpu_set_timing(self.h, 1 if on else 0))
 
+    def set_tuning(self, key: int, value: int):
+        self._check(lib().qe_gpu_set_tuning(self.h, key, value))
+
     def timing(self, name: str):
         ms, n = C.c_double(), C.c_uint64()
         self._check(lib().qe_gpu_timing(self.h, name.encode(), C.byref(ms), C.byref(n)))
@@ -435,10 +440,10 @@ class Context:
         self._check(lib().qe_gpu_gen_join_probe(self.h, seed, probe_n, build_n, zipf_s, miss_frac, pk.h, pp.h))
         return (pk, pp), (bk, bg)
 
-    def gen_asof(self, seed, right_n, left_n, nsym=5000, zipf_s=1.1):
+    def gen_asof(self, seed, right_n, left_n, nsym=5000, zipf_s=1.1, sessions=0):
         rs, rt, rp = self.alloc(I32, right_n), self.alloc(I64, right_n), self.alloc(F64, right_n)
         ls, lt = self.alloc(I32, left_n), self.alloc(I64, left_n)
-        self._check(lib().qe_gpu_gen_asof(self.h, seed, right_n, left_n, nsym, zipf_s, rs.h, rt.h, rp.h,
+        self._check(lib().qe_gpu_gen_asof(self.h, seed, right_n, left_n, nsym, zipf_s, sessions, rs.h, rt.h, rp.h,
                                           ls.h, lt.h))
         return (rs, rt, rp), (ls, lt)
 
@@ -509,7 +514,7 @@ class AsofIndex:
     def __init__(self, ctx, h):
         self.ctx, self.h = ctx, h
 
-    def train(self, piece_rows=64, steps=200, lr=0.01):
+    def train(self, piece_rows=64, steps=300, lr=0.003):
         self.ctx._check(lib().qe_gpu_asof_train(self.ctx.h, self.h, piece_rows, steps, lr))
 
     def probe_into(self, probe_keys, probe_time, out: DevBuffer, tolerance=None, mode=ASOF_BINARY):
@@ -554,6 +559,7 @@ class BatchTuning(C.Structure):
     _fields_ = [("tile_rows", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("lds_slots", C.c_uint32)]
 
 
+TUNE_JOIN_PAIRS = 0
 BATCH_PERSISTENT, BATCH_PER_BATCH, BATCH_GRAPH = 0, 1, 2
 
 

@@ -141,6 +141,53 @@ __device__ inline void lds_update(char* smem, uint32_t S, const AggProg& p, uint
     }
 }
 
+// Same fold with the values already in registers: agg a reads value column
+// vcol[a] (-1 for COUNT(*)) whose 64-bit pattern is bits[vcol[a]] (integers
+// sign-extended, F64 raw) and whose validity is bit vcol[a] of `valid`.
+__device__ inline void lds_update_vals(char* smem, uint32_t S, const AggProg& p, uint32_t slot,
+                                       const int32_t* vcol, const uint64_t* bits, uint32_t valid) {
+    uint64_t* acc = reinterpret_cast<uint64_t*>(smem);
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(smem + size_t(8) * S * p.nacc);
+    uint32_t* rows = cnt + size_t(S) * p.ncnt;
+    atomicAdd(&rows[slot], 1u);
+    for (int a = 0; a < p.n; ++a) {
+        const int k = p.kind[a];
+        if (k == K_COUNT_STAR) continue;
+        const int c = vcol[a];
+        if (!((valid >> c) & 1u)) continue;
+        if (p.cnt_of[a] >= 0) atomicAdd(&cnt[size_t(p.cnt_of[a]) * S + slot], 1u);
+        if (p.acc_of[a] < 0) continue;
+        uint64_t* ap = &acc[size_t(p.acc_of[a]) * S + slot];
+        const uint64_t v = bits[c];
+        switch (k) {
+            case K_SUM_I:
+                atomicAdd(reinterpret_cast<unsigned long long*>(ap), static_cast<unsigned long long>(v));
+                break;
+            case K_SUM_F:
+                atomicAdd(reinterpret_cast<double*>(ap), __longlong_as_double(static_cast<long long>(v)));
+                break;
+            case K_MIN_I:
+                atomicMin(reinterpret_cast<long long*>(ap), static_cast<long long>(v));
+                break;
+            case K_MAX_I:
+                atomicMax(reinterpret_cast<long long*>(ap), static_cast<long long>(v));
+                break;
+            case K_MIN_F:
+                atomicMin(reinterpret_cast<unsigned long long*>(ap),
+                          static_cast<unsigned long long>(
+                              f64_order_key(__longlong_as_double(static_cast<long long>(v)))));
+                break;
+            case K_MAX_F:
+                atomicMax(reinterpret_cast<unsigned long long*>(ap),
+                          static_cast<unsigned long long>(
+                              f64_order_key(__longlong_as_double(static_cast<long long>(v)))));
+                break;
+            default:
+                break;
+        }
+    }
+}
+
 // Flush the block's LDS partials into partial[block] (compact layout).
 __device__ inline void lds_flush(const char* smem, uint32_t S, const AggProg& p, uint64_t* part) {
     const uint64_t* acc = reinterpret_cast<const uint64_t*>(smem);

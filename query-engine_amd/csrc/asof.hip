@@ -43,15 +43,17 @@ constexpr int OFF_W1 = 0, OFF_B1 = 16, OFF_W2 = 32, OFF_B2 = 288, OFF_W3 = 304, 
 constexpr uint32_t kMinKnotsForMlp = 64;
 
 struct PartMeta {
-    uint64_t seg;    // first sorted position
-    uint32_t n;      // rows
-    uint32_t K;      // knots (pieces)
-    uint64_t kb;     // first knot index
-    uint64_t tb;     // first router-table entry
-    uint32_t R;      // router buckets
-    int32_t use_mlp;
-    int64_t t0;      // first knot time
-    double inv_span; // 1 / (last knot time - t0), 0 if single knot
+    uint64_t seg;     // first sorted position
+    uint32_t n;       // rows
+    uint32_t K;       // knots (pieces)
+    uint64_t kb;      // first knot index
+    uint64_t tb;      // first router-table entry
+    uint32_t R;       // router buckets
+    int32_t mlp;      // router of the learned probe: 1 MLP warp grid, 0 linear
+    int64_t t0;       // first knot time (the partition's first row)
+    double inv_span;  // 1 / (last knot time - t0), 0 if single knot
+    int32_t mlp_ok;   // enough knots (and a non-zero span) to train a router
+    int32_t pad;
 };
 
 }  // namespace qe_gpu
@@ -71,9 +73,9 @@ struct qe_dev_asof {
     bool trained = false;
     uint32_t B = 0;
     uint64_t nknots = 0, ntab = 0;
-    Scratch meta, tau, slope, err, tab_mlp, tab_lin, params;
+    Scratch meta, knots, tab_mlp, tab_lin, params, grid;
     double final_loss = 0, mean_knots_mlp = 0, mean_knots_lin = 0;
-    uint64_t max_err = 0, mlp_parts = 0;
+    uint64_t max_err = 0, mlp_candidates = 0, mlp_parts = 0;
     Scratch fallback;  // u64 counter of the last probe
     uint64_t last_fallbacks = 0;
 };
@@ -228,45 +230,60 @@ __global__ void asof_binary_kernel(ProbeArgs a) {
     }
 }
 
-// ---- learned index: MLP -----------------------------------------------------------
+// ---- learned index: pieces, router grid ------------------------------------------------
+// Per partition: K pieces of B sorted rows (the spline knots, one 16-byte
+// record each), a router of R = K buckets over normalised time, and a radix
+// table tab[b] = first knot whose bucket >= b. The router is either linear
+// (bucket = floor(x * R)) or a monotone warp y(x) given by kGrid + 1 values of
+// the trained MLP (evaluated in batch with MFMA, made non-decreasing and
+// normalised to [0, 1]) with linear interpolation inside a cell. The same
+// device function routes knots (table build) and probes, and it is monotone
+// in t, so the piece holding t always lies in the bucket's knot range.
+constexpr int kGrid = 64;
+
 __device__ inline float ramp(float z) { return fminf(fmaxf(z, 0.0f), 1.0f); }
 
-// The router function: monotone non-decreasing in x for w1 >= 0, W2 >= 0,
-// w3 >= 0. Explicit fmaf in a fixed order, so every call site (radix table
-// construction and probe routing) computes bit-identical values.
-__host__ __device__ inline float mlp_forward(const float* P, float x) {
-    float h1[kH];
-    for (int k = 0; k < kH; ++k) h1[k] = fminf(fmaxf(fmaf(P[OFF_W1 + k], x, P[OFF_B1 + k]), 0.0f), 1.0f);
-    float y = P[OFF_B3];
-    for (int m = 0; m < kH; ++m) {
-        float z = P[OFF_B2 + m];
-        for (int k = 0; k < kH; ++k) z = fmaf(P[OFF_W2 + m * kH + k], h1[k], z);
-        y = fmaf(P[OFF_W3 + m], fminf(fmaxf(z, 0.0f), 1.0f), y);
-    }
-    return y;
+struct Knot {
+    int64_t tau;   // time of the piece's first row
+    float slope;   // rows per time unit of the piece's linear position model
+    uint32_t err;  // max |predicted - actual| position over the piece's rows
+};
+
+// Position inside a piece predicted for t >= tau, in [0, cnt). Non-decreasing
+// in t: exact unsigned difference, monotone conversion and fmaf, clamps.
+__device__ inline uint32_t piece_pred(int64_t tau, float slope, uint32_t cnt, int64_t t) {
+    const float dt = static_cast<float>(static_cast<uint64_t>(t) - static_cast<uint64_t>(tau));
+    float p = fmaf(dt, slope, 0.5f);
+    if (!(p >= 0.0f)) p = 0.0f;
+    const float hi = static_cast<float>(cnt - 1);
+    if (p > hi) p = hi;
+    return static_cast<uint32_t>(p);
 }
 
+// Normalised time in [0, 1] for t >= pm.t0.
 __device__ inline float norm_x(const PartMeta& pm, int64_t t) {
-    double x = static_cast<double>(t - pm.t0) * pm.inv_span;
-    x = x < 0.0 ? 0.0 : (x > 1.0 ? 1.0 : x);
+    double x = static_cast<double>(static_cast<uint64_t>(t) - static_cast<uint64_t>(pm.t0)) * pm.inv_span;
+    x = x > 1.0 ? 1.0 : x;
     return static_cast<float>(x);
 }
-__device__ inline uint32_t bucket_of(const PartMeta& pm, const float* P, bool mlp, int64_t t) {
+
+// Router output in [0, 1]: x itself (linear) or the interpolated warp grid W.
+__device__ inline float route_y(const PartMeta& pm, const float* W, bool mlp, int64_t t) {
     const float x = norm_x(pm, t);
-    float y = mlp ? mlp_forward(P, x) : x;
-    float b = floorf(y * static_cast<float>(pm.R));
+    if (!mlp) return x;
+    const float s = x * static_cast<float>(kGrid);
+    int g = static_cast<int>(s);
+    if (g > kGrid - 1) g = kGrid - 1;
+    const float fr = s - static_cast<float>(g);
+    const float y = fmaf(W[g + 1] - W[g], fr, W[g]);
+    return y < W[g + 1] ? y : W[g + 1];  // never past the next cell's start
+}
+
+__device__ inline uint32_t bucket_of(const PartMeta& pm, const float* W, bool mlp, int64_t t) {
+    float b = floorf(route_y(pm, W, mlp, t) * static_cast<float>(pm.R));
     if (!(b >= 0.0f)) b = 0.0f;  // also maps NaN to 0
     if (b > static_cast<float>(pm.R - 1)) b = static_cast<float>(pm.R - 1);
     return static_cast<uint32_t>(b);
-}
-
-// Piece position model: integer prediction in [0, cnt).
-__device__ inline uint32_t piece_pred(int64_t tau, double slope, uint32_t cnt, int64_t t) {
-    double p = static_cast<double>(t - tau) * slope + 0.5;
-    if (!(p >= 0.0)) p = 0.0;
-    const double hi = static_cast<double>(cnt - 1);
-    if (p > hi) p = hi;
-    return static_cast<uint32_t>(p);
 }
 
 __global__ void meta_counts_kernel(const uint64_t* offs, uint64_t nparts, uint32_t B, uint32_t* K, uint32_t* R1) {
@@ -282,18 +299,21 @@ __global__ void meta_fill_kernel(const uint64_t* offs, uint64_t nparts, uint32_t
                                  const uint64_t* kb, const uint64_t* tb, const int64_t* ts, PartMeta* meta) {
     for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < nparts;
          p += uint64_t(gridDim.x) * blockDim.x) {
-        PartMeta m;
+        PartMeta m{};
         m.seg = offs[p];
         m.n = static_cast<uint32_t>(offs[p + 1] - offs[p]);
         m.K = K[p];
         m.kb = kb[p];
         m.tb = tb[p];
         m.R = K[p];
-        m.use_mlp = K[p] >= kMinKnotsForMlp;
+        m.mlp_ok = K[p] >= kMinKnotsForMlp;
+        m.mlp = 0;
         if (m.K) {
             m.t0 = ts[m.seg];
             const int64_t tl = ts[m.seg + uint64_t(m.K - 1) * B];
-            m.inv_span = tl > m.t0 ? 1.0 / static_cast<double>(tl - m.t0) : 0.0;
+            const uint64_t span = static_cast<uint64_t>(tl) - static_cast<uint64_t>(m.t0);
+            m.inv_span = span ? 1.0 / static_cast<double>(span) : 0.0;
+            if (!span) m.mlp_ok = 0;
         } else {
             m.t0 = 0;
             m.inv_span = 0.0;
@@ -310,10 +330,9 @@ __device__ inline uint32_t part_of_knot(const uint64_t* kb, uint64_t nparts, uin
     }
     return static_cast<uint32_t>(lo);
 }
-// Knot times, piece slopes and recorded max errors.
+// Knot records: first time, slope and recorded max error of each piece.
 __global__ void pieces_kernel(const PartMeta* meta, const uint64_t* kb, uint64_t nparts, uint64_t nknots, uint32_t B,
-                              const int64_t* ts, int64_t* tau, double* slope, uint32_t* err,
-                              unsigned long long* max_err) {
+                              const int64_t* ts, Knot* knots, unsigned long long* max_err) {
     uint64_t local_max = 0;
     for (uint64_t g = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; g < nknots;
          g += uint64_t(gridDim.x) * blockDim.x) {
@@ -323,16 +342,16 @@ __global__ void pieces_kernel(const PartMeta* meta, const uint64_t* kb, uint64_t
         const uint64_t pos0 = pm.seg + uint64_t(j) * B;
         const uint32_t cnt = min(B, pm.n - j * B);
         const int64_t t0 = ts[pos0], tl = ts[pos0 + cnt - 1];
-        const double s = (cnt > 1 && tl > t0) ? static_cast<double>(cnt - 1) / static_cast<double>(tl - t0) : 0.0;
+        const uint64_t span = static_cast<uint64_t>(tl) - static_cast<uint64_t>(t0);
+        const float s = (cnt > 1 && span) ? static_cast<float>(static_cast<double>(cnt - 1) / static_cast<double>(span))
+                                          : 0.0f;
         uint32_t e = 0;
         for (uint32_t i = 0; i < cnt; ++i) {
             const uint32_t pr = piece_pred(t0, s, cnt, ts[pos0 + i]);
             const uint32_t d = pr > i ? pr - i : i - pr;
             e = d > e ? d : e;
         }
-        tau[g] = t0;
-        slope[g] = s;
-        err[g] = e;
+        knots[g] = Knot{t0, s, e};
         local_max = e > local_max ? e : local_max;
     }
     for (int off = kWave / 2; off > 0; off >>= 1) {
@@ -342,60 +361,95 @@ __global__ void pieces_kernel(const PartMeta* meta, const uint64_t* kb, uint64_t
     if (lane_id() == 0) atomicMax(max_err, static_cast<unsigned long long>(local_max));
 }
 
+// Router choice for a table: 0 linear everywhere, 1 the partition's chosen
+// router, 2 the MLP wherever one was trained.
+__device__ inline bool router_mlp(const PartMeta& pm, int which) {
+    return which == 1 ? pm.mlp != 0 : (which == 2 ? pm.mlp_ok != 0 : false);
+}
+
 // Radix router table: tab[tb + b] = first knot whose bucket >= b (b in 0..R).
 __global__ void router_table_kernel(const PartMeta* meta, const uint64_t* kb, uint64_t nparts, uint64_t nknots,
-                                    const int64_t* tau, const float* params, int use_mlp, uint32_t* tab) {
+                                    const Knot* knots, const float* grid, int which, uint32_t* tab) {
     for (uint64_t g = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; g < nknots;
          g += uint64_t(gridDim.x) * blockDim.x) {
         const uint32_t p = part_of_knot(kb, nparts + 1, g);
         const PartMeta pm = meta[p];
         const uint32_t j = static_cast<uint32_t>(g - pm.kb);
-        const bool mlp = use_mlp && pm.use_mlp;
-        const float* P = params + size_t(p) * kParams;
-        const int64_t bj = bucket_of(pm, P, mlp, tau[g]);
-        const int64_t bprev = (j == 0) ? -1 : static_cast<int64_t>(bucket_of(pm, P, mlp, tau[g - 1]));
+        const bool mlp = router_mlp(pm, which);
+        const float* W = grid + size_t(p) * (kGrid + 1);
+        const int64_t bj = bucket_of(pm, W, mlp, knots[g].tau);
+        const int64_t bprev = (j == 0) ? -1 : static_cast<int64_t>(bucket_of(pm, W, mlp, knots[g - 1].tau));
         for (int64_t b = bprev + 1; b <= bj; ++b) tab[pm.tb + b] = j;
         if (j == pm.K - 1)
             for (int64_t b = bj + 1; b <= static_cast<int64_t>(pm.R); ++b) tab[pm.tb + b] = pm.K;
     }
 }
 
-// Mean knots a probe searches (probes distributed like the knots): for each
-// knot, its bucket's occupancy (+1 for the predecessor boundary).
+// Knots a probe searches, summed per partition and overall (probes assumed to
+// be distributed like the knots): for each knot, its bucket's occupancy + 1
+// (the predecessor boundary). Partitions are contiguous knot ranges, so a wave
+// whose lanes share one partition adds its sum with one atomic.
 __global__ void router_stats_kernel(const PartMeta* meta, const uint64_t* kb, uint64_t nparts, uint64_t nknots,
-                                    const int64_t* tau, const float* params, int use_mlp, const uint32_t* tab,
-                                    double* sum) {
+                                    const Knot* knots, const float* grid, int which, const uint32_t* tab,
+                                    double* per_part, double* total) {
     double local = 0;
-    for (uint64_t g = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; g < nknots;
-         g += uint64_t(gridDim.x) * blockDim.x) {
-        const uint32_t p = part_of_knot(kb, nparts + 1, g);
-        const PartMeta pm = meta[p];
-        const bool mlp = use_mlp && pm.use_mlp;
-        const uint32_t b = bucket_of(pm, params + size_t(p) * kParams, mlp, tau[g]);
-        local += static_cast<double>(tab[pm.tb + b + 1] - tab[pm.tb + b]) + 1.0;
+    for (uint64_t g0 = blockIdx.x * uint64_t(blockDim.x); g0 < nknots; g0 += uint64_t(gridDim.x) * blockDim.x) {
+        const uint64_t g = g0 + threadIdx.x;
+        uint32_t p = 0xFFFFFFFFu;
+        double v = 0;
+        if (g < nknots) {
+            p = part_of_knot(kb, nparts + 1, g);
+            const PartMeta pm = meta[p];
+            const uint32_t b = bucket_of(pm, grid + size_t(p) * (kGrid + 1), router_mlp(pm, which), knots[g].tau);
+            v = static_cast<double>(tab[pm.tb + b + 1] - tab[pm.tb + b]) + 1.0;
+        }
+        local += v;
+        if (per_part) {
+            const uint32_t p0 = __shfl(p, 0);
+            const bool same = __all(p == p0 || p == 0xFFFFFFFFu);
+            if (same) {
+                double s = v;
+                for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_down(s, off);
+                if (lane_id() == 0 && p0 != 0xFFFFFFFFu) atomicAdd(&per_part[p0], s);
+            } else if (p != 0xFFFFFFFFu) {
+                atomicAdd(&per_part[p], v);
+            }
+        }
     }
     for (int off = kWave / 2; off > 0; off >>= 1) local += __shfl_down(local, off);
-    if (lane_id() == 0) atomicAdd(sum, local);
+    if (lane_id() == 0) atomicAdd(total, local);
+}
+
+// Keep the MLP router where it searches fewer knots than the linear one.
+__global__ void router_choose_kernel(PartMeta* meta, uint64_t nparts, const double* occ_lin, const double* occ_mlp,
+                                     unsigned long long* chosen) {
+    uint64_t c = 0;
+    for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < nparts;
+         p += uint64_t(gridDim.x) * blockDim.x) {
+        const bool use = meta[p].mlp_ok && occ_mlp[p] < occ_lin[p];
+        meta[p].mlp = use ? 1 : 0;
+        c += use;
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) c += __shfl_down(c, off);
+    if (lane_id() == 0 && c) atomicAdd(chosen, static_cast<unsigned long long>(c));
 }
 
 // ---- learned probe ---------------------------------------------------------------------
 struct LearnedArgs {
     const PartMeta* meta;
-    const int64_t* tau;
-    const double* slope;
-    const uint32_t* err;
+    const Knot* knots;
     const uint32_t* tab;
-    const float* params;
+    const float* grid;
     int use_mlp;
     uint32_t B;
     unsigned long long* fallbacks;
 };
 
 // Largest knot j in [lo, hi] with tau[kb+j] <= t (caller guarantees tau[kb+lo] <= t).
-__device__ inline uint32_t knot_search(const int64_t* tau, uint64_t kb, uint32_t lo, uint32_t hi, int64_t t) {
+__device__ inline uint32_t knot_search(const Knot* knots, uint64_t kb, uint32_t lo, uint32_t hi, int64_t t) {
     while (lo < hi) {
         const uint32_t mid = lo + (hi - lo + 1) / 2;
-        if (tau[kb + mid] <= t) lo = mid;
+        if (knots[kb + mid].tau <= t) lo = mid;
         else hi = mid - 1;
     }
     return lo;
@@ -412,32 +466,33 @@ __global__ void asof_learned_kernel(ProbeArgs a, LearnedArgs L) {
         }
         const int64_t t = time_value(a.time, j);
         const PartMeta pm = L.meta[p];
-        if (pm.K == 0 || t < L.tau[pm.kb]) {  // before the first build row
+        if (pm.K == 0 || t < pm.t0) {  // before the partition's first row
             a.out[j] = kNoRow;
             continue;
         }
-        // 1. route to a knot range and locate the piece j* (tau[j*] <= t < tau[j*+1]).
-        const bool mlp = L.use_mlp && pm.use_mlp;
-        const uint32_t b = bucket_of(pm, L.params + size_t(p) * kParams, mlp, t);
+        // 1. route to a knot range and locate the piece (tau[piece] <= t < tau[piece+1]).
+        const bool mlp = L.use_mlp && pm.mlp;
+        const uint32_t b = bucket_of(pm, L.grid + size_t(p) * (kGrid + 1), mlp, t);
         const uint32_t t_lo = L.tab[pm.tb + b], t_hi = L.tab[pm.tb + b + 1];
         uint32_t lo = t_lo > 0 ? t_lo - 1 : 0;
         uint32_t hi = t_hi > 0 ? t_hi - 1 : 0;
         if (hi >= pm.K) hi = pm.K - 1;
         if (lo > hi) lo = hi;
         uint32_t piece;
-        const bool ok_lo = L.tau[pm.kb + lo] <= t;
-        const bool ok_hi = (hi + 1 >= pm.K) || L.tau[pm.kb + hi + 1] > t;
+        const bool ok_lo = L.knots[pm.kb + lo].tau <= t;
+        const bool ok_hi = (hi + 1 >= pm.K) || L.knots[pm.kb + hi + 1].tau > t;
         if (ok_lo && ok_hi) {
-            piece = knot_search(L.tau, pm.kb, lo, hi, t);
+            piece = knot_search(L.knots, pm.kb, lo, hi, t);
         } else {
             ++fb;
-            piece = knot_search(L.tau, pm.kb, 0, pm.K - 1, t);
+            piece = knot_search(L.knots, pm.kb, 0, pm.K - 1, t);
         }
         // 2. error-bounded window inside the piece.
+        const Knot kn = L.knots[pm.kb + piece];
         const uint64_t base = pm.seg + uint64_t(piece) * L.B;
         const uint32_t cnt = min(L.B, pm.n - piece * L.B);
-        const uint32_t pred = piece_pred(L.tau[pm.kb + piece], L.slope[pm.kb + piece], cnt, t);
-        const uint32_t e = L.err[pm.kb + piece] + 1;
+        const uint32_t pred = piece_pred(kn.tau, kn.slope, cnt, t);
+        const uint32_t e = kn.err + 1;
         const uint32_t wlo = pred > e ? pred - e : 0;
         const uint32_t whi = min(cnt - 1, pred + e);
         uint64_t c;
@@ -585,10 +640,10 @@ __device__ inline void init_params(float* P) {
 
 constexpr uint32_t kTrainBatch = 2048;
 
-// One block per partition with an MLP: Adam on the batch MSE of predicting the
-// knot CDF (j + 0.5) / K from the normalised knot time; projection keeps the
-// router monotone (w1, W2, w3 >= 0).
-__global__ void __launch_bounds__(256) mlp_train_kernel(const PartMeta* meta, const int64_t* tau, uint64_t nparts,
+// One block per partition with enough knots: Adam (cosine-decayed step size)
+// on the batch MSE of predicting the knot CDF (j + 0.5) / K from the
+// normalised knot time; projection keeps the router monotone (w1, W2, w3 >= 0).
+__global__ void __launch_bounds__(256) mlp_train_kernel(const PartMeta* meta, const Knot* knots, uint64_t nparts,
                                                         uint32_t steps, float lr, float* params, float* losses) {
     __shared__ float P[kParams], G[kParams], M1[kParams], M2[kParams];
     __shared__ float xs[kTrainBatch], cs[kTrainBatch];
@@ -596,7 +651,7 @@ __global__ void __launch_bounds__(256) mlp_train_kernel(const PartMeta* meta, co
     const uint64_t p = blockIdx.x;
     if (p >= nparts) return;
     const PartMeta pm = meta[p];
-    if (!pm.use_mlp) return;  // block-uniform exit
+    if (!pm.mlp_ok) return;  // block-uniform exit
     init_params(P);
     for (int i = threadIdx.x; i < kParams; i += blockDim.x) M1[i] = M2[i] = 0.0f;
     const uint32_t K = pm.K;
@@ -608,7 +663,7 @@ __global__ void __launch_bounds__(256) mlp_train_kernel(const PartMeta* meta, co
         for (uint32_t s = threadIdx.x; s < N; s += blockDim.x) {
             uint32_t j = static_cast<uint32_t>((static_cast<double>(s) + phase) * K / N);
             if (j >= K) j = K - 1;
-            xs[s] = norm_x(pm, tau[pm.kb + j]);
+            xs[s] = norm_x(pm, knots[pm.kb + j].tau);
             cs[s] = (static_cast<float>(j) + 0.5f) / static_cast<float>(K);
         }
         for (int i = threadIdx.x; i < kParams; i += blockDim.x) G[i] = 0.0f;
@@ -616,13 +671,14 @@ __global__ void __launch_bounds__(256) mlp_train_kernel(const PartMeta* meta, co
         __syncthreads();
         mlp_batch_grads(P, xs, cs, N, G, &loss);
         __syncthreads();
+        const float a = 0.5f * lr * (1.0f + cosf(3.14159265f * static_cast<float>(step) / static_cast<float>(steps)));
         const float b1c = 1.0f - powf(0.9f, static_cast<float>(step + 1));
         const float b2c = 1.0f - powf(0.999f, static_cast<float>(step + 1));
         for (int i = threadIdx.x; i < kParams; i += blockDim.x) {
             const float gi = G[i];
             M1[i] = 0.9f * M1[i] + 0.1f * gi;
             M2[i] = 0.999f * M2[i] + 0.001f * gi * gi;
-            float v = P[i] - lr * (M1[i] / b1c) / (sqrtf(M2[i] / b2c) + 1e-8f);
+            float v = P[i] - a * (M1[i] / b1c) / (sqrtf(M2[i] / b2c) + 1e-8f);
             const bool nonneg = (i < OFF_B1) || (i >= OFF_W2 && i < OFF_B2) || (i >= OFF_W3 && i < OFF_B3);
             if (nonneg && v < 0.0f) v = 0.0f;
             P[i] = v;
@@ -658,10 +714,82 @@ __global__ void __launch_bounds__(256) mlp_grad_kernel(const float* params, cons
     if (threadIdx.x == 0) *loss_out = loss / static_cast<float>(N);
 }
 
-__global__ void mlp_forward_kernel(const float* params, const float* x, uint64_t n, float* y) {
-    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n;
-         i += uint64_t(gridDim.x) * blockDim.x)
-        y[i] = mlp_forward(params, x[i]);
+// Batched inference of 16 samples on the matrix cores: lane 16q + r supplies
+// sample r's input x_r; on return every lane of quarter q holds y of samples
+// 4q .. 4q+3 (H1 [16x16] times W2^T as four 16x16x4 MFMAs, as in training).
+__device__ inline void mlp_infer_tile(const float* P, float x_r, float y[4]) {
+    const unsigned lane = lane_id();
+    const unsigned q = lane / 16, r = lane % 16;
+    floatx4 z2;
+    for (int v = 0; v < 4; ++v) z2[v] = P[OFF_B2 + r];
+    for (int c = 0; c < 4; ++c) {
+        const int k = 4 * c + q;
+        const float a = ramp(fmaf(P[OFF_W1 + k], x_r, P[OFF_B1 + k]));  // H1[s=r][k]
+        z2 = mfma16(a, P[OFF_W2 + r * kH + k], z2);                    // W2[m=r][k]
+    }
+    for (int v = 0; v < 4; ++v) {
+        float pv = P[OFF_W3 + r] * ramp(z2[v]);
+        for (int off = 8; off > 0; off >>= 1) pv += __shfl_xor(pv, off, 16);
+        y[v] = pv + P[OFF_B3];
+    }
+}
+
+// Router warp grid of each trained partition: the MLP at x = i / kGrid
+// (i = 0..kGrid) in batch, made non-decreasing and scaled to [0, 1]. A
+// partition whose warp is flat or not finite keeps the linear router.
+__global__ void __launch_bounds__(64) mlp_grid_kernel(PartMeta* meta, uint64_t nparts, const float* params,
+                                                      float* grid) {
+    __shared__ float P[kParams];
+    __shared__ float W[80];
+    const uint64_t p = blockIdx.x;
+    if (p >= nparts || !meta[p].mlp_ok) return;  // block-uniform exit
+    for (int i = threadIdx.x; i < kParams; i += blockDim.x) P[i] = params[p * kParams + i];
+    __syncthreads();
+    const unsigned lane = lane_id(), q = lane / 16, r = lane % 16;
+    for (int tile = 0; tile < 5; ++tile) {  // 80 >= kGrid + 1 samples
+        const int s = 16 * tile + static_cast<int>(r);
+        const float x = static_cast<float>(s < kGrid ? s : kGrid) / static_cast<float>(kGrid);
+        float y[4];
+        mlp_infer_tile(P, x, y);
+        if (r == 0)
+            for (int v = 0; v < 4; ++v) W[16 * tile + 4 * q + v] = y[v];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        bool ok = true;
+        float run = W[0];
+        for (int i = 0; i <= kGrid; ++i) {
+            if (!(W[i] == W[i]) || W[i] > 3.0e38f || W[i] < -3.0e38f) ok = false;
+            run = W[i] > run ? W[i] : run;
+            W[i] = run;
+        }
+        const float lo = W[0], span = W[kGrid] - W[0];
+        ok = ok && span > 0.0f;
+        if (ok) {
+            for (int i = 0; i <= kGrid; ++i) grid[p * (kGrid + 1) + i] = (W[i] - lo) / span;
+            grid[p * (kGrid + 1) + kGrid] = 1.0f;
+        } else {
+            meta[p].mlp_ok = 0;
+        }
+    }
+}
+
+// Test entry: batched inference over n inputs (one wave per 16 samples).
+__global__ void __launch_bounds__(256) mlp_forward_kernel(const float* params, const float* x, uint64_t n, float* y) {
+    __shared__ float P[kParams];
+    for (int i = threadIdx.x; i < kParams; i += blockDim.x) P[i] = params[i];
+    __syncthreads();
+    const unsigned lane = lane_id(), q = lane / 16, r = lane % 16;
+    const uint64_t wave = (blockIdx.x * uint64_t(blockDim.x) + threadIdx.x) / kWave;
+    const uint64_t nwaves = uint64_t(gridDim.x) * blockDim.x / kWave;
+    for (uint64_t s0 = wave * 16; s0 < n; s0 += nwaves * 16) {  // wave-uniform loop
+        const uint64_t s = s0 + r;
+        float out[4];
+        mlp_infer_tile(P, s < n ? x[s] : 0.0f, out);
+        if (r == 0)
+            for (int v = 0; v < 4; ++v)
+                if (s0 + 4 * q + v < n) y[s0 + 4 * q + v] = out[v];
+    }
 }
 
 static PartKeys part_keys(const qe_dev_asof* idx, const qe_dev_col* const* keys) {
@@ -800,55 +928,76 @@ qe_gpu_status qe_gpu_asof_train(qe_gpu_ctx* ctx, qe_dev_asof* idx, uint32_t piec
             idx->ts.as<int64_t>(), idx->meta.as<PartMeta>());
         QE_HIP(hipGetLastError());
         const uint64_t nk1 = std::max<uint64_t>(nk, 1);
-        idx->tau = Scratch(ctx, nk1 * 8);
-        idx->slope = Scratch(ctx, nk1 * 8);
-        idx->err = Scratch(ctx, nk1 * 4);
+        idx->knots = Scratch(ctx, nk1 * sizeof(Knot));
         idx->tab_mlp = Scratch(ctx, std::max<uint64_t>(nt, 1) * 4);
         idx->tab_lin = Scratch(ctx, std::max<uint64_t>(nt, 1) * 4);
-        idx->params = Scratch(ctx, P * kParams * 4);
-        Scratch losses(ctx, P * 4), maxe(ctx, 8), sums(ctx, 16);
-        memset_async(ctx, losses.p, 0, P * 4);
+        idx->params = Scratch(ctx, std::max<uint64_t>(P, 1) * kParams * 4);
+        idx->grid = Scratch(ctx, std::max<uint64_t>(P, 1) * (kGrid + 1) * 4);
+        Scratch losses(ctx, std::max<uint64_t>(P, 1) * 4), maxe(ctx, 8), sums(ctx, 24),
+            occ(ctx, std::max<uint64_t>(P, 1) * 16), chosen(ctx, 8);
+        memset_async(ctx, losses.p, 0, std::max<uint64_t>(P, 1) * 4);
         memset_async(ctx, maxe.p, 0, 8);
-        memset_async(ctx, sums.p, 0, 16);
+        memset_async(ctx, sums.p, 0, 24);
+        memset_async(ctx, occ.p, 0, std::max<uint64_t>(P, 1) * 16);
+        memset_async(ctx, chosen.p, 0, 8);
         if (nk) {
-            pieces_kernel<<<grid_for(nk, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(
-                idx->meta.as<PartMeta>(), kb.as<uint64_t>(), P, nk, piece_rows, idx->ts.as<int64_t>(),
-                idx->tau.as<int64_t>(), idx->slope.as<double>(), idx->err.as<uint32_t>(),
-                maxe.as<unsigned long long>());
+            const unsigned gk = grid_for(nk, 256, ctx->num_cus * 16);
+            PartMeta* meta = idx->meta.as<PartMeta>();
+            const Knot* knots = idx->knots.as<Knot>();
+            double* occ_lin = occ.as<double>();
+            double* occ_mlp = occ_lin + P;
+            pieces_kernel<<<gk, 256, 0, ctx->stream>>>(meta, kb.as<uint64_t>(), P, nk, piece_rows,
+                                                       idx->ts.as<int64_t>(), idx->knots.as<Knot>(),
+                                                       maxe.as<unsigned long long>());
             QE_HIP(hipGetLastError());
             {
                 TimedRegion tr(ctx, "asof_mlp_train");
-                mlp_train_kernel<<<P, 256, 0, ctx->stream>>>(idx->meta.as<PartMeta>(), idx->tau.as<int64_t>(), P, steps,
-                                                             lr, idx->params.as<float>(), losses.as<float>());
+                mlp_train_kernel<<<P, 256, 0, ctx->stream>>>(meta, knots, P, steps, lr, idx->params.as<float>(),
+                                                             losses.as<float>());
                 QE_HIP(hipGetLastError());
             }
-            for (int use = 0; use < 2; ++use) {
-                uint32_t* tab = use ? idx->tab_mlp.as<uint32_t>() : idx->tab_lin.as<uint32_t>();
-                router_table_kernel<<<grid_for(nk, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(
-                    idx->meta.as<PartMeta>(), kb.as<uint64_t>(), P, nk, idx->tau.as<int64_t>(),
-                    idx->params.as<float>(), use, tab);
-                router_stats_kernel<<<grid_for(nk, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(
-                    idx->meta.as<PartMeta>(), kb.as<uint64_t>(), P, nk, idx->tau.as<int64_t>(),
-                    idx->params.as<float>(), use, tab, sums.as<double>() + use);
-                QE_HIP(hipGetLastError());
-            }
+            mlp_grid_kernel<<<P, 64, 0, ctx->stream>>>(meta, P, idx->params.as<float>(), idx->grid.as<float>());
+            QE_HIP(hipGetLastError());
+            // Candidate MLP router and the linear one, with knots searched per partition.
+            const float* grid = idx->grid.as<float>();
+            router_table_kernel<<<gk, 256, 0, ctx->stream>>>(meta, kb.as<uint64_t>(), P, nk, knots, grid, 2,
+                                                             idx->tab_mlp.as<uint32_t>());
+            router_stats_kernel<<<gk, 256, 0, ctx->stream>>>(meta, kb.as<uint64_t>(), P, nk, knots, grid, 2,
+                                                             idx->tab_mlp.as<uint32_t>(), occ_mlp,
+                                                             sums.as<double>() + 2);
+            router_table_kernel<<<gk, 256, 0, ctx->stream>>>(meta, kb.as<uint64_t>(), P, nk, knots, grid, 0,
+                                                             idx->tab_lin.as<uint32_t>());
+            router_stats_kernel<<<gk, 256, 0, ctx->stream>>>(meta, kb.as<uint64_t>(), P, nk, knots, grid, 0,
+                                                             idx->tab_lin.as<uint32_t>(), occ_lin, sums.as<double>());
+            router_choose_kernel<<<grid_for(P, 256, 4096), 256, 0, ctx->stream>>>(meta, P, occ_lin, occ_mlp,
+                                                                                   chosen.as<unsigned long long>());
+            // Final learned router: the MLP only where it searches fewer knots.
+            router_table_kernel<<<gk, 256, 0, ctx->stream>>>(meta, kb.as<uint64_t>(), P, nk, knots, grid, 1,
+                                                             idx->tab_mlp.as<uint32_t>());
+            router_stats_kernel<<<gk, 256, 0, ctx->stream>>>(meta, kb.as<uint64_t>(), P, nk, knots, grid, 1,
+                                                             idx->tab_mlp.as<uint32_t>(), nullptr,
+                                                             sums.as<double>() + 1);
+            QE_HIP(hipGetLastError());
         }
-        std::vector<float> hl(P);
-        std::vector<PartMeta> hm(P);
-        double hs[2] = {0, 0};
-        QE_HIP(hipMemcpyAsync(hl.data(), losses.p, P * 4, hipMemcpyDeviceToHost, ctx->stream));
-        QE_HIP(hipMemcpyAsync(hm.data(), idx->meta.p, P * sizeof(PartMeta), hipMemcpyDeviceToHost, ctx->stream));
-        QE_HIP(hipMemcpyAsync(hs, sums.p, 16, hipMemcpyDeviceToHost, ctx->stream));
+        std::vector<float> hl(std::max<uint64_t>(P, 1));
+        std::vector<PartMeta> hm(std::max<uint64_t>(P, 1));
+        double hs[3] = {0, 0, 0};
+        if (P) {
+            QE_HIP(hipMemcpyAsync(hl.data(), losses.p, P * 4, hipMemcpyDeviceToHost, ctx->stream));
+            QE_HIP(hipMemcpyAsync(hm.data(), idx->meta.p, P * sizeof(PartMeta), hipMemcpyDeviceToHost, ctx->stream));
+        }
+        QE_HIP(hipMemcpyAsync(hs, sums.p, 24, hipMemcpyDeviceToHost, ctx->stream));
         idx->max_err = read_scalar(ctx, maxe.as<uint64_t>());
+        idx->mlp_parts = read_scalar(ctx, chosen.as<uint64_t>());
         double ls = 0;
-        uint64_t cnt = 0;
+        uint64_t cand = 0;
         for (uint64_t p = 0; p < P; ++p)
-            if (hm[p].use_mlp) {
+            if (hm[p].mlp_ok) {
                 ls += hl[p];
-                ++cnt;
+                ++cand;
             }
-        idx->mlp_parts = cnt;
-        idx->final_loss = cnt ? ls / cnt : 0.0;
+        idx->mlp_candidates = cand;
+        idx->final_loss = cand ? ls / cand : 0.0;
         idx->mean_knots_lin = nk ? hs[0] / nk : 0;
         idx->mean_knots_mlp = nk ? hs[1] / nk : 0;
         idx->fallback = Scratch(ctx, 8);
@@ -891,10 +1040,9 @@ qe_gpu_status qe_gpu_asof_probe(qe_gpu_ctx* ctx, const qe_dev_asof* idx, const q
         } else {
             auto* mi = const_cast<qe_dev_asof*>(idx);
             memset_async(ctx, mi->fallback.p, 0, 8);
-            LearnedArgs L{idx->meta.as<PartMeta>(), idx->tau.as<int64_t>(), idx->slope.as<double>(),
-                          idx->err.as<uint32_t>(),
+            LearnedArgs L{idx->meta.as<PartMeta>(), idx->knots.as<Knot>(),
                           mode == QE_ASOF_LEARNED ? idx->tab_mlp.as<uint32_t>() : idx->tab_lin.as<uint32_t>(),
-                          idx->params.as<float>(), mode == QE_ASOF_LEARNED ? 1 : 0, idx->B,
+                          idx->grid.as<float>(), mode == QE_ASOF_LEARNED ? 1 : 0, idx->B,
                           mi->fallback.as<unsigned long long>()};
             {
                 TimedRegion tr(ctx, mode == QE_ASOF_LEARNED ? "asof_probe_learned" : "asof_probe_learned_no_mlp");
@@ -963,6 +1111,8 @@ qe_gpu_status qe_gpu_asof_stats(qe_gpu_ctx* ctx, const qe_dev_asof* idx, qe_asof
         out->mean_bucket_knots_linear = idx->mean_knots_lin;
         out->fallbacks = idx->last_fallbacks;
         out->final_loss = idx->final_loss;
+        out->mlp_candidates = idx->mlp_candidates;
+        out->mlp_partitions = idx->mlp_parts;
     });
 }
 

@@ -308,7 +308,11 @@ uint64_t exclusive_scan_u64(qe_gpu_ctx* ctx, const uint64_t* in, uint64_t* out, 
 }
 
 // ---- min/max over valid rows -----------------------------------------------------
-__global__ void minmax_kernel(ColView c, uint64_t n, unsigned long long* out) {
+// Per-block (min, max, count) of the valid values, then one block folds the
+// partials (no contended atomics: 2048 blocks hitting one address serialise).
+__global__ void __launch_bounds__(256) minmax_kernel(ColView c, uint64_t n, int64_t* part) {
+    __shared__ int64_t smn[kMaxWaves], smx[kMaxWaves];
+    __shared__ uint64_t scnt[kMaxWaves];
     int64_t mn = INT64_MAX, mx = INT64_MIN;
     uint64_t cnt = 0;
     for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n;
@@ -326,28 +330,70 @@ __global__ void minmax_kernel(ColView c, uint64_t n, unsigned long long* out) {
         mx = b > mx ? b : mx;
         cnt += d;
     }
+    const int w = threadIdx.x / kWave;
     if (lane_id() == 0) {
-        atomicMin(reinterpret_cast<long long*>(out), static_cast<long long>(mn));
-        atomicMax(reinterpret_cast<long long*>(out + 1), static_cast<long long>(mx));
-        atomicAdd(out + 2, static_cast<unsigned long long>(cnt));
+        smn[w] = mn;
+        smx[w] = mx;
+        scnt[w] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < static_cast<int>(blockDim.x / kWave); ++k) {
+            mn = smn[k] < mn ? smn[k] : mn;
+            mx = smx[k] > mx ? smx[k] : mx;
+            cnt += scnt[k];
+        }
+        part[3 * blockIdx.x] = mn;
+        part[3 * blockIdx.x + 1] = mx;
+        part[3 * blockIdx.x + 2] = static_cast<int64_t>(cnt);
     }
 }
-__global__ void minmax_init(unsigned long long* out) {
-    out[0] = static_cast<unsigned long long>(INT64_MAX);
-    out[1] = static_cast<unsigned long long>(INT64_MIN);
-    out[2] = 0;
+__global__ void __launch_bounds__(256) minmax_final(const int64_t* part, uint32_t nparts, int64_t* out) {
+    __shared__ int64_t smn[kMaxWaves], smx[kMaxWaves];
+    __shared__ uint64_t scnt[kMaxWaves];
+    int64_t mn = INT64_MAX, mx = INT64_MIN;
+    uint64_t cnt = 0;
+    for (uint32_t b = threadIdx.x; b < nparts; b += blockDim.x) {
+        mn = part[3 * b] < mn ? part[3 * b] : mn;
+        mx = part[3 * b + 1] > mx ? part[3 * b + 1] : mx;
+        cnt += static_cast<uint64_t>(part[3 * b + 2]);
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) {
+        const int64_t a = __shfl_down(mn, off), b = __shfl_down(mx, off);
+        const uint64_t d = __shfl_down(cnt, off);
+        mn = a < mn ? a : mn;
+        mx = b > mx ? b : mx;
+        cnt += d;
+    }
+    const int w = threadIdx.x / kWave;
+    if (lane_id() == 0) {
+        smn[w] = mn;
+        smx[w] = mx;
+        scnt[w] = cnt;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < static_cast<int>(blockDim.x / kWave); ++k) {
+            mn = smn[k] < mn ? smn[k] : mn;
+            mx = smx[k] > mx ? smx[k] : mx;
+            cnt += scnt[k];
+        }
+        out[0] = mn;
+        out[1] = mx;
+        out[2] = static_cast<int64_t>(cnt);
+    }
 }
 
 MinMax column_minmax(qe_gpu_ctx* ctx, const qe_dev_col* c, uint64_t n) {
-    Scratch s(ctx, 3 * sizeof(uint64_t));
-    auto* o = s.as<unsigned long long>();
-    minmax_init<<<1, 1, 0, ctx->stream>>>(o);
-    minmax_kernel<<<grid_for(n, 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(view(c), n, o);
+    const uint32_t blocks = static_cast<uint32_t>(grid_for(n, 256, ctx->num_cus * 4));
+    Scratch part(ctx, size_t(blocks) * 3 * 8), s(ctx, 3 * 8);
+    minmax_kernel<<<blocks, 256, 0, ctx->stream>>>(view(c), n, part.as<int64_t>());
+    minmax_final<<<1, 256, 0, ctx->stream>>>(part.as<int64_t>(), blocks, s.as<int64_t>());
     QE_HIP(hipGetLastError());
-    unsigned long long* h = static_cast<unsigned long long*>(pinned(ctx, 24));
-    QE_HIP(hipMemcpyAsync(h, o, 24, hipMemcpyDeviceToHost, ctx->stream));
+    int64_t* h = static_cast<int64_t*>(pinned(ctx, 24));
+    QE_HIP(hipMemcpyAsync(h, s.p, 24, hipMemcpyDeviceToHost, ctx->stream));
     QE_HIP(hipStreamSynchronize(ctx->stream));
-    return MinMax{static_cast<int64_t>(h[0]), static_cast<int64_t>(h[1]), h[2]};
+    return MinMax{h[0], h[1], static_cast<uint64_t>(h[2])};
 }
 
 }  // namespace qe_gpu
@@ -402,6 +448,19 @@ qe_gpu_status qe_gpu_set_mem_limit(qe_gpu_ctx* ctx, uint64_t bytes) {
 
 qe_gpu_status qe_gpu_set_timing(qe_gpu_ctx* ctx, int enabled) {
     QE_ENTRY(ctx, ctx->timing = enabled != 0);
+}
+
+qe_gpu_status qe_gpu_set_tuning(qe_gpu_ctx* ctx, int32_t key, int64_t value) {
+    QE_ENTRY(ctx, {
+        switch (key) {
+            case QE_TUNE_JOIN_PAIRS:
+                invalid(value == 2 || value == 4, "QE_TUNE_JOIN_PAIRS must be 2 or 4");
+                ctx->tune_join_pairs = static_cast<int>(value);
+                break;
+            default:
+                invalid(false, "unknown tuning key");
+        }
+    });
 }
 
 qe_gpu_status qe_gpu_timing(qe_gpu_ctx* ctx, const char* kernel, double* ms, uint64_t* launches) {

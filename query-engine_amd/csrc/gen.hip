@@ -180,6 +180,37 @@ __global__ void gen_right_kernel(uint64_t seed, uint64_t n, const uint64_t* offs
         px[i] = 1.0 + 199.0 * unif(seed, 14, i);
     }
 }
+// Session-shaped times: each symbol's rows spread over `sessions` trading days
+// (6.5 h of each 24 h) with a U-shaped intraday intensity, density
+// proportional to 1 + 4 (2s - 1)^2 on the session fraction s (inverse CDF by
+// bisection); + k keeps the times strictly increasing.
+__device__ inline double ushape_inv(double u) {
+    double lo = 0.0, hi = 1.0;
+    for (int i = 0; i < 40; ++i) {
+        const double m = 0.5 * (lo + hi), d = 2.0 * m - 1.0;
+        const double F = (m + (2.0 / 3.0) * (d * d * d + 1.0)) * (3.0 / 7.0);
+        if (F < u) lo = m;
+        else hi = m;
+    }
+    return 0.5 * (lo + hi);
+}
+__global__ void gen_right_sessions_kernel(uint64_t seed, uint64_t n, const uint64_t* offs, uint32_t nsym,
+                                          uint32_t sessions, int32_t* sym, int64_t* ts, double* px) {
+    constexpr int64_t kDay = 86400000000ll, kSession = 23400000000ll;  // microseconds
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n;
+         i += uint64_t(gridDim.x) * blockDim.x) {
+        const uint32_t s = sym_of_row(offs, nsym, i);
+        const uint64_t k = i - offs[s], ns = offs[s + 1] - offs[s];
+        const uint64_t per = (ns + sessions - 1) / sessions;
+        const uint64_t d = k / per, kk = k - d * per;
+        const double u = (static_cast<double>(kk) + unif(seed, 17, i)) / static_cast<double>(per);
+        const int64_t t0 = 1700000000000000ll + static_cast<int64_t>(rnd(seed, 13, s) % 3600000000ull);
+        sym[i] = static_cast<int32_t>(s);
+        ts[i] = t0 + static_cast<int64_t>(d) * kDay + static_cast<int64_t>(ushape_inv(u) * kSession) +
+                static_cast<int64_t>(kk);
+        px[i] = 1.0 + 199.0 * unif(seed, 14, i);
+    }
+}
 __global__ void gen_left_kernel(uint64_t seed, uint64_t n, Zipf z, const uint64_t* offs, const int64_t* rts,
                                 int32_t* sym, int64_t* ts) {
     for (uint64_t j = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; j < n;
@@ -264,7 +295,7 @@ qe_gpu_status qe_gpu_gen_join_probe(qe_gpu_ctx* ctx, uint64_t seed, uint64_t n, 
 }
 
 qe_gpu_status qe_gpu_gen_asof(qe_gpu_ctx* ctx, uint64_t seed, uint64_t right_n, uint64_t left_n, uint32_t nsym,
-                              double zipf_s, qe_dev_col* r_sym, qe_dev_col* r_ts, qe_dev_col* r_px,
+                              double zipf_s, uint32_t sessions, qe_dev_col* r_sym, qe_dev_col* r_ts, qe_dev_col* r_px,
                               qe_dev_col* l_sym, qe_dev_col* l_ts) {
     QE_ENTRY(ctx, {
         require(r_sym, QE_I32, right_n, "right sym");
@@ -286,7 +317,12 @@ qe_gpu_status qe_gpu_gen_asof(qe_gpu_ctx* ctx, uint64_t seed, uint64_t right_n, 
         Scratch doffs(ctx, (nsym + 1) * 8), gap(ctx, std::max<uint64_t>(right_n, 1) * 8),
             excl(ctx, std::max<uint64_t>(right_n, 1) * 8);
         QE_HIP(hipMemcpyAsync(doffs.p, offs.data(), (nsym + 1) * 8, hipMemcpyHostToDevice, ctx->stream));
-        if (right_n) {
+        if (right_n && sessions) {
+            gen_right_sessions_kernel<<<grid_for(right_n, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(
+                seed, right_n, doffs.as<uint64_t>(), nsym, sessions, static_cast<int32_t*>(r_sym->data),
+                static_cast<int64_t*>(r_ts->data), static_cast<double*>(r_px->data));
+            QE_HIP(hipGetLastError());
+        } else if (right_n) {
             gen_gaps_kernel<<<grid_for(right_n, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(
                 seed, right_n, doffs.as<uint64_t>(), nsym, gap.as<uint64_t>());
             QE_HIP(hipGetLastError());

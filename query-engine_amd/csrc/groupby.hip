@@ -114,31 +114,47 @@ __device__ inline uint64_t combine(int kind, uint64_t a, uint64_t b) {
     }
 }
 
-// One thread per (field, slot); partial blocks folded in block order so the
-// FP64 result is reproducible run to run.
-__global__ void merge_partials_kernel(const uint64_t* part, uint32_t nblocks, uint64_t* st,
+// One thread per (field, slot) and chunk of `chunk` consecutive blocks: the
+// blocks of a chunk are folded in block order, then the chunk results in chunk
+// order, so the FP64 result is reproducible run to run.
+__global__ void merge_partials_kernel(const uint64_t* part, uint32_t nblocks, uint32_t chunk, uint64_t* out,
                                       uint64_t S, AggProg p) {
     const uint64_t W = state_words(S, p);
-    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < W;
-         i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint32_t nchunks = nblocks ? (nblocks + chunk - 1) / chunk : 1;  // 0 blocks: identities
+    for (uint64_t t = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; t < W * nchunks;
+         t += uint64_t(gridDim.x) * blockDim.x) {
+        const uint64_t i = t % W;
+        const uint32_t c = static_cast<uint32_t>(t / W);
+        const uint32_t b0 = c * chunk, b1 = min(nblocks, b0 + chunk);
         const uint64_t field = i / S;
         uint64_t v;
         if (field == 0 || field > uint64_t(p.nacc)) {  // rows or cnt: integer sums
             v = 0;
-            for (uint32_t b = 0; b < nblocks; ++b) v += part[size_t(b) * W + i];
+            for (uint32_t b = b0; b < b1; ++b) v += part[size_t(b) * W + i];
         } else {
             const int kind = p.acc_kind[field - 1];
             v = acc_identity(kind);
-            for (uint32_t b = 0; b < nblocks; ++b) v = combine(kind, v, part[size_t(b) * W + i]);
+            for (uint32_t b = b0; b < b1; ++b) v = combine(kind, v, part[size_t(b) * W + i]);
         }
-        st[i] = v;
+        out[size_t(c) * W + i] = v;
     }
 }
 
 void merge_partials(qe_gpu_ctx* ctx, const uint64_t* part, uint32_t nblocks, uint64_t* st,
                     uint64_t S, const AggProg& p) {
     const uint64_t W = state_words(S, p);
-    merge_partials_kernel<<<grid_for(W, 256, 8192), 256, 0, ctx->stream>>>(part, nblocks, st, S, p);
+    constexpr uint32_t kChunk = 16;
+    if (nblocks > 2 * kChunk) {
+        const uint32_t nchunks = (nblocks + kChunk - 1) / kChunk;
+        Scratch mid(ctx, size_t(nchunks) * W * 8);
+        merge_partials_kernel<<<grid_for(W * nchunks, 256, 16384), 256, 0, ctx->stream>>>(
+            part, nblocks, kChunk, mid.as<uint64_t>(), S, p);
+        merge_partials_kernel<<<grid_for(W, 256, 16384), 256, 0, ctx->stream>>>(mid.as<uint64_t>(), nchunks,
+                                                                                  nchunks, st, S, p);
+    } else {
+        merge_partials_kernel<<<grid_for(W, 256, 16384), 256, 0, ctx->stream>>>(part, nblocks,
+                                                                                  nblocks ? nblocks : 1, st, S, p);
+    }
     QE_HIP(hipGetLastError());
 }
 

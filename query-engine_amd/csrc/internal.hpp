@@ -16,6 +16,7 @@
 namespace qe_gpu {
 
 constexpr int kWave = 64;
+constexpr int kMaxWaves = 16;  // waves in a 1024-thread block
 constexpr uint32_t kNoRow = 0xFFFFFFFFu;
 constexpr uint64_t kNullWord = 0x9E3779B97F4A7C15ull;  // NULL lane word (never compared alone)
 constexpr uint64_t kCanonNaN = 0x7ff8000000000000ull;
@@ -95,6 +96,8 @@ struct qe_gpu_ctx {
     };
     std::map<std::string, Timing> timings;
     int num_cus = 256;
+    // Tuning knobs (qe_gpu_set_tuning).
+    int tune_join_pairs = 4;  // row pairs per thread per tile in the streaming probe
     // Small pinned host staging buffer for scalars read back after kernels.
     void* pinned = nullptr;
     size_t pinned_bytes = 0;

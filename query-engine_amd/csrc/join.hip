@@ -435,6 +435,117 @@ __global__ void join_agg_direct_global(const KeyT* __restrict__ keys, const uint
     }
 }
 
+// ---- streaming probe: coalesced vector loads of key and value columns ----------------
+// Value columns an aggregate program reads, deduplicated (at most kStreamCols).
+constexpr int kStreamCols = 2;
+struct StreamCols {
+    int32_t wide[kStreamCols];  // 1: 8-byte values, 0: I32 (sign-extended)
+    const void* data[kStreamCols];
+    const uint64_t* validity[kStreamCols];  // nullptr when all-valid
+    int32_t vcol[kMaxAggs];                 // agg -> value column, -1 for COUNT(*)
+};
+
+typedef unsigned long long u64x2_t __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+
+// Rows 2*pr and 2*pr+1 of an 8-byte or 4-byte column in one coalesced load.
+__device__ inline void load_pair(const void* base, bool wide, uint64_t pr, uint64_t& a, uint64_t& b) {
+    if (wide) {
+        const u64x2_t v = __builtin_nontemporal_load(static_cast<const u64x2_t*>(base) + pr);
+        a = v.x;
+        b = v.y;
+    } else {
+        const u32x2_t v = __builtin_nontemporal_load(static_cast<const u32x2_t*>(base) + pr);
+        a = static_cast<uint64_t>(static_cast<int64_t>(static_cast<int32_t>(v.x)));
+        b = static_cast<uint64_t>(static_cast<int64_t>(static_cast<int32_t>(v.y)));
+    }
+}
+
+// Fused probe over a direct unique map with LDS partials. Each thread owns
+// PAIRS row pairs per tile; pair i of thread t is tile_base + i*blockDim + t, so
+// every load instruction of a wave reads one contiguous 1 KiB (8-byte column)
+// or 512 B (4-byte column) span. All key and value loads of a tile are issued
+// before the map lookups and LDS updates. NV = value columns read.
+template <typename KeyT, typename MapT, int PAIRS, int NV>
+__global__ void __launch_bounds__(1024) join_agg_stream(const KeyT* __restrict__ keys, const uint64_t* key_valid,
+                                                        uint64_t n, const MapT* __restrict__ gmap, int64_t kmin,
+                                                        uint64_t range, uint32_t S, AggProg p, StreamCols sc,
+                                                        uint64_t* part) {
+    extern __shared__ __align__(16) char smem[];
+    lds_init(smem, S, p);
+    __syncthreads();
+    constexpr MapT kEmpty = static_cast<MapT>(~MapT(0));
+    constexpr bool kWideKey = sizeof(KeyT) == 8;
+    const uint64_t npairs = n >> 1;
+    const uint64_t tile = uint64_t(blockDim.x) * PAIRS;
+    for (uint64_t t0 = blockIdx.x * tile; t0 < npairs; t0 += uint64_t(gridDim.x) * tile) {
+        uint64_t kw[2 * PAIRS];
+        uint64_t vb[NV > 0 ? NV : 1][2 * PAIRS];
+#pragma unroll
+        for (int i = 0; i < PAIRS; ++i) {
+            const uint64_t pr = t0 + uint64_t(i) * blockDim.x + threadIdx.x;
+            kw[2 * i] = kw[2 * i + 1] = 0;
+            if (pr < npairs) {
+                load_pair(keys, kWideKey, pr, kw[2 * i], kw[2 * i + 1]);
+#pragma unroll
+                for (int c = 0; c < NV; ++c) load_pair(sc.data[c], sc.wide[c] != 0, pr, vb[c][2 * i], vb[c][2 * i + 1]);
+            }
+        }
+        MapT g[2 * PAIRS];
+#pragma unroll
+        for (int i = 0; i < PAIRS; ++i) {
+            const uint64_t pr = t0 + uint64_t(i) * blockDim.x + threadIdx.x;
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                const uint64_t r = 2 * pr + e;
+                const uint64_t off = kw[2 * i + e] - static_cast<uint64_t>(kmin);
+                bool ok = pr < npairs && off < range;
+                if (key_valid) ok = ok && ((key_valid[r >> 6] >> (r & 63)) & 1ull);
+                g[2 * i + e] = ok ? gmap[off] : kEmpty;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < PAIRS; ++i) {
+            const uint64_t pr = t0 + uint64_t(i) * blockDim.x + threadIdx.x;
+#pragma unroll
+            for (int e = 0; e < 2; ++e) {
+                if (g[2 * i + e] == kEmpty) continue;
+                const uint64_t r = 2 * pr + e;
+                uint64_t bits[NV > 0 ? NV : 1];
+                uint32_t valid = 0;
+#pragma unroll
+                for (int c = 0; c < NV; ++c) {
+                    bits[c] = vb[c][2 * i + e];
+                    const bool v = sc.validity[c] == nullptr || ((sc.validity[c][r >> 6] >> (r & 63)) & 1ull);
+                    valid |= static_cast<uint32_t>(v) << c;
+                }
+                lds_update_vals(smem, S, p, static_cast<uint32_t>(g[2 * i + e]), sc.vcol, bits, valid);
+            }
+        }
+    }
+    if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {  // last row of an odd-length column
+        const uint64_t r = n - 1;
+        const uint64_t off = static_cast<uint64_t>(static_cast<int64_t>(keys[r])) - static_cast<uint64_t>(kmin);
+        bool ok = off < range;
+        if (key_valid) ok = ok && ((key_valid[r >> 6] >> (r & 63)) & 1ull);
+        const MapT gg = ok ? gmap[off] : kEmpty;
+        if (gg != kEmpty) {
+            uint64_t bits[NV > 0 ? NV : 1];
+            uint32_t valid = 0;
+#pragma unroll
+            for (int c = 0; c < NV; ++c) {
+                bits[c] = sc.wide[c] ? static_cast<const uint64_t*>(sc.data[c])[r]
+                                     : static_cast<uint64_t>(static_cast<int64_t>(static_cast<const int32_t*>(sc.data[c])[r]));
+                const bool v = sc.validity[c] == nullptr || ((sc.validity[c][r >> 6] >> (r & 63)) & 1ull);
+                valid |= static_cast<uint32_t>(v) << c;
+            }
+            lds_update_vals(smem, S, p, static_cast<uint32_t>(gg), sc.vcol, bits, valid);
+        }
+    }
+    __syncthreads();
+    lds_flush(smem, S, p, part);
+}
+
 __global__ void __launch_bounds__(1024) join_agg_lists_lds(ProbeAggLists L, uint64_t n, uint32_t S, AggProg p,
                                                            uint64_t* part) {
     extern __shared__ __align__(16) char smem[];
@@ -484,6 +595,170 @@ __global__ void gmap_fill_kernel(ColView key, uint64_t n, int64_t kmin, const ui
         if (!is_valid(key, r)) continue;
         gmap[static_cast<uint64_t>(int_value(key, r) - kmin)] = static_cast<MapT>(gslot[r]);
     }
+}
+
+// ---- direct unique map, built by key-range partitions ----------------------------
+// gmap[key - kmin] = group slot of the build row holding that key. Build rows
+// are first partitioned by the high bits of their key offset (2^pb keys per
+// partition), then one block per partition assembles its slice of the map in
+// LDS, where a bitmap flags a repeated key, and writes the slice out with
+// contiguous stores: no scattered 2-byte writes to HBM, no separate clear.
+constexpr uint32_t kMapTile = 16384;     // build rows per partitioning block (1024 threads)
+constexpr uint32_t kMaxMapParts = 4096;  // partitions (LDS cursors per block)
+
+__global__ void __launch_bounds__(1024) map_part_hist(ColView key, uint64_t n, int64_t kmin, uint32_t pb,
+                                                     uint32_t nparts, uint32_t ntiles, uint32_t* hist) {
+    extern __shared__ uint32_t hcnt[];
+    for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x) hcnt[i] = 0;
+    __syncthreads();
+    const uint64_t t0 = uint64_t(blockIdx.x) * kMapTile, t1 = min(n, t0 + kMapTile);
+    for (uint64_t r = t0 + threadIdx.x; r < t1; r += blockDim.x) {
+        if (!is_valid(key, r)) continue;
+        const uint64_t off = static_cast<uint64_t>(int_value(key, r) - kmin);
+        atomicAdd(&hcnt[off >> pb], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x) hist[size_t(i) * ntiles + blockIdx.x] = hcnt[i];
+}
+
+__global__ void __launch_bounds__(1024) map_part_scatter(ColView key, uint64_t n, int64_t kmin, uint32_t pb,
+                                                        uint32_t nparts, uint32_t ntiles, const uint64_t* offs,
+                                                        const uint32_t* gslot, uint64_t* ent) {
+    extern __shared__ __align__(8) char msm[];
+    uint64_t* base = reinterpret_cast<uint64_t*>(msm);
+    uint32_t* cur = reinterpret_cast<uint32_t*>(base + nparts);
+    for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x) {
+        base[i] = offs[size_t(i) * ntiles + blockIdx.x];
+        cur[i] = 0;
+    }
+    __syncthreads();
+    const uint64_t mask = (uint64_t(1) << pb) - 1;
+    const uint64_t t0 = uint64_t(blockIdx.x) * kMapTile, t1 = min(n, t0 + kMapTile);
+    for (uint64_t r = t0 + threadIdx.x; r < t1; r += blockDim.x) {
+        if (!is_valid(key, r)) continue;
+        const uint64_t off = static_cast<uint64_t>(int_value(key, r) - kmin);
+        const uint32_t part = static_cast<uint32_t>(off >> pb);
+        const uint32_t l = atomicAdd(&cur[part], 1u);
+        ent[base[part] + l] = ((off & mask) << 32) | gslot[r];
+    }
+}
+
+template <typename MapT>
+__global__ void __launch_bounds__(1024) map_part_build(const uint64_t* ent, const uint64_t* offs, uint32_t ntiles,
+                                                       uint32_t nparts, uint64_t total, uint32_t pb, uint64_t range,
+                                                       MapT* gmap, unsigned int* dup) {
+    extern __shared__ __align__(16) char msm[];
+    const uint32_t P = 1u << pb;
+    MapT* m = reinterpret_cast<MapT*>(msm);
+    uint32_t* bm = reinterpret_cast<uint32_t*>(msm + size_t(P) * sizeof(MapT));
+    for (uint32_t i = threadIdx.x; i < P; i += blockDim.x) m[i] = static_cast<MapT>(~MapT(0));
+    for (uint32_t i = threadIdx.x; i < P / 32; i += blockDim.x) bm[i] = 0;
+    __syncthreads();
+    const uint32_t part = blockIdx.x;
+    const uint64_t b = offs[size_t(part) * ntiles];
+    const uint64_t e = (part + 1 < nparts) ? offs[size_t(part + 1) * ntiles] : total;
+    bool d = false;
+    for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
+        const uint64_t v = ent[i];
+        const uint32_t o = static_cast<uint32_t>(v >> 32);
+        const uint32_t bit = 1u << (o & 31);
+        if (atomicOr(&bm[o >> 5], bit) & bit) d = true;
+        m[o] = static_cast<MapT>(static_cast<uint32_t>(v));
+    }
+    if (d) atomicOr(dup, 1u);
+    __syncthreads();
+    const uint64_t lo = uint64_t(part) << pb;
+    const uint64_t cnt = min<uint64_t>(P, range - lo);
+    for (uint64_t i = threadIdx.x; i < cnt; i += blockDim.x) gmap[lo + i] = m[i];
+}
+
+enum class MapBuild { kOk, kDuplicate, kTooWide };
+
+template <typename MapT>
+MapBuild build_direct_map(qe_gpu_ctx* ctx, const qe_dev_col* key, uint64_t nb, int64_t kmin, uint64_t range,
+                          const uint32_t* gslot, Scratch& gmap) {
+    const uint32_t pb = sizeof(MapT) == 2 ? 16 : 15;  // 128 KiB of map per partition in LDS
+    const uint64_t np64 = (range + (uint64_t(1) << pb) - 1) >> pb;
+    if (np64 == 0 || np64 > kMaxMapParts) return MapBuild::kTooWide;
+    const uint32_t nparts = static_cast<uint32_t>(np64);
+    const uint32_t ntiles = static_cast<uint32_t>((nb + kMapTile - 1) / kMapTile);
+    const uint64_t nh = uint64_t(nparts) * ntiles;
+    Scratch hist(ctx, nh * 4), offs(ctx, nh * 8), dup(ctx, 4);
+    const ColView kv = view(key);
+    map_part_hist<<<ntiles, 1024, nparts * 4, ctx->stream>>>(kv, nb, kmin, pb, nparts, ntiles, hist.as<uint32_t>());
+    QE_HIP(hipGetLastError());
+    const uint64_t total = exclusive_scan_u32(ctx, hist.as<uint32_t>(), offs.as<uint64_t>(), nh);
+    Scratch ent(ctx, std::max<uint64_t>(total, 1) * 8);
+    map_part_scatter<<<ntiles, 1024, nparts * 12, ctx->stream>>>(kv, nb, kmin, pb, nparts, ntiles,
+                                                                 offs.as<uint64_t>(), gslot, ent.as<uint64_t>());
+    QE_HIP(hipGetLastError());
+    gmap = Scratch(ctx, range * sizeof(MapT));
+    memset_async(ctx, dup.p, 0, 4);
+    const size_t lds = (size_t(1) << pb) * sizeof(MapT) + ((size_t(1) << pb) / 32) * 4;
+    map_part_build<MapT><<<nparts, 1024, lds, ctx->stream>>>(ent.as<uint64_t>(), offs.as<uint64_t>(), ntiles, nparts,
+                                                             total, pb, range, gmap.as<MapT>(),
+                                                             dup.as<unsigned int>());
+    QE_HIP(hipGetLastError());
+    return read_scalar(ctx, dup.as<unsigned int>()) ? MapBuild::kDuplicate : MapBuild::kOk;
+}
+
+// Value columns of a program for the streaming probe: I32/I64/TS/F64 inputs,
+// at most kStreamCols distinct columns, every column (and the key column)
+// aligned for its pair loads.
+bool stream_cols(const AggProg& p, const void* keys, size_t key_width, StreamCols& sc, int& nv) {
+    auto aligned = [](const void* d, size_t w) { return (reinterpret_cast<uintptr_t>(d) % (2 * w)) == 0; };
+    if (!aligned(keys, key_width)) return false;
+    nv = 0;
+    for (int a = 0; a < p.n; ++a) {
+        sc.vcol[a] = -1;
+        if (p.kind[a] == K_COUNT_STAR) continue;
+        const ColView& v = p.in[a];
+        const bool wide = v.type == QE_I64 || v.type == QE_TS || v.type == QE_F64;
+        if (!wide && v.type != QE_I32) return false;
+        int c = 0;
+        while (c < nv && sc.data[c] != v.data) ++c;
+        if (c == nv) {
+            if (nv == kStreamCols || !aligned(v.data, wide ? 8 : 4)) return false;
+            sc.data[c] = v.data;
+            sc.validity[c] = v.validity;
+            sc.wide[c] = wide ? 1 : 0;
+            ++nv;
+        }
+        sc.vcol[a] = c;
+    }
+    return true;
+}
+
+template <typename KeyT, typename MapT, int PAIRS>
+void launch_stream_p(qe_gpu_ctx* ctx, int nv, unsigned blocks, size_t lb, const KeyT* keys, const uint64_t* kv,
+                     uint64_t n, const MapT* gm, int64_t kmin, uint64_t range, uint32_t S, const AggProg& p,
+                     const StreamCols& sc, uint64_t* part) {
+    switch (nv) {
+        case 0:
+            join_agg_stream<KeyT, MapT, PAIRS, 0><<<blocks, 1024, lb, ctx->stream>>>(keys, kv, n, gm, kmin, range, S,
+                                                                                     p, sc, part);
+            break;
+        case 1:
+            join_agg_stream<KeyT, MapT, PAIRS, 1><<<blocks, 1024, lb, ctx->stream>>>(keys, kv, n, gm, kmin, range, S,
+                                                                                     p, sc, part);
+            break;
+        default:
+            join_agg_stream<KeyT, MapT, PAIRS, 2><<<blocks, 1024, lb, ctx->stream>>>(keys, kv, n, gm, kmin, range, S,
+                                                                                     p, sc, part);
+            break;
+    }
+    QE_HIP(hipGetLastError());
+}
+
+// Row pairs per thread per tile: 2 or 4 (tuning knob QE_TUNE_JOIN_PAIRS).
+template <typename KeyT, typename MapT>
+void launch_stream(qe_gpu_ctx* ctx, int pairs, int nv, unsigned blocks, size_t lb, const KeyT* keys,
+                   const uint64_t* kv, uint64_t n, const MapT* gm, int64_t kmin, uint64_t range, uint32_t S,
+                   const AggProg& p, const StreamCols& sc, uint64_t* part) {
+    if (pairs <= 2)
+        launch_stream_p<KeyT, MapT, 2>(ctx, nv, blocks, lb, keys, kv, n, gm, kmin, range, S, p, sc, part);
+    else
+        launch_stream_p<KeyT, MapT, 4>(ctx, nv, blocks, lb, keys, kv, n, gm, kmin, range, S, p, sc, part);
 }
 
 }  // namespace qe_gpu
@@ -635,44 +910,40 @@ qe_gpu_status qe_gpu_join_aggregate(qe_gpu_ctx* ctx, const qe_dev_col* const* pr
             ke.nullmask = gtab.v.nullmask;
         }
 
-        JoinIndex ix;
-        if (nb > 0) build_index(ctx, ix, bk.data(), nkeys, nb);
-        else ix.empty = true;
-
         Scratch st(ctx, state_words(S, p) * 8);
         const size_t lb = lds_bytes(static_cast<uint32_t>(std::min<uint64_t>(S, UINT32_MAX)), p);
         const bool use_lds = S < (uint64_t(1) << 20) && lb <= kLdsBudget;
         const unsigned lds_blocks = static_cast<unsigned>(
             std::max<uint64_t>(1, std::min<uint64_t>(ctx->num_cus, (np + 1023) / 1024)));
+        const bool small = S < 0xFFFF;
+        const uint64_t* kv = pk[0]->all_valid ? nullptr : pk[0]->validity;
+        const bool int_key = nkeys == 1 && (pk[0]->type == QE_I64 || pk[0]->type == QE_TS || pk[0]->type == QE_I32);
 
-        if (ix.empty || np == 0) {
-            init_state(ctx, st.as<uint64_t>(), S, p);
-        } else if (ix.mode == 0 && ix.unique && (pk[0]->type == QE_I64 || pk[0]->type == QE_TS ||
-                                                 pk[0]->type == QE_I32)) {
-            // Direct unique index: the map stores each key's group slot.
-            const bool small = S < 0xFFFF;
-            const size_t mb = ix.range * (small ? 2 : 4);
-            Scratch gmap(ctx, mb);
-            memset_async(ctx, gmap.p, 0xFF, mb);
-            if (small)
-                gmap_fill_kernel<uint16_t><<<grid_for(nb, 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(
-                    view(bk[0]), nb, ix.kmin, gslot.as<uint32_t>(), gmap.as<uint16_t>());
-            else
-                gmap_fill_kernel<uint32_t><<<grid_for(nb, 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(
-                    view(bk[0]), nb, ix.kmin, gslot.as<uint32_t>(), gmap.as<uint32_t>());
-            QE_HIP(hipGetLastError());
-            const uint64_t* kv = pk[0]->all_valid ? nullptr : pk[0]->validity;
+        // Probe over a direct unique map (gmap[key - kmin] = group slot).
+        auto probe_direct = [&](const Scratch& gmap, int64_t kmin, uint64_t range) {
             auto launch = [&](auto key_tag, auto map_tag) {
                 using KeyT = decltype(key_tag);
                 using MapT = decltype(map_tag);
                 const KeyT* keys = static_cast<const KeyT*>(pk[0]->data);
                 const MapT* gm = static_cast<const MapT*>(gmap.p);
-                if (use_lds) {
+                StreamCols sc{};
+                int nv = 0;
+                const bool streamable = use_lds && stream_cols(p, keys, sizeof(KeyT), sc, nv);
+                if (streamable) {
+                    Scratch part(ctx, size_t(lds_blocks) * part_words(static_cast<uint32_t>(S), p) * 8);
+                    {
+                        TimedRegion tr(ctx, "join_agg_probe");
+                        launch_stream<KeyT, MapT>(ctx, ctx->tune_join_pairs, nv, lds_blocks, lb, keys, kv, np, gm,
+                                                  kmin, range, static_cast<uint32_t>(S), p, sc,
+                                                  part.as<uint64_t>());
+                    }
+                    merge_partials(ctx, part.as<uint64_t>(), lds_blocks, st.as<uint64_t>(), S, p);
+                } else if (use_lds) {
                     Scratch part(ctx, size_t(lds_blocks) * part_words(static_cast<uint32_t>(S), p) * 8);
                     {
                         TimedRegion tr(ctx, "join_agg_probe");
                         join_agg_direct_lds<KeyT, MapT><<<lds_blocks, 1024, lb, ctx->stream>>>(
-                            keys, kv, np, gm, ix.kmin, ix.range, static_cast<uint32_t>(S), p, part.as<uint64_t>());
+                            keys, kv, np, gm, kmin, range, static_cast<uint32_t>(S), p, part.as<uint64_t>());
                         QE_HIP(hipGetLastError());
                     }
                     merge_partials(ctx, part.as<uint64_t>(), lds_blocks, st.as<uint64_t>(), S, p);
@@ -680,7 +951,7 @@ qe_gpu_status qe_gpu_join_aggregate(qe_gpu_ctx* ctx, const qe_dev_col* const* pr
                     init_state(ctx, st.as<uint64_t>(), S, p);
                     TimedRegion tr(ctx, "join_agg_probe");
                     join_agg_direct_global<KeyT, MapT><<<grid_for(np, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(
-                        keys, kv, np, gm, ix.kmin, ix.range, S, p, st.as<uint64_t>());
+                        keys, kv, np, gm, kmin, range, S, p, st.as<uint64_t>());
                     QE_HIP(hipGetLastError());
                 }
             };
@@ -691,6 +962,53 @@ qe_gpu_status qe_gpu_join_aggregate(qe_gpu_ctx* ctx, const qe_dev_col* const* pr
                 if (small) launch(int64_t(0), uint16_t(0));
                 else launch(int64_t(0), uint32_t(0));
             }
+        };
+
+        // Dense unique integer keys: partitioned map build, then the streaming probe.
+        bool done = false;
+        if (int_key && nb > 0 && np > 0) {
+            const MinMax mm = column_minmax(ctx, bk[0], nb);
+            if (mm.valid == 0) {  // every build key NULL: nothing matches
+                init_state(ctx, st.as<uint64_t>(), S, p);
+                done = true;
+            } else {
+                const uint64_t R = static_cast<uint64_t>(mm.mx) - static_cast<uint64_t>(mm.mn) + 1;
+                if (R <= std::max<uint64_t>(4 * nb, uint64_t(1) << 16) && R < (uint64_t(1) << 31)) {
+                    Scratch gmap;
+                    const MapBuild mbld =
+                        small ? build_direct_map<uint16_t>(ctx, bk[0], nb, mm.mn, R, gslot.as<uint32_t>(), gmap)
+                              : build_direct_map<uint32_t>(ctx, bk[0], nb, mm.mn, R, gslot.as<uint32_t>(), gmap);
+                    if (mbld == MapBuild::kOk) {
+                        probe_direct(gmap, mm.mn, R);
+                        done = true;
+                    }
+                }
+            }
+        }
+
+        JoinIndex ix;
+        if (done) {
+        } else if (nb > 0) {
+            build_index(ctx, ix, bk.data(), nkeys, nb);
+        } else {
+            ix.empty = true;
+        }
+        if (done) {
+        } else if (ix.empty || np == 0) {
+            init_state(ctx, st.as<uint64_t>(), S, p);
+        } else if (ix.mode == 0 && ix.unique && int_key) {
+            // Direct unique index too wide for the partitioned build.
+            const size_t mb = ix.range * (small ? 2 : 4);
+            Scratch gmap(ctx, mb);
+            memset_async(ctx, gmap.p, 0xFF, mb);
+            if (small)
+                gmap_fill_kernel<uint16_t><<<grid_for(nb, 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(
+                    view(bk[0]), nb, ix.kmin, gslot.as<uint32_t>(), gmap.as<uint16_t>());
+            else
+                gmap_fill_kernel<uint32_t><<<grid_for(nb, 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(
+                    view(bk[0]), nb, ix.kmin, gslot.as<uint32_t>(), gmap.as<uint32_t>());
+            QE_HIP(hipGetLastError());
+            probe_direct(gmap, ix.kmin, ix.range);
         } else {
             ProbeAggLists L{ix.view(pk.data(), static_cast<int>(nkeys)), gslot.as<uint32_t>()};
             if (use_lds) {

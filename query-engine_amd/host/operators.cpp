@@ -412,7 +412,7 @@ void GpuAsofJoin::open() {
     try {
         int32_t mode = QE_ASOF_BINARY;
         if (method_ == tsx::AsofMethod::Learned) {
-            QE_CALL(qe_gpu_asof_train(ctx, idx, 64, 150, 0.01f));
+            QE_CALL(qe_gpu_asof_train(ctx, idx, 64, 300, 0.003f));
             mode = QE_ASOF_LEARNED;
         }
         QE_CALL(qe_gpu_malloc(ctx, std::max<size_t>(n, 1) * 4, &match));

@@ -170,6 +170,37 @@ def test_fused_join_groupby_small_config2_shape(ctx):
     assert 0.85 * (1 << 20) < matched < 0.95 * (1 << 20)
 
 
+@pytest.mark.parametrize("pairs", [2, 4])
+@pytest.mark.parametrize("case", range(8))
+def test_fused_direct_stream_shapes(ctx, qe, pairs, case):
+    # Direct unique build map with the streaming probe: odd and tiny probe
+    # lengths (tail row), I32 / I64 keys, NULL probe keys, narrow and nullable
+    # value columns, 0..3 distinct value columns (3 takes the per-row kernel).
+    r = np.random.default_rng(900 + case)
+    kt = I32 if case % 2 else I64
+    n_b = [1, 64, 3000, 70000][case % 4]
+    n_p = [1, 3, 4097, 100001, 65536, 262143, 2, 5][case]
+    keys = r.permutation(2 * n_b)[:n_b] - n_b  # unique, negative and positive, with gaps
+    bcols = [col(keys, kt), col(r.integers(0, 37, n_b), I32)]
+    knull = tuple(np.nonzero(r.random(n_p) < 0.03)[0]) if case % 3 == 0 else ()
+    vnull = tuple(np.nonzero(r.random(n_p) < 0.05)[0]) if case % 2 else ()
+    pcols = [col(r.integers(-n_b - 5, n_b + 5, n_p), kt, knull),
+             col(r.integers(-1000, 1000, n_p), I32, vnull),
+             col(r.normal(size=n_p) * 100, F64, vnull),
+             col(r.integers(-2**40, 2**40, n_p), I64)]
+    aggsets = [[(CS, 0)],
+               [(SUM, 2), (CS, 0)],
+               [(SUM, 1), (MIN, 1), (MAX, 1), (CNT, 1), (AVG, 1)],
+               [(SUM, 2), (MAX, 2), (MIN, 3), (SUM, 3), (CS, 0)],
+               [(SUM, 1), (SUM, 2), (SUM, 3)]]
+    ctx.set_tuning(qe.TUNE_JOIN_PAIRS, pairs)
+    try:
+        for aggs in aggsets:
+            join_agg_case(ctx, pcols, bcols, [0], [0], INNER, [5], aggs)
+    finally:
+        ctx.set_tuning(qe.TUNE_JOIN_PAIRS, 4)
+
+
 def test_fused_duplicate_build_keys_list_path(ctx, rng):
     n_b, n_p = 5000, 50000
     pcols = [col(rng.integers(0, 2000, n_p), I64), col(rng.uniform(0, 10, n_p), F64, nulls=(3, 5, 7))]
