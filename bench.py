@@ -156,12 +156,14 @@ def asof_workload(qe, ctx, args):
     idx = ctx.asof_build([rs], rt)
     ctx.sync()
     t1 = time.perf_counter()
+    ctx.set_tuning(qe.TUNE_ASOF_BUCKET_KNOTS, args.bucket_knots)
     idx.train(args.piece_rows, args.train_steps, args.train_lr)
+    ctx.set_tuning(qe.TUNE_ASOF_BUCKET_KNOTS, 1)
     ctx.sync()
     t2 = time.perf_counter()
     out = qe.DevBuffer(ctx, 4 * ln)
     res = {"right_rows": rn, "left_rows": ln, "build_sort_s": t1 - t0, "train_s": t2 - t1,
-           "piece_rows": args.piece_rows, "train_steps": args.train_steps}
+           "piece_rows": args.piece_rows, "train_steps": args.train_steps, "bucket_knots": args.bucket_knots}
     results = {}
     for name, mode, timer in (("binary", qe.ASOF_BINARY, "asof_probe_binary"),
                               ("learned", qe.ASOF_LEARNED, "asof_probe_learned"),
@@ -340,6 +342,7 @@ def main():
     ap.add_argument("--piece-rows", type=int, default=64)
     ap.add_argument("--train-steps", type=int, default=300)
     ap.add_argument("--train-lr", type=float, default=0.003)
+    ap.add_argument("--bucket-knots", type=int, default=1, help="spline knots per learned-router bucket")
     ap.add_argument("--asof-sessions", type=int, default=0,
                     help="0: bursty exponential gaps (the tick shape); >0: trading days with U-shaped activity")
     ap.add_argument("--cpu-asof-log2", type=int, default=22)

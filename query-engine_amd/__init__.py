@@ -83,6 +83,7 @@ _SIGS = {
     "qe_gpu_set_mem_limit": (C.c_int, [P, C.c_uint64]),
     "qe_gpu_set_timing": (C.c_int, [P, C.c_int]),
     "qe_gpu_set_tuning": (C.c_int, [P, C.c_int32, C.c_int64]),
+    "qe_gpu_xcd_of_blocks": (C.c_int, [P, C.c_uint32, P]),
     "qe_gpu_timing": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     "qe_gpu_column_upload": (C.c_int, [P, C.POINTER(ColDesc), C.POINTER(P)]),
     "qe_gpu_column_alloc": (C.c_int, [P, C.c_int32, C.c_uint64, C.c_int32, C.POINTER(P)]),
@@ -353,6 +354,11 @@ class Context:
     def set_timing(self, on: bool):
         self._check(lib().qe_gpu_set_timing(self.h, 1 if on else 0))
 
+    def xcd_of_blocks(self, nblocks: int) -> np.ndarray:
+        out = np.empty(nblocks, dtype=np.uint32)
+        self._check(lib().qe_gpu_xcd_of_blocks(self.h, nblocks, _ptr(out)))
+        return out
+
     def set_tuning(self, key: int, value: int):
         self._check(lib().qe_gpu_set_tuning(self.h, key, value))
 
@@ -559,7 +565,7 @@ class BatchTuning(C.Structure):
     _fields_ = [("tile_rows", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("lds_slots", C.c_uint32)]
 
 
-TUNE_JOIN_PAIRS = 0
+TUNE_JOIN_PAIRS, TUNE_ASOF_BUCKET_KNOTS = 0, 1
 BATCH_PERSISTENT, BATCH_PER_BATCH, BATCH_GRAPH = 0, 1, 2
 
 

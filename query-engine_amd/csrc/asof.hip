@@ -71,7 +71,7 @@ struct qe_dev_asof {
     Scratch ts, row, offs;    // sorted timestamps, original rows, CSR (nparts+1)
     // learned index
     bool trained = false;
-    uint32_t B = 0;
+    uint32_t B = 0, kpb = 1;  // rows per piece, knots per router bucket
     uint64_t nknots = 0, ntab = 0;
     Scratch meta, knots, tab_mlp, tab_lin, params, grid;
     double final_loss = 0, mean_knots_mlp = 0, mean_knots_lin = 0;
@@ -286,16 +286,19 @@ __device__ inline uint32_t bucket_of(const PartMeta& pm, const float* W, bool ml
     return static_cast<uint32_t>(b);
 }
 
-__global__ void meta_counts_kernel(const uint64_t* offs, uint64_t nparts, uint32_t B, uint32_t* K, uint32_t* R1) {
+// Knots K = ceil(n / B) and buckets R = ceil(K / kpb) per partition.
+__device__ inline uint32_t buckets_for(uint32_t k, uint32_t kpb) { return (k + kpb - 1) / kpb; }
+__global__ void meta_counts_kernel(const uint64_t* offs, uint64_t nparts, uint32_t B, uint32_t kpb, uint32_t* K,
+                                   uint32_t* R1) {
     for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < nparts;
          p += uint64_t(gridDim.x) * blockDim.x) {
         const uint64_t n = offs[p + 1] - offs[p];
         const uint32_t k = static_cast<uint32_t>((n + B - 1) / B);
         K[p] = k;
-        R1[p] = k ? k + 1 : 0;  // R = K buckets -> K+1 table entries
+        R1[p] = k ? buckets_for(k, kpb) + 1 : 0;  // R buckets -> R+1 table entries
     }
 }
-__global__ void meta_fill_kernel(const uint64_t* offs, uint64_t nparts, uint32_t B, const uint32_t* K,
+__global__ void meta_fill_kernel(const uint64_t* offs, uint64_t nparts, uint32_t B, uint32_t kpb, const uint32_t* K,
                                  const uint64_t* kb, const uint64_t* tb, const int64_t* ts, PartMeta* meta) {
     for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < nparts;
          p += uint64_t(gridDim.x) * blockDim.x) {
@@ -305,7 +308,7 @@ __global__ void meta_fill_kernel(const uint64_t* offs, uint64_t nparts, uint32_t
         m.K = K[p];
         m.kb = kb[p];
         m.tb = tb[p];
-        m.R = K[p];
+        m.R = buckets_for(K[p], kpb);
         m.mlp_ok = K[p] >= kMinKnotsForMlp;
         m.mlp = 0;
         if (m.K) {
@@ -914,8 +917,10 @@ qe_gpu_status qe_gpu_asof_train(qe_gpu_ctx* ctx, qe_dev_asof* idx, uint32_t piec
         const uint64_t P = idx->nparts;
         idx->B = piece_rows;
         Scratch K(ctx, P * 4), R1(ctx, P * 4), kb(ctx, (P + 1) * 8), tb(ctx, (P + 1) * 8);
+        const uint32_t kpb = static_cast<uint32_t>(ctx->tune_asof_bucket_knots);
+        idx->kpb = kpb;
         meta_counts_kernel<<<grid_for(P, 256, 4096), 256, 0, ctx->stream>>>(idx->offs.as<uint64_t>(), P, piece_rows,
-                                                                            K.as<uint32_t>(), R1.as<uint32_t>());
+                                                                            kpb, K.as<uint32_t>(), R1.as<uint32_t>());
         QE_HIP(hipGetLastError());
         const uint64_t nk = exclusive_scan_u32(ctx, K.as<uint32_t>(), kb.as<uint64_t>(), P);
         const uint64_t nt = exclusive_scan_u32(ctx, R1.as<uint32_t>(), tb.as<uint64_t>(), P);
@@ -924,7 +929,7 @@ qe_gpu_status qe_gpu_asof_train(qe_gpu_ctx* ctx, qe_dev_asof* idx, uint32_t piec
         idx->ntab = nt;
         idx->meta = Scratch(ctx, P * sizeof(PartMeta));
         meta_fill_kernel<<<grid_for(P, 256, 4096), 256, 0, ctx->stream>>>(
-            idx->offs.as<uint64_t>(), P, piece_rows, K.as<uint32_t>(), kb.as<uint64_t>(), tb.as<uint64_t>(),
+            idx->offs.as<uint64_t>(), P, piece_rows, kpb, K.as<uint32_t>(), kb.as<uint64_t>(), tb.as<uint64_t>(),
             idx->ts.as<int64_t>(), idx->meta.as<PartMeta>());
         QE_HIP(hipGetLastError());
         const uint64_t nk1 = std::max<uint64_t>(nk, 1);

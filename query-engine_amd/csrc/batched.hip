@@ -1,21 +1,27 @@
 // Batched group-by: many independent GROUP BY queries, each over its own
-// morsels of uneven size, in one persistent launch (the engine's morsel
-// driver, exec/parallel.cpp:452-480, fans morsels of one query over CPU
-// threads; here every query's morsels share the GPU at once).
+// morsels of uneven size, in a handful of persistent launches or one hipGraph
+// (the engine's morsel driver, exec/parallel.cpp:452-480, fans the morsels of
+// one query over CPU threads; here every query's morsels share the GPU at once).
 //
-// Work = tiles of `tile_rows` rows cut from every morsel of every query, in a
-// device queue. A persistent grid (blocks_per_cu blocks per CU) pulls tiles
-// with one atomic per tile; every block exits when the queue is drained. Each
-// row probes its query's device hash table (the shared open-addressing table,
-// NULL keys group together) and folds into that slot's state; a per-block LDS
-// cache of `lds_slots` slot entries absorbs repeated groups (small-cardinality
-// queries) and is flushed with one HBM atomic per cached field at tile end.
-//
-// Modes: persistent (one launch), per-batch (one launch per 2048-row batch,
-// the launch granularity of the pull engine's batches, as the baseline), and
-// graph (the persistent pipeline captured once in a hipGraph and replayed).
-// The autotuner times tile size x blocks per CU x LDS cache size on the actual
-// batch and keeps the fastest.
+// Each query picks a strategy at prepare time from its key column:
+//  * small : integer keys whose value range fits a dense LDS table of at most
+//            kSmallLds bytes; every tile aggregates into that table (slot =
+//            key - min, NULL key in the last slot) and folds it into the
+//            query's state with one atomic per touched slot field;
+//  * part  : integer keys with a larger dense range and aggregates over the
+//            value column; rows are radix-partitioned by slot (2^pbits slots
+//            per partition: a histogram pass, one scan over every such query,
+//            a scatter pass), then one block per partition aggregates it in LDS
+//            and writes its slots' final state: no atomics, the same result on
+//            every run;
+//  * hash  : anything else (sparse ranges, aggregates over the key); rows probe
+//            the shared device hash table, with an optional LDS slot cache.
+// Each phase pulls work from one device queue shared by all queries; every
+// block leaves when the queue is empty. Modes: persistent (the phase
+// launches), graph (the same pipeline captured once in a hipGraph and
+// replayed), per-batch (the baseline: one launch per 2048-row batch; direct
+// slots take global atomics, hash queries go through the table). The
+// autotuner times tile size x blocks per CU x LDS cache on the actual batch.
 #include "agg_common.hpp"
 #include "hash_common.hpp"
 
@@ -27,23 +33,58 @@ using namespace qe_gpu;
 
 namespace qe_gpu {
 
+bool is_int_like(int t);
+
 constexpr int kBatchMaxAcc = 6;
 constexpr uint32_t kEmptyTag = 0xFFFFFFFFu;
+constexpr size_t kSmallLds = 32 * 1024;  // LDS of a small query's dense table
+constexpr size_t kPartLds = 128 * 1024;  // LDS of one partition's table (1024-thread block)
+constexpr uint32_t kMaxParts = 8192;     // partitions of one query
+constexpr uint32_t kPartAggThreads = 1024;
+
+enum BatchKind : int32_t { BK_SMALL = 0, BK_PART = 1, BK_HASH = 2 };
 
 struct QueryDev {
-    HashKeys hk;
-    HashTableView t;
-    uint64_t* st;  // state, compact layout, S = t.cap slots
+    HashKeys hk;      // hk.key[0] = key column (every kind)
+    HashTableView t;  // hash kind
+    ColView val;      // value column
+    uint64_t* st;     // state, compact layout over S slots
     AggProg p;
+    int32_t kind;
+    uint32_t pbits;   // part: 2^pbits slots per partition
+    int64_t kmin;     // small / part: slot = key - kmin, NULL key -> S - 1
+    uint64_t S;       // slots (hash: table capacity)
+    uint32_t nparts;  // part
+    uint32_t ntiles;  // part: tiles of this query (histogram columns)
+    uint64_t hbase;   // part: first histogram entry, laid out [part][tile]
+    uint64_t ebase;   // part: first partitioned entry
+    uint64_t rows;
 };
 
 struct Tile {
     uint32_t q;
-    uint32_t pad;
+    uint32_t qt;  // tile index within its query
     uint64_t begin, end;
 };
+struct PartItem {
+    uint32_t q;
+    uint32_t part;
+};
 
-// LDS layout for lds_slots = C: tag u32[C] | rows u32[C] | acc u64[kBatchMaxAcc][C] | cnt u32[kBatchMaxAcc][C]
+__device__ inline uint32_t direct_slot(const QueryDev& q, uint64_t r) {
+    const ColView& k = q.hk.key[0];
+    if (!is_valid(k, r)) return static_cast<uint32_t>(q.S - 1);
+    return static_cast<uint32_t>(static_cast<uint64_t>(int_value(k, r) - q.kmin));
+}
+
+// 64-bit pattern of a value: integers sign-extended, F64 raw.
+__device__ inline uint64_t value_bits(const ColView& v, uint64_t r) {
+    if (v.type == QE_F64) return static_cast<const uint64_t*>(v.data)[r];
+    return static_cast<uint64_t>(int_value(v, r));
+}
+
+// ---- LDS slot cache of the hash kind ------------------------------------------------------
+// Layout for C entries: tag u32[C] | rows u32[C] | acc u64[kBatchMaxAcc][C] | cnt u32[kBatchMaxAcc][C]
 __host__ __device__ inline size_t cache_bytes(uint32_t C) {
     return size_t(C) * (4 + 4 + 8 * kBatchMaxAcc + 4 * kBatchMaxAcc);
 }
@@ -137,7 +178,7 @@ __device__ inline void combine_atomic(int kind, uint64_t* dst, uint64_t v) {
 
 __device__ inline void cache_flush(char* smem, uint32_t C, const QueryDev& q) {
     const AggProg& p = q.p;
-    const uint64_t S = q.t.cap;
+    const uint64_t S = q.S;
     uint32_t* tag = reinterpret_cast<uint32_t*>(smem);
     uint32_t* rows = tag + C;
     uint64_t* acc = reinterpret_cast<uint64_t*>(smem + size_t(8) * C);
@@ -153,7 +194,7 @@ __device__ inline void cache_flush(char* smem, uint32_t C, const QueryDev& q) {
     }
 }
 
-__device__ inline void process_tile(const QueryDev& q, const Tile& tile, char* smem, uint32_t C) {
+__device__ inline void hash_tile(const QueryDev& q, const Tile& tile, char* smem, uint32_t C) {
     uint64_t vrow[kMaxAggs];
     for (uint64_t r = tile.begin + threadIdx.x; r < tile.end; r += blockDim.x) {
         uint64_t w[kMaxKeys], nm, h;
@@ -162,14 +203,106 @@ __device__ inline void process_tile(const QueryDev& q, const Tile& tile, char* s
         if (s == kNoSlot) continue;  // overflow reported through the table flag
         if (C && cache_update(smem, C, q.p, s, r)) continue;
         for (int a = 0; a < q.p.n; ++a) vrow[a] = r;
-        global_update(q.st, q.t.cap, q.p, s, vrow);
+        global_update(q.st, q.S, q.p, s, vrow);
     }
 }
 
-// Persistent: blocks pull tiles until the queue is empty (every block reaches
-// the exit test each iteration).
-__global__ void __launch_bounds__(256) batch_persistent_kernel(const QueryDev* qs, const Tile* tiles, uint64_t ntiles,
-                                                               unsigned long long* next, uint32_t C) {
+// Small kind: dense LDS table over the query's S slots, folded into HBM with
+// one atomic per touched slot field.
+__device__ inline void small_tile(const QueryDev& q, const Tile& tile, char* smem) {
+    const uint32_t S = static_cast<uint32_t>(q.S);
+    lds_init(smem, S, q.p);
+    __syncthreads();
+    uint64_t vrow[kMaxAggs];
+    for (uint64_t r = tile.begin + threadIdx.x; r < tile.end; r += blockDim.x) {
+        for (int a = 0; a < q.p.n; ++a) vrow[a] = r;
+        lds_update(smem, S, q.p, direct_slot(q, r), vrow);
+    }
+    __syncthreads();
+    const uint64_t* acc = reinterpret_cast<const uint64_t*>(smem);
+    const uint32_t* cnt = reinterpret_cast<const uint32_t*>(smem + size_t(8) * S * q.p.nacc);
+    const uint32_t* rows = cnt + size_t(S) * q.p.ncnt;
+    for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
+        if (rows[s] == 0) continue;
+        atomicAdd(reinterpret_cast<unsigned long long*>(&q.st[s]), static_cast<unsigned long long>(rows[s]));
+        for (int a = 0; a < q.p.nacc; ++a)
+            combine_atomic(q.p.acc_kind[a], &q.st[(1 + size_t(a)) * q.S + s], acc[size_t(a) * S + s]);
+        for (int c = 0; c < q.p.ncnt; ++c)
+            if (cnt[size_t(c) * S + s])
+                atomicAdd(reinterpret_cast<unsigned long long*>(&q.st[(1 + size_t(q.p.nacc) + c) * q.S + s]),
+                          static_cast<unsigned long long>(cnt[size_t(c) * S + s]));
+    }
+}
+
+// Part kind, scatter: each row's entry (slot within its partition | value
+// validity << 31, value bits) goes to its partition's range of this tile.
+__device__ inline void scatter_tile(const QueryDev& q, const Tile& tile, char* smem, const uint64_t* offs,
+                                    uint32_t* ekey, uint64_t* eval) {
+    uint64_t* base = reinterpret_cast<uint64_t*>(smem);
+    uint32_t* cur = reinterpret_cast<uint32_t*>(base + q.nparts);
+    for (uint32_t i = threadIdx.x; i < q.nparts; i += blockDim.x) {
+        base[i] = offs[q.hbase + uint64_t(i) * q.ntiles + tile.qt];
+        cur[i] = 0;
+    }
+    __syncthreads();
+    const uint32_t mask = (1u << q.pbits) - 1;
+    for (uint64_t r = tile.begin + threadIdx.x; r < tile.end; r += blockDim.x) {
+        const uint32_t s = direct_slot(q, r);
+        const uint32_t part = s >> q.pbits;
+        const uint64_t pos = base[part] + atomicAdd(&cur[part], 1u);
+        ekey[pos] = (s & mask) | (static_cast<uint32_t>(is_valid(q.val, r)) << 31);
+        eval[pos] = value_bits(q.val, r);
+    }
+}
+
+// ---- kernels --------------------------------------------------------------------------------
+// State of every query the phases do not write in full: small and hash
+// queries (and, for the per-batch mode, part queries too).
+__global__ void batch_init_kernel(const QueryDev* qs, uint32_t nq, int all) {
+    for (uint32_t qi = 0; qi < nq; ++qi) {
+        const QueryDev& q = qs[qi];
+        if (q.kind == BK_PART && !all) continue;
+        const uint64_t S = q.S;
+        for (uint64_t s = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; s < S;
+             s += uint64_t(gridDim.x) * blockDim.x) {
+            if (q.kind == BK_HASH) q.t.state[s] = 0;
+            q.st[s] = 0;
+            for (int a = 0; a < q.p.nacc; ++a) q.st[(1 + size_t(a)) * S + s] = acc_identity(q.p.acc_kind[a]);
+            for (int c = 0; c < q.p.ncnt; ++c) q.st[(1 + size_t(q.p.nacc) + c) * S + s] = 0;
+        }
+    }
+}
+
+__global__ void batch_reset_kernel(unsigned long long* next, unsigned int* ovf) {
+    next[0] = next[1] = next[2] = 0;
+    *ovf = 0;
+}
+
+// Part queries, pass 1: per-(partition, tile) row counts.
+__global__ void __launch_bounds__(256) batch_hist_kernel(const QueryDev* qs, const Tile* tiles, uint64_t ntiles,
+                                                         unsigned long long* next, uint32_t* hist) {
+    extern __shared__ uint32_t hcnt[];
+    __shared__ uint64_t cur;
+    while (true) {
+        if (threadIdx.x == 0) cur = atomicAdd(next, 1ull);
+        __syncthreads();
+        const uint64_t i = cur;
+        if (i >= ntiles) break;
+        const Tile t = tiles[i];
+        const QueryDev& q = qs[t.q];
+        for (uint32_t p = threadIdx.x; p < q.nparts; p += blockDim.x) hcnt[p] = 0;
+        __syncthreads();
+        for (uint64_t r = t.begin + threadIdx.x; r < t.end; r += blockDim.x) atomicAdd(&hcnt[direct_slot(q, r) >> q.pbits], 1u);
+        __syncthreads();
+        for (uint32_t p = threadIdx.x; p < q.nparts; p += blockDim.x) hist[q.hbase + uint64_t(p) * q.ntiles + t.qt] = hcnt[p];
+        __syncthreads();
+    }
+}
+
+// Every query's tiles: small tables, part scatter, hash probes.
+__global__ void __launch_bounds__(256) batch_main_kernel(const QueryDev* qs, const Tile* tiles, uint64_t ntiles,
+                                                         unsigned long long* next, uint32_t C, const uint64_t* offs,
+                                                         uint32_t* ekey, uint64_t* eval) {
     extern __shared__ __align__(16) char smem[];
     __shared__ uint64_t cur;
     while (true) {
@@ -179,56 +312,91 @@ __global__ void __launch_bounds__(256) batch_persistent_kernel(const QueryDev* q
         if (i >= ntiles) break;
         const Tile tile = tiles[i];
         const QueryDev& q = qs[tile.q];
-        if (C) {
-            cache_reset(smem, C, q.p);
-            __syncthreads();
-        }
-        process_tile(q, tile, smem, C);
-        if (C) {
-            __syncthreads();
-            cache_flush(smem, C, q);
+        if (q.kind == BK_SMALL) {
+            small_tile(q, tile, smem);
+        } else if (q.kind == BK_PART) {
+            scatter_tile(q, tile, smem, offs, ekey, eval);
+        } else {
+            if (C) {
+                cache_reset(smem, C, q.p);
+                __syncthreads();
+            }
+            hash_tile(q, tile, smem, C);
+            if (C) {
+                __syncthreads();
+                cache_flush(smem, C, q);
+            }
         }
         __syncthreads();
     }
 }
 
-// One launch = one batch of <= 2048 rows of one query (the per-batch baseline).
+// Part queries, pass 2: one block per (query, partition) aggregates the
+// partition's entries in LDS and writes its slots' final state.
+__global__ void __launch_bounds__(kPartAggThreads) batch_part_agg_kernel(const QueryDev* qs, const PartItem* items,
+                                                                         uint64_t nitems, unsigned long long* next,
+                                                                         const uint64_t* offs, const uint32_t* ekey,
+                                                                         const uint64_t* eval) {
+    extern __shared__ __align__(16) char smem[];
+    __shared__ uint64_t cur;
+    int32_t vcol[kMaxAggs];
+    for (int a = 0; a < kMaxAggs; ++a) vcol[a] = 0;  // every aggregate reads the value column
+    while (true) {
+        if (threadIdx.x == 0) cur = atomicAdd(next, 1ull);
+        __syncthreads();
+        const uint64_t i = cur;
+        if (i >= nitems) break;
+        const PartItem it = items[i];
+        const QueryDev& q = qs[it.q];
+        const uint64_t lo = uint64_t(it.part) << q.pbits;
+        const uint32_t cnt = static_cast<uint32_t>(min<uint64_t>(uint64_t(1) << q.pbits, q.S - lo));
+        lds_init(smem, cnt, q.p);
+        __syncthreads();
+        const uint64_t b = offs[q.hbase + uint64_t(it.part) * q.ntiles];
+        const uint64_t e = (it.part + 1 < q.nparts) ? offs[q.hbase + uint64_t(it.part + 1) * q.ntiles] : q.ebase + q.rows;
+        for (uint64_t k = b + threadIdx.x; k < e; k += blockDim.x) {
+            const uint32_t w = ekey[k];
+            const uint64_t bits = eval[k];
+            lds_update_vals(smem, cnt, q.p, w & 0x7FFFFFFFu, vcol, &bits, w >> 31);
+        }
+        __syncthreads();
+        const uint64_t* acc = reinterpret_cast<const uint64_t*>(smem);
+        const uint32_t* cn = reinterpret_cast<const uint32_t*>(smem + size_t(8) * cnt * q.p.nacc);
+        const uint32_t* rows = cn + size_t(cnt) * q.p.ncnt;
+        for (uint32_t s = threadIdx.x; s < cnt; s += blockDim.x) {
+            q.st[lo + s] = rows[s];
+            for (int a = 0; a < q.p.nacc; ++a) q.st[(1 + size_t(a)) * q.S + lo + s] = acc[size_t(a) * cnt + s];
+            for (int c = 0; c < q.p.ncnt; ++c)
+                q.st[(1 + size_t(q.p.nacc) + c) * q.S + lo + s] = cn[size_t(c) * cnt + s];
+        }
+        __syncthreads();
+    }
+}
+
+// Per-batch baseline: one launch per <= 2048-row batch of one query.
 __global__ void __launch_bounds__(256) batch_one_kernel(const QueryDev* qs, uint32_t qi, uint64_t begin, uint64_t end,
                                                         uint32_t C) {
     extern __shared__ __align__(16) char smem[];
     const QueryDev& q = qs[qi];
-    // Each block takes a contiguous slice of the batch.
     const uint64_t per = (end - begin + gridDim.x - 1) / gridDim.x;
     Tile t{qi, 0, begin + per * blockIdx.x, min(end, begin + per * (blockIdx.x + 1))};
+    if (q.kind != BK_HASH) {
+        uint64_t vrow[kMaxAggs];
+        for (uint64_t r = t.begin + threadIdx.x; r < t.end; r += blockDim.x) {
+            for (int a = 0; a < q.p.n; ++a) vrow[a] = r;
+            global_update(q.st, q.S, q.p, direct_slot(q, r), vrow);
+        }
+        return;
+    }
     if (C) {
         cache_reset(smem, C, q.p);
         __syncthreads();
     }
-    if (t.begin < t.end) process_tile(q, t, smem, C);
+    if (t.begin < t.end) hash_tile(q, t, smem, C);
     if (C) {
         __syncthreads();
         cache_flush(smem, C, q);
     }
-}
-
-__global__ void batch_init_kernel(const QueryDev* qs, uint32_t nq) {
-    // grid-stride over (query, slot); slot counts differ per query.
-    for (uint32_t qi = 0; qi < nq; ++qi) {
-        const QueryDev& q = qs[qi];
-        const uint64_t S = q.t.cap;
-        for (uint64_t s = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; s < S;
-             s += uint64_t(gridDim.x) * blockDim.x) {
-            q.t.state[s] = 0;
-            q.st[s] = 0;
-            for (int a = 0; a < q.p.nacc; ++a) q.st[(1 + size_t(a)) * S + s] = acc_identity(q.p.acc_kind[a]);
-            for (int c = 0; c < q.p.ncnt; ++c) q.st[(1 + size_t(q.p.nacc) + c) * S + s] = 0;
-        }
-    }
-}
-
-__global__ void batch_reset_kernel(unsigned long long* next, unsigned int* ovf) {
-    *next = 0;
-    *ovf = 0;
 }
 
 }  // namespace qe_gpu
@@ -237,15 +405,18 @@ __global__ void batch_reset_kernel(unsigned long long* next, unsigned int* ovf) 
 struct BatchPlan {
     uint32_t nq = 0;
     std::vector<QueryDev> hq;
-    std::vector<Scratch> tables;  // per query: state, hash, words, nullmask, st
-    std::vector<uint64_t> caps;
-    Scratch dq, tiles, next, ovf;
-    uint64_t ntiles = 0;
+    std::vector<Scratch> bufs;  // per-query state and hash tables
+    std::vector<std::vector<uint64_t>> morsels;
+    Scratch dq, ovf, next;
+    bool any_hash = false;
+    size_t small_lds = 0;  // largest small-kind table
+    uint32_t max_parts = 0, max_pbits = 0;
+    // Layout for the current tile size.
     uint32_t tile_rows = 0;
-    std::vector<std::vector<uint64_t>> morsels;  // per query offsets
-    std::vector<uint64_t> rows;
+    Scratch mtiles, htiles, items, hist, offs, sums, ekey, eval;
+    uint64_t n_mtiles = 0, n_htiles = 0, n_items = 0, nhist = 0, nent = 0;
     hipGraphExec_t graph = nullptr;
-    uint32_t graph_C = 0, graph_blocks = 0;
+    uint32_t graph_C = 0, graph_blocks = 0, graph_tile = 0;
     ~BatchPlan() {
         if (graph) hipGraphExecDestroy(graph);
     }
@@ -254,10 +425,9 @@ struct BatchPlan {
 static void build_batch_plan(qe_gpu_ctx* ctx, BatchPlan& bp, const qe_batch_query* qs, uint32_t nq) {
     bp.nq = nq;
     bp.hq.resize(nq);
-    bp.caps.resize(nq);
     bp.morsels.resize(nq);
-    bp.rows.resize(nq);
     bp.ovf = Scratch(ctx, 8);
+    bp.next = Scratch(ctx, 3 * 8);
     for (uint32_t i = 0; i < nq; ++i) {
         const qe_batch_query& q = qs[i];
         invalid(q.key != nullptr && q.value != nullptr, "batched query needs key and value columns");
@@ -267,14 +437,16 @@ static void build_batch_plan(qe_gpu_ctx* ctx, BatchPlan& bp, const qe_batch_quer
         invalid(offs.front() == 0 && offs.back() == q.key->len, "morsel offsets must cover the columns");
         for (size_t m = 0; m + 1 < offs.size(); ++m) invalid(offs[m] <= offs[m + 1], "morsel offsets must ascend");
         bp.morsels[i] = offs;
-        bp.rows[i] = q.key->len;
+        const uint64_t n = q.key->len;
         ColView ins[kMaxAggs];
         int types[kMaxAggs];
         invalid(q.naggs >= 1 && q.aggs != nullptr, "batched query needs >=1 aggregate");
+        bool on_value = true;
         for (uint32_t a = 0; a < q.naggs && a < uint32_t(kMaxAggs); ++a) {
             invalid(q.aggs[a].func == QE_AGG_COUNT_STAR || q.aggs[a].input_col <= 1,
                     "batched aggregate input must be 0 (key) or 1 (value)");
             const qe_dev_col* c = q.aggs[a].input_col == 0 ? q.key : q.value;
+            if (q.aggs[a].func != QE_AGG_COUNT_STAR && q.aggs[a].input_col == 0) on_value = false;
             ins[a] = view(c);
             types[a] = c->type;
         }
@@ -284,72 +456,165 @@ static void build_batch_plan(qe_gpu_ctx* ctx, BatchPlan& bp, const qe_batch_quer
               "too many aggregates for the batched path");
         d.hk.nkeys = 1;
         d.hk.key[0] = view(q.key);
-        // Capacity from the distinct-group bound the caller supplies (or the
-        // row count), load <= 0.5.
-        const uint64_t bound = q.group_bound ? std::min<uint64_t>(q.group_bound, q.key->len) : q.key->len;
-        uint64_t cap = 1024;
-        while (cap < 2 * bound + 2) cap <<= 1;
-        check(cap <= (uint64_t(1) << 31), QE_GPU_UNSUPPORTED, "batched query too large");
-        bp.caps[i] = cap;
-        Scratch state(ctx, cap * 4), hash(ctx, cap * 8), words(ctx, cap * 8), nmask(ctx, cap * 8),
-            st(ctx, state_words(cap, d.p) * 8);
-        d.t = HashTableView{state.as<uint32_t>(), hash.as<uint64_t>(), words.as<uint64_t>(), nmask.as<uint64_t>(),
-                            cap, bp.ovf.as<unsigned int>()};
+        d.val = view(q.value);
+        d.rows = n;
+        d.kind = BK_HASH;
+        // Dense integer keys: direct slots (small or partitioned).
+        if (is_int_like(q.key->type) && q.key->type != QE_BOOL) {
+            const MinMax mm = column_minmax(ctx, q.key, n);
+            const uint64_t R = mm.valid ? static_cast<uint64_t>(mm.mx) - static_cast<uint64_t>(mm.mn) + 1 : 0;
+            if (R <= std::max<uint64_t>(2 * n, 1024) && R < (uint64_t(1) << 30)) {
+                const uint64_t S = R + 1;
+                d.kmin = mm.valid ? mm.mn : 0;
+                d.S = S;
+                if (lds_bytes(static_cast<uint32_t>(S), d.p) <= kSmallLds) {
+                    d.kind = BK_SMALL;
+                    bp.small_lds = std::max(bp.small_lds, lds_bytes(static_cast<uint32_t>(S), d.p));
+                } else if (on_value) {
+                    uint32_t pb = 16;
+                    while (pb > 8 && lds_bytes(1u << pb, d.p) > kPartLds) --pb;
+                    const uint64_t np64 = (S + (uint64_t(1) << pb) - 1) >> pb;
+                    if (lds_bytes(1u << pb, d.p) <= kPartLds && np64 <= kMaxParts) {
+                        d.kind = BK_PART;
+                        d.pbits = pb;
+                        d.nparts = static_cast<uint32_t>(np64);
+                        bp.max_parts = std::max(bp.max_parts, d.nparts);
+                        bp.max_pbits = std::max(bp.max_pbits, pb);
+                    }
+                }
+            }
+        }
+        if (d.kind == BK_HASH) {
+            // Capacity from the distinct-group bound the caller supplies (or the
+            // row count), load <= 0.5.
+            const uint64_t bound = q.group_bound ? std::min<uint64_t>(q.group_bound, n) : n;
+            uint64_t cap = 1024;
+            while (cap < 2 * bound + 2) cap <<= 1;
+            check(cap <= (uint64_t(1) << 31), QE_GPU_UNSUPPORTED, "batched query too large");
+            d.S = cap;
+            Scratch state(ctx, cap * 4), hash(ctx, cap * 8), words(ctx, cap * 8), nmask(ctx, cap * 8);
+            d.t = HashTableView{state.as<uint32_t>(), hash.as<uint64_t>(), words.as<uint64_t>(),
+                                nmask.as<uint64_t>(), cap, bp.ovf.as<unsigned int>()};
+            bp.bufs.push_back(std::move(state));
+            bp.bufs.push_back(std::move(hash));
+            bp.bufs.push_back(std::move(words));
+            bp.bufs.push_back(std::move(nmask));
+            bp.any_hash = true;
+        }
+        Scratch st(ctx, state_words(d.S, d.p) * 8);
         d.st = st.as<uint64_t>();
-        bp.tables.push_back(std::move(state));
-        bp.tables.push_back(std::move(hash));
-        bp.tables.push_back(std::move(words));
-        bp.tables.push_back(std::move(nmask));
-        bp.tables.push_back(std::move(st));
+        bp.bufs.push_back(std::move(st));
         bp.hq[i] = d;
     }
     bp.dq = Scratch(ctx, nq * sizeof(QueryDev));
-    QE_HIP(hipMemcpyAsync(bp.dq.p, bp.hq.data(), nq * sizeof(QueryDev), hipMemcpyHostToDevice, ctx->stream));
-    bp.next = Scratch(ctx, 8);
-    QE_HIP(hipStreamSynchronize(ctx->stream));
 }
 
-static void make_tiles(qe_gpu_ctx* ctx, BatchPlan& bp, uint32_t tile_rows) {
-    if (bp.tile_rows == tile_rows && bp.ntiles) return;
-    std::vector<Tile> tiles;
-    // Interleave queries so concurrent blocks work on different tables.
+// Work lists for a tile size: all tiles (main phase), part-query tiles
+// (histogram), (query, partition) items, and the histogram / entry layout.
+static void make_layout(qe_gpu_ctx* ctx, BatchPlan& bp, uint32_t tile_rows) {
+    if (bp.tile_rows == tile_rows) return;
     std::vector<std::vector<Tile>> per(bp.nq);
-    for (uint32_t q = 0; q < bp.nq; ++q)
+    for (uint32_t q = 0; q < bp.nq; ++q) {
+        uint32_t qt = 0;
         for (size_t m = 0; m + 1 < bp.morsels[q].size(); ++m)
             for (uint64_t b = bp.morsels[q][m]; b < bp.morsels[q][m + 1]; b += tile_rows)
-                per[q].push_back(Tile{q, 0, b, std::min<uint64_t>(bp.morsels[q][m + 1], b + tile_rows)});
+                per[q].push_back(Tile{q, qt++, b, std::min<uint64_t>(bp.morsels[q][m + 1], b + tile_rows)});
+    }
+    // Interleave queries so concurrent blocks work on different tables.
+    std::vector<Tile> mt, ht;
     for (size_t k = 0;; ++k) {
         bool any = false;
         for (uint32_t q = 0; q < bp.nq; ++q)
             if (k < per[q].size()) {
-                tiles.push_back(per[q][k]);
+                mt.push_back(per[q][k]);
+                if (bp.hq[q].kind == BK_PART) ht.push_back(per[q][k]);
                 any = true;
             }
         if (!any) break;
     }
-    bp.ntiles = tiles.size();
-    bp.tile_rows = tile_rows;
-    bp.tiles = Scratch(ctx, std::max<uint64_t>(bp.ntiles, 1) * sizeof(Tile));
-    if (bp.ntiles)
-        QE_HIP(hipMemcpyAsync(bp.tiles.p, tiles.data(), bp.ntiles * sizeof(Tile), hipMemcpyHostToDevice, ctx->stream));
+    uint64_t hbase = 0, ebase = 0;
+    std::vector<std::vector<PartItem>> qitems(bp.nq);
+    for (uint32_t q = 0; q < bp.nq; ++q) {
+        QueryDev& d = bp.hq[q];
+        if (d.kind != BK_PART) continue;
+        d.ntiles = static_cast<uint32_t>(per[q].size());
+        d.hbase = hbase;
+        d.ebase = ebase;
+        hbase += uint64_t(d.nparts) * d.ntiles;
+        ebase += d.rows;
+        for (uint32_t p = 0; p < d.nparts; ++p) qitems[q].push_back(PartItem{q, p});
+    }
+    std::vector<PartItem> it;
+    for (size_t k = 0;; ++k) {
+        bool any = false;
+        for (uint32_t q = 0; q < bp.nq; ++q)
+            if (k < qitems[q].size()) {
+                it.push_back(qitems[q][k]);
+                any = true;
+            }
+        if (!any) break;
+    }
+    bp.n_mtiles = mt.size();
+    bp.n_htiles = ht.size();
+    bp.n_items = it.size();
+    bp.nhist = hbase;
+    bp.nent = ebase;
+    bp.mtiles = Scratch(ctx, std::max<uint64_t>(mt.size(), 1) * sizeof(Tile));
+    bp.htiles = Scratch(ctx, std::max<uint64_t>(ht.size(), 1) * sizeof(Tile));
+    bp.items = Scratch(ctx, std::max<uint64_t>(it.size(), 1) * sizeof(PartItem));
+    bp.hist = Scratch(ctx, std::max<uint64_t>(hbase, 1) * 4);
+    bp.offs = Scratch(ctx, std::max<uint64_t>(hbase, 1) * 8);
+    bp.sums = Scratch(ctx, scan_sums_words(std::max<uint64_t>(hbase, 1)) * 8);
+    bp.ekey = Scratch(ctx, std::max<uint64_t>(ebase, 1) * 4);
+    bp.eval = Scratch(ctx, std::max<uint64_t>(ebase, 1) * 8);
+    if (!mt.empty())
+        QE_HIP(hipMemcpyAsync(bp.mtiles.p, mt.data(), mt.size() * sizeof(Tile), hipMemcpyHostToDevice, ctx->stream));
+    if (!ht.empty())
+        QE_HIP(hipMemcpyAsync(bp.htiles.p, ht.data(), ht.size() * sizeof(Tile), hipMemcpyHostToDevice, ctx->stream));
+    if (!it.empty())
+        QE_HIP(hipMemcpyAsync(bp.items.p, it.data(), it.size() * sizeof(PartItem), hipMemcpyHostToDevice, ctx->stream));
+    QE_HIP(hipMemcpyAsync(bp.dq.p, bp.hq.data(), bp.nq * sizeof(QueryDev), hipMemcpyHostToDevice, ctx->stream));
     QE_HIP(hipStreamSynchronize(ctx->stream));
+    bp.tile_rows = tile_rows;
+    if (bp.graph) {
+        QE_HIP(hipGraphExecDestroy(bp.graph));
+        bp.graph = nullptr;
+    }
+}
+
+static size_t main_lds(const BatchPlan& bp, uint32_t C) {
+    return std::max({bp.small_lds, size_t(bp.max_parts) * 12, bp.any_hash ? cache_bytes(C) : size_t(0),
+                     size_t(16)});
 }
 
 static void enqueue_persistent(qe_gpu_ctx* ctx, BatchPlan& bp, uint32_t blocks, uint32_t C) {
-    batch_reset_kernel<<<1, 1, 0, ctx->stream>>>(bp.next.as<unsigned long long>(), bp.ovf.as<unsigned int>());
-    batch_init_kernel<<<ctx->num_cus * 4, 256, 0, ctx->stream>>>(bp.dq.as<QueryDev>(), bp.nq);
-    batch_persistent_kernel<<<blocks, 256, cache_bytes(C), ctx->stream>>>(
-        bp.dq.as<QueryDev>(), bp.tiles.as<Tile>(), bp.ntiles, bp.next.as<unsigned long long>(), C);
+    unsigned long long* next = bp.next.as<unsigned long long>();
+    batch_reset_kernel<<<1, 1, 0, ctx->stream>>>(next, bp.ovf.as<unsigned int>());
+    batch_init_kernel<<<ctx->num_cus * 4, 256, 0, ctx->stream>>>(bp.dq.as<QueryDev>(), bp.nq, 0);
+    if (bp.n_htiles) {
+        batch_hist_kernel<<<blocks, 256, size_t(bp.max_parts) * 4, ctx->stream>>>(
+            bp.dq.as<QueryDev>(), bp.htiles.as<Tile>(), bp.n_htiles, next, bp.hist.as<uint32_t>());
+        exclusive_scan_u32_dev(ctx, bp.hist.as<uint32_t>(), bp.offs.as<uint64_t>(), bp.nhist, bp.sums.as<uint64_t>());
+    }
+    if (bp.n_mtiles)
+        batch_main_kernel<<<blocks, 256, main_lds(bp, C), ctx->stream>>>(
+            bp.dq.as<QueryDev>(), bp.mtiles.as<Tile>(), bp.n_mtiles, next + 1, C, bp.offs.as<uint64_t>(),
+            bp.ekey.as<uint32_t>(), bp.eval.as<uint64_t>());
+    if (bp.n_items)
+        batch_part_agg_kernel<<<ctx->num_cus, kPartAggThreads, kPartLds, ctx->stream>>>(
+            bp.dq.as<QueryDev>(), bp.items.as<PartItem>(), bp.n_items, next + 2, bp.offs.as<uint64_t>(),
+            bp.ekey.as<uint32_t>(), bp.eval.as<uint64_t>());
 }
 
 static void run_batch(qe_gpu_ctx* ctx, BatchPlan& bp, int mode, const qe_batch_tuning& tn) {
     const uint32_t C = tn.lds_slots;
     check(C == 0 || (C & (C - 1)) == 0, QE_GPU_INVALID_ARGUMENT, "lds_slots must be 0 or a power of two");
     check(cache_bytes(C) <= 64 * 1024, QE_GPU_INVALID_ARGUMENT, "lds_slots too large");
-    const uint32_t blocks = std::max<uint32_t>(1, tn.blocks_per_cu) * ctx->num_cus;
+    const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>(tn.blocks_per_cu, 16)) * ctx->num_cus;
+    make_layout(ctx, bp, std::max<uint32_t>(256, tn.tile_rows));
     if (mode == QE_BATCH_PER_BATCH) {
         batch_reset_kernel<<<1, 1, 0, ctx->stream>>>(bp.next.as<unsigned long long>(), bp.ovf.as<unsigned int>());
-        batch_init_kernel<<<ctx->num_cus * 4, 256, 0, ctx->stream>>>(bp.dq.as<QueryDev>(), bp.nq);
+        batch_init_kernel<<<ctx->num_cus * 4, 256, 0, ctx->stream>>>(bp.dq.as<QueryDev>(), bp.nq, 1);
         TimedRegion tr(ctx, "batch_per_batch");
         for (uint32_t q = 0; q < bp.nq; ++q)
             for (size_t m = 0; m + 1 < bp.morsels[q].size(); ++m)
@@ -360,9 +625,8 @@ static void run_batch(qe_gpu_ctx* ctx, BatchPlan& bp, int mode, const qe_batch_t
         QE_HIP(hipGetLastError());
         return;
     }
-    make_tiles(ctx, bp, std::max<uint32_t>(256, tn.tile_rows));
     if (mode == QE_BATCH_GRAPH) {
-        if (!bp.graph || bp.graph_C != C || bp.graph_blocks != blocks) {
+        if (!bp.graph || bp.graph_C != C || bp.graph_blocks != blocks || bp.graph_tile != bp.tile_rows) {
             if (bp.graph) QE_HIP(hipGraphExecDestroy(bp.graph));
             bp.graph = nullptr;
             hipStream_t cs;
@@ -379,6 +643,7 @@ static void run_batch(qe_gpu_ctx* ctx, BatchPlan& bp, int mode, const qe_batch_t
             QE_HIP(hipStreamDestroy(cs));
             bp.graph_C = C;
             bp.graph_blocks = blocks;
+            bp.graph_tile = bp.tile_rows;
         }
         TimedRegion tr(ctx, "batch_graph");
         QE_HIP(hipGraphLaunch(bp.graph, ctx->stream));
@@ -393,13 +658,20 @@ static void emit_batch(qe_gpu_ctx* ctx, BatchPlan& bp, const qe_batch_query* qs,
     check(read_scalar(ctx, bp.ovf.as<unsigned int>()) == 0, QE_GPU_OVERFLOW,
           "batched group table overflow (group_bound too small?)");
     for (uint32_t i = 0; i < bp.nq; ++i) {
+        const QueryDev& d = bp.hq[i];
         KeyEmit ke{};
-        ke.mode = 1;
         ke.nkeys = 1;
         ke.types[0] = qs[i].key->type;
-        ke.words = bp.hq[i].t.words;
-        ke.nullmask = bp.hq[i].t.nullmask;
-        outs[i] = emit_groups(ctx, bp.hq[i].st, bp.caps[i], bp.hq[i].p, ke, false);
+        if (d.kind == BK_HASH) {
+            ke.mode = 1;
+            ke.words = d.t.words;
+            ke.nullmask = d.t.nullmask;
+        } else {
+            ke.mode = 0;
+            ke.kmin = d.kmin;
+            ke.null_slot = d.S - 1;
+        }
+        outs[i] = emit_groups(ctx, d.st, d.S, d.p, ke, false);
     }
 }
 
@@ -428,7 +700,7 @@ qe_gpu_status qe_gpu_batch_run(qe_gpu_ctx* ctx, qe_batch_plan* plan, const qe_ba
     QE_ENTRY(ctx, {
         invalid(plan != nullptr && queries != nullptr, "null argument");
         invalid(mode >= QE_BATCH_PERSISTENT && mode <= QE_BATCH_GRAPH, "unknown batch mode");
-        qe_batch_tuning tn = tuning ? *tuning : qe_batch_tuning{8192, 4, 256};
+        qe_batch_tuning tn = tuning ? *tuning : qe_batch_tuning{16384, 4, 0};
         run_batch(ctx, plan->bp, mode, tn);
         if (outs) emit_batch(ctx, plan->bp, queries, outs);
         else QE_HIP(hipStreamSynchronize(ctx->stream));
@@ -439,12 +711,14 @@ qe_gpu_status qe_gpu_batch_autotune(qe_gpu_ctx* ctx, qe_batch_plan* plan, qe_bat
     QE_ENTRY(ctx, {
         invalid(plan != nullptr && out != nullptr, "null argument");
         double best = 1e30;
-        qe_batch_tuning bt{8192, 4, 256};
-        for (uint32_t tile : {4096u, 16384u, 65536u})
-            for (uint32_t bpc : {2u, 4u, 8u})
-                for (uint32_t C : {0u, 256u, 512u}) {
+        qe_batch_tuning bt{16384, 4, 0};
+        std::vector<uint32_t> caches{0u};
+        if (plan->bp.any_hash) caches.push_back(256u);
+        for (uint32_t tile : {16384u, 65536u})
+            for (uint32_t bpc : {2u, 4u})
+                for (uint32_t C : caches) {
                     qe_batch_tuning tn{tile, bpc, C};
-                    run_batch(ctx, plan->bp, QE_BATCH_PERSISTENT, tn);  // warm
+                    run_batch(ctx, plan->bp, QE_BATCH_PERSISTENT, tn);  // warm (and lay out)
                     QE_HIP(hipStreamSynchronize(ctx->stream));
                     const auto t0 = std::chrono::steady_clock::now();
                     run_batch(ctx, plan->bp, QE_BATCH_PERSISTENT, tn);

@@ -297,6 +297,15 @@ static uint64_t exclusive_scan_impl(qe_gpu_ctx* ctx, const T* in, uint64_t* out,
 uint64_t exclusive_scan_u32(qe_gpu_ctx* ctx, const uint32_t* in, uint64_t* out, uint64_t n) {
     return exclusive_scan_impl<uint32_t>(ctx, in, out, n);
 }
+uint64_t scan_sums_words(uint64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
+void exclusive_scan_u32_dev(qe_gpu_ctx* ctx, const uint32_t* in, uint64_t* out, uint64_t n, uint64_t* sums) {
+    if (n == 0) return;
+    const uint64_t tiles = (n + kScanTile - 1) / kScanTile;
+    scan_tile_sums<uint32_t><<<tiles, kScanBlock, 0, ctx->stream>>>(in, n, sums);
+    scan_sums_single<<<1, 1024, 0, ctx->stream>>>(sums, tiles);
+    scan_tile_apply<uint32_t><<<tiles, kScanBlock, 0, ctx->stream>>>(in, out, n, sums);
+    QE_HIP(hipGetLastError());
+}
 uint64_t exclusive_scan_u64(qe_gpu_ctx* ctx, const uint64_t* in, uint64_t* out, uint64_t n) {
     if (in == out && n > 0) {
         // In place: copy first (the apply kernel reads `in` while writing `out`).
@@ -384,6 +393,12 @@ __global__ void __launch_bounds__(256) minmax_final(const int64_t* part, uint32_
     }
 }
 
+// XCD (accelerator complex die) that runs each block: bits [3:0] of the
+// XCC_ID hardware register (s_getreg; id 20, offset 0, size 4).
+__global__ void xcd_of_block_kernel(uint32_t* out) {
+    if (threadIdx.x == 0) out[blockIdx.x] = static_cast<uint32_t>(__builtin_amdgcn_s_getreg((3 << 11) | 20));
+}
+
 MinMax column_minmax(qe_gpu_ctx* ctx, const qe_dev_col* c, uint64_t n) {
     const uint32_t blocks = static_cast<uint32_t>(grid_for(n, 256, ctx->num_cus * 4));
     Scratch part(ctx, size_t(blocks) * 3 * 8), s(ctx, 3 * 8);
@@ -450,12 +465,27 @@ qe_gpu_status qe_gpu_set_timing(qe_gpu_ctx* ctx, int enabled) {
     QE_ENTRY(ctx, ctx->timing = enabled != 0);
 }
 
+qe_gpu_status qe_gpu_xcd_of_blocks(qe_gpu_ctx* ctx, uint32_t nblocks, uint32_t* out) {
+    QE_ENTRY(ctx, {
+        invalid(out != nullptr && nblocks >= 1 && nblocks <= 65536, "bad argument");
+        Scratch d(ctx, size_t(nblocks) * 4);
+        xcd_of_block_kernel<<<nblocks, 64, 0, ctx->stream>>>(d.as<uint32_t>());
+        QE_HIP(hipGetLastError());
+        QE_HIP(hipMemcpyAsync(out, d.p, size_t(nblocks) * 4, hipMemcpyDeviceToHost, ctx->stream));
+        QE_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
 qe_gpu_status qe_gpu_set_tuning(qe_gpu_ctx* ctx, int32_t key, int64_t value) {
     QE_ENTRY(ctx, {
         switch (key) {
             case QE_TUNE_JOIN_PAIRS:
                 invalid(value == 2 || value == 4, "QE_TUNE_JOIN_PAIRS must be 2 or 4");
                 ctx->tune_join_pairs = static_cast<int>(value);
+                break;
+            case QE_TUNE_ASOF_BUCKET_KNOTS:
+                invalid(value >= 1 && value <= 64, "QE_TUNE_ASOF_BUCKET_KNOTS must be in [1, 64]");
+                ctx->tune_asof_bucket_knots = static_cast<int>(value);
                 break;
             default:
                 invalid(false, "unknown tuning key");

@@ -98,6 +98,7 @@ struct qe_gpu_ctx {
     int num_cus = 256;
     // Tuning knobs (qe_gpu_set_tuning).
     int tune_join_pairs = 4;  // row pairs per thread per tile in the streaming probe
+    int tune_asof_bucket_knots = 1;  // knots per learned-router bucket (set at train time)
     // Small pinned host staging buffer for scalars read back after kernels.
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
@@ -258,6 +259,10 @@ inline unsigned grid_for(uint64_t n, unsigned block, unsigned max_blocks = 65535
 // ---- device-wide exclusive scan of u32 flags/counts into u64 offsets ---------
 // out[i] = sum_{j<i} in[j]; returns the total (synchronises the stream).
 uint64_t exclusive_scan_u32(qe_gpu_ctx* ctx, const uint32_t* in, uint64_t* out, uint64_t n);
+// Device-only variant (no allocation, no host read; safe under graph
+// capture): `sums` holds scan_sums_words(n) words.
+uint64_t scan_sums_words(uint64_t n);
+void exclusive_scan_u32_dev(qe_gpu_ctx* ctx, const uint32_t* in, uint64_t* out, uint64_t n, uint64_t* sums);
 // Same for u64 inputs (in-place allowed).
 uint64_t exclusive_scan_u64(qe_gpu_ctx* ctx, const uint64_t* in, uint64_t* out, uint64_t n);
 

@@ -205,6 +205,33 @@ def test_router_mlp_gain_on_session_shape(ctx, qe):
     assert st["mean_bucket_knots"] < 0.8 * st["mean_bucket_knots_linear"], st
 
 
+@pytest.mark.parametrize("kpb", [2, 8, 64])
+def test_coarse_router_buckets_stay_exact(ctx, qe, kpb):
+    # Several knots per router bucket (a smaller radix table): the learned
+    # probe still returns exactly the binary-search matches, for both shapes.
+    for sessions in (0, 3):
+        (rs, rt, rp), (ls, lt) = ctx.gen_asof(77 + kpb, 1 << 20, 1 << 18, 64, 1.1, sessions=sessions)
+        try:
+            idx = ctx.asof_build([rs], rt)
+            ctx.set_tuning(qe.TUNE_ASOF_BUCKET_KNOTS, kpb)
+            try:
+                idx.train(32, 200, 0.003)
+            finally:
+                ctx.set_tuning(qe.TUNE_ASOF_BUCKET_KNOTS, 1)
+            b = idx.probe([ls], lt, mode=qe.ASOF_BINARY)
+            l1 = idx.probe([ls], lt, mode=qe.ASOF_LEARNED)
+            st = idx.stats()
+            l2 = idx.probe([ls], lt, mode=qe.ASOF_LEARNED_NO_MLP)
+            idx.free()
+        finally:
+            for c in (rs, rt, rp, ls, lt):
+                c.free()
+        assert (b == l1).all() and (b == l2).all()
+        assert st["fallbacks"] == 0
+        assert st["router_buckets"] < st["pieces"]
+        assert st["mean_bucket_knots"] <= st["mean_bucket_knots_linear"]
+
+
 def test_mlp_forward_matches_cpu(ctx):
     r = np.random.default_rng(8)
     p = M.identity_params() + np.abs(r.normal(scale=0.1, size=M.N_PARAMS))

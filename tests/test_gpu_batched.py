@@ -87,3 +87,43 @@ def test_autotune_returns_a_valid_shape(ctx, qe):
         for k, v, *_ in queries:
             k.free()
             v.free()
+
+
+def test_strategies_small_partitioned_hash(ctx, qe):
+    # One batch mixing every strategy: small dense tables, partitioned dense
+    # ranges (with NULL keys), hash tables for sparse keys, F64 keys and
+    # aggregates over the key column; each mode against the oracle.
+    r = np.random.default_rng(77)
+    specs = [
+        ("small", I32, lambda n: r.integers(-5, 60, n), False),
+        ("part", I64, lambda n: r.integers(0, 150000, n), False),
+        ("part-nullkeys", I32, lambda n: r.integers(1000, 90000, n), True),
+        ("hash-sparse", I64, lambda n: r.integers(0, 3000, n) * 1000003, False),
+        ("hash-f64", F64, lambda n: r.integers(0, 500, n) * 0.5, False),
+    ]
+    queries, hostcols = [], []
+    for i, (name, kt, gen, knull) in enumerate(specs):
+        n = int(r.integers(50000, 400000))
+        sizes = np.diff(np.sort(np.concatenate([[0, n], r.integers(0, n, 5)])))
+        offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+        kv = np.asarray(gen(n), dtype=O.NP[kt])
+        kvalid = (r.random(n) >= 0.01) if knull else None
+        vv = r.normal(size=n) * 100
+        vvalid = r.random(n) >= 0.02
+        aggs = [(CS, 0), (SUM, 1), (AVG, 1), (MAX, 1)]
+        if name == "hash-sparse":
+            aggs = aggs + [(SUM, 0), (MIN, 0)]  # aggregates over the key column
+        k = ctx.upload(kv, kt, kvalid)
+        v = ctx.upload(vv, F64, vvalid)
+        queries.append((k, v, offs, aggs, 0))
+        hostcols.append(([(kv, kvalid, kt), (vv, vvalid, F64)], aggs))
+    plan = ctx.batch(queries)
+    try:
+        for mode in (qe.BATCH_PERSISTENT, qe.BATCH_GRAPH, qe.BATCH_PER_BATCH):
+            for tuning in ((16384, 2, 0), (4096, 4, 256)):
+                check_outputs(plan.run(mode, tuning=tuning), hostcols)
+    finally:
+        plan.free()
+        for k, v, *_ in queries:
+            k.free()
+            v.free()

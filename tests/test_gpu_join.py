@@ -230,3 +230,14 @@ def test_key_type_mismatch_is_invalid_argument(ctx):
     finally:
         for d in P + B:
             d.free()
+
+
+def test_xcd_of_blocks_reports_dies(ctx):
+    # Block placement over the dies, printed for the record (consecutive
+    # blocks on consecutive XCDs); the ids must be valid die numbers and all
+    # eight dies must take blocks.
+    ids = ctx.xcd_of_blocks(256)
+    print("xcd of blocks 0..15:", ids[:16].tolist(),
+          "round-robin:", bool((ids == np.arange(256) % 8).all()))
+    assert ids.max() < 8
+    assert len(set(ids.tolist())) == 8
