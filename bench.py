@@ -263,9 +263,10 @@ def batched_workload(qe, ctx, args):
 
     res["persistent_s"] = timed(qe.BATCH_PERSISTENT, 3)
     res["graph_s"] = timed(qe.BATCH_GRAPH, 3)
-    res["per_batch_s"] = timed(qe.BATCH_PER_BATCH, 1)
-    res["speedup_persistent_vs_per_batch"] = res["per_batch_s"] / res["persistent_s"]
-    res["speedup_graph_vs_per_batch"] = res["per_batch_s"] / res["graph_s"]
+    if not args.no_per_batch:
+        res["per_batch_s"] = timed(qe.BATCH_PER_BATCH, 1)
+        res["speedup_persistent_vs_per_batch"] = res["per_batch_s"] / res["persistent_s"]
+        res["speedup_graph_vs_per_batch"] = res["per_batch_s"] / res["graph_s"]
     res["rows_per_s_graph"] = total_rows / res["graph_s"]
     # parity: the three modes produce identical groups on one query sample
     outs = {m: plan.run(m, tuning=tune) for m in (qe.BATCH_PERSISTENT, qe.BATCH_GRAPH)}
@@ -349,6 +350,7 @@ def main():
     ap.add_argument("--batch-queries", type=int, default=64)
     ap.add_argument("--batch-morsels", type=int, default=4096)
     ap.add_argument("--batch-max-log2", type=int, default=21)
+    ap.add_argument("--no-per-batch", action="store_true", help="skip the per-batch baseline (profiling runs)")
     ap.add_argument("--only", choices=["join", "asof", "batched", "all"], default="all",
                     help="join: headline workload only; asof / batched: that secondary workload alone")
     ap.add_argument("--quick", action="store_true", help="small sizes (smoke)")

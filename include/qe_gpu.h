@@ -85,8 +85,9 @@ qe_gpu_status qe_gpu_ctx_create(int device, qe_gpu_ctx** out);
 void qe_gpu_ctx_destroy(qe_gpu_ctx* ctx);
 const char* qe_gpu_last_error(const qe_gpu_ctx* ctx);
 qe_gpu_status qe_gpu_sync(qe_gpu_ctx* ctx);
-/* Device memory limit for this context (0 = no limit): lets a test force the
- * OUT_OF_MEMORY path from the host. */
+/* Device memory limit for this context (0 = no limit): bytes in live device
+ * buffers may not exceed it (cached free blocks are released first when a new
+ * allocation would pass it). Lets a test force the OUT_OF_MEMORY path. */
 qe_gpu_status qe_gpu_set_mem_limit(qe_gpu_ctx* ctx, uint64_t bytes);
 /* Kernel timing: when enabled, the dominant kernel of each call is bracketed by
  * HIP events on the context stream; qe_gpu_timing reads the accumulated time

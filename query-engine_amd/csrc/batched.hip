@@ -11,9 +11,10 @@
 //  * part  : integer keys with a larger dense range and aggregates over the
 //            value column; rows are radix-partitioned by slot (2^pbits slots
 //            per partition: a histogram pass, one scan over every such query,
-//            a scatter pass), then one block per partition aggregates it in LDS
-//            and writes its slots' final state: no atomics, the same result on
-//            every run;
+//            a scatter pass), then each partition is aggregated in LDS by one
+//            block (its slots' final state written with plain stores) or, when
+//            the query's partitions are large, by `chunks` blocks that each take
+//            an equal slice of it and fold their tables in with atomics;
 //  * hash  : anything else (sparse ranges, aggregates over the key); rows probe
 //            the shared device hash table, with an optional LDS slot cache.
 // Each phase pulls work from one device queue shared by all queries; every
@@ -41,6 +42,7 @@ constexpr size_t kSmallLds = 32 * 1024;  // LDS of a small query's dense table
 constexpr size_t kPartLds = 128 * 1024;  // LDS of one partition's table (1024-thread block)
 constexpr uint32_t kMaxParts = 8192;     // partitions of one query
 constexpr uint32_t kPartAggThreads = 1024;
+constexpr uint64_t kChunkEntries = 1 << 16;  // expected entries per partition-aggregation block
 
 enum BatchKind : int32_t { BK_SMALL = 0, BK_PART = 1, BK_HASH = 2 };
 
@@ -59,6 +61,8 @@ struct QueryDev {
     uint64_t hbase;   // part: first histogram entry, laid out [part][tile]
     uint64_t ebase;   // part: first partitioned entry
     uint64_t rows;
+    uint32_t chunks;  // part: blocks per partition (slices of equal length)
+    uint32_t pad2;
 };
 
 struct Tile {
@@ -69,6 +73,8 @@ struct Tile {
 struct PartItem {
     uint32_t q;
     uint32_t part;
+    uint32_t chunk;
+    uint32_t pad;
 };
 
 __device__ inline uint32_t direct_slot(const QueryDev& q, uint64_t r) {
@@ -261,7 +267,7 @@ __device__ inline void scatter_tile(const QueryDev& q, const Tile& tile, char* s
 __global__ void batch_init_kernel(const QueryDev* qs, uint32_t nq, int all) {
     for (uint32_t qi = 0; qi < nq; ++qi) {
         const QueryDev& q = qs[qi];
-        if (q.kind == BK_PART && !all) continue;
+        if (q.kind == BK_PART && q.chunks == 1 && !all) continue;
         const uint64_t S = q.S;
         for (uint64_t s = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; s < S;
              s += uint64_t(gridDim.x) * blockDim.x) {
@@ -274,7 +280,7 @@ __global__ void batch_init_kernel(const QueryDev* qs, uint32_t nq, int all) {
 }
 
 __global__ void batch_reset_kernel(unsigned long long* next, unsigned int* ovf) {
-    next[0] = next[1] = next[2] = 0;
+    for (int k = 0; k < 5; ++k) next[k] = 0;
     *ovf = 0;
 }
 
@@ -299,10 +305,12 @@ __global__ void __launch_bounds__(256) batch_hist_kernel(const QueryDev* qs, con
     }
 }
 
-// Every query's tiles: small tables, part scatter, hash probes.
-__global__ void __launch_bounds__(256) batch_main_kernel(const QueryDev* qs, const Tile* tiles, uint64_t ntiles,
-                                                         unsigned long long* next, uint32_t C, const uint64_t* offs,
-                                                         uint32_t* ekey, uint64_t* eval) {
+// One persistent kernel per strategy over that strategy's tile list, so each
+// gets its own register budget: small tables, part scatter, hash probes.
+template <int KIND>
+__global__ void __launch_bounds__(256) batch_tiles_kernel(const QueryDev* qs, const Tile* tiles, uint64_t ntiles,
+                                                          unsigned long long* next, uint32_t C, const uint64_t* offs,
+                                                          uint32_t* ekey, uint64_t* eval) {
     extern __shared__ __align__(16) char smem[];
     __shared__ uint64_t cur;
     while (true) {
@@ -312,9 +320,9 @@ __global__ void __launch_bounds__(256) batch_main_kernel(const QueryDev* qs, con
         if (i >= ntiles) break;
         const Tile tile = tiles[i];
         const QueryDev& q = qs[tile.q];
-        if (q.kind == BK_SMALL) {
+        if (KIND == BK_SMALL) {
             small_tile(q, tile, smem);
-        } else if (q.kind == BK_PART) {
+        } else if (KIND == BK_PART) {
             scatter_tile(q, tile, smem, offs, ekey, eval);
         } else {
             if (C) {
@@ -352,8 +360,9 @@ __global__ void __launch_bounds__(kPartAggThreads) batch_part_agg_kernel(const Q
         const uint32_t cnt = static_cast<uint32_t>(min<uint64_t>(uint64_t(1) << q.pbits, q.S - lo));
         lds_init(smem, cnt, q.p);
         __syncthreads();
-        const uint64_t b = offs[q.hbase + uint64_t(it.part) * q.ntiles];
-        const uint64_t e = (it.part + 1 < q.nparts) ? offs[q.hbase + uint64_t(it.part + 1) * q.ntiles] : q.ebase + q.rows;
+        const uint64_t pb = offs[q.hbase + uint64_t(it.part) * q.ntiles];
+        const uint64_t pe = (it.part + 1 < q.nparts) ? offs[q.hbase + uint64_t(it.part + 1) * q.ntiles] : q.ebase + q.rows;
+        const uint64_t b = pb + (pe - pb) * it.chunk / q.chunks, e = pb + (pe - pb) * (it.chunk + 1) / q.chunks;
         for (uint64_t k = b + threadIdx.x; k < e; k += blockDim.x) {
             const uint32_t w = ekey[k];
             const uint64_t bits = eval[k];
@@ -363,11 +372,24 @@ __global__ void __launch_bounds__(kPartAggThreads) batch_part_agg_kernel(const Q
         const uint64_t* acc = reinterpret_cast<const uint64_t*>(smem);
         const uint32_t* cn = reinterpret_cast<const uint32_t*>(smem + size_t(8) * cnt * q.p.nacc);
         const uint32_t* rows = cn + size_t(cnt) * q.p.ncnt;
-        for (uint32_t s = threadIdx.x; s < cnt; s += blockDim.x) {
-            q.st[lo + s] = rows[s];
-            for (int a = 0; a < q.p.nacc; ++a) q.st[(1 + size_t(a)) * q.S + lo + s] = acc[size_t(a) * cnt + s];
-            for (int c = 0; c < q.p.ncnt; ++c)
-                q.st[(1 + size_t(q.p.nacc) + c) * q.S + lo + s] = cn[size_t(c) * cnt + s];
+        if (q.chunks == 1) {  // sole owner of these slots
+            for (uint32_t s = threadIdx.x; s < cnt; s += blockDim.x) {
+                q.st[lo + s] = rows[s];
+                for (int a = 0; a < q.p.nacc; ++a) q.st[(1 + size_t(a)) * q.S + lo + s] = acc[size_t(a) * cnt + s];
+                for (int c = 0; c < q.p.ncnt; ++c)
+                    q.st[(1 + size_t(q.p.nacc) + c) * q.S + lo + s] = cn[size_t(c) * cnt + s];
+            }
+        } else {
+            for (uint32_t s = threadIdx.x; s < cnt; s += blockDim.x) {
+                if (rows[s] == 0) continue;
+                atomicAdd(reinterpret_cast<unsigned long long*>(&q.st[lo + s]), static_cast<unsigned long long>(rows[s]));
+                for (int a = 0; a < q.p.nacc; ++a)
+                    combine_atomic(q.p.acc_kind[a], &q.st[(1 + size_t(a)) * q.S + lo + s], acc[size_t(a) * cnt + s]);
+                for (int c = 0; c < q.p.ncnt; ++c)
+                    if (cn[size_t(c) * cnt + s])
+                        atomicAdd(reinterpret_cast<unsigned long long*>(&q.st[(1 + size_t(q.p.nacc) + c) * q.S + lo + s]),
+                                  static_cast<unsigned long long>(cn[size_t(c) * cnt + s]));
+            }
         }
         __syncthreads();
     }
@@ -413,8 +435,8 @@ struct BatchPlan {
     uint32_t max_parts = 0, max_pbits = 0;
     // Layout for the current tile size.
     uint32_t tile_rows = 0;
-    Scratch mtiles, htiles, items, hist, offs, sums, ekey, eval;
-    uint64_t n_mtiles = 0, n_htiles = 0, n_items = 0, nhist = 0, nent = 0;
+    Scratch ktiles[3], htiles, items, hist, offs, sums, ekey, eval;  // ktiles: per strategy
+    uint64_t n_ktiles[3] = {0, 0, 0}, n_htiles = 0, n_items = 0, nhist = 0, nent = 0;
     hipGraphExec_t graph = nullptr;
     uint32_t graph_C = 0, graph_blocks = 0, graph_tile = 0;
     ~BatchPlan() {
@@ -427,7 +449,7 @@ static void build_batch_plan(qe_gpu_ctx* ctx, BatchPlan& bp, const qe_batch_quer
     bp.hq.resize(nq);
     bp.morsels.resize(nq);
     bp.ovf = Scratch(ctx, 8);
-    bp.next = Scratch(ctx, 3 * 8);
+    bp.next = Scratch(ctx, 5 * 8);
     for (uint32_t i = 0; i < nq; ++i) {
         const qe_batch_query& q = qs[i];
         invalid(q.key != nullptr && q.value != nullptr, "batched query needs key and value columns");
@@ -521,12 +543,12 @@ static void make_layout(qe_gpu_ctx* ctx, BatchPlan& bp, uint32_t tile_rows) {
                 per[q].push_back(Tile{q, qt++, b, std::min<uint64_t>(bp.morsels[q][m + 1], b + tile_rows)});
     }
     // Interleave queries so concurrent blocks work on different tables.
-    std::vector<Tile> mt, ht;
+    std::vector<Tile> kt[3], ht;
     for (size_t k = 0;; ++k) {
         bool any = false;
         for (uint32_t q = 0; q < bp.nq; ++q)
             if (k < per[q].size()) {
-                mt.push_back(per[q][k]);
+                kt[bp.hq[q].kind].push_back(per[q][k]);
                 if (bp.hq[q].kind == BK_PART) ht.push_back(per[q][k]);
                 any = true;
             }
@@ -542,7 +564,11 @@ static void make_layout(qe_gpu_ctx* ctx, BatchPlan& bp, uint32_t tile_rows) {
         d.ebase = ebase;
         hbase += uint64_t(d.nparts) * d.ntiles;
         ebase += d.rows;
-        for (uint32_t p = 0; p < d.nparts; ++p) qitems[q].push_back(PartItem{q, p});
+        // Blocks per partition from the expected partition length.
+        const uint64_t expect = (d.rows + d.nparts - 1) / d.nparts;
+        d.chunks = static_cast<uint32_t>(std::max<uint64_t>(1, (expect + kChunkEntries - 1) / kChunkEntries));
+        for (uint32_t p = 0; p < d.nparts; ++p)
+            for (uint32_t c = 0; c < d.chunks; ++c) qitems[q].push_back(PartItem{q, p, c, 0});
     }
     std::vector<PartItem> it;
     for (size_t k = 0;; ++k) {
@@ -554,12 +580,12 @@ static void make_layout(qe_gpu_ctx* ctx, BatchPlan& bp, uint32_t tile_rows) {
             }
         if (!any) break;
     }
-    bp.n_mtiles = mt.size();
+    for (int k = 0; k < 3; ++k) bp.n_ktiles[k] = kt[k].size();
     bp.n_htiles = ht.size();
     bp.n_items = it.size();
     bp.nhist = hbase;
     bp.nent = ebase;
-    bp.mtiles = Scratch(ctx, std::max<uint64_t>(mt.size(), 1) * sizeof(Tile));
+    for (int k = 0; k < 3; ++k) bp.ktiles[k] = Scratch(ctx, std::max<uint64_t>(kt[k].size(), 1) * sizeof(Tile));
     bp.htiles = Scratch(ctx, std::max<uint64_t>(ht.size(), 1) * sizeof(Tile));
     bp.items = Scratch(ctx, std::max<uint64_t>(it.size(), 1) * sizeof(PartItem));
     bp.hist = Scratch(ctx, std::max<uint64_t>(hbase, 1) * 4);
@@ -567,8 +593,10 @@ static void make_layout(qe_gpu_ctx* ctx, BatchPlan& bp, uint32_t tile_rows) {
     bp.sums = Scratch(ctx, scan_sums_words(std::max<uint64_t>(hbase, 1)) * 8);
     bp.ekey = Scratch(ctx, std::max<uint64_t>(ebase, 1) * 4);
     bp.eval = Scratch(ctx, std::max<uint64_t>(ebase, 1) * 8);
-    if (!mt.empty())
-        QE_HIP(hipMemcpyAsync(bp.mtiles.p, mt.data(), mt.size() * sizeof(Tile), hipMemcpyHostToDevice, ctx->stream));
+    for (int k = 0; k < 3; ++k)
+        if (!kt[k].empty())
+            QE_HIP(hipMemcpyAsync(bp.ktiles[k].p, kt[k].data(), kt[k].size() * sizeof(Tile), hipMemcpyHostToDevice,
+                                  ctx->stream));
     if (!ht.empty())
         QE_HIP(hipMemcpyAsync(bp.htiles.p, ht.data(), ht.size() * sizeof(Tile), hipMemcpyHostToDevice, ctx->stream));
     if (!it.empty())
@@ -582,10 +610,6 @@ static void make_layout(qe_gpu_ctx* ctx, BatchPlan& bp, uint32_t tile_rows) {
     }
 }
 
-static size_t main_lds(const BatchPlan& bp, uint32_t C) {
-    return std::max({bp.small_lds, size_t(bp.max_parts) * 12, bp.any_hash ? cache_bytes(C) : size_t(0),
-                     size_t(16)});
-}
 
 static void enqueue_persistent(qe_gpu_ctx* ctx, BatchPlan& bp, uint32_t blocks, uint32_t C) {
     unsigned long long* next = bp.next.as<unsigned long long>();
@@ -596,14 +620,23 @@ static void enqueue_persistent(qe_gpu_ctx* ctx, BatchPlan& bp, uint32_t blocks, 
             bp.dq.as<QueryDev>(), bp.htiles.as<Tile>(), bp.n_htiles, next, bp.hist.as<uint32_t>());
         exclusive_scan_u32_dev(ctx, bp.hist.as<uint32_t>(), bp.offs.as<uint64_t>(), bp.nhist, bp.sums.as<uint64_t>());
     }
-    if (bp.n_mtiles)
-        batch_main_kernel<<<blocks, 256, main_lds(bp, C), ctx->stream>>>(
-            bp.dq.as<QueryDev>(), bp.mtiles.as<Tile>(), bp.n_mtiles, next + 1, C, bp.offs.as<uint64_t>(),
+    const uint64_t* offs = bp.offs.as<uint64_t>();
+    if (bp.n_ktiles[BK_SMALL])
+        batch_tiles_kernel<BK_SMALL><<<blocks, 256, std::max<size_t>(bp.small_lds, 16), ctx->stream>>>(
+            bp.dq.as<QueryDev>(), bp.ktiles[BK_SMALL].as<Tile>(), bp.n_ktiles[BK_SMALL], next + 1, C, offs,
+            bp.ekey.as<uint32_t>(), bp.eval.as<uint64_t>());
+    if (bp.n_ktiles[BK_PART])
+        batch_tiles_kernel<BK_PART><<<blocks, 256, size_t(bp.max_parts) * 12, ctx->stream>>>(
+            bp.dq.as<QueryDev>(), bp.ktiles[BK_PART].as<Tile>(), bp.n_ktiles[BK_PART], next + 2, C, offs,
+            bp.ekey.as<uint32_t>(), bp.eval.as<uint64_t>());
+    if (bp.n_ktiles[BK_HASH])
+        batch_tiles_kernel<BK_HASH><<<blocks, 256, std::max<size_t>(cache_bytes(C), 16), ctx->stream>>>(
+            bp.dq.as<QueryDev>(), bp.ktiles[BK_HASH].as<Tile>(), bp.n_ktiles[BK_HASH], next + 3, C, offs,
             bp.ekey.as<uint32_t>(), bp.eval.as<uint64_t>());
     if (bp.n_items)
         batch_part_agg_kernel<<<ctx->num_cus, kPartAggThreads, kPartLds, ctx->stream>>>(
-            bp.dq.as<QueryDev>(), bp.items.as<PartItem>(), bp.n_items, next + 2, bp.offs.as<uint64_t>(),
-            bp.ekey.as<uint32_t>(), bp.eval.as<uint64_t>());
+            bp.dq.as<QueryDev>(), bp.items.as<PartItem>(), bp.n_items, next + 4, offs, bp.ekey.as<uint32_t>(),
+            bp.eval.as<uint64_t>());
 }
 
 static void run_batch(qe_gpu_ctx* ctx, BatchPlan& bp, int mode, const qe_batch_tuning& tn) {

@@ -8,49 +8,49 @@ using namespace qe_gpu;
 
 namespace qe_gpu {
 
+// Drop every cached block (after the stream drains, since cached blocks may
+// still be read by queued work).
+static void release_cache(qe_gpu_ctx* ctx) {
+    QE_HIP(hipStreamSynchronize(ctx->stream));
+    for (auto& kv : ctx->free_blocks) {
+        hipFree(kv.second);
+        ctx->mem_in_use -= kv.first;
+    }
+    ctx->free_blocks.clear();
+}
+
 void* dev_alloc(qe_gpu_ctx* ctx, size_t bytes) {
     bytes = (bytes + 255) & ~size_t(255);
     if (bytes == 0) bytes = 256;
+    // The context memory limit bounds the bytes in use (cached blocks are
+    // released before a fresh allocation would push the total past it).
+    if (ctx->mem_limit && ctx->live_bytes + bytes > ctx->mem_limit)
+        throw Error{QE_GPU_OUT_OF_MEMORY, "device allocation of " + std::to_string(bytes) +
+                                              " bytes exceeds the context memory limit"};
     // Best fit among cached blocks no more than 2x the request.
     auto it = ctx->free_blocks.lower_bound(bytes);
     if (it != ctx->free_blocks.end() && it->first <= bytes * 2) {
         void* p = it->second;
         ctx->live_blocks[p] = it->first;
+        ctx->live_bytes += it->first;
         ctx->free_blocks.erase(it);
         return p;
     }
-    if (ctx->mem_limit && ctx->mem_in_use + bytes > ctx->mem_limit) {
-        // Release the cache before declaring failure.
-        QE_HIP(hipStreamSynchronize(ctx->stream));
-        for (auto& kv : ctx->free_blocks) {
-            hipFree(kv.second);
-            ctx->mem_in_use -= kv.first;
-        }
-        ctx->free_blocks.clear();
-        if (ctx->mem_in_use + bytes > ctx->mem_limit)
-            throw Error{QE_GPU_OUT_OF_MEMORY,
-                        "device allocation of " + std::to_string(bytes) +
-                            " bytes exceeds the context memory limit"};
-    }
+    if (ctx->mem_limit && ctx->mem_in_use + bytes > ctx->mem_limit) release_cache(ctx);
     void* p = nullptr;
     hipError_t e = hipMalloc(&p, bytes);
     if (e != hipSuccess) {
         (void)hipGetLastError();
-        QE_HIP(hipStreamSynchronize(ctx->stream));
-        for (auto& kv : ctx->free_blocks) {
-            hipFree(kv.second);
-            ctx->mem_in_use -= kv.first;
-        }
-        ctx->free_blocks.clear();
+        release_cache(ctx);
         e = hipMalloc(&p, bytes);
         if (e != hipSuccess) {
             (void)hipGetLastError();
-            throw Error{QE_GPU_OUT_OF_MEMORY,
-                        "hipMalloc(" + std::to_string(bytes) + ") failed"};
+            throw Error{QE_GPU_OUT_OF_MEMORY, "hipMalloc(" + std::to_string(bytes) + ") failed"};
         }
     }
     ctx->mem_in_use += bytes;
     ctx->live_blocks[p] = bytes;
+    ctx->live_bytes += bytes;
     return p;
 }
 
@@ -60,6 +60,7 @@ void dev_free(qe_gpu_ctx* ctx, void* p) {
     if (it == ctx->live_blocks.end()) return;
     // Stream-ordered reuse: blocks go back to the cache; later work on the same
     // stream is ordered after the work that used them.
+    ctx->live_bytes -= it->second;
     ctx->free_blocks.emplace(it->second, p);
     ctx->live_blocks.erase(it);
 }
@@ -324,8 +325,25 @@ __global__ void __launch_bounds__(256) minmax_kernel(ColView c, uint64_t n, int6
     __shared__ uint64_t scnt[kMaxWaves];
     int64_t mn = INT64_MAX, mx = INT64_MIN;
     uint64_t cnt = 0;
-    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n;
-         i += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+    uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {  // four independent loads in flight
+        int64_t v[4];
+        bool ok[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            ok[k] = is_valid(c, i + k * stride);
+            v[k] = int_value(c, i + k * stride);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (!ok[k]) continue;
+            mn = v[k] < mn ? v[k] : mn;
+            mx = v[k] > mx ? v[k] : mx;
+            ++cnt;
+        }
+    }
+    for (; i < n; i += stride) {
         if (!is_valid(c, i)) continue;
         const int64_t v = int_value(c, i);
         mn = v < mn ? v : mn;

@@ -86,7 +86,8 @@ struct qe_gpu_ctx {
     hipStream_t stream = nullptr;
     std::string last_error;
     uint64_t mem_limit = 0;
-    uint64_t mem_in_use = 0;
+    uint64_t mem_in_use = 0;  // held from the device: live + cached blocks
+    uint64_t live_bytes = 0;  // live blocks only
     std::multimap<size_t, void*> free_blocks;  // size -> block (cached)
     std::map<void*, size_t> live_blocks;       // block -> size
     bool timing = false;

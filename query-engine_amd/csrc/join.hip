@@ -643,8 +643,97 @@ __global__ void __launch_bounds__(1024) map_part_scatter(ColView key, uint64_t n
     }
 }
 
-template <typename MapT>
-__global__ void __launch_bounds__(1024) map_part_build(const uint64_t* ent, const uint64_t* offs, uint32_t ntiles,
+// In-place exclusive scan of a[0, n) (n <= 4 * blockDim.x) by one block;
+// returns the total. `ws` holds kMaxWaves + 1 words of scratch.
+__device__ inline uint32_t block_exclusive_scan(uint32_t* a, uint32_t n, uint32_t* ws) {
+    const uint32_t t = threadIdx.x, lane = lane_id(), w = t / kWave, nw = blockDim.x / kWave;
+    uint32_t v[4], sum = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t i = 4 * t + k;
+        v[k] = i < n ? a[i] : 0u;
+        sum += v[k];
+    }
+    uint32_t x = sum;  // inclusive wave scan of the per-thread sums
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if (lane >= static_cast<unsigned>(off)) x += y;
+    }
+    if (lane == kWave - 1) ws[w] = x;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t run = 0;
+        for (uint32_t k = 0; k < nw; ++k) {
+            const uint32_t c = ws[k];
+            ws[k] = run;
+            run += c;
+        }
+        ws[kMaxWaves] = run;
+    }
+    __syncthreads();
+    uint32_t base = ws[w] + x - sum;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t i = 4 * t + k;
+        if (i < n) a[i] = base;
+        base += v[k];
+    }
+    const uint32_t total = ws[kMaxWaves];
+    __syncthreads();
+    return total;
+}
+
+// Scatter for 16-bit maps (2^16 keys per partition): the tile is sorted by
+// partition in LDS first, then each partition's run is written contiguously.
+// Entry = (key offset within the partition) << 16 | group slot.
+__global__ void __launch_bounds__(1024) map_part_scatter16(ColView key, uint64_t n, int64_t kmin, uint32_t nparts,
+                                                          uint32_t ntiles, const uint64_t* offs,
+                                                          const uint32_t* gslot, uint32_t* ent) {
+    extern __shared__ __align__(16) char msm[];
+    uint64_t* gbase = reinterpret_cast<uint64_t*>(msm);                 // [nparts]
+    uint32_t* stage = reinterpret_cast<uint32_t*>(gbase + nparts);      // [kMapTile]
+    uint16_t* spart = reinterpret_cast<uint16_t*>(stage + kMapTile);    // [kMapTile]
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(spart + kMapTile);      // [nparts] counts, then local bases
+    uint32_t* ws = cnt + nparts;                                        // [kMaxWaves + 1]
+    for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x) {
+        gbase[i] = offs[size_t(i) * ntiles + blockIdx.x];
+        cnt[i] = 0;
+    }
+    __syncthreads();
+    constexpr int kRpt = kMapTile / 1024;
+    const uint64_t t0 = uint64_t(blockIdx.x) * kMapTile;
+    const uint32_t m = static_cast<uint32_t>(min<uint64_t>(kMapTile, n - t0));
+    uint32_t part[kRpt], pos[kRpt], ev[kRpt];
+#pragma unroll
+    for (int k = 0; k < kRpt; ++k) {
+        const uint32_t i = k * blockDim.x + threadIdx.x;
+        const uint64_t r = t0 + i;
+        part[k] = 0xFFFFFFFFu;
+        if (i < m && is_valid(key, r)) {
+            const uint64_t off = static_cast<uint64_t>(int_value(key, r) - kmin);
+            part[k] = static_cast<uint32_t>(off >> 16);
+            pos[k] = atomicAdd(&cnt[part[k]], 1u);
+            ev[k] = static_cast<uint32_t>((off & 0xFFFFu) << 16) | (gslot[r] & 0xFFFFu);
+        }
+    }
+    __syncthreads();
+    const uint32_t total = block_exclusive_scan(cnt, nparts, ws);
+#pragma unroll
+    for (int k = 0; k < kRpt; ++k) {
+        if (part[k] == 0xFFFFFFFFu) continue;
+        const uint32_t at = cnt[part[k]] + pos[k];
+        stage[at] = ev[k];
+        spart[at] = static_cast<uint16_t>(part[k]);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
+        const uint32_t pp = spart[i];
+        ent[gbase[pp] + (i - cnt[pp])] = stage[i];
+    }
+}
+
+template <typename MapT, typename EntT>
+__global__ void __launch_bounds__(1024) map_part_build(const EntT* ent, const uint64_t* offs, uint32_t ntiles,
                                                        uint32_t nparts, uint64_t total, uint32_t pb, uint64_t range,
                                                        MapT* gmap, unsigned int* dup) {
     extern __shared__ __align__(16) char msm[];
@@ -660,10 +749,11 @@ __global__ void __launch_bounds__(1024) map_part_build(const uint64_t* ent, cons
     bool d = false;
     for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
         const uint64_t v = ent[i];
-        const uint32_t o = static_cast<uint32_t>(v >> 32);
+        constexpr int kShift = sizeof(EntT) == 4 ? 16 : 32;  // offset | slot packing of the scatter
+        const uint32_t o = static_cast<uint32_t>(v >> kShift);
         const uint32_t bit = 1u << (o & 31);
         if (atomicOr(&bm[o >> 5], bit) & bit) d = true;
-        m[o] = static_cast<MapT>(static_cast<uint32_t>(v));
+        m[o] = static_cast<MapT>(static_cast<uint32_t>(v & ((uint64_t(1) << kShift) - 1)));
     }
     if (d) atomicOr(dup, 1u);
     __syncthreads();
@@ -688,16 +778,27 @@ MapBuild build_direct_map(qe_gpu_ctx* ctx, const qe_dev_col* key, uint64_t nb, i
     map_part_hist<<<ntiles, 1024, nparts * 4, ctx->stream>>>(kv, nb, kmin, pb, nparts, ntiles, hist.as<uint32_t>());
     QE_HIP(hipGetLastError());
     const uint64_t total = exclusive_scan_u32(ctx, hist.as<uint32_t>(), offs.as<uint64_t>(), nh);
-    Scratch ent(ctx, std::max<uint64_t>(total, 1) * 8);
-    map_part_scatter<<<ntiles, 1024, nparts * 12, ctx->stream>>>(kv, nb, kmin, pb, nparts, ntiles,
-                                                                 offs.as<uint64_t>(), gslot, ent.as<uint64_t>());
+    constexpr bool k16 = sizeof(MapT) == 2;  // 4-byte entries, LDS-sorted scatter
+    Scratch ent(ctx, std::max<uint64_t>(total, 1) * (k16 ? 4 : 8));
+    if (k16)
+        map_part_scatter16<<<ntiles, 1024, size_t(nparts) * 12 + size_t(kMapTile) * 6 + (kMaxWaves + 1) * 4,
+                             ctx->stream>>>(kv, nb, kmin, nparts, ntiles, offs.as<uint64_t>(), gslot,
+                                            ent.as<uint32_t>());
+    else
+        map_part_scatter<<<ntiles, 1024, nparts * 12, ctx->stream>>>(kv, nb, kmin, pb, nparts, ntiles,
+                                                                     offs.as<uint64_t>(), gslot, ent.as<uint64_t>());
     QE_HIP(hipGetLastError());
     gmap = Scratch(ctx, range * sizeof(MapT));
     memset_async(ctx, dup.p, 0, 4);
     const size_t lds = (size_t(1) << pb) * sizeof(MapT) + ((size_t(1) << pb) / 32) * 4;
-    map_part_build<MapT><<<nparts, 1024, lds, ctx->stream>>>(ent.as<uint64_t>(), offs.as<uint64_t>(), ntiles, nparts,
-                                                             total, pb, range, gmap.as<MapT>(),
-                                                             dup.as<unsigned int>());
+    if (k16)
+        map_part_build<MapT, uint32_t><<<nparts, 1024, lds, ctx->stream>>>(
+            ent.as<uint32_t>(), offs.as<uint64_t>(), ntiles, nparts, total, pb, range, gmap.as<MapT>(),
+            dup.as<unsigned int>());
+    else
+        map_part_build<MapT, uint64_t><<<nparts, 1024, lds, ctx->stream>>>(
+            ent.as<uint64_t>(), offs.as<uint64_t>(), ntiles, nparts, total, pb, range, gmap.as<MapT>(),
+            dup.as<unsigned int>());
     QE_HIP(hipGetLastError());
     return read_scalar(ctx, dup.as<unsigned int>()) ? MapBuild::kDuplicate : MapBuild::kOk;
 }
