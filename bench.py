@@ -285,6 +285,21 @@ def batched_workload(qe, ctx, args):
     return res
 
 
+PMC_SUMMARY = ROOT / "profiles" / "join_agg_stream_pmc_build2e24_probe2e28.md"
+
+
+def measured_traffic(args):
+    """HBM bytes per launch of the probe kernel, from the committed rocprofv3
+    PMC summary of this workload (FETCH_SIZE and WRITE_SIZE per dispatch, each
+    scaled by what the counter reads for a copy kernel of known size); None for
+    other workload sizes or without the summary."""
+    import re
+    if (args.build_log2, args.probe_log2, args.zipf, args.miss) != (24, 28, 1.0, 0.1) or not PMC_SUMMARY.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", PMC_SUMMARY.read_text())
+    return float(m.group(1)) if m else None
+
+
 def copy_ceiling(qe, ctx, nbytes=1 << 30, reps=5):
     """Measured HBM ceiling: a plain 16-byte streaming copy (read + write)."""
     import ctypes as C
@@ -409,7 +424,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "int64 keys, f64 accumulate",
+        "dtype": "int64,f64",
         "data": "synthetic (seeded, generated in HBM)",
         "config": {
             "workload": "inner hash join + group-by: build 2^%d rows (unique I64 key permutation, I32 group in "
@@ -420,7 +435,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm", "achieved": kach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": (kach / HBM_PEAK_GBS) if kach else None, "traffic": None,
+            "frac": (kach / HBM_PEAK_GBS) if kach else None, "traffic": measured_traffic(args),
             "kernel": res["kernel"], "kernel_ms": res["kernel_ms"],
             "algorithmic_bytes_per_launch": res["probe_bytes"],
             "measured_copy_ceiling_gbs": ceiling["gbs"],

@@ -35,6 +35,11 @@ struct AggProg {
     // non-NULL count equals the group's row count.
     int32_t acc_of[kMaxAggs];  // index among accumulators, or -1
     int32_t cnt_of[kMaxAggs];  // index among counters, or -1
+    // Aggregates over the same column share state: SUM(x) and AVG(x) one
+    // accumulator, COUNT/SUM/AVG/MIN/MAX(x) one non-NULL counter. Only the
+    // owner (first such aggregate) updates a shared field.
+    int32_t own_acc[kMaxAggs];
+    int32_t own_cnt[kMaxAggs];
     int32_t acc_kind[kMaxAggs];  // AggKind of accumulator i
     int32_t nacc, ncnt;
 };
@@ -110,8 +115,8 @@ __device__ inline void lds_update(char* smem, uint32_t S, const AggProg& p, uint
         if (k == K_COUNT_STAR) continue;
         const uint64_t r = vrow[a];
         if (!is_valid(p.in[a], r)) continue;
-        if (p.cnt_of[a] >= 0) atomicAdd(&cnt[size_t(p.cnt_of[a]) * S + slot], 1u);
-        if (p.acc_of[a] < 0) continue;
+        if (p.cnt_of[a] >= 0 && p.own_cnt[a]) atomicAdd(&cnt[size_t(p.cnt_of[a]) * S + slot], 1u);
+        if (p.acc_of[a] < 0 || !p.own_acc[a]) continue;
         uint64_t* ap = &acc[size_t(p.acc_of[a]) * S + slot];
         switch (k) {
             case K_SUM_I:
@@ -155,8 +160,8 @@ __device__ inline void lds_update_vals(char* smem, uint32_t S, const AggProg& p,
         if (k == K_COUNT_STAR) continue;
         const int c = vcol[a];
         if (!((valid >> c) & 1u)) continue;
-        if (p.cnt_of[a] >= 0) atomicAdd(&cnt[size_t(p.cnt_of[a]) * S + slot], 1u);
-        if (p.acc_of[a] < 0) continue;
+        if (p.cnt_of[a] >= 0 && p.own_cnt[a]) atomicAdd(&cnt[size_t(p.cnt_of[a]) * S + slot], 1u);
+        if (p.acc_of[a] < 0 || !p.own_acc[a]) continue;
         uint64_t* ap = &acc[size_t(p.acc_of[a]) * S + slot];
         const uint64_t v = bits[c];
         switch (k) {
@@ -216,10 +221,10 @@ __device__ inline void global_update(uint64_t* st, uint64_t S, const AggProg& p,
         if (k == K_COUNT_STAR) continue;
         const uint64_t r = vrow[a];
         if (!is_valid(p.in[a], r)) continue;
-        if (p.cnt_of[a] >= 0)
+        if (p.cnt_of[a] >= 0 && p.own_cnt[a])
             atomicAdd(reinterpret_cast<unsigned long long*>(&st[(1 + size_t(p.nacc) + p.cnt_of[a]) * S + slot]),
                       1ull);
-        if (p.acc_of[a] < 0) continue;
+        if (p.acc_of[a] < 0 || !p.own_acc[a]) continue;
         uint64_t* acc = st + (1 + size_t(p.acc_of[a])) * S + slot;
         switch (k) {
             case K_SUM_I:

@@ -43,6 +43,7 @@ constexpr size_t kPartLds = 128 * 1024;  // LDS of one partition's table (1024-t
 constexpr uint32_t kMaxParts = 8192;     // partitions of one query
 constexpr uint32_t kPartAggThreads = 1024;
 constexpr uint64_t kChunkEntries = 1 << 16;  // expected entries per partition-aggregation block
+constexpr uint32_t kSmallTileFactor = 4;     // small-kind tiles are this many times longer
 
 enum BatchKind : int32_t { BK_SMALL = 0, BK_PART = 1, BK_HASH = 2 };
 
@@ -62,7 +63,7 @@ struct QueryDev {
     uint64_t ebase;   // part: first partitioned entry
     uint64_t rows;
     uint32_t chunks;  // part: blocks per partition (slices of equal length)
-    uint32_t pad2;
+    int32_t on_value; // every aggregate reads the value column (or is COUNT(*))
 };
 
 struct Tile {
@@ -122,8 +123,8 @@ __device__ inline bool cache_update(char* smem, uint32_t C, const AggProg& p, ui
     for (int a = 0; a < p.n; ++a) {
         const int k = p.kind[a];
         if (k == K_COUNT_STAR || !is_valid(p.in[a], r)) continue;
-        if (p.cnt_of[a] >= 0) atomicAdd(&cnt[size_t(p.cnt_of[a]) * C + e], 1u);
-        if (p.acc_of[a] < 0) continue;
+        if (p.cnt_of[a] >= 0 && p.own_cnt[a]) atomicAdd(&cnt[size_t(p.cnt_of[a]) * C + e], 1u);
+        if (p.acc_of[a] < 0 || !p.own_acc[a]) continue;
         uint64_t* ap = &acc[size_t(p.acc_of[a]) * C + e];
         switch (k) {
             case K_SUM_I:
@@ -213,16 +214,40 @@ __device__ inline void hash_tile(const QueryDev& q, const Tile& tile, char* smem
     }
 }
 
+constexpr int kRowUnroll = 4;  // rows per thread per step, loads issued first
+__constant__ int32_t kValueCol[kMaxAggs] = {};  // every aggregate reads value column 0
+
 // Small kind: dense LDS table over the query's S slots, folded into HBM with
 // one atomic per touched slot field.
 __device__ inline void small_tile(const QueryDev& q, const Tile& tile, char* smem) {
     const uint32_t S = static_cast<uint32_t>(q.S);
     lds_init(smem, S, q.p);
     __syncthreads();
-    uint64_t vrow[kMaxAggs];
-    for (uint64_t r = tile.begin + threadIdx.x; r < tile.end; r += blockDim.x) {
-        for (int a = 0; a < q.p.n; ++a) vrow[a] = r;
-        lds_update(smem, S, q.p, direct_slot(q, r), vrow);
+    if (q.on_value) {
+        for (uint64_t r0 = tile.begin; r0 < tile.end; r0 += kRowUnroll * uint64_t(blockDim.x)) {
+            uint32_t slot[kRowUnroll];
+            uint64_t bits[kRowUnroll];
+            uint32_t valid[kRowUnroll];
+#pragma unroll
+            for (int u = 0; u < kRowUnroll; ++u) {
+                const uint64_t r = r0 + uint64_t(u) * blockDim.x + threadIdx.x;
+                slot[u] = kNoSlot;
+                if (r < tile.end) {
+                    slot[u] = direct_slot(q, r);
+                    bits[u] = value_bits(q.val, r);
+                    valid[u] = is_valid(q.val, r) ? 1u : 0u;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kRowUnroll; ++u)
+                if (slot[u] != kNoSlot) lds_update_vals(smem, S, q.p, slot[u], kValueCol, &bits[u], valid[u]);
+        }
+    } else {
+        uint64_t vrow[kMaxAggs];
+        for (uint64_t r = tile.begin + threadIdx.x; r < tile.end; r += blockDim.x) {
+            for (int a = 0; a < q.p.n; ++a) vrow[a] = r;
+            lds_update(smem, S, q.p, direct_slot(q, r), vrow);
+        }
     }
     __syncthreads();
     const uint64_t* acc = reinterpret_cast<const uint64_t*>(smem);
@@ -252,12 +277,28 @@ __device__ inline void scatter_tile(const QueryDev& q, const Tile& tile, char* s
     }
     __syncthreads();
     const uint32_t mask = (1u << q.pbits) - 1;
-    for (uint64_t r = tile.begin + threadIdx.x; r < tile.end; r += blockDim.x) {
-        const uint32_t s = direct_slot(q, r);
-        const uint32_t part = s >> q.pbits;
-        const uint64_t pos = base[part] + atomicAdd(&cur[part], 1u);
-        ekey[pos] = (s & mask) | (static_cast<uint32_t>(is_valid(q.val, r)) << 31);
-        eval[pos] = value_bits(q.val, r);
+    for (uint64_t r0 = tile.begin; r0 < tile.end; r0 += kRowUnroll * uint64_t(blockDim.x)) {
+        uint32_t slot[kRowUnroll];
+        uint64_t bits[kRowUnroll];
+        uint32_t valid[kRowUnroll];
+#pragma unroll
+        for (int u = 0; u < kRowUnroll; ++u) {
+            const uint64_t r = r0 + uint64_t(u) * blockDim.x + threadIdx.x;
+            slot[u] = kNoSlot;
+            if (r < tile.end) {
+                slot[u] = direct_slot(q, r);
+                bits[u] = value_bits(q.val, r);
+                valid[u] = is_valid(q.val, r) ? 1u : 0u;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kRowUnroll; ++u) {
+            if (slot[u] == kNoSlot) continue;
+            const uint32_t part = slot[u] >> q.pbits;
+            const uint64_t pos = base[part] + atomicAdd(&cur[part], 1u);
+            ekey[pos] = (slot[u] & mask) | (valid[u] << 31);
+            eval[pos] = bits[u];
+        }
     }
 }
 
@@ -298,7 +339,17 @@ __global__ void __launch_bounds__(256) batch_hist_kernel(const QueryDev* qs, con
         const QueryDev& q = qs[t.q];
         for (uint32_t p = threadIdx.x; p < q.nparts; p += blockDim.x) hcnt[p] = 0;
         __syncthreads();
-        for (uint64_t r = t.begin + threadIdx.x; r < t.end; r += blockDim.x) atomicAdd(&hcnt[direct_slot(q, r) >> q.pbits], 1u);
+        for (uint64_t r0 = t.begin; r0 < t.end; r0 += kRowUnroll * uint64_t(blockDim.x)) {
+            uint32_t slot[kRowUnroll];
+#pragma unroll
+            for (int u = 0; u < kRowUnroll; ++u) {
+                const uint64_t r = r0 + uint64_t(u) * blockDim.x + threadIdx.x;
+                slot[u] = r < t.end ? direct_slot(q, r) : kNoSlot;
+            }
+#pragma unroll
+            for (int u = 0; u < kRowUnroll; ++u)
+                if (slot[u] != kNoSlot) atomicAdd(&hcnt[slot[u] >> q.pbits], 1u);
+        }
         __syncthreads();
         for (uint32_t p = threadIdx.x; p < q.nparts; p += blockDim.x) hist[q.hbase + uint64_t(p) * q.ntiles + t.qt] = hcnt[p];
         __syncthreads();
@@ -347,8 +398,6 @@ __global__ void __launch_bounds__(kPartAggThreads) batch_part_agg_kernel(const Q
                                                                          const uint64_t* eval) {
     extern __shared__ __align__(16) char smem[];
     __shared__ uint64_t cur;
-    int32_t vcol[kMaxAggs];
-    for (int a = 0; a < kMaxAggs; ++a) vcol[a] = 0;  // every aggregate reads the value column
     while (true) {
         if (threadIdx.x == 0) cur = atomicAdd(next, 1ull);
         __syncthreads();
@@ -363,10 +412,21 @@ __global__ void __launch_bounds__(kPartAggThreads) batch_part_agg_kernel(const Q
         const uint64_t pb = offs[q.hbase + uint64_t(it.part) * q.ntiles];
         const uint64_t pe = (it.part + 1 < q.nparts) ? offs[q.hbase + uint64_t(it.part + 1) * q.ntiles] : q.ebase + q.rows;
         const uint64_t b = pb + (pe - pb) * it.chunk / q.chunks, e = pb + (pe - pb) * (it.chunk + 1) / q.chunks;
-        for (uint64_t k = b + threadIdx.x; k < e; k += blockDim.x) {
-            const uint32_t w = ekey[k];
-            const uint64_t bits = eval[k];
-            lds_update_vals(smem, cnt, q.p, w & 0x7FFFFFFFu, vcol, &bits, w >> 31);
+        for (uint64_t k0 = b; k0 < e; k0 += kRowUnroll * uint64_t(blockDim.x)) {
+            uint32_t w[kRowUnroll];
+            uint64_t bits[kRowUnroll];
+#pragma unroll
+            for (int u = 0; u < kRowUnroll; ++u) {
+                const uint64_t k = k0 + uint64_t(u) * blockDim.x + threadIdx.x;
+                w[u] = kNoSlot;
+                if (k < e) {
+                    w[u] = __builtin_nontemporal_load(ekey + k);
+                    bits[u] = __builtin_nontemporal_load(eval + k);
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kRowUnroll; ++u)
+                if (w[u] != kNoSlot) lds_update_vals(smem, cnt, q.p, w[u] & 0x7FFFFFFFu, kValueCol, &bits[u], w[u] >> 31);
         }
         __syncthreads();
         const uint64_t* acc = reinterpret_cast<const uint64_t*>(smem);
@@ -480,6 +540,7 @@ static void build_batch_plan(qe_gpu_ctx* ctx, BatchPlan& bp, const qe_batch_quer
         d.hk.key[0] = view(q.key);
         d.val = view(q.value);
         d.rows = n;
+        d.on_value = on_value ? 1 : 0;
         d.kind = BK_HASH;
         // Dense integer keys: direct slots (small or partitioned).
         if (is_int_like(q.key->type) && q.key->type != QE_BOOL) {
@@ -537,10 +598,12 @@ static void make_layout(qe_gpu_ctx* ctx, BatchPlan& bp, uint32_t tile_rows) {
     if (bp.tile_rows == tile_rows) return;
     std::vector<std::vector<Tile>> per(bp.nq);
     for (uint32_t q = 0; q < bp.nq; ++q) {
+        // Small tables fold into shared state once per tile: longer tiles, fewer flushes.
+        const uint64_t tr = bp.hq[q].kind == BK_SMALL ? uint64_t(tile_rows) * kSmallTileFactor : tile_rows;
         uint32_t qt = 0;
         for (size_t m = 0; m + 1 < bp.morsels[q].size(); ++m)
-            for (uint64_t b = bp.morsels[q][m]; b < bp.morsels[q][m + 1]; b += tile_rows)
-                per[q].push_back(Tile{q, qt++, b, std::min<uint64_t>(bp.morsels[q][m + 1], b + tile_rows)});
+            for (uint64_t b = bp.morsels[q][m]; b < bp.morsels[q][m + 1]; b += tr)
+                per[q].push_back(Tile{q, qt++, b, std::min<uint64_t>(bp.morsels[q][m + 1], b + tr)});
     }
     // Interleave queries so concurrent blocks work on different tables.
     std::vector<Tile> kt[3], ht;

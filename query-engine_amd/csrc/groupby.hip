@@ -61,15 +61,32 @@ AggProg make_prog(const qe_agg_spec* aggs, uint32_t naggs, const ColView* inputs
         }
     }
     p.nacc = p.ncnt = 0;
+    auto same_col = [&](uint32_t a, uint32_t b) {
+        return p.in[a].data == p.in[b].data && p.in[a].validity == p.in[b].validity && p.in[a].type == p.in[b].type;
+    };
     for (uint32_t a = 0; a < naggs; ++a) {
         const int k = p.kind[a];
         p.acc_of[a] = -1;
         p.cnt_of[a] = -1;
+        p.own_acc[a] = 0;
+        p.own_cnt[a] = 0;
         if (k != K_COUNT_STAR && k != K_COUNT) {
-            p.acc_kind[p.nacc] = k;
-            p.acc_of[a] = p.nacc++;
+            for (uint32_t b = 0; b < a && p.acc_of[a] < 0; ++b)
+                if (p.kind[b] == k && p.acc_of[b] >= 0 && same_col(a, b)) p.acc_of[a] = p.acc_of[b];
+            if (p.acc_of[a] < 0) {
+                p.acc_kind[p.nacc] = k;
+                p.acc_of[a] = p.nacc++;
+                p.own_acc[a] = 1;
+            }
         }
-        if (k != K_COUNT_STAR && p.in[a].validity != nullptr) p.cnt_of[a] = p.ncnt++;
+        if (k != K_COUNT_STAR && p.in[a].validity != nullptr) {
+            for (uint32_t b = 0; b < a && p.cnt_of[a] < 0; ++b)
+                if (p.cnt_of[b] >= 0 && same_col(a, b)) p.cnt_of[a] = p.cnt_of[b];
+            if (p.cnt_of[a] < 0) {
+                p.cnt_of[a] = p.ncnt++;
+                p.own_cnt[a] = 1;
+            }
+        }
     }
     return p;
 }
