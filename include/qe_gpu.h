@@ -145,6 +145,36 @@ qe_gpu_status qe_gpu_aggregate(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, u
                                const uint32_t* key_cols, uint32_t nkeys,
                                const qe_agg_spec* aggs, uint32_t naggs, qe_dev_table** out);
 
+/* ---- filter ---------------------------------------------------------------------*/
+/* Comparison operators, same order as expr::CmpOp (expr/expr.h:79). */
+typedef enum qe_cmp_op {
+    QE_CMP_LT = 0, QE_CMP_LE = 1, QE_CMP_GT = 2, QE_CMP_GE = 3, QE_CMP_EQ = 4, QE_CMP_NE = 5
+} qe_cmp_op;
+/* One term `column op literal` (expr::cmp(op, col(t, i), lit(s))). The literal
+ * is typed like expr::Scalar (expr/expr.h:84-104): lit_i carries I32/I64/TS/
+ * BOOL, lit_f carries F64, lit_is_null marks a typed NULL. */
+typedef struct qe_pred_term {
+    uint32_t col;
+    int32_t op;
+    int32_t lit_type;
+    int32_t lit_is_null;
+    int64_t lit_i;
+    double lit_f;
+} qe_pred_term;
+/* Replaces qe::Filter(child, predicate) (ops/filter.h, ops/filter.cpp:28-66)
+ * for a predicate that is a conjunction (expr::logic_and) of such terms: rows
+ * where every term is TRUE pass (a NULL comparison does not), operands are
+ * promoted as expr::cmp promotes them, and every column is compacted with its
+ * validity, passing rows in input order (compact_column,
+ * core/owned_batch.cpp:101-166). */
+qe_gpu_status qe_gpu_filter(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, uint32_t ncols,
+                            const qe_pred_term* terms, uint32_t nterms, qe_dev_table** out);
+/* The same predicate, returning the passing row indices (a device selection
+ * vector, ascending) in sel_out (device u32[len]) and their count. */
+qe_gpu_status qe_gpu_filter_select(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, uint32_t ncols,
+                                   const qe_pred_term* terms, uint32_t nterms, uint32_t* sel_out,
+                                   uint64_t* count_out);
+
 /* ---- hash join ----------------------------------------------------------------*/
 /* Replaces qe::HashJoin(probe, build, probe_keys, build_keys, type)
  * (ops/join.h:79-81; build_side ops/join.cpp:133-197, build_pairs_for_probe

@@ -21,6 +21,10 @@
 //                             :118-318 and tests/catalog_checks_asof.cpp:38-92,
 //                             brute force O(n*m), numpy searchsorted, _ref.
 //   * sort_perm             : numpy lexsort (stable) on random keys.
+//   * filter_select         : literal cases of tests/ops_test.cpp:99-146 and
+//                             tests/expr_null_truthtable_test.cpp, an
+//                             independent numpy model, _ref (Filter over
+//                             expr::cmp / logic_and).
 //   * morsel driver         : equality with the single-thread path.
 //
 // Mutants (the mutation catalog) are planted HERE, on the CPU, selected by an
@@ -854,6 +858,84 @@ int orc_sort_perm(const orc_col* cols, const uint32_t* keys, const int32_t* desc
     const auto p = sort_perm_impl(cols, keys, desc, nulls_first, nk, n);
     std::memcpy(perm, p.data(), n * 4);
     return 0;
+}
+
+}  // extern "C"
+
+// --- filter ----------------------------------------------------------------
+// Filter with a conjunction of column-versus-literal comparisons, restating
+// ops/filter.cpp:28-66 (a row passes iff the predicate is valid AND true;
+// passing rows keep input order) over expr::cmp (expr/expr.cpp:133-151: two
+// numerics are promoted to the higher of F64 > I64 > I32, otherwise BOOL/BOOL
+// and TS/TS compare in place), null-if-any comparison and IEEE NaN
+// (expr/expr.h:22-40, expr/compare_scalar.cpp:1-40), and three-valued AND
+// (a conjunction is TRUE only if every operand is TRUE).
+//
+// Mutants: 1 = a NULL comparison passes; 2 = `<=`/`>=` evaluated strictly;
+// 3 = an F64 comparison done on integers (literal truncated, no promotion);
+// 4 = the decision inverted (oracle_mutation_test.cpp:48-79 BuggyFilter).
+
+extern "C" {
+typedef struct {
+    uint32_t col;
+    int32_t op;  // 0 Lt, 1 Le, 2 Gt, 3 Ge, 4 Eq, 5 Ne (expr/expr.h:79)
+    int32_t lit_type;
+    int32_t lit_is_null;
+    int64_t lit_i;
+    double lit_f;
+} orc_pred;
+}
+
+namespace {
+
+template <typename T>
+bool cmp_apply(int op, T a, T b, int mutant) {
+    if (mutant == 2 && op == 1) op = 0;
+    if (mutant == 2 && op == 3) op = 2;
+    switch (op) {
+        case 0: return a < b;
+        case 1: return a <= b;
+        case 2: return a > b;
+        case 3: return a >= b;
+        case 4: return a == b;
+        default: return a != b;
+    }
+}
+
+// Three-valued result of one comparison: -1 NULL, 0 FALSE, 1 TRUE.
+int cmp_term(const orc_col& c, const orc_pred& t, uint64_t r, int mutant) {
+    if (t.lit_is_null || !valid_at(c, r)) return -1;
+    const bool as_float = (c.type == F64 || t.lit_type == F64) && mutant != 3;
+    if (as_float) {
+        const double a = c.type == F64 ? f64_at(c, r) : static_cast<double>(int_at(c, r));
+        const double b = t.lit_type == F64 ? t.lit_f : static_cast<double>(t.lit_i);
+        return cmp_apply<double>(t.op, a, b, mutant) ? 1 : 0;
+    }
+    const int64_t a = c.type == F64 ? static_cast<int64_t>(f64_at(c, r)) : int_at(c, r);
+    const int64_t b = t.lit_type == F64 ? static_cast<int64_t>(t.lit_f) : t.lit_i;
+    return cmp_apply<int64_t>(t.op, a, b, mutant) ? 1 : 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+// Writes the passing row indices (ascending) to `sel`; returns their count.
+uint64_t orc_filter_select(const orc_col* cols, uint64_t n, const orc_pred* terms,
+                           uint32_t nterms, uint32_t* sel, int32_t mutant) {
+    uint64_t m = 0;
+    for (uint64_t r = 0; r < n; ++r) {
+        int v = 1;  // AND identity
+        for (uint32_t j = 0; j < nterms; ++j) {
+            const int x = cmp_term(cols[terms[j].col], terms[j], r, mutant);
+            if (x == 0) { v = 0; break; }  // FALSE AND anything = FALSE
+            if (x < 0) v = -1;             // NULL AND TRUE = NULL
+        }
+        bool pass = v == 1 || (mutant == 1 && v < 0);
+        if (mutant == 4) pass = !(v == 1);
+        if (pass) sel[m++] = static_cast<uint32_t>(r);
+    }
+    return m;
 }
 
 }  // extern "C"

@@ -5,7 +5,8 @@
 // library goes to oracle/_ref/ and is never committed. It lets the tests pin the
 // CPU restatement (oracle/qe_oracle.cpp) against the reference's own
 // HashTable (ops/hashtable.h:108-190), Aggregate (ops/aggregate.h:102-144),
-// HashJoin (ops/join.h:69-123) and AsofJoin (tsx/asof.h:86-150), driven
+// HashJoin (ops/join.h:69-123), AsofJoin (tsx/asof.h:86-150) and Filter
+// (ops/filter.h) over expr::cmp / expr::logic_and (expr/expr.h:160-180), driven
 // through Scan (ops/scan.h:37) exactly as the reference's tests drive them, and
 // it serves as the "reference" CPU baseline.
 //
@@ -20,7 +21,9 @@
 
 #include "core/owned_batch.h"
 #include "core/validity.h"
+#include "expr/expr.h"
 #include "ops/aggregate.h"
+#include "ops/filter.h"
 #include "ops/hashtable.h"
 #include "ops/join.h"
 #include "ops/scan.h"
@@ -265,6 +268,36 @@ void* ref_asof_run(const orc_col* pcols, uint32_t npc, uint64_t np, const orc_co
                      std::vector<uint32_t>(pkeys, pkeys + nkeys),
                      std::vector<uint32_t>(bkeys, bkeys + nkeys), ptime, btime,
                      type ? tsx::AsofType::Left : tsx::AsofType::Inner, t);
+    return drain(op);
+}
+
+// Filter(Scan(t), cmp_0 AND cmp_1 AND ...) with a row-id column appended; the
+// result's last column holds the passing rows' input indices.
+typedef struct {
+    uint32_t col;
+    int32_t op;
+    int32_t lit_type;
+    int32_t lit_is_null;
+    int64_t lit_i;
+    double lit_f;
+} ref_pred;
+
+void* ref_filter_run(const orc_col* cols, uint32_t ncols, uint64_t n, const ref_pred* terms,
+                     uint32_t nterms, uint64_t batch_size) {
+    auto t = make_table(cols, ncols, n, true);
+    expr::Expr pred;
+    for (uint32_t j = 0; j < nterms; ++j) {
+        const ref_pred& p = terms[j];
+        expr::Scalar s;
+        s.type = ty(p.lit_type);
+        s.is_null = p.lit_is_null != 0;
+        s.i = p.lit_i;
+        s.f = p.lit_f;
+        expr::Expr c = expr::cmp(static_cast<expr::CmpOp>(p.op),
+                                 expr::col(ty(cols[p.col].type), p.col), expr::lit(s));
+        pred = j == 0 ? c : expr::logic_and(pred, c);
+    }
+    Filter op(std::make_unique<Scan>(*t, batch_size), pred);
     return drain(op);
 }
 

@@ -66,6 +66,11 @@ class AggSpec(C.Structure):
     _fields_ = [("func", C.c_int32), ("input_col", C.c_uint32)]
 
 
+class PredTerm(C.Structure):
+    _fields_ = [("col", C.c_uint32), ("op", C.c_int32), ("lit_type", C.c_int32), ("lit_is_null", C.c_int32),
+                ("lit_i", C.c_int64), ("lit_f", C.c_double)]
+
+
 class AsofStats(C.Structure):
     _fields_ = [("build_rows", C.c_uint64), ("partitions", C.c_uint64), ("pieces", C.c_uint64),
                 ("max_piece_error", C.c_uint64), ("router_buckets", C.c_uint64),
@@ -82,6 +87,8 @@ _SIGS = {
     "qe_gpu_sync": (C.c_int, [P]),
     "qe_gpu_set_mem_limit": (C.c_int, [P, C.c_uint64]),
     "qe_gpu_set_timing": (C.c_int, [P, C.c_int]),
+    "qe_gpu_filter": (C.c_int, [P, P, C.c_uint32, P, C.c_uint32, C.POINTER(P)]),
+    "qe_gpu_filter_select": (C.c_int, [P, P, C.c_uint32, P, C.c_uint32, P, C.POINTER(C.c_uint64)]),
     "qe_gpu_set_tuning": (C.c_int, [P, C.c_int32, C.c_int64]),
     "qe_gpu_xcd_of_blocks": (C.c_int, [P, C.c_uint32, P]),
     "qe_gpu_timing": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
@@ -377,6 +384,39 @@ class Context:
                                            len(aggs), C.byref(out)))
         return Table(self, out)
 
+    @staticmethod
+    def _terms(terms):
+        """terms: (col, op, lit_type, value) with value None for a NULL literal."""
+        arr = (PredTerm * max(len(terms), 1))()
+        for i, (col, op, lt, v) in enumerate(terms):
+            arr[i].col, arr[i].op, arr[i].lit_type = col, op, lt
+            arr[i].lit_is_null = 1 if v is None else 0
+            if v is not None:
+                if lt == F64:
+                    arr[i].lit_f = float(v)
+                else:
+                    arr[i].lit_i = int(v)
+        return arr
+
+    def filter(self, cols, terms) -> Table:
+        """Rows where every `col op literal` term is TRUE, every column compacted."""
+        out = P()
+        self._check(lib().qe_gpu_filter(self.h, _cols(cols), len(cols), C.cast(self._terms(terms), P), len(terms),
+                                        C.byref(out)))
+        return Table(self, out)
+
+    def filter_select(self, cols, terms) -> np.ndarray:
+        """Passing row indices (ascending) of the same predicate."""
+        n = cols[0].len
+        buf = DevBuffer(self, 4 * max(n, 1))
+        cnt = C.c_uint64()
+        try:
+            self._check(lib().qe_gpu_filter_select(self.h, _cols(cols), len(cols), C.cast(self._terms(terms), P),
+                                                   len(terms), buf.p, C.byref(cnt)))
+            return buf.numpy(np.uint32, cnt.value)
+        finally:
+            buf.free()
+
     def hash_join(self, probe_keys, build_keys, join_type=INNER) -> Pairs:
         out = P()
         self._check(lib().qe_gpu_hash_join(self.h, _cols(probe_keys), _cols(build_keys), len(probe_keys),
@@ -566,6 +606,7 @@ class BatchTuning(C.Structure):
 
 
 TUNE_JOIN_PAIRS, TUNE_ASOF_BUCKET_KNOTS = 0, 1
+CMP_LT, CMP_LE, CMP_GT, CMP_GE, CMP_EQ, CMP_NE = range(6)
 BATCH_PERSISTENT, BATCH_PER_BATCH, BATCH_GRAPH = 0, 1, 2
 
 

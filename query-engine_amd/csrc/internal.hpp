@@ -282,6 +282,9 @@ void radix_sort_pairs(qe_gpu_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t 
 // Column length helper with validation.
 uint64_t common_len(const qe_dev_col* const* cols, uint32_t n);
 
+// Gather rows idx[0, m) of src into a new column (idx == kNoRow gives NULL).
+qe_dev_col* gather_col(qe_gpu_ctx* ctx, const qe_dev_col* src, const uint32_t* idx, uint64_t m);
+
 // Result column construction helpers (host-side allocation of result tables).
 qe_dev_table* new_table(uint64_t rows);
 

@@ -93,6 +93,7 @@ def oracle() -> C.CDLL:
                                                 C.c_uint64, P, P, P]),
             "orc_bl_asof_upper_bound": (None, [P, P, C.c_uint32, P, P, C.c_uint64, C.c_uint32, P]),
             "orc_bl_hw_threads": (C.c_uint32, []),
+            "orc_filter_select": (C.c_uint64, [P, C.c_uint64, P, C.c_uint32, P, C.c_int32]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -129,6 +130,7 @@ def ref() -> C.CDLL:
                                      C.c_int32, P, C.c_uint32, P, P, C.c_uint32, C.c_uint64]),
             "ref_asof_run": (P, [P, C.c_uint32, C.c_uint64, P, C.c_uint32, C.c_uint64, P, P, C.c_uint32,
                                  C.c_uint32, C.c_uint32, C.c_int32, C.c_int32, C.c_int64, C.c_uint64]),
+            "ref_filter_run": (P, [P, C.c_uint32, C.c_uint64, P, C.c_uint32, C.c_uint64]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -272,6 +274,34 @@ def asof_pairs(probe_cols, build_cols, pkeys, bkeys, ptime, btime, join_type=0, 
     return _pairs(L, h)
 
 
+# ---- filter ----------------------------------------------------------------------
+LT, LE, GT, GE, EQ, NE = range(6)
+
+
+class OrcPred(C.Structure):
+    _fields_ = [("col", C.c_uint32), ("op", C.c_int32), ("lit_type", C.c_int32), ("lit_is_null", C.c_int32),
+                ("lit_i", C.c_int64), ("lit_f", C.c_double)]
+
+
+def _preds(terms):
+    """terms: [(col, op, lit_type, value)] with value None for a NULL literal."""
+    arr = (OrcPred * max(len(terms), 1))()
+    for j, (c, op, lt, v) in enumerate(terms):
+        isnull = v is None
+        arr[j] = OrcPred(c, op, lt, 1 if isnull else 0, 0 if (isnull or lt == F64) else int(v),
+                         float(v) if (lt == F64 and not isnull) else 0.0)
+    return arr
+
+
+def filter_select(cols, terms, mutant=0):
+    """Passing row indices of a conjunction of column-vs-literal comparisons."""
+    L = oracle()
+    cs = Cols(cols)
+    sel = np.empty(max(cs.n, 1), dtype=np.uint32)
+    m = L.orc_filter_select(cs.arr, cs.n, _preds(terms), len(terms), sel.ctypes.data, mutant)
+    return sel[:m].copy()
+
+
 def sort_perm(cols, keys, desc=None, nulls_first=None):
     L = oracle()
     cs = Cols(cols)
@@ -338,3 +368,10 @@ def ref_asof_pairs(probe_cols, build_cols, pkeys, bkeys, ptime, btime, join_type
     p = np.array([r[npc] for r in rows], dtype=np.int64)
     b = np.array([NO_ROW if r[npc + nbc + 1] is None else r[npc + nbc + 1] for r in rows], dtype=np.int64)
     return p.astype(np.uint32), b.astype(np.uint32)
+
+
+def ref_filter_select(cols, terms, batch_size=2048):
+    cs = Cols(cols)
+    h = ref().ref_filter_run(cs.arr, len(cols), cs.n, _preds(terms), len(terms), batch_size)
+    rows = _ref_rows(h)
+    return np.array([r[len(cols)] for r in rows], dtype=np.int64).astype(np.uint32)

@@ -559,3 +559,150 @@ def test_golden_fixtures_match_oracle():
         elif kind == "hashtable":
             t = O.OracleHT(fx["types"], policy=fx["policy"], init_cap=fx["init_cap"], lf=fx["lf"])
             assert t.insert_or_find(cols(fx["cols"])).tolist() == fx["expected_ids"], f.name
+
+
+# ---- filter ------------------------------------------------------------------------------------
+LT, LE, GT, GE, EQ, NE = O.LT, O.LE, O.GT, O.GE, O.EQ, O.NE
+
+
+def seq_table(n):
+    """tests/ops_test.cpp:42-56: c0 = 0..n-1 (NULL where i % 7 == 0), c1 = 1.5 * c0."""
+    i = np.arange(n)
+    return [col(i, I32, tuple(np.nonzero(i % 7 == 0)[0])), col(i * 1.5, F64)]
+
+
+def test_filter_literals():
+    # tests/ops_test.cpp:99-124: c0 >= 100 over 300 rows keeps 100..299 minus the NULL rows.
+    t = seq_table(300)
+    got = O.filter_select(t, [(0, GE, I32, 100)])
+    assert got.tolist() == [i for i in range(100, 300) if i % 7]
+    assert len(O.filter_select(t, [(1, GE, F64, 0.0)])) == 300   # TRUE everywhere keeps all
+    assert len(O.filter_select(t, [(1, LT, F64, 0.0)])) == 0     # FALSE everywhere keeps none
+    # tests/ops_test.cpp:126-146 (c0 > 15 keeps {30, 50}), on the physical rows.
+    assert O.filter_select([col([10, 20, 30, 40, 50], I32)], [(0, GT, I32, 15)]).tolist() == [1, 2, 3, 4]
+    # null-if-any comparison (tests/expr_null_truthtable_test.cpp:74-107): NULL rows never pass.
+    a = col([10, 20, 30, 40], I32, (1, 3))
+    assert O.filter_select([a], [(0, GT, I32, 0)]).tolist() == [0, 2]
+    assert O.filter_select([a], [(0, GT, I32, None)]).tolist() == []   # a NULL literal
+    assert O.filter_select([a], [(0, NE, I32, None)]).tolist() == []
+    # IEEE NaN (expr/expr.h:36-40): every comparison with NaN is FALSE except !=.
+    x = col([np.nan, 1.0, -0.0, 0.0, np.inf], F64)
+    for op, want in [(LT, [2, 3]), (LE, [1, 2, 3]), (GT, [4]), (GE, [1, 4]), (EQ, [1]), (NE, [0, 2, 3, 4])]:
+        assert O.filter_select([x], [(0, op, F64, 1.0)]).tolist() == want, op
+    assert O.filter_select([x], [(0, EQ, F64, 0.0)]).tolist() == [2, 3]         # -0 == +0
+    assert O.filter_select([x], [(0, NE, F64, float("nan"))]).tolist() == [0, 1, 2, 3, 4]
+    assert O.filter_select([x], [(0, EQ, F64, float("nan"))]).tolist() == []
+    # promotion (expr/expr.cpp:133-151): I64 vs F64 literal compares as double;
+    # 2^53 + 1 rounds to 2^53 as a double, so it equals the literal 2^53.
+    big = col([2**53, 2**53 + 1, 2**53 + 2, -(2**63)], I64)
+    assert O.filter_select([big], [(0, EQ, F64, float(2**53))]).tolist() == [0, 1]
+    assert O.filter_select([big], [(0, EQ, I64, 2**53)]).tolist() == [0]
+    assert O.filter_select([big], [(0, LT, F64, -9.2e18)]).tolist() == [3]
+    # I32 vs F64 literal: 2 < 2.5 as a double (not 2 < 2 on truncated integers).
+    assert O.filter_select([col([1, 2, 3], I32)], [(0, LT, F64, 2.5)]).tolist() == [0, 1]
+    # BOOL compares BOOL, TS compares TS.
+    assert O.filter_select([col([0, 1, 1, 0], BOOL)], [(0, EQ, BOOL, 1)]).tolist() == [1, 2]
+    assert O.filter_select([col([5, -5, 7], TS)], [(0, LE, TS, 5)]).tolist() == [0, 1]
+
+
+def test_filter_three_valued_and():
+    # tests/expr_null_truthtable_test.cpp:1-72: AND is TRUE only when both sides
+    # are TRUE; F AND N = F, N AND T = N, and a Filter keeps TRUE rows only.
+    # Rows enumerate (left, right) over {F, T, N} x {F, T, N}.
+    L = col([0, 0, 0, 1, 1, 1, 0, 0, 0], I32, (6, 7, 8))
+    R = col([0, 1, 0, 0, 1, 0, 0, 1, 0], I32, (2, 5, 8))
+    got = O.filter_select([L, R], [(0, EQ, I32, 1), (1, EQ, I32, 1)])
+    assert got.tolist() == [4]
+
+
+def _numpy_filter(cols, terms):
+    """Independent model: numpy comparisons after promotion, ANDed with validity."""
+    n = len(cols[0][0])
+    keep = np.ones(n, dtype=bool)
+    ops = [np.less, np.less_equal, np.greater, np.greater_equal, np.equal, np.not_equal]
+    for c, op, lt, v in terms:
+        vals, valid, t = cols[c]
+        if v is None:
+            return np.zeros(0, dtype=np.uint32)
+        if t == F64 or lt == F64:
+            res = ops[op](np.asarray(vals).astype(np.float64), np.float64(v))
+        else:
+            res = ops[op](np.asarray(vals).astype(np.int64), np.int64(v))
+        keep &= res if valid is None else (res & valid)
+    return np.nonzero(keep)[0].astype(np.uint32)
+
+
+def _rand_filter_case(r, n):
+    def nulls():
+        return np.ones(n, dtype=bool) if r.random() < 0.3 else r.random(n) >= 0.1
+    f = r.normal(0, 10, n)
+    f[r.random(n) < 0.05] = np.nan
+    cols = [(r.integers(-50, 50, n).astype(np.int32), nulls(), I32),
+            (r.integers(-10**12, 10**12, n), nulls(), I64),
+            (np.round(f, 1), nulls(), F64),
+            (r.integers(0, 2, n).astype(np.uint8), nulls(), BOOL),
+            (r.integers(0, 1000, n), nulls(), TS)]
+    cols = [(v, None if m.all() else m, t) for v, m, t in cols]
+    terms = []
+    for _ in range(int(r.integers(1, 5))):
+        c = int(r.integers(0, 5))
+        op = int(r.integers(0, 6))
+        t = cols[c][2]
+        if t == I32:
+            lt = [I32, I64, F64][int(r.integers(0, 3))]
+            v = float(np.round(r.normal(0, 30), 1)) if lt == F64 else int(r.integers(-60, 60))
+        elif t == I64:
+            lt = [I32, I64, F64][int(r.integers(0, 3))]
+            v = (float(r.integers(-10**12, 10**12)) if lt == F64 else
+                 int(r.integers(-2**31, 2**31)) if lt == I32 else int(r.integers(-10**12, 10**12)))
+        elif t == F64:
+            lt = [I32, I64, F64][int(r.integers(0, 3))]
+            v = float(np.round(r.normal(0, 10), 1)) if lt == F64 else int(r.integers(-20, 20))
+            if lt == F64 and r.random() < 0.1:
+                v = float("nan")
+        elif t == BOOL:
+            lt, v = BOOL, int(r.integers(0, 2))
+        else:
+            lt, v = TS, int(r.integers(0, 1000))
+        if r.random() < 0.03:
+            v = None
+        terms.append((c, op, lt, v))
+    return cols, terms
+
+
+@pytest.mark.parametrize("case", range(12))
+def test_filter_vs_numpy_model(case):
+    r = np.random.default_rng(1200 + case)
+    n = int(r.integers(0, 5000))
+    cols, terms = _rand_filter_case(r, n)
+    assert (O.filter_select(cols, terms) == _numpy_filter(cols, terms)).all()
+
+
+@needs_ref
+@pytest.mark.parametrize("case", range(12))
+def test_filter_vs_compiled_reference(case):
+    r = np.random.default_rng(1300 + case)
+    n = int(r.integers(1, 5000))
+    cols, terms = _rand_filter_case(r, n)
+    if any(cols[c][2] == F64 and op == NE for c, op, _, _ in terms):
+        # The reference's vector compare evaluates NaN != x as FALSE in full SIMD
+        # lanes (Highway's ordered not-equal) but TRUE in the scalar tail and the
+        # scalar twin (expr/compare_kernels.cpp:55 vs :34, compare_scalar.cpp:5);
+        # the documented rule (expr/expr.h:36-40, IEEE) is pinned by
+        # test_filter_literals, so NaN is kept out of != terms here.
+        v, m, t = cols[2]
+        cols[2] = (np.where(np.isnan(v), 0.5, v), m, t)
+        terms = [(c, op, lt, 0.25 if (lt == F64 and v is not None and v != v) else v) for c, op, lt, v in terms]
+    got = O.filter_select(cols, terms)
+    want = O.ref_filter_select(cols, terms, batch_size=[64, 256, 2048][case % 3])
+    assert got.tolist() == want.tolist()
+
+
+def test_filter_mutants_are_caught():
+    t = seq_table(300)
+    clean = O.filter_select(t, [(0, GE, I32, 100)])
+    assert not np.array_equal(O.filter_select(t, [(0, GE, I32, 100)], mutant=1), clean)   # NULL passes
+    assert not np.array_equal(O.filter_select(t, [(0, GE, I32, 100)], mutant=2), clean)   # >= made strict
+    assert not np.array_equal(O.filter_select(t, [(0, GE, I32, 100)], mutant=4), clean)   # inverted
+    x = [col([1, 2, 3], I32)]
+    assert O.filter_select(x, [(0, LT, F64, 2.5)], mutant=3).tolist() != [0, 1]          # no promotion
