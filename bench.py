@@ -332,6 +332,71 @@ def groupby_1m(qe, ctx, args):
     return {"rows": n, "ms_per_query": s * 1e3, "rows_per_s": n / s}
 
 
+def filter_workload(qe, ctx, args):
+    """Trade-table filter: 2^30 rows (sym I32 in [0,5000), ts I64, px F64
+    uniform [0,200) with 2% NULLs), WHERE px > 100, all three columns
+    compacted. One step = qe_gpu_filter (predicate bitmap, tile scan, fused
+    compaction, result table)."""
+    n = 1 << args.filter_log2
+    log(f"generating filter workload: {n} rows")
+    sym = ctx.gen_uniform(20260617, 0, qe.I32, n, 0, 5000)
+    ts = ctx.gen_uniform(20260617, 1, qe.I64, n, 0, 1 << 40)
+    px = ctx.gen_uniform(20260617, 2, qe.F64, n, 0.0, 200.0, null_frac=args.filter_null_frac)
+    cols = [sym, ts, px]
+    terms = [(2, qe.CMP_GT, qe.F64, 100.0)]
+    for _ in range(args.warmup):
+        ctx.filter(cols, terms).free()
+    ctx.sync()
+    t0 = time.perf_counter()
+    m = 0
+    for _ in range(args.steps):
+        t = ctx.filter(cols, terms)
+        m = t.rows
+        t.free()
+    ctx.sync()
+    wall = (time.perf_counter() - t0) / args.steps
+    # Per-kernel times in a separate pass (the event brackets synchronize).
+    ctx.set_timing(True)
+    ctx.timing("filter_pred_mask")
+    ctx.timing("filter_compact")
+    for _ in range(args.steps):
+        ctx.filter(cols, terms).free()
+    mask_ms, k1 = ctx.timing("filter_pred_mask")
+    comp_ms, k2 = ctx.timing("filter_compact")
+    ctx.set_timing(False)
+    mask_ms /= max(k1, 1)
+    comp_ms /= max(k2, 1)
+    row_bytes = 4 + 8 + 8
+    # Algorithmic bytes: the bitmap pass reads px (8 B/row) and its validity
+    # (1/8 B/row) and writes 1/8 B/row; the compaction reads every column and the
+    # bitmap once and writes the m surviving rows.
+    mask_bytes = n * 8 + n / 8 + n / 8
+    comp_bytes = n * row_bytes + n / 8 + n / 8 + m * row_bytes + m / 8
+    res = {"rows": n, "passing": m, "selectivity": m / n, "px_null_frac": args.filter_null_frac, "ms_per_step": wall * 1e3,
+           "rows_per_s": n / wall, "pred_mask_ms": mask_ms, "compact_ms": comp_ms,
+           "pred_mask_gbs": mask_bytes / (mask_ms / 1e3) / 1e9,
+           "compact_gbs": comp_bytes / (comp_ms / 1e3) / 1e9,
+           "step_gbs": (n * row_bytes + n / 8 + m * row_bytes) / wall / 1e9}
+    if not args.skip_cpu:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import _oracle as O
+        sample = min(1 << 24, n)
+        head = ctx_slice(qe, ctx, px, sample)
+        pv, pvalid = head.numpy()
+        c = [(pv, pvalid, O.F64)]
+        t0 = time.perf_counter()
+        sel = O.filter_select(c, [(0, O.GT, O.F64, 100.0)])
+        t1 = time.perf_counter()
+        gsel = ctx.filter_select([head], [(0, qe.CMP_GT, qe.F64, 100.0)])
+        head.free()
+        res["cpu_baseline"] = {"value": sample / (t1 - t0), "unit": "rows/s", "cores": 1, "kind": "port",
+                               "sample": f"first {sample} rows of px; oracle filter_select (selection only)",
+                               "gpu_parity_on_sample": bool(np.array_equal(sel, gsel))}
+    for c in cols:
+        c.free()
+    return res
+
+
 def ctx_slice(qe, ctx, col, n):
     import ctypes as C
     out = C.c_void_p()
@@ -366,7 +431,9 @@ def main():
     ap.add_argument("--batch-morsels", type=int, default=4096)
     ap.add_argument("--batch-max-log2", type=int, default=21)
     ap.add_argument("--no-per-batch", action="store_true", help="skip the per-batch baseline (profiling runs)")
-    ap.add_argument("--only", choices=["join", "asof", "batched", "all"], default="all",
+    ap.add_argument("--filter-log2", type=int, default=30)
+    ap.add_argument("--filter-null-frac", type=float, default=0.02)
+    ap.add_argument("--only", choices=["join", "asof", "batched", "filter", "all"], default="all",
                     help="join: headline workload only; asof / batched: that secondary workload alone")
     ap.add_argument("--quick", action="store_true", help="small sizes (smoke)")
     ap.add_argument("--calibrate-copy", action="store_true", help="only run one 1 GiB stream copy")
@@ -379,15 +446,16 @@ def main():
         args.build_log2, args.probe_log2, args.cpu_probe_log2 = 16, 20, 18
         args.asof_right, args.asof_left, args.cpu_asof_log2 = 1 << 22, 1 << 20, 18
         args.batch_queries, args.batch_morsels, args.batch_max_log2 = 8, 64, 16
+        args.filter_log2 = 20
     qe = load_pkg()
     ctx = qe.Context(0)
     if args.calibrate_copy:
         # PMC calibration run: one copy of known size (1 GiB read + 1 GiB write).
         print(json.dumps({"copy": copy_ceiling(qe, ctx, reps=0)}), flush=True)
         return
-    if args.only in ("asof", "batched"):
+    if args.only in ("asof", "batched", "filter"):
         # One secondary workload on its own (diagnostics; not the headline line).
-        fn = asof_workload if args.only == "asof" else batched_workload
+        fn = {"asof": asof_workload, "batched": batched_workload, "filter": filter_workload}[args.only]
         print(json.dumps({args.only: fn(qe, ctx, args)}), flush=True)
         return
     ceiling = copy_ceiling(qe, ctx)
@@ -408,6 +476,10 @@ def main():
         a = secondary["asof"]
         log(f"as-of: binary {a['binary_ms']:.3f} ms, learned {a['learned_ms']:.3f} ms "
             f"(x{a['speedup_learned_vs_binary']:.2f}), identical={a['learned_equals_binary']}")
+        secondary["filter"] = filter_workload(qe, ctx, args)
+        f = secondary["filter"]
+        log(f"filter: {f['ms_per_step']:.3f} ms/step ({f['rows_per_s']:.3e} rows/s), mask {f['pred_mask_gbs']:.0f} GB/s, "
+            f"compact {f['compact_gbs']:.0f} GB/s")
         secondary["batched"] = batched_workload(qe, ctx, args)
         b = secondary["batched"]
         log(f"batched: graph {b['graph_s'] * 1e3:.2f} ms, per-batch {b['per_batch_s'] * 1e3:.1f} ms "

@@ -35,7 +35,7 @@ HIP_FLAGS = [
 ]
 HIP_SOURCES = ["context.hip", "groupby.hip", "join.hip", "hashtable.hip", "sort.hip", "batched.hip",
                "asof.hip", "gen.hip", "filter.hip"]
-HOST_SOURCES = ["operators.cpp", "plan.cpp"]
+HOST_SOURCES = ["operators.cpp", "plan.cpp", "expr.cpp"]
 
 
 def _run(cmd: list[str]) -> None:

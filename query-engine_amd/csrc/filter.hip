@@ -4,19 +4,29 @@
 // by logic_and):
 //  * a row passes iff every comparison is TRUE (SQL WHERE): a NULL operand or a
 //    NULL literal makes the comparison NULL, which does not pass; NaN compares
-//    FALSE except under != (IEEE, as the reference's compare kernels);
+//    FALSE except under != (IEEE, expr/expr.h:36-40);
 //  * numeric operands are promoted as expr::cmp promotes them: any F64 operand
 //    gives a double comparison, otherwise a 64-bit integer comparison (I32
 //    sign-extends); TS compares TS and BOOL compares BOOL;
 //  * passing rows keep their order, and every column is compacted with its
-//    validity (compact_column, core/owned_batch.cpp:101-166).
+//    validity; an output column is all-valid when no NULL survives
+//    (compact_column, core/owned_batch.cpp:101-166).
 //
-// Kernels: pred_count_kernel evaluates a 4096-row tile (16 rows per thread,
-// rows k*256 + t so each wave's rows are consecutive) and counts the passing
-// rows with wave64 ballots; one scan over the tiles; pred_emit_kernel repeats
-// the ballots and writes each passing row's index at tile offset + rows of
-// earlier rounds + rows of earlier waves + its rank among the wave's passing
-// lanes; the columns are then gathered through those indices.
+// Two passes over 4096-row tiles (256 threads x 16 rounds; round k of wave w
+// covers the 64-aligned rows base + 256k + 64w + lane, i.e. one bitmap word):
+//  1. pred_mask_kernel evaluates the predicate term by term over all 16 rounds
+//     (independent loads in flight), writes the passing-row bitmap (one ballot
+//     per word) and the tile's count;
+//  2. after an exclusive scan of the tile counts, compact_kernel reads the
+//     bitmap and ranks each passing lane within the tile; per output column it
+//     loads all 16 rounds (full-wave loads, no branches), stages the passing
+//     values in LDS at their tile-local positions and writes the tile's
+//     contiguous output range with coalesced stores (no selection-vector round
+//     trip). Output validity is staged the same way, one byte per output row,
+//     and each output word is then one wave ballot; only the tile's two edge
+//     words, shared with the neighbouring tiles, use global atomics.
+// qe_gpu_filter_select runs the same passes with the row indices as the only
+// output.
 #include "internal.hpp"
 
 using namespace qe_gpu;
@@ -28,6 +38,7 @@ constexpr uint32_t kPredBlock = 256;
 constexpr uint32_t kPredRounds = 16;
 constexpr uint32_t kPredTile = kPredBlock * kPredRounds;
 constexpr uint32_t kPredWaves = kPredBlock / kWave;
+constexpr uint32_t kMaxCompactCols = 8;
 
 struct PredTerm {
     ColView c;
@@ -42,90 +53,285 @@ struct Pred {
     PredTerm t[kMaxPredTerms];
 };
 
+struct CompactCol {
+    const void* src;
+    const uint64_t* sv;  // source validity, nullptr when all-valid
+    void* dst;
+    uint64_t* dv;        // output validity (zeroed), nullptr when all-valid
+    int32_t width;
+    int32_t pad;
+};
+struct CompactCols {
+    int32_t n;
+    CompactCol c[kMaxCompactCols];
+};
+
+// Lane k of a wave holds word k of the wave's 16 rounds; round k reads it back
+// as a wave-uniform value.
+__device__ inline uint64_t round_word(uint64_t lane_words, uint32_t k) {
+    const uint32_t lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(lane_words), k);
+    const uint32_t hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(lane_words >> 32), k);
+    return (uint64_t(hi) << 32) | lo;
+}
+
+// pass[k] &= (v[k] op lit) for the 16 rounds; op is uniform, so each case is a
+// straight unrolled loop.
 template <typename T>
-__device__ inline bool cmp_vals(int op, T a, T b) {
+__device__ inline void cmp_rounds(int op, const T (&v)[kPredRounds], T lit, bool (&pass)[kPredRounds]) {
+#define QE_CMP_LOOP(expr)                                              \
+    _Pragma("unroll") for (uint32_t k = 0; k < kPredRounds; ++k) { \
+        const T a = v[k];                                              \
+        pass[k] = pass[k] && (expr);                                   \
+    }                                                                  \
+    break;
     switch (op) {
         case QE_CMP_LT:
-            return a < b;
+            QE_CMP_LOOP(a < lit)
         case QE_CMP_LE:
-            return a <= b;
+            QE_CMP_LOOP(a <= lit)
         case QE_CMP_GT:
-            return a > b;
+            QE_CMP_LOOP(a > lit)
         case QE_CMP_GE:
-            return a >= b;
+            QE_CMP_LOOP(a >= lit)
         case QE_CMP_EQ:
-            return a == b;
+            QE_CMP_LOOP(a == lit)
         default:
-            return a != b;
+            QE_CMP_LOOP(a != lit)
+    }
+#undef QE_CMP_LOOP
+}
+
+// One term over the 16 rounds of this thread: loads for every still-passing
+// row first, then the comparison in the promoted type.
+template <typename S>
+__device__ inline void term_rounds(const PredTerm& t, uint64_t base, uint64_t n, bool (&pass)[kPredRounds]) {
+    const S* src = static_cast<const S*>(t.c.data);
+    const uint64_t last = n - 1;  // rows past the end load the last row; pass[k] is false there
+    S raw[kPredRounds];
+#pragma unroll
+    for (uint32_t k = 0; k < kPredRounds; ++k) {
+        const uint64_t r = base + k * kPredBlock;
+        raw[k] = src[r < last ? r : last];
+    }
+    if (t.as_float) {
+        double v[kPredRounds];
+#pragma unroll
+        for (uint32_t k = 0; k < kPredRounds; ++k) v[k] = static_cast<double>(raw[k]);
+        cmp_rounds<double>(t.op, v, t.lf, pass);
+    } else {
+        int64_t v[kPredRounds];
+#pragma unroll
+        for (uint32_t k = 0; k < kPredRounds; ++k) v[k] = static_cast<int64_t>(raw[k]);
+        cmp_rounds<int64_t>(t.op, v, t.li, pass);
     }
 }
 
-__device__ inline bool pred_row(const Pred& P, uint64_t r) {
-    if (P.never) return false;
+// pass[k] for rows base + 256k (base = tile start + threadIdx.x); word0 is the
+// bitmap word of round 0 of this wave (wave-uniform).
+__device__ inline void eval_pred(const Pred& P, uint64_t base, uint64_t word0, uint64_t n, bool (&pass)[kPredRounds]) {
+#pragma unroll
+    for (uint32_t k = 0; k < kPredRounds; ++k) pass[k] = !P.never && base + k * kPredBlock < n;
     for (int j = 0; j < P.n; ++j) {
         const PredTerm& t = P.t[j];
-        if (!is_valid(t.c, r)) return false;  // NULL comparison: not TRUE
-        bool res;
-        if (t.as_float) {
-            const double a = t.c.type == QE_F64 ? f64_value(t.c, r) : static_cast<double>(int_value(t.c, r));
-            res = cmp_vals<double>(t.op, a, t.lf);
-        } else {
-            res = cmp_vals<int64_t>(t.op, int_value(t.c, r), t.li);
+        if (t.c.validity) {  // NULL operand: the comparison is NULL, not TRUE
+            // Round k of this wave needs one validity word: lane k fetches it.
+            const uint32_t lane = lane_id();
+            const uint64_t my_word = word0 + lane * kPredWaves;
+            const uint64_t my_valid = (lane < kPredRounds && my_word < words_for(n)) ? t.c.validity[my_word] : 0;
+#pragma unroll
+            for (uint32_t k = 0; k < kPredRounds; ++k)
+                pass[k] = pass[k] && ((round_word(my_valid, k) >> lane) & 1ull);
         }
-        if (!res) return false;
+        switch (t.c.type) {
+            case QE_F64:
+                term_rounds<double>(t, base, n, pass);
+                break;
+            case QE_I32:
+                term_rounds<int32_t>(t, base, n, pass);
+                break;
+            case QE_BOOL:
+                term_rounds<uint8_t>(t, base, n, pass);
+                break;
+            default:  // I64, TS
+                term_rounds<int64_t>(t, base, n, pass);
+        }
     }
-    return true;
 }
 
-__global__ void __launch_bounds__(kPredBlock) pred_count_kernel(Pred P, uint64_t n, uint32_t* tile_cnt) {
+__global__ void __launch_bounds__(kPredBlock) pred_mask_kernel(Pred P, uint64_t n, uint64_t* mask, uint32_t* tile_cnt) {
     __shared__ uint32_t wsum[kPredWaves];
-    const uint64_t base = uint64_t(blockIdx.x) * kPredTile;
+    const uint64_t tile = blockIdx.x;
+    const uint64_t base = tile * kPredTile + threadIdx.x;
+    const unsigned w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const uint64_t nwords = words_for(n);
+    bool pass[kPredRounds];
+    eval_pred(P, base, tile * (kPredTile / 64) + w, n, pass);
     uint32_t cnt = 0;
+#pragma unroll
     for (uint32_t k = 0; k < kPredRounds; ++k) {
-        const uint64_t r = base + k * kPredBlock + threadIdx.x;
-        cnt += (r < n && pred_row(P, r)) ? 1u : 0u;
+        const uint64_t m = __ballot(pass[k]);
+        const uint64_t word = tile * (kPredTile / 64) + k * kPredWaves + w;
+        if (lane_id() == 0 && word < nwords) mask[word] = m;
+        cnt += static_cast<uint32_t>(__popcll(m));
     }
-    for (int off = kWave / 2; off > 0; off >>= 1) cnt += __shfl_down(cnt, off);
-    if (lane_id() == 0) wsum[threadIdx.x / kWave] = cnt;
+    if (lane_id() == 0) wsum[w] = cnt;
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t s = 0;
-        for (uint32_t w = 0; w < kPredWaves; ++w) s += wsum[w];
-        tile_cnt[blockIdx.x] = s;
+        for (uint32_t i = 0; i < kPredWaves; ++i) s += wsum[i];
+        tile_cnt[tile] = s;
     }
 }
 
-__global__ void __launch_bounds__(kPredBlock) pred_emit_kernel(Pred P, uint64_t n, const uint64_t* tile_off,
-                                                               uint32_t* sel) {
-    __shared__ uint32_t cnt[kPredRounds * kPredWaves];  // passing rows per (round, wave), then exclusive offsets
-    const uint64_t base = uint64_t(blockIdx.x) * kPredTile;
-    const unsigned lane = lane_id(), w = threadIdx.x / kWave;
-    uint64_t mask[kPredRounds];
-    bool pass[kPredRounds];
+// Loads round k's row (clamped into range, so every load is a full-wave load
+// with no branch around it) and stages the passing values at their tile-local
+// output positions in LDS.
+template <typename T>
+__device__ inline void stage_rounds(const CompactCol& c, uint64_t base, uint64_t last, const uint32_t (&lp)[kPredRounds],
+                                    uint32_t bits, void* stage) {
+    const T* src = static_cast<const T*>(c.src);
+    T v[kPredRounds];
 #pragma unroll
     for (uint32_t k = 0; k < kPredRounds; ++k) {
-        const uint64_t r = base + k * kPredBlock + threadIdx.x;
-        pass[k] = r < n && pred_row(P, r);
-        mask[k] = __ballot(pass[k]);
-        if (lane == 0) cnt[k * kPredWaves + w] = static_cast<uint32_t>(__popcll(mask[k]));
+        const uint64_t r = base + k * kPredBlock;
+        v[k] = __builtin_nontemporal_load(src + (r < last ? r : last));
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {  // rows are ordered by (round, wave, lane)
-        uint32_t run = 0;
-        for (uint32_t i = 0; i < kPredRounds * kPredWaves; ++i) {
-            const uint32_t c = cnt[i];
-            cnt[i] = run;
-            run += c;
-        }
-    }
-    __syncthreads();
-    const uint64_t toff = tile_off[blockIdx.x];
+    T* st = static_cast<T*>(stage);
+#pragma unroll
+    for (uint32_t k = 0; k < kPredRounds; ++k)
+        if ((bits >> k) & 1u) st[lp[k]] = v[k];
+}
+
+template <typename T>
+__device__ inline void copy_out(const void* stage, void* dst, uint64_t toff, uint32_t cnt) {
+    const T* st = static_cast<const T*>(stage);
+    T* d = static_cast<T*>(dst) + toff;
+    for (uint32_t i = threadIdx.x; i < cnt; i += kPredBlock) d[i] = st[i];
+}
+
+// sel (optional) receives the passing row indices; cols are written at the
+// same positions. nulls[c] counts the NULLs that survive in column c.
+__global__ void __launch_bounds__(kPredBlock) compact_kernel(const uint64_t* __restrict__ mask, uint64_t n,
+                                                             const uint64_t* __restrict__ tile_off, uint64_t total,
+                                                             CompactCols C, uint32_t* __restrict__ sel,
+                                                             unsigned long long* nulls) {
+    // Output staging for one column of one tile; its validity as one byte per
+    // output row, placed at (toff mod 64) + position so that byte i*64 + lane
+    // is bit `lane` of output word (toff / 64) + i; the (round, wave) offsets.
+    __shared__ __align__(16) uint64_t stage[kPredTile];
+    __shared__ uint8_t vstage[kPredTile + 128];
+    __shared__ uint32_t off[kPredRounds * kPredWaves];
+    __shared__ uint32_t valid_sum;
+    const uint64_t tile = blockIdx.x;
+    const unsigned lane = lane_id();
+    const unsigned w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const uint64_t nwords = words_for(n);
     const uint64_t below = (uint64_t(1) << lane) - 1;
+    // Lane k < 16 fetches the bitmap word of round k (one load per wave).
+    const uint64_t my_word = tile * (kPredTile / 64) + lane * kPredWaves + w;
+    const bool word_lane = lane < kPredRounds && my_word < nwords;
+    const uint64_t my_mask = word_lane ? mask[my_word] : 0;
+    if (lane < kPredRounds) off[lane * kPredWaves + w] = static_cast<uint32_t>(__popcll(my_mask));
+    __syncthreads();
+    if (threadIdx.x < kWave) {  // wave 0: exclusive scan of the 64 (round, wave) counts
+        static_assert(kPredRounds * kPredWaves == kWave, "one count per lane");
+        const uint32_t c = off[threadIdx.x];
+        uint32_t s = c;
+        for (int d = 1; d < kWave; d <<= 1) {
+            const uint32_t u = __shfl_up(s, d);
+            if (static_cast<int>(threadIdx.x) >= d) s += u;
+        }
+        off[threadIdx.x] = s - c;
+    }
+    __syncthreads();
+    const uint64_t toff = tile_off[tile];
+    const uint32_t tcnt = static_cast<uint32_t>((tile + 1 < gridDim.x ? tile_off[tile + 1] : total) - toff);
+    if (tcnt == 0) return;  // uniform across the block
+    const uint64_t base = tile * kPredTile + threadIdx.x;
+    const uint64_t last = n - 1;
+    uint32_t lp[kPredRounds];  // tile-local output position of round k's row
+    uint32_t bits = 0;
 #pragma unroll
     for (uint32_t k = 0; k < kPredRounds; ++k) {
-        if (!pass[k]) continue;
-        const uint64_t pos = toff + cnt[k * kPredWaves + w] + __popcll(mask[k] & below);
-        sel[pos] = static_cast<uint32_t>(base + k * kPredBlock + threadIdx.x);
+        const uint64_t m = round_word(my_mask, k);
+        bits |= static_cast<uint32_t>((m >> lane) & 1ull) << k;
+        lp[k] = off[k * kPredWaves + w] + static_cast<uint32_t>(__popcll(m & below));
+    }
+    if (sel) {
+        uint32_t* st = reinterpret_cast<uint32_t*>(stage);
+#pragma unroll
+        for (uint32_t k = 0; k < kPredRounds; ++k)
+            if ((bits >> k) & 1u) st[lp[k]] = static_cast<uint32_t>(base + k * kPredBlock);
+        __syncthreads();
+        copy_out<uint32_t>(stage, sel, toff, tcnt);
+        __syncthreads();
+    }
+    const unsigned sh0 = static_cast<unsigned>(toff & 63);
+    const uint32_t vwords = (sh0 + tcnt + 63) / 64;
+    for (int c = 0; c < C.n; ++c) {
+        const CompactCol& col = C.c[c];
+        const bool nullable = col.dv != nullptr;
+        switch (col.width) {
+            case 1:
+                stage_rounds<uint8_t>(col, base, last, lp, bits, stage);
+                break;
+            case 4:
+                stage_rounds<uint32_t>(col, base, last, lp, bits, stage);
+                break;
+            default:
+                stage_rounds<uint64_t>(col, base, last, lp, bits, stage);
+        }
+        if (nullable) {
+            // Round k of this wave reads source validity word my_word of lane k.
+            const uint64_t my_valid = word_lane ? col.sv[my_word] : 0;
+#pragma unroll
+            for (uint32_t k = 0; k < kPredRounds; ++k) {
+                const uint64_t vw = round_word(my_valid, k);
+                if ((bits >> k) & 1u) vstage[sh0 + lp[k]] = static_cast<uint8_t>((vw >> lane) & 1ull);
+            }
+            if (threadIdx.x == 0) valid_sum = 0;
+        }
+        __syncthreads();
+        switch (col.width) {
+            case 1:
+                copy_out<uint8_t>(stage, col.dst, toff, tcnt);
+                break;
+            case 4:
+                copy_out<uint32_t>(stage, col.dst, toff, tcnt);
+                break;
+            default:
+                copy_out<uint64_t>(stage, col.dst, toff, tcnt);
+        }
+        if (nullable) {
+            // Output word (toff / 64) + i from one ballot; interior words belong
+            // to this tile alone, the first and last are shared with the
+            // neighbouring tiles.
+            unsigned long long* dv = reinterpret_cast<unsigned long long*>(col.dv) + (toff >> 6);
+            uint32_t valid = 0;
+            for (uint32_t i = w; i < vwords; i += kPredWaves) {
+                const uint32_t idx = i * 64 + lane;
+                const bool b = idx >= sh0 && idx < sh0 + tcnt && vstage[idx] != 0;
+                const uint64_t word = __ballot(b);
+                valid += static_cast<uint32_t>(__popcll(word));
+                if (lane == 0) {
+                    if (i == 0 || i + 1 == vwords) {
+                        if (word) atomicOr(dv + i, static_cast<unsigned long long>(word));
+                    } else {
+                        dv[i] = word;
+                    }
+                }
+            }
+            if (lane == 0 && valid) atomicAdd(&valid_sum, valid);
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                if (valid_sum != tcnt) atomicAdd(nulls + c, static_cast<unsigned long long>(tcnt - valid_sum));
+                // Padding bits past the last row are set (the all-valid padding convention).
+                if (tile + 1 == gridDim.x && (total & 63))
+                    atomicOr(reinterpret_cast<unsigned long long*>(col.dv) + (total >> 6),
+                             static_cast<unsigned long long>(~uint64_t(0) << (total & 63)));
+            }
+        }
+        __syncthreads();  // stage / vstage are reused by the next column
     }
 }
 
@@ -162,17 +368,33 @@ Pred make_pred(const qe_dev_col* const* cols, uint32_t ncols, const qe_pred_term
     return P;
 }
 
-// Passing row indices (in order) into `sel`; returns their count.
-uint64_t filter_select(qe_gpu_ctx* ctx, const Pred& P, uint64_t n, uint32_t* sel) {
-    if (n == 0) return 0;
-    const uint64_t tiles = (n + kPredTile - 1) / kPredTile;
-    Scratch cnt(ctx, tiles * 4), off(ctx, tiles * 8);
-    pred_count_kernel<<<tiles, kPredBlock, 0, ctx->stream>>>(P, n, cnt.as<uint32_t>());
+// Bitmap + tile offsets of the passing rows; returns their count.
+struct FilterMask {
+    Scratch mask, off;
+    uint64_t tiles = 0, total = 0;
+};
+FilterMask filter_mask(qe_gpu_ctx* ctx, const Pred& P, uint64_t n) {
+    FilterMask f;
+    if (n == 0) return f;
+    f.tiles = (n + kPredTile - 1) / kPredTile;
+    f.mask = Scratch(ctx, words_for(n) * 8);
+    f.off = Scratch(ctx, f.tiles * 8);
+    Scratch cnt(ctx, f.tiles * 4);
+    {
+        TimedRegion tr(ctx, "filter_pred_mask");
+        pred_mask_kernel<<<f.tiles, kPredBlock, 0, ctx->stream>>>(P, n, f.mask.as<uint64_t>(), cnt.as<uint32_t>());
+        QE_HIP(hipGetLastError());
+    }
+    f.total = exclusive_scan_u32(ctx, cnt.as<uint32_t>(), f.off.as<uint64_t>(), f.tiles);
+    return f;
+}
+
+void launch_compact(qe_gpu_ctx* ctx, const FilterMask& f, uint64_t n, const CompactCols& C, uint32_t* sel,
+                    unsigned long long* nulls) {
+    TimedRegion tr(ctx, "filter_compact");
+    compact_kernel<<<f.tiles, kPredBlock, 0, ctx->stream>>>(f.mask.as<uint64_t>(), n, f.off.as<uint64_t>(), f.total,
+                                                            C, sel, nulls);
     QE_HIP(hipGetLastError());
-    const uint64_t total = exclusive_scan_u32(ctx, cnt.as<uint32_t>(), off.as<uint64_t>(), tiles);
-    pred_emit_kernel<<<tiles, kPredBlock, 0, ctx->stream>>>(P, n, off.as<uint64_t>(), sel);
-    QE_HIP(hipGetLastError());
-    return total;
 }
 
 }  // namespace qe_gpu
@@ -186,11 +408,41 @@ qe_gpu_status qe_gpu_filter(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, uint
         const uint64_t n = common_len(cols, ncols);
         check(n < kNoRow, QE_GPU_UNSUPPORTED, "row indices are 32-bit");
         const Pred P = make_pred(cols, ncols, terms, nterms);
-        Scratch sel(ctx, std::max<uint64_t>(n, 1) * 4);
-        const uint64_t m = filter_select(ctx, P, n, sel.as<uint32_t>());
+        FilterMask f = filter_mask(ctx, P, n);
+        const uint64_t m = f.total;
         qe_dev_table* t = new_table(m);
         try {
-            for (uint32_t c = 0; c < ncols; ++c) t->cols.push_back(gather_col(ctx, cols[c], sel.as<uint32_t>(), m));
+            for (uint32_t c = 0; c < ncols; ++c) {
+                const bool nullable = !cols[c]->all_valid && cols[c]->validity != nullptr && m > 0;
+                t->cols.push_back(new_col(ctx, cols[c]->type, m, nullable));
+                if (nullable) memset_async(ctx, t->cols.back()->validity, 0, words_for(m) * 8);
+            }
+            if (m > 0) {
+                Scratch nulls(ctx, kMaxCompactCols * 8);
+                for (uint32_t c0 = 0; c0 < ncols; c0 += kMaxCompactCols) {
+                    CompactCols C{};
+                    C.n = static_cast<int32_t>(std::min<uint32_t>(kMaxCompactCols, ncols - c0));
+                    for (int i = 0; i < C.n; ++i) {
+                        const qe_dev_col* s = cols[c0 + i];
+                        qe_dev_col* d = t->cols[c0 + i];
+                        C.c[i] = CompactCol{s->data, d->validity ? s->validity : nullptr, d->data, d->validity,
+                                            static_cast<int32_t>(type_width(s->type)), 0};
+                    }
+                    memset_async(ctx, nulls.p, 0, kMaxCompactCols * 8);
+                    launch_compact(ctx, f, n, C, nullptr, nulls.as<unsigned long long>());
+                    uint64_t h[kMaxCompactCols];
+                    QE_HIP(hipMemcpyAsync(h, nulls.p, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+                    QE_HIP(hipStreamSynchronize(ctx->stream));
+                    for (int i = 0; i < C.n; ++i) {  // refresh_all_valid: no NULL survived
+                        qe_dev_col* d = t->cols[c0 + i];
+                        if (d->validity && h[i] == 0) {
+                            dev_free(ctx, d->validity);
+                            d->validity = nullptr;
+                            d->all_valid = true;
+                        }
+                    }
+                }
+            }
         } catch (...) {
             qe_gpu_table_free(ctx, t);
             throw;
@@ -209,7 +461,9 @@ qe_gpu_status qe_gpu_filter_select(qe_gpu_ctx* ctx, const qe_dev_col* const* col
         check(n < kNoRow, QE_GPU_UNSUPPORTED, "row indices are 32-bit");
         invalid(n == 0 || sel_out != nullptr, "null selection output");
         const Pred P = make_pred(cols, ncols, terms, nterms);
-        *count_out = filter_select(ctx, P, n, sel_out);
+        FilterMask f = filter_mask(ctx, P, n);
+        if (f.total > 0) launch_compact(ctx, f, n, CompactCols{}, sel_out, nullptr);
+        *count_out = f.total;
         QE_HIP(hipStreamSynchronize(ctx->stream));
     });
 }

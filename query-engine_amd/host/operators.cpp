@@ -1,5 +1,6 @@
 // Host drop-in operators over the C ABI. See qe_host.h.
 #include "qe_host.h"
+#include "qe_expr.h"
 
 #include <algorithm>
 #include <cstring>
@@ -514,6 +515,144 @@ uint64_t GpuHashTable::group_key_word(uint32_t g, size_t col) const {
     std::vector<uint8_t> nl(std::max<size_t>(n, 1));
     QE_CALL(qe_gpu_ht_key_words(device_context(), ht_, static_cast<uint32_t>(col), w.data(), nl.data()));
     return w.at(g);
+}
+
+// ---- GpuFilter ----------------------------------------------------------------------------------
+namespace {
+
+struct FlatPred {
+    std::vector<qe_pred_term> terms;
+    bool never = false;  // a FALSE or NULL conjunct: no row passes
+};
+
+[[noreturn]] void filter_shape(const char* what) {
+    throw std::runtime_error(std::string("GpuFilter: ") + what +
+                             " is not on the device path (conjunctions of column-versus-literal comparisons)");
+}
+
+expr::CmpOp mirrored(expr::CmpOp op) {  // a < c  <=>  c > a
+    switch (op) {
+        case expr::CmpOp::Lt:
+            return expr::CmpOp::Gt;
+        case expr::CmpOp::Le:
+            return expr::CmpOp::Ge;
+        case expr::CmpOp::Gt:
+            return expr::CmpOp::Lt;
+        case expr::CmpOp::Ge:
+            return expr::CmpOp::Le;
+        default:
+            return op;
+    }
+}
+
+void check_column(const expr::Node& n, const Schema& s) {  // plan/plan.cpp:38-54
+    if (n.col_index >= s.fields.size())
+        throw std::invalid_argument("Filter: predicate column " + std::to_string(n.col_index) + " out of range");
+    if (s.fields[n.col_index].second != n.type)
+        throw std::invalid_argument("Filter: predicate column " + std::to_string(n.col_index) + " type mismatch");
+}
+
+void flatten(const expr::Expr& e, const Schema& s, FlatPred& fp) {
+    const expr::Node& n = e.node();
+    switch (n.kind) {
+        case expr::NodeKind::Logic:
+            if (n.logic_op != expr::LogicOp::And) filter_shape("OR / NOT");
+            for (const expr::Expr& c : n.children) flatten(c, s, fp);
+            return;
+        case expr::NodeKind::Literal:  // a BOOL literal conjunct
+            if (n.literal.is_null || n.literal.i == 0) fp.never = true;
+            return;
+        case expr::NodeKind::Column: {  // a BOOL column conjunct: TRUE iff valid and != 0
+            check_column(n, s);
+            qe_pred_term t{};
+            t.col = n.col_index;
+            t.op = QE_CMP_NE;
+            t.lit_type = QE_BOOL;
+            fp.terms.push_back(t);
+            return;
+        }
+        case expr::NodeKind::Cmp: {
+            const expr::Node* c = &n.children[0].node();
+            const expr::Node* l = &n.children[1].node();
+            expr::CmpOp op = n.cmp_op;
+            if (c->kind == expr::NodeKind::Literal && l->kind == expr::NodeKind::Column) {
+                std::swap(c, l);
+                op = mirrored(op);
+            }
+            if (c->kind != expr::NodeKind::Column || l->kind != expr::NodeKind::Literal)
+                filter_shape("a comparison that is not column-versus-literal");
+            if (n.cmp_type == Type::STR) filter_shape("a STR comparison");
+            check_column(*c, s);
+            qe_pred_term t{};
+            t.col = c->col_index;
+            t.op = static_cast<int32_t>(op);
+            t.lit_type = static_cast<int32_t>(l->literal.type);
+            t.lit_is_null = l->literal.is_null ? 1 : 0;
+            t.lit_i = l->literal.i;
+            t.lit_f = l->literal.f;
+            fp.terms.push_back(t);
+            return;
+        }
+    }
+}
+
+}  // namespace
+
+GpuFilter::GpuFilter(std::unique_ptr<Operator> child, expr::Expr predicate)
+    : child_(std::move(child)), predicate_(std::move(predicate)) {}
+
+qe_dev_table* GpuFilter::run() {
+    if (!predicate_ || predicate_.type() != Type::BOOL)  // ops/filter.cpp:18-20
+        throw std::invalid_argument("Filter: predicate must be BOOL");
+    const Schema s = child_->output_schema();
+    FlatPred fp;
+    flatten(predicate_, s, fp);
+    if (fp.never) {  // no row passes: one NULL-literal term yields the empty result
+        fp.terms.assign(1, qe_pred_term{});
+        fp.terms[0].lit_type = static_cast<int32_t>(s.fields.at(0).second);
+        fp.terms[0].op = QE_CMP_EQ;
+        fp.terms[0].lit_is_null = 1;
+    }
+    if (fp.terms.empty()) return nullptr;
+    auto in = device_input(*child_);
+    qe_dev_table* t = nullptr;
+    QE_CALL(qe_gpu_filter(device_context(), in->cols.data(), static_cast<uint32_t>(in->cols.size()),
+                          fp.terms.data(), static_cast<uint32_t>(fp.terms.size()), &t));
+    return t;
+}
+
+void GpuFilter::open() {
+    qe_dev_table* t = run();
+    pass_through_ = t == nullptr;
+    if (pass_through_) child_->open();
+    emit_.reset(t, types_of(output_schema()));
+}
+
+std::optional<Batch> GpuFilter::next() {
+    if (!pass_through_) return emit_.next();
+    while (auto b = child_->next())  // TRUE for every row: the child's non-empty batches
+        if (b->row_count) return b;
+    return std::nullopt;
+}
+
+void GpuFilter::close() {
+    if (pass_through_) child_->close();
+    pass_through_ = false;
+    emit_.reset(nullptr, {});
+}
+
+std::shared_ptr<DeviceColumns> GpuFilter::device_columns() {
+    qe_dev_table* t = run();
+    if (!t) return device_input(*child_);
+    auto keep = std::make_shared<TableColumns>();
+    keep->table = t;
+    keep->rows = qe_gpu_table_rows(t);
+    for (uint32_t i = 0; i < qe_gpu_table_cols(t); ++i) {
+        qe_dev_col* v = nullptr;
+        QE_CALL(qe_gpu_column_slice(device_context(), qe_gpu_table_col(t, i), 0, keep->rows, &v));
+        keep->cols.push_back(v);
+    }
+    return keep;
 }
 
 }  // namespace qe

@@ -57,6 +57,37 @@ PlanBuilder scan(const Table& table) {
     return PlanBuilder(Plan(std::move(n)));
 }
 
+// plan/plan.cpp:38-54: every column reference in range and of the schema's type.
+static void validate_expr(const expr::Expr& e, const Schema& s) {
+    const expr::Node& n = e.node();
+    if (n.kind == expr::NodeKind::Column) {
+        if (n.col_index >= s.fields.size())
+            throw std::invalid_argument("plan: expression column index " + std::to_string(n.col_index) +
+                                        " out of range (schema has " + std::to_string(s.fields.size()) +
+                                        " columns)");
+        if (s.fields[n.col_index].second != n.type)
+            throw std::invalid_argument("plan: expression column " + std::to_string(n.col_index) +
+                                        " type mismatch (expr says " + type_name(n.type) + ", schema says " +
+                                        type_name(s.fields[n.col_index].second) + ")");
+    }
+    for (const expr::Expr& c : n.children) validate_expr(c, s);
+}
+
+// plan/plan.cpp:307-321: a non-empty BOOL predicate; the schema is unchanged.
+PlanBuilder PlanBuilder::filter(expr::Expr predicate) const {
+    const Schema& cs = plan_.output_schema();
+    if (!predicate) throw std::invalid_argument("plan: filter predicate is empty");
+    if (predicate.type() != Type::BOOL)
+        throw std::invalid_argument(std::string("plan: filter predicate must be BOOL, got ") +
+                                    type_name(predicate.type()));
+    validate_expr(predicate, cs);
+    auto n = make(PlanKind::Filter);
+    n->children.push_back(plan_);
+    n->predicate = std::move(predicate);
+    n->out_schema = cs;
+    return PlanBuilder(Plan(std::move(n)));
+}
+
 PlanBuilder PlanBuilder::aggregate(std::vector<ColRef> keys, std::vector<AggSpec> aggs) const {
     const Schema& cs = plan_.output_schema();
     if (aggs.empty()) throw std::invalid_argument("plan: aggregate needs >=1 aggregate");
@@ -134,6 +165,8 @@ std::unique_ptr<Operator> Plan::lower(const LowerOptions& opt) const {
     switch (n.kind) {
         case PlanKind::Scan:
             return std::make_unique<Scan>(*n.table, opt.batch_size);
+        case PlanKind::Filter:
+            return std::make_unique<GpuFilter>(n.children[0].lower(opt), n.predicate);
         case PlanKind::Aggregate: {
             const Plan& c = n.children[0];
             if (opt.fuse_join_aggregate && c.kind() == PlanKind::Join) {
@@ -162,6 +195,9 @@ void print(const Plan& p, int depth, std::ostringstream& os) {
     switch (n.kind) {
         case PlanKind::Scan:
             os << "Scan(" << n.out_schema.fields.size() << " cols)";
+            break;
+        case PlanKind::Filter:
+            os << "Filter";
             break;
         case PlanKind::Aggregate:
             os << "Aggregate(keys:" << n.group_keys.size() << " aggs:" << n.aggs.size() << ")";

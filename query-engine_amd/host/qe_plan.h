@@ -1,5 +1,5 @@
-// Dataframe plan builder for the device path: the scan / join / aggregate /
-// asof_join subset of the reference PlanBuilder (plan/plan.h:205-227), with
+// Dataframe plan builder for the device path: the scan / filter / join /
+// aggregate / asof_join subset of the reference PlanBuilder (plan/plan.h:205-227), with
 // the same validation (key lists resolved by index or name, positional key
 // type equality, time columns I32/I64/TS) and the same output schemas, and a
 // lowering that produces the device operators (plan/plan.cpp:570-615 lowers
@@ -12,11 +12,12 @@
 #include <string>
 #include <vector>
 
+#include "qe_expr.h"
 #include "qe_host.h"
 
 namespace qe::plan {
 
-enum class PlanKind { Scan, Aggregate, Join, AsofJoin };
+enum class PlanKind { Scan, Filter, Aggregate, Join, AsofJoin };
 
 struct ColRef {
     bool by_name = false;
@@ -35,6 +36,7 @@ struct PlanNode {
     Schema out_schema;
     std::vector<Plan> children;
     const Table* table = nullptr;
+    expr::Expr predicate;  // Filter (BOOL)
     std::vector<uint32_t> group_keys;
     std::vector<AggSpec> aggs;
     std::vector<uint32_t> left_keys, right_keys;
@@ -67,6 +69,7 @@ class Plan {
 class PlanBuilder {
    public:
     explicit PlanBuilder(Plan p) : plan_(std::move(p)) {}
+    PlanBuilder filter(expr::Expr predicate) const;
     PlanBuilder aggregate(std::vector<ColRef> keys, std::vector<AggSpec> aggs) const;
     PlanBuilder join(const Plan& build, std::vector<ColRef> left_keys, std::vector<ColRef> right_keys,
                      JoinType type) const;

@@ -2,7 +2,8 @@
 // reference's Operator contract and plan builder, driven the way the
 // reference's own suites drive HashJoin / Aggregate / AsofJoin / HashTable
 // (tests/join_differential_test.cpp, tests/aggregate_differential_test.cpp,
-// tests/asof_differential_test.cpp, tests/hashtable_test.cpp), with
+// tests/asof_differential_test.cpp, tests/hashtable_test.cpp, the Filter cases
+// of tests/ops_test.cpp), with
 // hand-derived expected results. Needs a GPU; run from tests/test_gpu_host.py.
 #include <algorithm>
 #include <cmath>
@@ -14,6 +15,7 @@
 #include <tuple>
 #include <vector>
 
+#include "qe_expr.h"
 #include "qe_host.h"
 #include "qe_plan.h"
 
@@ -41,7 +43,7 @@ OwnedColumn mk(Type t, const std::vector<T>& v, std::vector<size_t> nulls = {}) 
 // Drain an operator into rows of optional doubles (integers widened exactly for
 // the small values used here).
 using Row = std::vector<std::optional<double>>;
-std::vector<Row> drain(Operator& op, size_t* batches = nullptr) {
+std::vector<Row> drain(Operator& op, size_t* batches = nullptr, bool sorted = true) {
     std::vector<Row> rows;
     size_t nb = 0;
     op.open();
@@ -78,7 +80,7 @@ std::vector<Row> drain(Operator& op, size_t* batches = nullptr) {
     }
     op.close();
     if (batches) *batches = nb;
-    std::sort(rows.begin(), rows.end());
+    if (sorted) std::sort(rows.begin(), rows.end());
     return rows;
 }
 
@@ -267,6 +269,140 @@ void test_hash_table_api() {
     CHECK(ng[0] == 0 && ng[1] == 1 && ng[2] == 2 && ng[3] == 1);
 }
 
+// tests/ops_test.cpp:42-56: c0 = 0..n-1 (NULL where i % 7 == 0), c1 = 1.5 * c0.
+Table seq_table(size_t n) {
+    std::vector<int32_t> a(n);
+    std::vector<double> b(n);
+    std::vector<size_t> nulls;
+    for (size_t i = 0; i < n; ++i) {
+        a[i] = static_cast<int32_t>(i);
+        b[i] = 1.5 * static_cast<double>(i);
+        if (i % 7 == 0) nulls.push_back(i);
+    }
+    std::vector<OwnedColumn> c;
+    c.push_back(mk<int32_t>(Type::I32, a, nulls));
+    c.push_back(mk<double>(Type::F64, b));
+    return Table(sch({{"c0", Type::I32}, {"c1", Type::F64}}), std::move(c));
+}
+
+void test_filter_reference_cases() {
+    using namespace qe::expr;
+    Table t = seq_table(300);
+    {  // TRUE keeps all, FALSE keeps none (tests/ops_test.cpp:99-113)
+        GpuFilter f(std::make_unique<Scan>(t, 64), lit(Scalar::boolean(true)));
+        CHECK(drain(f).size() == 300);
+        GpuFilter g(std::make_unique<Scan>(t, 64), lit(Scalar::boolean(false)));
+        CHECK(drain(g).empty());
+        GpuFilter h(std::make_unique<Scan>(t, 64), lit(Scalar::null(Type::BOOL)));
+        CHECK(drain(h).empty());
+    }
+    {  // c0 >= 100 keeps rows 100..299 minus NULL c0, in input order
+        GpuFilter f(std::make_unique<Scan>(t, 64), ge(col(Type::I32, 0), lit(Scalar::i32(100))));
+        size_t nb = 0;
+        const auto rows = drain(f, &nb, false);
+        std::vector<Row> want;
+        for (int i = 100; i < 300; ++i)
+            if (i % 7) want.push_back(Row{double(i), 1.5 * i});
+        CHECK(rows == want);
+        CHECK(nb >= 1);
+    }
+    {  // literal on the left mirrors the operator: 100 <= c0
+        GpuFilter f(std::make_unique<Scan>(t, 64), le(lit(Scalar::i32(100)), col(Type::I32, 0)));
+        CHECK(drain(f).size() == 200 - 28);  // 100..299 minus the 28 multiples of 7 (NULL c0)
+    }
+    {  // conjunction with promotion: c0 > 10.5 AND c1 < 30 (c1 = 1.5 * c0)
+        GpuFilter f(std::make_unique<Scan>(t),
+                    logic_and(gt(col(Type::I32, 0), lit(Scalar::f64(10.5))), lt(col(Type::F64, 1), lit(Scalar::i64(30)))));
+        const auto rows = drain(f, nullptr, false);
+        std::vector<Row> want;
+        for (int i = 11; i < 20; ++i)
+            if (i % 7) want.push_back(Row{double(i), 1.5 * i});
+        CHECK(rows == want);
+    }
+}
+
+void test_filter_plan_and_device_chaining() {
+    using namespace qe::expr;
+    Table t = seq_table(5000);
+    // Aggregate(Filter(Scan)): the filtered columns stay on the device.
+    auto op = plan::scan(t)
+                  .filter(gt(col(t.schema(), "c1"), lit(Scalar::f64(100.0))))
+                  .aggregate({}, {AggSpec::count_star("n"), AggSpec::count(0, "c"), AggSpec::sum(0, "s")})
+                  .build();
+    const auto rows = drain(*op);
+    double n = 0, c = 0, s = 0;
+    for (int i = 0; i < 5000; ++i)
+        if (1.5 * i > 100.0) {
+            ++n;
+            if (i % 7) {
+                ++c;
+                s += i;
+            }
+        }
+    CHECK(rows.size() == 1 && rows[0] == (Row{n, c, s}));
+    // Join(Filter(Scan), Scan) and output batches of <= 2048 rows.
+    Table dim(sch({{"k", Type::I32}, {"g", Type::I64}}), [] {
+        std::vector<OwnedColumn> c;
+        std::vector<int32_t> k(5000);
+        std::vector<int64_t> g(5000);
+        for (int i = 0; i < 5000; ++i) {
+            k[i] = i;
+            g[i] = i % 3;
+        }
+        c.push_back(mk<int32_t>(Type::I32, k));
+        c.push_back(mk<int64_t>(Type::I64, g));
+        return c;
+    }());
+    auto j = plan::scan(t)
+                 .filter(logic_and(ge(col(Type::I32, 0), lit(Scalar::i32(1000))), lt(col(Type::I32, 0), lit(Scalar::i64(4000)))))
+                 .join(plan::scan(dim), {"c0"}, {"k"}, JoinType::Inner)
+                 .aggregate({"g"}, {AggSpec::count_star("n")})
+                 .build();
+    const auto jr = drain(*j);
+    double cnt[3] = {0, 0, 0};
+    for (int i = 1000; i < 4000; ++i)
+        if (i % 7) cnt[i % 3] += 1;
+    CHECK(jr.size() == 3 && jr[0] == (Row{0.0, cnt[0]}) && jr[1] == (Row{1.0, cnt[1]}) && jr[2] == (Row{2.0, cnt[2]}));
+    CHECK(plan::scan(t).filter(gt(col(Type::I32, 0), lit(Scalar::i32(0)))).plan().to_string().rfind("Filter", 0) == 0);
+}
+
+void test_filter_exceptions() {
+    using namespace qe::expr;
+    Table t = seq_table(10);
+    auto expect = [](auto fn, bool runtime) {
+        bool threw = false;
+        try {
+            fn();
+        } catch (const std::invalid_argument&) {
+            threw = !runtime;
+        } catch (const std::runtime_error&) {
+            threw = runtime;
+        }
+        CHECK(threw);
+    };
+    // type errors at build time (expr/expr.cpp:133-151) and in the plan (plan/plan.cpp:307-321)
+    expect([&] { lt(col(Type::I32, 0), lit(Scalar::boolean(true))); }, false);
+    expect([&] { logic_and(col(Type::I32, 0), lit(Scalar::boolean(true))); }, false);
+    expect([&] { plan::scan(t).filter(col(Type::I32, 0)); }, false);
+    expect([&] { plan::scan(t).filter(gt(col(Type::I64, 0), lit(Scalar::i64(0)))); }, false);
+    expect([&] { plan::scan(t).filter(gt(col(Type::I32, 5), lit(Scalar::i32(0)))); }, false);
+    // Filter::open rejects a non-BOOL predicate (ops/filter.cpp:18-20)
+    expect([&] {
+        GpuFilter f(std::make_unique<Scan>(t), col(Type::I32, 0));
+        f.open();
+    }, false);
+    // shapes outside the device path
+    expect([&] {
+        GpuFilter f(std::make_unique<Scan>(t), logic_or(gt(col(Type::I32, 0), lit(Scalar::i32(1))),
+                                                       lt(col(Type::I32, 0), lit(Scalar::i32(0)))));
+        f.open();
+    }, true);
+    expect([&] {
+        GpuFilter f(std::make_unique<Scan>(t), lt(col(Type::I32, 0), col(Type::F64, 1)));
+        f.open();
+    }, true);
+}
+
 int main() {
     const std::vector<std::pair<const char*, std::function<void()>>> tests = {
         {"join_groupby_plan", test_join_groupby_plan},
@@ -275,6 +411,9 @@ int main() {
         {"asof_plan", test_asof_plan},
         {"exceptions", test_exceptions},
         {"hash_table_api", test_hash_table_api},
+        {"filter_reference_cases", test_filter_reference_cases},
+        {"filter_plan_and_device_chaining", test_filter_plan_and_device_chaining},
+        {"filter_exceptions", test_filter_exceptions},
     };
     for (const auto& [name, fn] : tests) {
         std::printf("[host] %s\n", name);

@@ -165,3 +165,25 @@ def test_rejects_bad_terms(ctx, qe):
     finally:
         for d in dev:
             d.free()
+
+
+def test_many_columns_all_widths_and_tile_edges(ctx):
+    # 11 columns (more than one compaction launch), every value width, NULLs in
+    # several columns, over 300 tiles so output validity words straddle tiles.
+    r = np.random.default_rng(77)
+    n = 1_234_567
+    types = [I32, I64, F64, BOOL, TS, I32, F64, I64, BOOL, F64, I32]
+    cols = []
+    for j, t in enumerate(types):
+        if t == F64:
+            v = r.normal(size=n)
+        elif t == BOOL:
+            v = r.integers(0, 2, n)
+        else:
+            v = r.integers(-1000, 1000, n)
+        valid = (r.random(n) >= 0.05) if j % 3 == 0 else None
+        cols.append((np.asarray(v, dtype=O.NP[t]), valid, t))
+    for terms in ([(2, GT, F64, 0.0)], [(0, LT, I32, 500), (4, GE, TS, -900)], [(3, EQ, BOOL, 1)],
+                  [(6, GT, F64, 3.5)]):
+        check_compact(ctx, cols, terms)
+        check(ctx, cols, terms)
