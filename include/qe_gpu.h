@@ -127,6 +127,33 @@ int32_t qe_gpu_column_all_valid(const qe_dev_col* col);
  * the validity sub-view is word aligned, as ops/scan.cpp:15-18 requires). */
 qe_gpu_status qe_gpu_column_slice(qe_gpu_ctx* ctx, const qe_dev_col* col, uint64_t offset, uint64_t len, qe_dev_col** out);
 
+/* ---- dictionary-encoded VARCHAR (STR) ------------------------------------------*/
+/* A STR column's int32 codes index a string dictionary, which replaces the
+ * reference's StringDict (core/string_dict.h:34-72) on the device path: `n`
+ * strings concatenated in `bytes`, string i = bytes[offsets[i], offsets[i+1])
+ * (its bytes_/offsets_ layout, core/string_dict.h:21-23). The library copies
+ * it. STR values compare by their bytes, never by code (core/string_dict.h:
+ * 9-14): group keys and join / as-of keys match by value (ops/join.cpp:48-71,
+ * ops/aggregate.cpp:164-187), MIN/MAX and sort keys order by byte value
+ * (ops/aggregate.cpp:189-205, ops/sort.cpp:82-86). STR outputs (group keys,
+ * MIN/MAX, gathered / filtered / sliced columns) carry the dictionary of the
+ * column they come from; a dictionary must outlive every column and as-of
+ * index that refers to it. */
+typedef struct qe_gpu_dict qe_gpu_dict;
+qe_gpu_status qe_gpu_dict_create(qe_gpu_ctx* ctx, const char* bytes, const uint32_t* offsets, uint32_t n,
+                                 qe_gpu_dict** out);
+void qe_gpu_dict_free(qe_gpu_ctx* ctx, qe_gpu_dict* d);
+uint32_t qe_gpu_dict_size(const qe_gpu_dict* d);
+/* Attach (or, with NULL, detach) the dictionary of a STR column. */
+qe_gpu_status qe_gpu_column_set_dict(qe_gpu_ctx* ctx, qe_dev_col* col, const qe_gpu_dict* d);
+const qe_gpu_dict* qe_gpu_column_dict(const qe_dev_col* col);
+/* out[r] = table[col[r]] as out_type (I32 or STR, no dictionary attached);
+ * NULL rows stay NULL; a valid code outside [0, table_len) is
+ * QE_GPU_INVALID_ARGUMENT. For callers re-encoding codes between dictionaries
+ * (the value-id canonicalisation of ops/join.cpp:48-71). */
+qe_gpu_status qe_gpu_column_remap(qe_gpu_ctx* ctx, const qe_dev_col* col, const int32_t* table, uint32_t table_len,
+                                  int32_t out_type, qe_dev_col** out);
+
 /* ---- result tables ------------------------------------------------------------*/
 uint64_t qe_gpu_table_rows(const qe_dev_table* t);
 uint32_t qe_gpu_table_cols(const qe_dev_table* t);
@@ -231,10 +258,12 @@ qe_gpu_status qe_gpu_ht_key_words(qe_gpu_ctx* ctx, const qe_dev_ht* ht, uint32_t
 void qe_gpu_ht_free(qe_gpu_ctx* ctx, qe_dev_ht* ht);
 
 /* ---- sort ---------------------------------------------------------------------------*/
-/* Stable sort permutation over integer-family key columns (I32/I64/TS/BOOL),
- * keys[0] most significant, per-key desc / nulls_first flags (either array may
- * be NULL = ASC / NULLS LAST). Replaces Sort's radix path (ops/sort.cpp:125-188,
- * Sort ctor ops/sort.h:84). perm_out: device u32[n]. */
+/* Stable sort permutation, keys[0] most significant, per-key desc /
+ * nulls_first flags (either array may be NULL = ASC / NULLS LAST). Replaces
+ * Sort (ops/sort.h:84): integer-family keys as its radix path
+ * (ops/sort.cpp:125-188); F64 keys in the NaN-greatest total order and STR keys
+ * by value, as its comparison path (ops/sort.cpp:60-100). perm_out: device
+ * u32[n]. */
 qe_gpu_status qe_gpu_sort_perm(qe_gpu_ctx* ctx, const qe_dev_col* const* keys, uint32_t nkeys, const int32_t* desc,
                                const int32_t* nulls_first, uint32_t* perm_out);
 

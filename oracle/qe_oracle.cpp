@@ -39,6 +39,8 @@
 #include <map>
 #include <numeric>
 #include <stdexcept>
+#include <string>
+#include <string_view>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -47,12 +49,17 @@ extern "C" {
 
 // Mirrors the reference Column view (core/column.h:55-67): type, length, data,
 // LSB-first u64 validity words, all_valid authoritative (core/column.h:42-54).
+// STR columns carry their dictionary as the reference's StringDict lays it out
+// (core/string_dict.h:21-23): all strings concatenated, dict_n + 1 offsets.
 typedef struct {
     int32_t type;  // 0 I32, 1 I64, 2 F64, 3 BOOL, 4 TS, 5 STR (core/types.h:18)
     int32_t all_valid;
     uint64_t len;
     const void* data;
     const uint64_t* validity;
+    const char* dict_bytes;
+    const uint32_t* dict_off;
+    uint64_t dict_n;
 } orc_col;
 
 }  // extern "C"
@@ -139,6 +146,58 @@ double f64_at(const orc_col& c, uint64_t r) {
     double d;
     std::memcpy(&d, static_cast<const char*>(c.data) + r * 8, 8);
     return d;
+}
+
+// The bytes of a STR code (StringDict::at, core/string_dict.h:54-61).
+std::string_view str_of_code(const orc_col& c, int64_t code) {
+    if (c.dict_off == nullptr || code < 0 || static_cast<uint64_t>(code) >= c.dict_n)
+        throw std::out_of_range("STR code outside its dictionary");
+    const uint32_t lo = c.dict_off[code], hi = c.dict_off[code + 1];
+    return std::string_view(c.dict_bytes + lo, hi - lo);
+}
+
+// A column this library owns (canonicalised keys, gathered join output).
+struct OwnedCol {
+    int32_t type = I32;
+    std::vector<char> data;
+    std::vector<uint64_t> validity;
+    bool all_valid = true;
+    const char* dict_bytes = nullptr;
+    const uint32_t* dict_off = nullptr;
+    uint64_t dict_n = 0;
+    orc_col view() const {
+        orc_col c;
+        c.type = type;
+        c.all_valid = all_valid;
+        c.len = data.size() / width(type);
+        c.data = data.data();
+        c.validity = all_valid ? nullptr : validity.data();
+        c.dict_bytes = dict_bytes;
+        c.dict_off = dict_off;
+        c.dict_n = dict_n;
+        return c;
+    }
+};
+
+// STR key canonicalisation (ops/join.cpp:48-71, ops/aggregate.cpp:164-187): the
+// codes of a STR key column become I32 value ids interned by string VALUE in
+// row order into `ids` (shared by both join sides), so equal strings from
+// different dictionaries get one id; NULL rows stay NULL.
+OwnedCol canon_str(const orc_col& c, uint64_t n, std::unordered_map<std::string, int32_t>& ids) {
+    OwnedCol o;
+    o.type = I32;
+    o.data.assign(n * 4, 0);
+    o.validity.assign((n + 63) / 64, ~0ull);
+    for (uint64_t r = 0; r < n; ++r) {
+        if (!valid_at(c, r)) {
+            o.all_valid = false;
+            o.validity[r >> 6] &= ~(1ull << (r & 63));
+            continue;
+        }
+        auto it = ids.emplace(std::string(str_of_code(c, int_at(c, r))), static_cast<int32_t>(ids.size())).first;
+        std::memcpy(o.data.data() + r * 4, &it->second, 4);
+    }
+    return o;
 }
 
 // ---------------------------------------------------------------------------
@@ -302,7 +361,8 @@ struct Oht {
 // against). Mutants: 1 = NULL folded into SUM as 0 and counted
 // (aggregate_mutants kFoldNullInSum); 2 = empty group emits 0 instead of NULL
 // (kEmptyGroupZero); 3 = MAX drops NaN (kMaxDropsNan); 4 = the last row of the
-// input is not aggregated (a tail off-by-one).
+// input is not aggregated (a tail off-by-one); 5 = STR keys grouped by raw code
+// instead of by value.
 
 bool f64_less_total(double a, double b) {  // ops/agg_internal.h:34-38
     if (std::isnan(b)) return !std::isnan(a);
@@ -416,11 +476,21 @@ AggResult* aggregate_impl(const orc_col* cols, uint64_t nrows, const uint32_t* k
     std::vector<uint32_t> gid(nrows, 0);
     Oht* ht = nullptr;
     std::vector<orc_col> kc;
+    std::vector<OwnedCol> canon(nkeys);
     if (nkeys > 0) {
         std::vector<int32_t> kt;
         for (uint32_t j = 0; j < nkeys; ++j) {
             kc.push_back(cols[key_cols[j]]);
-            kt.push_back(cols[key_cols[j]].type);
+            if (kc.back().type == STR) {  // group STR keys by value (mutant 5: by raw code)
+                if (mutant != 5) {
+                    std::unordered_map<std::string, int32_t> ids;
+                    canon[j] = canon_str(kc.back(), nrows, ids);
+                    kc.back() = canon[j].view();
+                } else {
+                    kc.back().type = I32;
+                }
+            }
+            kt.push_back(kc.back().type);
         }
         ht = new Oht(kt.data(), nkeys, kEqual, 1024, 0.7, 0x51E57A11AB1E5EEDull);
         if (nrows) ht->insert_or_find(kc.data(), nullptr, nrows, gid.data());
@@ -444,6 +514,20 @@ AggResult* aggregate_impl(const orc_col* cols, uint64_t nrows, const uint32_t* k
                     else iv = int_at(*in, r);
                 }
             }
+            if (in && in->type == STR && (f == Min || f == Max)) {
+                // ops/aggregate.cpp:189-205: the extremum by lexicographic byte
+                // value; the cell keeps the winning row's code.
+                Cell& c = cells[a][gid[r]];
+                if (!v) continue;
+                if (c.cnt == 0) {
+                    c.i = iv;
+                } else {
+                    const std::string_view sv = str_of_code(*in, iv), cur = str_of_code(*in, c.i);
+                    if (f == Min ? sv < cur : sv > cur) c.i = iv;
+                }
+                ++c.cnt;
+                continue;
+            }
             scatter(cells[a][gid[r]], f, isf[a], v, iv, dv, sum_mode, mutant);
         }
     }
@@ -454,6 +538,20 @@ AggResult* aggregate_impl(const orc_col* cols, uint64_t nrows, const uint32_t* k
             res->key_words[j][g] = ht->words[j][g];
             res->key_null[j][g] = (ht->null_mask[g] >> j) & 1ull;
         }
+    // A STR key reads back as the source code of its group's first row (a code of
+    // the input's dictionary with the group's value).
+    std::vector<uint64_t> first_row;
+    for (uint32_t j = 0; j < nkeys; ++j) {
+        if (cols[key_cols[j]].type != STR) continue;
+        if (first_row.empty()) {
+            first_row.assign(res->ngroups, UINT64_MAX);
+            for (uint64_t r = 0; r < nrows; ++r)
+                if (first_row[gid[r]] == UINT64_MAX) first_row[gid[r]] = r;
+        }
+        for (uint64_t g = 0; g < res->ngroups; ++g)
+            if (!res->key_null[j][g] && first_row[g] != UINT64_MAX)
+                res->key_words[j][g] = static_cast<uint32_t>(int_at(cols[key_cols[j]], first_row[g]));
+    }
     res->out_is_float.resize(naggs);
     res->out_i.assign(naggs, std::vector<int64_t>(res->ngroups, 0));
     res->out_d.assign(naggs, std::vector<double>(res->ngroups, 0.0));
@@ -499,7 +597,8 @@ AggResult* aggregate_impl(const orc_col* cols, uint64_t nrows, const uint32_t* k
 // Mutants: 1 = the last build row of every multi-row group is dropped
 // (join_mutants kFanoutTailOffByOne); 2 = the first matched probe row is
 // dropped (kDropProbeMatch); 3 = composite keys compare only the first
-// column (kCompositeFirstKeyOnly).
+// column (kCompositeFirstKeyOnly); 4 = STR keys compared by raw dictionary code
+// (string_key_mutants kHashRawCode).
 struct JoinResult {
     std::vector<uint32_t> probe_rows, build_rows;
 };
@@ -511,10 +610,23 @@ JoinResult* join_impl(const orc_col* pcols, uint64_t np, const orc_col* bcols,
     const uint32_t nk = (mutant == 3) ? 1 : nkeys;
     std::vector<orc_col> bk, pk;
     std::vector<int32_t> kt;
+    std::vector<OwnedCol> canon(2 * nk);
     for (uint32_t j = 0; j < nk; ++j) {
         bk.push_back(bcols[bkeys[j]]);
         pk.push_back(pcols[pkeys[j]]);
-        kt.push_back(bcols[bkeys[j]].type);
+        if (bk.back().type == STR && pk.back().type == STR) {
+            if (mutant != 4) {  // mutant 4: STR keys by raw code (string_key_mutants kHashRawCode)
+                std::unordered_map<std::string, int32_t> ids;  // one id space per key position
+                canon[2 * j] = canon_str(bk.back(), nb, ids);
+                canon[2 * j + 1] = canon_str(pk.back(), np, ids);
+                bk.back() = canon[2 * j].view();
+                pk.back() = canon[2 * j + 1].view();
+            } else {
+                bk.back().type = I32;
+                pk.back().type = I32;
+            }
+        }
+        kt.push_back(bk.back().type);
     }
     Oht ht(kt.data(), nk, kNeverMatch, 1024, 0.7, 0x51E57A11AB1E5EEDull);
     std::vector<uint32_t> bg(nb), pg(np);
@@ -544,26 +656,14 @@ JoinResult* join_impl(const orc_col* pcols, uint64_t np, const orc_col* bcols,
 }
 
 // Gather the join-output column `c` (probe cols ++ build cols) for the pair list
-// into an owned column (ops/join_internal.h:136-179 emit semantics).
-struct OwnedCol {
-    int32_t type;
-    std::vector<char> data;
-    std::vector<uint64_t> validity;
-    bool all_valid = true;
-    orc_col view() const {
-        orc_col c;
-        c.type = type;
-        c.all_valid = all_valid;
-        c.len = data.size() / width(type);
-        c.data = data.data();
-        c.validity = all_valid ? nullptr : validity.data();
-        return c;
-    }
-};
-
+// into an owned column (ops/join_internal.h:136-179 emit semantics); a STR
+// column keeps its codes and dictionary.
 OwnedCol gather_col(const orc_col& src, const std::vector<uint32_t>& idx) {
     OwnedCol o;
     o.type = src.type;
+    o.dict_bytes = src.dict_bytes;
+    o.dict_off = src.dict_off;
+    o.dict_n = src.dict_n;
     const size_t w = width(src.type);
     o.data.assign(idx.size() * w, 0);
     o.validity.assign((idx.size() + 63) / 64, ~0ull);

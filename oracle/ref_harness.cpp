@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "core/owned_batch.h"
+#include "core/string_dict.h"
 #include "core/validity.h"
 #include "expr/expr.h"
 #include "ops/aggregate.h"
@@ -37,6 +38,9 @@ typedef struct {
     uint64_t len;
     const void* data;
     const uint64_t* validity;
+    const char* dict_bytes;  // STR: the dictionary's strings, concatenated
+    const uint32_t* dict_off;  // STR: dict_n + 1 offsets
+    uint64_t dict_n;
 } orc_col;
 }
 
@@ -59,6 +63,23 @@ Column view_of(const orc_col& c) {
 }
 
 OwnedColumn owned_of(const orc_col& c, uint64_t n) {
+    if (c.type == static_cast<int32_t>(Type::STR)) {
+        // A fresh StringDict interning the given strings in order; the column's
+        // codes index that list, so they are translated through intern().
+        auto dict = std::make_shared<StringDict>();
+        std::vector<int32_t> code_of(c.dict_n);
+        for (uint64_t i = 0; i < c.dict_n; ++i)
+            code_of[i] = dict->intern(std::string_view(c.dict_bytes + c.dict_off[i], c.dict_off[i + 1] - c.dict_off[i]));
+        OwnedColumn o = OwnedColumn::make_str(n, dict);
+        auto* codes = reinterpret_cast<int32_t*>(o.mutable_data());
+        const auto* src = static_cast<const int32_t*>(c.data);
+        for (uint64_t r = 0; r < n; ++r) {
+            const bool valid = c.all_valid || c.validity == nullptr || validity::get_bit(c.validity, r);
+            if (valid) codes[r] = code_of[src[r]];
+            else o.set_null(r);
+        }
+        return o;
+    }
     OwnedColumn o = OwnedColumn::make(ty(c.type), n);
     std::memcpy(o.mutable_data(), c.data, n * byte_width(ty(c.type)));
     if (!(c.all_valid || c.validity == nullptr))
@@ -91,6 +112,7 @@ struct ResultRows {
     std::vector<std::vector<int64_t>> iv;
     std::vector<std::vector<double>> dv;
     std::vector<std::vector<uint8_t>> nulls;
+    std::vector<std::vector<std::string>> sv;  // STR columns: resolved values
     uint64_t nrows = 0;
 };
 
@@ -102,6 +124,7 @@ ResultRows* drain(Operator& op) {
     rs->iv.resize(nc);
     rs->dv.resize(nc);
     rs->nulls.resize(nc);
+    rs->sv.resize(nc);
     op.open();
     while (std::optional<Batch> b = op.next()) {
         for (size_t k = 0; k < b->row_count; ++k) {
@@ -115,8 +138,12 @@ ResultRows* drain(Operator& op) {
                 if (valid) {
                     switch (col.type) {
                         case Type::I32:
+                            i = reinterpret_cast<const int32_t*>(col.data)[p];
+                            break;
                         case Type::STR:
                             i = reinterpret_cast<const int32_t*>(col.data)[p];
+                            rs->sv[c].resize(rs->iv[c].size());
+                            rs->sv[c].emplace_back(col.dict->at(static_cast<int32_t>(i)));
                             break;
                         case Type::BOOL:
                             i = reinterpret_cast<const uint8_t*>(col.data)[p];
@@ -130,6 +157,7 @@ ResultRows* drain(Operator& op) {
                 }
                 rs->iv[c].push_back(i);
                 rs->dv[c].push_back(d);
+                if (col.type == Type::STR) rs->sv[c].resize(rs->iv[c].size());
             }
         }
         rs->nrows += b->row_count;
@@ -199,6 +227,26 @@ void ref_rs_col(void* h, uint32_t c, int64_t* iv, double* dv, uint8_t* nulls) {
     std::memcpy(nulls, r->nulls[c].data(), r->nrows);
 }
 void ref_rs_free(void* h) { delete static_cast<ResultRows*>(h); }
+// STR column c: total bytes of its values; then the values concatenated with
+// nrows + 1 offsets (a NULL row is an empty string; ref_rs_col has its NULL flag).
+uint64_t ref_rs_str_bytes(void* h, uint32_t c) {
+    uint64_t b = 0;
+    for (const auto& x : static_cast<ResultRows*>(h)->sv[c]) b += x.size();
+    return b;
+}
+void ref_rs_str_get(void* h, uint32_t c, char* bytes, uint64_t* offs) {
+    const auto& v = static_cast<ResultRows*>(h)->sv[c];
+    const uint64_t n = static_cast<ResultRows*>(h)->nrows;
+    uint64_t o = 0;
+    for (uint64_t r = 0; r < n; ++r) {
+        offs[r] = o;
+        if (r < v.size()) {
+            std::memcpy(bytes + o, v[r].data(), v[r].size());
+            o += v[r].size();
+        }
+    }
+    offs[n] = o;
+}
 
 static std::vector<AggSpec> specs_of(const int32_t* funcs, const uint32_t* inputs,
                                      uint32_t naggs) {

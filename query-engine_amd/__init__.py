@@ -101,6 +101,12 @@ _SIGS = {
     "qe_gpu_column_type": (C.c_int32, [P]),
     "qe_gpu_column_all_valid": (C.c_int32, [P]),
     "qe_gpu_column_slice": (C.c_int, [P, P, C.c_uint64, C.c_uint64, C.POINTER(P)]),
+    "qe_gpu_dict_create": (C.c_int, [P, P, P, C.c_uint32, C.POINTER(P)]),
+    "qe_gpu_dict_free": (None, [P, P]),
+    "qe_gpu_dict_size": (C.c_uint32, [P]),
+    "qe_gpu_column_set_dict": (C.c_int, [P, P, P]),
+    "qe_gpu_column_dict": (P, [P]),
+    "qe_gpu_column_remap": (C.c_int, [P, P, P, C.c_uint32, C.c_int32, C.POINTER(P)]),
     "qe_gpu_table_rows": (C.c_uint64, [P]),
     "qe_gpu_table_cols": (C.c_uint32, [P]),
     "qe_gpu_table_col": (P, [P, C.c_uint32]),
@@ -190,11 +196,54 @@ def unpack_validity(words: np.ndarray, n: int) -> np.ndarray:
     return np.unpackbits(words.view(np.uint8), bitorder="little")[:n].astype(bool)
 
 
+class Dict:
+    """A string dictionary for STR columns (the StringDict layout: strings
+    concatenated plus offsets). `strings[code]` is a code's value."""
+
+    def __init__(self, ctx: "Context", strings):
+        self.ctx = ctx
+        self.strings = list(strings)
+        enc = [x.encode() for x in self.strings]
+        off = np.zeros(len(enc) + 1, dtype=np.uint32)
+        if enc:
+            off[1:] = np.cumsum([len(e) for e in enc])
+        blob = np.frombuffer(b"".join(enc) + b"\0", dtype=np.uint8).copy()
+        h = P()
+        ctx._check(lib().qe_gpu_dict_create(ctx.h, _ptr(blob), _ptr(off), len(enc), C.byref(h)))
+        self.h = h
+        ctx._dicts[h.value] = self
+
+    def __len__(self):
+        return len(self.strings)
+
+    def free(self):
+        if self.h:
+            self.ctx._dicts.pop(self.h.value, None)
+            lib().qe_gpu_dict_free(self.ctx.h, self.h)
+        self.h = None
+
+
 class DevCol:
     """A device column (HBM). Owned columns are freed with the context."""
 
     def __init__(self, ctx: "Context", handle, owned: bool = True):
         self.ctx, self.h, self.owned = ctx, handle, owned
+
+    @property
+    def dict(self) -> "Dict | None":
+        """The dictionary of a STR column (a Dict of this context), or None."""
+        d = lib().qe_gpu_column_dict(self.h)
+        return self.ctx._dicts.get(d) if d else None
+
+    def set_dict(self, d: "Dict | None"):
+        self.ctx._check(lib().qe_gpu_column_set_dict(self.ctx.h, self.h, d.h if d is not None else None))
+
+    def strings(self):
+        """A STR column's values as Python strings (None for NULL)."""
+        vals, valid = self.numpy()
+        d = self.dict
+        assert d is not None, "STR column without a dictionary"
+        return [None if (valid is not None and not valid[i]) else d.strings[c] for i, c in enumerate(vals.tolist())]
 
     @property
     def len(self) -> int:
@@ -321,9 +370,12 @@ class Context:
         if st != 0:
             raise HipError(f"qe_gpu_ctx_create failed with status {st} (is a GPU visible?)")
         self.h = h
+        self._dicts = {}  # dictionary handle -> Dict
 
     def close(self):
         if self.h:
+            for d in list(self._dicts.values()):
+                d.free()
             lib().qe_gpu_ctx_destroy(self.h)
         self.h = None
 
@@ -339,7 +391,9 @@ class Context:
             raise _ERRORS.get(st, QeError)(msg)
 
     # ---- columns -----------------------------------------------------------
-    def upload(self, values: np.ndarray, type: int | None = None, valid: np.ndarray | None = None) -> DevCol:
+    def upload(self, values: np.ndarray, type: int | None = None, valid: np.ndarray | None = None,
+               dict: "Dict | None" = None) -> DevCol:
+        """A device column; a STR column's values are int32 codes into `dict`."""
         if type is None:
             type = {np.dtype(np.int32): I32, np.dtype(np.int64): I64, np.dtype(np.float64): F64,
                     np.dtype(np.uint8): BOOL, np.dtype(np.bool_): BOOL}[np.asarray(values).dtype]
@@ -348,6 +402,35 @@ class Context:
         d = ColDesc(type, 1 if valid is None else 0, len(v), _ptr(v), _ptr(words))
         out = P()
         self._check(lib().qe_gpu_column_upload(self.h, C.byref(d), C.byref(out)))
+        col = DevCol(self, out)
+        if dict is not None:
+            col.set_dict(dict)
+        return col
+
+    def dict(self, strings) -> Dict:
+        """A string dictionary (freed with the context or by Dict.free)."""
+        return Dict(self, strings)
+
+    def upload_strings(self, values, d: "Dict | None" = None) -> DevCol:
+        """A STR column from Python strings (None = NULL), encoded in `d` or in a
+        new dictionary of its distinct values in first-seen order."""
+        if d is None:
+            d = self.dict(list(dict.fromkeys(x for x in values if x is not None)))
+        idx = {x: i for i, x in enumerate(d.strings)}
+        codes = np.array([idx[x] if x is not None else 0 for x in values], dtype=np.int32)
+        valid = None if all(x is not None for x in values) else np.array([x is not None for x in values])
+        return self.upload(codes, STR, valid, dict=d)
+
+    def slice(self, col: DevCol, offset: int, n: int) -> DevCol:
+        """Rows [offset, offset + n) of `col` without a copy (offset % 64 == 0)."""
+        out = P()
+        self._check(lib().qe_gpu_column_slice(self.h, col.h, offset, n, C.byref(out)))
+        return DevCol(self, out)
+
+    def remap(self, col: DevCol, table, out_type: int = I32) -> DevCol:
+        t = np.ascontiguousarray(table, dtype=np.int32)
+        out = P()
+        self._check(lib().qe_gpu_column_remap(self.h, col.h, _ptr(t), len(t), out_type, C.byref(out)))
         return DevCol(self, out)
 
     def alloc(self, type: int, n: int, nullable: bool = False) -> DevCol:

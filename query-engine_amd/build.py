@@ -34,7 +34,7 @@ HIP_FLAGS = [
     "-Wno-unused-result", "-Wno-unused-value", f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 HIP_SOURCES = ["context.hip", "groupby.hip", "join.hip", "hashtable.hip", "sort.hip", "batched.hip",
-               "asof.hip", "gen.hip", "filter.hip"]
+               "asof.hip", "gen.hip", "filter.hip", "strings.hip"]
 HOST_SOURCES = ["operators.cpp", "plan.cpp", "expr.cpp"]
 
 

@@ -61,6 +61,7 @@ struct PartMeta {
 struct qe_dev_asof {
     int nkeys = 0;
     int32_t key_types[kMaxKeys] = {};
+    const qe_gpu_dict* key_dicts[kMaxKeys] = {};  // STR keys: the build side's dictionary
     int mode = 0;  // 0 zero keys, 1 direct, 2 hash
     int64_t kmin = 0;
     uint64_t range = 0;
@@ -813,8 +814,12 @@ static bool time_type_ok(int t) { return t == QE_I32 || t == QE_I64 || t == QE_T
 
 extern "C" {
 
-qe_gpu_status qe_gpu_asof_build(qe_gpu_ctx* ctx, const qe_dev_col* const* build_keys, uint32_t nkeys,
-                                const qe_dev_col* build_time, qe_dev_asof** out) {
+}  // extern "C"
+
+namespace qe_gpu {
+
+static qe_gpu_status asof_build_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* build_keys, uint32_t nkeys,
+                                      const qe_dev_col* build_time, qe_dev_asof** out) {
     QE_ENTRY(ctx, {
         invalid(out != nullptr && build_time != nullptr, "null argument");
         // tsx/asof.cpp:144-152: time columns must be I32/I64/TS.
@@ -907,6 +912,31 @@ qe_gpu_status qe_gpu_asof_build(qe_gpu_ctx* ctx, const qe_dev_col* const* build_
             throw;
         }
         *out = idx;
+    });
+}
+
+}  // namespace qe_gpu
+
+extern "C" {
+
+// STR keys partition by value: a dictionary with repeated values is read as
+// canonical codes (strings.hip); the index keeps the build dictionary so the
+// probe side maps its codes onto the same values.
+qe_gpu_status qe_gpu_asof_build(qe_gpu_ctx* ctx, const qe_dev_col* const* build_keys, uint32_t nkeys,
+                                const qe_dev_col* build_time, qe_dev_asof** out) {
+    QE_ENTRY(ctx, {
+        invalid(out != nullptr && (build_keys != nullptr || nkeys == 0), "null argument");
+        check(nkeys <= kMaxKeys, QE_GPU_UNSUPPORTED, "too many as-of keys for the device path");
+        OwnedCols own(ctx);
+        std::vector<const qe_dev_col*> bk(build_keys, build_keys + nkeys);
+        for (uint32_t j = 0; j < nkeys; ++j) {
+            if (bk[j] == nullptr || bk[j]->type != QE_STR) continue;
+            const qe_gpu_dict* d = need_dict(bk[j]);
+            if (!d->unique) bk[j] = own.keep(remap_col(ctx, bk[j], d->canon, QE_STR, d));
+        }
+        const qe_gpu_status st = asof_build_codes(ctx, bk.data(), nkeys, build_time, out);
+        if (st != QE_GPU_OK) throw Error{st, ctx->last_error};
+        for (uint32_t j = 0; j < nkeys; ++j) (*out)->key_dicts[j] = bk[j]->type == QE_STR ? bk[j]->dict : nullptr;
     });
 }
 
@@ -1010,9 +1040,13 @@ qe_gpu_status qe_gpu_asof_train(qe_gpu_ctx* ctx, qe_dev_asof* idx, uint32_t piec
     });
 }
 
-qe_gpu_status qe_gpu_asof_probe(qe_gpu_ctx* ctx, const qe_dev_asof* idx, const qe_dev_col* const* probe_keys,
-                                uint32_t nkeys, const qe_dev_col* probe_time, int32_t has_tolerance,
-                                int64_t tolerance, int32_t mode, uint32_t* match_out) {
+}  // extern "C"
+
+namespace qe_gpu {
+
+static qe_gpu_status asof_probe_codes(qe_gpu_ctx* ctx, const qe_dev_asof* idx, const qe_dev_col* const* probe_keys,
+                                      uint32_t nkeys, const qe_dev_col* probe_time, int32_t has_tolerance,
+                                      int64_t tolerance, int32_t mode, uint32_t* match_out) {
     QE_ENTRY(ctx, {
         invalid(idx != nullptr && probe_time != nullptr, "null argument");
         invalid(time_type_ok(probe_time->type), "AsofJoin: time columns must be I32/I64/TS");
@@ -1073,6 +1107,34 @@ __global__ void asof_emit_kernel(const uint32_t* match, uint64_t n, const uint32
         pp[pos[j]] = static_cast<uint32_t>(j);
         bp[pos[j]] = match[j];
     }
+}
+
+}  // namespace qe_gpu
+
+extern "C" {
+
+// STR probe keys map onto the build dictionary's values (strings.hip): a value
+// the build side lacks gets an id no partition has.
+qe_gpu_status qe_gpu_asof_probe(qe_gpu_ctx* ctx, const qe_dev_asof* idx, const qe_dev_col* const* probe_keys,
+                                uint32_t nkeys, const qe_dev_col* probe_time, int32_t has_tolerance,
+                                int64_t tolerance, int32_t mode, uint32_t* match_out) {
+    QE_ENTRY(ctx, {
+        invalid(idx != nullptr && (probe_keys != nullptr || nkeys == 0), "null argument");
+        invalid(static_cast<int>(nkeys) == idx->nkeys, "as-of needs equal-length key lists");
+        OwnedCols own(ctx);
+        std::vector<const qe_dev_col*> pk(probe_keys, probe_keys + nkeys);
+        for (uint32_t j = 0; j < nkeys; ++j) {
+            if (pk[j] == nullptr || pk[j]->type != QE_STR || idx->key_types[j] != QE_STR) continue;
+            const qe_gpu_dict* pd = need_dict(pk[j]);
+            const qe_gpu_dict* bd = idx->key_dicts[j];
+            invalid(bd != nullptr, "as-of index has no dictionary for a STR key");
+            if (pd == bd && bd->unique) continue;
+            pk[j] = own.keep(remap_col(ctx, pk[j], pd == bd ? bd->canon : probe_ids(bd, pd), QE_STR, nullptr));
+        }
+        const qe_gpu_status st = asof_probe_codes(ctx, idx, pk.data(), nkeys, probe_time, has_tolerance, tolerance,
+                                                  mode, match_out);
+        if (st != QE_GPU_OK) throw Error{st, ctx->last_error};
+    });
 }
 
 qe_gpu_status qe_gpu_asof_pairs(qe_gpu_ctx* ctx, const uint32_t* match, uint64_t n, int32_t join_type,

@@ -623,6 +623,7 @@ qe_gpu_status qe_gpu_column_slice(qe_gpu_ctx* ctx, const qe_dev_col* c, uint64_t
         s->all_valid = c->all_valid;
         s->validity = c->all_valid ? nullptr : c->validity + offset / 64;
         s->owns = false;
+        s->dict = c->dict;
         *out = s;
     });
 }

@@ -415,6 +415,7 @@ qe_gpu_status qe_gpu_filter(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, uint
             for (uint32_t c = 0; c < ncols; ++c) {
                 const bool nullable = !cols[c]->all_valid && cols[c]->validity != nullptr && m > 0;
                 t->cols.push_back(new_col(ctx, cols[c]->type, m, nullable));
+                t->cols.back()->dict = cols[c]->dict;
                 if (nullable) memset_async(ctx, t->cols.back()->validity, 0, words_for(m) * 8);
             }
             if (m > 0) {

@@ -389,9 +389,9 @@ bool is_int_like(int t) { return t == QE_I32 || t == QE_I64 || t == QE_TS || t =
 // Direct-slot limit: at most this many slots before falling back to hashing.
 static constexpr uint64_t kDirectMaxSlots = uint64_t(1) << 24;
 
-extern "C" qe_gpu_status qe_gpu_aggregate(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, uint32_t ncols,
-                                          const uint32_t* key_cols, uint32_t nkeys,
-                                          const qe_agg_spec* aggs, uint32_t naggs, qe_dev_table** out) {
+qe_gpu_status qe_gpu::aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, uint32_t ncols,
+                                      const uint32_t* key_cols, uint32_t nkeys, const qe_agg_spec* aggs,
+                                      uint32_t naggs, qe_dev_table** out) {
     QE_ENTRY(ctx, {
         invalid(out != nullptr, "null out");
         invalid(ncols >= 1, "aggregate needs an input with >=1 column");
@@ -469,5 +469,36 @@ extern "C" qe_gpu_status qe_gpu_aggregate(qe_gpu_ctx* ctx, const qe_dev_col* con
         ke.words = words.as<uint64_t>();
         ke.nullmask = nmask.as<uint64_t>();
         *out = emit_groups(ctx, st.as<uint64_t>(), cap, p, ke, false);
+    });
+}
+
+// STR group keys and MIN/MAX over STR compare by value (strings.hip), then the
+// group-by proper.
+extern "C" qe_gpu_status qe_gpu_aggregate(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, uint32_t ncols,
+                                          const uint32_t* key_cols, uint32_t nkeys,
+                                          const qe_agg_spec* aggs, uint32_t naggs, qe_dev_table** out) {
+    QE_ENTRY(ctx, {
+        invalid(out != nullptr, "null out");
+        invalid(cols != nullptr && ncols >= 1, "aggregate needs an input with >=1 column");
+        invalid(aggs != nullptr || naggs == 0, "null aggregate list");
+        invalid(key_cols != nullptr || nkeys == 0, "null key list");
+        OwnedCols own(ctx);
+        std::vector<const qe_dev_col*> c2(cols, cols + ncols);
+        std::vector<qe_agg_spec> a2(aggs, aggs + naggs);
+        StrAggFix fix;
+        const bool any = str_agg_prepare(ctx, c2, key_cols, nkeys, a2, fix, own);
+        qe_dev_table* t = nullptr;
+        const qe_gpu_status st = aggregate_codes(ctx, c2.data(), static_cast<uint32_t>(c2.size()), key_cols, nkeys,
+                                                 a2.data(), naggs, &t);
+        if (st != QE_GPU_OK) throw Error{st, ctx->last_error};
+        if (any) {
+            try {
+                str_agg_finish(ctx, t, fix);
+            } catch (...) {
+                qe_gpu_table_free(ctx, t);
+                throw;
+            }
+        }
+        *out = t;
     });
 }

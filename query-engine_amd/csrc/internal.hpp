@@ -9,6 +9,8 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <string_view>
+#include <unordered_map>
 #include <vector>
 
 #include "qe_gpu.h"
@@ -61,6 +63,8 @@ __host__ __device__ inline uint64_t words_for(uint64_t n) { return (n + 63) / 64
 
 // ---------------------------------------------------------------------------
 // Opaque handle bodies (C ABI names).
+struct qe_gpu_dict;
+
 struct qe_dev_col {
     int32_t type = QE_I64;
     uint64_t len = 0;
@@ -68,6 +72,21 @@ struct qe_dev_col {
     uint64_t* validity = nullptr;  // nullptr <=> all_valid
     bool all_valid = true;
     bool owns = true;  // false for slices
+    const qe_gpu_dict* dict = nullptr;  // STR: the dictionary its codes index (not owned)
+};
+
+// A string dictionary (core/string_dict.h): string i = bytes[off[i], off[i+1]).
+// Derived tables are built on first use and kept with it.
+struct qe_gpu_dict {
+    std::vector<char> bytes;
+    std::vector<uint32_t> off;  // n + 1
+    uint32_t n = 0;
+    bool unique = true;           // every value appears under one code
+    std::vector<int32_t> canon;   // code -> first code with the same value
+    std::unordered_map<std::string_view, int32_t> index;  // value -> first code
+    std::vector<int32_t> rank;     // code -> rank of its value in byte order
+    std::vector<int32_t> by_rank;  // rank -> first code with that value
+    std::string_view at(uint32_t c) const { return std::string_view(bytes.data() + off[c], off[c + 1] - off[c]); }
 };
 
 struct qe_dev_table {
@@ -260,6 +279,52 @@ inline unsigned grid_for(uint64_t n, unsigned block, unsigned max_blocks = 65535
 // ---- device-wide exclusive scan of u32 flags/counts into u64 offsets ---------
 // out[i] = sum_{j<i} in[j]; returns the total (synchronises the stream).
 uint64_t exclusive_scan_u32(qe_gpu_ctx* ctx, const uint32_t* in, uint64_t* out, uint64_t n);
+
+// ---- dictionary-encoded strings (strings.hip) ---------------------------------
+// out[r] = table[codes[r]] as `out_type` (I32 or STR), NULL rows stay NULL; a
+// valid code outside the table is an invalid_argument error.
+qe_dev_col* remap_col(qe_gpu_ctx* ctx, const qe_dev_col* col, const std::vector<int32_t>& table, int out_type,
+                      const qe_gpu_dict* out_dict);
+const qe_gpu_dict* need_dict(const qe_dev_col* c);  // a STR column's dictionary, or invalid_argument
+// Join-key id tables: build codes -> canonical codes; probe codes -> the build
+// dictionary's code of the same value, or build.n + code when the build side
+// lacks the value (never matches).
+std::vector<int32_t> probe_ids(const qe_gpu_dict* build, const qe_gpu_dict* probe);
+
+// Columns owned for the duration of one entry point.
+struct OwnedCols {
+    qe_gpu_ctx* ctx;
+    std::vector<qe_dev_col*> cols;
+    explicit OwnedCols(qe_gpu_ctx* c) : ctx(c) {}
+    qe_dev_col* keep(qe_dev_col* c) {
+        cols.push_back(c);
+        return c;
+    }
+    ~OwnedCols();
+};
+
+// STR join keys made comparable by value: returns the columns to hash in place
+// of (probe, build) key j (either unchanged or remapped id columns owned by `own`).
+void str_join_keys(qe_gpu_ctx* ctx, const qe_dev_col* const* pk, const qe_dev_col* const* bk, uint32_t nkeys,
+                   std::vector<const qe_dev_col*>& pk_out, std::vector<const qe_dev_col*>& bk_out, OwnedCols& own);
+
+// An aggregate's STR handling: group keys on a dictionary with repeated values
+// read canonical codes; MIN/MAX over STR read value ranks. After the aggregate,
+// str_agg_finish gives STR key outputs their dictionary and maps rank outputs
+// back to codes.
+struct StrAggFix {
+    std::vector<const qe_gpu_dict*> key_dict;  // per group key: STR dictionary or null
+    std::vector<const qe_gpu_dict*> rank_dict; // per aggregate: STR MIN/MAX dictionary or null
+};
+// Rewrites `cols` (rank columns appended) and `aggs` for the STR rules above;
+// returns whether any applied.
+bool str_agg_prepare(qe_gpu_ctx* ctx, std::vector<const qe_dev_col*>& cols, const uint32_t* keys, uint32_t nkeys,
+                     std::vector<qe_agg_spec>& aggs, StrAggFix& fix, OwnedCols& own);
+void str_agg_finish(qe_gpu_ctx* ctx, qe_dev_table* t, const StrAggFix& fix);
+// The group-by proper over prepared columns (no STR rewriting).
+qe_gpu_status aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, uint32_t ncols, const uint32_t* key_cols,
+                              uint32_t nkeys, const qe_agg_spec* aggs, uint32_t naggs, qe_dev_table** out);
+qe_dev_col* gather_col(qe_gpu_ctx* ctx, const qe_dev_col* src, const uint32_t* idx, uint64_t m);
 // Device-only variant (no allocation, no host read; safe under graph
 // capture): `sums` holds scan_sums_words(n) words.
 uint64_t scan_sums_words(uint64_t n);
@@ -283,7 +348,6 @@ void radix_sort_pairs(qe_gpu_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t 
 uint64_t common_len(const qe_dev_col* const* cols, uint32_t n);
 
 // Gather rows idx[0, m) of src into a new column (idx == kNoRow gives NULL).
-qe_dev_col* gather_col(qe_gpu_ctx* ctx, const qe_dev_col* src, const uint32_t* idx, uint64_t m);
 
 // Result column construction helpers (host-side allocation of result tables).
 qe_dev_table* new_table(uint64_t rows);

@@ -345,6 +345,7 @@ __global__ void gather_kernel(const T* src, const uint64_t* src_valid, const uin
 
 qe_dev_col* gather_col(qe_gpu_ctx* ctx, const qe_dev_col* src, const uint32_t* idx, uint64_t m) {
     qe_dev_col* out = new_col(ctx, src->type, m, true);
+    out->dict = src->dict;
     Scratch vb(ctx, std::max<uint64_t>(m, 1));
     const uint64_t* sv = src->all_valid ? nullptr : src->validity;
     const unsigned g = grid_for(m, 256, ctx->num_cus * 16);
@@ -871,7 +872,11 @@ qe_gpu_status qe_gpu_hash_join(qe_gpu_ctx* ctx, const qe_dev_col* const* probe_k
                                qe_dev_pairs** out) {
     QE_ENTRY(ctx, {
         invalid(out != nullptr && probe_keys != nullptr && build_keys != nullptr, "null argument");
-        *out = join_pairs(ctx, probe_keys, build_keys, nkeys, join_type);
+        check_key_types(probe_keys, build_keys, nkeys);
+        OwnedCols own(ctx);  // STR keys compare by value (strings.hip)
+        std::vector<const qe_dev_col*> pk, bk;
+        str_join_keys(ctx, probe_keys, build_keys, nkeys, pk, bk, own);
+        *out = join_pairs(ctx, pk.data(), bk.data(), nkeys, join_type);
     });
 }
 
@@ -897,11 +902,16 @@ qe_gpu_status qe_gpu_pairs_gather(qe_gpu_ctx* ctx, const qe_dev_pairs* p, uint64
     });
 }
 
-qe_gpu_status qe_gpu_join_aggregate(qe_gpu_ctx* ctx, const qe_dev_col* const* probe_cols, uint32_t nprobe,
-                                    const qe_dev_col* const* build_cols, uint32_t nbuild,
-                                    const uint32_t* probe_keys, const uint32_t* build_keys, uint32_t nkeys,
-                                    int32_t join_type, const uint32_t* group_keys, uint32_t ngk,
-                                    const qe_agg_spec* aggs, uint32_t naggs, qe_dev_table** out) {
+}  // extern "C"
+
+namespace qe_gpu {
+
+// The fused join -> group-by over prepared columns (no STR rewriting).
+qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* probe_cols, uint32_t nprobe,
+                                   const qe_dev_col* const* build_cols, uint32_t nbuild, const uint32_t* probe_keys,
+                                   const uint32_t* build_keys, uint32_t nkeys, int32_t join_type,
+                                   const uint32_t* group_keys, uint32_t ngk, const qe_agg_spec* aggs, uint32_t naggs,
+                                   qe_dev_table** out) {
     QE_ENTRY(ctx, {
         invalid(out != nullptr, "null out");
         invalid(nprobe >= 1 && nbuild >= 1, "join inputs need >=1 column each");
@@ -959,7 +969,7 @@ qe_gpu_status qe_gpu_join_aggregate(qe_gpu_ctx* ctx, const qe_dev_col* const* pr
                 qe_dev_col* any = owned.empty() ? nullptr : owned[0];
                 for (uint32_t c = 0; c < ncols; ++c)
                     if (!gathered[c]) gathered[c] = any;
-                const qe_gpu_status st = qe_gpu_aggregate(ctx, gathered.data(), ncols, group_keys, ngk, aggs, naggs, &res);
+                const qe_gpu_status st = aggregate_codes(ctx, gathered.data(), ncols, group_keys, ngk, aggs, naggs, &res);
                 if (st != QE_GPU_OK) throw Error{st, ctx->last_error};
             } catch (...) {
                 for (auto* c : owned) free_col(ctx, c);
@@ -1133,4 +1143,106 @@ qe_gpu_status qe_gpu_join_aggregate(qe_gpu_ctx* ctx, const qe_dev_col* const* pr
     });
 }
 
-}  // extern "C"
+}  // namespace qe_gpu
+
+extern "C" qe_gpu_status qe_gpu_join_aggregate(qe_gpu_ctx* ctx, const qe_dev_col* const* probe_cols, uint32_t nprobe,
+                                               const qe_dev_col* const* build_cols, uint32_t nbuild,
+                                               const uint32_t* probe_keys, const uint32_t* build_keys,
+                                               uint32_t nkeys, int32_t join_type, const uint32_t* group_keys,
+                                               uint32_t ngk, const qe_agg_spec* aggs, uint32_t naggs,
+                                               qe_dev_table** out) {
+    // STR join keys, group keys and MIN/MAX inputs compare by value
+    // (strings.hip): the rewritten columns are appended to their own side, and
+    // the indices of the join output (probe columns ++ build columns) follow.
+    QE_ENTRY(ctx, {
+        invalid(out != nullptr && probe_cols != nullptr && build_cols != nullptr, "null argument");
+        invalid(nprobe >= 1 && nbuild >= 1, "join inputs need >=1 column each");
+        invalid(nkeys >= 1 && probe_keys != nullptr && build_keys != nullptr, "equi-join needs >=1 key");
+        invalid(group_keys != nullptr || ngk == 0, "null group key list");
+        invalid(aggs != nullptr || naggs == 0, "null aggregate list");
+        const uint32_t ncols = nprobe + nbuild;
+        auto col_at = [&](uint32_t c) { return c < nprobe ? probe_cols[c] : build_cols[c - nprobe]; };
+        std::vector<const qe_dev_col*> P(probe_cols, probe_cols + nprobe), B(build_cols, build_cols + nbuild);
+        OwnedCols own(ctx);
+        StrAggFix fix;
+        fix.key_dict.assign(ngk, nullptr);
+        fix.rank_dict.assign(naggs, nullptr);
+        bool any = false;
+        for (uint32_t k = 0; k < ngk; ++k) {
+            invalid(group_keys[k] < ncols, "group key column out of range");
+            const qe_dev_col* c = col_at(group_keys[k]);
+            if (c->type != QE_STR) continue;
+            const qe_gpu_dict* d = need_dict(c);
+            fix.key_dict[k] = d;
+            any = true;
+            if (d->unique) continue;
+            qe_dev_col* r = own.keep(remap_col(ctx, c, d->canon, QE_STR, d));
+            if (group_keys[k] < nprobe) P[group_keys[k]] = r;
+            else B[group_keys[k] - nprobe] = r;
+        }
+        // Appended rank columns: (side, index within that side).
+        std::vector<std::pair<int, uint32_t>> rank_at(naggs, {-1, 0});
+        for (uint32_t a = 0; a < naggs; ++a) {
+            if (aggs[a].func != QE_AGG_MIN && aggs[a].func != QE_AGG_MAX) continue;
+            invalid(aggs[a].input_col < ncols, "aggregate input column out of range");
+            const qe_dev_col* c = col_at(aggs[a].input_col);
+            if (c->type != QE_STR) continue;
+            const qe_gpu_dict* d = need_dict(c);
+            fix.rank_dict[a] = d;
+            any = true;
+            qe_dev_col* r = own.keep(remap_col(ctx, c, d->rank, QE_I32, nullptr));
+            if (aggs[a].input_col < nprobe) {
+                P.push_back(r);
+                rank_at[a] = {0, static_cast<uint32_t>(P.size() - 1)};
+            } else {
+                B.push_back(r);
+                rank_at[a] = {1, static_cast<uint32_t>(B.size() - 1)};
+            }
+        }
+        std::vector<uint32_t> pk2(probe_keys, probe_keys + nkeys), bk2(build_keys, build_keys + nkeys);
+        {
+            std::vector<const qe_dev_col*> pkc(nkeys), bkc(nkeys), pko, bko;
+            for (uint32_t j = 0; j < nkeys; ++j) {
+                invalid(probe_keys[j] < nprobe && build_keys[j] < nbuild, "join key column out of range");
+                pkc[j] = probe_cols[probe_keys[j]];
+                bkc[j] = build_cols[build_keys[j]];
+            }
+            check_key_types(pkc.data(), bkc.data(), nkeys);
+            str_join_keys(ctx, pkc.data(), bkc.data(), nkeys, pko, bko, own);
+            for (uint32_t j = 0; j < nkeys; ++j) {
+                if (pko[j] != pkc[j]) {
+                    P.push_back(pko[j]);
+                    pk2[j] = static_cast<uint32_t>(P.size() - 1);
+                }
+                if (bko[j] != bkc[j]) {
+                    B.push_back(bko[j]);
+                    bk2[j] = static_cast<uint32_t>(B.size() - 1);
+                }
+            }
+        }
+        const uint32_t np2 = static_cast<uint32_t>(P.size());
+        auto remap_index = [&](uint32_t c) { return c < nprobe ? c : c - nprobe + np2; };
+        std::vector<uint32_t> gk2(ngk);
+        for (uint32_t k = 0; k < ngk; ++k) gk2[k] = remap_index(group_keys[k]);
+        std::vector<qe_agg_spec> a2(aggs, aggs + naggs);
+        for (uint32_t a = 0; a < naggs; ++a) {
+            if (rank_at[a].first == 0) a2[a].input_col = rank_at[a].second;
+            else if (rank_at[a].first == 1) a2[a].input_col = np2 + rank_at[a].second;
+            else if (a2[a].func != QE_AGG_COUNT_STAR && a2[a].input_col < ncols) a2[a].input_col = remap_index(a2[a].input_col);
+        }
+        qe_dev_table* t = nullptr;
+        const qe_gpu_status st = join_aggregate_codes(ctx, P.data(), np2, B.data(), static_cast<uint32_t>(B.size()),
+                                                      pk2.data(), bk2.data(), nkeys, join_type, gk2.data(), ngk,
+                                                      a2.data(), naggs, &t);
+        if (st != QE_GPU_OK) throw Error{st, ctx->last_error};
+        if (any) {
+            try {
+                str_agg_finish(ctx, t, fix);
+            } catch (...) {
+                qe_gpu_table_free(ctx, t);
+                throw;
+            }
+        }
+        *out = t;
+    });
+}

@@ -182,8 +182,15 @@ __global__ void encode_key_kernel(ColView c, const uint32_t* perm, uint64_t n, i
         const uint32_t r = perm[i];
         uint64_t u = 0;
         if (is_valid(c, r)) {
-            const int64_t v = int_value(c, r);
-            u = static_cast<uint64_t>(v) ^ 0x8000000000000000ull;
+            if (c.type == QE_F64) {
+                // NaN-greatest total order with all NaNs tied and -0 == +0
+                // (ops/sort_internal.h:100-104).
+                double d = f64_value(c, r);
+                if (d == 0.0) d = 0.0;
+                u = f64_order_key(d);
+            } else {
+                u = static_cast<uint64_t>(int_value(c, r)) ^ 0x8000000000000000ull;
+            }
             if (desc) u = ~u;
             u -= base;
         }
@@ -200,21 +207,22 @@ __global__ void null_flag_kernel(ColView c, const uint32_t* perm, uint64_t n, in
 
 }  // namespace qe_gpu
 
-// Stable sort permutation over integer-family keys (I32/I64/TS/BOOL), keys[0]
-// most significant, per-key ASC/DESC and NULLS FIRST/LAST (null placement is
-// absolute, ops/sort.h:35-40). Replaces Sort::build_perm_radix
+// Stable sort permutation, keys[0] most significant, per-key ASC/DESC and NULLS
+// FIRST/LAST (null placement is absolute, ops/sort.h:35-40). Integer-family
+// keys take the radix encoding directly; F64 keys use the NaN-greatest total
+// order of the comparison path (ops/sort.cpp:66-100); STR keys sort by value
+// through their dictionary's rank table (ops/sort.cpp:82-86). Replaces Sort::build_perm_radix
 // (ops/sort.cpp:125-188): LSD over keys from the last to the first, each key
 // a value pass then a NULL-placement pass, all stable.
-extern "C" qe_gpu_status qe_gpu_sort_perm(qe_gpu_ctx* ctx, const qe_dev_col* const* keys, uint32_t nkeys,
+extern "C" qe_gpu_status qe_gpu_sort_perm(qe_gpu_ctx* ctx, const qe_dev_col* const* keys_in, uint32_t nkeys,
                                           const int32_t* desc, const int32_t* nulls_first, uint32_t* perm_out) {
     QE_ENTRY(ctx, {
-        invalid(nkeys >= 1 && keys != nullptr && perm_out != nullptr, "sort needs >=1 key");
-        const uint64_t n = common_len(keys, nkeys);
-        for (uint32_t j = 0; j < nkeys; ++j) {
-            const int t = keys[j]->type;
-            check(t == QE_I32 || t == QE_I64 || t == QE_TS || t == QE_BOOL, QE_GPU_UNSUPPORTED,
-                  "device sort supports integer-family keys");
-        }
+        invalid(nkeys >= 1 && keys_in != nullptr && perm_out != nullptr, "sort needs >=1 key");
+        const uint64_t n = common_len(keys_in, nkeys);
+        OwnedCols own(ctx);
+        std::vector<const qe_dev_col*> keys(keys_in, keys_in + nkeys);
+        for (uint32_t j = 0; j < nkeys; ++j)
+            if (keys[j]->type == QE_STR) keys[j] = own.keep(remap_col(ctx, keys[j], need_dict(keys[j])->rank, QE_I32, nullptr));
         check(n < kNoRow, QE_GPU_UNSUPPORTED, "row indices are 32-bit");
         if (n == 0) return QE_GPU_OK;
         const unsigned g = grid_for(n, 256, ctx->num_cus * 16);
@@ -222,9 +230,10 @@ extern "C" qe_gpu_status qe_gpu_sort_perm(qe_gpu_ctx* ctx, const qe_dev_col* con
         Scratch k64(ctx, n * 8);
         for (uint32_t jj = nkeys; jj-- > 0;) {
             const qe_dev_col* c = keys[jj];
-            const MinMax mm = column_minmax(ctx, c, n);
+            const bool fkey = c->type == QE_F64;
+            const MinMax mm = fkey ? MinMax{} : column_minmax(ctx, c, n);
             const int d = desc ? desc[jj] : 0;
-            uint64_t lo = 0, hi = 0;
+            uint64_t lo = 0, hi = fkey ? ~uint64_t(0) : 0;
             if (mm.valid) {
                 uint64_t a = static_cast<uint64_t>(mm.mn) ^ 0x8000000000000000ull;
                 uint64_t b = static_cast<uint64_t>(mm.mx) ^ 0x8000000000000000ull;
@@ -240,7 +249,7 @@ extern "C" qe_gpu_status qe_gpu_sort_perm(qe_gpu_ctx* ctx, const qe_dev_col* con
             encode_key_kernel<<<g, 256, 0, ctx->stream>>>(view(c), perm_out, n, d, lo, k64.as<uint64_t>());
             QE_HIP(hipGetLastError());
             radix_sort_pairs(ctx, k64.as<uint64_t>(), perm_out, n, 0, (bits + 7) / 8 * 8);
-            if (!c->all_valid && mm.valid < n) {
+            if (!c->all_valid && (fkey || mm.valid < n)) {
                 null_flag_kernel<<<g, 256, 0, ctx->stream>>>(view(c), perm_out, n, nulls_first ? nulls_first[jj] : 0,
                                                              k64.as<uint64_t>());
                 QE_HIP(hipGetLastError());

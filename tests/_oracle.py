@@ -2,7 +2,9 @@
 was built, the compiled reference harness (oracle/_ref/libqe_ref.so).
 
 Test infrastructure: the checker the GPU path is compared against. Columns are
-passed as (values ndarray, valid ndarray|None, type) triples.
+passed as (values ndarray, valid ndarray|None, type) triples; a STR column is a
+4-tuple whose last item is its dictionary (a list of strings the int32 codes
+index).
 """
 from __future__ import annotations
 
@@ -24,7 +26,17 @@ NO_ROW = 0xFFFFFFFF
 
 class OrcCol(C.Structure):
     _fields_ = [("type", C.c_int32), ("all_valid", C.c_int32), ("len", C.c_uint64),
-                ("data", C.c_void_p), ("validity", C.c_void_p)]
+                ("data", C.c_void_p), ("validity", C.c_void_p), ("dict_bytes", C.c_void_p),
+                ("dict_off", C.c_void_p), ("dict_n", C.c_uint64)]
+
+
+def dict_blob(strings):
+    """(bytes ndarray, offsets ndarray) in StringDict's layout."""
+    enc = [x.encode() for x in strings]
+    off = np.zeros(len(enc) + 1, dtype=np.uint32)
+    off[1:] = np.cumsum([len(e) for e in enc]) if enc else []
+    blob = np.frombuffer(b"".join(enc) + b"\0", dtype=np.uint8).copy()
+    return blob, off
 
 
 P = C.c_void_p
@@ -46,12 +58,20 @@ class Cols:
     def __init__(self, cols):
         self.keep = []
         arr = (OrcCol * max(len(cols), 1))()
-        for i, (vals, valid, t) in enumerate(cols):
+        for i, col in enumerate(cols):
+            vals, valid, t = col[:3]
             v = np.ascontiguousarray(vals, dtype=NP[t])
             w = None if valid is None else _pack(np.asarray(valid, dtype=bool))
             self.keep += [v, w]
+            db = do = None
+            dn = 0
+            if t == STR and len(col) > 3:
+                db, do = dict_blob(col[3])
+                dn = len(col[3])
+                self.keep += [db, do]
             arr[i] = OrcCol(t, 1 if valid is None else 0, len(v), v.ctypes.data,
-                            None if w is None else w.ctypes.data)
+                            None if w is None else w.ctypes.data, None if db is None else db.ctypes.data,
+                            None if do is None else do.ctypes.data, dn)
         self.arr = arr
         self.n = len(cols[0][0]) if cols else 0
 
@@ -123,6 +143,8 @@ def ref() -> C.CDLL:
             "ref_rs_type": (C.c_int32, [P, C.c_uint32]),
             "ref_rs_col": (None, [P, C.c_uint32, P, P, P]),
             "ref_rs_free": (None, [P]),
+            "ref_rs_str_bytes": (C.c_uint64, [P, C.c_uint32]),
+            "ref_rs_str_get": (None, [P, C.c_uint32, P, P]),
             "ref_agg_run": (P, [P, C.c_uint32, C.c_uint64, P, C.c_uint32, P, P, C.c_uint32, C.c_uint64]),
             "ref_join_run": (P, [P, C.c_uint32, C.c_uint64, P, C.c_uint32, C.c_uint64, P, P, C.c_uint32,
                                  C.c_int32, C.c_uint64]),
@@ -204,16 +226,29 @@ def _decode_key(word, t):
 
 
 def aggregate(cols, keys, aggs, sum_mode=1, mutant=0):
-    """Group-by rows [(key..., agg...)] in first-seen order; None for NULL."""
+    """Group-by rows [(key..., agg...)] in first-seen order; None for NULL; STR
+    keys and STR MIN/MAX as strings."""
     L = oracle()
     cs = Cols(cols)
     funcs = _i32([f for f, _ in aggs])
     ins = _u32([c for _, c in aggs])
     h = L.orc_agg_run(cs.arr, cs.n, _u32(keys), len(keys), funcs, ins, len(aggs), sum_mode, mutant)
     try:
-        return _agg_rows(L, h, [cols[k][2] for k in keys], len(aggs))
+        rows = _agg_rows(L, h, [cols[k][2] for k in keys], len(aggs))
     finally:
         L.orc_agg_free(h)
+    return _decode_str(rows, [cols[k] for k in keys], [(f, cols[c] if f not in (COUNT_STAR, COUNT) else None)
+                                                       for f, c in aggs])
+
+
+def _decode_str(rows, key_cols, agg_inputs):
+    """Codes -> strings for STR key columns and STR MIN/MAX outputs."""
+    dicts = [c[3] if (c[2] == STR and len(c) > 3) else None for c in key_cols]
+    dicts += [c[3] if (c is not None and f in (MIN, MAX) and c[2] == STR and len(c) > 3) else None
+              for f, c in agg_inputs]
+    if not any(d is not None for d in dicts):
+        return rows
+    return [tuple(v if (d is None or v is None) else d[v] for v, d in zip(r, dicts)) for r in rows]
 
 
 def _agg_rows(L, h, key_types, naggs):
@@ -261,9 +296,11 @@ def join_aggregate(probe_cols, build_cols, pkeys, bkeys, join_type, gkeys, aggs,
                            _u32(bkeys), len(pkeys), join_type, _u32(gkeys), len(gkeys),
                            _i32([f for f, _ in aggs]), _u32([c for _, c in aggs]), len(aggs), sum_mode, mutant)
     try:
-        return _agg_rows(L, h, [allcols[k][2] for k in gkeys], len(aggs))
+        rows = _agg_rows(L, h, [allcols[k][2] for k in gkeys], len(aggs))
     finally:
         L.orc_agg_free(h)
+    return _decode_str(rows, [allcols[k] for k in gkeys],
+                       [(f, allcols[c] if f not in (COUNT_STAR, COUNT) else None) for f, c in aggs])
 
 
 def asof_pairs(probe_cols, build_cols, pkeys, bkeys, ptime, btime, join_type=0, tolerance=None, mutant=0):
@@ -324,7 +361,16 @@ def _ref_rows(h):
             dv = np.empty(max(n, 1), dtype=np.float64)
             nl = np.empty(max(n, 1), dtype=np.uint8)
             L.ref_rs_col(h, c, iv.ctypes.data, dv.ctypes.data, nl.ctypes.data)
-            fl = L.ref_rs_type(h, c) == F64
+            ty = L.ref_rs_type(h, c)
+            if ty == STR:
+                nb = int(L.ref_rs_str_bytes(h, c))
+                blob = np.empty(max(nb, 1), dtype=np.uint8)
+                offs = np.empty(n + 1, dtype=np.uint64)
+                L.ref_rs_str_get(h, c, blob.ctypes.data, offs.ctypes.data)
+                raw = blob.tobytes()
+                cols.append([None if nl[i] else raw[offs[i]:offs[i + 1]].decode() for i in range(n)])
+                continue
+            fl = ty == F64
             cols.append([None if nl[i] else (float(dv[i]) if fl else int(iv[i])) for i in range(n)])
         return [tuple(col[i] for col in cols) for i in range(n)]
     finally:

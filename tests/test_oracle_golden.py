@@ -706,3 +706,130 @@ def test_filter_mutants_are_caught():
     assert not np.array_equal(O.filter_select(t, [(0, GE, I32, 100)], mutant=4), clean)   # inverted
     x = [col([1, 2, 3], I32)]
     assert O.filter_select(x, [(0, LT, F64, 2.5)], mutant=3).tolist() != [0, 1]          # no promotion
+
+
+# ---- dictionary-encoded VARCHAR (STR) ----------------------------------------------------------
+STR = O.STR
+ALPHA = ["AAPL", "MSFT", "GOOG", "AMZN", "TSLA", "NVDA", "META", "x"]  # wp7b_strings_differential_test.cpp:69-72
+
+
+def str_col(vals, nulls_at=(), dict_order=None):
+    """A STR column from values; each column gets its own dictionary (in
+    `dict_order`, default first-seen), so equal strings carry different codes
+    across columns (the cross-dictionary case)."""
+    d = list(dict_order) if dict_order is not None else list(dict.fromkeys(v for v in vals if v is not None))
+    idx = {s: i for i, s in enumerate(d)}
+    codes = np.array([idx[v] if v is not None else 0 for v in vals], dtype=np.int32)
+    valid = None
+    if nulls_at or any(v is None for v in vals):
+        valid = np.array([v is not None for v in vals], dtype=bool)
+        valid[list(nulls_at)] = False
+    return (codes, valid, STR, d)
+
+
+def rand_str_col(r, n, null_pct):
+    """gen_string_column's shape: values from ALPHA, NULLs, a shuffled dictionary."""
+    d = list(r.permutation(ALPHA))
+    vals = [None if r.random() * 100 < null_pct else str(r.choice(ALPHA)) for _ in range(n)]
+    return str_col(vals, dict_order=d)
+
+
+def test_str_join_literals_cross_dictionary():
+    # wp7b_strings_differential_test.cpp:249-268: build "AAPL" is code 0, probe's is code 1.
+    b = [str_col(["AAPL", "MSFT"]), col([10, 20], I64)]
+    p = [str_col(["ZZZ", "AAPL"])]
+    assert pairs(*O.join_pairs([p[0]], [b[0]])) == [(1, 0)]
+    # catalog_checks_strings.cpp:34-70: the same values interned in other orders.
+    b = [str_col(["AAPL", "MSFT", "GOOG"])]
+    p = [str_col(["GOOG", "MSFT", "AAPL", "ZZZ"])]
+    assert pairs(*O.join_pairs(p, b)) == [(0, 2), (1, 1), (2, 0)]
+    # the by-code mutant (string_key_mutants kHashRawCode) pairs codes, not values
+    assert pairs(*O.join_pairs(p, b, mutant=4)) != [(0, 2), (1, 1), (2, 0)]
+    # NULL STR keys never join
+    b = [str_col(["A", None, "B"])]
+    p = [str_col([None, "B", "A"])]
+    assert pairs(*O.join_pairs(p, b)) == [(1, 2), (2, 0)]
+
+
+def _py_str_groupby(vals, payload):
+    """Independent model: dict keyed by the string value (None groups)."""
+    out = {}
+    for k, v in zip(vals, payload):
+        g = out.setdefault(k, [0, 0, None, None])
+        g[0] += 1
+        if v is not None:
+            g[1] += 1
+            g[2] = v if g[2] is None or v < g[2] else g[2]
+            g[3] = v if g[3] is None or v > g[3] else g[3]
+    return [(k, *g) for k, g in out.items()]
+
+
+def _strs(c):
+    codes, valid, _, d = c
+    return [None if (valid is not None and not valid[i]) else d[codes[i]] for i in range(len(codes))]
+
+
+@pytest.mark.parametrize("case", range(8))
+def test_str_groupby_vs_python_model(case):
+    r = np.random.default_rng(1500 + case)
+    n = int(r.integers(1, 1200))
+    k, v = rand_str_col(r, n, 15), rand_str_col(r, n, 15)
+    got = O.aggregate([k, v], [0], [(CS, 0), (CNT, 1), (MIN, 1), (MAX, 1)])
+    assert canon(got) == canon(_py_str_groupby(_strs(k), _strs(v)))
+
+
+def test_str_groupby_duplicate_dictionary_values_and_mutant():
+    # A dictionary holding one value under two codes still groups by value.
+    c = (np.array([0, 1, 2, 0, 2], dtype=np.int32), None, STR, ["A", "B", "A"])
+    got = O.aggregate([c], [0], [(CS, 0)])
+    assert canon(got) == [("A", 4), ("B", 1)]
+    assert canon(O.aggregate([c], [0], [(CS, 0)], mutant=5)) != [("A", 4), ("B", 1)]
+
+
+@needs_ref
+@pytest.mark.parametrize("case", range(10))
+def test_str_groupby_vs_compiled_reference(case):
+    # wp7b_strings_differential_test.cpp:132-165: GROUP BY a STR key with
+    # COUNT/MIN/MAX over a STR payload; composite STR + I64 keys.
+    r = np.random.default_rng(1600 + case)
+    n = int(r.integers(1, 1200))
+    cols = [rand_str_col(r, n, 15), rand_str_col(r, n, 15), col(r.integers(0, 1000, n), I64)]
+    if case % 2:
+        keys, aggs = [0, 2], [(CS, 0), (MAX, 0)]
+    else:
+        keys, aggs = [0], [(CS, 0), (CNT, 1), (MIN, 1), (MAX, 1)]
+    got = O.aggregate(cols, keys, aggs)
+    want = O.ref_aggregate(cols, keys, aggs, batch_size=[64, 256, 2048][case % 3])
+    assert diff(got, want, rel=0, abs_=0) is None
+
+
+@needs_ref
+@pytest.mark.parametrize("case", range(10))
+def test_str_join_vs_compiled_reference(case):
+    # wp7b_strings_differential_test.cpp:167-200: independent dictionaries on
+    # each side, inner and left.
+    r = np.random.default_rng(1700 + case)
+    nb, np_ = int(r.integers(0, 300)), int(r.integers(1, 300))
+    b = [rand_str_col(r, nb, 12), col(np.arange(nb), I64)]
+    p = [rand_str_col(r, np_, 12), col(1000 + np.arange(np_), I64)]
+    jt = case % 2
+    got = pairs(*O.join_pairs([p[0]], [b[0]], jt))
+    want = pairs(*O.ref_join_pairs(p, b, [0], [0], jt, batch_size=[64, 2048][case % 2]))
+    assert got == want
+
+
+@needs_ref
+def test_str_join_aggregate_vs_compiled_reference():
+    # The tick pipeline's tail: trades join a symbol table on an I32 id, grouped
+    # by the symbol's dictionary-encoded name.
+    r = np.random.default_rng(1800)
+    nsym = 300
+    names = [f"S{i:03d}{'X' * (i % 7)}" for i in range(nsym)]
+    order = list(r.permutation(nsym))
+    sym_tab = [col(np.arange(nsym), I32), str_col([names[i] for i in order], dict_order=list(r.permutation(names)))]
+    n = 20000
+    trades = [col(r.integers(0, nsym + 20, n), I32), col(np.round(r.uniform(0, 200, n), 2), F64)]
+    aggs = [(CS, 0), (AVG, 1), (MIN, 3), (MAX, 3)]
+    got = O.join_aggregate(trades, sym_tab, [0], [0], 0, [3], aggs, sum_mode=0)
+    want = O.ref_join_aggregate(trades, sym_tab, [0], [0], 0, [3], aggs)
+    assert diff(got, want, rel=1e-12, abs_=0) is None
