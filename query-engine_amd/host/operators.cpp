@@ -24,8 +24,25 @@ void OwnedColumn::set_null(size_t i) {
     all_valid_ = false;
 }
 
+OwnedColumn OwnedColumn::make_str(size_t len, std::shared_ptr<const StringDict> dict) {
+    OwnedColumn c = make(Type::STR, len);
+    c.dict_ = std::move(dict);
+    return c;
+}
+
 Column OwnedColumn::view() const {
-    return Column{type_, len_, data_.data(), all_valid_ ? nullptr : validity_.data(), all_valid_, nullptr};
+    return Column{type_, len_, data_.data(), all_valid_ ? nullptr : validity_.data(), all_valid_, dict_.get()};
+}
+
+int32_t StringDict::intern(std::string_view s) {
+    std::string key(s);
+    auto it = index_.find(key);
+    if (it != index_.end()) return it->second;
+    const auto code = static_cast<int32_t>(offsets_.size() - 1);
+    bytes_.insert(bytes_.end(), s.begin(), s.end());
+    offsets_.push_back(static_cast<uint32_t>(bytes_.size()));
+    index_.emplace(std::move(key), code);
+    return code;
 }
 
 Batch OwnedBatch::view() const {
@@ -75,16 +92,69 @@ qe_gpu_ctx* device_context() {
         if (st_ != QE_GPU_OK) throw_status(st_, device_context()); \
     } while (0)
 
-DeviceColumns::~DeviceColumns() {
-    if (!owned) return;
-    for (auto* c : cols) qe_gpu_column_free(device_context(), c);
+// Device dictionary -> the host dictionary it was made from (kept alive while
+// registered), so STR results come back with a host dictionary.
+namespace {
+std::mutex g_dict_mu;
+std::map<const qe_gpu_dict*, std::shared_ptr<const StringDict>> g_dicts;
+
+qe_gpu_dict* register_dict(std::shared_ptr<const StringDict> host) {
+    std::vector<char> bytes;
+    std::vector<uint32_t> off{0};
+    for (size_t i = 0; i < host->size(); ++i) {
+        const std::string_view v = host->at(static_cast<int32_t>(i));
+        bytes.insert(bytes.end(), v.begin(), v.end());
+        off.push_back(static_cast<uint32_t>(bytes.size()));
+    }
+    qe_gpu_dict* d = nullptr;
+    QE_CALL(qe_gpu_dict_create(device_context(), bytes.data(), off.data(), static_cast<uint32_t>(host->size()), &d));
+    std::lock_guard<std::mutex> g(g_dict_mu);
+    g_dicts[d] = std::move(host);
+    return d;
 }
 
-static qe_dev_col* upload_view(const Column& c) {
+void unregister_dict(qe_gpu_dict* d) {
+    {
+        std::lock_guard<std::mutex> g(g_dict_mu);
+        g_dicts.erase(d);
+    }
+    qe_gpu_dict_free(device_context(), d);
+}
+}  // namespace
+
+std::shared_ptr<const StringDict> host_dict_of(const qe_dev_col* c) {
+    const qe_gpu_dict* d = qe_gpu_column_dict(c);
+    std::lock_guard<std::mutex> g(g_dict_mu);
+    auto it = g_dicts.find(d);
+    if (it == g_dicts.end()) throw std::logic_error("device STR column without a registered host dictionary");
+    return it->second;
+}
+
+DeviceColumns::~DeviceColumns() {
+    if (owned)
+        for (auto* c : cols) qe_gpu_column_free(device_context(), c);
+    for (auto* d : dicts) unregister_dict(d);
+}
+
+// One column to HBM; a STR column's dictionary goes with it (one device copy
+// per host dictionary per upload).
+static qe_dev_col* upload_view(const Column& c, DeviceColumns& dc, std::shared_ptr<const StringDict> owner,
+                               std::map<const StringDict*, qe_gpu_dict*>& made) {
     qe_col_desc d{static_cast<int32_t>(c.type), c.all_valid ? 1 : 0, c.len, c.data, c.validity};
-    if (c.type == Type::STR) throw std::invalid_argument("STR columns need their dictionary on the host path");
     qe_dev_col* out = nullptr;
     QE_CALL(qe_gpu_column_upload(device_context(), &d, &out));
+    if (c.type == Type::STR) {
+        if (!c.dict || !owner) {
+            qe_gpu_column_free(device_context(), out);
+            throw std::invalid_argument("STR column without a dictionary");
+        }
+        qe_gpu_dict*& gd = made[c.dict];
+        if (!gd) {
+            gd = register_dict(std::move(owner));
+            dc.dicts.push_back(gd);
+        }
+        QE_CALL(qe_gpu_column_set_dict(device_context(), out, gd));
+    }
     return out;
 }
 
@@ -92,7 +162,9 @@ std::shared_ptr<DeviceColumns> upload_table(const Table& t) {
     if (t.device_) return t.device_;
     auto dc = std::make_shared<DeviceColumns>();
     dc->rows = t.num_rows();
-    for (size_t i = 0; i < t.schema().fields.size(); ++i) dc->cols.push_back(upload_view(t.column(i).view()));
+    std::map<const StringDict*, qe_gpu_dict*> made;
+    for (size_t i = 0; i < t.schema().fields.size(); ++i)
+        dc->cols.push_back(upload_view(t.column(i).view(), *dc, t.column(i).dict_shared(), made));
     t.device_ = dc;  // uploaded once per Table (inputs stay resident in HBM)
     return dc;
 }
@@ -106,6 +178,11 @@ std::shared_ptr<DeviceColumns> device_input(Operator& op) {
     std::vector<OwnedColumn> acc;
     std::vector<std::vector<std::byte>> data(s.fields.size());
     std::vector<std::vector<uint8_t>> valid(s.fields.size());
+    // STR values are re-interned into one dictionary per column: batches may
+    // carry different dictionaries (the BuildStore rule, ops/join.cpp:150-172).
+    std::vector<std::shared_ptr<StringDict>> dicts(s.fields.size());
+    for (size_t c = 0; c < s.fields.size(); ++c)
+        if (s.fields[c].second == Type::STR) dicts[c] = std::make_shared<StringDict>();
     size_t rows = 0;
     op.open();
     while (auto b = op.next()) {
@@ -114,9 +191,17 @@ std::shared_ptr<DeviceColumns> device_input(Operator& op) {
             const size_t w = byte_width(col.type);
             for (size_t k = 0; k < b->row_count; ++k) {
                 const size_t p = b->sel ? b->sel->idx[k] : k;
-                const std::byte* src = col.data + p * w;
-                data[c].insert(data[c].end(), src, src + w);
-                valid[c].push_back(col.all_valid || ((col.validity[p >> 6] >> (p & 63)) & 1ull));
+                const bool v = col.all_valid || ((col.validity[p >> 6] >> (p & 63)) & 1ull);
+                if (dicts[c]) {
+                    const int32_t code =
+                        v ? dicts[c]->intern(col.dict->at(reinterpret_cast<const int32_t*>(col.data)[p])) : 0;
+                    const auto* cb = reinterpret_cast<const std::byte*>(&code);
+                    data[c].insert(data[c].end(), cb, cb + 4);
+                } else {
+                    const std::byte* src = col.data + p * w;
+                    data[c].insert(data[c].end(), src, src + w);
+                }
+                valid[c].push_back(v);
             }
         }
         rows += b->row_count;
@@ -124,12 +209,13 @@ std::shared_ptr<DeviceColumns> device_input(Operator& op) {
     op.close();
     auto dc = std::make_shared<DeviceColumns>();
     dc->rows = rows;
+    std::map<const StringDict*, qe_gpu_dict*> made;
     for (size_t c = 0; c < s.fields.size(); ++c) {
-        OwnedColumn oc = OwnedColumn::make(s.fields[c].second, rows);
+        OwnedColumn oc = dicts[c] ? OwnedColumn::make_str(rows, dicts[c]) : OwnedColumn::make(s.fields[c].second, rows);
         if (rows) std::memcpy(oc.mutable_data(), data[c].data(), data[c].size());
         for (size_t r = 0; r < rows; ++r)
             if (!valid[c][r]) oc.set_null(r);
-        dc->cols.push_back(upload_view(oc.view()));
+        dc->cols.push_back(upload_view(oc.view(), *dc, dicts[c], made));
     }
     return dc;
 }
@@ -158,10 +244,12 @@ std::optional<Batch> Scan::next() {
 }
 
 // ---- result emission ----------------------------------------------------------------------
-void ResultEmitter::reset(qe_dev_table* t, std::vector<Type> types) {
+void ResultEmitter::reset(qe_dev_table* t, std::vector<Type> types,
+                          std::vector<std::shared_ptr<const DeviceColumns>> sources) {
     if (table_) qe_gpu_table_free(device_context(), table_);
     table_ = t;
     types_ = std::move(types);
+    sources_ = std::move(sources);
     cursor_ = 0;
 }
 
@@ -178,6 +266,7 @@ std::optional<Batch> ResultEmitter::next() {
     std::vector<uint8_t> vb(m);
     for (uint32_t c = 0; c < qe_gpu_table_cols(table_); ++c) {
         OwnedColumn oc = OwnedColumn::make(types_[c], m);
+        if (types_[c] == Type::STR) oc.set_dict(host_dict_of(qe_gpu_table_col(table_, c)));
         QE_CALL(qe_gpu_column_download_range(device_context(), qe_gpu_table_col(table_, c), cursor_, m,
                                              oc.mutable_data(), vb.data()));
         for (size_t r = 0; r < m; ++r)
@@ -249,7 +338,7 @@ void GpuAggregate::open() {
     QE_CALL(qe_gpu_aggregate(device_context(), in->cols.data(), static_cast<uint32_t>(in->cols.size()),
                              key_cols_.data(), static_cast<uint32_t>(key_cols_.size()), specs.data(),
                              static_cast<uint32_t>(specs.size()), &t));
-    emit_.reset(t, types_of(output_schema()));
+    emit_.reset(t, types_of(output_schema()), {in});
 }
 
 void GpuAggregate::close() { emit_.reset(nullptr, {}); }
@@ -296,6 +385,7 @@ struct TableColumns : DeviceColumns {
 std::shared_ptr<DeviceColumns> GpuHashJoin::device_columns() {
     open();
     auto keep = std::make_shared<TableColumns>();
+    keep->sources = {pcols_, bcols_};
     QE_CALL(qe_gpu_pairs_gather(device_context(), pairs_, 0, qe_gpu_pairs_count(pairs_), pcols_->cols.data(),
                                 static_cast<uint32_t>(pcols_->cols.size()), bcols_->cols.data(),
                                 static_cast<uint32_t>(bcols_->cols.size()), &keep->table));
@@ -376,7 +466,7 @@ void GpuJoinAggregate::open() {
                                   type_ == JoinType::Left ? QE_JOIN_LEFT : QE_JOIN_INNER, group_keys_.data(),
                                   static_cast<uint32_t>(group_keys_.size()), specs.data(),
                                   static_cast<uint32_t>(specs.size()), &t));
-    emit_.reset(t, types_of(output_schema()));
+    emit_.reset(t, types_of(output_schema()), {pc, bc});
 }
 
 void GpuJoinAggregate::close() { emit_.reset(nullptr, {}); }
@@ -474,7 +564,15 @@ void GpuHashTable::run(const KeyColumns& keys, size_t n, uint32_t* out, bool ins
     void* sel = nullptr;
     void* dout = nullptr;
     try {
-        for (size_t j = 0; j < keys.num_cols; ++j) cols.push_back(upload_view(keys.cols[j]));
+        // The HashTable hashes STR keys by raw code (ops/hash_internal.cpp:29-60);
+        // callers canonicalise values first, as Aggregate and HashJoin do.
+        for (size_t j = 0; j < keys.num_cols; ++j) {
+            const Column& c = keys.cols[j];
+            qe_col_desc d{static_cast<int32_t>(c.type), c.all_valid ? 1 : 0, c.len, c.data, c.validity};
+            qe_dev_col* dc = nullptr;
+            QE_CALL(qe_gpu_column_upload(ctx, &d, &dc));
+            cols.push_back(dc);
+        }
         if (keys.sel) {
             QE_CALL(qe_gpu_malloc(ctx, std::max<size_t>(n, 1) * 4, &sel));
             QE_CALL(qe_gpu_memcpy_h2d(ctx, sel, keys.sel->idx, n * 4));
@@ -614,9 +712,9 @@ qe_dev_table* GpuFilter::run() {
         fp.terms[0].lit_is_null = 1;
     }
     if (fp.terms.empty()) return nullptr;
-    auto in = device_input(*child_);
+    in_ = device_input(*child_);
     qe_dev_table* t = nullptr;
-    QE_CALL(qe_gpu_filter(device_context(), in->cols.data(), static_cast<uint32_t>(in->cols.size()),
+    QE_CALL(qe_gpu_filter(device_context(), in_->cols.data(), static_cast<uint32_t>(in_->cols.size()),
                           fp.terms.data(), static_cast<uint32_t>(fp.terms.size()), &t));
     return t;
 }
@@ -639,6 +737,7 @@ void GpuFilter::close() {
     if (pass_through_) child_->close();
     pass_through_ = false;
     emit_.reset(nullptr, {});
+    in_.reset();
 }
 
 std::shared_ptr<DeviceColumns> GpuFilter::device_columns() {
@@ -646,6 +745,8 @@ std::shared_ptr<DeviceColumns> GpuFilter::device_columns() {
     if (!t) return device_input(*child_);
     auto keep = std::make_shared<TableColumns>();
     keep->table = t;
+    keep->sources = {in_};
+    in_.reset();
     keep->rows = qe_gpu_table_rows(t);
     for (uint32_t i = 0; i < qe_gpu_table_cols(t); ++i) {
         qe_dev_col* v = nullptr;

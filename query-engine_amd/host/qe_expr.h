@@ -110,6 +110,7 @@ class GpuFilter : public Operator, public DeviceSource {
     std::unique_ptr<Operator> child_;
     expr::Expr predicate_;
     bool pass_through_ = false;
+    std::shared_ptr<DeviceColumns> in_;  // the filtered input (its dictionaries back STR results)
     ResultEmitter emit_;
 };
 

@@ -16,6 +16,8 @@
 #include <optional>
 #include <stdexcept>
 #include <string>
+#include <string_view>
+#include <unordered_map>
 #include <utility>
 #include <vector>
 
@@ -38,7 +40,24 @@ inline constexpr std::size_t byte_width(Type t) {
     }
 }
 
-class StringDict;
+// The code -> bytes table behind a STR column (core/string_dict.h:34-72):
+// intern() dedups so equal strings share one code within a dictionary; codes
+// are assigned 0, 1, 2, ... in first-seen order. STR values compare by bytes,
+// never by code.
+class StringDict {
+   public:
+    StringDict() { offsets_.push_back(0); }
+    int32_t intern(std::string_view s);
+    std::string_view at(int32_t code) const {
+        return std::string_view(bytes_.data() + offsets_[code], offsets_[code + 1] - offsets_[code]);
+    }
+    size_t size() const noexcept { return offsets_.size() - 1; }
+
+   private:
+    std::vector<char> bytes_;
+    std::vector<uint32_t> offsets_;
+    std::unordered_map<std::string, int32_t> index_;
+};
 
 struct Column {
     Type type;
@@ -78,6 +97,11 @@ class Operator {
 class OwnedColumn {
    public:
     static OwnedColumn make(Type t, size_t len);
+    // A STR column of `len` codes into `dict` (core/owned_batch.h:37-41).
+    static OwnedColumn make_str(size_t len, std::shared_ptr<const StringDict> dict);
+    void set_dict(std::shared_ptr<const StringDict> d) { dict_ = std::move(d); }
+    const StringDict* dict() const noexcept { return dict_.get(); }
+    const std::shared_ptr<const StringDict>& dict_shared() const noexcept { return dict_; }
     Type type() const { return type_; }
     size_t len() const { return len_; }
     std::byte* mutable_data() { return data_.data(); }
@@ -93,6 +117,7 @@ class OwnedColumn {
     std::vector<std::byte> data_;
     std::vector<uint64_t> validity_;
     bool all_valid_ = true;
+    std::shared_ptr<const StringDict> dict_;  // STR only
 };
 
 class OwnedBatch {
@@ -119,6 +144,8 @@ class Table {
 
    private:
     friend std::shared_ptr<DeviceColumns> upload_table(const Table& t);
+// The host dictionary a device STR column's codes index (uploads register it).
+std::shared_ptr<const StringDict> host_dict_of(const qe_dev_col* c);
     friend void drop_device_copy(const Table& t);
     Schema schema_;
     std::vector<OwnedColumn> cols_;
@@ -141,9 +168,16 @@ struct DeviceColumns {
     std::vector<qe_dev_col*> cols;
     size_t rows = 0;
     bool owned = true;
+    // Device copies of the STR columns' dictionaries, registered so that device
+    // results map back to the host dictionary (released with these columns).
+    std::vector<qe_gpu_dict*> dicts;
+    // Inputs whose dictionaries these columns' STR codes still index.
+    std::vector<std::shared_ptr<const DeviceColumns>> sources;
     ~DeviceColumns();
 };
 std::shared_ptr<DeviceColumns> upload_table(const Table& t);
+// The host dictionary a device STR column's codes index (uploads register it).
+std::shared_ptr<const StringDict> host_dict_of(const qe_dev_col* c);
 void drop_device_copy(const Table& t);
 
 // Operators that can hand their whole output to a device parent without a
@@ -197,13 +231,16 @@ enum class AsofMethod { Binary, Learned };
 // Device result drained into host batches of <= 2048 rows.
 class ResultEmitter {
    public:
-    void reset(qe_dev_table* t, std::vector<Type> types);
+    // `sources`: the inputs whose dictionaries the table's STR columns index.
+    void reset(qe_dev_table* t, std::vector<Type> types,
+               std::vector<std::shared_ptr<const DeviceColumns>> sources = {});
     std::optional<Batch> next();
     ~ResultEmitter();
 
    private:
     qe_dev_table* table_ = nullptr;
     std::vector<Type> types_;
+    std::vector<std::shared_ptr<const DeviceColumns>> sources_;
     size_t cursor_ = 0;
     OwnedBatch current_;
 };

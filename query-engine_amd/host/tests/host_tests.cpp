@@ -11,6 +11,7 @@
 #include <cstring>
 #include <functional>
 #include <map>
+#include <optional>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -403,6 +404,122 @@ void test_filter_exceptions() {
     }, true);
 }
 
+// ---- dictionary-encoded VARCHAR ---------------------------------------------------------------
+OwnedColumn str_col(const std::vector<std::optional<std::string>>& vals) {
+    auto dict = std::make_shared<StringDict>();
+    OwnedColumn c = OwnedColumn::make_str(vals.size(), dict);
+    auto* codes = reinterpret_cast<int32_t*>(c.mutable_data());
+    for (size_t i = 0; i < vals.size(); ++i) {
+        if (vals[i]) codes[i] = dict->intern(*vals[i]);
+        else c.set_null(i);
+    }
+    return c;
+}
+
+// Every cell as text ("NULL", integers, doubles with %.6g, strings resolved
+// through the column's dictionary), rows sorted.
+std::vector<std::vector<std::string>> drain_text(Operator& op) {
+    std::vector<std::vector<std::string>> rows;
+    op.open();
+    while (auto b = op.next()) {
+        for (size_t k = 0; k < b->row_count; ++k) {
+            const size_t p = b->sel ? b->sel->idx[k] : k;
+            std::vector<std::string> r;
+            for (const Column& c : b->cols) {
+                const bool valid = c.all_valid || ((c.validity[p >> 6] >> (p & 63)) & 1ull);
+                if (!valid) {
+                    r.push_back("NULL");
+                    continue;
+                }
+                char buf[64];
+                switch (c.type) {
+                    case Type::STR:
+                        r.emplace_back(c.dict->at(reinterpret_cast<const int32_t*>(c.data)[p]));
+                        continue;
+                    case Type::I32:
+                        std::snprintf(buf, sizeof buf, "%d", reinterpret_cast<const int32_t*>(c.data)[p]);
+                        break;
+                    case Type::F64:
+                        std::snprintf(buf, sizeof buf, "%.6g", reinterpret_cast<const double*>(c.data)[p]);
+                        break;
+                    case Type::BOOL:
+                        std::snprintf(buf, sizeof buf, "%d", reinterpret_cast<const uint8_t*>(c.data)[p]);
+                        break;
+                    default:
+                        std::snprintf(buf, sizeof buf, "%lld",
+                                      static_cast<long long>(reinterpret_cast<const int64_t*>(c.data)[p]));
+                }
+                r.emplace_back(buf);
+            }
+            rows.push_back(std::move(r));
+        }
+    }
+    op.close();
+    std::sort(rows.begin(), rows.end());
+    return rows;
+}
+
+void test_str_join_by_value() {
+    // tests/wp7b_strings_differential_test.cpp:249-268: build "AAPL" is code 0,
+    // the probe's "AAPL" is code 1 in its own dictionary.
+    Table build(sch({{"c0", Type::STR}, {"c1", Type::I64}}), [] {
+        std::vector<OwnedColumn> c;
+        c.push_back(str_col({"AAPL", "MSFT"}));
+        c.push_back(mk<int64_t>(Type::I64, {10, 20}));
+        return c;
+    }());
+    Table probe(sch({{"c0", Type::STR}}), [] {
+        std::vector<OwnedColumn> c;
+        c.push_back(str_col({"ZZZ", "AAPL", std::nullopt}));
+        return c;
+    }());
+    GpuHashJoin j(std::make_unique<Scan>(probe), std::make_unique<Scan>(build), {0}, {0}, JoinType::Inner);
+    const auto rows = drain_text(j);
+    CHECK(rows == (std::vector<std::vector<std::string>>{{"AAPL", "AAPL", "10"}}));
+    GpuHashJoin l(std::make_unique<Scan>(probe), std::make_unique<Scan>(build), {0}, {0}, JoinType::Left);
+    const auto lrows = drain_text(l);
+    CHECK(lrows == (std::vector<std::vector<std::string>>{
+                       {"AAPL", "AAPL", "10"}, {"NULL", "NULL", "NULL"}, {"ZZZ", "NULL", "NULL"}}));
+}
+
+void test_str_group_by_name_plan() {
+    // The tick pipeline's tail: trades (sym id, px) join the symbol table on
+    // the id, GROUP BY the dictionary-encoded name with COUNT, AVG(px) and
+    // MIN/MAX(name); two ids share the name "ACME".
+    Table syms(sch({{"id", Type::I32}, {"name", Type::STR}}), [] {
+        std::vector<OwnedColumn> c;
+        c.push_back(mk<int32_t>(Type::I32, {0, 1, 2, 3}));
+        c.push_back(str_col({"ZETA", "ACME", "beta", "ACME"}));
+        return c;
+    }());
+    Table trades(sch({{"sym", Type::I32}, {"px", Type::F64}}), [] {
+        std::vector<OwnedColumn> c;
+        c.push_back(mk<int32_t>(Type::I32, {1, 3, 0, 2, 2, 9, 1}));
+        c.push_back(mk<double>(Type::F64, {10, 20, 30, 40, 50, 60, 70}));
+        return c;
+    }());
+    for (bool fuse : {true, false}) {
+        plan::LowerOptions opt;
+        opt.fuse_join_aggregate = fuse;
+        auto op = plan::scan(trades)
+                      .join(plan::scan(syms), {"sym"}, {"id"}, JoinType::Inner)
+                      .aggregate({"name"}, {AggSpec::count_star("n"), AggSpec::avg(1, "avg_px"),
+                                            AggSpec::min(3, "lo"), AggSpec::max(3, "hi")})
+                      .build(opt);
+        const auto rows = drain_text(*op);
+        CHECK(rows == (std::vector<std::vector<std::string>>{{"ACME", "3", "33.3333", "ACME", "ACME"},
+                                                              {"ZETA", "1", "30", "ZETA", "ZETA"},
+                                                              {"beta", "2", "45", "beta", "beta"}}));
+    }
+    // MIN/MAX over names by byte value: "ACME" < "ZETA" < "beta"
+    auto g = plan::scan(syms).aggregate({}, {AggSpec::min(1, "lo"), AggSpec::max(1, "hi"), AggSpec::count(1, "n")}).build();
+    CHECK(drain_text(*g) == (std::vector<std::vector<std::string>>{{"ACME", "beta", "4"}}));
+    // Filter keeps the STR payload (and its dictionary)
+    using namespace qe::expr;
+    auto f = plan::scan(syms).filter(ge(col(Type::I32, 0), lit(Scalar::i32(2)))).build();
+    CHECK(drain_text(*f) == (std::vector<std::vector<std::string>>{{"2", "beta"}, {"3", "ACME"}}));
+}
+
 int main() {
     const std::vector<std::pair<const char*, std::function<void()>>> tests = {
         {"join_groupby_plan", test_join_groupby_plan},
@@ -414,6 +531,8 @@ int main() {
         {"filter_reference_cases", test_filter_reference_cases},
         {"filter_plan_and_device_chaining", test_filter_plan_and_device_chaining},
         {"filter_exceptions", test_filter_exceptions},
+        {"str_join_by_value", test_str_join_by_value},
+        {"str_group_by_name_plan", test_str_group_by_name_plan},
     };
     for (const auto& [name, fn] : tests) {
         std::printf("[host] %s\n", name);
