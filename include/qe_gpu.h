@@ -267,6 +267,12 @@ void qe_gpu_ht_free(qe_gpu_ctx* ctx, qe_dev_ht* ht);
 qe_gpu_status qe_gpu_sort_perm(qe_gpu_ctx* ctx, const qe_dev_col* const* keys, uint32_t nkeys, const int32_t* desc,
                                const int32_t* nulls_first, uint32_t* perm_out);
 
+/* Rows idx[0..m) of `cols` (device u32 indices, each < the columns' length)
+ * as a new table, validity and dictionaries kept: Sort's whole-row gather by
+ * its permutation (gather_rows, ops/sort_internal.cpp:95-151). */
+qe_gpu_status qe_gpu_take(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, uint32_t ncols, const uint32_t* idx,
+                          uint64_t m, qe_dev_table** out);
+
 /* ---- as-of join -------------------------------------------------------------------*/
 /* Build side of tsx::AsofJoin (tsx/asof.h:101-105; build_side
  * tsx/asof.cpp:177-232): partition by key tuple (NULL keys never match),

@@ -726,6 +726,49 @@ std::vector<uint32_t> sort_perm_impl(const orc_col* cols, const uint32_t* keys,
     return perm;
 }
 
+// The comparison path (ops/sort.cpp:60-100): a stable sort by a row "less"
+// over the keys in order; NULL placement is absolute, F64 uses the
+// NaN-greatest total order with all NaNs tied and -0 == +0
+// (ops/sort_internal.h:100-104), STR compares the strings, BOOL compares 0/1.
+int cmp_key(const orc_col& c, uint64_t a, uint64_t b) {
+    if (c.type == F64) {
+        const double x = f64_at(c, a), y = f64_at(c, b);
+        const bool nx = std::isnan(x), ny = std::isnan(y);
+        if (nx || ny) return nx == ny ? 0 : (nx ? 1 : -1);
+        return x < y ? -1 : (x > y ? 1 : 0);
+    }
+    if (c.type == STR) {
+        const int r = str_of_code(c, int_at(c, a)).compare(str_of_code(c, int_at(c, b)));
+        return r < 0 ? -1 : (r > 0 ? 1 : 0);
+    }
+    int64_t x = int_at(c, a), y = int_at(c, b);
+    if (c.type == BOOL) {
+        x = x != 0;
+        y = y != 0;
+    }
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+std::vector<uint32_t> sort_perm_comparison(const orc_col* cols, const uint32_t* keys, const int32_t* desc,
+                                           const int32_t* nulls_first, uint32_t nk, uint64_t n) {
+    std::vector<uint32_t> perm(n);
+    std::iota(perm.begin(), perm.end(), 0u);
+    std::stable_sort(perm.begin(), perm.end(), [&](uint32_t a, uint32_t b) {
+        for (uint32_t j = 0; j < nk; ++j) {
+            const orc_col& c = cols[keys[j]];
+            const bool av = valid_at(c, a), bv = valid_at(c, b);
+            if (!av || !bv) {
+                if (!av && !bv) continue;
+                return nulls_first[j] ? !av : av;
+            }
+            const int r = cmp_key(c, a, b);
+            if (r != 0) return desc[j] ? r > 0 : r < 0;
+        }
+        return false;
+    });
+    return perm;
+}
+
 // ---------------------------------------------------------------------------
 // Backward as-of join: tsx/asof.cpp:122-308. Both sides stably sorted by
 // (keys ASC NULLS LAST, ts ASC NULLS LAST) (:51-59, :163-166); build rows with
@@ -951,11 +994,13 @@ void* orc_asof_run(const orc_col* pcols, uint64_t np, const orc_col* bcols, uint
 // --- sort ------------------------------------------------------------------
 int orc_sort_perm(const orc_col* cols, const uint32_t* keys, const int32_t* desc,
                   const int32_t* nulls_first, uint32_t nk, uint64_t n, uint32_t* perm) {
+    bool radix = true;  // the reference's kAuto choice (ops/sort.cpp:192-215)
     for (uint32_t j = 0; j < nk; ++j) {
         const int t = cols[keys[j]].type;
-        if (t != I32 && t != I64 && t != TS) return 1;
+        if (t != I32 && t != I64 && t != TS) radix = false;
     }
-    const auto p = sort_perm_impl(cols, keys, desc, nulls_first, nk, n);
+    const auto p = radix ? sort_perm_impl(cols, keys, desc, nulls_first, nk, n)
+                         : sort_perm_comparison(cols, keys, desc, nulls_first, nk, n);
     std::memcpy(perm, p.data(), n * 4);
     return 0;
 }

@@ -28,6 +28,7 @@
 #include "ops/hashtable.h"
 #include "ops/join.h"
 #include "ops/scan.h"
+#include "ops/sort.h"
 #include "ops/table.h"
 #include "tsx/asof.h"
 
@@ -346,6 +347,19 @@ void* ref_filter_run(const orc_col* cols, uint32_t ncols, uint64_t n, const ref_
         pred = j == 0 ? c : expr::logic_and(pred, c);
     }
     Filter op(std::make_unique<Scan>(*t, batch_size), pred);
+    return drain(op);
+}
+
+// Sort(Scan(t), keys) with a row-id column appended: the last output column is
+// the permutation (ops/sort.h:84).
+void* ref_sort_run(const orc_col* cols, uint32_t ncols, uint64_t n, const uint32_t* keys, const int32_t* desc,
+                   const int32_t* nulls_first, uint32_t nk, uint64_t batch_size) {
+    auto t = make_table(cols, ncols, n, true);
+    std::vector<SortKey> sk;
+    for (uint32_t j = 0; j < nk; ++j)
+        sk.push_back(SortKey{keys[j], desc[j] ? SortDir::Desc : SortDir::Asc,
+                             nulls_first[j] ? NullOrder::First : NullOrder::Last});
+    Sort op(std::make_unique<Scan>(*t, batch_size), sk);
     return drain(op);
 }
 

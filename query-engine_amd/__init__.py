@@ -107,6 +107,7 @@ _SIGS = {
     "qe_gpu_column_set_dict": (C.c_int, [P, P, P]),
     "qe_gpu_column_dict": (P, [P]),
     "qe_gpu_column_remap": (C.c_int, [P, P, P, C.c_uint32, C.c_int32, C.POINTER(P)]),
+    "qe_gpu_take": (C.c_int, [P, C.POINTER(P), C.c_uint32, P, C.c_uint64, C.POINTER(P)]),
     "qe_gpu_table_rows": (C.c_uint64, [P]),
     "qe_gpu_table_cols": (C.c_uint32, [P]),
     "qe_gpu_table_col": (P, [P, C.c_uint32]),
@@ -426,6 +427,12 @@ class Context:
         out = P()
         self._check(lib().qe_gpu_column_slice(self.h, col.h, offset, n, C.byref(out)))
         return DevCol(self, out)
+
+    def take(self, cols, idx: "DevBuffer", m: int) -> Table:
+        """Rows idx[0..m) (a device u32 buffer) of `cols` as a new table."""
+        out = P()
+        self._check(lib().qe_gpu_take(self.h, _cols(cols), len(cols), idx.p, m, C.byref(out)))
+        return Table(self, out)
 
     def remap(self, col: DevCol, table, out_type: int = I32) -> DevCol:
         t = np.ascontiguousarray(table, dtype=np.int32)

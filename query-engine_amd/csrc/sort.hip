@@ -205,6 +205,11 @@ __global__ void null_flag_kernel(ColView c, const uint32_t* perm, uint64_t n, in
     }
 }
 
+__global__ void check_index_kernel(const uint32_t* idx, uint64_t m, uint64_t n, unsigned int* bad) {
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < m; i += uint64_t(gridDim.x) * blockDim.x)
+        if (idx[i] >= n) atomicOr(bad, 1u);
+}
+
 }  // namespace qe_gpu
 
 // Stable sort permutation, keys[0] most significant, per-key ASC/DESC and NULLS
@@ -257,5 +262,31 @@ extern "C" qe_gpu_status qe_gpu_sort_perm(qe_gpu_ctx* ctx, const qe_dev_col* con
             }
         }
         QE_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+extern "C" qe_gpu_status qe_gpu_take(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, uint32_t ncols,
+                                     const uint32_t* idx, uint64_t m, qe_dev_table** out) {
+    QE_ENTRY(ctx, {
+        invalid(out != nullptr && cols != nullptr && ncols >= 1, "take needs >=1 column");
+        invalid(idx != nullptr || m == 0, "null index array");
+        const uint64_t n = common_len(cols, ncols);
+        if (m > 0) {  // every index must address a row
+            Scratch bad(ctx, 4);
+            memset_async(ctx, bad.p, 0, 4);
+            check_index_kernel<<<grid_for(m, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(idx, m, n,
+                                                                                             bad.as<unsigned int>());
+            QE_HIP(hipGetLastError());
+            invalid(read_scalar(ctx, bad.as<unsigned int>()) == 0, "take index out of range");
+        }
+        qe_dev_table* t = new_table(m);
+        try {
+            for (uint32_t c = 0; c < ncols; ++c) t->cols.push_back(gather_col(ctx, cols[c], idx, m));
+        } catch (...) {
+            qe_gpu_table_free(ctx, t);
+            throw;
+        }
+        QE_HIP(hipStreamSynchronize(ctx->stream));
+        *out = t;
     });
 }

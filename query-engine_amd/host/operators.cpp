@@ -243,6 +243,31 @@ std::optional<Batch> Scan::next() {
     return b;
 }
 
+// The materialised join output stays on the device for a device parent: the
+// columns are whole-column views into a gathered table that lives as long as
+// the returned object.
+struct TableColumns : DeviceColumns {
+    qe_dev_table* table = nullptr;
+    ~TableColumns() {
+        for (auto* c : cols) qe_gpu_column_free(device_context(), c);  // views: frees the handle only
+        cols.clear();
+        if (table) qe_gpu_table_free(device_context(), table);
+    }
+};
+
+std::shared_ptr<DeviceColumns> table_columns(qe_dev_table* t, std::vector<std::shared_ptr<const DeviceColumns>> sources) {
+    auto keep = std::make_shared<TableColumns>();
+    keep->table = t;
+    keep->sources = std::move(sources);
+    keep->rows = qe_gpu_table_rows(t);
+    for (uint32_t i = 0; i < qe_gpu_table_cols(t); ++i) {
+        qe_dev_col* v = nullptr;
+        QE_CALL(qe_gpu_column_slice(device_context(), qe_gpu_table_col(t, i), 0, keep->rows, &v));
+        keep->cols.push_back(v);
+    }
+    return keep;
+}
+
 // ---- result emission ----------------------------------------------------------------------
 void ResultEmitter::reset(qe_dev_table* t, std::vector<Type> types,
                           std::vector<std::shared_ptr<const DeviceColumns>> sources) {
@@ -327,17 +352,25 @@ GpuAggregate::GpuAggregate(std::unique_ptr<Operator> child, std::vector<uint32_t
 
 Schema GpuAggregate::output_schema() const { return agg_schema(child_schema_, key_cols_, aggs_); }
 
-std::shared_ptr<DeviceColumns> GpuAggregate::device_columns() {
-    throw std::logic_error("GpuAggregate output is emitted to the host");
-}
-
-void GpuAggregate::open() {
-    auto in = device_input(*child_);
+qe_dev_table* GpuAggregate::run(std::shared_ptr<DeviceColumns>& in) {
+    in = device_input(*child_);
     auto specs = abi_aggs(aggs_);
     qe_dev_table* t = nullptr;
     QE_CALL(qe_gpu_aggregate(device_context(), in->cols.data(), static_cast<uint32_t>(in->cols.size()),
                              key_cols_.data(), static_cast<uint32_t>(key_cols_.size()), specs.data(),
                              static_cast<uint32_t>(specs.size()), &t));
+    return t;
+}
+
+std::shared_ptr<DeviceColumns> GpuAggregate::device_columns() {
+    std::shared_ptr<DeviceColumns> in;
+    qe_dev_table* t = run(in);
+    return table_columns(t, {in});
+}
+
+void GpuAggregate::open() {
+    std::shared_ptr<DeviceColumns> in;
+    qe_dev_table* t = run(in);
     emit_.reset(t, types_of(output_schema()), {in});
 }
 
@@ -370,31 +403,15 @@ GpuHashJoin::GpuHashJoin(std::unique_ptr<Operator> probe, std::unique_ptr<Operat
 
 Schema GpuHashJoin::output_schema() const { return concat(probe_schema_, build_schema_); }
 
-// The materialised join output stays on the device for a device parent: the
-// columns are whole-column views into a gathered table that lives as long as
-// the returned object.
-struct TableColumns : DeviceColumns {
-    qe_dev_table* table = nullptr;
-    ~TableColumns() {
-        for (auto* c : cols) qe_gpu_column_free(device_context(), c);  // views: frees the handle only
-        cols.clear();
-        if (table) qe_gpu_table_free(device_context(), table);
-    }
-};
+
 
 std::shared_ptr<DeviceColumns> GpuHashJoin::device_columns() {
     open();
-    auto keep = std::make_shared<TableColumns>();
-    keep->sources = {pcols_, bcols_};
+    qe_dev_table* t = nullptr;
     QE_CALL(qe_gpu_pairs_gather(device_context(), pairs_, 0, qe_gpu_pairs_count(pairs_), pcols_->cols.data(),
                                 static_cast<uint32_t>(pcols_->cols.size()), bcols_->cols.data(),
-                                static_cast<uint32_t>(bcols_->cols.size()), &keep->table));
-    keep->rows = qe_gpu_table_rows(keep->table);
-    for (uint32_t i = 0; i < qe_gpu_table_cols(keep->table); ++i) {
-        qe_dev_col* s = nullptr;
-        QE_CALL(qe_gpu_column_slice(device_context(), qe_gpu_table_col(keep->table, i), 0, keep->rows, &s));
-        keep->cols.push_back(s);
-    }
+                                static_cast<uint32_t>(bcols_->cols.size()), &t));
+    auto keep = table_columns(t, {pcols_, bcols_});
     close();
     return keep;
 }
@@ -452,12 +469,20 @@ GpuJoinAggregate::GpuJoinAggregate(std::unique_ptr<Operator> probe, std::unique_
 Schema GpuJoinAggregate::output_schema() const { return agg_schema(join_schema_, group_keys_, aggs_); }
 
 std::shared_ptr<DeviceColumns> GpuJoinAggregate::device_columns() {
-    throw std::logic_error("GpuJoinAggregate output is emitted to the host");
+    std::shared_ptr<DeviceColumns> pc, bc;
+    qe_dev_table* t = run(pc, bc);
+    return table_columns(t, {pc, bc});
 }
 
 void GpuJoinAggregate::open() {
-    auto pc = device_input(*probe_);
-    auto bc = device_input(*build_);
+    std::shared_ptr<DeviceColumns> pc, bc;
+    qe_dev_table* t = run(pc, bc);
+    emit_.reset(t, types_of(output_schema()), {pc, bc});
+}
+
+qe_dev_table* GpuJoinAggregate::run(std::shared_ptr<DeviceColumns>& pc, std::shared_ptr<DeviceColumns>& bc) {
+    pc = device_input(*probe_);
+    bc = device_input(*build_);
     auto specs = abi_aggs(aggs_);
     qe_dev_table* t = nullptr;
     QE_CALL(qe_gpu_join_aggregate(device_context(), pc->cols.data(), static_cast<uint32_t>(pc->cols.size()),
@@ -466,7 +491,7 @@ void GpuJoinAggregate::open() {
                                   type_ == JoinType::Left ? QE_JOIN_LEFT : QE_JOIN_INNER, group_keys_.data(),
                                   static_cast<uint32_t>(group_keys_.size()), specs.data(),
                                   static_cast<uint32_t>(specs.size()), &t));
-    emit_.reset(t, types_of(output_schema()), {pc, bc});
+    return t;
 }
 
 void GpuJoinAggregate::close() { emit_.reset(nullptr, {}); }
@@ -496,16 +521,25 @@ void GpuAsofJoin::open() {
     for (uint32_t k : probe_keys_) pk.push_back(pcols_->cols.at(k));
     for (uint32_t k : build_keys_) bk.push_back(bcols_->cols.at(k));
     qe_gpu_ctx* ctx = device_context();
-    qe_dev_asof* idx = nullptr;
-    QE_CALL(qe_gpu_asof_build(ctx, bk.data(), static_cast<uint32_t>(bk.size()), bcols_->cols.at(build_time_), &idx));
+    // The index (sorted build side, and the learned router) is kept with the
+    // build columns, so a plan re-run over the same Table reuses it.
+    std::string sig = "asof";
+    for (uint32_t k : build_keys_) sig += ":" + std::to_string(k);
+    sig += "|" + std::to_string(build_time_) + (method_ == tsx::AsofMethod::Learned ? "|learned" : "|binary");
+    std::shared_ptr<void>& slot = bcols_->derived[sig];
+    if (!slot) {
+        qe_dev_asof* built = nullptr;
+        QE_CALL(qe_gpu_asof_build(ctx, bk.data(), static_cast<uint32_t>(bk.size()), bcols_->cols.at(build_time_),
+                                  &built));
+        std::shared_ptr<void> owner(built, [](void* p) { qe_gpu_asof_free(device_context(), static_cast<qe_dev_asof*>(p)); });
+        if (method_ == tsx::AsofMethod::Learned) QE_CALL(qe_gpu_asof_train(ctx, built, 64, 300, 0.003f));
+        slot = std::move(owner);
+    }
+    qe_dev_asof* idx = static_cast<qe_dev_asof*>(slot.get());
     const size_t n = pcols_->rows;
     void* match = nullptr;
     try {
-        int32_t mode = QE_ASOF_BINARY;
-        if (method_ == tsx::AsofMethod::Learned) {
-            QE_CALL(qe_gpu_asof_train(ctx, idx, 64, 300, 0.003f));
-            mode = QE_ASOF_LEARNED;
-        }
+        const int32_t mode = method_ == tsx::AsofMethod::Learned ? QE_ASOF_LEARNED : QE_ASOF_BINARY;
         QE_CALL(qe_gpu_malloc(ctx, std::max<size_t>(n, 1) * 4, &match));
         QE_CALL(qe_gpu_asof_probe(ctx, idx, pk.data(), static_cast<uint32_t>(pk.size()), pcols_->cols.at(probe_time_),
                                   tolerance_ ? 1 : 0, tolerance_ ? *tolerance_ : 0, mode,
@@ -516,12 +550,21 @@ void GpuAsofJoin::open() {
                                   type_ == tsx::AsofType::Left ? QE_JOIN_LEFT : QE_JOIN_INNER, &pairs_));
     } catch (...) {
         if (match) qe_gpu_free(ctx, match);
-        qe_gpu_asof_free(ctx, idx);
         throw;
     }
     qe_gpu_free(ctx, match);
-    qe_gpu_asof_free(ctx, idx);
     cursor_ = 0;
+}
+
+std::shared_ptr<DeviceColumns> GpuAsofJoin::device_columns() {
+    open();
+    qe_dev_table* t = nullptr;
+    QE_CALL(qe_gpu_pairs_gather(device_context(), pairs_, 0, qe_gpu_pairs_count(pairs_), pcols_->cols.data(),
+                                static_cast<uint32_t>(pcols_->cols.size()), bcols_->cols.data(),
+                                static_cast<uint32_t>(bcols_->cols.size()), &t));
+    auto keep = table_columns(t, {pcols_, bcols_});
+    close();
+    return keep;
 }
 
 std::optional<Batch> GpuAsofJoin::next() {
@@ -544,6 +587,56 @@ void GpuAsofJoin::close() {
     pairs_ = nullptr;
     pcols_.reset();
     bcols_.reset();
+}
+
+// ---- GpuSort ---------------------------------------------------------------------------------------
+GpuSort::GpuSort(std::unique_ptr<Operator> child, std::vector<SortKey> keys)
+    : child_(std::move(child)), keys_(std::move(keys)) {
+    if (keys_.empty()) throw std::invalid_argument("Sort needs >=1 key");
+    child_schema_ = child_->output_schema();
+    for (const SortKey& k : keys_)
+        if (k.col >= child_schema_.fields.size()) throw std::invalid_argument("Sort: key column out of range");
+}
+
+qe_dev_table* GpuSort::run(std::shared_ptr<DeviceColumns>& in) {
+    in = device_input(*child_);
+    qe_gpu_ctx* ctx = device_context();
+    const size_t n = in->rows;
+    std::vector<qe_dev_col*> kc;
+    std::vector<int32_t> desc, nf;
+    for (const SortKey& k : keys_) {
+        kc.push_back(in->cols.at(k.col));
+        desc.push_back(k.dir == SortDir::Desc ? 1 : 0);
+        nf.push_back(k.nulls == NullOrder::First ? 1 : 0);
+    }
+    void* perm = nullptr;
+    qe_dev_table* t = nullptr;
+    try {
+        QE_CALL(qe_gpu_malloc(ctx, std::max<size_t>(n, 1) * 4, &perm));
+        if (n) QE_CALL(qe_gpu_sort_perm(ctx, kc.data(), static_cast<uint32_t>(kc.size()), desc.data(), nf.data(),
+                                        static_cast<uint32_t*>(perm)));
+        QE_CALL(qe_gpu_take(ctx, in->cols.data(), static_cast<uint32_t>(in->cols.size()),
+                            static_cast<uint32_t*>(perm), n, &t));
+    } catch (...) {
+        if (perm) qe_gpu_free(ctx, perm);
+        throw;
+    }
+    qe_gpu_free(ctx, perm);
+    return t;
+}
+
+void GpuSort::open() {
+    std::shared_ptr<DeviceColumns> in;
+    qe_dev_table* t = run(in);
+    emit_.reset(t, types_of(output_schema()), {in});
+}
+
+void GpuSort::close() { emit_.reset(nullptr, {}); }
+
+std::shared_ptr<DeviceColumns> GpuSort::device_columns() {
+    std::shared_ptr<DeviceColumns> in;
+    qe_dev_table* t = run(in);
+    return table_columns(t, {in});
 }
 
 // ---- GpuHashTable ----------------------------------------------------------------------------------
@@ -743,16 +836,8 @@ void GpuFilter::close() {
 std::shared_ptr<DeviceColumns> GpuFilter::device_columns() {
     qe_dev_table* t = run();
     if (!t) return device_input(*child_);
-    auto keep = std::make_shared<TableColumns>();
-    keep->table = t;
-    keep->sources = {in_};
+    auto keep = table_columns(t, {in_});
     in_.reset();
-    keep->rows = qe_gpu_table_rows(t);
-    for (uint32_t i = 0; i < qe_gpu_table_cols(t); ++i) {
-        qe_dev_col* v = nullptr;
-        QE_CALL(qe_gpu_column_slice(device_context(), qe_gpu_table_col(t, i), 0, keep->rows, &v));
-        keep->cols.push_back(v);
-    }
     return keep;
 }
 

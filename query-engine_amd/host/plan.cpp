@@ -88,6 +88,27 @@ PlanBuilder PlanBuilder::filter(expr::Expr predicate) const {
     return PlanBuilder(Plan(std::move(n)));
 }
 
+// plan/plan.cpp:548-566: >=1 key, each in range; the schema is unchanged.
+PlanBuilder PlanBuilder::sort(std::vector<SortKey> keys) const {
+    const Schema& cs = plan_.output_schema();
+    if (keys.empty()) throw std::invalid_argument("plan: sort needs >=1 key");
+    for (const SortKey& k : keys)
+        if (k.col >= cs.fields.size())
+            throw std::invalid_argument("plan: sort key index " + std::to_string(k.col) + " out of range");
+    auto n = make(PlanKind::Sort);
+    n->children.push_back(plan_);
+    n->sort_keys = std::move(keys);
+    n->out_schema = cs;
+    return PlanBuilder(Plan(std::move(n)));
+}
+
+PlanBuilder PlanBuilder::sort(std::vector<SortBy> keys) const {
+    const Schema& cs = plan_.output_schema();
+    std::vector<SortKey> resolved;
+    for (const SortBy& k : keys) resolved.push_back(SortKey{k.col.resolve(cs), k.dir, k.nulls});
+    return sort(std::move(resolved));
+}
+
 PlanBuilder PlanBuilder::aggregate(std::vector<ColRef> keys, std::vector<AggSpec> aggs) const {
     const Schema& cs = plan_.output_schema();
     if (aggs.empty()) throw std::invalid_argument("plan: aggregate needs >=1 aggregate");
@@ -167,6 +188,8 @@ std::unique_ptr<Operator> Plan::lower(const LowerOptions& opt) const {
             return std::make_unique<Scan>(*n.table, opt.batch_size);
         case PlanKind::Filter:
             return std::make_unique<GpuFilter>(n.children[0].lower(opt), n.predicate);
+        case PlanKind::Sort:
+            return std::make_unique<GpuSort>(n.children[0].lower(opt), n.sort_keys);
         case PlanKind::Aggregate: {
             const Plan& c = n.children[0];
             if (opt.fuse_join_aggregate && c.kind() == PlanKind::Join) {
@@ -198,6 +221,9 @@ void print(const Plan& p, int depth, std::ostringstream& os) {
             break;
         case PlanKind::Filter:
             os << "Filter";
+            break;
+        case PlanKind::Sort:
+            os << "Sort(keys:" << n.sort_keys.size() << ")";
             break;
         case PlanKind::Aggregate:
             os << "Aggregate(keys:" << n.group_keys.size() << " aggs:" << n.aggs.size() << ")";

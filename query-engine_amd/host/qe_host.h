@@ -173,6 +173,9 @@ struct DeviceColumns {
     std::vector<qe_gpu_dict*> dicts;
     // Inputs whose dictionaries these columns' STR codes still index.
     std::vector<std::shared_ptr<const DeviceColumns>> sources;
+    // Indexes built over these columns and kept with them (e.g. a Table's
+    // as-of index), by a signature of what was built.
+    mutable std::map<std::string, std::shared_ptr<void>> derived;
     ~DeviceColumns();
 };
 std::shared_ptr<DeviceColumns> upload_table(const Table& t);
@@ -246,6 +249,9 @@ class ResultEmitter {
 };
 
 // ---- device operators -------------------------------------------------------------------
+// Device result tables handed to a device parent as columns.
+std::shared_ptr<DeviceColumns> table_columns(qe_dev_table* t, std::vector<std::shared_ptr<const DeviceColumns>> sources);
+
 // Aggregate (ops/aggregate.h:110-111): GROUP BY key_cols with aggs; output =
 // key columns ++ aggregate columns; zero keys -> exactly one row.
 class GpuAggregate : public Operator, public DeviceSource {
@@ -259,6 +265,7 @@ class GpuAggregate : public Operator, public DeviceSource {
     std::shared_ptr<DeviceColumns> device_columns() override;
 
    private:
+    qe_dev_table* run(std::shared_ptr<DeviceColumns>& in);
     std::unique_ptr<Operator> child_;
     std::vector<uint32_t> key_cols_;
     std::vector<AggSpec> aggs_;
@@ -302,6 +309,7 @@ class GpuJoinAggregate : public Operator, public DeviceSource {
     std::shared_ptr<DeviceColumns> device_columns() override;
 
    private:
+    qe_dev_table* run(std::shared_ptr<DeviceColumns>& pc, std::shared_ptr<DeviceColumns>& bc);
     std::unique_ptr<Operator> probe_, build_;
     std::vector<uint32_t> probe_keys_, build_keys_;
     JoinType type_;
@@ -313,7 +321,7 @@ class GpuJoinAggregate : public Operator, public DeviceSource {
 
 // tsx::AsofJoin (tsx/asof.h:101-105): backward as-of; output = probe columns ++
 // build columns; the method selects binary search or the learned index.
-class GpuAsofJoin : public Operator {
+class GpuAsofJoin : public Operator, public DeviceSource {
    public:
     static constexpr size_t kOutBatch = 2048;
     GpuAsofJoin(std::unique_ptr<Operator> probe, std::unique_ptr<Operator> build,
@@ -324,6 +332,7 @@ class GpuAsofJoin : public Operator {
     std::optional<Batch> next() override;
     void close() override;
     Schema output_schema() const override;
+    std::shared_ptr<DeviceColumns> device_columns() override;
 
    private:
     std::unique_ptr<Operator> probe_, build_;
@@ -337,6 +346,37 @@ class GpuAsofJoin : public Operator {
     qe_dev_pairs* pairs_ = nullptr;
     size_t cursor_ = 0;
     ResultEmitter chunk_;
+};
+
+// ---- sort vocabulary (ops/sort.h:33-42) ---------------------------------------------------
+enum class SortDir { Asc, Desc };
+enum class NullOrder { First, Last };
+struct SortKey {
+    uint32_t col = 0;
+    SortDir dir = SortDir::Asc;
+    NullOrder nulls = NullOrder::Last;
+};
+
+// Sort (ops/sort.h:84): a stable sort of the child's rows by `keys` (ORDER BY
+// precedence), every column gathered by the permutation; NULL placement is
+// absolute. Integer-family keys, F64 (NaN greatest, -0 == +0) and STR (by
+// value) on the device. Output batches of <= 2048 rows.
+class GpuSort : public Operator, public DeviceSource {
+   public:
+    static constexpr size_t kOutBatch = 2048;
+    GpuSort(std::unique_ptr<Operator> child, std::vector<SortKey> keys);
+    void open() override;
+    std::optional<Batch> next() override { return emit_.next(); }
+    void close() override;
+    Schema output_schema() const override { return child_schema_; }
+    std::shared_ptr<DeviceColumns> device_columns() override;
+
+   private:
+    qe_dev_table* run(std::shared_ptr<DeviceColumns>& in);
+    std::unique_ptr<Operator> child_;
+    std::vector<SortKey> keys_;
+    Schema child_schema_;
+    ResultEmitter emit_;
 };
 
 // Device HashTable with the reference's API (ops/hashtable.h:108-190).

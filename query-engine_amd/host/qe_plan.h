@@ -1,5 +1,6 @@
 // Dataframe plan builder for the device path: the scan / filter / join /
-// aggregate / asof_join subset of the reference PlanBuilder (plan/plan.h:205-227), with
+// aggregate / asof_join / sort subset of the reference PlanBuilder
+// (plan/plan.h:205-227), with
 // the same validation (key lists resolved by index or name, positional key
 // type equality, time columns I32/I64/TS) and the same output schemas, and a
 // lowering that produces the device operators (plan/plan.cpp:570-615 lowers
@@ -17,7 +18,7 @@
 
 namespace qe::plan {
 
-enum class PlanKind { Scan, Filter, Aggregate, Join, AsofJoin };
+enum class PlanKind { Scan, Filter, Aggregate, Join, AsofJoin, Sort };
 
 struct ColRef {
     bool by_name = false;
@@ -28,6 +29,13 @@ struct ColRef {
     ColRef(const char* n) : by_name(true), name(n) {}
     ColRef(std::string n) : by_name(true), name(std::move(n)) {}
     uint32_t resolve(const Schema& s) const;
+};
+
+// A sort key by name or index (plan/plan.h:101-105).
+struct SortBy {
+    ColRef col;
+    SortDir dir = SortDir::Asc;
+    NullOrder nulls = NullOrder::Last;
 };
 
 class Plan;
@@ -44,6 +52,7 @@ struct PlanNode {
     uint32_t left_time = 0, right_time = 0;
     tsx::AsofType asof_type = tsx::AsofType::Inner;
     std::optional<int64_t> tolerance;
+    std::vector<SortKey> sort_keys;  // Sort
 };
 
 struct LowerOptions {
@@ -70,6 +79,8 @@ class PlanBuilder {
    public:
     explicit PlanBuilder(Plan p) : plan_(std::move(p)) {}
     PlanBuilder filter(expr::Expr predicate) const;
+    PlanBuilder sort(std::vector<SortKey> keys) const;  // index path
+    PlanBuilder sort(std::vector<SortBy> keys) const;   // name-friendly path
     PlanBuilder aggregate(std::vector<ColRef> keys, std::vector<AggSpec> aggs) const;
     PlanBuilder join(const Plan& build, std::vector<ColRef> left_keys, std::vector<ColRef> right_keys,
                      JoinType type) const;

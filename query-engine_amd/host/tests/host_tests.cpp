@@ -520,6 +520,129 @@ void test_str_group_by_name_plan() {
     CHECK(drain_text(*f) == (std::vector<std::vector<std::string>>{{"2", "beta"}, {"3", "ACME"}}));
 }
 
+void test_sort_operator() {
+    // ops/sort.h:84: stable; DESC; NULL placement absolute; F64 NaN greatest; STR by value.
+    Table t(sch({{"k", Type::I64}, {"f", Type::F64}, {"s", Type::STR}}), [] {
+        std::vector<OwnedColumn> c;
+        c.push_back(mk<int64_t>(Type::I64, {3, 1, 3, 2, 1}, {3}));
+        c.push_back(mk<double>(Type::F64, {0.5, std::nan(""), -1.0, 2.0, 0.5}));
+        c.push_back(str_col({"b", "a", "C", std::nullopt, "a"}));
+        return c;
+    }());
+    auto text_rows = [](Operator& op) {  // in output order
+        std::vector<std::vector<std::string>> rows;
+        op.open();
+        while (auto b = op.next()) {
+            for (size_t k = 0; k < b->row_count; ++k) {
+                std::vector<std::string> r;
+                for (const Column& c : b->cols) {
+                    const bool valid = c.all_valid || ((c.validity[k >> 6] >> (k & 63)) & 1ull);
+                    if (!valid) {
+                        r.push_back("NULL");
+                    } else if (c.type == Type::STR) {
+                        r.emplace_back(c.dict->at(reinterpret_cast<const int32_t*>(c.data)[k]));
+                    } else if (c.type == Type::F64) {
+                        char buf[32];
+                        std::snprintf(buf, sizeof buf, "%g", reinterpret_cast<const double*>(c.data)[k]);
+                        r.emplace_back(buf);
+                    } else {
+                        r.push_back(std::to_string(reinterpret_cast<const int64_t*>(c.data)[k]));
+                    }
+                }
+                rows.push_back(r);
+            }
+        }
+        op.close();
+        return rows;
+    };
+    {
+        auto op = plan::scan(t).sort({plan::SortBy{"k", SortDir::Desc, NullOrder::Last}}).build();
+        const auto rows = text_rows(*op);
+        CHECK(rows == (std::vector<std::vector<std::string>>{{"3", "0.5", "b"}, {"3", "-1", "C"},
+                                                              {"1", "nan", "a"}, {"1", "0.5", "a"},
+                                                              {"NULL", "2", "NULL"}}));
+    }
+    {
+        GpuSort op(std::make_unique<Scan>(t), {SortKey{1, SortDir::Asc, NullOrder::First}});
+        const auto rows = text_rows(op);
+        CHECK(rows.size() == 5 && rows[0][1] == "-1" && rows[1][1] == "0.5" && rows[1][0] == "3" &&
+              rows[2][1] == "0.5" && rows[2][0] == "1" && rows[3][1] == "2" && rows[4][1] == "nan");
+    }
+    {
+        GpuSort op(std::make_unique<Scan>(t), {SortKey{2, SortDir::Asc, NullOrder::First}});
+        const auto rows = text_rows(op);  // NULL, then byte order "C" < "a" < "b", stable
+        CHECK(rows.size() == 5 && rows[0][2] == "NULL" && rows[1][2] == "C" && rows[2][2] == "a" &&
+              rows[2][0] == "1" && rows[2][1] == "nan" && rows[3][2] == "a" && rows[4][2] == "b");
+    }
+    bool threw = false;
+    try {
+        plan::scan(t).sort(std::vector<SortKey>{});
+    } catch (const std::invalid_argument&) {
+        threw = true;
+    }
+    CHECK(threw);
+}
+
+void test_tick_pipeline_plan() {
+    // The tick pipeline end to end on the device: trades WHERE px > 100, as-of
+    // joined to quotes (the latest quote at or before the trade, same symbol),
+    // joined to the symbol table on the id, GROUP BY name COUNT(*) / AVG(px),
+    // ORDER BY the count descending.
+    Table trades(sch({{"sym", Type::I32}, {"ts", Type::TS}, {"px", Type::F64}}), [] {
+        std::vector<OwnedColumn> c;
+        c.push_back(mk<int32_t>(Type::I32, {0, 0, 1, 1, 2, 2, 2, 0, 1}));
+        c.push_back(mk<int64_t>(Type::TS, {10, 20, 5, 30, 40, 50, 60, 1, 31}));
+        c.push_back(mk<double>(Type::F64, {150, 50, 120, 130, 101, 102, 99, 200, 140}));
+        return c;
+    }());
+    Table quotes(sch({{"sym", Type::I32}, {"ts", Type::TS}, {"bid", Type::F64}}), [] {
+        std::vector<OwnedColumn> c;
+        c.push_back(mk<int32_t>(Type::I32, {0, 1, 2, 1}));
+        c.push_back(mk<int64_t>(Type::TS, {5, 25, 45, 3}));
+        c.push_back(mk<double>(Type::F64, {1, 2, 3, 4}));
+        return c;
+    }());
+    Table syms(sch({{"id", Type::I32}, {"name", Type::STR}}), [] {
+        std::vector<OwnedColumn> c;
+        c.push_back(mk<int32_t>(Type::I32, {2, 0, 1}));
+        c.push_back(str_col({"CCC", "AAA", "BB"}));
+        return c;
+    }());
+    using namespace qe::expr;
+    for (auto method : {tsx::AsofMethod::Binary, tsx::AsofMethod::Learned}) {
+        plan::LowerOptions opt;
+        opt.asof_method = method;
+        auto op = plan::scan(trades)
+                      .filter(gt(col(Type::F64, 2), lit(Scalar::f64(100.0))))
+                      .asof_join(plan::scan(quotes), {"sym"}, {"sym"}, "ts", "ts", tsx::AsofType::Inner)
+                      .join(plan::scan(syms), {0}, {"id"}, JoinType::Inner)
+                      .aggregate({"name"}, {AggSpec::count_star("n"), AggSpec::avg(2, "avg_px")})
+                      .sort({plan::SortBy{"n", SortDir::Desc, NullOrder::Last}})
+                      .build(opt);
+        // px > 100: (0,10,150) (1,5,120) (1,30,130) (2,40,101) (2,50,102) (0,1,200) (1,31,140).
+        // As-of (quote ts <= trade ts, same sym): sym0 quote@5 matches ts 10 (not ts 1);
+        // sym1 quotes @3,@25 match ts 5, 30, 31; sym2 quote@45 matches ts 50 (not 40).
+        // -> BB: 3 rows avg 130; AAA: 1 row 150; CCC: 1 row 102.
+        std::vector<std::vector<std::string>> rows;
+        op->open();
+        while (auto b = op->next()) {
+            for (size_t k = 0; k < b->row_count; ++k) {
+                const Column& name = b->cols[0];
+                char buf[64];
+                std::snprintf(buf, sizeof buf, "%lld|%g",
+                              static_cast<long long>(reinterpret_cast<const int64_t*>(b->cols[1].data)[k]),
+                              reinterpret_cast<const double*>(b->cols[2].data)[k]);
+                rows.push_back({std::string(name.dict->at(reinterpret_cast<const int32_t*>(name.data)[k])), buf});
+            }
+        }
+        op->close();
+        CHECK(rows.size() == 3 && rows[0] == (std::vector<std::string>{"BB", "3|130"}));
+        std::sort(rows.begin() + 1, rows.end());
+        CHECK(rows.size() == 3 && rows[1] == (std::vector<std::string>{"AAA", "1|150"}) &&
+              rows[2] == (std::vector<std::string>{"CCC", "1|102"}));
+    }
+}
+
 int main() {
     const std::vector<std::pair<const char*, std::function<void()>>> tests = {
         {"join_groupby_plan", test_join_groupby_plan},
@@ -533,6 +656,8 @@ int main() {
         {"filter_exceptions", test_filter_exceptions},
         {"str_join_by_value", test_str_join_by_value},
         {"str_group_by_name_plan", test_str_group_by_name_plan},
+        {"sort_operator", test_sort_operator},
+        {"tick_pipeline_plan", test_tick_pipeline_plan},
     };
     for (const auto& [name, fn] : tests) {
         std::printf("[host] %s\n", name);

@@ -153,6 +153,7 @@ def ref() -> C.CDLL:
             "ref_asof_run": (P, [P, C.c_uint32, C.c_uint64, P, C.c_uint32, C.c_uint64, P, P, C.c_uint32,
                                  C.c_uint32, C.c_uint32, C.c_int32, C.c_int32, C.c_int64, C.c_uint64]),
             "ref_filter_run": (P, [P, C.c_uint32, C.c_uint64, P, C.c_uint32, C.c_uint64]),
+            "ref_sort_run": (P, [P, C.c_uint32, C.c_uint64, P, P, P, C.c_uint32, C.c_uint64]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -419,5 +420,14 @@ def ref_asof_pairs(probe_cols, build_cols, pkeys, bkeys, ptime, btime, join_type
 def ref_filter_select(cols, terms, batch_size=2048):
     cs = Cols(cols)
     h = ref().ref_filter_run(cs.arr, len(cols), cs.n, _preds(terms), len(terms), batch_size)
+    rows = _ref_rows(h)
+    return np.array([r[len(cols)] for r in rows], dtype=np.int64).astype(np.uint32)
+
+
+def ref_sort_perm(cols, keys, desc=None, nulls_first=None, batch_size=2048):
+    cs = Cols(cols)
+    k = len(keys)
+    h = ref().ref_sort_run(cs.arr, len(cols), cs.n, _u32(keys), _i32(desc or [0] * k), _i32(nulls_first or [0] * k),
+                           k, batch_size)
     rows = _ref_rows(h)
     return np.array([r[len(cols)] for r in rows], dtype=np.int64).astype(np.uint32)

@@ -43,3 +43,63 @@ def test_sort_perm_matches_oracle(ctx, case):
             x.free()
     want = O.sort_perm(cols, list(range(nk)), desc, nf)
     assert (got == want).all()
+
+
+F64, BOOL, STR = O.F64, O.BOOL, O.STR
+
+
+@pytest.mark.parametrize("case", range(10))
+def test_sort_perm_f64_str_bool_mixed(ctx, case):
+    # The comparison path's key types (ops/sort.cpp:60-100): F64 in the
+    # NaN-greatest total order with -0 == +0, STR by value, BOOL as 0/1, mixed
+    # with integer keys; the oracle's comparison path is pinned to the
+    # compiled reference Sort.
+    from test_oracle_golden import _rand_sort_cols
+    r = np.random.default_rng(6100 + case)
+    n = int(r.choice([1, 100, 4097, 60000, 250000]))
+    cols = _rand_sort_cols(r, n)
+    nk = 1 + case % 3
+    keys = [int(k) for k in r.permutation(4)[:nk]]
+    desc = [int(r.integers(0, 2)) for _ in range(nk)]
+    nf = [int(r.integers(0, 2)) for _ in range(nk)]
+    dicts = []
+    dev = []
+    for c in cols:
+        d = None
+        if c[2] == STR:
+            d = ctx.dict(c[3])
+            dicts.append(d)
+        dev.append(ctx.upload(c[0], c[2], c[1], dict=d))
+    try:
+        got = ctx.sort_perm([dev[k] for k in keys], desc, nf)
+    finally:
+        for x in dev:
+            x.free()
+        for d in dicts:
+            d.free()
+    want = O.sort_perm(cols, keys, desc, nf)
+    assert (got == want).all()
+
+
+def test_take_gathers_rows_and_rejects_bad_indices(ctx, qe):
+    r = np.random.default_rng(6200)
+    n = 10000
+    v = r.integers(-100, 100, n)
+    valid = r.random(n) >= 0.1
+    a = ctx.upload(v, I64, valid)
+    idx = np.concatenate([r.permutation(n)[:5000], r.integers(0, n, 3000)]).astype(np.uint32)
+    buf = qe.DevBuffer(ctx, 4 * len(idx))
+    try:
+        buf.upload(idx)
+        t = ctx.take([a], buf, len(idx))
+        gv, gvalid = t.col(0).numpy()
+        t.free()
+        assert (gvalid == valid[idx]).all() and (gv[gvalid] == v[idx][valid[idx]]).all()
+        bad = idx.copy()
+        bad[17] = n
+        buf.upload(bad)
+        with pytest.raises(qe.InvalidArgument):
+            ctx.take([a], buf, len(idx))
+    finally:
+        buf.free()
+        a.free()

@@ -833,3 +833,42 @@ def test_str_join_aggregate_vs_compiled_reference():
     got = O.join_aggregate(trades, sym_tab, [0], [0], 0, [3], aggs, sum_mode=0)
     want = O.ref_join_aggregate(trades, sym_tab, [0], [0], 0, [3], aggs)
     assert diff(got, want, rel=1e-12, abs_=0) is None
+
+
+# ---- sort: comparison path (F64, STR, BOOL and mixed keys) --------------------------------------
+def _rand_sort_cols(r, n):
+    f = np.round(r.normal(size=n), 1)
+    f[r.random(n) < 0.05] = np.nan
+    f[r.random(n) < 0.05] = -0.0
+    f[r.random(n) < 0.05] = 0.0
+    return [col(f, F64, tuple(np.nonzero(r.random(n) < 0.05)[0])),
+            rand_str_col(r, n, 8),
+            col(r.integers(0, 2, n), BOOL, tuple(np.nonzero(r.random(n) < 0.05)[0])),
+            col(r.integers(-3, 3, n), I64)]
+
+
+@needs_ref
+@pytest.mark.parametrize("case", range(12))
+def test_sort_comparison_path_vs_compiled_reference(case):
+    # wp7b_strings_differential_test.cpp:217-246 (STR ORDER BY, desc and NULL
+    # placement) and mixed F64 / BOOL / integer keys through the reference Sort.
+    r = np.random.default_rng(1900 + case)
+    n = int(r.integers(1, 3000))
+    cols = _rand_sort_cols(r, n)
+    nk = 1 + case % 3
+    keys = [int(k) for k in r.permutation(4)[:nk]]
+    desc = [int(r.integers(0, 2)) for _ in range(nk)]
+    nf = [int(r.integers(0, 2)) for _ in range(nk)]
+    got = O.sort_perm(cols, keys, desc, nf)
+    want = O.ref_sort_perm(cols, keys, desc, nf, batch_size=[64, 256, 2048][case % 3])
+    assert got.tolist() == want.tolist()
+
+
+def test_sort_f64_total_order_literals():
+    # ops/sort_internal.h:100-104: NaN greatest and tied, -0 == +0 (stable)
+    c = col([1.0, np.nan, -0.0, 0.0, -np.inf, np.nan, 0.0], F64)
+    assert O.sort_perm([c], [0]).tolist() == [4, 2, 3, 6, 0, 1, 5]
+    assert O.sort_perm([c], [0], [1]).tolist() == [1, 5, 0, 2, 3, 6, 4]
+    s = str_col(["b", None, "a", "B", "b"])
+    assert O.sort_perm([s], [0]).tolist() == [3, 2, 0, 4, 1]       # byte order: "B" < "a" < "b"
+    assert O.sort_perm([s], [0], [1], [1]).tolist() == [1, 0, 4, 2, 3]
