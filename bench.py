@@ -397,6 +397,115 @@ def filter_workload(qe, ctx, args):
     return res
 
 
+def tick_workload(qe, ctx, args):
+    """The full tick pipeline: trades (2^30 rows: sym I32, ts I64 us, px F64
+    uniform [0,200)) WHERE px > 100, as-of joined (backward, learned index) to
+    quotes (2^31 rows: sym Zipf(1.1) over 50,000 symbols, bursty per-symbol
+    microsecond times), joined on the symbol id to a 50,000-row symbol table
+    whose names are dictionary-encoded VARCHAR (3-12 characters), GROUP BY
+    name COUNT(*) / AVG(px), ORDER BY the count descending. The quote index
+    (device sort + learned router) is built once, outside the timed steps; one
+    step is the whole query over the trades."""
+    nq, nt, nsym = 1 << args.tick_quotes_log2, 1 << args.tick_trades_log2, 50000
+    log(f"generating tick workload: quotes {nq} trades {nt}")
+    (qs, qt, qp), (ts_, tt) = ctx.gen_asof(20260618, nq, nt, nsym, 1.1)
+    tp = ctx.gen_uniform(20260618, 5, qe.F64, nt, 0.0, 200.0)
+    r = np.random.default_rng(20260618)
+    letters = np.array(list("ABCDEFGHIJKLMNOPQRSTUVWXYZ"))
+    names = set()
+    while len(names) < nsym:
+        k = int(r.integers(3, 13))
+        names.add("".join(r.choice(letters, k)))
+    names = sorted(names)
+    order = r.permutation(nsym)
+    d = ctx.dict([names[i] for i in r.permutation(nsym)])  # dictionary in its own order
+    code = {x: i for i, x in enumerate(d.strings)}
+    sym_id = ctx.upload(np.arange(nsym, dtype=np.int32), qe.I32)
+    sym_name = ctx.upload(np.array([code[names[i]] for i in order], dtype=np.int32), qe.STR, dict=d)
+    t0 = time.perf_counter()
+    idx = ctx.asof_build([qs], qt)
+    ctx.sync()
+    t1 = time.perf_counter()
+    idx.train(args.piece_rows, args.train_steps, args.train_lr)
+    ctx.sync()
+    t2 = time.perf_counter()
+    match = qe.DevBuffer(ctx, 4 * nt)
+    perm = qe.DevBuffer(ctx, 4 * nsym)
+    stages = {}
+
+    def stamp(name, t):
+        ctx.sync()
+        now = time.perf_counter()
+        stages[name] = stages.get(name, 0.0) + (now - t)
+        return now
+
+    def step(mode, timed_stages=False):
+        t = time.perf_counter()
+        f = ctx.filter([ts_, tt, tp], [(2, qe.CMP_GT, qe.F64, 100.0)])
+        if timed_stages:
+            t = stamp("filter", t)
+        m = f.rows
+        idx.probe_into([f.col(0)], f.col(1), match, mode=mode)
+        pairs = ctx.asof_pairs(match, m, qe.INNER)
+        if timed_stages:
+            t = stamp("asof", t)
+        g = pairs.gather([f.col(0), f.col(2)], [])
+        if timed_stages:
+            t = stamp("asof_gather", t)
+        agg = ctx.join_aggregate([g.col(0), g.col(1)], [sym_id, sym_name], [0], [0], qe.INNER, [3],
+                                 [(qe.COUNT_STAR, 0), (qe.AVG, 1)])
+        if timed_stages:
+            t = stamp("join_groupby", t)
+        ng = agg.rows
+        cnt = agg.col(1)
+        ctx._check(qe.lib().qe_gpu_sort_perm(ctx.h, qe._cols([cnt]), 1, (qe.C.c_int32 * 1)(1), None, perm.p))
+        out = ctx.take([agg.col(0), agg.col(1), agg.col(2)], perm, ng)
+        if timed_stages:
+            t = stamp("sort", t)
+        res = (m, pairs.count, ng, out)
+        pairs.free()
+        g.free()
+        agg.free()
+        f.free()
+        return res
+
+    for _ in range(max(args.warmup, 1)):
+        step(qe.ASOF_LEARNED)[3].free()
+    ctx.sync()
+    t3 = time.perf_counter()
+    for _ in range(args.steps):
+        last = step(qe.ASOF_LEARNED)
+        if _ + 1 < args.steps:
+            last[3].free()
+    ctx.sync()
+    wall = (time.perf_counter() - t3) / args.steps
+    m, matched, ng, out = last
+    top = [(n, c, a) for n, c, a in zip(out.col(0).strings()[:5], out.col(1).numpy()[0][:5].tolist(),
+                                         out.col(2).numpy()[0][:5].tolist())]
+    counts = out.col(1).numpy()[0]
+    # Identical result with the binary-search as-of (the learned index is exact).
+    ref = step(qe.ASOF_BINARY)
+    same = (ref[0] == m and ref[1] == matched and ref[2] == ng and
+            ref[3].col(0).strings() == out.col(0).strings() and
+            np.array_equal(ref[3].col(1).numpy()[0], counts))
+    ref[3].free()
+    step(qe.ASOF_LEARNED, timed_stages=True)[3].free()
+    res = {"quotes": nq, "trades": nt, "symbols": nsym, "passing_filter": m, "asof_matched": matched,
+           "groups": ng, "ms_per_step": wall * 1e3, "trade_rows_per_s": nt / wall,
+           "index_build_s": t1 - t0, "index_train_s": t2 - t1,
+           "stage_ms": {k: v * 1e3 for k, v in stages.items()},
+           "counts_sorted_desc": bool(np.all(np.diff(counts.astype(np.int64)) <= 0)),
+           "top5": top, "learned_equals_binary": bool(same)}
+    out.free()
+    match.free()
+    perm.free()
+    idx.free()
+    for c in (qs, qt, qp, ts_, tt, tp, sym_id, sym_name):
+        c.free()
+    d.free()
+    return res
+
+
 def ctx_slice(qe, ctx, col, n):
     import ctypes as C
     out = C.c_void_p()
@@ -432,8 +541,10 @@ def main():
     ap.add_argument("--batch-max-log2", type=int, default=21)
     ap.add_argument("--no-per-batch", action="store_true", help="skip the per-batch baseline (profiling runs)")
     ap.add_argument("--filter-log2", type=int, default=30)
+    ap.add_argument("--tick-trades-log2", type=int, default=30)
+    ap.add_argument("--tick-quotes-log2", type=int, default=31)
     ap.add_argument("--filter-null-frac", type=float, default=0.02)
-    ap.add_argument("--only", choices=["join", "asof", "batched", "filter", "all"], default="all",
+    ap.add_argument("--only", choices=["join", "asof", "batched", "filter", "tick", "all"], default="all",
                     help="join: headline workload only; asof / batched: that secondary workload alone")
     ap.add_argument("--quick", action="store_true", help="small sizes (smoke)")
     ap.add_argument("--calibrate-copy", action="store_true", help="only run one 1 GiB stream copy")
@@ -447,15 +558,17 @@ def main():
         args.asof_right, args.asof_left, args.cpu_asof_log2 = 1 << 22, 1 << 20, 18
         args.batch_queries, args.batch_morsels, args.batch_max_log2 = 8, 64, 16
         args.filter_log2 = 20
+        args.tick_trades_log2, args.tick_quotes_log2 = 18, 19
     qe = load_pkg()
     ctx = qe.Context(0)
     if args.calibrate_copy:
         # PMC calibration run: one copy of known size (1 GiB read + 1 GiB write).
         print(json.dumps({"copy": copy_ceiling(qe, ctx, reps=0)}), flush=True)
         return
-    if args.only in ("asof", "batched", "filter"):
+    if args.only in ("asof", "batched", "filter", "tick"):
         # One secondary workload on its own (diagnostics; not the headline line).
-        fn = {"asof": asof_workload, "batched": batched_workload, "filter": filter_workload}[args.only]
+        fn = {"asof": asof_workload, "batched": batched_workload, "filter": filter_workload,
+              "tick": tick_workload}[args.only]
         print(json.dumps({args.only: fn(qe, ctx, args)}), flush=True)
         return
     ceiling = copy_ceiling(qe, ctx)

@@ -428,6 +428,12 @@ class Context:
         self._check(lib().qe_gpu_column_slice(self.h, col.h, offset, n, C.byref(out)))
         return DevCol(self, out)
 
+    def asof_pairs(self, match: "DevBuffer", n: int, join_type=INNER) -> Pairs:
+        """Join pairs from per-probe as-of matches (INNER drops unmatched rows)."""
+        out = P()
+        self._check(lib().qe_gpu_asof_pairs(self.h, match.p, n, join_type, C.byref(out)))
+        return Pairs(self, out)
+
     def take(self, cols, idx: "DevBuffer", m: int) -> Table:
         """Rows idx[0..m) (a device u32 buffer) of `cols` as a new table."""
         out = P()

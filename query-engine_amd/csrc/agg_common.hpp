@@ -256,9 +256,11 @@ __device__ inline void global_update(uint64_t* st, uint64_t S, const AggProg& p,
 
 // Host launchers (agg_common.hip).
 void init_state(qe_gpu_ctx* ctx, uint64_t* st, uint64_t S, const AggProg& p);
-// Merge nblocks partials into st in block order (deterministic).
+// Merge nblocks partials of S slots into st in block order (deterministic). The
+// partials may cover slots [out_lo, out_lo + S) of a state laid out for out_S
+// slots (out_S = 0: st is laid out for S slots).
 void merge_partials(qe_gpu_ctx* ctx, const uint64_t* part, uint32_t nblocks, uint64_t* st,
-                    uint64_t S, const AggProg& p);
+                    uint64_t S, const AggProg& p, uint64_t out_S = 0, uint64_t out_lo = 0);
 
 // Key source for emission.
 struct KeyEmit {

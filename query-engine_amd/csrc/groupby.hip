@@ -135,8 +135,9 @@ __device__ inline uint64_t combine(int kind, uint64_t a, uint64_t b) {
 // blocks of a chunk are folded in block order, then the chunk results in chunk
 // order, so the FP64 result is reproducible run to run.
 __global__ void merge_partials_kernel(const uint64_t* part, uint32_t nblocks, uint32_t chunk, uint64_t* out,
-                                      uint64_t S, AggProg p) {
+                                      uint64_t S, AggProg p, uint64_t out_S, uint64_t out_lo) {
     const uint64_t W = state_words(S, p);
+    const uint64_t Wout = state_words(out_S, p);
     const uint32_t nchunks = nblocks ? (nblocks + chunk - 1) / chunk : 1;  // 0 blocks: identities
     for (uint64_t t = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; t < W * nchunks;
          t += uint64_t(gridDim.x) * blockDim.x) {
@@ -153,24 +154,25 @@ __global__ void merge_partials_kernel(const uint64_t* part, uint32_t nblocks, ui
             v = acc_identity(kind);
             for (uint32_t b = b0; b < b1; ++b) v = combine(kind, v, part[size_t(b) * W + i]);
         }
-        out[size_t(c) * W + i] = v;
+        out[size_t(c) * Wout + field * out_S + out_lo + (i - field * S)] = v;
     }
 }
 
 void merge_partials(qe_gpu_ctx* ctx, const uint64_t* part, uint32_t nblocks, uint64_t* st,
-                    uint64_t S, const AggProg& p) {
+                    uint64_t S, const AggProg& p, uint64_t out_S, uint64_t out_lo) {
     const uint64_t W = state_words(S, p);
+    if (out_S == 0) out_S = S;
     constexpr uint32_t kChunk = 16;
     if (nblocks > 2 * kChunk) {
         const uint32_t nchunks = (nblocks + kChunk - 1) / kChunk;
         Scratch mid(ctx, size_t(nchunks) * W * 8);
         merge_partials_kernel<<<grid_for(W * nchunks, 256, 16384), 256, 0, ctx->stream>>>(
-            part, nblocks, kChunk, mid.as<uint64_t>(), S, p);
+            part, nblocks, kChunk, mid.as<uint64_t>(), S, p, S, 0);
         merge_partials_kernel<<<grid_for(W, 256, 16384), 256, 0, ctx->stream>>>(mid.as<uint64_t>(), nchunks,
-                                                                                  nchunks, st, S, p);
+                                                                                  nchunks, st, S, p, out_S, out_lo);
     } else {
-        merge_partials_kernel<<<grid_for(W, 256, 16384), 256, 0, ctx->stream>>>(part, nblocks,
-                                                                                  nblocks ? nblocks : 1, st, S, p);
+        merge_partials_kernel<<<grid_for(W, 256, 16384), 256, 0, ctx->stream>>>(
+            part, nblocks, nblocks ? nblocks : 1, st, S, p, out_S, out_lo);
     }
     QE_HIP(hipGetLastError());
 }

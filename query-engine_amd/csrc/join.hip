@@ -381,7 +381,8 @@ template <typename KeyT, typename MapT>
 __global__ void __launch_bounds__(1024) join_agg_direct_lds(const KeyT* __restrict__ keys,
                                                             const uint64_t* key_valid, uint64_t n,
                                                             const MapT* __restrict__ gmap, int64_t kmin,
-                                                            uint64_t range, uint32_t S, AggProg p, uint64_t* part) {
+                                                            uint64_t range, uint32_t S, uint32_t slot_lo, AggProg p,
+                                                            uint64_t* part) {
     extern __shared__ __align__(16) char smem[];
     lds_init(smem, S, p);
     __syncthreads();
@@ -409,9 +410,10 @@ __global__ void __launch_bounds__(1024) join_agg_direct_lds(const KeyT* __restri
         }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            if (g[j] == kEmpty) continue;
+            const uint32_t sl = static_cast<uint32_t>(g[j]) - slot_lo;
+            if (g[j] == kEmpty || sl >= S) continue;
             for (int a = 0; a < p.n; ++a) vrow[a] = r0 + j;
-            lds_update(smem, S, p, g[j], vrow);
+            lds_update(smem, S, p, sl, vrow);
         }
     }
     __syncthreads();
@@ -470,8 +472,10 @@ __device__ inline void load_pair(const void* base, bool wide, uint64_t pr, uint6
 template <typename KeyT, typename MapT, int PAIRS, int NV>
 __global__ void __launch_bounds__(1024) join_agg_stream(const KeyT* __restrict__ keys, const uint64_t* key_valid,
                                                         uint64_t n, const MapT* __restrict__ gmap, int64_t kmin,
-                                                        uint64_t range, uint32_t S, AggProg p, StreamCols sc,
-                                                        uint64_t* part) {
+                                                        uint64_t range, uint32_t S, uint32_t slot_lo, AggProg p,
+                                                        StreamCols sc, uint64_t* part) {
+    // S slots of LDS partials for slots [slot_lo, slot_lo + S) of the map's
+    // slot space (several launches cover a slot space larger than LDS).
     extern __shared__ __align__(16) char smem[];
     lds_init(smem, S, p);
     __syncthreads();
@@ -510,7 +514,8 @@ __global__ void __launch_bounds__(1024) join_agg_stream(const KeyT* __restrict__
             const uint64_t pr = t0 + uint64_t(i) * blockDim.x + threadIdx.x;
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
-                if (g[2 * i + e] == kEmpty) continue;
+                const uint32_t sl = static_cast<uint32_t>(g[2 * i + e]) - slot_lo;
+                if (g[2 * i + e] == kEmpty || sl >= S) continue;
                 const uint64_t r = 2 * pr + e;
                 uint64_t bits[NV > 0 ? NV : 1];
                 uint32_t valid = 0;
@@ -520,7 +525,7 @@ __global__ void __launch_bounds__(1024) join_agg_stream(const KeyT* __restrict__
                     const bool v = sc.validity[c] == nullptr || ((sc.validity[c][r >> 6] >> (r & 63)) & 1ull);
                     valid |= static_cast<uint32_t>(v) << c;
                 }
-                lds_update_vals(smem, S, p, static_cast<uint32_t>(g[2 * i + e]), sc.vcol, bits, valid);
+                lds_update_vals(smem, S, p, sl, sc.vcol, bits, valid);
             }
         }
     }
@@ -530,7 +535,7 @@ __global__ void __launch_bounds__(1024) join_agg_stream(const KeyT* __restrict__
         bool ok = off < range;
         if (key_valid) ok = ok && ((key_valid[r >> 6] >> (r & 63)) & 1ull);
         const MapT gg = ok ? gmap[off] : kEmpty;
-        if (gg != kEmpty) {
+        if (gg != kEmpty && static_cast<uint32_t>(gg) - slot_lo < S) {
             uint64_t bits[NV > 0 ? NV : 1];
             uint32_t valid = 0;
 #pragma unroll
@@ -540,7 +545,7 @@ __global__ void __launch_bounds__(1024) join_agg_stream(const KeyT* __restrict__
                 const bool v = sc.validity[c] == nullptr || ((sc.validity[c][r >> 6] >> (r & 63)) & 1ull);
                 valid |= static_cast<uint32_t>(v) << c;
             }
-            lds_update_vals(smem, S, p, static_cast<uint32_t>(gg), sc.vcol, bits, valid);
+            lds_update_vals(smem, S, p, static_cast<uint32_t>(gg) - slot_lo, sc.vcol, bits, valid);
         }
     }
     __syncthreads();
@@ -833,20 +838,20 @@ bool stream_cols(const AggProg& p, const void* keys, size_t key_width, StreamCol
 
 template <typename KeyT, typename MapT, int PAIRS>
 void launch_stream_p(qe_gpu_ctx* ctx, int nv, unsigned blocks, size_t lb, const KeyT* keys, const uint64_t* kv,
-                     uint64_t n, const MapT* gm, int64_t kmin, uint64_t range, uint32_t S, const AggProg& p,
-                     const StreamCols& sc, uint64_t* part) {
+                     uint64_t n, const MapT* gm, int64_t kmin, uint64_t range, uint32_t S, uint32_t slot_lo,
+                     const AggProg& p, const StreamCols& sc, uint64_t* part) {
     switch (nv) {
         case 0:
             join_agg_stream<KeyT, MapT, PAIRS, 0><<<blocks, 1024, lb, ctx->stream>>>(keys, kv, n, gm, kmin, range, S,
-                                                                                     p, sc, part);
+                                                                                     slot_lo, p, sc, part);
             break;
         case 1:
             join_agg_stream<KeyT, MapT, PAIRS, 1><<<blocks, 1024, lb, ctx->stream>>>(keys, kv, n, gm, kmin, range, S,
-                                                                                     p, sc, part);
+                                                                                     slot_lo, p, sc, part);
             break;
         default:
             join_agg_stream<KeyT, MapT, PAIRS, 2><<<blocks, 1024, lb, ctx->stream>>>(keys, kv, n, gm, kmin, range, S,
-                                                                                     p, sc, part);
+                                                                                     slot_lo, p, sc, part);
             break;
     }
     QE_HIP(hipGetLastError());
@@ -856,11 +861,11 @@ void launch_stream_p(qe_gpu_ctx* ctx, int nv, unsigned blocks, size_t lb, const 
 template <typename KeyT, typename MapT>
 void launch_stream(qe_gpu_ctx* ctx, int pairs, int nv, unsigned blocks, size_t lb, const KeyT* keys,
                    const uint64_t* kv, uint64_t n, const MapT* gm, int64_t kmin, uint64_t range, uint32_t S,
-                   const AggProg& p, const StreamCols& sc, uint64_t* part) {
+                   uint32_t slot_lo, const AggProg& p, const StreamCols& sc, uint64_t* part) {
     if (pairs <= 2)
-        launch_stream_p<KeyT, MapT, 2>(ctx, nv, blocks, lb, keys, kv, n, gm, kmin, range, S, p, sc, part);
+        launch_stream_p<KeyT, MapT, 2>(ctx, nv, blocks, lb, keys, kv, n, gm, kmin, range, S, slot_lo, p, sc, part);
     else
-        launch_stream_p<KeyT, MapT, 4>(ctx, nv, blocks, lb, keys, kv, n, gm, kmin, range, S, p, sc, part);
+        launch_stream_p<KeyT, MapT, 4>(ctx, nv, blocks, lb, keys, kv, n, gm, kmin, range, S, slot_lo, p, sc, part);
 }
 
 }  // namespace qe_gpu
@@ -905,6 +910,9 @@ qe_gpu_status qe_gpu_pairs_gather(qe_gpu_ctx* ctx, const qe_dev_pairs* p, uint64
 }  // extern "C"
 
 namespace qe_gpu {
+
+// Most LDS slot-range passes over the probe before HBM atomics are used.
+constexpr uint64_t kMaxSlotPasses = 32;
 
 // The fused join -> group-by over prepared columns (no STR rewriting).
 qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* probe_cols, uint32_t nprobe,
@@ -1024,6 +1032,12 @@ qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* pro
         Scratch st(ctx, state_words(S, p) * 8);
         const size_t lb = lds_bytes(static_cast<uint32_t>(std::min<uint64_t>(S, UINT32_MAX)), p);
         const bool use_lds = S < (uint64_t(1) << 20) && lb <= kLdsBudget;
+        // A slot space larger than LDS: LDS partials for one slot range per
+        // pass over the probe (each pass streams the probe columns again),
+        // instead of HBM atomics that serialise on the heavy groups.
+        const uint64_t pass_slots = std::max<uint64_t>(1, kLdsBudget / lds_bytes(1, p));
+        const uint64_t npasses = (S + pass_slots - 1) / pass_slots;
+        const bool multipass = !use_lds && npasses <= kMaxSlotPasses;
         const unsigned lds_blocks = static_cast<unsigned>(
             std::max<uint64_t>(1, std::min<uint64_t>(ctx->num_cus, (np + 1023) / 1024)));
         const bool small = S < 0xFFFF;
@@ -1039,25 +1053,28 @@ qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* pro
                 const MapT* gm = static_cast<const MapT*>(gmap.p);
                 StreamCols sc{};
                 int nv = 0;
-                const bool streamable = use_lds && stream_cols(p, keys, sizeof(KeyT), sc, nv);
-                if (streamable) {
-                    Scratch part(ctx, size_t(lds_blocks) * part_words(static_cast<uint32_t>(S), p) * 8);
-                    {
-                        TimedRegion tr(ctx, "join_agg_probe");
-                        launch_stream<KeyT, MapT>(ctx, ctx->tune_join_pairs, nv, lds_blocks, lb, keys, kv, np, gm,
-                                                  kmin, range, static_cast<uint32_t>(S), p, sc,
-                                                  part.as<uint64_t>());
+                const bool streamable = (use_lds || multipass) && stream_cols(p, keys, sizeof(KeyT), sc, nv);
+                if (use_lds || multipass) {
+                    const uint64_t per = use_lds ? S : pass_slots;
+                    Scratch part(ctx, size_t(lds_blocks) * part_words(static_cast<uint32_t>(per), p) * 8);
+                    for (uint64_t lo = 0; lo < S; lo += per) {
+                        const uint32_t Sp = static_cast<uint32_t>(std::min<uint64_t>(per, S - lo));
+                        const size_t lbp = lds_bytes(Sp, p);
+                        {
+                            TimedRegion tr(ctx, "join_agg_probe");
+                            if (streamable) {
+                                launch_stream<KeyT, MapT>(ctx, ctx->tune_join_pairs, nv, lds_blocks, lbp, keys, kv, np,
+                                                          gm, kmin, range, Sp, static_cast<uint32_t>(lo), p, sc,
+                                                          part.as<uint64_t>());
+                            } else {
+                                join_agg_direct_lds<KeyT, MapT><<<lds_blocks, 1024, lbp, ctx->stream>>>(
+                                    keys, kv, np, gm, kmin, range, Sp, static_cast<uint32_t>(lo), p,
+                                    part.as<uint64_t>());
+                                QE_HIP(hipGetLastError());
+                            }
+                        }
+                        merge_partials(ctx, part.as<uint64_t>(), lds_blocks, st.as<uint64_t>(), Sp, p, S, lo);
                     }
-                    merge_partials(ctx, part.as<uint64_t>(), lds_blocks, st.as<uint64_t>(), S, p);
-                } else if (use_lds) {
-                    Scratch part(ctx, size_t(lds_blocks) * part_words(static_cast<uint32_t>(S), p) * 8);
-                    {
-                        TimedRegion tr(ctx, "join_agg_probe");
-                        join_agg_direct_lds<KeyT, MapT><<<lds_blocks, 1024, lb, ctx->stream>>>(
-                            keys, kv, np, gm, kmin, range, static_cast<uint32_t>(S), p, part.as<uint64_t>());
-                        QE_HIP(hipGetLastError());
-                    }
-                    merge_partials(ctx, part.as<uint64_t>(), lds_blocks, st.as<uint64_t>(), S, p);
                 } else {
                     init_state(ctx, st.as<uint64_t>(), S, p);
                     TimedRegion tr(ctx, "join_agg_probe");

@@ -241,3 +241,26 @@ def test_xcd_of_blocks_reports_dies(ctx):
           "round-robin:", bool((ids == np.arange(256) % 8).all()))
     assert ids.max() < 8
     assert len(set(ids.tolist())) == 8
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_fused_many_groups_slot_range_passes(ctx, case):
+    # More groups than one block's LDS holds (the tick pipeline's 50,000
+    # symbol names): LDS partials per slot range, one probe pass per range,
+    # each merged into its range of the result. Zipf-skewed probe keys put
+    # heavy groups in every range; streamed (<= 2 value columns) and per-row
+    # (3 value columns) probe kernels; I32 and I64 keys.
+    r = np.random.default_rng(1000 + case)
+    kt = I32 if case % 2 else I64
+    n_b = 60000
+    groups = [50000, 25000, 60000, 12000][case]
+    n_p = [400000, 1000003, 200000, 300001][case]
+    keys = r.permutation(n_b)
+    zipf = np.minimum(r.zipf(1.3, n_p) - 1, n_b + 99)
+    probe_keys = np.where(r.random(n_p) < 0.1, n_b + 5, keys[zipf % n_b])
+    bcols = [col(keys, kt), col(r.integers(0, groups, n_b), I32, tuple(np.nonzero(r.random(n_b) < 0.01)[0]))]
+    pcols = [col(probe_keys, kt), col(np.round(r.uniform(0, 200, n_p), 2), F64,
+                                      tuple(np.nonzero(r.random(n_p) < 0.02)[0])),
+             col(r.integers(-1000, 1000, n_p), I32), col(r.integers(-2**40, 2**40, n_p), I64)]
+    aggs = [(CS, 0), (AVG, 1)] if case < 2 else [(CS, 0), (SUM, 1), (MIN, 2), (MAX, 3), (CNT, 1)]
+    join_agg_case(ctx, pcols, bcols, [0], [0], INNER, [5], aggs)
