@@ -1,6 +1,7 @@
 """Benchmark: hash join + group-by on one MI355X (the headline workload), with
-the as-of join (binary search vs learned index) and the 1M-row group-by
-reported beside it.
+the as-of join (binary search vs learned index), the 1M-row group-by, the
+trade-table filter, the full tick pipeline and the batched group-by reported
+beside it.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--skip-cpu] [--quick]
 
@@ -593,6 +594,10 @@ def main():
         f = secondary["filter"]
         log(f"filter: {f['ms_per_step']:.3f} ms/step ({f['rows_per_s']:.3e} rows/s), mask {f['pred_mask_gbs']:.0f} GB/s, "
             f"compact {f['compact_gbs']:.0f} GB/s")
+        secondary["tick_pipeline"] = tick_workload(qe, ctx, args)
+        tk = secondary["tick_pipeline"]
+        log(f"tick pipeline: {tk['ms_per_step']:.1f} ms/step ({tk['trade_rows_per_s']:.3e} trade rows/s), "
+            f"stages {', '.join(f'{k} {v:.1f}' for k, v in tk['stage_ms'].items())} ms")
         secondary["batched"] = batched_workload(qe, ctx, args)
         b = secondary["batched"]
         log(f"batched: graph {b['graph_s'] * 1e3:.2f} ms, per-batch {b['per_batch_s'] * 1e3:.1f} ms "
