@@ -84,12 +84,21 @@ def join_groupby_workload(qe, ctx, args):
     ctx.sync()
     t3 = time.perf_counter()
     ms_step = (t3 - t2) * 1e3 / args.steps
+    # Step-time distribution over separate repetitions (each step ends with the
+    # result on the host, so each is timed on its own).
+    reps = []
+    for _ in range(max(20, args.steps)):
+        ta = time.perf_counter()
+        step()
+        reps.append((time.perf_counter() - ta) * 1e3)
     kernel_ms = kms / max(launches, 1)
     probe_bytes = probe_n * 16
     step_bytes = probe_n * 16 + build_n * 12 + groups * 20
     res = {
         "ms_per_step": ms_step,
         "ms_per_step_with_events": (t1 - t0) * 1e3 / args.steps,
+        "step_ms_p50": float(np.percentile(reps, 50)), "step_ms_p99": float(np.percentile(reps, 99)),
+        "step_reps": len(reps),
         "probe_rows": probe_n, "build_rows": build_n, "groups_out": int(nrows),
         "kernel": "join_agg_stream (direct-map fused probe)",
         "kernel_ms": kernel_ms,
@@ -630,6 +639,7 @@ def main():
             "algorithmic_bytes_per_launch": res["probe_bytes"],
             "measured_copy_ceiling_gbs": ceiling["gbs"],
         },
+        "step_latency_ms": {"p50": res["step_ms_p50"], "p99": res["step_ms_p99"], "reps": res["step_reps"]},
         "step_roofline": {"achieved": res["step_achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": res["step_achieved_gbs"] / HBM_PEAK_GBS,
                           "algorithmic_bytes_per_step": res["step_bytes"]},

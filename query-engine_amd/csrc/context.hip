@@ -150,6 +150,66 @@ __global__ void pack_validity_kernel(const uint8_t* vb, uint64_t n, uint64_t* wo
     if (lane_id() == 0 && local) atomicAdd(nulls, static_cast<unsigned long long>(local));
 }
 
+// Several columns at once: one launch, one read-back of the NULL counts.
+struct PackSet {
+    int n;
+    uint64_t len;
+    const uint8_t* vb[kMaxPackCols];
+    uint64_t* words[kMaxPackCols];
+};
+
+__global__ void pack_validity_many_kernel(PackSet ps, unsigned long long* nulls) {
+    const uint64_t nw = words_for(ps.len);
+    for (uint64_t t = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; t < nw * ps.n;
+         t += uint64_t(gridDim.x) * blockDim.x) {
+        const int c = static_cast<int>(t / nw);
+        const uint64_t w = t - uint64_t(c) * nw;
+        uint64_t bits = 0;
+        unsigned long long local = 0;
+        const uint64_t base = w * 64;
+        for (unsigned b = 0; b < 64; ++b) {
+            const uint64_t r = base + b;
+            const bool v = (r >= ps.len) || ps.vb[c][r];
+            bits |= (uint64_t(v) << b);
+            local += (r < ps.len && !ps.vb[c][r]) ? 1 : 0;
+        }
+        ps.words[c][w] = bits;
+        if (local) atomicAdd(nulls + c, local);
+    }
+}
+
+void pack_validity_many(qe_gpu_ctx* ctx, qe_dev_col* const* cols, const uint8_t* const* valid_bytes, int ncol) {
+    check(ncol <= kMaxPackCols, QE_GPU_UNSUPPORTED, "too many columns to pack");
+    const uint64_t len = ncol ? cols[0]->len : 0;
+    if (len == 0) {
+        for (int c = 0; c < ncol; ++c) pack_validity(ctx, cols[c], valid_bytes[c]);
+        return;
+    }
+    PackSet ps{};
+    ps.n = ncol;
+    ps.len = len;
+    for (int c = 0; c < ncol; ++c) {
+        invalid(cols[c]->len == len && cols[c]->validity != nullptr, "pack needs equal-length nullable columns");
+        ps.vb[c] = valid_bytes[c];
+        ps.words[c] = cols[c]->validity;
+    }
+    Scratch cnt(ctx, size_t(ncol) * 8);
+    memset_async(ctx, cnt.p, 0, size_t(ncol) * 8);
+    pack_validity_many_kernel<<<grid_for(words_for(len) * ncol, 64, 2048), 64, 0, ctx->stream>>>(
+        ps, cnt.as<unsigned long long>());
+    QE_HIP(hipGetLastError());
+    uint64_t* h = static_cast<uint64_t*>(pinned(ctx, size_t(ncol) * 8));
+    QE_HIP(hipMemcpyAsync(h, cnt.p, size_t(ncol) * 8, hipMemcpyDeviceToHost, ctx->stream));
+    QE_HIP(hipStreamSynchronize(ctx->stream));
+    for (int c = 0; c < ncol; ++c) {
+        if (h[c] == 0) {  // no NULL: drop the bitmap (all_valid)
+            dev_free(ctx, cols[c]->validity);
+            cols[c]->validity = nullptr;
+            cols[c]->all_valid = true;
+        }
+    }
+}
+
 void pack_validity(qe_gpu_ctx* ctx, qe_dev_col* col, const uint8_t* valid_bytes) {
     if (col->validity == nullptr) return;
     if (col->len == 0) {
@@ -418,15 +478,24 @@ __global__ void xcd_of_block_kernel(uint32_t* out) {
 }
 
 MinMax column_minmax(qe_gpu_ctx* ctx, const qe_dev_col* c, uint64_t n) {
+    MinMax mm;
+    column_minmax_n(ctx, &c, 1, n, &mm);
+    return mm;
+}
+
+void column_minmax_n(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, uint32_t ncol, uint64_t n, MinMax* out) {
     const uint32_t blocks = static_cast<uint32_t>(grid_for(n, 256, ctx->num_cus * 4));
-    Scratch part(ctx, size_t(blocks) * 3 * 8), s(ctx, 3 * 8);
-    minmax_kernel<<<blocks, 256, 0, ctx->stream>>>(view(c), n, part.as<int64_t>());
-    minmax_final<<<1, 256, 0, ctx->stream>>>(part.as<int64_t>(), blocks, s.as<int64_t>());
+    Scratch part(ctx, size_t(blocks) * 3 * 8 * ncol), s(ctx, size_t(3) * 8 * ncol);
+    for (uint32_t c = 0; c < ncol; ++c) {
+        minmax_kernel<<<blocks, 256, 0, ctx->stream>>>(view(cols[c]), n, part.as<int64_t>() + size_t(c) * blocks * 3);
+        minmax_final<<<1, 256, 0, ctx->stream>>>(part.as<int64_t>() + size_t(c) * blocks * 3, blocks,
+                                                 s.as<int64_t>() + 3 * c);
+    }
     QE_HIP(hipGetLastError());
-    int64_t* h = static_cast<int64_t*>(pinned(ctx, 24));
-    QE_HIP(hipMemcpyAsync(h, s.p, 24, hipMemcpyDeviceToHost, ctx->stream));
+    int64_t* h = static_cast<int64_t*>(pinned(ctx, size_t(24) * ncol));
+    QE_HIP(hipMemcpyAsync(h, s.p, size_t(24) * ncol, hipMemcpyDeviceToHost, ctx->stream));
     QE_HIP(hipStreamSynchronize(ctx->stream));
-    return MinMax{h[0], h[1], static_cast<uint64_t>(h[2])};
+    for (uint32_t c = 0; c < ncol; ++c) out[c] = MinMax{h[3 * c], h[3 * c + 1], static_cast<uint64_t>(h[3 * c + 2])};
 }
 
 }  // namespace qe_gpu

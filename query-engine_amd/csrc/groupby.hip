@@ -289,7 +289,9 @@ qe_dev_table* emit_groups(qe_gpu_ctx* ctx, const uint64_t* st, uint64_t S, const
                                                                           offs.as<uint64_t>(), oc);
             QE_HIP(hipGetLastError());
         }
-        for (int c = 0; c < ncol; ++c) pack_validity(ctx, t->cols[c], vb[c].as<uint8_t>());
+        std::vector<const uint8_t*> vbp(ncol);
+        for (int c = 0; c < ncol; ++c) vbp[c] = vb[c].as<uint8_t>();
+        pack_validity_many(ctx, t->cols.data(), vbp.data(), ncol);
     } catch (...) {
         qe_gpu_table_free(ctx, t);
         throw;

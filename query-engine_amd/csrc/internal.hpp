@@ -338,6 +338,8 @@ struct MinMax {
     uint64_t valid;  // number of valid rows
 };
 MinMax column_minmax(qe_gpu_ctx* ctx, const qe_dev_col* c, uint64_t n);
+// Several columns of length n, one read-back.
+void column_minmax_n(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, uint32_t ncol, uint64_t n, MinMax* out);
 
 // Stable LSD radix sort of (u64 key, u32 value) pairs on bits [begin_bit,
 // end_bit), in place (sort.hip).
@@ -356,6 +358,9 @@ qe_dev_table* new_table(uint64_t rows);
 // apply the all_valid rule (core/column.h:42-54): a column with no NULL drops
 // its bitmap. The column must have been allocated nullable.
 void pack_validity(qe_gpu_ctx* ctx, qe_dev_col* col, const uint8_t* valid_bytes);
+// The same for up to kMaxPackCols columns of one length: one launch, one read-back.
+constexpr int kMaxPackCols = 32;
+void pack_validity_many(qe_gpu_ctx* ctx, qe_dev_col* const* cols, const uint8_t* const* valid_bytes, int ncol);
 
 }  // namespace qe_gpu
 

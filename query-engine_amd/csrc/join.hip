@@ -770,20 +770,24 @@ __global__ void __launch_bounds__(1024) map_part_build(const EntT* ent, const ui
 
 enum class MapBuild { kOk, kDuplicate, kTooWide };
 
+// `total` is the number of valid build keys (all within [kmin, kmin + range)).
+// The duplicate-key flag is left in `dup` (device) for the caller to read once
+// the probe is queued, so no host round trip sits between build and probe.
 template <typename MapT>
-MapBuild build_direct_map(qe_gpu_ctx* ctx, const qe_dev_col* key, uint64_t nb, int64_t kmin, uint64_t range,
-                          const uint32_t* gslot, Scratch& gmap) {
+MapBuild build_direct_map(qe_gpu_ctx* ctx, const qe_dev_col* key, uint64_t nb, uint64_t total, int64_t kmin,
+                          uint64_t range, const uint32_t* gslot, Scratch& gmap, Scratch& dup) {
     const uint32_t pb = sizeof(MapT) == 2 ? 16 : 15;  // 128 KiB of map per partition in LDS
     const uint64_t np64 = (range + (uint64_t(1) << pb) - 1) >> pb;
     if (np64 == 0 || np64 > kMaxMapParts) return MapBuild::kTooWide;
     const uint32_t nparts = static_cast<uint32_t>(np64);
     const uint32_t ntiles = static_cast<uint32_t>((nb + kMapTile - 1) / kMapTile);
     const uint64_t nh = uint64_t(nparts) * ntiles;
-    Scratch hist(ctx, nh * 4), offs(ctx, nh * 8), dup(ctx, 4);
+    Scratch hist(ctx, nh * 4), offs(ctx, nh * 8), sums(ctx, scan_sums_words(nh) * 8);
+    dup = Scratch(ctx, 4);
     const ColView kv = view(key);
     map_part_hist<<<ntiles, 1024, nparts * 4, ctx->stream>>>(kv, nb, kmin, pb, nparts, ntiles, hist.as<uint32_t>());
     QE_HIP(hipGetLastError());
-    const uint64_t total = exclusive_scan_u32(ctx, hist.as<uint32_t>(), offs.as<uint64_t>(), nh);
+    exclusive_scan_u32_dev(ctx, hist.as<uint32_t>(), offs.as<uint64_t>(), nh, sums.as<uint64_t>());
     constexpr bool k16 = sizeof(MapT) == 2;  // 4-byte entries, LDS-sorted scatter
     Scratch ent(ctx, std::max<uint64_t>(total, 1) * (k16 ? 4 : 8));
     if (k16)
@@ -806,7 +810,7 @@ MapBuild build_direct_map(qe_gpu_ctx* ctx, const qe_dev_col* key, uint64_t nb, i
             ent.as<uint64_t>(), offs.as<uint64_t>(), ntiles, nparts, total, pb, range, gmap.as<MapT>(),
             dup.as<unsigned int>());
     QE_HIP(hipGetLastError());
-    return read_scalar(ctx, dup.as<unsigned int>()) ? MapBuild::kDuplicate : MapBuild::kOk;
+    return MapBuild::kOk;
 }
 
 // Value columns of a program for the streaming probe: I32/I64/TS/F64 inputs,
@@ -999,8 +1003,23 @@ qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* pro
         uint64_t S = 0;
         const qe_dev_col* g0 = col_at(group_keys[0]);
         bool gdirect = false;
-        if (ngk == 1 && is_int_like(g0->type) && nb > 0) {
-            const MinMax mm = column_minmax(ctx, g0, nb);
+        const bool int_key = nkeys == 1 && (pk[0]->type == QE_I64 || pk[0]->type == QE_TS || pk[0]->type == QE_I32);
+        // Group-key and build-key ranges in one pass and one read-back.
+        const bool want_g = ngk == 1 && is_int_like(g0->type) && nb > 0;
+        const bool want_k = int_key && nb > 0 && np > 0;
+        MinMax g_mm{}, k_mm{};
+        {
+            const qe_dev_col* mc[2];
+            uint32_t nm = 0;
+            if (want_g) mc[nm++] = g0;
+            if (want_k) mc[nm++] = bk[0];
+            MinMax r[2];
+            if (nm) column_minmax_n(ctx, mc, nm, nb, r);
+            if (want_g) g_mm = r[0];
+            if (want_k) k_mm = r[want_g ? 1 : 0];
+        }
+        if (want_g) {
+            const MinMax mm = g_mm;
             const uint64_t R = mm.valid ? static_cast<uint64_t>(mm.mx) - static_cast<uint64_t>(mm.mn) + 1 : 0;
             if (R <= std::max<uint64_t>(2 * nb, 1024) && R < (uint64_t(1) << 30)) {
                 gdirect = true;
@@ -1042,7 +1061,6 @@ qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* pro
             std::max<uint64_t>(1, std::min<uint64_t>(ctx->num_cus, (np + 1023) / 1024)));
         const bool small = S < 0xFFFF;
         const uint64_t* kv = pk[0]->all_valid ? nullptr : pk[0]->validity;
-        const bool int_key = nkeys == 1 && (pk[0]->type == QE_I64 || pk[0]->type == QE_TS || pk[0]->type == QE_I32);
 
         // Probe over a direct unique map (gmap[key - kmin] = group slot).
         auto probe_direct = [&](const Scratch& gmap, int64_t kmin, uint64_t range) {
@@ -1094,21 +1112,25 @@ qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* pro
 
         // Dense unique integer keys: partitioned map build, then the streaming probe.
         bool done = false;
-        if (int_key && nb > 0 && np > 0) {
-            const MinMax mm = column_minmax(ctx, bk[0], nb);
+        if (want_k) {
+            const MinMax mm = k_mm;
             if (mm.valid == 0) {  // every build key NULL: nothing matches
                 init_state(ctx, st.as<uint64_t>(), S, p);
                 done = true;
             } else {
                 const uint64_t R = static_cast<uint64_t>(mm.mx) - static_cast<uint64_t>(mm.mn) + 1;
                 if (R <= std::max<uint64_t>(4 * nb, uint64_t(1) << 16) && R < (uint64_t(1) << 31)) {
-                    Scratch gmap;
+                    Scratch gmap, dup;
                     const MapBuild mbld =
-                        small ? build_direct_map<uint16_t>(ctx, bk[0], nb, mm.mn, R, gslot.as<uint32_t>(), gmap)
-                              : build_direct_map<uint32_t>(ctx, bk[0], nb, mm.mn, R, gslot.as<uint32_t>(), gmap);
+                        small ? build_direct_map<uint16_t>(ctx, bk[0], nb, mm.valid, mm.mn, R, gslot.as<uint32_t>(),
+                                                           gmap, dup)
+                              : build_direct_map<uint32_t>(ctx, bk[0], nb, mm.valid, mm.mn, R, gslot.as<uint32_t>(),
+                                                           gmap, dup);
                     if (mbld == MapBuild::kOk) {
                         probe_direct(gmap, mm.mn, R);
-                        done = true;
+                        // Repeated build keys make the map wrong: then the
+                        // partials are discarded and the row-list path runs.
+                        done = read_scalar(ctx, dup.as<unsigned int>()) == 0;
                     }
                 }
             }
