@@ -387,16 +387,16 @@ __global__ void __launch_bounds__(256) minmax_kernel(ColView c, uint64_t n, int6
     uint64_t cnt = 0;
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
     uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
-    for (; i + 3 * stride < n; i += 4 * stride) {  // four independent loads in flight
-        int64_t v[4];
-        bool ok[4];
+    for (; i + 7 * stride < n; i += 8 * stride) {  // eight independent loads in flight
+        int64_t v[8];
+        bool ok[8];
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < 8; ++k) {
             ok[k] = is_valid(c, i + k * stride);
             v[k] = int_value(c, i + k * stride);
         }
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < 8; ++k) {
             if (!ok[k]) continue;
             mn = v[k] < mn ? v[k] : mn;
             mx = v[k] > mx ? v[k] : mx;
@@ -484,7 +484,8 @@ MinMax column_minmax(qe_gpu_ctx* ctx, const qe_dev_col* c, uint64_t n) {
 }
 
 void column_minmax_n(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, uint32_t ncol, uint64_t n, MinMax* out) {
-    const uint32_t blocks = static_cast<uint32_t>(grid_for(n, 256, ctx->num_cus * 4));
+    // About 8-16 rows per thread: one or two batches of eight independent loads.
+    const uint32_t blocks = static_cast<uint32_t>(grid_for((n + 7) / 8, 256, ctx->num_cus * 16));
     Scratch part(ctx, size_t(blocks) * 3 * 8 * ncol), s(ctx, size_t(3) * 8 * ncol);
     for (uint32_t c = 0; c < ncol; ++c) {
         minmax_kernel<<<blocks, 256, 0, ctx->stream>>>(view(cols[c]), n, part.as<int64_t>() + size_t(c) * blocks * 3);
