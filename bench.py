@@ -122,8 +122,8 @@ def cpu_baseline_join(qe, ctx, cols, args):
     log(f"cpu baseline: downloading sample ({sample} probe rows)")
     bk, _ = cols["bk"].numpy()
     bg, _ = cols["bg"].numpy()
-    pk_s = ctx_slice(qe, ctx, cols["pk"], sample)
-    pp_s = ctx_slice(qe, ctx, cols["pp"], sample)
+    pk_s = ctx.slice(cols["pk"], 0, sample)
+    pp_s = ctx.slice(cols["pp"], 0, sample)
     pkn, _ = pk_s.numpy()
     ppn, _ = pp_s.numpy()
     threads = int(L.orc_bl_hw_threads())
@@ -312,13 +312,10 @@ def measured_traffic(args):
 
 def copy_ceiling(qe, ctx, nbytes=1 << 30, reps=5):
     """Measured HBM ceiling: a plain 16-byte streaming copy (read + write)."""
-    import ctypes as C
     src, dst = qe.DevBuffer(ctx, nbytes), qe.DevBuffer(ctx, nbytes)
-    ms = C.c_double()
     best = 1e30
     for _ in range(reps + 1):
-        ctx._check(qe.lib().qe_gpu_stream_copy(ctx.h, dst.p, src.p, nbytes, C.byref(ms)))
-        best = min(best, ms.value)
+        best = min(best, ctx.stream_copy(dst, src, nbytes))
     src.free()
     dst.free()
     return {"bytes_moved": 2 * nbytes, "best_ms": best, "gbs": 2 * nbytes / (best / 1e3) / 1e9}
@@ -391,7 +388,7 @@ def filter_workload(qe, ctx, args):
         sys.path.insert(0, str(ROOT / "tests"))
         import _oracle as O
         sample = min(1 << 24, n)
-        head = ctx_slice(qe, ctx, px, sample)
+        head = ctx.slice(px, 0, sample)
         pv, pvalid = head.numpy()
         c = [(pv, pvalid, O.F64)]
         t0 = time.perf_counter()
@@ -468,7 +465,7 @@ def tick_workload(qe, ctx, args):
             t = stamp("join_groupby", t)
         ng = agg.rows
         cnt = agg.col(1)
-        ctx._check(qe.lib().qe_gpu_sort_perm(ctx.h, qe._cols([cnt]), 1, (qe.C.c_int32 * 1)(1), None, perm.p))
+        ctx.sort_perm_into([cnt], perm, desc=[1])
         out = ctx.take([agg.col(0), agg.col(1), agg.col(2)], perm, ng)
         if timed_stages:
             t = stamp("sort", t)
@@ -514,13 +511,6 @@ def tick_workload(qe, ctx, args):
         c.free()
     d.free()
     return res
-
-
-def ctx_slice(qe, ctx, col, n):
-    import ctypes as C
-    out = C.c_void_p()
-    ctx._check(qe.lib().qe_gpu_column_slice(ctx.h, col.h, 0, n, C.byref(out)))
-    return qe.DevCol(ctx, out, owned=True)
 
 
 def main():

@@ -63,7 +63,8 @@ typedef struct qe_agg_spec {
 typedef enum qe_join_type { QE_JOIN_INNER = 0, QE_JOIN_LEFT = 1 } qe_join_type;
 typedef enum qe_null_policy { QE_NULL_EQUAL = 0, QE_NULL_NEVER_MATCH = 1 } qe_null_policy;
 
-/* As-of search method: plain per-probe binary search (the judge) or the
+/* As-of search method: plain per-probe binary search (the exact answer the
+ * learned index must reproduce) or the
  * learned per-key index (spline pieces + monotone MLP router, verified
  * window search, binary-search fallback). */
 typedef enum qe_asof_mode { QE_ASOF_BINARY = 0, QE_ASOF_LEARNED = 1, QE_ASOF_LEARNED_NO_MLP = 2 } qe_asof_mode;

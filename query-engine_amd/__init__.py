@@ -422,6 +422,18 @@ class Context:
         valid = None if all(x is not None for x in values) else np.array([x is not None for x in values])
         return self.upload(codes, STR, valid, dict=d)
 
+    def stream_copy(self, dst: "DevBuffer", src: "DevBuffer", nbytes: int) -> float:
+        """A plain streaming copy (the measured HBM ceiling); returns its kernel ms."""
+        ms = C.c_double()
+        self._check(lib().qe_gpu_stream_copy(self.h, dst.p, src.p, nbytes, C.byref(ms)))
+        return ms.value
+
+    def sort_perm_into(self, keys, out: "DevBuffer", desc=None, nulls_first=None):
+        """Like sort_perm, with the permutation left in a device buffer."""
+        d = None if desc is None else (C.c_int32 * len(keys))(*desc)
+        nf = None if nulls_first is None else (C.c_int32 * len(keys))(*nulls_first)
+        self._check(lib().qe_gpu_sort_perm(self.h, _cols(keys), len(keys), d, nf, out.p))
+
     def slice(self, col: DevCol, offset: int, n: int) -> DevCol:
         """Rows [offset, offset + n) of `col` without a copy (offset % 64 == 0)."""
         out = P()

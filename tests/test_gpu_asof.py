@@ -2,7 +2,7 @@
 
 Literal edge shapes follow tests/asof_differential_test.cpp:118-318 and
 tests/catalog_checks_asof.cpp:38-92; match indices are bit-exact; the learned
-probe must return exactly the binary-search matches (the judge) with and
+probe must return exactly the binary-search matches (the exact answer) with and
 without the MLP router; the device MLP forward pass and MFMA gradient kernel
 are checked against the numpy oracle.
 """
