@@ -195,8 +195,24 @@ __device__ inline uint64_t upper_bound_count(const int64_t* ts, uint64_t lo, uin
     return first - lo;
 }
 
+// Partition functors for the probe kernels: the general key tuple, and the two
+// common shapes specialised so the kernels carry small arguments (fewer
+// scalar registers, more waves in flight for these latency-bound searches).
+struct ZeroKeyPart {
+    __device__ uint32_t part(uint64_t) const { return 0; }
+};
+struct DirectKeyPart {  // one integer key, partition = key - kmin
+    ColView key;
+    int64_t kmin;
+    uint64_t range;
+    __device__ uint32_t part(uint64_t r) const {
+        if (!is_valid(key, r)) return kNoRow;
+        const uint64_t off = static_cast<uint64_t>(int_value(key, r) - kmin);
+        return off < range ? static_cast<uint32_t>(off) : kNoRow;
+    }
+};
+
 struct ProbeArgs {
-    PartKeys pk;
     ColView time;
     uint64_t n;
     const int64_t* ts;
@@ -216,10 +232,11 @@ __device__ inline void finish_match(const ProbeArgs& a, uint64_t j, int64_t t, u
     a.out[j] = res;
 }
 
-__global__ void asof_binary_kernel(ProbeArgs a) {
+template <class PK>
+__global__ void asof_binary_kernel(PK pk, ProbeArgs a) {
     for (uint64_t j = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; j < a.n;
          j += uint64_t(gridDim.x) * blockDim.x) {
-        const uint32_t p = is_valid(a.time, j) ? a.pk.part(j) : kNoRow;
+        const uint32_t p = is_valid(a.time, j) ? pk.part(j) : kNoRow;
         if (p == kNoRow) {
             a.out[j] = kNoRow;
             continue;
@@ -459,11 +476,12 @@ __device__ inline uint32_t knot_search(const Knot* knots, uint64_t kb, uint32_t 
     return lo;
 }
 
-__global__ void asof_learned_kernel(ProbeArgs a, LearnedArgs L) {
+template <class PK>
+__global__ void asof_learned_kernel(PK pk, ProbeArgs a, LearnedArgs L) {
     uint32_t fb = 0;
     for (uint64_t j = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; j < a.n;
          j += uint64_t(gridDim.x) * blockDim.x) {
-        const uint32_t p = is_valid(a.time, j) ? a.pk.part(j) : kNoRow;
+        const uint32_t p = is_valid(a.time, j) ? pk.part(j) : kNoRow;
         if (p == kNoRow) {
             a.out[j] = kNoRow;
             continue;
@@ -1062,7 +1080,7 @@ static qe_gpu_status asof_probe_codes(qe_gpu_ctx* ctx, const qe_dev_asof* idx, c
         if (n == 0) return QE_GPU_OK;
         invalid(match_out != nullptr, "null output");
         ProbeArgs a{};
-        a.pk = part_keys(idx, probe_keys);
+        const PartKeys pkeys = part_keys(idx, probe_keys);
         a.time = view(probe_time);
         a.n = n;
         a.ts = idx->ts.as<int64_t>();
@@ -1072,9 +1090,19 @@ static qe_gpu_status asof_probe_codes(qe_gpu_ctx* ctx, const qe_dev_asof* idx, c
         a.tol = static_cast<uint64_t>(tolerance);
         a.out = match_out;
         const unsigned g = grid_for(n, 256, ctx->num_cus * 32);
+        // The partition functor for this index's key shape.
+        auto with_part = [&](auto&& launch) {
+            if (idx->mode == 0) {
+                launch(ZeroKeyPart{});
+            } else if (idx->mode == 1) {
+                launch(DirectKeyPart{view(probe_keys[0]), idx->kmin, idx->range});
+            } else {
+                launch(pkeys);
+            }
+        };
         if (mode == QE_ASOF_BINARY) {
             TimedRegion tr(ctx, "asof_probe_binary");
-            asof_binary_kernel<<<g, 256, 0, ctx->stream>>>(a);
+            with_part([&](auto part) { asof_binary_kernel<<<g, 256, 0, ctx->stream>>>(part, a); });
             QE_HIP(hipGetLastError());
         } else {
             auto* mi = const_cast<qe_dev_asof*>(idx);
@@ -1085,7 +1113,7 @@ static qe_gpu_status asof_probe_codes(qe_gpu_ctx* ctx, const qe_dev_asof* idx, c
                           mi->fallback.as<unsigned long long>()};
             {
                 TimedRegion tr(ctx, mode == QE_ASOF_LEARNED ? "asof_probe_learned" : "asof_probe_learned_no_mlp");
-                asof_learned_kernel<<<g, 256, 0, ctx->stream>>>(a, L);
+                with_part([&](auto part) { asof_learned_kernel<<<g, 256, 0, ctx->stream>>>(part, a, L); });
                 QE_HIP(hipGetLastError());
             }
             mi->last_fallbacks = read_scalar(ctx, mi->fallback.as<uint64_t>());
