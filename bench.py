@@ -404,6 +404,50 @@ def filter_workload(qe, ctx, args):
     return res
 
 
+def tick_cpu_baseline(qe, ctx, args, d, sym_id, sym_name, names, order, code):
+    """The whole pipeline on a bounded sample (2^23 trades against 2^25 quotes
+    from the same generator) through the CPU oracle (filter, as-of, join,
+    group-by; single thread), timed, and compared with the GPU pipeline on the
+    same sample."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import _oracle as O
+    nq, nt, nsym = 1 << 25, 1 << 23, len(names)
+    (qs, qt, qp), (ts_, tt) = ctx.gen_asof(20260619, nq, nt, nsym, 1.1)
+    tp = ctx.gen_uniform(20260619, 5, qe.F64, nt, 0.0, 200.0)
+    # GPU pipeline on the sample (binary-search as-of: an index per run)
+    f = ctx.filter([ts_, tt, tp], [(2, qe.CMP_GT, qe.F64, 100.0)])
+    idx = ctx.asof_build([qs], qt)
+    match = qe.DevBuffer(ctx, 4 * max(f.rows, 1))
+    idx.probe_into([f.col(0)], f.col(1), match, mode=qe.ASOF_BINARY)
+    pairs = ctx.asof_pairs(match, f.rows, qe.INNER)
+    g = pairs.gather([f.col(0), f.col(2)], [])
+    agg = ctx.join_aggregate([g.col(0), g.col(1)], [sym_id, sym_name], [0], [0], qe.INNER, [3],
+                             [(qe.COUNT_STAR, 0), (qe.AVG, 1)])
+    gpu_rows = list(zip(agg.col(0).strings(), agg.col(1).numpy()[0].tolist(), agg.col(2).numpy()[0].tolist()))
+    for x in (agg, g, pairs, match, idx, f):
+        x.free()
+    host = [c.numpy()[0] for c in (qs, qt, ts_, tt, tp)]
+    for c in (qs, qt, qp, ts_, tt, tp):
+        c.free()
+    qsym, qts, tsym, tts, tpx = host
+    sym_tab = [(np.arange(nsym, dtype=np.int32), None, O.I32),
+               (np.array([code[names[i]] for i in order], dtype=np.int32), None, O.STR, d.strings)]
+    t0 = time.perf_counter()
+    keep = O.filter_select([(tpx, None, O.F64)], [(0, O.GT, O.F64, 100.0)])
+    fs, ft, fp = tsym[keep], tts[keep], tpx[keep]
+    p_idx, _ = O.asof_pairs([(fs, None, O.I32), (ft, None, O.I64)], [(qsym, None, O.I32), (qts, None, O.I64)],
+                            [0], [0], 1, 1, 0)
+    cpu_rows = O.join_aggregate([(fs[p_idx], None, O.I32), (fp[p_idx], None, O.F64)], sym_tab, [0], [0], 0, [3],
+                                [(O.COUNT_STAR, 0), (O.AVG, 1)])
+    t1 = time.perf_counter()
+    from _compare import diff
+    return {"value": nt / (t1 - t0), "unit": "trade rows/s", "cores": 1, "kind": "port",
+            "sample": f"{nt} trades against {nq} quotes (same generator, seed 20260619), 50,000-name symbol "
+                      f"table; oracle filter_select, asof (sort + per-probe search), join_aggregate",
+            "seconds": t1 - t0, "groups": len(cpu_rows),
+            "gpu_parity_on_sample": diff(gpu_rows, cpu_rows, rel=1e-12, abs_=1e-12) is None}
+
+
 def tick_workload(qe, ctx, args):
     """The full tick pipeline: trades (2^30 rows: sym I32, ts I64 us, px F64
     uniform [0,200)) WHERE px > 100, as-of joined (backward, learned index) to
@@ -497,12 +541,13 @@ def tick_workload(qe, ctx, args):
             np.array_equal(ref[3].col(1).numpy()[0], counts))
     ref[3].free()
     step(qe.ASOF_LEARNED, timed_stages=True)[3].free()
+    cpu = None if args.skip_cpu else tick_cpu_baseline(qe, ctx, args, d, sym_id, sym_name, names, order, code)
     res = {"quotes": nq, "trades": nt, "symbols": nsym, "passing_filter": m, "asof_matched": matched,
            "groups": ng, "ms_per_step": wall * 1e3, "trade_rows_per_s": nt / wall,
            "index_build_s": t1 - t0, "index_train_s": t2 - t1,
            "stage_ms": {k: v * 1e3 for k, v in stages.items()},
            "counts_sorted_desc": bool(np.all(np.diff(counts.astype(np.int64)) <= 0)),
-           "top5": top, "learned_equals_binary": bool(same)}
+           "top5": top, "learned_equals_binary": bool(same), "cpu_baseline": cpu}
     out.free()
     match.free()
     perm.free()
