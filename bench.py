@@ -146,13 +146,28 @@ def cpu_baseline_join(qe, ctx, cols, args):
     cpu = {int(og[i]): (float(osum[i]), int(ocnt[i])) for i in range(g)}
     ok = gpu.keys() == cpu.keys() and all(
         gpu[k][1] == cpu[k][1] and abs(gpu[k][0] - cpu[k][0]) <= 1e-9 * max(1.0, abs(cpu[k][0])) for k in cpu)
-    return {
+    res = {
         "value": sample / (t1 - t0), "unit": "probe rows/s", "cores": threads, "kind": "port",
         "sample": f"full 2^{args.build_log2}-row build + first {sample} probe rows; build single-threaded "
                   f"then morsel-parallel probe (1024-row morsels) with per-thread partial aggregates "
                   f"(oracle/qe_baseline.cpp)",
         "seconds": t1 - t0, "gpu_parity_on_sample": bool(ok),
     }
+    # The reference's own operators (compiled from its sources into oracle/_ref
+    # where this tree was built), single-threaded: Aggregate(HashJoin(Scan, Scan)).
+    if O.ref_available():
+        rs = min(1 << args.cpu_ref_log2, sample)
+        pc = [(pkn[:rs], None, O.I64), (ppn[:rs], None, O.F64)]
+        bc = [(bk, None, O.I64), (bg, None, O.I32)]
+        t2 = time.perf_counter()
+        rows = O.ref_join_aggregate(pc, bc, [0], [0], 0, [3], [(O.SUM, 1), (O.COUNT_STAR, 0)])
+        t3 = time.perf_counter()
+        res["reference_single_thread"] = {
+            "value": rs / (t3 - t2), "unit": "probe rows/s", "cores": 1, "kind": "reference",
+            "sample": f"full 2^{args.build_log2}-row build + first {rs} probe rows through the reference's "
+                      f"HashJoin and Aggregate (2048-row batches), table construction included",
+            "seconds": t3 - t2, "groups": len(rows)}
+    return res
 
 
 def asof_workload(qe, ctx, args):
@@ -566,6 +581,7 @@ def main():
     ap.add_argument("--build-log2", type=int, default=24)
     ap.add_argument("--probe-log2", type=int, default=28)
     ap.add_argument("--cpu-probe-log2", type=int, default=24)
+    ap.add_argument("--cpu-ref-log2", type=int, default=22, help="probe rows through the compiled reference")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--zipf", type=float, default=1.0, help="Zipf exponent of the probe keys")
     ap.add_argument("--miss", type=float, default=0.1, help="fraction of probe keys outside the build domain")
@@ -599,7 +615,7 @@ def main():
               f"run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.quick:
-        args.build_log2, args.probe_log2, args.cpu_probe_log2 = 16, 20, 18
+        args.build_log2, args.probe_log2, args.cpu_probe_log2, args.cpu_ref_log2 = 16, 20, 18, 16
         args.asof_right, args.asof_left, args.cpu_asof_log2 = 1 << 22, 1 << 20, 18
         args.batch_queries, args.batch_morsels, args.batch_max_log2 = 8, 64, 16
         args.filter_log2 = 20
