@@ -64,6 +64,7 @@ struct QueryDev {
     uint64_t rows;
     uint32_t chunks;  // part: blocks per partition (slices of equal length)
     int32_t on_value; // every aggregate reads the value column (or is COUNT(*))
+    int32_t kw, vw;   // key / value element width in bytes (1, 4 or 8)
 };
 
 struct Tile {
@@ -84,10 +85,130 @@ __device__ inline uint32_t direct_slot(const QueryDev& q, uint64_t r) {
     return static_cast<uint32_t>(static_cast<uint64_t>(int_value(k, r) - q.kmin));
 }
 
-// 64-bit pattern of a value: integers sign-extended, F64 raw.
-__device__ inline uint64_t value_bits(const ColView& v, uint64_t r) {
-    if (v.type == QE_F64) return static_cast<const uint64_t*>(v.data)[r];
-    return static_cast<uint64_t>(int_value(v, r));
+// ---- typed row access for the streaming phases ---------------------------------------------
+// Each phase picks a specialisation per tile (a uniform branch) on the key and
+// value widths, so its row loop issues every load of a row group before using
+// any of them: loads are unconditional at a row index clamped to the tile, and go
+// through global-address-space pointers (generic loads would also hold up the
+// LDS wait counter). A column without NULLs reads one all-valid word instead of
+// a bitmap (index mask 0).
+template <typename T>
+using gptr = const T __attribute__((address_space(1)))*;
+
+template <typename T>
+__device__ inline gptr<T> gcast(const void* p) {
+    return (gptr<T>)(p);
+}
+
+__device__ const uint64_t kAllValidWord = ~0ull;
+
+struct BitsView {
+    gptr<uint64_t> w;
+    uint64_t mask;  // word index mask: ~0, or 0 for kAllValidWord
+};
+__device__ inline BitsView bits_view(const uint64_t* v) {
+    return v ? BitsView{gcast<uint64_t>(v), ~0ull} : BitsView{gcast<uint64_t>(&kAllValidWord), 0ull};
+}
+__device__ inline uint32_t bit_at(const BitsView& b, uint64_t r) {
+    return static_cast<uint32_t>((b.w[(r >> 6) & b.mask] >> (r & 63)) & 1ull);
+}
+
+// 64-bit pattern of a W-byte element: BOOL zero-extended, I32/STR sign-extended,
+// I64/TS/F64 raw (the integer reading of int_value, F64 bits unchanged).
+template <int W>
+__device__ inline uint64_t elem_bits(const void* base, uint64_t r) {
+    if constexpr (W == 8) return gcast<uint64_t>(base)[r];
+    else if constexpr (W == 4) return static_cast<uint64_t>(static_cast<int64_t>(gcast<int32_t>(base)[r]));
+    else return gcast<uint8_t>(base)[r];
+}
+
+// Direct slots of N rows per lane, rows r0 + u * blockDim.x + lane (kNoSlot
+// past the tile end), with the value column's bits and validity when VAL.
+constexpr int kRowUnroll = 4;  // rows per thread per step, loads issued first
+
+template <int KW, int VW, bool VAL, int N = kRowUnroll>
+__device__ inline void load_rows(const QueryDev& q, uint64_t r0, uint64_t end, uint32_t* slot, uint64_t* bits,
+                                 uint32_t* valid) {
+    const void* key = q.hk.key[0].data;
+    const BitsView kv = bits_view(q.hk.key[0].validity);
+    const void* val = q.val.data;
+    const BitsView vv = bits_view(q.val.validity);
+    uint64_t k[N];
+    uint32_t kok[N];
+#pragma unroll
+    for (int u = 0; u < N; ++u) {
+        const uint64_t r = r0 + uint64_t(u) * blockDim.x + threadIdx.x;
+        const uint64_t rc = r < end ? r : end - 1;
+        k[u] = elem_bits<KW>(key, rc);
+        kok[u] = bit_at(kv, rc);
+        if constexpr (VAL) {
+            bits[u] = elem_bits<VW>(val, rc);
+            valid[u] = bit_at(vv, rc);
+        }
+    }
+    const uint32_t null_slot = static_cast<uint32_t>(q.S - 1);
+#pragma unroll
+    for (int u = 0; u < N; ++u) {
+        const uint64_t r = r0 + uint64_t(u) * blockDim.x + threadIdx.x;
+        const uint32_t s = kok[u] ? static_cast<uint32_t>(k[u] - static_cast<uint64_t>(q.kmin)) : null_slot;
+        slot[u] = r < end ? s : kNoSlot;
+    }
+}
+
+// Calls f.template run<KW, VW>() for the query's widths.
+template <typename F>
+__device__ inline void with_widths(const QueryDev& q, const F& f) {
+    if (q.kw == 4) {
+        if (q.vw == 8) f.template run<4, 8>();
+        else if (q.vw == 4) f.template run<4, 4>();
+        else f.template run<4, 1>();
+    } else {
+        if (q.vw == 8) f.template run<8, 8>();
+        else if (q.vw == 4) f.template run<8, 4>();
+        else f.template run<8, 1>();
+    }
+}
+
+// One value's fold into LDS slot partials (layout acc u64[nacc][S] | cnt
+// u32[ncnt][S] | rows u32[S]) for a query whose aggregates all read the value
+// column: the same updates as lds_update_vals, with the shared accumulators and
+// the one non-NULL counter addressed directly.
+__device__ inline void lds_combine(int kind, uint64_t* ap, uint64_t v) {
+    switch (kind) {
+        case K_SUM_I:
+            atomicAdd(reinterpret_cast<unsigned long long*>(ap), static_cast<unsigned long long>(v));
+            break;
+        case K_SUM_F:
+            atomicAdd(reinterpret_cast<double*>(ap), __longlong_as_double(static_cast<long long>(v)));
+            break;
+        case K_MIN_I:
+            atomicMin(reinterpret_cast<long long*>(ap), static_cast<long long>(v));
+            break;
+        case K_MAX_I:
+            atomicMax(reinterpret_cast<long long*>(ap), static_cast<long long>(v));
+            break;
+        case K_MIN_F:
+            atomicMin(reinterpret_cast<unsigned long long*>(ap),
+                      static_cast<unsigned long long>(f64_order_key(__longlong_as_double(static_cast<long long>(v)))));
+            break;
+        case K_MAX_F:
+            atomicMax(reinterpret_cast<unsigned long long*>(ap),
+                      static_cast<unsigned long long>(f64_order_key(__longlong_as_double(static_cast<long long>(v)))));
+            break;
+        default:
+            break;
+    }
+}
+
+__device__ inline void fold_value(char* smem, uint32_t S, const AggProg& p, uint32_t slot, uint64_t v,
+                                  uint32_t valid) {
+    uint64_t* acc = reinterpret_cast<uint64_t*>(smem);
+    uint32_t* cnt = reinterpret_cast<uint32_t*>(smem + size_t(8) * S * p.nacc);
+    uint32_t* rows = cnt + size_t(S) * p.ncnt;
+    atomicAdd(&rows[slot], 1u);
+    if (!valid) return;
+    if (p.ncnt) atomicAdd(&cnt[slot], 1u);
+    for (int a = 0; a < p.nacc; ++a) lds_combine(p.acc_kind[a], &acc[size_t(a) * S + slot], v);
 }
 
 // ---- LDS slot cache of the hash kind ------------------------------------------------------
@@ -214,8 +335,53 @@ __device__ inline void hash_tile(const QueryDev& q, const Tile& tile, char* smem
     }
 }
 
-constexpr int kRowUnroll = 4;  // rows per thread per step, loads issued first
-__constant__ int32_t kValueCol[kMaxAggs] = {};  // every aggregate reads value column 0
+// Row loops of the small kind (fold into the LDS table) and of the part
+// kind's scatter (each row's entry, slot within its partition | value validity
+// << 31 and the value bits, to its partition's range of this tile).
+struct SmallRows {
+    const QueryDev& q;
+    const Tile& tile;
+    char* smem;
+    template <int KW, int VW>
+    __device__ void run() const {
+        const uint32_t S = static_cast<uint32_t>(q.S);
+        for (uint64_t r0 = tile.begin; r0 < tile.end; r0 += kRowUnroll * uint64_t(blockDim.x)) {
+            uint32_t slot[kRowUnroll], valid[kRowUnroll];
+            uint64_t bits[kRowUnroll];
+            load_rows<KW, VW, true>(q, r0, tile.end, slot, bits, valid);
+#pragma unroll
+            for (int u = 0; u < kRowUnroll; ++u)
+                if (slot[u] != kNoSlot) fold_value(smem, S, q.p, slot[u], bits[u], valid[u]);
+        }
+    }
+};
+
+struct ScatterRows {
+    const QueryDev& q;
+    const Tile& tile;
+    const uint64_t* base;
+    uint32_t* cur;
+    uint32_t* ekey;
+    uint64_t* eval;
+    template <int KW, int VW>
+    __device__ void run() const {
+        const uint32_t pbits = q.pbits;
+        const uint32_t mask = (1u << pbits) - 1;
+        for (uint64_t r0 = tile.begin; r0 < tile.end; r0 += kRowUnroll * uint64_t(blockDim.x)) {
+            uint32_t slot[kRowUnroll], valid[kRowUnroll];
+            uint64_t bits[kRowUnroll];
+            load_rows<KW, VW, true>(q, r0, tile.end, slot, bits, valid);
+#pragma unroll
+            for (int u = 0; u < kRowUnroll; ++u) {
+                if (slot[u] == kNoSlot) continue;
+                const uint32_t part = slot[u] >> pbits;
+                const uint64_t pos = base[part] + atomicAdd(&cur[part], 1u);
+                ekey[pos] = (slot[u] & mask) | (valid[u] << 31);
+                eval[pos] = bits[u];
+            }
+        }
+    }
+};
 
 // Small kind: dense LDS table over the query's S slots, folded into HBM with
 // one atomic per touched slot field.
@@ -224,24 +390,7 @@ __device__ inline void small_tile(const QueryDev& q, const Tile& tile, char* sme
     lds_init(smem, S, q.p);
     __syncthreads();
     if (q.on_value) {
-        for (uint64_t r0 = tile.begin; r0 < tile.end; r0 += kRowUnroll * uint64_t(blockDim.x)) {
-            uint32_t slot[kRowUnroll];
-            uint64_t bits[kRowUnroll];
-            uint32_t valid[kRowUnroll];
-#pragma unroll
-            for (int u = 0; u < kRowUnroll; ++u) {
-                const uint64_t r = r0 + uint64_t(u) * blockDim.x + threadIdx.x;
-                slot[u] = kNoSlot;
-                if (r < tile.end) {
-                    slot[u] = direct_slot(q, r);
-                    bits[u] = value_bits(q.val, r);
-                    valid[u] = is_valid(q.val, r) ? 1u : 0u;
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < kRowUnroll; ++u)
-                if (slot[u] != kNoSlot) lds_update_vals(smem, S, q.p, slot[u], kValueCol, &bits[u], valid[u]);
-        }
+        with_widths(q, SmallRows{q, tile, smem});
     } else {
         uint64_t vrow[kMaxAggs];
         for (uint64_t r = tile.begin + threadIdx.x; r < tile.end; r += blockDim.x) {
@@ -265,41 +414,95 @@ __device__ inline void small_tile(const QueryDev& q, const Tile& tile, char* sme
     }
 }
 
-// Part kind, scatter: each row's entry (slot within its partition | value
-// validity << 31, value bits) goes to its partition's range of this tile.
+// Staged scatter for queries with few partitions: each step of kStageRows rows
+// is counting-sorted by partition in LDS (rank within the partition from an
+// LDS atomic, partition offsets from a block scan), then written out in LDS
+// order, so consecutive lanes store consecutive entries of one partition's
+// range instead of one entry in each of up to 64 ranges.
+// LDS: gbase u64[np] | sval u64[S] | skey u32[S] | cnt u32[np] | ws u32[kMaxWaves + 1] | spart u16[S].
+constexpr int kStageUnroll = 2 * kRowUnroll;               // rows per thread per step
+constexpr uint32_t kStageRows = kStageUnroll * 256;        // rows per step (256-thread blocks)
+constexpr uint32_t kStageMaxParts = 1024;                  // partitions of a staged query (block scan: 4 per thread)
+
+__host__ __device__ inline size_t stage_lds_bytes(uint32_t np) {
+    return size_t(np) * 12 + size_t(kStageRows) * 14 + 4 * (kMaxWaves + 1);
+}
+
+struct StagedScatterRows {
+    const QueryDev& q;
+    const Tile& tile;
+    char* smem;
+    uint32_t* ekey;
+    uint64_t* eval;
+    template <int KW, int VW>
+    __device__ void run() const {
+        const uint32_t np = q.nparts, pbits = q.pbits, mask = (1u << pbits) - 1;
+        uint64_t* gbase = reinterpret_cast<uint64_t*>(smem);
+        uint64_t* sval = gbase + np;
+        uint32_t* skey = reinterpret_cast<uint32_t*>(sval + kStageRows);
+        uint32_t* cnt = skey + kStageRows;
+        uint32_t* ws = cnt + np;
+        uint16_t* spart = reinterpret_cast<uint16_t*>(ws + kMaxWaves + 1);
+        for (uint64_t r0 = tile.begin; r0 < tile.end; r0 += kStageRows) {
+            uint32_t slot[kStageUnroll], valid[kStageUnroll], rank[kStageUnroll];
+            uint64_t bits[kStageUnroll];
+            load_rows<KW, VW, true, kStageUnroll>(q, r0, tile.end, slot, bits, valid);
+#pragma unroll
+            for (int u = 0; u < int(kStageUnroll); ++u)
+                if (slot[u] != kNoSlot) rank[u] = atomicAdd(&cnt[slot[u] >> pbits], 1u);
+            __syncthreads();
+            const uint32_t total = block_exclusive_scan(cnt, np, ws);
+#pragma unroll
+            for (int u = 0; u < int(kStageUnroll); ++u) {
+                if (slot[u] == kNoSlot) continue;
+                const uint32_t part = slot[u] >> pbits;
+                const uint32_t at = cnt[part] + rank[u];
+                skey[at] = (slot[u] & mask) | (valid[u] << 31);
+                sval[at] = bits[u];
+                spart[at] = static_cast<uint16_t>(part);
+            }
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < total; i += 256) {
+                const uint32_t part = spart[i];
+                const uint64_t g = gbase[part] + (i - cnt[part]);
+                ekey[g] = skey[i];
+                eval[g] = sval[i];
+            }
+            uint32_t mine[kStageMaxParts / 256];
+#pragma unroll
+            for (int j = 0; j < int(kStageMaxParts / 256); ++j) {
+                const uint32_t t = threadIdx.x + 256u * j;
+                mine[j] = t < np ? (t + 1 < np ? cnt[t + 1] : total) - cnt[t] : 0u;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < int(kStageMaxParts / 256); ++j) {
+                const uint32_t t = threadIdx.x + 256u * j;
+                if (t < np) {
+                    gbase[t] += mine[j];
+                    cnt[t] = 0;
+                }
+            }
+            __syncthreads();
+        }
+    }
+};
+
+// Part kind, scatter (staged or direct) after this tile's partition bases are read.
 __device__ inline void scatter_tile(const QueryDev& q, const Tile& tile, char* smem, const uint64_t* offs,
                                     uint32_t* ekey, uint64_t* eval) {
+    const uint32_t np = q.nparts;
+    const bool staged = np <= kStageMaxParts;
     uint64_t* base = reinterpret_cast<uint64_t*>(smem);
-    uint32_t* cur = reinterpret_cast<uint32_t*>(base + q.nparts);
-    for (uint32_t i = threadIdx.x; i < q.nparts; i += blockDim.x) {
+    uint32_t* cur = staged ? reinterpret_cast<uint32_t*>(smem + size_t(np) * 8 + size_t(kStageRows) * 12)
+                           : reinterpret_cast<uint32_t*>(base + np);
+    for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) {
         base[i] = offs[q.hbase + uint64_t(i) * q.ntiles + tile.qt];
         cur[i] = 0;
     }
     __syncthreads();
-    const uint32_t mask = (1u << q.pbits) - 1;
-    for (uint64_t r0 = tile.begin; r0 < tile.end; r0 += kRowUnroll * uint64_t(blockDim.x)) {
-        uint32_t slot[kRowUnroll];
-        uint64_t bits[kRowUnroll];
-        uint32_t valid[kRowUnroll];
-#pragma unroll
-        for (int u = 0; u < kRowUnroll; ++u) {
-            const uint64_t r = r0 + uint64_t(u) * blockDim.x + threadIdx.x;
-            slot[u] = kNoSlot;
-            if (r < tile.end) {
-                slot[u] = direct_slot(q, r);
-                bits[u] = value_bits(q.val, r);
-                valid[u] = is_valid(q.val, r) ? 1u : 0u;
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < kRowUnroll; ++u) {
-            if (slot[u] == kNoSlot) continue;
-            const uint32_t part = slot[u] >> q.pbits;
-            const uint64_t pos = base[part] + atomicAdd(&cur[part], 1u);
-            ekey[pos] = (slot[u] & mask) | (valid[u] << 31);
-            eval[pos] = bits[u];
-        }
-    }
+    if (staged) with_widths(q, StagedScatterRows{q, tile, smem, ekey, eval});
+    else with_widths(q, ScatterRows{q, tile, base, cur, ekey, eval});
 }
 
 // ---- kernels --------------------------------------------------------------------------------
@@ -325,31 +528,44 @@ __global__ void batch_reset_kernel(unsigned long long* next, unsigned int* ovf) 
     *ovf = 0;
 }
 
+// The tile index every lane of the block read from `cur`, as a scalar: the
+// tile and its query's fields are then uniform (scalar loads, uniform branches).
+__device__ inline uint64_t uniform_index(uint64_t i) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(i));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(i >> 32));
+    return (uint64_t(hi) << 32) | lo;
+}
+
+template <int KW>
+__device__ inline void hist_rows(const QueryDev& q, const Tile& t, uint32_t* hcnt) {
+    const uint32_t pbits = q.pbits;
+    for (uint64_t r0 = t.begin; r0 < t.end; r0 += kRowUnroll * uint64_t(blockDim.x)) {
+        uint32_t slot[kRowUnroll], valid[kRowUnroll];
+        uint64_t bits[kRowUnroll];
+        load_rows<KW, 8, false>(q, r0, t.end, slot, bits, valid);
+#pragma unroll
+        for (int u = 0; u < kRowUnroll; ++u)
+            if (slot[u] != kNoSlot) atomicAdd(&hcnt[slot[u] >> pbits], 1u);
+    }
+}
+
 // Part queries, pass 1: per-(partition, tile) row counts.
-__global__ void __launch_bounds__(256) batch_hist_kernel(const QueryDev* qs, const Tile* tiles, uint64_t ntiles,
+__global__ void __launch_bounds__(256) batch_hist_kernel(const QueryDev* __restrict__ qs,
+                                                         const Tile* __restrict__ tiles, uint64_t ntiles,
                                                          unsigned long long* next, uint32_t* hist) {
     extern __shared__ uint32_t hcnt[];
     __shared__ uint64_t cur;
     while (true) {
         if (threadIdx.x == 0) cur = atomicAdd(next, 1ull);
         __syncthreads();
-        const uint64_t i = cur;
+        const uint64_t i = uniform_index(cur);
         if (i >= ntiles) break;
         const Tile t = tiles[i];
         const QueryDev& q = qs[t.q];
         for (uint32_t p = threadIdx.x; p < q.nparts; p += blockDim.x) hcnt[p] = 0;
         __syncthreads();
-        for (uint64_t r0 = t.begin; r0 < t.end; r0 += kRowUnroll * uint64_t(blockDim.x)) {
-            uint32_t slot[kRowUnroll];
-#pragma unroll
-            for (int u = 0; u < kRowUnroll; ++u) {
-                const uint64_t r = r0 + uint64_t(u) * blockDim.x + threadIdx.x;
-                slot[u] = r < t.end ? direct_slot(q, r) : kNoSlot;
-            }
-#pragma unroll
-            for (int u = 0; u < kRowUnroll; ++u)
-                if (slot[u] != kNoSlot) atomicAdd(&hcnt[slot[u] >> q.pbits], 1u);
-        }
+        if (q.kw == 4) hist_rows<4>(q, t, hcnt);
+        else hist_rows<8>(q, t, hcnt);
         __syncthreads();
         for (uint32_t p = threadIdx.x; p < q.nparts; p += blockDim.x) hist[q.hbase + uint64_t(p) * q.ntiles + t.qt] = hcnt[p];
         __syncthreads();
@@ -359,7 +575,8 @@ __global__ void __launch_bounds__(256) batch_hist_kernel(const QueryDev* qs, con
 // One persistent kernel per strategy over that strategy's tile list, so each
 // gets its own register budget: small tables, part scatter, hash probes.
 template <int KIND>
-__global__ void __launch_bounds__(256) batch_tiles_kernel(const QueryDev* qs, const Tile* tiles, uint64_t ntiles,
+__global__ void __launch_bounds__(256) batch_tiles_kernel(const QueryDev* __restrict__ qs,
+                                                          const Tile* __restrict__ tiles, uint64_t ntiles,
                                                           unsigned long long* next, uint32_t C, const uint64_t* offs,
                                                           uint32_t* ekey, uint64_t* eval) {
     extern __shared__ __align__(16) char smem[];
@@ -367,7 +584,7 @@ __global__ void __launch_bounds__(256) batch_tiles_kernel(const QueryDev* qs, co
     while (true) {
         if (threadIdx.x == 0) cur = atomicAdd(next, 1ull);
         __syncthreads();
-        const uint64_t i = cur;
+        const uint64_t i = uniform_index(cur);
         if (i >= ntiles) break;
         const Tile tile = tiles[i];
         const QueryDev& q = qs[tile.q];
@@ -392,7 +609,8 @@ __global__ void __launch_bounds__(256) batch_tiles_kernel(const QueryDev* qs, co
 
 // Part queries, pass 2: one block per (query, partition) aggregates the
 // partition's entries in LDS and writes its slots' final state.
-__global__ void __launch_bounds__(kPartAggThreads) batch_part_agg_kernel(const QueryDev* qs, const PartItem* items,
+__global__ void __launch_bounds__(kPartAggThreads) batch_part_agg_kernel(const QueryDev* __restrict__ qs,
+                                                                         const PartItem* __restrict__ items,
                                                                          uint64_t nitems, unsigned long long* next,
                                                                          const uint64_t* offs, const uint32_t* ekey,
                                                                          const uint64_t* eval) {
@@ -401,7 +619,7 @@ __global__ void __launch_bounds__(kPartAggThreads) batch_part_agg_kernel(const Q
     while (true) {
         if (threadIdx.x == 0) cur = atomicAdd(next, 1ull);
         __syncthreads();
-        const uint64_t i = cur;
+        const uint64_t i = uniform_index(cur);
         if (i >= nitems) break;
         const PartItem it = items[i];
         const QueryDev& q = qs[it.q];
@@ -418,15 +636,15 @@ __global__ void __launch_bounds__(kPartAggThreads) batch_part_agg_kernel(const Q
 #pragma unroll
             for (int u = 0; u < kRowUnroll; ++u) {
                 const uint64_t k = k0 + uint64_t(u) * blockDim.x + threadIdx.x;
-                w[u] = kNoSlot;
-                if (k < e) {
-                    w[u] = __builtin_nontemporal_load(ekey + k);
-                    bits[u] = __builtin_nontemporal_load(eval + k);
-                }
+                const uint64_t kc = k < e ? k : e - 1;
+                w[u] = __builtin_nontemporal_load(ekey + kc);
+                bits[u] = __builtin_nontemporal_load(eval + kc);
             }
 #pragma unroll
-            for (int u = 0; u < kRowUnroll; ++u)
-                if (w[u] != kNoSlot) lds_update_vals(smem, cnt, q.p, w[u] & 0x7FFFFFFFu, kValueCol, &bits[u], w[u] >> 31);
+            for (int u = 0; u < kRowUnroll; ++u) {
+                const uint64_t k = k0 + uint64_t(u) * blockDim.x + threadIdx.x;
+                if (k < e) fold_value(smem, cnt, q.p, w[u] & 0x7FFFFFFFu, bits[u], w[u] >> 31);
+            }
         }
         __syncthreads();
         const uint64_t* acc = reinterpret_cast<const uint64_t*>(smem);
@@ -493,6 +711,7 @@ struct BatchPlan {
     bool any_hash = false;
     size_t small_lds = 0;  // largest small-kind table
     uint32_t max_parts = 0, max_pbits = 0;
+    size_t part_lds = 16;  // dynamic LDS of the part scatter
     // Layout for the current tile size.
     uint32_t tile_rows = 0;
     Scratch ktiles[3], htiles, items, hist, offs, sums, ekey, eval;  // ktiles: per strategy
@@ -541,6 +760,8 @@ static void build_batch_plan(qe_gpu_ctx* ctx, BatchPlan& bp, const qe_batch_quer
         d.val = view(q.value);
         d.rows = n;
         d.on_value = on_value ? 1 : 0;
+        d.kw = static_cast<int32_t>(type_width(q.key->type));
+        d.vw = static_cast<int32_t>(type_width(q.value->type));
         d.kind = BK_HASH;
         // Dense integer keys: direct slots (small or partitioned).
         if (is_int_like(q.key->type) && q.key->type != QE_BOOL) {
@@ -562,6 +783,8 @@ static void build_batch_plan(qe_gpu_ctx* ctx, BatchPlan& bp, const qe_batch_quer
                         d.pbits = pb;
                         d.nparts = static_cast<uint32_t>(np64);
                         bp.max_parts = std::max(bp.max_parts, d.nparts);
+                        bp.part_lds = std::max(bp.part_lds, d.nparts <= kStageMaxParts ? stage_lds_bytes(d.nparts)
+                                                                                       : size_t(d.nparts) * 12);
                         bp.max_pbits = std::max(bp.max_pbits, pb);
                     }
                 }
@@ -689,7 +912,7 @@ static void enqueue_persistent(qe_gpu_ctx* ctx, BatchPlan& bp, uint32_t blocks, 
             bp.dq.as<QueryDev>(), bp.ktiles[BK_SMALL].as<Tile>(), bp.n_ktiles[BK_SMALL], next + 1, C, offs,
             bp.ekey.as<uint32_t>(), bp.eval.as<uint64_t>());
     if (bp.n_ktiles[BK_PART])
-        batch_tiles_kernel<BK_PART><<<blocks, 256, size_t(bp.max_parts) * 12, ctx->stream>>>(
+        batch_tiles_kernel<BK_PART><<<blocks, 256, bp.part_lds, ctx->stream>>>(
             bp.dq.as<QueryDev>(), bp.ktiles[BK_PART].as<Tile>(), bp.n_ktiles[BK_PART], next + 2, C, offs,
             bp.ekey.as<uint32_t>(), bp.eval.as<uint64_t>());
     if (bp.n_ktiles[BK_HASH])

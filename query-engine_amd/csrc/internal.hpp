@@ -268,6 +268,46 @@ __host__ __device__ inline double f64_from_order_key(uint64_t k) {
 
 __device__ inline int lane_id() { return __lane_id(); }
 
+// In-place exclusive scan of a[0, n) (n <= 4 * blockDim.x) by one block;
+// returns the total. `ws` holds kMaxWaves + 1 words of scratch.
+__device__ inline uint32_t block_exclusive_scan(uint32_t* a, uint32_t n, uint32_t* ws) {
+    const uint32_t t = threadIdx.x, lane = lane_id(), w = t / kWave, nw = blockDim.x / kWave;
+    uint32_t v[4], sum = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t i = 4 * t + k;
+        v[k] = i < n ? a[i] : 0u;
+        sum += v[k];
+    }
+    uint32_t x = sum;  // inclusive wave scan of the per-thread sums
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if (lane >= static_cast<unsigned>(off)) x += y;
+    }
+    if (lane == kWave - 1) ws[w] = x;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t run = 0;
+        for (uint32_t k = 0; k < nw; ++k) {
+            const uint32_t c = ws[k];
+            ws[k] = run;
+            run += c;
+        }
+        ws[kMaxWaves] = run;
+    }
+    __syncthreads();
+    uint32_t base = ws[w] + x - sum;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const uint32_t i = 4 * t + k;
+        if (i < n) a[i] = base;
+        base += v[k];
+    }
+    const uint32_t total = ws[kMaxWaves];
+    __syncthreads();
+    return total;
+}
+
 // ---- launch geometry ----------------------------------------------------------
 inline unsigned grid_for(uint64_t n, unsigned block, unsigned max_blocks = 65535u * 16) {
     uint64_t g = (n + block - 1) / block;

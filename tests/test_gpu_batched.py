@@ -127,3 +127,76 @@ def test_strategies_small_partitioned_hash(ctx, qe):
         for k, v, *_ in queries:
             k.free()
             v.free()
+
+
+def test_key_and_value_widths(ctx, qe):
+    # The streaming phases are specialised on key width (4: I32/STR codes,
+    # 8: I64/TS) and value width (1: BOOL, 4: I32, 8: I64/TS/F64); every pair
+    # through both direct strategies (small LDS tables and partitioned ranges),
+    # nullable keys and values, against the oracle.
+    r = np.random.default_rng(91)
+    BOOL, TS = O.BOOL, O.TS
+    queries, hostcols = [], []
+    for i, (kt, vt) in enumerate([(k, v) for k in (I32, TS) for v in (BOOL, I32, TS, F64)]):
+        for span in (300, 50000):  # small table / partitioned range (span <= 2 x rows)
+            n = int(r.integers(30000, 250000))
+            sizes = np.diff(np.sort(np.concatenate([[0, n], r.integers(0, n, 3)])))
+            offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+            kv = np.asarray(r.integers(-span // 2, span // 2, n), dtype=O.NP[kt])
+            kvalid = r.random(n) >= 0.01
+            if vt == BOOL:
+                vv = r.integers(0, 2, n).astype(np.uint8)
+            elif vt == F64:
+                vv = r.normal(size=n) * 1e3
+            else:
+                vv = np.asarray(r.integers(-(1 << 30), 1 << 30, n), dtype=O.NP[vt])
+            vvalid = r.random(n) >= 0.03
+            aggs = [(CS, 0), (SUM, 1), (AVG, 1), (CNT, 1), (MIN, 1), (MAX, 1)]
+            k = ctx.upload(kv, kt, kvalid)
+            v = ctx.upload(vv, vt, vvalid)
+            queries.append((k, v, offs, aggs, 0))
+            hostcols.append(([(kv, kvalid, kt), (vv, vvalid, vt)], aggs))
+    plan = ctx.batch(queries)
+    try:
+        for mode in (qe.BATCH_PERSISTENT, qe.BATCH_GRAPH):
+            check_outputs(plan.run(mode, tuning=(4096, 2, 0)), hostcols)
+    finally:
+        plan.free()
+        for k, v, *_ in queries:
+            k.free()
+            v.free()
+
+
+def test_partitioned_staged_and_direct_scatter(ctx, qe):
+    # Partitioned queries with up to 1024 partitions are counting-sorted per
+    # step in LDS before they are written; those with more (here 4.3M slots in
+    # 4096-slot partitions: 1050) write each entry straight to its range. Both
+    # in one batch (513, 30 and 3 partitions staged), with NULL keys and values,
+    # tiles that end inside a step.
+    r = np.random.default_rng(93)
+    queries, hostcols = [], []
+    for n, span in ((2_200_000, 4_300_000), (1_300_000, 2_100_000), (70_001, 120_000), (5_000, 9_000)):
+        sizes = np.diff(np.sort(np.concatenate([[0, n], r.integers(0, n, 4)])))
+        offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+        kv = r.integers(0, span, n).astype(np.int64)
+        kvalid = r.random(n) >= 0.005
+        vv = r.normal(size=n) * 50
+        vvalid = r.random(n) >= 0.02
+        aggs = [(CS, 0), (SUM, 1), (MAX, 1)] + ([(MIN, 1)] if n > 2_000_000 else [])
+        k = ctx.upload(kv, I64, kvalid)
+        v = ctx.upload(vv, F64, vvalid)
+        queries.append((k, v, offs, aggs, 0))
+        hostcols.append(([(kv, kvalid, I64), (vv, vvalid, F64)], aggs))
+    expect = [O.aggregate(cols, [0], aggs) for cols, aggs in hostcols]
+    plan = ctx.batch(queries)
+    try:
+        for tuning in ((4096, 2, 0), (3000, 1, 0), (65536, 4, 0)):
+            for t, e in zip(plan.run(qe.BATCH_PERSISTENT, tuning=tuning), expect):
+                got = table_rows(t.numpy())
+                t.free()
+                assert_same(got, e)
+    finally:
+        plan.free()
+        for k, v, *_ in queries:
+            k.free()
+            v.free()
