@@ -466,7 +466,38 @@ struct LearnedArgs {
     unsigned long long* fallbacks;
 };
 
-// Largest knot j in [lo, hi] with tau[kb+j] <= t (caller guarantees tau[kb+lo] <= t).
+// Number of elements of the sorted run ts[base, base + cnt) that are <= t,
+// searched outward from the predicted index p < cnt: steps of 1, 2, 4, ... away
+// from p until the answer is bracketed, then a binary search of the bracket.
+// Exact for any p; the elements read lie within twice the prediction's error of
+// p, so a close prediction reads one or two cache lines.
+__device__ inline uint32_t gallop_count(const int64_t* ts, uint64_t base, uint32_t cnt, uint32_t p, int64_t t) {
+    uint32_t lo, hi;  // ts[base + lo - 1] <= t (or lo == 0) and ts[base + hi] > t (or hi == cnt)
+    if (ts[base + p] <= t) {
+        lo = p + 1;
+        hi = cnt;
+        for (uint32_t step = 1; p + step < cnt; step <<= 1) {
+            if (ts[base + p + step] > t) {
+                hi = p + step;
+                break;
+            }
+            lo = p + step + 1;
+        }
+    } else {
+        lo = 0;
+        hi = p;
+        for (uint32_t step = 1; step <= p; step <<= 1) {
+            if (ts[base + p - step] <= t) {
+                lo = p - step + 1;
+                break;
+            }
+            hi = p - step;
+        }
+    }
+    return lo + static_cast<uint32_t>(upper_bound_count(ts, base + lo, base + hi, t));
+}
+
+// Largest knot j in [lo, hi] with tau[kb+j] <= t (lo itself when tau[kb+lo] > t).
 __device__ inline uint32_t knot_search(const Knot* knots, uint64_t kb, uint32_t lo, uint32_t hi, int64_t t) {
     while (lo < hi) {
         const uint32_t mid = lo + (hi - lo + 1) / 2;
@@ -500,31 +531,29 @@ __global__ void asof_learned_kernel(PK pk, ProbeArgs a, LearnedArgs L) {
         uint32_t hi = t_hi > 0 ? t_hi - 1 : 0;
         if (hi >= pm.K) hi = pm.K - 1;
         if (lo > hi) lo = hi;
-        uint32_t piece;
-        const bool ok_lo = L.knots[pm.kb + lo].tau <= t;
-        const bool ok_hi = (hi + 1 >= pm.K) || L.knots[pm.kb + hi + 1].tau > t;
-        if (ok_lo && ok_hi) {
-            piece = knot_search(L.knots, pm.kb, lo, hi, t);
-        } else {
+        // Search the bucket's range, then verify the piece by its own bound and
+        // its successor's (the records the search ended on): the piece is the
+        // last knot with tau <= t, found in the whole partition if the bucket
+        // range did not hold it.
+        uint32_t piece = knot_search(L.knots, pm.kb, lo, hi, t);
+        Knot kn = L.knots[pm.kb + piece];
+        if (!(kn.tau <= t && (piece + 1 >= pm.K || L.knots[pm.kb + piece + 1].tau > t))) {
             ++fb;
             piece = knot_search(L.knots, pm.kb, 0, pm.K - 1, t);
+            kn = L.knots[pm.kb + piece];
         }
-        // 2. error-bounded window inside the piece.
-        const Knot kn = L.knots[pm.kb + piece];
+        // 2. inside the piece, search outward from the spline's prediction; the
+        //    result is exact wherever the prediction lands. A result outside the
+        //    piece's recorded error bound counts as a fallback (never, by the
+        //    bound's construction; the count stays as a check of the index).
         const uint64_t base = pm.seg + uint64_t(piece) * L.B;
         const uint32_t cnt = min(L.B, pm.n - piece * L.B);
         const uint32_t pred = piece_pred(kn.tau, kn.slope, cnt, t);
+        const uint32_t c = gallop_count(a.ts, base, cnt, pred, t);
         const uint32_t e = kn.err + 1;
         const uint32_t wlo = pred > e ? pred - e : 0;
         const uint32_t whi = min(cnt - 1, pred + e);
-        uint64_t c;
-        const bool w_ok = (wlo == 0 || a.ts[base + wlo] <= t) && (whi == cnt - 1 || a.ts[base + whi + 1] > t);
-        if (w_ok) {
-            c = wlo + upper_bound_count(a.ts, base + wlo, base + whi + 1, t);
-        } else {
-            ++fb;
-            c = upper_bound_count(a.ts, base, base + cnt, t);
-        }
+        fb += !((wlo == 0 || c > wlo) && (whi == cnt - 1 || c <= whi + 1));
         // tau[piece] <= t guarantees c >= 1.
         finish_match(a, j, t, base + c - 1, c > 0);
     }

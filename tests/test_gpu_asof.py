@@ -118,6 +118,23 @@ def test_random_unsorted_inputs(ctx, qe, case):
     check_all(ctx, qe, pcols, bcols, [0], [0], 1, 1, tol=tol, piece_rows=int(r.choice([4, 16, 64])))
 
 
+@pytest.mark.parametrize("piece_rows", [4, 16, 64])
+def test_duplicate_time_runs_across_pieces(ctx, qe, piece_rows):
+    # Runs of equal build timestamps longer than a piece (several knots share
+    # one tau, and a piece's rows may all be equal), rows in shuffled order,
+    # probes at, between, before and after the runs: the search inside a piece
+    # starts from the spline's prediction and must end on the run's last row
+    # in the stable (key, time) order.
+    r = np.random.default_rng(808 + piece_rows)
+    nb, np_ = 30000, 20000
+    bsym = r.integers(0, 3, nb)
+    bts = r.integers(0, 100, nb) * 10
+    psym = r.integers(0, 4, np_)
+    pts = r.integers(-20, 1020, np_)
+    check_all(ctx, qe, [col(psym, I32), col(pts, TS)], [col(bsym, I32), col(bts, TS)], [0], [0], 1, 1,
+              piece_rows=piece_rows)
+
+
 def test_hash_partitioned_keys(ctx, qe):
     r = np.random.default_rng(77)
     nb, np_ = 20000, 20000
