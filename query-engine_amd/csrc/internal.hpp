@@ -365,6 +365,10 @@ void str_agg_finish(qe_gpu_ctx* ctx, qe_dev_table* t, const StrAggFix& fix);
 qe_gpu_status aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, uint32_t ncols, const uint32_t* key_cols,
                               uint32_t nkeys, const qe_agg_spec* aggs, uint32_t naggs, qe_dev_table** out);
 qe_dev_col* gather_col(qe_gpu_ctx* ctx, const qe_dev_col* src, const uint32_t* idx, uint64_t m);
+// Column c of the result = srcs[c] gathered by idxs[c] (m rows each), with one
+// host read-back for all of their NULL counts.
+std::vector<qe_dev_col*> gather_cols(qe_gpu_ctx* ctx, const qe_dev_col* const* srcs, const uint32_t* const* idxs,
+                                     size_t n, uint64_t m);
 // Device-only variant (no allocation, no host read; safe under graph
 // capture): `sums` holds scan_sums_words(n) words.
 uint64_t scan_sums_words(uint64_t n);

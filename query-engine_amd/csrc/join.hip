@@ -331,44 +331,97 @@ qe_dev_pairs* join_pairs(qe_gpu_ctx* ctx, const qe_dev_col* const* pk, const qe_
 }
 
 // ---- gather (emit) --------------------------------------------------------------------
+// out[r] = src[idx[r]], NULL where idx[r] is kNoRow or the source row is NULL.
+// Each wave takes 64 consecutive output rows, so the row validity is one ballot
+// and the wave's lane 0 writes the output bitmap word (rows past the end read
+// as valid) and counts the NULLs.
 template <typename T>
-__global__ void gather_kernel(const T* src, const uint64_t* src_valid, const uint32_t* idx, uint64_t m,
-                              T* dst, uint8_t* vb) {
-    for (uint64_t r = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; r < m;
-         r += uint64_t(gridDim.x) * blockDim.x) {
-        const uint32_t s = idx[r];
-        const bool ok = s != kNoRow && (src_valid == nullptr || ((src_valid[s >> 6] >> (s & 63)) & 1ull));
-        dst[r] = ok ? src[s] : T(0);
-        vb[r] = ok ? 1 : 0;
+__global__ void gather_kernel(const T* __restrict__ src, const uint64_t* __restrict__ src_valid,
+                              const uint32_t* __restrict__ idx, uint64_t m, T* __restrict__ dst,
+                              uint64_t* __restrict__ words, unsigned long long* nulls) {
+    const uint64_t nw = words_for(m);
+    const unsigned lane = lane_id();
+    const uint64_t waves = uint64_t(gridDim.x) * (blockDim.x / kWave);
+    uint64_t local = 0;
+    for (uint64_t w = blockIdx.x * uint64_t(blockDim.x / kWave) + threadIdx.x / kWave; w < nw; w += waves) {
+        const uint64_t r = w * 64 + lane;
+        bool ok = false;
+        if (r < m) {
+            const uint32_t s = idx[r];
+            ok = s != kNoRow && (src_valid == nullptr || ((src_valid[s >> 6] >> (s & 63)) & 1ull));
+            dst[r] = ok ? src[s] : T(0);
+        }
+        const uint64_t mask = __ballot(ok);
+        if (lane == 0) {
+            const uint64_t left = m - w * 64;
+            const uint64_t in = left >= 64 ? ~0ull : ((1ull << left) - 1);
+            words[w] = mask | ~in;
+            local += __popcll(in & ~mask);
+        }
     }
+    if (lane == 0 && local) atomicAdd(nulls, static_cast<unsigned long long>(local));
+}
+
+std::vector<qe_dev_col*> gather_cols(qe_gpu_ctx* ctx, const qe_dev_col* const* srcs, const uint32_t* const* idxs,
+                                     size_t n, uint64_t m) {
+    std::vector<qe_dev_col*> out;
+    try {
+        Scratch cnt(ctx, std::max<size_t>(n, 1) * 8);
+        if (m) memset_async(ctx, cnt.p, 0, n * 8);
+        const unsigned g = grid_for(words_for(m) * kWave, 256, ctx->num_cus * 16);
+        for (size_t c = 0; c < n; ++c) {
+            const qe_dev_col* src = srcs[c];
+            qe_dev_col* o = new_col(ctx, src->type, m, true);
+            out.push_back(o);
+            o->dict = src->dict;
+            if (!m) continue;
+            const uint64_t* sv = src->all_valid ? nullptr : src->validity;
+            unsigned long long* nc = cnt.as<unsigned long long>() + c;
+            switch (type_width(src->type)) {
+                case 1:
+                    gather_kernel<uint8_t><<<g, 256, 0, ctx->stream>>>(static_cast<const uint8_t*>(src->data), sv,
+                                                                       idxs[c], m, static_cast<uint8_t*>(o->data),
+                                                                       o->validity, nc);
+                    break;
+                case 4:
+                    gather_kernel<uint32_t><<<g, 256, 0, ctx->stream>>>(static_cast<const uint32_t*>(src->data), sv,
+                                                                        idxs[c], m, static_cast<uint32_t*>(o->data),
+                                                                        o->validity, nc);
+                    break;
+                default:
+                    gather_kernel<uint64_t><<<g, 256, 0, ctx->stream>>>(static_cast<const uint64_t*>(src->data), sv,
+                                                                        idxs[c], m, static_cast<uint64_t*>(o->data),
+                                                                        o->validity, nc);
+            }
+            QE_HIP(hipGetLastError());
+        }
+        // One read-back of every column's NULL count; a column without NULLs
+        // drops its bitmap (all_valid).
+        std::vector<uint64_t> nulls(n, 0);
+        if (m && n) {
+            uint64_t* h = static_cast<uint64_t*>(pinned(ctx, n * 8));
+            QE_HIP(hipMemcpyAsync(h, cnt.p, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+            QE_HIP(hipStreamSynchronize(ctx->stream));
+            std::copy(h, h + n, nulls.begin());
+        }
+        for (size_t c = 0; c < n; ++c) {
+            if (nulls[c] == 0) {
+                dev_free(ctx, out[c]->validity);
+                out[c]->validity = nullptr;
+                out[c]->all_valid = true;
+            } else {
+                out[c]->all_valid = false;
+            }
+        }
+    } catch (...) {
+        for (auto* c : out) free_col(ctx, c);
+        throw;
+    }
+    return out;
 }
 
 qe_dev_col* gather_col(qe_gpu_ctx* ctx, const qe_dev_col* src, const uint32_t* idx, uint64_t m) {
-    qe_dev_col* out = new_col(ctx, src->type, m, true);
-    out->dict = src->dict;
-    Scratch vb(ctx, std::max<uint64_t>(m, 1));
-    const uint64_t* sv = src->all_valid ? nullptr : src->validity;
-    const unsigned g = grid_for(m, 256, ctx->num_cus * 16);
-    if (m) {
-        switch (type_width(src->type)) {
-            case 1:
-                gather_kernel<uint8_t><<<g, 256, 0, ctx->stream>>>(static_cast<const uint8_t*>(src->data), sv, idx, m,
-                                                                   static_cast<uint8_t*>(out->data), vb.as<uint8_t>());
-                break;
-            case 4:
-                gather_kernel<uint32_t><<<g, 256, 0, ctx->stream>>>(static_cast<const uint32_t*>(src->data), sv, idx,
-                                                                    m, static_cast<uint32_t*>(out->data),
-                                                                    vb.as<uint8_t>());
-                break;
-            default:
-                gather_kernel<uint64_t><<<g, 256, 0, ctx->stream>>>(static_cast<const uint64_t*>(src->data), sv, idx,
-                                                                    m, static_cast<uint64_t*>(out->data),
-                                                                    vb.as<uint8_t>());
-        }
-        QE_HIP(hipGetLastError());
-    }
-    pack_validity(ctx, out, vb.as<uint8_t>());
-    return out;
+    return gather_cols(ctx, &src, &idx, 1, m)[0];
 }
 
 // ---- fused join -> group-by -----------------------------------------------------------
@@ -859,10 +912,17 @@ qe_gpu_status qe_gpu_pairs_gather(qe_gpu_ctx* ctx, const qe_dev_pairs* p, uint64
         common_len(build_cols, nbuild);
         qe_dev_table* t = new_table(count);
         try {
-            for (uint32_t c = 0; c < nprobe; ++c)
-                t->cols.push_back(gather_col(ctx, probe_cols[c], p->probe + offset, count));
-            for (uint32_t c = 0; c < nbuild; ++c)
-                t->cols.push_back(gather_col(ctx, build_cols[c], p->build + offset, count));
+            std::vector<const qe_dev_col*> src;
+            std::vector<const uint32_t*> idx;
+            for (uint32_t c = 0; c < nprobe; ++c) {
+                src.push_back(probe_cols[c]);
+                idx.push_back(p->probe + offset);
+            }
+            for (uint32_t c = 0; c < nbuild; ++c) {
+                src.push_back(build_cols[c]);
+                idx.push_back(p->build + offset);
+            }
+            t->cols = gather_cols(ctx, src.data(), idx.data(), src.size(), count);
         } catch (...) {
             qe_gpu_table_free(ctx, t);
             throw;
@@ -931,11 +991,17 @@ qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* pro
                 for (uint32_t a = 0; a < naggs; ++a)
                     if (aggs[a].func != QE_AGG_COUNT_STAR) need[aggs[a].input_col] = 1;
                 if (ngk == 0 && naggs > 0) need[0] = 1;  // keep the row count
+                std::vector<const qe_dev_col*> src;
+                std::vector<const uint32_t*> idx;
+                std::vector<uint32_t> at;
                 for (uint32_t c = 0; c < ncols; ++c) {
                     if (!need[c]) continue;
-                    gathered[c] = gather_col(ctx, col_at(c), c < nprobe ? pr->probe : pr->build, pr->count);
-                    owned.push_back(gathered[c]);
+                    src.push_back(col_at(c));
+                    idx.push_back(c < nprobe ? pr->probe : pr->build);
+                    at.push_back(c);
                 }
+                owned = gather_cols(ctx, src.data(), idx.data(), src.size(), pr->count);
+                for (size_t k = 0; k < at.size(); ++k) gathered[at[k]] = owned[k];
                 // Unreferenced positions get a referenced column of the right
                 // length so indices stay those of the join output.
                 qe_dev_col* any = owned.empty() ? nullptr : owned[0];

@@ -281,7 +281,8 @@ extern "C" qe_gpu_status qe_gpu_take(qe_gpu_ctx* ctx, const qe_dev_col* const* c
         }
         qe_dev_table* t = new_table(m);
         try {
-            for (uint32_t c = 0; c < ncols; ++c) t->cols.push_back(gather_col(ctx, cols[c], idx, m));
+            std::vector<const uint32_t*> idxs(ncols, idx);
+            t->cols = gather_cols(ctx, cols, idxs.data(), ncols, m);
         } catch (...) {
             qe_gpu_table_free(ctx, t);
             throw;
