@@ -1151,21 +1151,6 @@ static qe_gpu_status asof_probe_codes(qe_gpu_ctx* ctx, const qe_dev_asof* idx, c
     });
 }
 
-__global__ void asof_count_kernel(const uint32_t* match, uint64_t n, int left, uint32_t* flag) {
-    for (uint64_t j = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; j < n;
-         j += uint64_t(gridDim.x) * blockDim.x)
-        flag[j] = (left || match[j] != kNoRow) ? 1u : 0u;
-}
-__global__ void asof_emit_kernel(const uint32_t* match, uint64_t n, const uint32_t* flag, const uint64_t* pos,
-                                 uint32_t* pp, uint32_t* bp) {
-    for (uint64_t j = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; j < n;
-         j += uint64_t(gridDim.x) * blockDim.x) {
-        if (!flag[j]) continue;
-        pp[pos[j]] = static_cast<uint32_t>(j);
-        bp[pos[j]] = match[j];
-    }
-}
-
 }  // namespace qe_gpu
 
 extern "C" {
@@ -1202,17 +1187,8 @@ qe_gpu_status qe_gpu_asof_pairs(qe_gpu_ctx* ctx, const uint32_t* match, uint64_t
         auto* pr = new qe_dev_pairs();
         try {
             if (n) {
-                Scratch flag(ctx, n * 4), pos(ctx, n * 8);
-                asof_count_kernel<<<grid_for(n, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(
-                    match, n, join_type == QE_JOIN_LEFT, flag.as<uint32_t>());
-                QE_HIP(hipGetLastError());
-                const uint64_t total = exclusive_scan_u32(ctx, flag.as<uint32_t>(), pos.as<uint64_t>(), n);
-                pr->count = total;
-                pr->probe = static_cast<uint32_t*>(dev_alloc(ctx, std::max<uint64_t>(total, 1) * 4));
-                pr->build = static_cast<uint32_t*>(dev_alloc(ctx, std::max<uint64_t>(total, 1) * 4));
-                asof_emit_kernel<<<grid_for(n, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(
-                    match, n, flag.as<uint32_t>(), pos.as<uint64_t>(), pr->probe, pr->build);
-                QE_HIP(hipGetLastError());
+                invalid(match != nullptr, "null match array");
+                pr->count = select_pairs(ctx, match, n, join_type == QE_JOIN_LEFT, &pr->probe, &pr->build);
                 QE_HIP(hipStreamSynchronize(ctx->stream));
             }
         } catch (...) {

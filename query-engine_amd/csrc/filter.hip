@@ -397,6 +397,34 @@ void launch_compact(qe_gpu_ctx* ctx, const FilterMask& f, uint64_t n, const Comp
     QE_HIP(hipGetLastError());
 }
 
+// Join pairs from per-probe matches (kNoRow = no match): every row when
+// keep_all, else the rows with a match; probe = row index, build = match, in
+// row order. The same two passes as a filter on `match != kNoRow` with the row
+// indices as selection output and `match` as the one compacted column.
+uint64_t select_pairs(qe_gpu_ctx* ctx, const uint32_t* match, uint64_t n, bool keep_all, uint32_t** probe,
+                      uint32_t** build) {
+    check(n < kNoRow, QE_GPU_UNSUPPORTED, "row indices are 32-bit");
+    Pred P{};
+    if (!keep_all) {
+        P.n = 1;
+        P.t[0].c = ColView{match, nullptr, QE_I32, 0};
+        P.t[0].op = QE_CMP_NE;
+        P.t[0].as_float = 0;
+        P.t[0].li = static_cast<int32_t>(kNoRow);  // the I32 reading of kNoRow (-1)
+        P.t[0].lf = 0.0;
+    }
+    FilterMask f = filter_mask(ctx, P, n);
+    *probe = static_cast<uint32_t*>(dev_alloc(ctx, std::max<uint64_t>(f.total, 1) * 4));
+    *build = static_cast<uint32_t*>(dev_alloc(ctx, std::max<uint64_t>(f.total, 1) * 4));
+    if (f.total > 0) {
+        CompactCols C{};
+        C.n = 1;
+        C.c[0] = CompactCol{match, nullptr, *build, nullptr, 4, 0};
+        launch_compact(ctx, f, n, C, *probe, nullptr);
+    }
+    return f.total;
+}
+
 }  // namespace qe_gpu
 
 extern "C" {

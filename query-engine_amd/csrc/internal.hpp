@@ -365,6 +365,10 @@ void str_agg_finish(qe_gpu_ctx* ctx, qe_dev_table* t, const StrAggFix& fix);
 qe_gpu_status aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, uint32_t ncols, const uint32_t* key_cols,
                               uint32_t nkeys, const qe_agg_spec* aggs, uint32_t naggs, qe_dev_table** out);
 qe_dev_col* gather_col(qe_gpu_ctx* ctx, const qe_dev_col* src, const uint32_t* idx, uint64_t m);
+// Join pairs from per-probe matches (kNoRow = no match), in row order: every
+// row when keep_all, else the matched ones; probe/build are dev_alloc'ed.
+uint64_t select_pairs(qe_gpu_ctx* ctx, const uint32_t* match, uint64_t n, bool keep_all, uint32_t** probe,
+                      uint32_t** build);
 // Column c of the result = srcs[c] gathered by idxs[c] (m rows each), with one
 // host read-back for all of their NULL counts.
 std::vector<qe_dev_col*> gather_cols(qe_gpu_ctx* ctx, const qe_dev_col* const* srcs, const uint32_t* const* idxs,

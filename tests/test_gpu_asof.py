@@ -135,6 +135,30 @@ def test_duplicate_time_runs_across_pieces(ctx, qe, piece_rows):
               piece_rows=piece_rows)
 
 
+@pytest.mark.parametrize("n", [1, 63, 4095, 4096, 4097, 100_003])
+def test_pairs_from_match_arrays(ctx, qe, n):
+    # Per-probe matches to join pairs: INNER keeps the matched probes, LEFT
+    # every probe with kNoRow for the unmatched; both in probe order. Match
+    # arrays with none, all and a random half matched, across tile edges.
+    r = np.random.default_rng(n)
+    for frac in (0.0, 1.0, 0.5):
+        match = np.where(r.random(n) < frac, r.integers(0, 1 << 31, n), NO).astype(np.uint32)
+        buf = qe.DevBuffer(ctx, 4 * n)
+        try:
+            buf.upload(match)
+            for jt in (qe.INNER, qe.LEFT):
+                pr = ctx.asof_pairs(buf, n, jt)
+                try:
+                    p, b = pr.numpy()
+                finally:
+                    pr.free()
+                keep = np.arange(n) if jt == qe.LEFT else np.nonzero(match != NO)[0]
+                assert p.tolist() == keep.tolist()
+                assert b.tolist() == match[keep].tolist()
+        finally:
+            buf.free()
+
+
 def test_hash_partitioned_keys(ctx, qe):
     r = np.random.default_rng(77)
     nb, np_ = 20000, 20000
