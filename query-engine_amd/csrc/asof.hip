@@ -75,6 +75,7 @@ struct qe_dev_asof {
     uint32_t B = 0, kpb = 1;  // rows per piece, knots per router bucket
     uint64_t nknots = 0, ntab = 0;
     Scratch meta, knots, tab_mlp, tab_lin, params, grid;
+    Scratch bkt_mlp, bkt_lin;  // per router-table entry: the bucket's knots inline (BucketRec x 8)
     double final_loss = 0, mean_knots_mlp = 0, mean_knots_lin = 0;
     uint64_t max_err = 0, mlp_candidates = 0, mlp_parts = 0;
     Scratch fallback;  // u64 counter of the last probe
@@ -267,6 +268,25 @@ struct Knot {
     uint32_t err;  // max |predicted - actual| position over the piece's rows
 };
 
+// A router bucket's knots inline in one 128-byte cache line, so the probe's
+// routing reads one line instead of the table entry and then the knot records.
+// Record 0 is the last knot of an earlier bucket (the piece of a time before
+// the bucket's first knot; tau = INT64_MIN when there is none), records 1..7
+// the bucket's first knots in order; unused records hold tau = INT64_MAX. The
+// eight times come first (four 16-byte loads), then (slope, piece) pairs. A
+// bucket of more than 7 knots sets kBucketMore in record 7's piece: a time at
+// or past that knot is searched through the table and the knot records.
+constexpr int kBucketRecs = 8;
+constexpr uint32_t kBucketMore = 0x80000000u;
+struct SlopePiece {
+    float slope;
+    uint32_t piece;  // knot index within the partition (| kBucketMore)
+};
+struct BucketLine {
+    int64_t tau[kBucketRecs];
+    SlopePiece sp[kBucketRecs];
+};
+
 // Position inside a piece predicted for t >= tau, in [0, cnt). Non-decreasing
 // in t: exact unsigned difference, monotone conversion and fmaf, clamps.
 __device__ inline uint32_t piece_pred(int64_t tau, float slope, uint32_t cnt, int64_t t) {
@@ -406,6 +426,31 @@ __global__ void router_table_kernel(const PartMeta* meta, const uint64_t* kb, ui
     }
 }
 
+// Inline bucket lines from the router table (entries tb .. tb + R - 1 of each
+// partition; entry tb + R is the table's end marker and gets no line).
+__global__ void bucket_lines_kernel(const PartMeta* meta, const uint64_t* tbs, uint64_t nparts, uint64_t nt,
+                                    const Knot* knots, const uint32_t* tab, BucketLine* out) {
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < nt;
+         i += uint64_t(gridDim.x) * blockDim.x) {
+        const uint32_t p = part_of_knot(tbs, nparts + 1, i);
+        const PartMeta pm = meta[p];
+        if (i - pm.tb >= pm.R) continue;
+        const uint32_t lo = tab[i], hi = tab[i + 1];
+        const Knot* kp = knots + pm.kb;
+        BucketLine L;
+        SlopePiece* sp = L.sp;
+        L.tau[0] = lo > 0 ? kp[lo - 1].tau : INT64_MIN;
+        sp[0] = SlopePiece{lo > 0 ? kp[lo - 1].slope : 0.0f, lo > 0 ? lo - 1 : 0u};
+        for (uint32_t k = 1; k < kBucketRecs; ++k) {
+            const uint32_t j = lo + k - 1;
+            L.tau[k] = j < hi ? kp[j].tau : INT64_MAX;
+            sp[k] = SlopePiece{j < hi ? kp[j].slope : 0.0f, j < hi ? j : 0u};
+        }
+        if (hi - lo > kBucketRecs - 1) sp[kBucketRecs - 1].piece |= kBucketMore;
+        out[i] = L;
+    }
+}
+
 // Knots a probe searches, summed per partition and overall (probes assumed to
 // be distributed like the knots): for each knot, its bucket's occupancy + 1
 // (the predecessor boundary). Partitions are contiguous knot ranges, so a wave
@@ -460,6 +505,7 @@ struct LearnedArgs {
     const PartMeta* meta;
     const Knot* knots;
     const uint32_t* tab;
+    const BucketLine* bkt;
     const float* grid;
     int use_mlp;
     uint32_t B;
@@ -523,37 +569,62 @@ __global__ void asof_learned_kernel(PK pk, ProbeArgs a, LearnedArgs L) {
             a.out[j] = kNoRow;
             continue;
         }
-        // 1. route to a knot range and locate the piece (tau[piece] <= t < tau[piece+1]).
+        // 1. route to a bucket and locate the piece (tau[piece] <= t < tau[piece+1]).
+        //    The router is monotone in t and built the buckets, so the piece is
+        //    the last of the bucket's records (its knots and the predecessor)
+        //    with tau <= t.
         const bool mlp = L.use_mlp && pm.mlp;
         const uint32_t b = bucket_of(pm, L.grid + size_t(p) * (kGrid + 1), mlp, t);
-        const uint32_t t_lo = L.tab[pm.tb + b], t_hi = L.tab[pm.tb + b + 1];
-        uint32_t lo = t_lo > 0 ? t_lo - 1 : 0;
-        uint32_t hi = t_hi > 0 ? t_hi - 1 : 0;
-        if (hi >= pm.K) hi = pm.K - 1;
-        if (lo > hi) lo = hi;
-        // Search the bucket's range, then verify the piece by its own bound and
-        // its successor's (the records the search ended on): the piece is the
-        // last knot with tau <= t, found in the whole partition if the bucket
-        // range did not hold it.
-        uint32_t piece = knot_search(L.knots, pm.kb, lo, hi, t);
-        Knot kn = L.knots[pm.kb + piece];
-        if (!(kn.tau <= t && (piece + 1 >= pm.K || L.knots[pm.kb + piece + 1].tau > t))) {
-            ++fb;
-            piece = knot_search(L.knots, pm.kb, 0, pm.K - 1, t);
-            kn = L.knots[pm.kb + piece];
+        const BucketLine* bl = L.bkt + (pm.tb + b);
+        typedef int64_t i64x2 __attribute__((ext_vector_type(2)));
+        const i64x2* tv = reinterpret_cast<const i64x2*>(bl->tau);
+        int64_t rt[kBucketRecs];
+#pragma unroll
+        for (int k = 0; k < kBucketRecs / 2; ++k) {
+            const i64x2 v = tv[k];
+            rt[2 * k] = v.x;
+            rt[2 * k + 1] = v.y;
+        }
+        uint32_t c0 = 0;
+#pragma unroll
+        for (int k = 0; k < kBucketRecs; ++k) c0 += rt[k] <= t ? 1u : 0u;
+        SlopePiece sp{0.0f, kBucketMore};
+        if (c0 > 0) sp = bl->sp[c0 - 1];
+        uint32_t piece;
+        int64_t tau;
+        float slope;
+        if (!(sp.piece & kBucketMore)) {
+            piece = sp.piece;
+            tau = rt[0];
+#pragma unroll
+            for (int k = 1; k < kBucketRecs; ++k)
+                if (static_cast<uint32_t>(k) == c0 - 1) tau = rt[k];
+            slope = sp.slope;
+        } else {
+            // At or past the 8th record of a bucket of more than 7 knots: search its knot range, then verify
+            // the piece by its own bound and its successor's (the records the
+            // search ended on), else search the partition's whole knot list.
+            const uint32_t t_lo = L.tab[pm.tb + b], t_hi = L.tab[pm.tb + b + 1];
+            uint32_t lo = t_lo > 0 ? t_lo - 1 : 0;
+            uint32_t hi = t_hi > 0 ? t_hi - 1 : 0;
+            if (hi >= pm.K) hi = pm.K - 1;
+            if (lo > hi) lo = hi;
+            piece = knot_search(L.knots, pm.kb, lo, hi, t);
+            Knot kn = L.knots[pm.kb + piece];
+            if (!(kn.tau <= t && (piece + 1 >= pm.K || L.knots[pm.kb + piece + 1].tau > t))) {
+                ++fb;
+                piece = knot_search(L.knots, pm.kb, 0, pm.K - 1, t);
+                kn = L.knots[pm.kb + piece];
+            }
+            tau = kn.tau;
+            slope = kn.slope;
         }
         // 2. inside the piece, search outward from the spline's prediction; the
-        //    result is exact wherever the prediction lands. A result outside the
-        //    piece's recorded error bound counts as a fallback (never, by the
-        //    bound's construction; the count stays as a check of the index).
+        //    result is exact wherever the prediction lands.
         const uint64_t base = pm.seg + uint64_t(piece) * L.B;
         const uint32_t cnt = min(L.B, pm.n - piece * L.B);
-        const uint32_t pred = piece_pred(kn.tau, kn.slope, cnt, t);
+        const uint32_t pred = piece_pred(tau, slope, cnt, t);
         const uint32_t c = gallop_count(a.ts, base, cnt, pred, t);
-        const uint32_t e = kn.err + 1;
-        const uint32_t wlo = pred > e ? pred - e : 0;
-        const uint32_t whi = min(cnt - 1, pred + e);
-        fb += !((wlo == 0 || c > wlo) && (whi == cnt - 1 || c <= whi + 1));
         // tau[piece] <= t guarantees c >= 1.
         finish_match(a, j, t, base + c - 1, c > 0);
     }
@@ -1002,6 +1073,7 @@ qe_gpu_status qe_gpu_asof_train(qe_gpu_ctx* ctx, qe_dev_asof* idx, uint32_t piec
         const uint64_t nk = exclusive_scan_u32(ctx, K.as<uint32_t>(), kb.as<uint64_t>(), P);
         const uint64_t nt = exclusive_scan_u32(ctx, R1.as<uint32_t>(), tb.as<uint64_t>(), P);
         QE_HIP(hipMemcpyAsync(kb.as<uint64_t>() + P, &nk, 8, hipMemcpyHostToDevice, ctx->stream));
+        QE_HIP(hipMemcpyAsync(tb.as<uint64_t>() + P, &nt, 8, hipMemcpyHostToDevice, ctx->stream));
         idx->nknots = nk;
         idx->ntab = nt;
         idx->meta = Scratch(ctx, P * sizeof(PartMeta));
@@ -1013,6 +1085,8 @@ qe_gpu_status qe_gpu_asof_train(qe_gpu_ctx* ctx, qe_dev_asof* idx, uint32_t piec
         idx->knots = Scratch(ctx, nk1 * sizeof(Knot));
         idx->tab_mlp = Scratch(ctx, std::max<uint64_t>(nt, 1) * 4);
         idx->tab_lin = Scratch(ctx, std::max<uint64_t>(nt, 1) * 4);
+        idx->bkt_mlp = Scratch(ctx, std::max<uint64_t>(nt, 1) * sizeof(BucketLine));
+        idx->bkt_lin = Scratch(ctx, std::max<uint64_t>(nt, 1) * sizeof(BucketLine));
         idx->params = Scratch(ctx, std::max<uint64_t>(P, 1) * kParams * 4);
         idx->grid = Scratch(ctx, std::max<uint64_t>(P, 1) * (kGrid + 1) * 4);
         Scratch losses(ctx, std::max<uint64_t>(P, 1) * 4), maxe(ctx, 8), sums(ctx, 24),
@@ -1059,6 +1133,12 @@ qe_gpu_status qe_gpu_asof_train(qe_gpu_ctx* ctx, qe_dev_asof* idx, uint32_t piec
             router_stats_kernel<<<gk, 256, 0, ctx->stream>>>(meta, kb.as<uint64_t>(), P, nk, knots, grid, 1,
                                                              idx->tab_mlp.as<uint32_t>(), nullptr,
                                                              sums.as<double>() + 1);
+            QE_HIP(hipGetLastError());
+            const unsigned gt = grid_for(nt, 256, ctx->num_cus * 16);
+            bucket_lines_kernel<<<gt, 256, 0, ctx->stream>>>(meta, tb.as<uint64_t>(), P, nt, knots,
+                                                             idx->tab_mlp.as<uint32_t>(), idx->bkt_mlp.as<BucketLine>());
+            bucket_lines_kernel<<<gt, 256, 0, ctx->stream>>>(meta, tb.as<uint64_t>(), P, nt, knots,
+                                                             idx->tab_lin.as<uint32_t>(), idx->bkt_lin.as<BucketLine>());
             QE_HIP(hipGetLastError());
         }
         std::vector<float> hl(std::max<uint64_t>(P, 1));
@@ -1136,10 +1216,11 @@ static qe_gpu_status asof_probe_codes(qe_gpu_ctx* ctx, const qe_dev_asof* idx, c
         } else {
             auto* mi = const_cast<qe_dev_asof*>(idx);
             memset_async(ctx, mi->fallback.p, 0, 8);
+            const bool lm = mode == QE_ASOF_LEARNED;
             LearnedArgs L{idx->meta.as<PartMeta>(), idx->knots.as<Knot>(),
-                          mode == QE_ASOF_LEARNED ? idx->tab_mlp.as<uint32_t>() : idx->tab_lin.as<uint32_t>(),
-                          idx->grid.as<float>(), mode == QE_ASOF_LEARNED ? 1 : 0, idx->B,
-                          mi->fallback.as<unsigned long long>()};
+                          lm ? idx->tab_mlp.as<uint32_t>() : idx->tab_lin.as<uint32_t>(),
+                          lm ? idx->bkt_mlp.as<BucketLine>() : idx->bkt_lin.as<BucketLine>(), idx->grid.as<float>(),
+                          lm ? 1 : 0, idx->B, mi->fallback.as<unsigned long long>()};
             {
                 TimedRegion tr(ctx, mode == QE_ASOF_LEARNED ? "asof_probe_learned" : "asof_probe_learned_no_mlp");
                 with_part([&](auto part) { asof_learned_kernel<<<g, 256, 0, ctx->stream>>>(part, a, L); });

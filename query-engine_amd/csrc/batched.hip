@@ -774,7 +774,7 @@ static void build_batch_plan(qe_gpu_ctx* ctx, BatchPlan& bp, const qe_batch_quer
                 if (lds_bytes(static_cast<uint32_t>(S), d.p) <= kSmallLds) {
                     d.kind = BK_SMALL;
                     bp.small_lds = std::max(bp.small_lds, lds_bytes(static_cast<uint32_t>(S), d.p));
-                } else if (on_value) {
+                } else if (on_value && n < (uint64_t(1) << 32)) {  // LDS row counters are 32-bit
                     uint32_t pb = 16;
                     while (pb > 8 && lds_bytes(1u << pb, d.p) > kPartLds) --pb;
                     const uint64_t np64 = (S + (uint64_t(1) << pb) - 1) >> pb;
@@ -930,7 +930,8 @@ static void run_batch(qe_gpu_ctx* ctx, BatchPlan& bp, int mode, const qe_batch_t
     check(C == 0 || (C & (C - 1)) == 0, QE_GPU_INVALID_ARGUMENT, "lds_slots must be 0 or a power of two");
     check(cache_bytes(C) <= 64 * 1024, QE_GPU_INVALID_ARGUMENT, "lds_slots too large");
     const uint32_t blocks = std::max<uint32_t>(1, std::min<uint32_t>(tn.blocks_per_cu, 16)) * ctx->num_cus;
-    make_layout(ctx, bp, std::max<uint32_t>(256, tn.tile_rows));
+    // Tiles of at most 2^22 rows (2^24 for small tables) keep the per-tile LDS counters in 32 bits.
+    make_layout(ctx, bp, std::min<uint32_t>(std::max<uint32_t>(256, tn.tile_rows), 1u << 22));
     if (mode == QE_BATCH_PER_BATCH) {
         batch_reset_kernel<<<1, 1, 0, ctx->stream>>>(bp.next.as<unsigned long long>(), bp.ovf.as<unsigned int>());
         batch_init_kernel<<<ctx->num_cus * 4, 256, 0, ctx->stream>>>(bp.dq.as<QueryDev>(), bp.nq, 1);
