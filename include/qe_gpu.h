@@ -64,9 +64,10 @@ typedef enum qe_join_type { QE_JOIN_INNER = 0, QE_JOIN_LEFT = 1 } qe_join_type;
 typedef enum qe_null_policy { QE_NULL_EQUAL = 0, QE_NULL_NEVER_MATCH = 1 } qe_null_policy;
 
 /* As-of search method: plain per-probe binary search (the exact answer the
- * learned index must reproduce) or the
- * learned per-key index (spline pieces + monotone MLP router, verified
- * window search, binary-search fallback). */
+ * learned index must reproduce) or the learned per-key index (spline pieces +
+ * monotone MLP router over buckets holding their knots inline, a search
+ * outward from the piece's predicted row, a verified knot-range search with a
+ * whole-partition fallback for buckets of more than seven knots). */
 typedef enum qe_asof_mode { QE_ASOF_BINARY = 0, QE_ASOF_LEARNED = 1, QE_ASOF_LEARNED_NO_MLP = 2 } qe_asof_mode;
 
 /* Sentinel for "no match" in match-index outputs: kNoGroup / kNullBuildRow
@@ -307,7 +308,8 @@ typedef struct qe_asof_stats {
     uint64_t router_buckets;   /* radix buckets addressed by the router          */
     double mean_bucket_knots;  /* mean knots searched per probe bucket (learned router: MLP where it helps) */
     double mean_bucket_knots_linear; /* same with the linear (no-MLP) router everywhere */
-    uint64_t fallbacks;        /* probes that fell back to binary search (last probe) */
+    uint64_t fallbacks;        /* probes whose piece was not in its bucket's knot range and was
+                                  searched over the whole partition (last probe) */
     double final_loss;         /* mean router training loss over trained partitions */
     uint64_t mlp_candidates;   /* partitions with a trained router               */
     uint64_t mlp_partitions;   /* partitions whose learned probe uses the MLP    */
