@@ -129,25 +129,29 @@ qe_dev_table* new_table(uint64_t rows) {
     return t;
 }
 
+// One wave per validity word: lane l reads row 64w + l's byte, the word is the
+// wave's ballot (padding bits past n set: the all-valid padding convention),
+// and lane 0 counts the word's NULLs.
+__device__ inline uint64_t pack_word(const uint8_t* vb, uint64_t n, uint64_t w, unsigned long long& nulls) {
+    const uint64_t r = w * 64 + lane_id();
+    const bool in = r < n;
+    const bool v = !in || vb[r] != 0;
+    const uint64_t bits = __ballot(v);
+    const uint64_t inmask = __ballot(in);
+    nulls += static_cast<unsigned long long>(__popcll(inmask & ~bits));
+    return bits;
+}
+
 __global__ void pack_validity_kernel(const uint8_t* vb, uint64_t n, uint64_t* words,
                                      unsigned long long* nulls) {
     const uint64_t nw = words_for(n);
-    uint64_t local = 0;
-    for (uint64_t w = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; w < nw;
-         w += uint64_t(gridDim.x) * blockDim.x) {
-        uint64_t bits = 0;
-        const uint64_t base = w * 64;
-        for (unsigned b = 0; b < 64; ++b) {
-            const uint64_t r = base + b;
-            // Padding bits past n are set (canonical all-valid padding).
-            const bool v = (r >= n) || vb[r];
-            bits |= (uint64_t(v) << b);
-            if (r < n && !vb[r]) ++local;
-        }
-        words[w] = bits;
+    const uint64_t waves = uint64_t(gridDim.x) * (blockDim.x / kWave);
+    unsigned long long local = 0;
+    for (uint64_t w = blockIdx.x * uint64_t(blockDim.x / kWave) + threadIdx.x / kWave; w < nw; w += waves) {
+        const uint64_t bits = pack_word(vb, n, w, local);
+        if (lane_id() == 0) words[w] = bits;
     }
-    for (int off = kWave / 2; off > 0; off >>= 1) local += __shfl_down(local, off);
-    if (lane_id() == 0 && local) atomicAdd(nulls, static_cast<unsigned long long>(local));
+    if (lane_id() == 0 && local) atomicAdd(nulls, local);
 }
 
 // Several columns at once: one launch, one read-back of the NULL counts.
@@ -160,21 +164,16 @@ struct PackSet {
 
 __global__ void pack_validity_many_kernel(PackSet ps, unsigned long long* nulls) {
     const uint64_t nw = words_for(ps.len);
-    for (uint64_t t = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; t < nw * ps.n;
-         t += uint64_t(gridDim.x) * blockDim.x) {
+    const uint64_t waves = uint64_t(gridDim.x) * (blockDim.x / kWave);
+    for (uint64_t t = blockIdx.x * uint64_t(blockDim.x / kWave) + threadIdx.x / kWave; t < nw * ps.n; t += waves) {
         const int c = static_cast<int>(t / nw);
         const uint64_t w = t - uint64_t(c) * nw;
-        uint64_t bits = 0;
         unsigned long long local = 0;
-        const uint64_t base = w * 64;
-        for (unsigned b = 0; b < 64; ++b) {
-            const uint64_t r = base + b;
-            const bool v = (r >= ps.len) || ps.vb[c][r];
-            bits |= (uint64_t(v) << b);
-            local += (r < ps.len && !ps.vb[c][r]) ? 1 : 0;
+        const uint64_t bits = pack_word(ps.vb[c], ps.len, w, local);
+        if (lane_id() == 0) {
+            ps.words[c][w] = bits;
+            if (local) atomicAdd(nulls + c, local);
         }
-        ps.words[c][w] = bits;
-        if (local) atomicAdd(nulls + c, local);
     }
 }
 
@@ -195,7 +194,7 @@ void pack_validity_many(qe_gpu_ctx* ctx, qe_dev_col* const* cols, const uint8_t*
     }
     Scratch cnt(ctx, size_t(ncol) * 8);
     memset_async(ctx, cnt.p, 0, size_t(ncol) * 8);
-    pack_validity_many_kernel<<<grid_for(words_for(len) * ncol, 64, 2048), 64, 0, ctx->stream>>>(
+    pack_validity_many_kernel<<<grid_for(words_for(len) * ncol * kWave, 256, 2048), 256, 0, ctx->stream>>>(
         ps, cnt.as<unsigned long long>());
     QE_HIP(hipGetLastError());
     uint64_t* h = static_cast<uint64_t*>(pinned(ctx, size_t(ncol) * 8));
@@ -220,7 +219,7 @@ void pack_validity(qe_gpu_ctx* ctx, qe_dev_col* col, const uint8_t* valid_bytes)
     }
     Scratch cnt(ctx, 8);
     memset_async(ctx, cnt.p, 0, 8);
-    pack_validity_kernel<<<grid_for(words_for(col->len), 256, 2048), 256, 0, ctx->stream>>>(
+    pack_validity_kernel<<<grid_for(words_for(col->len) * kWave, 256, 2048), 256, 0, ctx->stream>>>(
         valid_bytes, col->len, col->validity, cnt.as<unsigned long long>());
     QE_HIP(hipGetLastError());
     const uint64_t nulls = read_scalar(ctx, cnt.as<uint64_t>());
@@ -378,22 +377,29 @@ uint64_t exclusive_scan_u64(qe_gpu_ctx* ctx, const uint64_t* in, uint64_t* out, 
 }
 
 // ---- min/max over valid rows -----------------------------------------------------
-// Per-block (min, max, count) of the valid values, then one block folds the
-// partials (no contended atomics: 2048 blocks hitting one address serialise).
-__global__ void __launch_bounds__(256) minmax_kernel(ColView c, uint64_t n, int64_t* part) {
-    __shared__ int64_t smn[kMaxWaves], smx[kMaxWaves];
-    __shared__ uint64_t scnt[kMaxWaves];
-    int64_t mn = INT64_MAX, mx = INT64_MIN;
-    uint64_t cnt = 0;
+// One launch for up to kMaxMinMaxCols columns (grid.y = column): per-block (min,
+// max, count) of the valid values, with eight independent typed loads in flight
+// per thread, then one block per column folds the partials (no contended
+// atomics: 2048 blocks hitting one address serialise).
+constexpr uint32_t kMaxMinMaxCols = 4;
+struct MinMaxCols {
+    ColView c[kMaxMinMaxCols];
+    int32_t width[kMaxMinMaxCols];
+};
+
+template <int W>
+__device__ inline void minmax_rows(const ColView& c, uint64_t n, int64_t& mn, int64_t& mx, uint64_t& cnt) {
+    const BitsView vb = bits_view(c.validity);
     const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-    uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x;
-    for (; i + 7 * stride < n; i += 8 * stride) {  // eight independent loads in flight
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += 8 * stride) {
         int64_t v[8];
-        bool ok[8];
+        uint32_t ok[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            ok[k] = is_valid(c, i + k * stride);
-            v[k] = int_value(c, i + k * stride);
+            const uint64_t r = i + k * stride;
+            const uint64_t rc = r < n ? r : n - 1;
+            v[k] = static_cast<int64_t>(elem_bits<W>(c.data, rc));
+            ok[k] = bit_at(vb, rc) & (r < n ? 1u : 0u);
         }
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
@@ -403,48 +409,12 @@ __global__ void __launch_bounds__(256) minmax_kernel(ColView c, uint64_t n, int6
             ++cnt;
         }
     }
-    for (; i < n; i += stride) {
-        if (!is_valid(c, i)) continue;
-        const int64_t v = int_value(c, i);
-        mn = v < mn ? v : mn;
-        mx = v > mx ? v : mx;
-        ++cnt;
-    }
-    for (int off = kWave / 2; off > 0; off >>= 1) {
-        const int64_t a = __shfl_down(mn, off), b = __shfl_down(mx, off);
-        const uint64_t d = __shfl_down(cnt, off);
-        mn = a < mn ? a : mn;
-        mx = b > mx ? b : mx;
-        cnt += d;
-    }
-    const int w = threadIdx.x / kWave;
-    if (lane_id() == 0) {
-        smn[w] = mn;
-        smx[w] = mx;
-        scnt[w] = cnt;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int k = 1; k < static_cast<int>(blockDim.x / kWave); ++k) {
-            mn = smn[k] < mn ? smn[k] : mn;
-            mx = smx[k] > mx ? smx[k] : mx;
-            cnt += scnt[k];
-        }
-        part[3 * blockIdx.x] = mn;
-        part[3 * blockIdx.x + 1] = mx;
-        part[3 * blockIdx.x + 2] = static_cast<int64_t>(cnt);
-    }
 }
-__global__ void __launch_bounds__(256) minmax_final(const int64_t* part, uint32_t nparts, int64_t* out) {
+
+// Block-wide (min, max, count) into out[0..2] by thread 0.
+__device__ inline void minmax_block_store(int64_t mn, int64_t mx, uint64_t cnt, int64_t* out) {
     __shared__ int64_t smn[kMaxWaves], smx[kMaxWaves];
     __shared__ uint64_t scnt[kMaxWaves];
-    int64_t mn = INT64_MAX, mx = INT64_MIN;
-    uint64_t cnt = 0;
-    for (uint32_t b = threadIdx.x; b < nparts; b += blockDim.x) {
-        mn = part[3 * b] < mn ? part[3 * b] : mn;
-        mx = part[3 * b + 1] > mx ? part[3 * b + 1] : mx;
-        cnt += static_cast<uint64_t>(part[3 * b + 2]);
-    }
     for (int off = kWave / 2; off > 0; off >>= 1) {
         const int64_t a = __shfl_down(mn, off), b = __shfl_down(mx, off);
         const uint64_t d = __shfl_down(cnt, off);
@@ -471,6 +441,35 @@ __global__ void __launch_bounds__(256) minmax_final(const int64_t* part, uint32_
     }
 }
 
+__global__ void __launch_bounds__(256) minmax_kernel(MinMaxCols C, uint64_t n, int64_t* part) {
+    const uint32_t y = blockIdx.y;
+    int64_t mn = INT64_MAX, mx = INT64_MIN;
+    uint64_t cnt = 0;
+    switch (C.width[y]) {
+        case 1:
+            minmax_rows<1>(C.c[y], n, mn, mx, cnt);
+            break;
+        case 4:
+            minmax_rows<4>(C.c[y], n, mn, mx, cnt);
+            break;
+        default:
+            minmax_rows<8>(C.c[y], n, mn, mx, cnt);
+    }
+    minmax_block_store(mn, mx, cnt, part + 3 * (size_t(y) * gridDim.x + blockIdx.x));
+}
+// Block y folds column y's nparts partials into out[3y..3y+2].
+__global__ void __launch_bounds__(256) minmax_final(const int64_t* part, uint32_t nparts, int64_t* out) {
+    const int64_t* p = part + 3 * size_t(blockIdx.x) * nparts;
+    int64_t mn = INT64_MAX, mx = INT64_MIN;
+    uint64_t cnt = 0;
+    for (uint32_t b = threadIdx.x; b < nparts; b += blockDim.x) {
+        mn = p[3 * b] < mn ? p[3 * b] : mn;
+        mx = p[3 * b + 1] > mx ? p[3 * b + 1] : mx;
+        cnt += static_cast<uint64_t>(p[3 * b + 2]);
+    }
+    minmax_block_store(mn, mx, cnt, out + 3 * blockIdx.x);
+}
+
 // XCD (accelerator complex die) that runs each block: bits [3:0] of the
 // XCC_ID hardware register (s_getreg; id 20, offset 0, size 4).
 __global__ void xcd_of_block_kernel(uint32_t* out) {
@@ -484,14 +483,21 @@ MinMax column_minmax(qe_gpu_ctx* ctx, const qe_dev_col* c, uint64_t n) {
 }
 
 void column_minmax_n(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, uint32_t ncol, uint64_t n, MinMax* out) {
+    check(ncol >= 1 && ncol <= kMaxMinMaxCols, QE_GPU_UNSUPPORTED, "min/max of too many columns at once");
+    if (n == 0) {
+        for (uint32_t c = 0; c < ncol; ++c) out[c] = MinMax{INT64_MAX, INT64_MIN, 0};
+        return;
+    }
     // About 8-16 rows per thread: one or two batches of eight independent loads.
     const uint32_t blocks = static_cast<uint32_t>(grid_for((n + 7) / 8, 256, ctx->num_cus * 16));
     Scratch part(ctx, size_t(blocks) * 3 * 8 * ncol), s(ctx, size_t(3) * 8 * ncol);
+    MinMaxCols C{};
     for (uint32_t c = 0; c < ncol; ++c) {
-        minmax_kernel<<<blocks, 256, 0, ctx->stream>>>(view(cols[c]), n, part.as<int64_t>() + size_t(c) * blocks * 3);
-        minmax_final<<<1, 256, 0, ctx->stream>>>(part.as<int64_t>() + size_t(c) * blocks * 3, blocks,
-                                                 s.as<int64_t>() + 3 * c);
+        C.c[c] = view(cols[c]);
+        C.width[c] = static_cast<int32_t>(type_width(cols[c]->type));
     }
+    minmax_kernel<<<dim3(blocks, ncol), 256, 0, ctx->stream>>>(C, n, part.as<int64_t>());
+    minmax_final<<<ncol, 256, 0, ctx->stream>>>(part.as<int64_t>(), blocks, s.as<int64_t>());
     QE_HIP(hipGetLastError());
     int64_t* h = static_cast<int64_t*>(pinned(ctx, size_t(24) * ncol));
     QE_HIP(hipMemcpyAsync(h, s.p, size_t(24) * ncol, hipMemcpyDeviceToHost, ctx->stream));

@@ -203,61 +203,176 @@ __device__ inline void store_typed(void* base, int type, uint64_t o, uint64_t bi
     }
 }
 
+// Output row o from slot s: keys then finalised aggregates; bit c of the
+// result is column c's validity.
+__device__ inline uint32_t emit_row(const uint64_t* st, uint64_t S, const AggProg& p, const KeyEmit& ke,
+                                    const OutCols& oc, uint64_t s, uint64_t o) {
+    uint32_t vbits = 0;
+    for (int k = 0; k < ke.nkeys; ++k) {
+        bool isnull;
+        uint64_t w;
+        if (ke.mode == 0) {
+            isnull = (s == ke.null_slot);
+            w = static_cast<uint64_t>(ke.kmin + static_cast<int64_t>(s));
+        } else {
+            isnull = (ke.nullmask[s] >> k) & 1ull;
+            w = ke.words[size_t(k) * S + s];
+        }
+        vbits |= (isnull ? 0u : 1u) << k;
+        store_typed(oc.data[k], ke.types[k], o, isnull ? 0 : w);
+    }
+    const uint64_t rows = st[s];
+    for (int a = 0; a < p.n; ++a) {
+        const int c = ke.nkeys + a;
+        const uint64_t acc = p.acc_of[a] >= 0 ? st[(1 + size_t(p.acc_of[a])) * S + s] : 0;
+        const uint64_t cnt = p.cnt_of[a] >= 0 ? st[(1 + size_t(p.nacc) + p.cnt_of[a]) * S + s] : rows;
+        const int f = p.func[a];
+        const int k = p.kind[a];
+        uint64_t out = 0;
+        bool valid = true;
+        if (f == QE_AGG_COUNT_STAR) {
+            out = rows;
+        } else if (f == QE_AGG_COUNT) {
+            out = cnt;
+        } else if (cnt == 0) {
+            valid = false;  // no non-NULL input: SUM/MIN/MAX/AVG are NULL
+        } else if (f == QE_AGG_AVG) {
+            double sum;
+            if (k == K_SUM_F) memcpy(&sum, &acc, 8);
+            else sum = static_cast<double>(static_cast<int64_t>(acc));
+            const double avg = sum / static_cast<double>(cnt);
+            memcpy(&out, &avg, 8);
+        } else if (k == K_MIN_F || k == K_MAX_F) {
+            const double d = f64_from_order_key(acc);
+            memcpy(&out, &d, 8);
+        } else {
+            out = acc;
+        }
+        vbits |= (valid ? 1u : 0u) << c;
+        store_typed(oc.data[c], oc.type[c], o, valid ? out : 0);
+    }
+    return vbits;
+}
+
 __global__ void emit_kernel(const uint64_t* st, uint64_t S, AggProg p, KeyEmit ke,
                             const uint32_t* flags, const uint64_t* offs, OutCols oc) {
+    const int ncol = ke.nkeys + p.n;
     for (uint64_t s = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; s < S;
          s += uint64_t(gridDim.x) * blockDim.x) {
         if (!flags[s]) continue;
         const uint64_t o = offs[s];
-        for (int k = 0; k < ke.nkeys; ++k) {
-            bool isnull;
-            uint64_t w;
-            if (ke.mode == 0) {
-                isnull = (s == ke.null_slot);
-                w = static_cast<uint64_t>(ke.kmin + static_cast<int64_t>(s));
-            } else {
-                isnull = (ke.nullmask[s] >> k) & 1ull;
-                w = ke.words[size_t(k) * S + s];
-            }
-            oc.valid[k][o] = isnull ? 0 : 1;
-            store_typed(oc.data[k], ke.types[k], o, isnull ? 0 : w);
+        const uint32_t vb = emit_row(st, S, p, ke, oc, s, o);
+        for (int c = 0; c < ncol; ++c) oc.valid[c][o] = (vb >> c) & 1u;
+    }
+}
+
+// Slot counts up to kEmitSmall: one 1024-thread block finds the groups (a
+// block scan of the slot flags, slot of each output row kept in LDS), emits
+// the rows in order and writes each column's validity words straight from
+// wave ballots; out[0] = group count, out[1 + c] = column c's NULLs, read back
+// together by the host.
+constexpr uint32_t kEmitSmall = 32768;
+__global__ void __launch_bounds__(1024) emit_small_kernel(const uint64_t* st, uint32_t S, AggProg p, KeyEmit ke,
+                                                          int all_slots, OutCols oc, uint64_t* const* vwords,
+                                                          unsigned long long* out) {
+    __shared__ uint16_t slot_of[kEmitSmall];
+    __shared__ uint32_t wcnt[kMaxWaves];
+    __shared__ uint32_t base_s;
+    __shared__ unsigned long long nulls[kMaxKeys + kMaxAggs];
+    const unsigned lane = lane_id(), w = threadIdx.x / kWave;
+    const uint64_t below = (uint64_t(1) << lane) - 1;
+    const int ncol = ke.nkeys + p.n;
+    if (threadIdx.x == 0) base_s = 0;
+    if (threadIdx.x < static_cast<unsigned>(ncol)) nulls[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint32_t c0 = 0; c0 < S; c0 += 1024) {
+        const uint32_t s = c0 + threadIdx.x;
+        const bool f = s < S && (all_slots || st[s] > 0);
+        const uint64_t m = __ballot(f);
+        if (lane == 0) wcnt[w] = static_cast<uint32_t>(__popcll(m));
+        __syncthreads();
+        uint32_t off = base_s;
+        for (unsigned k = 0; k < w; ++k) off += wcnt[k];
+        if (f) slot_of[off + static_cast<uint32_t>(__popcll(m & below))] = static_cast<uint16_t>(s);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t t = 0;
+            for (unsigned k = 0; k < blockDim.x / kWave; ++k) t += wcnt[k];
+            base_s += t;
         }
-        const uint64_t rows = st[s];
-        for (int a = 0; a < p.n; ++a) {
-            const int c = ke.nkeys + a;
-            const uint64_t acc = p.acc_of[a] >= 0 ? st[(1 + size_t(p.acc_of[a])) * S + s] : 0;
-            const uint64_t cnt =
-                p.cnt_of[a] >= 0 ? st[(1 + size_t(p.nacc) + p.cnt_of[a]) * S + s] : rows;
-            const int f = p.func[a];
-            const int k = p.kind[a];
-            uint64_t out = 0;
-            bool valid = true;
-            if (f == QE_AGG_COUNT_STAR) {
-                out = rows;
-            } else if (f == QE_AGG_COUNT) {
-                out = cnt;
-            } else if (cnt == 0) {
-                valid = false;  // no non-NULL input: SUM/MIN/MAX/AVG are NULL
-            } else if (f == QE_AGG_AVG) {
-                double sum;
-                if (k == K_SUM_F) memcpy(&sum, &acc, 8);
-                else sum = static_cast<double>(static_cast<int64_t>(acc));
-                const double avg = sum / static_cast<double>(cnt);
-                memcpy(&out, &avg, 8);
-            } else if (k == K_MIN_F || k == K_MAX_F) {
-                const double d = f64_from_order_key(acc);
-                memcpy(&out, &d, 8);
-            } else {
-                out = acc;
+        __syncthreads();
+    }
+    const uint32_t G = base_s;
+    for (uint32_t g0 = 0; g0 < G; g0 += 1024) {
+        const uint32_t g = g0 + threadIdx.x;
+        const bool in = g < G;
+        const uint32_t vb = in ? emit_row(st, S, p, ke, oc, slot_of[g], g) : 0u;
+        const uint64_t inmask = __ballot(in);
+        for (int c = 0; c < ncol; ++c) {
+            const uint64_t bits = __ballot(!in || ((vb >> c) & 1u));
+            if (lane == 0 && (inmask & 1ull)) {
+                vwords[c][g >> 6] = bits;
+                const unsigned long long z = static_cast<unsigned long long>(__popcll(inmask & ~bits));
+                if (z) atomicAdd(&nulls[c], z);
             }
-            oc.valid[c][o] = valid ? 1 : 0;
-            store_typed(oc.data[c], oc.type[c], o, valid ? out : 0);
         }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = G;
+    if (threadIdx.x < static_cast<unsigned>(ncol)) out[1 + threadIdx.x] = nulls[threadIdx.x];
+}
+
+static int emit_type(const AggProg& p, const KeyEmit& ke, int c) {
+    if (c < ke.nkeys) return ke.types[c];
+    const int a = c - ke.nkeys;
+    return agg_result_type(p.func[a], p.kind[a] == K_COUNT_STAR ? QE_I64 : p.in[a].type);
+}
+
+// Small slot counts: columns allocated for every slot, one kernel, one read-back.
+static qe_dev_table* emit_groups_small(qe_gpu_ctx* ctx, const uint64_t* st, uint32_t S, const AggProg& p,
+                                       const KeyEmit& ke, bool all_slots) {
+    const int ncol = ke.nkeys + p.n;
+    std::vector<qe_dev_col*> cols;
+    try {
+        OutCols oc{};
+        std::vector<uint64_t*> vw(ncol);
+        for (int c = 0; c < ncol; ++c) {
+            const int type = emit_type(p, ke, c);
+            cols.push_back(new_col(ctx, type, S, true));
+            oc.data[c] = cols.back()->data;
+            oc.type[c] = type;
+            vw[c] = cols.back()->validity;
+        }
+        Scratch vws(ctx, size_t(ncol) * 8), res(ctx, size_t(1 + ncol) * 8);
+        QE_HIP(hipMemcpyAsync(vws.p, vw.data(), size_t(ncol) * 8, hipMemcpyHostToDevice, ctx->stream));
+        emit_small_kernel<<<1, 1024, 0, ctx->stream>>>(st, S, p, ke, all_slots ? 1 : 0, oc, vws.as<uint64_t*>(),
+                                                       res.as<unsigned long long>());
+        QE_HIP(hipGetLastError());
+        uint64_t* h = static_cast<uint64_t*>(pinned(ctx, size_t(1 + ncol) * 8));
+        QE_HIP(hipMemcpyAsync(h, res.p, size_t(1 + ncol) * 8, hipMemcpyDeviceToHost, ctx->stream));
+        QE_HIP(hipStreamSynchronize(ctx->stream));
+        const uint64_t G = h[0];
+        qe_dev_table* t = new_table(G);
+        for (int c = 0; c < ncol; ++c) {
+            qe_dev_col* col = cols[c];
+            col->len = G;
+            if (h[1 + c] == 0) {  // no NULL: drop the bitmap (all_valid)
+                dev_free(ctx, col->validity);
+                col->validity = nullptr;
+                col->all_valid = true;
+            }
+            t->cols.push_back(col);
+        }
+        return t;
+    } catch (...) {
+        for (auto* c : cols) free_col(ctx, c);
+        throw;
     }
 }
 
 qe_dev_table* emit_groups(qe_gpu_ctx* ctx, const uint64_t* st, uint64_t S, const AggProg& p,
                           const KeyEmit& ke, bool all_slots) {
+    if (S > 0 && S <= kEmitSmall) return emit_groups_small(ctx, st, static_cast<uint32_t>(S), p, ke, all_slots);
     Scratch flags(ctx, S * 4), offs(ctx, S * 8);
     group_flags_kernel<<<grid_for(S, 256, 8192), 256, 0, ctx->stream>>>(st, S, flags.as<uint32_t>(),
                                                                          all_slots ? 1 : 0);

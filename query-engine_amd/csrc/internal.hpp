@@ -268,6 +268,43 @@ __host__ __device__ inline double f64_from_order_key(uint64_t k) {
 
 __device__ inline int lane_id() { return __lane_id(); }
 
+// ---- typed column access -----------------------------------------------------------
+// Streaming kernels specialise on element width and issue every load of a row
+// group before using any of them: unconditional loads at a row index clamped to
+// the range, through global-address-space pointers (generic loads would also
+// hold up the LDS wait counter). A column without NULLs reads one all-valid
+// word instead of a bitmap (index mask 0).
+template <typename T>
+using gptr = const T __attribute__((address_space(1)))*;
+
+template <typename T>
+__device__ inline gptr<T> gcast(const void* p) {
+    return (gptr<T>)(p);
+}
+
+static __device__ const uint64_t kAllValidWord = ~0ull;
+
+struct BitsView {
+    gptr<uint64_t> w;
+    uint64_t mask;  // word index mask: ~0, or 0 for kAllValidWord
+};
+__device__ inline BitsView bits_view(const uint64_t* v) {
+    return v ? BitsView{gcast<uint64_t>(v), ~0ull} : BitsView{gcast<uint64_t>(&kAllValidWord), 0ull};
+}
+__device__ inline uint32_t bit_at(const BitsView& b, uint64_t r) {
+    return static_cast<uint32_t>((b.w[(r >> 6) & b.mask] >> (r & 63)) & 1ull);
+}
+
+// 64-bit pattern of a W-byte element: BOOL zero-extended, I32/STR sign-extended,
+// I64/TS/F64 raw (the integer reading of int_value, F64 bits unchanged).
+template <int W>
+__device__ inline uint64_t elem_bits(const void* base, uint64_t r) {
+    if constexpr (W == 8) return gcast<uint64_t>(base)[r];
+    else if constexpr (W == 4) return static_cast<uint64_t>(static_cast<int64_t>(gcast<int32_t>(base)[r]));
+    else return gcast<uint8_t>(base)[r];
+}
+
+
 // In-place exclusive scan of a[0, n) (n <= 4 * blockDim.x) by one block;
 // returns the total. `ws` holds kMaxWaves + 1 words of scratch.
 __device__ inline uint32_t block_exclusive_scan(uint32_t* a, uint32_t n, uint32_t* ws) {

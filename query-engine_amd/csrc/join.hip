@@ -665,17 +665,56 @@ __global__ void gmap_fill_kernel(ColView key, uint64_t n, int64_t kmin, const ui
 constexpr uint32_t kMapTile = 16384;     // build rows per partitioning block (1024 threads)
 constexpr uint32_t kMaxMapParts = 4096;  // partitions (LDS cursors per block)
 
+constexpr int kMapRpt = kMapTile / 1024;  // build rows per thread
+
+// Group slot of build row r for the map: from the computed array, or straight
+// from a dense integer group column (value - gmin, NULL -> null_slot), both as
+// unconditional loads.
+struct SlotArray {
+    const uint32_t* s;
+    __device__ uint32_t operator()(uint64_t r) const { return gcast<uint32_t>(s)[r]; }
+};
+template <int GW>
+struct SlotDirect {
+    const void* g;
+    const uint64_t* gvalid;
+    int64_t gmin;
+    uint32_t null_slot;
+    __device__ uint32_t operator()(uint64_t r) const {
+        const uint64_t v = elem_bits<GW>(g, r);
+        return bit_at(bits_view(gvalid), r) ? static_cast<uint32_t>(v - static_cast<uint64_t>(gmin)) : null_slot;
+    }
+};
+
+// The tile's key offsets (key - kmin; kNoSlot-marked by ok = 0 for NULL keys and
+// rows past the end), all loads issued before any is used.
+template <int KW>
+__device__ inline void map_tile_keys(const ColView& key, uint64_t t0, uint32_t m, int64_t kmin,
+                                     uint64_t (&off)[kMapRpt], uint32_t (&ok)[kMapRpt]) {
+    const BitsView kv = bits_view(key.validity);
+#pragma unroll
+    for (int k = 0; k < kMapRpt; ++k) {
+        const uint32_t i = k * 1024 + threadIdx.x;
+        const uint64_t r = t0 + (i < m ? i : m - 1);
+        off[k] = elem_bits<KW>(key.data, r) - static_cast<uint64_t>(kmin);
+        ok[k] = bit_at(kv, r) & (i < m ? 1u : 0u);
+    }
+}
+
+template <int KW>
 __global__ void __launch_bounds__(1024) map_part_hist(ColView key, uint64_t n, int64_t kmin, uint32_t pb,
                                                      uint32_t nparts, uint32_t ntiles, uint32_t* hist) {
     extern __shared__ uint32_t hcnt[];
     for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x) hcnt[i] = 0;
     __syncthreads();
-    const uint64_t t0 = uint64_t(blockIdx.x) * kMapTile, t1 = min(n, t0 + kMapTile);
-    for (uint64_t r = t0 + threadIdx.x; r < t1; r += blockDim.x) {
-        if (!is_valid(key, r)) continue;
-        const uint64_t off = static_cast<uint64_t>(int_value(key, r) - kmin);
-        atomicAdd(&hcnt[off >> pb], 1u);
-    }
+    const uint64_t t0 = uint64_t(blockIdx.x) * kMapTile;
+    const uint32_t m = static_cast<uint32_t>(min<uint64_t>(kMapTile, n - t0));
+    uint64_t off[kMapRpt];
+    uint32_t ok[kMapRpt];
+    map_tile_keys<KW>(key, t0, m, kmin, off, ok);
+#pragma unroll
+    for (int k = 0; k < kMapRpt; ++k)
+        if (ok[k]) atomicAdd(&hcnt[off[k] >> pb], 1u);
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x) hist[size_t(i) * ntiles + blockIdx.x] = hcnt[i];
 }
@@ -705,9 +744,10 @@ __global__ void __launch_bounds__(1024) map_part_scatter(ColView key, uint64_t n
 // Scatter for 16-bit maps (2^16 keys per partition): the tile is sorted by
 // partition in LDS first, then each partition's run is written contiguously.
 // Entry = (key offset within the partition) << 16 | group slot.
+template <int KW, class Slots>
 __global__ void __launch_bounds__(1024) map_part_scatter16(ColView key, uint64_t n, int64_t kmin, uint32_t nparts,
-                                                          uint32_t ntiles, const uint64_t* offs,
-                                                          const uint32_t* gslot, uint32_t* ent) {
+                                                          uint32_t ntiles, const uint64_t* offs, Slots slots,
+                                                          uint32_t* ent) {
     extern __shared__ __align__(16) char msm[];
     uint64_t* gbase = reinterpret_cast<uint64_t*>(msm);                 // [nparts]
     uint32_t* stage = reinterpret_cast<uint32_t*>(gbase + nparts);      // [kMapTile]
@@ -719,20 +759,24 @@ __global__ void __launch_bounds__(1024) map_part_scatter16(ColView key, uint64_t
         cnt[i] = 0;
     }
     __syncthreads();
-    constexpr int kRpt = kMapTile / 1024;
+    constexpr int kRpt = kMapRpt;
     const uint64_t t0 = uint64_t(blockIdx.x) * kMapTile;
     const uint32_t m = static_cast<uint32_t>(min<uint64_t>(kMapTile, n - t0));
-    uint32_t part[kRpt], pos[kRpt], ev[kRpt];
+    uint64_t off[kRpt];
+    uint32_t ok[kRpt], sl[kRpt], part[kRpt], pos[kRpt], ev[kRpt];
+    map_tile_keys<KW>(key, t0, m, kmin, off, ok);
 #pragma unroll
     for (int k = 0; k < kRpt; ++k) {
-        const uint32_t i = k * blockDim.x + threadIdx.x;
-        const uint64_t r = t0 + i;
+        const uint32_t i = k * 1024 + threadIdx.x;
+        sl[k] = slots(t0 + (i < m ? i : m - 1));
+    }
+#pragma unroll
+    for (int k = 0; k < kRpt; ++k) {
         part[k] = 0xFFFFFFFFu;
-        if (i < m && is_valid(key, r)) {
-            const uint64_t off = static_cast<uint64_t>(int_value(key, r) - kmin);
-            part[k] = static_cast<uint32_t>(off >> 16);
+        if (ok[k]) {
+            part[k] = static_cast<uint32_t>(off[k] >> 16);
             pos[k] = atomicAdd(&cnt[part[k]], 1u);
-            ev[k] = static_cast<uint32_t>((off & 0xFFFFu) << 16) | (gslot[r] & 0xFFFFu);
+            ev[k] = static_cast<uint32_t>((off[k] & 0xFFFFu) << 16) | (sl[k] & 0xFFFFu);
         }
     }
     __syncthreads();
@@ -783,12 +827,46 @@ __global__ void __launch_bounds__(1024) map_part_build(const EntT* ent, const ui
 
 enum class MapBuild { kOk, kDuplicate, kTooWide };
 
+// Where the map's group slots come from: the per-row array, or (16-bit maps)
+// a dense integer group column read by the scatter itself.
+struct MapSlots {
+    const uint32_t* gslot = nullptr;
+    const qe_dev_col* g = nullptr;
+    int64_t gmin = 0;
+    uint32_t null_slot = 0;
+};
+
+template <int KW>
+static void launch_scatter16(qe_gpu_ctx* ctx, uint32_t ntiles, size_t lds, const ColView& kv, uint64_t nb,
+                             int64_t kmin, uint32_t nparts, const uint64_t* offs, const MapSlots& ms, uint32_t* ent) {
+    if (ms.gslot) {
+        map_part_scatter16<KW, SlotArray><<<ntiles, 1024, lds, ctx->stream>>>(kv, nb, kmin, nparts, ntiles, offs,
+                                                                             SlotArray{ms.gslot}, ent);
+        return;
+    }
+    const uint64_t* gv = ms.g->all_valid ? nullptr : ms.g->validity;
+    switch (type_width(ms.g->type)) {
+        case 1:
+            map_part_scatter16<KW, SlotDirect<1>><<<ntiles, 1024, lds, ctx->stream>>>(
+                kv, nb, kmin, nparts, ntiles, offs, SlotDirect<1>{ms.g->data, gv, ms.gmin, ms.null_slot}, ent);
+            break;
+        case 4:
+            map_part_scatter16<KW, SlotDirect<4>><<<ntiles, 1024, lds, ctx->stream>>>(
+                kv, nb, kmin, nparts, ntiles, offs, SlotDirect<4>{ms.g->data, gv, ms.gmin, ms.null_slot}, ent);
+            break;
+        default:
+            map_part_scatter16<KW, SlotDirect<8>><<<ntiles, 1024, lds, ctx->stream>>>(
+                kv, nb, kmin, nparts, ntiles, offs, SlotDirect<8>{ms.g->data, gv, ms.gmin, ms.null_slot}, ent);
+    }
+}
+
 // `total` is the number of valid build keys (all within [kmin, kmin + range)).
 // The duplicate-key flag is left in `dup` (device) for the caller to read once
 // the probe is queued, so no host round trip sits between build and probe.
+// 32-bit maps take the slots from the array (ms.gslot).
 template <typename MapT>
 MapBuild build_direct_map(qe_gpu_ctx* ctx, const qe_dev_col* key, uint64_t nb, uint64_t total, int64_t kmin,
-                          uint64_t range, const uint32_t* gslot, Scratch& gmap, Scratch& dup) {
+                          uint64_t range, const MapSlots& ms, Scratch& gmap, Scratch& dup) {
     const uint32_t pb = sizeof(MapT) == 2 ? 16 : 15;  // 128 KiB of map per partition in LDS
     const uint64_t np64 = (range + (uint64_t(1) << pb) - 1) >> pb;
     if (np64 == 0 || np64 > kMaxMapParts) return MapBuild::kTooWide;
@@ -798,18 +876,34 @@ MapBuild build_direct_map(qe_gpu_ctx* ctx, const qe_dev_col* key, uint64_t nb, u
     Scratch hist(ctx, nh * 4), offs(ctx, nh * 8), sums(ctx, scan_sums_words(nh) * 8);
     dup = Scratch(ctx, 4);
     const ColView kv = view(key);
-    map_part_hist<<<ntiles, 1024, nparts * 4, ctx->stream>>>(kv, nb, kmin, pb, nparts, ntiles, hist.as<uint32_t>());
+    const int kw = static_cast<int>(type_width(key->type));
+    if (kw == 8)
+        map_part_hist<8><<<ntiles, 1024, nparts * 4, ctx->stream>>>(kv, nb, kmin, pb, nparts, ntiles,
+                                                                    hist.as<uint32_t>());
+    else if (kw == 4)
+        map_part_hist<4><<<ntiles, 1024, nparts * 4, ctx->stream>>>(kv, nb, kmin, pb, nparts, ntiles,
+                                                                    hist.as<uint32_t>());
+    else
+        map_part_hist<1><<<ntiles, 1024, nparts * 4, ctx->stream>>>(kv, nb, kmin, pb, nparts, ntiles,
+                                                                    hist.as<uint32_t>());
     QE_HIP(hipGetLastError());
     exclusive_scan_u32_dev(ctx, hist.as<uint32_t>(), offs.as<uint64_t>(), nh, sums.as<uint64_t>());
     constexpr bool k16 = sizeof(MapT) == 2;  // 4-byte entries, LDS-sorted scatter
     Scratch ent(ctx, std::max<uint64_t>(total, 1) * (k16 ? 4 : 8));
-    if (k16)
-        map_part_scatter16<<<ntiles, 1024, size_t(nparts) * 12 + size_t(kMapTile) * 6 + (kMaxWaves + 1) * 4,
-                             ctx->stream>>>(kv, nb, kmin, nparts, ntiles, offs.as<uint64_t>(), gslot,
-                                            ent.as<uint32_t>());
-    else
+    if (k16) {
+        const size_t lds = size_t(nparts) * 12 + size_t(kMapTile) * 6 + (kMaxWaves + 1) * 4;
+        if (kw == 8)
+            launch_scatter16<8>(ctx, ntiles, lds, kv, nb, kmin, nparts, offs.as<uint64_t>(), ms, ent.as<uint32_t>());
+        else if (kw == 4)
+            launch_scatter16<4>(ctx, ntiles, lds, kv, nb, kmin, nparts, offs.as<uint64_t>(), ms, ent.as<uint32_t>());
+        else
+            launch_scatter16<1>(ctx, ntiles, lds, kv, nb, kmin, nparts, offs.as<uint64_t>(), ms, ent.as<uint32_t>());
+    } else {
+        invalid(ms.gslot != nullptr, "32-bit map build needs the slot array");
         map_part_scatter<<<ntiles, 1024, nparts * 12, ctx->stream>>>(kv, nb, kmin, pb, nparts, ntiles,
-                                                                     offs.as<uint64_t>(), gslot, ent.as<uint64_t>());
+                                                                     offs.as<uint64_t>(), ms.gslot,
+                                                                     ent.as<uint64_t>());
+    }
     QE_HIP(hipGetLastError());
     gmap = Scratch(ctx, range * sizeof(MapT));
     memset_async(ctx, dup.p, 0, 4);
@@ -1028,6 +1122,15 @@ qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* pro
         DeviceHashTable gtab;
         uint64_t S = 0;
         const qe_dev_col* g0 = col_at(group_keys[0]);
+        bool gslot_ready = true;
+        // The per-row group slots of a direct group key, when a path needs the array.
+        auto ensure_gslot = [&]() {
+            if (gslot_ready) return;
+            gslot_direct_kernel<<<grid_for(nb, 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(
+                view(g0), nb, ke.kmin, static_cast<uint32_t>(ke.null_slot), gslot.as<uint32_t>());
+            QE_HIP(hipGetLastError());
+            gslot_ready = true;
+        };
         bool gdirect = false;
         const bool int_key = nkeys == 1 && (pk[0]->type == QE_I64 || pk[0]->type == QE_TS || pk[0]->type == QE_I32);
         // Group-key and build-key ranges in one pass and one read-back.
@@ -1053,9 +1156,7 @@ qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* pro
                 ke.mode = 0;
                 ke.kmin = mm.valid ? mm.mn : 0;
                 ke.null_slot = S - 1;
-                gslot_direct_kernel<<<grid_for(nb, 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(
-                    view(g0), nb, ke.kmin, static_cast<uint32_t>(S - 1), gslot.as<uint32_t>());
-                QE_HIP(hipGetLastError());
+                gslot_ready = false;  // computed only by the paths that read the array
             }
         }
         if (!gdirect) {
@@ -1147,11 +1248,19 @@ qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* pro
                 const uint64_t R = static_cast<uint64_t>(mm.mx) - static_cast<uint64_t>(mm.mn) + 1;
                 if (R <= std::max<uint64_t>(4 * nb, uint64_t(1) << 16) && R < (uint64_t(1) << 31)) {
                     Scratch gmap, dup;
-                    const MapBuild mbld =
-                        small ? build_direct_map<uint16_t>(ctx, bk[0], nb, mm.valid, mm.mn, R, gslot.as<uint32_t>(),
-                                                           gmap, dup)
-                              : build_direct_map<uint32_t>(ctx, bk[0], nb, mm.valid, mm.mn, R, gslot.as<uint32_t>(),
-                                                           gmap, dup);
+                    MapSlots ms;
+                    if (small && !gslot_ready) {  // the scatter reads the group column itself
+                        ms.g = g0;
+                        ms.gmin = ke.kmin;
+                        ms.null_slot = static_cast<uint32_t>(ke.null_slot);
+                    } else {
+                        ensure_gslot();
+                        ms.gslot = gslot.as<uint32_t>();
+                    }
+                    const MapBuild mbld = small ? build_direct_map<uint16_t>(ctx, bk[0], nb, mm.valid, mm.mn, R, ms,
+                                                                             gmap, dup)
+                                                : build_direct_map<uint32_t>(ctx, bk[0], nb, mm.valid, mm.mn, R, ms,
+                                                                             gmap, dup);
                     if (mbld == MapBuild::kOk) {
                         probe_direct(gmap, mm.mn, R);
                         // Repeated build keys make the map wrong: then the
@@ -1169,6 +1278,7 @@ qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* pro
         } else {
             ix.empty = true;
         }
+        if (!done) ensure_gslot();
         if (done) {
         } else if (ix.empty || np == 0) {
             init_state(ctx, st.as<uint64_t>(), S, p);
