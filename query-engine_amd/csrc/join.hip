@@ -686,15 +686,16 @@ struct SlotDirect {
     }
 };
 
-// The tile's key offsets (key - kmin; kNoSlot-marked by ok = 0 for NULL keys and
-// rows past the end), all loads issued before any is used.
-template <int KW>
-__device__ inline void map_tile_keys(const ColView& key, uint64_t t0, uint32_t m, int64_t kmin,
-                                     uint64_t (&off)[kMapRpt], uint32_t (&ok)[kMapRpt]) {
+// Key offsets (key - kmin) of the tile's rows k0 .. k0 + N - 1 of this thread
+// (row k * 1024 + thread), ok = 0 for NULL keys and rows past the end; all
+// loads issued before any is used.
+template <int KW, int N>
+__device__ inline void map_tile_keys(const ColView& key, uint64_t t0, uint32_t m, int64_t kmin, int k0,
+                                     uint64_t (&off)[N], uint32_t (&ok)[N]) {
     const BitsView kv = bits_view(key.validity);
 #pragma unroll
-    for (int k = 0; k < kMapRpt; ++k) {
-        const uint32_t i = k * 1024 + threadIdx.x;
+    for (int k = 0; k < N; ++k) {
+        const uint32_t i = (k0 + k) * 1024 + threadIdx.x;
         const uint64_t r = t0 + (i < m ? i : m - 1);
         off[k] = elem_bits<KW>(key.data, r) - static_cast<uint64_t>(kmin);
         ok[k] = bit_at(kv, r) & (i < m ? 1u : 0u);
@@ -711,7 +712,7 @@ __global__ void __launch_bounds__(1024) map_part_hist(ColView key, uint64_t n, i
     const uint32_t m = static_cast<uint32_t>(min<uint64_t>(kMapTile, n - t0));
     uint64_t off[kMapRpt];
     uint32_t ok[kMapRpt];
-    map_tile_keys<KW>(key, t0, m, kmin, off, ok);
+    map_tile_keys<KW, kMapRpt>(key, t0, m, kmin, 0, off, ok);
 #pragma unroll
     for (int k = 0; k < kMapRpt; ++k)
         if (ok[k]) atomicAdd(&hcnt[off[k] >> pb], 1u);
@@ -762,21 +763,29 @@ __global__ void __launch_bounds__(1024) map_part_scatter16(ColView key, uint64_t
     constexpr int kRpt = kMapRpt;
     const uint64_t t0 = uint64_t(blockIdx.x) * kMapTile;
     const uint32_t m = static_cast<uint32_t>(min<uint64_t>(kMapTile, n - t0));
-    uint64_t off[kRpt];
-    uint32_t ok[kRpt], sl[kRpt], part[kRpt], pos[kRpt], ev[kRpt];
-    map_tile_keys<KW>(key, t0, m, kmin, off, ok);
+    uint32_t part[kRpt], pos[kRpt], ev[kRpt];
+    // Two halves of eight rows: each half's loads in flight together, within
+    // the 128 registers of a 1024-thread block.
+    constexpr int kHalf = kRpt / 2;
 #pragma unroll
-    for (int k = 0; k < kRpt; ++k) {
-        const uint32_t i = k * 1024 + threadIdx.x;
-        sl[k] = slots(t0 + (i < m ? i : m - 1));
-    }
+    for (int h = 0; h < 2; ++h) {
+        uint64_t off[kHalf];
+        uint32_t ok[kHalf], sl[kHalf];
+        map_tile_keys<KW, kHalf>(key, t0, m, kmin, h * kHalf, off, ok);
 #pragma unroll
-    for (int k = 0; k < kRpt; ++k) {
-        part[k] = 0xFFFFFFFFu;
-        if (ok[k]) {
-            part[k] = static_cast<uint32_t>(off[k] >> 16);
-            pos[k] = atomicAdd(&cnt[part[k]], 1u);
-            ev[k] = static_cast<uint32_t>((off[k] & 0xFFFFu) << 16) | (sl[k] & 0xFFFFu);
+        for (int k = 0; k < kHalf; ++k) {
+            const uint32_t i = (h * kHalf + k) * 1024 + threadIdx.x;
+            sl[k] = slots(t0 + (i < m ? i : m - 1));
+        }
+#pragma unroll
+        for (int k = 0; k < kHalf; ++k) {
+            const int kk = h * kHalf + k;
+            part[kk] = 0xFFFFFFFFu;
+            if (ok[k]) {
+                part[kk] = static_cast<uint32_t>(off[k] >> 16);
+                pos[kk] = atomicAdd(&cnt[part[kk]], 1u);
+                ev[kk] = static_cast<uint32_t>((off[k] & 0xFFFFu) << 16) | (sl[k] & 0xFFFFu);
+            }
         }
     }
     __syncthreads();
