@@ -742,13 +742,13 @@ __global__ void __launch_bounds__(1024) map_part_scatter(ColView key, uint64_t n
     }
 }
 
-// Scatter for 16-bit maps (2^16 keys per partition): the tile is sorted by
-// partition in LDS first, then each partition's run is written contiguously.
+// Scatter for 16-bit maps (2^pb <= 2^16 keys per partition): the tile is sorted
+// by partition in LDS first, then each partition's run is written contiguously.
 // Entry = (key offset within the partition) << 16 | group slot.
 template <int KW, class Slots>
-__global__ void __launch_bounds__(1024) map_part_scatter16(ColView key, uint64_t n, int64_t kmin, uint32_t nparts,
-                                                          uint32_t ntiles, const uint64_t* offs, Slots slots,
-                                                          uint32_t* ent) {
+__global__ void __launch_bounds__(1024) map_part_scatter16(ColView key, uint64_t n, int64_t kmin, uint32_t pb,
+                                                          uint32_t nparts, uint32_t ntiles, const uint64_t* offs,
+                                                          Slots slots, uint32_t* ent) {
     extern __shared__ __align__(16) char msm[];
     uint64_t* gbase = reinterpret_cast<uint64_t*>(msm);                 // [nparts]
     uint32_t* stage = reinterpret_cast<uint32_t*>(gbase + nparts);      // [kMapTile]
@@ -782,9 +782,9 @@ __global__ void __launch_bounds__(1024) map_part_scatter16(ColView key, uint64_t
             const int kk = h * kHalf + k;
             part[kk] = 0xFFFFFFFFu;
             if (ok[k]) {
-                part[kk] = static_cast<uint32_t>(off[k] >> 16);
+                part[kk] = static_cast<uint32_t>(off[k] >> pb);
                 pos[kk] = atomicAdd(&cnt[part[kk]], 1u);
-                ev[kk] = static_cast<uint32_t>((off[k] & 0xFFFFu) << 16) | (sl[k] & 0xFFFFu);
+                ev[kk] = static_cast<uint32_t>((off[k] & ((uint64_t(1) << pb) - 1)) << 16) | (sl[k] & 0xFFFFu);
             }
         }
     }
@@ -847,9 +847,10 @@ struct MapSlots {
 
 template <int KW>
 static void launch_scatter16(qe_gpu_ctx* ctx, uint32_t ntiles, size_t lds, const ColView& kv, uint64_t nb,
-                             int64_t kmin, uint32_t nparts, const uint64_t* offs, const MapSlots& ms, uint32_t* ent) {
+                             int64_t kmin, uint32_t pb, uint32_t nparts, const uint64_t* offs, const MapSlots& ms,
+                             uint32_t* ent) {
     if (ms.gslot) {
-        map_part_scatter16<KW, SlotArray><<<ntiles, 1024, lds, ctx->stream>>>(kv, nb, kmin, nparts, ntiles, offs,
+        map_part_scatter16<KW, SlotArray><<<ntiles, 1024, lds, ctx->stream>>>(kv, nb, kmin, pb, nparts, ntiles, offs,
                                                                              SlotArray{ms.gslot}, ent);
         return;
     }
@@ -857,15 +858,15 @@ static void launch_scatter16(qe_gpu_ctx* ctx, uint32_t ntiles, size_t lds, const
     switch (type_width(ms.g->type)) {
         case 1:
             map_part_scatter16<KW, SlotDirect<1>><<<ntiles, 1024, lds, ctx->stream>>>(
-                kv, nb, kmin, nparts, ntiles, offs, SlotDirect<1>{ms.g->data, gv, ms.gmin, ms.null_slot}, ent);
+                kv, nb, kmin, pb, nparts, ntiles, offs, SlotDirect<1>{ms.g->data, gv, ms.gmin, ms.null_slot}, ent);
             break;
         case 4:
             map_part_scatter16<KW, SlotDirect<4>><<<ntiles, 1024, lds, ctx->stream>>>(
-                kv, nb, kmin, nparts, ntiles, offs, SlotDirect<4>{ms.g->data, gv, ms.gmin, ms.null_slot}, ent);
+                kv, nb, kmin, pb, nparts, ntiles, offs, SlotDirect<4>{ms.g->data, gv, ms.gmin, ms.null_slot}, ent);
             break;
         default:
             map_part_scatter16<KW, SlotDirect<8>><<<ntiles, 1024, lds, ctx->stream>>>(
-                kv, nb, kmin, nparts, ntiles, offs, SlotDirect<8>{ms.g->data, gv, ms.gmin, ms.null_slot}, ent);
+                kv, nb, kmin, pb, nparts, ntiles, offs, SlotDirect<8>{ms.g->data, gv, ms.gmin, ms.null_slot}, ent);
     }
 }
 
@@ -876,7 +877,9 @@ static void launch_scatter16(qe_gpu_ctx* ctx, uint32_t ntiles, size_t lds, const
 template <typename MapT>
 MapBuild build_direct_map(qe_gpu_ctx* ctx, const qe_dev_col* key, uint64_t nb, uint64_t total, int64_t kmin,
                           uint64_t range, const MapSlots& ms, Scratch& gmap, Scratch& dup) {
-    const uint32_t pb = sizeof(MapT) == 2 ? 16 : 15;  // 128 KiB of map per partition in LDS
+    // 2^15 keys per partition (64 KiB of 16-bit map: two building blocks per
+    // CU; 128 KiB of 32-bit map); 16-bit maps of very wide ranges take 2^16.
+    const uint32_t pb = (sizeof(MapT) == 2 && range > (uint64_t(kMaxMapParts) << 15)) ? 16 : 15;
     const uint64_t np64 = (range + (uint64_t(1) << pb) - 1) >> pb;
     if (np64 == 0 || np64 > kMaxMapParts) return MapBuild::kTooWide;
     const uint32_t nparts = static_cast<uint32_t>(np64);
@@ -902,11 +905,14 @@ MapBuild build_direct_map(qe_gpu_ctx* ctx, const qe_dev_col* key, uint64_t nb, u
     if (k16) {
         const size_t lds = size_t(nparts) * 12 + size_t(kMapTile) * 6 + (kMaxWaves + 1) * 4;
         if (kw == 8)
-            launch_scatter16<8>(ctx, ntiles, lds, kv, nb, kmin, nparts, offs.as<uint64_t>(), ms, ent.as<uint32_t>());
+            launch_scatter16<8>(ctx, ntiles, lds, kv, nb, kmin, pb, nparts, offs.as<uint64_t>(), ms,
+                                 ent.as<uint32_t>());
         else if (kw == 4)
-            launch_scatter16<4>(ctx, ntiles, lds, kv, nb, kmin, nparts, offs.as<uint64_t>(), ms, ent.as<uint32_t>());
+            launch_scatter16<4>(ctx, ntiles, lds, kv, nb, kmin, pb, nparts, offs.as<uint64_t>(), ms,
+                                 ent.as<uint32_t>());
         else
-            launch_scatter16<1>(ctx, ntiles, lds, kv, nb, kmin, nparts, offs.as<uint64_t>(), ms, ent.as<uint32_t>());
+            launch_scatter16<1>(ctx, ntiles, lds, kv, nb, kmin, pb, nparts, offs.as<uint64_t>(), ms,
+                                 ent.as<uint32_t>());
     } else {
         invalid(ms.gslot != nullptr, "32-bit map build needs the slot array");
         map_part_scatter<<<ntiles, 1024, nparts * 12, ctx->stream>>>(kv, nb, kmin, pb, nparts, ntiles,
