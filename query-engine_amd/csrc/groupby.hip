@@ -266,24 +266,19 @@ __global__ void emit_kernel(const uint64_t* st, uint64_t S, AggProg p, KeyEmit k
     }
 }
 
-// Slot counts up to kEmitSmall: one 1024-thread block finds the groups (a
-// block scan of the slot flags, slot of each output row kept in LDS), emits
-// the rows in order and writes each column's validity words straight from
-// wave ballots; out[0] = group count, out[1 + c] = column c's NULLs, read back
-// together by the host.
+// Slot counts up to kEmitSmall: one 1024-thread block lists the groups (a
+// block scan of the slot flags: slot_of[g] = slot of output row g, out[0] =
+// the group count), then one thread per output row emits it in order, each
+// wave writing its 64 rows' validity word per column straight from a ballot
+// and counting NULLs into out[1 + c]; the host reads the counts back together.
 constexpr uint32_t kEmitSmall = 32768;
-__global__ void __launch_bounds__(1024) emit_small_kernel(const uint64_t* st, uint32_t S, AggProg p, KeyEmit ke,
-                                                          int all_slots, OutCols oc, uint64_t* const* vwords,
-                                                          unsigned long long* out) {
-    __shared__ uint16_t slot_of[kEmitSmall];
+__global__ void __launch_bounds__(1024) group_rows_kernel(const uint64_t* st, uint32_t S, int all_slots,
+                                                          uint32_t* slot_of, unsigned long long* out) {
     __shared__ uint32_t wcnt[kMaxWaves];
     __shared__ uint32_t base_s;
-    __shared__ unsigned long long nulls[kMaxKeys + kMaxAggs];
     const unsigned lane = lane_id(), w = threadIdx.x / kWave;
     const uint64_t below = (uint64_t(1) << lane) - 1;
-    const int ncol = ke.nkeys + p.n;
     if (threadIdx.x == 0) base_s = 0;
-    if (threadIdx.x < static_cast<unsigned>(ncol)) nulls[threadIdx.x] = 0;
     __syncthreads();
     for (uint32_t c0 = 0; c0 < S; c0 += 1024) {
         const uint32_t s = c0 + threadIdx.x;
@@ -293,7 +288,7 @@ __global__ void __launch_bounds__(1024) emit_small_kernel(const uint64_t* st, ui
         __syncthreads();
         uint32_t off = base_s;
         for (unsigned k = 0; k < w; ++k) off += wcnt[k];
-        if (f) slot_of[off + static_cast<uint32_t>(__popcll(m & below))] = static_cast<uint16_t>(s);
+        if (f) slot_of[off + static_cast<uint32_t>(__popcll(m & below))] = s;
         __syncthreads();
         if (threadIdx.x == 0) {
             uint32_t t = 0;
@@ -302,24 +297,28 @@ __global__ void __launch_bounds__(1024) emit_small_kernel(const uint64_t* st, ui
         }
         __syncthreads();
     }
-    const uint32_t G = base_s;
-    for (uint32_t g0 = 0; g0 < G; g0 += 1024) {
-        const uint32_t g = g0 + threadIdx.x;
-        const bool in = g < G;
-        const uint32_t vb = in ? emit_row(st, S, p, ke, oc, slot_of[g], g) : 0u;
-        const uint64_t inmask = __ballot(in);
-        for (int c = 0; c < ncol; ++c) {
-            const uint64_t bits = __ballot(!in || ((vb >> c) & 1u));
-            if (lane == 0 && (inmask & 1ull)) {
-                vwords[c][g >> 6] = bits;
-                const unsigned long long z = static_cast<unsigned long long>(__popcll(inmask & ~bits));
-                if (z) atomicAdd(&nulls[c], z);
-            }
+    if (threadIdx.x == 0) out[0] = base_s;
+}
+
+__global__ void __launch_bounds__(256) emit_rows_kernel(const uint64_t* st, uint32_t S, AggProg p, KeyEmit ke,
+                                                        const uint32_t* slot_of, OutCols oc, uint64_t* const* vwords,
+                                                        unsigned long long* out) {
+    const uint64_t G = out[0];
+    if (uint64_t(blockIdx.x) * blockDim.x >= G) return;  // uniform across the block
+    const unsigned lane = lane_id();
+    const int ncol = ke.nkeys + p.n;
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool in = g < G;
+    const uint32_t vb = in ? emit_row(st, S, p, ke, oc, slot_of[g], g) : 0u;
+    const uint64_t inmask = __ballot(in);
+    for (int c = 0; c < ncol; ++c) {
+        const uint64_t bits = __ballot(!in || ((vb >> c) & 1u));
+        if (lane == 0 && (inmask & 1ull)) {
+            vwords[c][g >> 6] = bits;
+            const unsigned long long z = static_cast<unsigned long long>(__popcll(inmask & ~bits));
+            if (z) atomicAdd(&out[1 + c], z);
         }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) out[0] = G;
-    if (threadIdx.x < static_cast<unsigned>(ncol)) out[1 + threadIdx.x] = nulls[threadIdx.x];
 }
 
 static int emit_type(const AggProg& p, const KeyEmit& ke, int c) {
@@ -343,10 +342,13 @@ static qe_dev_table* emit_groups_small(qe_gpu_ctx* ctx, const uint64_t* st, uint
             oc.type[c] = type;
             vw[c] = cols.back()->validity;
         }
-        Scratch vws(ctx, size_t(ncol) * 8), res(ctx, size_t(1 + ncol) * 8);
+        Scratch vws(ctx, size_t(ncol) * 8), res(ctx, size_t(1 + ncol) * 8), slot_of(ctx, size_t(S) * 4);
         QE_HIP(hipMemcpyAsync(vws.p, vw.data(), size_t(ncol) * 8, hipMemcpyHostToDevice, ctx->stream));
-        emit_small_kernel<<<1, 1024, 0, ctx->stream>>>(st, S, p, ke, all_slots ? 1 : 0, oc, vws.as<uint64_t*>(),
+        memset_async(ctx, res.p, 0, size_t(1 + ncol) * 8);
+        group_rows_kernel<<<1, 1024, 0, ctx->stream>>>(st, S, all_slots ? 1 : 0, slot_of.as<uint32_t>(),
                                                        res.as<unsigned long long>());
+        emit_rows_kernel<<<(S + 255) / 256, 256, 0, ctx->stream>>>(st, S, p, ke, slot_of.as<uint32_t>(), oc,
+                                                                   vws.as<uint64_t*>(), res.as<unsigned long long>());
         QE_HIP(hipGetLastError());
         uint64_t* h = static_cast<uint64_t*>(pinned(ctx, size_t(1 + ncol) * 8));
         QE_HIP(hipMemcpyAsync(h, res.p, size_t(1 + ncol) * 8, hipMemcpyDeviceToHost, ctx->stream));
