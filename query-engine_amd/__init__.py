@@ -540,6 +540,22 @@ class Context:
             len(aggs), C.byref(out)))
         return Table(self, out)
 
+    def prepare_join_aggregate(self, probe_cols, build_cols, probe_keys, build_keys, join_type, group_keys,
+                               aggs):
+        """join_aggregate with its arguments marshalled once, for repeated calls
+        on the same columns (which must stay alive): returns a callable."""
+        fn = lib().qe_gpu_join_aggregate
+        args = (self.h, _cols(probe_cols), len(probe_cols), _cols(build_cols), len(build_cols), _u32(probe_keys),
+                _u32(build_keys), len(probe_keys), join_type, _u32(group_keys), len(group_keys), _aggs(aggs),
+                len(aggs))
+
+        def run() -> Table:
+            out = P()
+            self._check(fn(*args, C.byref(out)))
+            return Table(self, out)
+
+        return run
+
     def sort_perm(self, keys, desc=None, nulls_first=None) -> np.ndarray:
         n = keys[0].len
         out = DevBuffer(self, 4 * max(n, 1))

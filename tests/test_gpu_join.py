@@ -170,6 +170,25 @@ def test_fused_join_groupby_small_config2_shape(ctx):
     assert 0.85 * (1 << 20) < matched < 0.95 * (1 << 20)
 
 
+def test_prepared_join_groupby_repeats(ctx):
+    # The prepared call (arguments marshalled once) returns the oracle's groups
+    # on every repetition, with results freed in between.
+    (pk, pp), (bk, bg) = ctx.gen_join(8, 1 << 15, 1 << 18, 3000)
+    try:
+        pcols = [(pk.numpy()[0], None, I64), (pp.numpy()[0], None, F64)]
+        bcols = [(bk.numpy()[0], None, I64), (bg.numpy()[0], None, I32)]
+        want = O.join_aggregate(pcols, bcols, [0], [0], INNER, [3], [(SUM, 1), (CS, 0)])
+        run = ctx.prepare_join_aggregate([pk, pp], [bk, bg], [0], [0], INNER, [3], [(SUM, 1), (CS, 0)])
+        for _ in range(3):
+            t = run()
+            got = table_rows(t.numpy())
+            t.free()
+            assert_same(got, want)
+    finally:
+        for d in (pk, pp, bk, bg):
+            d.free()
+
+
 @pytest.mark.parametrize("pairs", [2, 4])
 @pytest.mark.parametrize("case", range(8))
 def test_fused_direct_stream_shapes(ctx, qe, pairs, case):
