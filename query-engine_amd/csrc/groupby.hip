@@ -300,8 +300,13 @@ __global__ void __launch_bounds__(1024) group_rows_kernel(const uint64_t* st, ui
     if (threadIdx.x == 0) out[0] = base_s;
 }
 
+// Output validity words of each column (kernel argument: no upload).
+struct OutWords {
+    uint64_t* w[kMaxKeys + kMaxAggs];
+};
+
 __global__ void __launch_bounds__(256) emit_rows_kernel(const uint64_t* st, uint32_t S, AggProg p, KeyEmit ke,
-                                                        const uint32_t* slot_of, OutCols oc, uint64_t* const* vwords,
+                                                        const uint32_t* slot_of, OutCols oc, OutWords vwords,
                                                         unsigned long long* out) {
     const uint64_t G = out[0];
     if (uint64_t(blockIdx.x) * blockDim.x >= G) return;  // uniform across the block
@@ -314,7 +319,7 @@ __global__ void __launch_bounds__(256) emit_rows_kernel(const uint64_t* st, uint
     for (int c = 0; c < ncol; ++c) {
         const uint64_t bits = __ballot(!in || ((vb >> c) & 1u));
         if (lane == 0 && (inmask & 1ull)) {
-            vwords[c][g >> 6] = bits;
+            vwords.w[c][g >> 6] = bits;
             const unsigned long long z = static_cast<unsigned long long>(__popcll(inmask & ~bits));
             if (z) atomicAdd(&out[1 + c], z);
         }
@@ -334,21 +339,20 @@ static qe_dev_table* emit_groups_small(qe_gpu_ctx* ctx, const uint64_t* st, uint
     std::vector<qe_dev_col*> cols;
     try {
         OutCols oc{};
-        std::vector<uint64_t*> vw(ncol);
+        OutWords vw{};
         for (int c = 0; c < ncol; ++c) {
             const int type = emit_type(p, ke, c);
             cols.push_back(new_col(ctx, type, S, true));
             oc.data[c] = cols.back()->data;
             oc.type[c] = type;
-            vw[c] = cols.back()->validity;
+            vw.w[c] = cols.back()->validity;
         }
-        Scratch vws(ctx, size_t(ncol) * 8), res(ctx, size_t(1 + ncol) * 8), slot_of(ctx, size_t(S) * 4);
-        QE_HIP(hipMemcpyAsync(vws.p, vw.data(), size_t(ncol) * 8, hipMemcpyHostToDevice, ctx->stream));
+        Scratch res(ctx, size_t(1 + ncol) * 8), slot_of(ctx, size_t(S) * 4);
         memset_async(ctx, res.p, 0, size_t(1 + ncol) * 8);
         group_rows_kernel<<<1, 1024, 0, ctx->stream>>>(st, S, all_slots ? 1 : 0, slot_of.as<uint32_t>(),
                                                        res.as<unsigned long long>());
-        emit_rows_kernel<<<(S + 255) / 256, 256, 0, ctx->stream>>>(st, S, p, ke, slot_of.as<uint32_t>(), oc,
-                                                                   vws.as<uint64_t*>(), res.as<unsigned long long>());
+        emit_rows_kernel<<<(S + 255) / 256, 256, 0, ctx->stream>>>(st, S, p, ke, slot_of.as<uint32_t>(), oc, vw,
+                                                                   res.as<unsigned long long>());
         QE_HIP(hipGetLastError());
         uint64_t* h = static_cast<uint64_t*>(pinned(ctx, size_t(1 + ncol) * 8));
         QE_HIP(hipMemcpyAsync(h, res.p, size_t(1 + ncol) * 8, hipMemcpyDeviceToHost, ctx->stream));
