@@ -274,9 +274,12 @@ struct KeyEmit {
 };
 
 // Emit one output row per slot with rows > 0 (or every slot when `all_slots`):
-// key columns then finalised aggregates. Returns the table.
+// key columns then finalised aggregates. Returns the table. A device flag
+// (`flag`, optional) is read back with the emission's own counts into
+// *flag_out, so a caller's check of the aggregation costs no extra round trip.
 qe_dev_table* emit_groups(qe_gpu_ctx* ctx, const uint64_t* st, uint64_t S, const AggProg& p,
-                          const KeyEmit& ke, bool all_slots);
+                          const KeyEmit& ke, bool all_slots, const unsigned int* flag = nullptr,
+                          unsigned int* flag_out = nullptr);
 
 // LDS budget per block for partial aggregation (bytes). One block per CU can
 // take up to 160 KiB on gfx950; smaller footprints allow several.

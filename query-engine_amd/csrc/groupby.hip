@@ -334,7 +334,8 @@ static int emit_type(const AggProg& p, const KeyEmit& ke, int c) {
 
 // Small slot counts: columns allocated for every slot, one kernel, one read-back.
 static qe_dev_table* emit_groups_small(qe_gpu_ctx* ctx, const uint64_t* st, uint32_t S, const AggProg& p,
-                                       const KeyEmit& ke, bool all_slots) {
+                                       const KeyEmit& ke, bool all_slots, const unsigned int* flag,
+                                       unsigned int* flag_out) {
     const int ncol = ke.nkeys + p.n;
     std::vector<qe_dev_col*> cols;
     try {
@@ -354,9 +355,11 @@ static qe_dev_table* emit_groups_small(qe_gpu_ctx* ctx, const uint64_t* st, uint
         emit_rows_kernel<<<(S + 255) / 256, 256, 0, ctx->stream>>>(st, S, p, ke, slot_of.as<uint32_t>(), oc, vw,
                                                                    res.as<unsigned long long>());
         QE_HIP(hipGetLastError());
-        uint64_t* h = static_cast<uint64_t*>(pinned(ctx, size_t(1 + ncol) * 8));
+        uint64_t* h = static_cast<uint64_t*>(pinned(ctx, size_t(2 + ncol) * 8));
         QE_HIP(hipMemcpyAsync(h, res.p, size_t(1 + ncol) * 8, hipMemcpyDeviceToHost, ctx->stream));
+        if (flag) QE_HIP(hipMemcpyAsync(h + 1 + ncol, flag, 4, hipMemcpyDeviceToHost, ctx->stream));
         QE_HIP(hipStreamSynchronize(ctx->stream));
+        if (flag_out) *flag_out = flag ? *reinterpret_cast<const unsigned int*>(h + 1 + ncol) : 0u;
         const uint64_t G = h[0];
         qe_dev_table* t = new_table(G);
         for (int c = 0; c < ncol; ++c) {
@@ -377,8 +380,10 @@ static qe_dev_table* emit_groups_small(qe_gpu_ctx* ctx, const uint64_t* st, uint
 }
 
 qe_dev_table* emit_groups(qe_gpu_ctx* ctx, const uint64_t* st, uint64_t S, const AggProg& p,
-                          const KeyEmit& ke, bool all_slots) {
-    if (S > 0 && S <= kEmitSmall) return emit_groups_small(ctx, st, static_cast<uint32_t>(S), p, ke, all_slots);
+                          const KeyEmit& ke, bool all_slots, const unsigned int* flag, unsigned int* flag_out) {
+    if (S > 0 && S <= kEmitSmall)
+        return emit_groups_small(ctx, st, static_cast<uint32_t>(S), p, ke, all_slots, flag, flag_out);
+    if (flag_out) *flag_out = flag ? read_scalar(ctx, flag) : 0u;
     Scratch flags(ctx, S * 4), offs(ctx, S * 8);
     group_flags_kernel<<<grid_for(S, 256, 8192), 256, 0, ctx->stream>>>(st, S, flags.as<uint32_t>(),
                                                                          all_slots ? 1 : 0);

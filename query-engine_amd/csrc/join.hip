@@ -1279,8 +1279,16 @@ qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* pro
                     if (mbld == MapBuild::kOk) {
                         probe_direct(gmap, mm.mn, R);
                         // Repeated build keys make the map wrong: then the
-                        // partials are discarded and the row-list path runs.
-                        done = read_scalar(ctx, dup.as<unsigned int>()) == 0;
+                        // groups are discarded and the row-list path runs. The
+                        // flag comes back with the emission's counts.
+                        unsigned int dupv = 0;
+                        qe_dev_table* t = emit_groups(ctx, st.as<uint64_t>(), S, p, ke, false,
+                                                      dup.as<unsigned int>(), &dupv);
+                        if (dupv == 0) {
+                            *out = t;
+                            return QE_GPU_OK;
+                        }
+                        qe_gpu_table_free(ctx, t);
                     }
                 }
             }
