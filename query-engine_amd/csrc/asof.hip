@@ -75,7 +75,7 @@ struct qe_dev_asof {
     uint32_t B = 0, kpb = 1;  // rows per piece, knots per router bucket
     uint64_t nknots = 0, ntab = 0;
     Scratch meta, knots, tab_mlp, tab_lin, params, grid;
-    Scratch bkt_mlp, bkt_lin;  // per router-table entry: the bucket's knots inline (BucketRec x 8)
+    Scratch bkt_mlp, bkt_lin;  // per router-table entry: the bucket's knots inline (BucketLine)
     double final_loss = 0, mean_knots_mlp = 0, mean_knots_lin = 0;
     uint64_t max_err = 0, mlp_candidates = 0, mlp_parts = 0;
     Scratch fallback;  // u64 counter of the last probe
