@@ -156,6 +156,23 @@ def test_hash_path_sparse_i64_keys(ctx, rng):
     assert len(got) == len(np.unique(keys))
 
 
+@pytest.mark.parametrize("span", [32766, 32767, 32768])
+def test_emission_at_the_one_block_limit(ctx, rng, span):
+    # Direct slots = key range + the NULL slot: 32767, 32768 (the largest slot
+    # count emitted by the one-block group listing) and 32769 (the scan path);
+    # some keys absent, NULL keys, an all-NULL value group, both emissions
+    # against the oracle.
+    n = 150000
+    k = rng.integers(0, span - 1, n)
+    k[0] = 0
+    k[2:50] = span - 1  # the top key: present only here, every value NULL
+    kn = tuple(range(100, 160))
+    v = rng.normal(size=n) * 100
+    vn = tuple(range(2, 50)) + tuple(rng.integers(200, n, 500).tolist())
+    got = check(ctx, [col(k, I32, kn), col(v, F64, vn)], [0], [(CS, 0), (CNT, 1), (SUM, 1), (MIN, 1), (AVG, 1)])
+    assert len(got) == len(np.unique(np.delete(k, kn))) + 1  # + the NULL-key group
+
+
 def test_config1_shape_1m_rows(ctx, qe):
     # 2^20 rows, key I64 uniform [0,1024), F64 [0,1000) with 5% NULLs,
     # COUNT(*)/SUM/AVG per key.

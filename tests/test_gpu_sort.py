@@ -103,3 +103,38 @@ def test_take_gathers_rows_and_rejects_bad_indices(ctx, qe):
     finally:
         buf.free()
         a.free()
+
+
+@pytest.mark.parametrize("m", [0, 1, 63, 64, 65, 4097])
+def test_take_every_width_one_null_count_read_back(ctx, qe, m):
+    # BOOL, I32, F64 and TS columns gathered together (one NULL-count read-back
+    # for all of them): values and validity per row, all-valid sources give
+    # all-valid outputs, a nullable source whose gathered rows are all valid
+    # drops its bitmap too.
+    r = np.random.default_rng(6300 + m)
+    n = 5000
+    cols = [(r.integers(0, 2, n).astype(np.uint8), O.BOOL, r.random(n) >= 0.2),
+            (r.integers(-2**31, 2**31, n).astype(np.int32), I32, None),
+            (r.normal(size=n), O.F64, r.random(n) >= 0.3),
+            (r.integers(-2**60, 2**60, n), TS, np.ones(n, dtype=bool))]
+    dev = [ctx.upload(v, t, valid) for v, t, valid in cols]
+    idx = r.integers(0, n, m).astype(np.uint32)
+    buf = qe.DevBuffer(ctx, 4 * max(m, 1))
+    try:
+        if m:
+            buf.upload(idx)
+        t = ctx.take(dev, buf, m)
+        assert t.rows == m
+        for c, (v, typ, valid) in enumerate(cols):
+            gv, gvalid = t.col(c).numpy()
+            gvalid = np.ones(m, dtype=bool) if gvalid is None else gvalid
+            want_valid = np.ones(m, dtype=bool) if valid is None else valid[idx]
+            assert (gvalid == want_valid).all(), c
+            assert (gv[want_valid] == v[idx][want_valid]).all(), c
+            if want_valid.all():
+                assert t.col(c).all_valid, c
+        t.free()
+    finally:
+        buf.free()
+        for d in dev:
+            d.free()
