@@ -38,8 +38,8 @@ def qe():
 
 @pytest.fixture(scope="session")
 def ctx(qe):
-    # On a GPU box the HIP library must load and a context must open; any
-    # failure here fails the GPU tests loudly (there is no CPU fallback).
+    # With a GPU present the HIP library must load and a context must open;
+    # any failure here fails the GPU tests loudly (there is no CPU fallback).
     c = qe.Context(0)
     yield c
     c.close()
