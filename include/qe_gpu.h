@@ -101,8 +101,15 @@ qe_gpu_status qe_gpu_timing(qe_gpu_ctx* ctx, const char* kernel, double* ms, uin
  * benchmark. QE_TUNE_JOIN_PAIRS: row pairs per thread per tile of the
  * streaming join probe, 2 or 4 (default 4). QE_TUNE_ASOF_BUCKET_KNOTS:
  * spline knots per learned-router bucket, 1..64 (default 1), read by
- * qe_gpu_asof_train: larger values shrink the radix table. */
-typedef enum qe_tune_key { QE_TUNE_JOIN_PAIRS = 0, QE_TUNE_ASOF_BUCKET_KNOTS = 1 } qe_tune_key;
+ * qe_gpu_asof_train: larger values shrink the radix table.
+ * QE_TUNE_JOIN_SPILL_CAP: records per block in the spill lists of a fused
+ * join -> group-by whose groups exceed LDS (0 = a quarter of the block's rows;
+ * rows past the cap fold with HBM atomics). */
+typedef enum qe_tune_key {
+    QE_TUNE_JOIN_PAIRS = 0,
+    QE_TUNE_ASOF_BUCKET_KNOTS = 1,
+    QE_TUNE_JOIN_SPILL_CAP = 2
+} qe_tune_key;
 qe_gpu_status qe_gpu_set_tuning(qe_gpu_ctx* ctx, int32_t key, int64_t value);
 /* Inspection: the XCD (die) id that ran each of nblocks one-wave blocks of a
  * plain launch (block b lands on XCD b % 8 on MI355X). No counterpart in the

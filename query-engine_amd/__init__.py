@@ -729,7 +729,7 @@ class BatchTuning(C.Structure):
     _fields_ = [("tile_rows", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("lds_slots", C.c_uint32)]
 
 
-TUNE_JOIN_PAIRS, TUNE_ASOF_BUCKET_KNOTS = 0, 1
+TUNE_JOIN_PAIRS, TUNE_ASOF_BUCKET_KNOTS, TUNE_JOIN_SPILL_CAP = 0, 1, 2
 CMP_LT, CMP_LE, CMP_GT, CMP_GE, CMP_EQ, CMP_NE = range(6)
 BATCH_PERSISTENT, BATCH_PER_BATCH, BATCH_GRAPH = 0, 1, 2
 

@@ -254,13 +254,60 @@ __device__ inline void global_update(uint64_t* st, uint64_t S, const AggProg& p,
     }
 }
 
+// Same fold as global_update with the values already in registers, as
+// lds_update_vals takes them (64-bit counters in the HBM state).
+__device__ inline void global_update_vals(uint64_t* st, uint64_t S, const AggProg& p, uint64_t slot,
+                                          const int32_t* vcol, const uint64_t* bits, uint32_t valid) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(&st[slot]), 1ull);
+    for (int a = 0; a < p.n; ++a) {
+        const int k = p.kind[a];
+        if (k == K_COUNT_STAR) continue;
+        const int c = vcol[a];
+        if (!((valid >> c) & 1u)) continue;
+        if (p.cnt_of[a] >= 0 && p.own_cnt[a])
+            atomicAdd(reinterpret_cast<unsigned long long*>(&st[(1 + size_t(p.nacc) + p.cnt_of[a]) * S + slot]),
+                      1ull);
+        if (p.acc_of[a] < 0 || !p.own_acc[a]) continue;
+        uint64_t* acc = st + (1 + size_t(p.acc_of[a])) * S + slot;
+        const uint64_t v = bits[c];
+        switch (k) {
+            case K_SUM_I:
+                atomicAdd(reinterpret_cast<unsigned long long*>(acc), static_cast<unsigned long long>(v));
+                break;
+            case K_SUM_F:
+                unsafeAtomicAdd(reinterpret_cast<double*>(acc), __longlong_as_double(static_cast<long long>(v)));
+                break;
+            case K_MIN_I:
+                atomicMin(reinterpret_cast<long long*>(acc), static_cast<long long>(v));
+                break;
+            case K_MAX_I:
+                atomicMax(reinterpret_cast<long long*>(acc), static_cast<long long>(v));
+                break;
+            case K_MIN_F:
+                atomicMin(reinterpret_cast<unsigned long long*>(acc),
+                          static_cast<unsigned long long>(
+                              f64_order_key(__longlong_as_double(static_cast<long long>(v)))));
+                break;
+            case K_MAX_F:
+                atomicMax(reinterpret_cast<unsigned long long*>(acc),
+                          static_cast<unsigned long long>(
+                              f64_order_key(__longlong_as_double(static_cast<long long>(v)))));
+                break;
+            default:
+                break;
+        }
+    }
+}
+
 // Host launchers (agg_common.hip).
 void init_state(qe_gpu_ctx* ctx, uint64_t* st, uint64_t S, const AggProg& p);
 // Merge nblocks partials of S slots into st in block order (deterministic). The
 // partials may cover slots [out_lo, out_lo + S) of a state laid out for out_S
-// slots (out_S = 0: st is laid out for S slots).
+// slots (out_S = 0: st is laid out for S slots). With `accumulate` the merge
+// folds into the state's values instead of replacing them.
 void merge_partials(qe_gpu_ctx* ctx, const uint64_t* part, uint32_t nblocks, uint64_t* st,
-                    uint64_t S, const AggProg& p, uint64_t out_S = 0, uint64_t out_lo = 0);
+                    uint64_t S, const AggProg& p, uint64_t out_S = 0, uint64_t out_lo = 0,
+                    bool accumulate = false);
 
 // Key source for emission.
 struct KeyEmit {

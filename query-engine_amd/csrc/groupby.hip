@@ -133,9 +133,10 @@ __device__ inline uint64_t combine(int kind, uint64_t a, uint64_t b) {
 
 // One thread per (field, slot) and chunk of `chunk` consecutive blocks: the
 // blocks of a chunk are folded in block order, then the chunk results in chunk
-// order, so the FP64 result is reproducible run to run.
+// order, so the FP64 result is reproducible run to run. With `accumulate` the
+// result folds into what `out` holds instead of replacing it.
 __global__ void merge_partials_kernel(const uint64_t* part, uint32_t nblocks, uint32_t chunk, uint64_t* out,
-                                      uint64_t S, AggProg p, uint64_t out_S, uint64_t out_lo) {
+                                      uint64_t S, AggProg p, uint64_t out_S, uint64_t out_lo, int accumulate) {
     const uint64_t W = state_words(S, p);
     const uint64_t Wout = state_words(out_S, p);
     const uint32_t nchunks = nblocks ? (nblocks + chunk - 1) / chunk : 1;  // 0 blocks: identities
@@ -145,21 +146,23 @@ __global__ void merge_partials_kernel(const uint64_t* part, uint32_t nblocks, ui
         const uint32_t c = static_cast<uint32_t>(t / W);
         const uint32_t b0 = c * chunk, b1 = min(nblocks, b0 + chunk);
         const uint64_t field = i / S;
+        const size_t o = size_t(c) * Wout + field * out_S + out_lo + (i - field * S);
         uint64_t v;
         if (field == 0 || field > uint64_t(p.nacc)) {  // rows or cnt: integer sums
-            v = 0;
+            v = accumulate ? out[o] : 0;
             for (uint32_t b = b0; b < b1; ++b) v += part[size_t(b) * W + i];
         } else {
             const int kind = p.acc_kind[field - 1];
-            v = acc_identity(kind);
+            v = accumulate ? out[o] : acc_identity(kind);
             for (uint32_t b = b0; b < b1; ++b) v = combine(kind, v, part[size_t(b) * W + i]);
         }
-        out[size_t(c) * Wout + field * out_S + out_lo + (i - field * S)] = v;
+        out[o] = v;
     }
 }
 
 void merge_partials(qe_gpu_ctx* ctx, const uint64_t* part, uint32_t nblocks, uint64_t* st,
-                    uint64_t S, const AggProg& p, uint64_t out_S, uint64_t out_lo) {
+                    uint64_t S, const AggProg& p, uint64_t out_S, uint64_t out_lo, bool accumulate) {
+    const int acc = accumulate ? 1 : 0;
     const uint64_t W = state_words(S, p);
     if (out_S == 0) out_S = S;
     constexpr uint32_t kChunk = 16;
@@ -167,12 +170,12 @@ void merge_partials(qe_gpu_ctx* ctx, const uint64_t* part, uint32_t nblocks, uin
         const uint32_t nchunks = (nblocks + kChunk - 1) / kChunk;
         Scratch mid(ctx, size_t(nchunks) * W * 8);
         merge_partials_kernel<<<grid_for(W * nchunks, 256, 16384), 256, 0, ctx->stream>>>(
-            part, nblocks, kChunk, mid.as<uint64_t>(), S, p, S, 0);
-        merge_partials_kernel<<<grid_for(W, 256, 16384), 256, 0, ctx->stream>>>(mid.as<uint64_t>(), nchunks,
-                                                                                  nchunks, st, S, p, out_S, out_lo);
+            part, nblocks, kChunk, mid.as<uint64_t>(), S, p, S, 0, 0);
+        merge_partials_kernel<<<grid_for(W, 256, 16384), 256, 0, ctx->stream>>>(
+            mid.as<uint64_t>(), nchunks, nchunks, st, S, p, out_S, out_lo, acc);
     } else {
         merge_partials_kernel<<<grid_for(W, 256, 16384), 256, 0, ctx->stream>>>(
-            part, nblocks, nblocks ? nblocks : 1, st, S, p, out_S, out_lo);
+            part, nblocks, nblocks ? nblocks : 1, st, S, p, out_S, out_lo, acc);
     }
     QE_HIP(hipGetLastError());
 }

@@ -522,15 +522,48 @@ __device__ inline void load_pair(const void* base, bool wide, uint64_t pr, uint6
 // every load instruction of a wave reads one contiguous 1 KiB (8-byte column)
 // or 512 B (4-byte column) span. All key and value loads of a tile are issued
 // before the map lookups and LDS updates. NV = value columns read.
-template <typename KeyT, typename MapT, int PAIRS, int NV>
+//
+// Rows whose slot lies outside the LDS range are skipped (another launch covers
+// their range) or, with SPILL, written to the block's spill list: record i of
+// block b is words [(b * cap + i) * (1 + NV), ...) of `rec`, a header (slot |
+// validity bits << 32) then the NV value words; `count[b]` receives the
+// block's record count. Rows beyond `cap` records fold with HBM atomics into
+// the state `st` of `S` slots. One probe pass then covers a slot space larger
+// than LDS, and the lists (a few percent of the rows when the LDS range holds
+// nearly all of them) are aggregated range by range afterwards.
+struct Spill {
+    uint64_t* st = nullptr;
+    uint64_t S = 0;
+    uint64_t* rec = nullptr;
+    uint32_t cap = 0;
+    uint32_t* count = nullptr;
+};
+
+template <int NV>
+__device__ inline void spill_row(const Spill& spill, uint32_t* spill_n, const AggProg& p, uint32_t slot,
+                                 const int32_t* vcol, const uint64_t* bits, uint32_t valid) {
+    const uint32_t pos = atomicAdd(spill_n, 1u);
+    if (pos < spill.cap) {
+        uint64_t* r = spill.rec + (size_t(blockIdx.x) * spill.cap + pos) * (1 + NV);
+        r[0] = slot | (static_cast<uint64_t>(valid) << 32);
+#pragma unroll
+        for (int c = 0; c < NV; ++c) r[1 + c] = bits[c];
+    } else {
+        global_update_vals(spill.st, spill.S, p, slot, vcol, bits, valid);
+    }
+}
+
+template <typename KeyT, typename MapT, int PAIRS, int NV, bool SPILL>
 __global__ void __launch_bounds__(1024) join_agg_stream(const KeyT* __restrict__ keys, const uint64_t* key_valid,
                                                         uint64_t n, const MapT* __restrict__ gmap, int64_t kmin,
                                                         uint64_t range, uint32_t S, uint32_t slot_lo, AggProg p,
-                                                        StreamCols sc, uint64_t* part) {
+                                                        StreamCols sc, uint64_t* part, Spill spill) {
     // S slots of LDS partials for slots [slot_lo, slot_lo + S) of the map's
-    // slot space (several launches cover a slot space larger than LDS).
+    // slot space.
     extern __shared__ __align__(16) char smem[];
+    __shared__ uint32_t spill_n;
     lds_init(smem, S, p);
+    if (SPILL && threadIdx.x == 0) spill_n = 0;
     __syncthreads();
     constexpr MapT kEmpty = static_cast<MapT>(~MapT(0));
     constexpr bool kWideKey = sizeof(KeyT) == 8;
@@ -568,7 +601,8 @@ __global__ void __launch_bounds__(1024) join_agg_stream(const KeyT* __restrict__
 #pragma unroll
             for (int e = 0; e < 2; ++e) {
                 const uint32_t sl = static_cast<uint32_t>(g[2 * i + e]) - slot_lo;
-                if (g[2 * i + e] == kEmpty || sl >= S) continue;
+                const bool in = g[2 * i + e] != kEmpty && sl < S;
+                if (!in && !(SPILL && g[2 * i + e] != kEmpty)) continue;
                 const uint64_t r = 2 * pr + e;
                 uint64_t bits[NV > 0 ? NV : 1];
                 uint32_t valid = 0;
@@ -578,7 +612,10 @@ __global__ void __launch_bounds__(1024) join_agg_stream(const KeyT* __restrict__
                     const bool v = sc.validity[c] == nullptr || ((sc.validity[c][r >> 6] >> (r & 63)) & 1ull);
                     valid |= static_cast<uint32_t>(v) << c;
                 }
-                lds_update_vals(smem, S, p, sl, sc.vcol, bits, valid);
+                if (!SPILL || in)
+                    lds_update_vals(smem, S, p, sl, sc.vcol, bits, valid);
+                else
+                    spill_row<NV>(spill, &spill_n, p, static_cast<uint32_t>(g[2 * i + e]), sc.vcol, bits, valid);
             }
         }
     }
@@ -588,7 +625,8 @@ __global__ void __launch_bounds__(1024) join_agg_stream(const KeyT* __restrict__
         bool ok = off < range;
         if (key_valid) ok = ok && ((key_valid[r >> 6] >> (r & 63)) & 1ull);
         const MapT gg = ok ? gmap[off] : kEmpty;
-        if (gg != kEmpty && static_cast<uint32_t>(gg) - slot_lo < S) {
+        const bool in = gg != kEmpty && static_cast<uint32_t>(gg) - slot_lo < S;
+        if (in || (SPILL && gg != kEmpty)) {
             uint64_t bits[NV > 0 ? NV : 1];
             uint32_t valid = 0;
 #pragma unroll
@@ -598,11 +636,155 @@ __global__ void __launch_bounds__(1024) join_agg_stream(const KeyT* __restrict__
                 const bool v = sc.validity[c] == nullptr || ((sc.validity[c][r >> 6] >> (r & 63)) & 1ull);
                 valid |= static_cast<uint32_t>(v) << c;
             }
-            lds_update_vals(smem, S, p, static_cast<uint32_t>(gg) - slot_lo, sc.vcol, bits, valid);
+            if (in)
+                lds_update_vals(smem, S, p, static_cast<uint32_t>(gg) - slot_lo, sc.vcol, bits, valid);
+            else
+                spill_row<NV>(spill, &spill_n, p, static_cast<uint32_t>(gg), sc.vcol, bits, valid);
         }
     }
     __syncthreads();
+    if (SPILL && threadIdx.x == 0) spill.count[blockIdx.x] = min(spill_n, spill.cap);
     lds_flush(smem, S, p, part);
+}
+
+// One slot range [slot_lo, slot_lo + S) of the spill lists in LDS partials;
+// block b folds list b (one list per block of the probe pass).
+template <int NV>
+__global__ void __launch_bounds__(1024) spill_agg_kernel(const uint64_t* __restrict__ rec,
+                                                         const uint32_t* __restrict__ count, uint32_t cap,
+                                                         uint32_t slot_lo, uint32_t S, AggProg p, StreamCols sc,
+                                                         uint64_t* part) {
+    extern __shared__ __align__(16) char smem[];
+    lds_init(smem, S, p);
+    __syncthreads();
+    const uint32_t n = count[blockIdx.x];
+    const uint64_t* rb = rec + size_t(blockIdx.x) * cap * (1 + NV);
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const uint64_t h = rb[size_t(i) * (1 + NV)];
+        const uint32_t sl = static_cast<uint32_t>(h) - slot_lo;
+        if (sl >= S) continue;
+        uint64_t bits[NV > 0 ? NV : 1];
+#pragma unroll
+        for (int c = 0; c < NV; ++c) bits[c] = rb[size_t(i) * (1 + NV) + 1 + c];
+        lds_update_vals(smem, S, p, sl, sc.vcol, bits, static_cast<uint32_t>(h >> 32));
+    }
+    __syncthreads();
+    lds_flush(smem, S, p, part);
+}
+
+// Group slots of every `stride`-th probe row, counted in cnt[S] (HBM atomics;
+// about 2^16 samples, so the heaviest slot's atomics stay short).
+template <typename KeyT, typename MapT>
+__global__ void slot_sample_kernel(const KeyT* __restrict__ keys, const uint64_t* key_valid, uint64_t n,
+                                   uint64_t stride, const MapT* __restrict__ gmap, int64_t kmin, uint64_t range,
+                                   uint32_t* cnt) {
+    constexpr MapT kEmpty = static_cast<MapT>(~MapT(0));
+    const uint64_t ns = (n + stride - 1) / stride;
+    for (uint64_t j = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; j < ns;
+         j += uint64_t(gridDim.x) * blockDim.x) {
+        const uint64_t r = j * stride;
+        const uint64_t off = static_cast<uint64_t>(static_cast<int64_t>(keys[r])) - static_cast<uint64_t>(kmin);
+        bool ok = off < range;
+        if (key_valid) ok = ok && ((key_valid[r >> 6] >> (r & 63)) & 1ull);
+        const MapT g = ok ? gmap[off] : kEmpty;
+        if (g != kEmpty) atomicAdd(&cnt[g], 1u);
+    }
+}
+
+// Hot set of at most W slots by sampled count: the slots with cnt >= c for the
+// smallest c >= 1 that leaves at most W of them. perm[s] relabels the slots, the
+// hot ones first (ids 0..h-1), each part in slot order. out = {h, samples in
+// the hot set, all samples, largest count outside it}. One block of 1024
+// threads.
+__global__ void __launch_bounds__(1024) hot_set_kernel(const uint32_t* cnt, uint32_t S, uint32_t W, uint32_t* perm,
+                                                       uint32_t* out) {
+    __shared__ uint32_t csum[1024];
+    __shared__ uint32_t ws[kMaxWaves + 1];
+    __shared__ uint32_t n_at, top, inside, cold_max;
+    const uint32_t t = threadIdx.x;
+    const uint32_t C = (S + blockDim.x - 1) / blockDim.x;
+    const uint32_t b = min(S, t * C), e = min(S, b + C);
+    uint32_t m = 0, sum = 0;
+    for (uint32_t i = b; i < e; ++i) {
+        m = max(m, cnt[i]);
+        sum += cnt[i];
+    }
+    if (t == 0) top = 0;
+    __syncthreads();
+    atomicMax(&top, m);
+    csum[t] = sum;
+    __syncthreads();
+    const uint32_t total = block_exclusive_scan(csum, blockDim.x, ws);
+    // Smallest c in [1, top + 1] with #{cnt >= c} <= W (c = top + 1 leaves none).
+    uint32_t lo = 1, hi = top + 1;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (t == 0) n_at = 0;
+        __syncthreads();
+        uint32_t k = 0;
+        for (uint32_t i = b; i < e; ++i) k += cnt[i] >= mid ? 1u : 0u;
+        atomicAdd(&n_at, k);
+        __syncthreads();
+        if (n_at <= W)
+            hi = mid;
+        else
+            lo = mid + 1;
+        __syncthreads();
+    }
+    const uint32_t c = lo;
+    uint32_t k = 0, in = 0, cm = 0;
+    for (uint32_t i = b; i < e; ++i) {
+        const bool hot = cnt[i] >= c;
+        k += hot ? 1u : 0u;
+        in += hot ? cnt[i] : 0u;
+        cm = hot ? cm : max(cm, cnt[i]);
+    }
+    if (t == 0) {
+        inside = 0;
+        cold_max = 0;
+    }
+    csum[t] = k;
+    __syncthreads();
+    atomicAdd(&inside, in);
+    atomicMax(&cold_max, cm);
+    const uint32_t h = block_exclusive_scan(csum, blockDim.x, ws);
+    uint32_t hp = csum[t];  // hot slots before slot b
+    for (uint32_t i = b; i < e; ++i) {
+        if (cnt[i] >= c) {
+            perm[i] = hp++;
+        } else {
+            perm[i] = h + (i - hp);
+        }
+    }
+    __syncthreads();
+    if (t == 0) {
+        out[0] = h;
+        out[1] = inside;
+        out[2] = total;
+        out[3] = cold_max;
+    }
+}
+
+// gout[i] = perm[gin[i]]; the empty entry stays empty.
+template <typename MapT>
+__global__ void remap_slots_kernel(const MapT* __restrict__ gin, uint64_t n, const uint32_t* __restrict__ perm,
+                                   MapT* __restrict__ gout) {
+    constexpr MapT kEmpty = static_cast<MapT>(~MapT(0));
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n;
+         i += uint64_t(gridDim.x) * blockDim.x) {
+        const MapT g = gin[i];
+        gout[i] = g == kEmpty ? kEmpty : static_cast<MapT>(perm[g]);
+    }
+}
+
+// st[f * S + s] = src[f * S + perm[s]] for each of the state's `fields` fields.
+__global__ void unpermute_state_kernel(const uint64_t* __restrict__ src, uint64_t S, uint32_t fields,
+                                       const uint32_t* __restrict__ perm, uint64_t* __restrict__ st) {
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < S * fields;
+         i += uint64_t(gridDim.x) * blockDim.x) {
+        const uint64_t f = i / S, sl = i - f * S;
+        st[i] = src[f * S + perm[sl]];
+    }
 }
 
 __global__ void __launch_bounds__(1024) join_agg_lists_lds(ProbeAggLists L, uint64_t n, uint32_t S, AggProg p,
@@ -962,36 +1144,102 @@ bool stream_cols(const AggProg& p, const void* keys, size_t key_width, StreamCol
     return true;
 }
 
-template <typename KeyT, typename MapT, int PAIRS>
-void launch_stream_p(qe_gpu_ctx* ctx, int nv, unsigned blocks, size_t lb, const KeyT* keys, const uint64_t* kv,
-                     uint64_t n, const MapT* gm, int64_t kmin, uint64_t range, uint32_t S, uint32_t slot_lo,
-                     const AggProg& p, const StreamCols& sc, uint64_t* part) {
+// Sampled hot set for a slot space larger than LDS: at most W slots relabelled
+// first by perm (caller's buffer of S entries), the sampled rows in the set and
+// in all, and the heaviest slot outside it.
+struct HotSet {
+    uint32_t h = 0, inside = 0, total = 0, cold_max = 0;
+    uint64_t stride = 1;  // probe rows per sample
+};
+
+template <typename KeyT, typename MapT>
+HotSet hot_set(qe_gpu_ctx* ctx, const KeyT* keys, const uint64_t* kv, uint64_t n, const MapT* gm, int64_t kmin,
+               uint64_t range, uint32_t S, uint32_t W, uint32_t* perm) {
+    TimedRegion tr(ctx, "join_agg_hot_set");
+    Scratch buf(ctx, (size_t(S) + 4) * 4);  // cnt[S] | out[4]
+    uint32_t* cnt = buf.as<uint32_t>();
+    memset_async(ctx, cnt, 0, size_t(S) * 4);
+    const uint64_t stride = std::max<uint64_t>(1, n >> 16);
+    const uint64_t ns = (n + stride - 1) / stride;
+    slot_sample_kernel<KeyT, MapT><<<grid_for(ns, 256, 1024), 256, 0, ctx->stream>>>(keys, kv, n, stride, gm, kmin,
+                                                                                     range, cnt);
+    QE_HIP(hipGetLastError());
+    hot_set_kernel<<<1, 1024, 0, ctx->stream>>>(cnt, S, W, perm, cnt + S);
+    QE_HIP(hipGetLastError());
+    uint32_t* h = static_cast<uint32_t*>(pinned(ctx, 16));
+    QE_HIP(hipMemcpyAsync(h, cnt + S, 16, hipMemcpyDeviceToHost, ctx->stream));
+    QE_HIP(hipStreamSynchronize(ctx->stream));
+    HotSet hs;
+    hs.h = h[0];
+    hs.inside = h[1];
+    hs.total = h[2];
+    hs.cold_max = h[3];
+    hs.stride = stride;
+    return hs;
+}
+
+void launch_spill_agg(qe_gpu_ctx* ctx, int nv, unsigned blocks, size_t lb, const uint64_t* rec,
+                      const uint32_t* count, uint32_t cap, uint32_t slot_lo, uint32_t S, const AggProg& p,
+                      const StreamCols& sc, uint64_t* part) {
     switch (nv) {
         case 0:
-            join_agg_stream<KeyT, MapT, PAIRS, 0><<<blocks, 1024, lb, ctx->stream>>>(keys, kv, n, gm, kmin, range, S,
-                                                                                     slot_lo, p, sc, part);
+            spill_agg_kernel<0><<<blocks, 1024, lb, ctx->stream>>>(rec, count, cap, slot_lo, S, p, sc, part);
             break;
         case 1:
-            join_agg_stream<KeyT, MapT, PAIRS, 1><<<blocks, 1024, lb, ctx->stream>>>(keys, kv, n, gm, kmin, range, S,
-                                                                                     slot_lo, p, sc, part);
+            spill_agg_kernel<1><<<blocks, 1024, lb, ctx->stream>>>(rec, count, cap, slot_lo, S, p, sc, part);
             break;
         default:
-            join_agg_stream<KeyT, MapT, PAIRS, 2><<<blocks, 1024, lb, ctx->stream>>>(keys, kv, n, gm, kmin, range, S,
-                                                                                     slot_lo, p, sc, part);
+            spill_agg_kernel<2><<<blocks, 1024, lb, ctx->stream>>>(rec, count, cap, slot_lo, S, p, sc, part);
             break;
     }
     QE_HIP(hipGetLastError());
+}
+
+template <typename KeyT, typename MapT, int PAIRS, int NV>
+void launch_stream_nv(qe_gpu_ctx* ctx, unsigned blocks, size_t lb, const KeyT* keys, const uint64_t* kv, uint64_t n,
+                      const MapT* gm, int64_t kmin, uint64_t range, uint32_t S, uint32_t slot_lo, const AggProg& p,
+                      const StreamCols& sc, uint64_t* part, const Spill& spill) {
+    if (spill.st)
+        join_agg_stream<KeyT, MapT, PAIRS, NV, true><<<blocks, 1024, lb, ctx->stream>>>(keys, kv, n, gm, kmin, range, S,
+                                                                                        slot_lo, p, sc, part, spill);
+    else
+        join_agg_stream<KeyT, MapT, PAIRS, NV, false><<<blocks, 1024, lb, ctx->stream>>>(
+            keys, kv, n, gm, kmin, range, S, slot_lo, p, sc, part, spill);
+    QE_HIP(hipGetLastError());
+}
+
+template <typename KeyT, typename MapT, int PAIRS>
+void launch_stream_p(qe_gpu_ctx* ctx, int nv, unsigned blocks, size_t lb, const KeyT* keys, const uint64_t* kv,
+                     uint64_t n, const MapT* gm, int64_t kmin, uint64_t range, uint32_t S, uint32_t slot_lo,
+                     const AggProg& p, const StreamCols& sc, uint64_t* part, const Spill& spill) {
+    switch (nv) {
+        case 0:
+            launch_stream_nv<KeyT, MapT, PAIRS, 0>(ctx, blocks, lb, keys, kv, n, gm, kmin, range, S, slot_lo, p, sc,
+                                                   part, spill);
+            break;
+        case 1:
+            launch_stream_nv<KeyT, MapT, PAIRS, 1>(ctx, blocks, lb, keys, kv, n, gm, kmin, range, S, slot_lo, p, sc,
+                                                   part, spill);
+            break;
+        default:
+            launch_stream_nv<KeyT, MapT, PAIRS, 2>(ctx, blocks, lb, keys, kv, n, gm, kmin, range, S, slot_lo, p, sc,
+                                                   part, spill);
+            break;
+    }
 }
 
 // Row pairs per thread per tile: 2 or 4 (tuning knob QE_TUNE_JOIN_PAIRS).
 template <typename KeyT, typename MapT>
 void launch_stream(qe_gpu_ctx* ctx, int pairs, int nv, unsigned blocks, size_t lb, const KeyT* keys,
                    const uint64_t* kv, uint64_t n, const MapT* gm, int64_t kmin, uint64_t range, uint32_t S,
-                   uint32_t slot_lo, const AggProg& p, const StreamCols& sc, uint64_t* part) {
+                   uint32_t slot_lo, const AggProg& p, const StreamCols& sc, uint64_t* part,
+                   const Spill& spill = Spill{}) {
     if (pairs <= 2)
-        launch_stream_p<KeyT, MapT, 2>(ctx, nv, blocks, lb, keys, kv, n, gm, kmin, range, S, slot_lo, p, sc, part);
+        launch_stream_p<KeyT, MapT, 2>(ctx, nv, blocks, lb, keys, kv, n, gm, kmin, range, S, slot_lo, p, sc, part,
+                                       spill);
     else
-        launch_stream_p<KeyT, MapT, 4>(ctx, nv, blocks, lb, keys, kv, n, gm, kmin, range, S, slot_lo, p, sc, part);
+        launch_stream_p<KeyT, MapT, 4>(ctx, nv, blocks, lb, keys, kv, n, gm, kmin, range, S, slot_lo, p, sc, part,
+                                       spill);
 }
 
 }  // namespace qe_gpu
@@ -1046,6 +1294,8 @@ namespace qe_gpu {
 
 // Most LDS slot-range passes over the probe before HBM atomics are used.
 constexpr uint64_t kMaxSlotPasses = 32;
+// Fewest probe rows for which a sampled hot window may replace the passes.
+constexpr uint64_t kSpillMinRows = uint64_t(1) << 22;
 
 // The fused join -> group-by over prepared columns (no STR rewriting).
 qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* probe_cols, uint32_t nprobe,
@@ -1214,6 +1464,70 @@ qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* pro
                 StreamCols sc{};
                 int nv = 0;
                 const bool streamable = (use_lds || multipass) && stream_cols(p, keys, sizeof(KeyT), sc, nv);
+                // More slots than LDS holds: when a sample of the probe puts at
+                // least 7/8 of the rows in at most an LDS range of slots, and no
+                // slot outside them carries more than 1/4096 of the rows or
+                // about 2^16 rows, those slots are relabelled first (the map is
+                // rewritten through the relabelling) and one pass keeps them in
+                // LDS while the other rows go to per-block spill lists; the lists
+                // are then folded one LDS range at a time, and the relabelled
+                // state returns to slot order.
+                bool spill = false;
+                if (multipass && streamable && np >= kSpillMinRows) {
+                    const uint32_t S32 = static_cast<uint32_t>(S), W = static_cast<uint32_t>(pass_slots);
+                    Scratch perm(ctx, size_t(S) * 4);
+                    const HotSet hs = hot_set<KeyT, MapT>(ctx, keys, kv, np, gm, kmin, range, S32, W,
+                                                          perm.as<uint32_t>());
+                    spill = hs.total >= 1024 && uint64_t(hs.total - hs.inside) * 8 <= hs.total &&
+                            (uint64_t(hs.cold_max) * 4096 <= hs.total ||
+                             uint64_t(hs.cold_max) * hs.stride <= (uint64_t(1) << 16));
+                    if (spill) {
+                        Scratch gm2(ctx, size_t(range) * sizeof(MapT));
+                        remap_slots_kernel<MapT><<<grid_for(range, 256, 4096), 256, 0, ctx->stream>>>(
+                            gm, range, perm.as<uint32_t>(), gm2.as<MapT>());
+                        QE_HIP(hipGetLastError());
+                        Scratch st2(ctx, state_words(S, p) * 8);
+                        Scratch part(ctx, size_t(lds_blocks) * part_words(W, p) * 8);
+                        // Spill lists sized for a quarter of each block's rows
+                        // (the sample put at most an eighth outside the LDS range).
+                        const uint64_t per_block = (np + lds_blocks - 1) / lds_blocks;
+                        const uint32_t cap = ctx->tune_join_spill_cap
+                                                 ? ctx->tune_join_spill_cap
+                                                 : static_cast<uint32_t>(std::min<uint64_t>(per_block / 4 + 1024, 1u << 30));
+                        Scratch rec(ctx, size_t(lds_blocks) * cap * (1 + nv) * 8);
+                        Scratch rcount(ctx, size_t(lds_blocks) * 4);
+                        init_state(ctx, st2.as<uint64_t>(), S, p);
+                        Spill sp;
+                        sp.st = st2.as<uint64_t>();
+                        sp.S = S;
+                        sp.rec = rec.as<uint64_t>();
+                        sp.cap = cap;
+                        sp.count = rcount.as<uint32_t>();
+                        {
+                            TimedRegion tr(ctx, "join_agg_probe");
+                            launch_stream<KeyT, MapT>(ctx, ctx->tune_join_pairs, nv, lds_blocks, lds_bytes(W, p), keys,
+                                                      kv, np, gm2.as<MapT>(), kmin, range, W, 0, p, sc,
+                                                      part.as<uint64_t>(), sp);
+                        }
+                        merge_partials(ctx, part.as<uint64_t>(), lds_blocks, st2.as<uint64_t>(), W, p, S, 0, true);
+                        for (uint64_t lo = W; lo < S; lo += W) {
+                            const uint32_t Sp = static_cast<uint32_t>(std::min<uint64_t>(W, S - lo));
+                            {
+                                TimedRegion tr(ctx, "join_agg_spill");
+                                launch_spill_agg(ctx, nv, lds_blocks, lds_bytes(Sp, p), rec.as<uint64_t>(),
+                                                 rcount.as<uint32_t>(), cap, static_cast<uint32_t>(lo), Sp, p, sc,
+                                                 part.as<uint64_t>());
+                            }
+                            merge_partials(ctx, part.as<uint64_t>(), lds_blocks, st2.as<uint64_t>(), Sp, p, S, lo,
+                                           true);
+                        }
+                        const uint32_t fields = static_cast<uint32_t>(1 + p.nacc + p.ncnt);
+                        unpermute_state_kernel<<<grid_for(S * fields, 256, 4096), 256, 0, ctx->stream>>>(
+                            st2.as<uint64_t>(), S, fields, perm.as<uint32_t>(), st.as<uint64_t>());
+                        QE_HIP(hipGetLastError());
+                    }
+                }
+                if (spill) return;
                 if (use_lds || multipass) {
                     const uint64_t per = use_lds ? S : pass_slots;
                     Scratch part(ctx, size_t(lds_blocks) * part_words(static_cast<uint32_t>(per), p) * 8);

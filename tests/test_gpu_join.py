@@ -283,3 +283,57 @@ def test_fused_many_groups_slot_range_passes(ctx, case):
              col(r.integers(-1000, 1000, n_p), I32), col(r.integers(-2**40, 2**40, n_p), I64)]
     aggs = [(CS, 0), (AVG, 1)] if case < 2 else [(CS, 0), (SUM, 1), (MIN, 2), (MAX, 3), (CNT, 1)]
     join_agg_case(ctx, pcols, bcols, [0], [0], INNER, [5], aggs)
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_fused_many_groups_hot_set_one_pass(ctx, qe, case):
+    # More groups than LDS holds, but the probe puts nearly all rows on a few
+    # thousand groups: a sample of 2^16 probe rows finds them, they are
+    # relabelled to the first slots, and one probe pass keeps them in LDS while
+    # the other rows fold with HBM atomics; the state returns to slot order.
+    # Hot groups form one range of group values (case 0) or are scattered
+    # (1, 2); case 3 spreads the rows over all groups and keeps one pass per
+    # slot range. Case 2 caps the spill lists at 64 records per block, so most
+    # spilled rows take the HBM-atomic fold. I32 and I64 keys; odd probe
+    # lengths (tail row); NULL groups; the streamed probe (one or two value
+    # columns).
+    r = np.random.default_rng(1100 + case)
+    kt = I32 if case % 2 else I64
+    n_b, groups = 60000, 60000
+    n_p = (1 << 22) + [3, 1, 0, 5][case]
+    keys = r.permutation(n_b)
+    grp = r.integers(0, groups, n_b)
+    if case == 0:
+        hot = (grp >= 20000) & (grp < 24000)
+    else:
+        hot = np.isin(grp, r.permutation(groups)[:4000])
+    hot_rows = np.nonzero(hot)[0]
+    u = r.random(n_p)
+    if case < 3:
+        probe_keys = np.where(u < 0.90, keys[hot_rows[r.integers(0, len(hot_rows), n_p)]],
+                              np.where(u < 0.95, keys[r.integers(0, n_b, n_p)], n_b + 5))
+    else:
+        probe_keys = np.where(u < 0.95, keys[r.integers(0, n_b, n_p)], n_b + 5)
+    bcols = [col(keys, kt), col(grp, I32, tuple(np.nonzero(r.random(n_b) < 0.01)[0]))]
+    pcols = [col(probe_keys, kt), col(np.round(r.uniform(0, 200, n_p), 2), F64,
+                                      tuple(np.nonzero(r.random(n_p) < 0.02)[0])),
+             col(r.integers(-1000, 1000, n_p), I32), col(r.integers(-2**40, 2**40, n_p), I64)]
+    aggs = [(CS, 0), (AVG, 1)] if case in (0, 3) else [(CS, 0), (SUM, 1), (MIN, 1), (MAX, 3), (SUM, 3), (CNT, 1)]
+    ctx.set_timing(True)
+    if case == 2:
+        ctx.set_tuning(qe.TUNE_JOIN_SPILL_CAP, 64)
+    try:
+        for name in ("join_agg_hot_set", "join_agg_probe", "join_agg_spill"):
+            ctx.timing(name)
+        join_agg_case(ctx, pcols, bcols, [0], [0], INNER, [5], aggs)
+        _, sampled = ctx.timing("join_agg_hot_set")
+        _, probes = ctx.timing("join_agg_probe")
+        _, spill_passes = ctx.timing("join_agg_spill")
+    finally:
+        ctx.set_timing(False)
+        ctx.set_tuning(qe.TUNE_JOIN_SPILL_CAP, 0)
+    assert sampled == 1
+    if case == 3:
+        assert probes > 1 and spill_passes == 0
+    else:
+        assert probes == 1 and spill_passes >= 1
