@@ -695,34 +695,47 @@ __global__ void slot_sample_kernel(const KeyT* __restrict__ keys, const uint64_t
 // smallest c >= 1 that leaves at most W of them. perm[s] relabels the slots, the
 // hot ones first (ids 0..h-1), each part in slot order. out = {h, samples in
 // the hot set, all samples, largest count outside it}. One block of 1024
-// threads.
-__global__ void __launch_bounds__(1024) hot_set_kernel(const uint32_t* cnt, uint32_t S, uint32_t W, uint32_t* perm,
-                                                       uint32_t* out) {
+// threads; with `staged`, the threshold search reads the counts (capped at
+// 65535, above any threshold it can pick) from S u16 of dynamic LDS.
+constexpr uint32_t kHotStagedSlots = 75000;
+
+__global__ void __launch_bounds__(1024) hot_set_kernel(const uint32_t* cnt, uint32_t S, uint32_t W, int staged,
+                                                       uint32_t* perm, uint32_t* out) {
+    extern __shared__ uint16_t lc[];
     __shared__ uint32_t csum[1024];
     __shared__ uint32_t ws[kMaxWaves + 1];
-    __shared__ uint32_t n_at, top, inside, cold_max;
+    __shared__ uint32_t n_at, top, total, cold_sum, cold_max;
     const uint32_t t = threadIdx.x;
+    if (t == 0) {
+        top = 0;
+        total = 0;
+        cold_sum = 0;
+        cold_max = 0;
+    }
+    __syncthreads();
+    uint32_t m = 0, sum = 0;
+    for (uint32_t i = t; i < S; i += blockDim.x) {  // coalesced
+        const uint32_t v = cnt[i];
+        m = max(m, v);
+        sum += v;
+        if (staged) lc[i] = static_cast<uint16_t>(min(v, 65535u));
+    }
+    atomicMax(&top, m);
+    atomicAdd(&total, sum);
+    __syncthreads();
+    // Counts capped at 65535 keep every comparison against c <= 65535 exact.
+    auto val = [&](uint32_t i) -> uint32_t { return staged ? lc[i] : min(cnt[i], 65535u); };
     const uint32_t C = (S + blockDim.x - 1) / blockDim.x;
     const uint32_t b = min(S, t * C), e = min(S, b + C);
-    uint32_t m = 0, sum = 0;
-    for (uint32_t i = b; i < e; ++i) {
-        m = max(m, cnt[i]);
-        sum += cnt[i];
-    }
-    if (t == 0) top = 0;
-    __syncthreads();
-    atomicMax(&top, m);
-    csum[t] = sum;
-    __syncthreads();
-    const uint32_t total = block_exclusive_scan(csum, blockDim.x, ws);
     // Smallest c in [1, top + 1] with #{cnt >= c} <= W (c = top + 1 leaves none).
-    uint32_t lo = 1, hi = top + 1;
+    // #{cnt >= c} <= total / c, so c <= 65535 whenever W >= 2^17 / 65535.
+    uint32_t lo = 1, hi = min(top, 65535u) + 1;
     while (lo < hi) {
         const uint32_t mid = lo + (hi - lo) / 2;
         if (t == 0) n_at = 0;
         __syncthreads();
         uint32_t k = 0;
-        for (uint32_t i = b; i < e; ++i) k += cnt[i] >= mid ? 1u : 0u;
+        for (uint32_t i = b; i < e; ++i) k += val(i) >= mid ? 1u : 0u;
         atomicAdd(&n_at, k);
         __syncthreads();
         if (n_at <= W)
@@ -732,25 +745,24 @@ __global__ void __launch_bounds__(1024) hot_set_kernel(const uint32_t* cnt, uint
         __syncthreads();
     }
     const uint32_t c = lo;
-    uint32_t k = 0, in = 0, cm = 0;
+    uint32_t k = 0, cs = 0, cm = 0;
     for (uint32_t i = b; i < e; ++i) {
-        const bool hot = cnt[i] >= c;
-        k += hot ? 1u : 0u;
-        in += hot ? cnt[i] : 0u;
-        cm = hot ? cm : max(cm, cnt[i]);
-    }
-    if (t == 0) {
-        inside = 0;
-        cold_max = 0;
+        const uint32_t v = val(i);
+        if (v >= c) {
+            ++k;
+        } else {  // below c, so exact
+            cs += v;
+            cm = max(cm, v);
+        }
     }
     csum[t] = k;
-    __syncthreads();
-    atomicAdd(&inside, in);
+    atomicAdd(&cold_sum, cs);
     atomicMax(&cold_max, cm);
+    __syncthreads();  // the scan reads other threads' csum entries
     const uint32_t h = block_exclusive_scan(csum, blockDim.x, ws);
     uint32_t hp = csum[t];  // hot slots before slot b
     for (uint32_t i = b; i < e; ++i) {
-        if (cnt[i] >= c) {
+        if (val(i) >= c) {
             perm[i] = hp++;
         } else {
             perm[i] = h + (i - hp);
@@ -759,7 +771,7 @@ __global__ void __launch_bounds__(1024) hot_set_kernel(const uint32_t* cnt, uint
     __syncthreads();
     if (t == 0) {
         out[0] = h;
-        out[1] = inside;
+        out[1] = total - cold_sum;
         out[2] = total;
         out[3] = cold_max;
     }
@@ -1164,7 +1176,8 @@ HotSet hot_set(qe_gpu_ctx* ctx, const KeyT* keys, const uint64_t* kv, uint64_t n
     slot_sample_kernel<KeyT, MapT><<<grid_for(ns, 256, 1024), 256, 0, ctx->stream>>>(keys, kv, n, stride, gm, kmin,
                                                                                      range, cnt);
     QE_HIP(hipGetLastError());
-    hot_set_kernel<<<1, 1024, 0, ctx->stream>>>(cnt, S, W, perm, cnt + S);
+    const int staged = S <= kHotStagedSlots ? 1 : 0;
+    hot_set_kernel<<<1, 1024, staged ? size_t(S) * 2 : 0, ctx->stream>>>(cnt, S, W, staged, perm, cnt + S);
     QE_HIP(hipGetLastError());
     uint32_t* h = static_cast<uint32_t*>(pinned(ctx, 16));
     QE_HIP(hipMemcpyAsync(h, cnt + S, 16, hipMemcpyDeviceToHost, ctx->stream));
