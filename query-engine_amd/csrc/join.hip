@@ -1307,8 +1307,10 @@ namespace qe_gpu {
 
 // Most LDS slot-range passes over the probe before HBM atomics are used.
 constexpr uint64_t kMaxSlotPasses = 32;
-// Fewest probe rows for which a sampled hot window may replace the passes.
+// Fewest probe rows for which a sampled hot set may replace the passes, and the
+// largest spill-list allocation it may take.
 constexpr uint64_t kSpillMinRows = uint64_t(1) << 22;
+constexpr size_t kSpillMaxBytes = size_t(8) << 30;
 
 // The fused join -> group-by over prepared columns (no STR rewriting).
 qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* probe_cols, uint32_t nprobe,
@@ -1491,9 +1493,17 @@ qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* pro
                     Scratch perm(ctx, size_t(S) * 4);
                     const HotSet hs = hot_set<KeyT, MapT>(ctx, keys, kv, np, gm, kmin, range, S32, W,
                                                           perm.as<uint32_t>());
+                    // Spill lists sized for a quarter of each block's rows (the
+                    // sample put at most an eighth outside the LDS range).
+                    const uint64_t per_block = (np + lds_blocks - 1) / lds_blocks;
+                    const uint32_t cap = ctx->tune_join_spill_cap
+                                             ? ctx->tune_join_spill_cap
+                                             : static_cast<uint32_t>(std::min<uint64_t>(per_block / 4 + 1024, 1u << 30));
+                    const size_t rec_bytes = size_t(lds_blocks) * cap * (1 + nv) * 8;
                     spill = hs.total >= 1024 && uint64_t(hs.total - hs.inside) * 8 <= hs.total &&
                             (uint64_t(hs.cold_max) * 4096 <= hs.total ||
-                             uint64_t(hs.cold_max) * hs.stride <= (uint64_t(1) << 16));
+                             uint64_t(hs.cold_max) * hs.stride <= (uint64_t(1) << 16)) &&
+                            rec_bytes <= kSpillMaxBytes;
                     if (spill) {
                         Scratch gm2(ctx, size_t(range) * sizeof(MapT));
                         remap_slots_kernel<MapT><<<grid_for(range, 256, 4096), 256, 0, ctx->stream>>>(
@@ -1501,13 +1511,7 @@ qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* pro
                         QE_HIP(hipGetLastError());
                         Scratch st2(ctx, state_words(S, p) * 8);
                         Scratch part(ctx, size_t(lds_blocks) * part_words(W, p) * 8);
-                        // Spill lists sized for a quarter of each block's rows
-                        // (the sample put at most an eighth outside the LDS range).
-                        const uint64_t per_block = (np + lds_blocks - 1) / lds_blocks;
-                        const uint32_t cap = ctx->tune_join_spill_cap
-                                                 ? ctx->tune_join_spill_cap
-                                                 : static_cast<uint32_t>(std::min<uint64_t>(per_block / 4 + 1024, 1u << 30));
-                        Scratch rec(ctx, size_t(lds_blocks) * cap * (1 + nv) * 8);
+                        Scratch rec(ctx, rec_bytes);
                         Scratch rcount(ctx, size_t(lds_blocks) * 4);
                         init_state(ctx, st2.as<uint64_t>(), S, p);
                         Spill sp;
