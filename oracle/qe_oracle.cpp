@@ -779,7 +779,9 @@ std::vector<uint32_t> sort_perm_comparison(const orc_col* cols, const uint32_t* 
 // the sorted-probe order of the reference.
 // Mutants: 1 = strict boundary `<` (asof_mutants kBoundaryStrict);
 // 2 = nearest following instead of preceding (kNearestFollowing);
-// 3 = the last key column is ignored (kIgnoreLastKey).
+// 3 = the last key column is ignored (kIgnoreLastKey); 4 = a LEFT probe row
+// without a match carries build row 0 instead of NULL (kLeftWrongNull,
+// tsx/asof_mutants.cpp:233-239).
 int64_t time_at(const orc_col& c, uint64_t r) { return int_at(c, r); }
 
 JoinResult* asof_impl(const orc_col* pcols, uint64_t np, const orc_col* bcols,
@@ -866,7 +868,7 @@ JoinResult* asof_impl(const orc_col* pcols, uint64_t np, const orc_col* bcols,
             res->build_rows.push_back(matched);
         } else if (type == 1) {
             res->probe_rows.push_back(r);
-            res->build_rows.push_back(kNoGroup);
+            res->build_rows.push_back((mutant == 4 && nb > 0) ? 0u : kNoGroup);
         }
     }
     delete ht;

@@ -948,7 +948,7 @@ static qe_gpu_status asof_build_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* 
             invalid(build_keys[j] != nullptr, "null key column");
             invalid(build_keys[j]->len == n, "build key and time columns differ in length");
         }
-        check(n < kNoRow, QE_GPU_UNSUPPORTED, "row indices are 32-bit");
+        check_rows32(n);
         auto* idx = new qe_dev_asof();
         try {
             idx->nkeys = static_cast<int>(nkeys);
@@ -1244,6 +1244,8 @@ qe_gpu_status qe_gpu_asof_probe(qe_gpu_ctx* ctx, const qe_dev_asof* idx, const q
     QE_ENTRY(ctx, {
         invalid(idx != nullptr && (probe_keys != nullptr || nkeys == 0), "null argument");
         invalid(static_cast<int>(nkeys) == idx->nkeys, "as-of needs equal-length key lists");
+        invalid(probe_time != nullptr, "null argument");
+        check_rows32(probe_time->len);
         OwnedCols own(ctx);
         std::vector<const qe_dev_col*> pk(probe_keys, probe_keys + nkeys);
         for (uint32_t j = 0; j < nkeys; ++j) {

@@ -700,6 +700,7 @@ static void build_batch_plan(qe_gpu_ctx* ctx, BatchPlan& bp, const qe_batch_quer
         const qe_batch_query& q = qs[i];
         invalid(q.key != nullptr && q.value != nullptr, "batched query needs key and value columns");
         invalid(q.key->len == q.value->len, "key and value columns differ in length");
+        check_rows32(q.key->len);
         invalid(q.morsel_offsets != nullptr && q.nmorsels >= 1, "batched query needs >=1 morsel");
         std::vector<uint64_t> offs(q.morsel_offsets, q.morsel_offsets + q.nmorsels + 1);
         invalid(offs.front() == 0 && offs.back() == q.key->len, "morsel offsets must cover the columns");

@@ -259,7 +259,15 @@ uint64_t common_len(const qe_dev_col* const* cols, uint32_t n) {
         if (i == 0) len = cols[i]->len;
         invalid(cols[i]->len == len, "columns of one input must have equal length");
     }
+    check_rows32(len);
     return len;
+}
+
+void check_rows32(uint64_t n) {
+    // Row ids, join pairs, permutations and first-seen ids are 32-bit with
+    // 0xFFFFFFFF as the no-row sentinel: inputs of 2^32 - 1 rows or more
+    // would alias it.
+    check(n < kNoRow, QE_GPU_UNSUPPORTED, "inputs of 2^32-1 rows or more are not supported (row ids are 32-bit)");
 }
 
 // ---- scan ------------------------------------------------------------------------

@@ -310,6 +310,7 @@ qe_gpu_status qe_gpu_ht_insert_or_find(qe_gpu_ctx* ctx, qe_dev_ht* ht, const qe_
         invalid(ht != nullptr, "null table");
         if (n == 0) return QE_GPU_OK;
         invalid(out_groups != nullptr, "null output");
+        check_rows32(n);
         const HashKeys hk = keys_of(ht, keys);
         ensure_capacity(ctx, ht, n);
         Scratch slot_of(ctx, n * 4), is_new(ctx, n * 4), rank(ctx, n * 8);
@@ -344,6 +345,7 @@ qe_gpu_status qe_gpu_ht_find(qe_gpu_ctx* ctx, const qe_dev_ht* ht, const qe_dev_
         invalid(ht != nullptr, "null table");
         if (n == 0) return QE_GPU_OK;
         invalid(out_groups != nullptr, "null output");
+        check_rows32(n);
         const HashKeys hk = keys_of(ht, keys);
         HashTableView t{ht->state, ht->hash, ht->words, ht->nullmask, ht->cap, ht->overflow};
         ht_find_kernel<<<grid_for(n, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(

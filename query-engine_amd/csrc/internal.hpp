@@ -433,7 +433,10 @@ void radix_sort_pairs(qe_gpu_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t 
                       unsigned end_bit);
 
 // Column length helper with validation.
+// Length shared by the columns of one input; rejects 2^32-1 rows or more
+// (check_rows32).
 uint64_t common_len(const qe_dev_col* const* cols, uint32_t n);
+void check_rows32(uint64_t n);
 
 // Gather rows idx[0, m) of src into a new column (idx == kNoRow gives NULL).
 

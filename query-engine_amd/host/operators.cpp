@@ -688,8 +688,10 @@ void GpuHashTable::run(const KeyColumns& keys, size_t n, uint32_t* out, bool ins
     qe_gpu_free(ctx, dout);
 }
 
-void GpuHashTable::insert_or_find(const KeyColumns& keys, size_t n, uint32_t* out) { run(keys, n, out, true); }
-void GpuHashTable::find(const KeyColumns& keys, size_t n, uint32_t* out) const { run(keys, n, out, false); }
+void GpuHashTable::insert_or_find(const KeyColumns& keys, size_t n, uint32_t* out, HashPath) {
+    run(keys, n, out, true);
+}
+void GpuHashTable::find(const KeyColumns& keys, size_t n, uint32_t* out, HashPath) const { run(keys, n, out, false); }
 size_t GpuHashTable::num_groups() const { return qe_gpu_ht_num_groups(ht_); }
 
 bool GpuHashTable::group_is_null(uint32_t g, size_t col) const {

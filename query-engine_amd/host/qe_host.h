@@ -231,6 +231,15 @@ enum class AsofType { Inner, Left };
 enum class AsofMethod { Binary, Learned };
 }  // namespace tsx
 
+// Kernel-path selectors of the reference's scalar==vector test seams
+// (ops/hashtable.h:77, ops/join.h:67, ops/aggregate.h:100, ops/sort.h:83). The
+// device has one kernel per operator, so the device operators accept and
+// record them (the same results for every value) and tests written against
+// the seams run unchanged.
+enum class HashPath { kVector, kScalar };
+enum class GatherPath { kVector, kScalar };
+enum class AggKernelPath { kVector, kScalar };
+
 // Device result drained into host batches of <= 2048 rows.
 class ResultEmitter {
    public:
@@ -258,6 +267,12 @@ class GpuAggregate : public Operator, public DeviceSource {
    public:
     static constexpr size_t kOutBatch = 2048;
     GpuAggregate(std::unique_ptr<Operator> child, std::vector<uint32_t> key_cols, std::vector<AggSpec> aggs);
+    // Test seam (ops/aggregate.h:116-119), call before open(): accepted; the
+    // device path is the same for every value.
+    void set_paths(HashPath hash_path, AggKernelPath kernel_path) {
+        hash_path_ = hash_path;
+        kernel_path_ = kernel_path;
+    }
     void open() override;
     std::optional<Batch> next() override { return emit_.next(); }
     void close() override;
@@ -271,6 +286,8 @@ class GpuAggregate : public Operator, public DeviceSource {
     std::vector<AggSpec> aggs_;
     Schema child_schema_;
     ResultEmitter emit_;
+    HashPath hash_path_ = HashPath::kVector;
+    AggKernelPath kernel_path_ = AggKernelPath::kVector;
 };
 
 // HashJoin (ops/join.h:79-81): output = probe columns ++ build columns.
@@ -279,6 +296,12 @@ class GpuHashJoin : public Operator, public DeviceSource {
     static constexpr size_t kOutBatch = 2048;
     GpuHashJoin(std::unique_ptr<Operator> probe, std::unique_ptr<Operator> build,
                 std::vector<uint32_t> probe_keys, std::vector<uint32_t> build_keys, JoinType type);
+    // Test seam (ops/join.h:86-89), call before open(): accepted; the device path is
+    // the same for every value.
+    void set_paths(HashPath hash_path, GatherPath gather_path) {
+        hash_path_ = hash_path;
+        gather_path_ = gather_path;
+    }
     void open() override;
     std::optional<Batch> next() override;
     void close() override;
@@ -294,6 +317,8 @@ class GpuHashJoin : public Operator, public DeviceSource {
     qe_dev_pairs* pairs_ = nullptr;
     size_t cursor_ = 0;
     ResultEmitter chunk_;
+    HashPath hash_path_ = HashPath::kVector;
+    GatherPath gather_path_ = GatherPath::kVector;
 };
 
 // Aggregate(HashJoin(probe, build)) fused on the device (one streaming probe).
@@ -328,6 +353,12 @@ class GpuAsofJoin : public Operator, public DeviceSource {
                 std::vector<uint32_t> probe_keys, std::vector<uint32_t> build_keys, uint32_t probe_time,
                 uint32_t build_time, tsx::AsofType type, std::optional<int64_t> tolerance = std::nullopt,
                 tsx::AsofMethod method = tsx::AsofMethod::Learned);
+    // Test seam (tsx/asof.h:110-113), call before open(): accepted; the device path is
+    // the same for every value.
+    void set_paths(HashPath hash_path, GatherPath gather_path) {
+        hash_path_ = hash_path;
+        gather_path_ = gather_path;
+    }
     void open() override;
     std::optional<Batch> next() override;
     void close() override;
@@ -346,6 +377,8 @@ class GpuAsofJoin : public Operator, public DeviceSource {
     qe_dev_pairs* pairs_ = nullptr;
     size_t cursor_ = 0;
     ResultEmitter chunk_;
+    HashPath hash_path_ = HashPath::kVector;
+    GatherPath gather_path_ = GatherPath::kVector;
 };
 
 // ---- sort vocabulary (ops/sort.h:33-42) ---------------------------------------------------
@@ -365,6 +398,8 @@ class GpuSort : public Operator, public DeviceSource {
    public:
     static constexpr size_t kOutBatch = 2048;
     GpuSort(std::unique_ptr<Operator> child, std::vector<SortKey> keys);
+    // Test seam (ops/sort.h:90), call before open(): accepted; one device path.
+    void set_gather_path(GatherPath g) { gather_path_ = g; }
     void open() override;
     std::optional<Batch> next() override { return emit_.next(); }
     void close() override;
@@ -377,6 +412,7 @@ class GpuSort : public Operator, public DeviceSource {
     std::vector<SortKey> keys_;
     Schema child_schema_;
     ResultEmitter emit_;
+    GatherPath gather_path_ = GatherPath::kVector;
 };
 
 // Device HashTable with the reference's API (ops/hashtable.h:108-190).
@@ -396,8 +432,11 @@ class GpuHashTable {
     explicit GpuHashTable(std::vector<Type> key_types, NullPolicy p = NullPolicy::kEqual,
                           HashTableConfig cfg = {}, uint64_t seed = 0x51E57A11AB1E5EEDull);
     ~GpuHashTable();
-    void insert_or_find(const KeyColumns& keys, size_t n, uint32_t* out_groups);
-    void find(const KeyColumns& keys, size_t n, uint32_t* out_groups) const;
+    // `path` is the reference's scalar/vector seam (ops/hashtable.h:125-132):
+    // accepted, one device path.
+    void insert_or_find(const KeyColumns& keys, size_t n, uint32_t* out_groups,
+                        HashPath path = HashPath::kVector);
+    void find(const KeyColumns& keys, size_t n, uint32_t* out_groups, HashPath path = HashPath::kVector) const;
     size_t num_groups() const;
     bool group_is_null(uint32_t g, size_t col) const;
     uint64_t group_key_word(uint32_t g, size_t col) const;

@@ -270,6 +270,59 @@ void test_hash_table_api() {
     CHECK(ng[0] == 0 && ng[1] == 1 && ng[2] == 2 && ng[3] == 1);
 }
 
+// The reference's scalar==vector seams (ops/join.h:86-89, ops/aggregate.h:116-119,
+// tsx/asof.h:110-113, ops/hashtable.h:125-132): every path selection gives the
+// same result through the device operators.
+void test_path_seams() {
+    Table build(sch({{"c0", Type::I32}, {"c1", Type::TS}, {"c2", Type::I64}}), [] {
+        std::vector<OwnedColumn> c;
+        c.push_back(mk<int32_t>(Type::I32, {1, 1, 1, 2}));
+        c.push_back(mk<int64_t>(Type::TS, {10, 20, 30, 15}));
+        c.push_back(mk<int64_t>(Type::I64, {100, 200, 300, 150}));
+        return c;
+    }());
+    Table probe(sch({{"c0", Type::I32}, {"c1", Type::TS}}), [] {
+        std::vector<OwnedColumn> c;
+        c.push_back(mk<int32_t>(Type::I32, {1, 1, 1, 2, 3, 1}));
+        c.push_back(mk<int64_t>(Type::TS, {5, 10, 25, 15, 99, 50}));
+        return c;
+    }());
+    auto sorted = [](std::vector<Row> r) {
+        std::sort(r.begin(), r.end());
+        return r;
+    };
+    std::vector<std::vector<Row>> joins, aggs, asofs;
+    for (HashPath hp : {HashPath::kVector, HashPath::kScalar}) {
+        for (int k = 0; k < 2; ++k) {
+            const GatherPath gp = k ? GatherPath::kScalar : GatherPath::kVector;
+            GpuHashJoin j(std::make_unique<Scan>(probe), std::make_unique<Scan>(build), {0}, {0}, JoinType::Left);
+            j.set_paths(hp, gp);
+            joins.push_back(sorted(drain(j)));
+            GpuAggregate a(std::make_unique<Scan>(build), {0}, {AggSpec::count_star("cs"), AggSpec::sum(2, "s")});
+            a.set_paths(hp, k ? AggKernelPath::kScalar : AggKernelPath::kVector);
+            aggs.push_back(sorted(drain(a)));
+            GpuAsofJoin s(std::make_unique<Scan>(probe), std::make_unique<Scan>(build), {0}, {0}, 1, 1,
+                          tsx::AsofType::Inner);
+            s.set_paths(hp, gp);
+            asofs.push_back(sorted(drain(s)));
+        }
+    }
+    for (size_t i = 1; i < joins.size(); ++i) {
+        CHECK(joins[i] == joins[0]);
+        CHECK(aggs[i] == aggs[0]);
+        CHECK(asofs[i] == asofs[0]);
+    }
+    CHECK(joins[0].size() == 7 && aggs[0].size() == 2 && asofs[0].size() == 4);
+    GpuHashTable t({Type::I32});
+    OwnedColumn c = mk<int32_t>(Type::I32, {7, 7, 3});
+    Column v = c.view();
+    KeyColumns kc{&v, 1, nullptr};
+    uint32_t g[3], f[3];
+    t.insert_or_find(kc, 3, g, HashPath::kScalar);
+    t.find(kc, 3, f, HashPath::kScalar);
+    CHECK(g[0] == 0 && g[1] == 0 && g[2] == 1 && f[0] == 0 && f[2] == 1);
+}
+
 // tests/ops_test.cpp:42-56: c0 = 0..n-1 (NULL where i % 7 == 0), c1 = 1.5 * c0.
 Table seq_table(size_t n) {
     std::vector<int32_t> a(n);
@@ -651,6 +704,7 @@ int main() {
         {"asof_plan", test_asof_plan},
         {"exceptions", test_exceptions},
         {"hash_table_api", test_hash_table_api},
+        {"path_seams", test_path_seams},
         {"filter_reference_cases", test_filter_reference_cases},
         {"filter_plan_and_device_chaining", test_filter_plan_and_device_chaining},
         {"filter_exceptions", test_filter_exceptions},

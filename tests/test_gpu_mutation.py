@@ -103,3 +103,13 @@ def test_asof_mutants(ctx, qe):
         gp = _pairs(np.arange(len(got))[got != NO], got[got != NO])
         assert gp == _pairs(*O.asof_pairs(pc, bc, keys, keys, tcol, tcol, 0))
         assert gp != _pairs(*O.asof_pairs(pc, bc, keys, keys, tcol, tcol, 0, mutant=m)), m
+    # LEFT (kLeftWrongNull): unmatched probe rows keep the no-row index.
+    lp = [col([1, 3, 1], I32), col([20, 20, 5], I64)]
+    P, B = up(ctx, lp), up(ctx, kb)
+    idx = ctx.asof_build([B[0]], B[1])
+    got = idx.probe([P[0]], P[1])
+    idx.free()
+    free(P + B)
+    gl = _pairs(np.arange(len(got)), got)
+    assert gl == _pairs(*O.asof_pairs(lp, kb, [0], [0], 1, 1, 1))
+    assert gl != _pairs(*O.asof_pairs(lp, kb, [0], [0], 1, 1, 1, mutant=4))

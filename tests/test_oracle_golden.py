@@ -477,6 +477,13 @@ def test_asof_literals():
     for m in (1, 2):
         assert pairs(*O.asof_pairs(kp, kb, [0], [0], 1, 1, 0, mutant=m)) != [(0, 1), (1, 1), (2, 2)]
     assert pairs(*O.asof_pairs(cp, cb, [0, 1], [0, 1], 2, 2, 0, mutant=3)) != [(0, 1), (1, 2)]
+    # LEFT: an unmatched probe row (key 3, and key 1 before the first tick)
+    # carries NULL build columns; kLeftWrongNull (tsx/asof_mutants.cpp:233-239)
+    # gives it build row 0 instead.
+    lp = [col([1, 3, 1], I32), col([20, 20, 5], I64)]
+    left = pairs(*O.asof_pairs(lp, kb, [0], [0], 1, 1, 1))
+    assert left == sorted([(0, 1), (1, NO), (2, NO)])
+    assert pairs(*O.asof_pairs(lp, kb, [0], [0], 1, 1, 1, mutant=4)) != left
 
 
 def brute_asof(pk, pt, bk, bt):
@@ -872,3 +879,68 @@ def test_sort_f64_total_order_literals():
     s = str_col(["b", None, "a", "B", "b"])
     assert O.sort_perm([s], [0]).tolist() == [3, 2, 0, 4, 1]       # byte order: "B" < "a" < "b"
     assert O.sort_perm([s], [0], [1], [1]).tolist() == [1, 0, 4, 2, 3]
+
+
+# ---- the morsel-parallel CPU baseline (oracle/qe_baseline.cpp) ------------------------------
+def test_morsel_baseline_equals_oracle(rng):
+    """The bench's CPU baseline (exec/parallel.cpp:431-599 restated: atomic
+    morsel cursor, per-worker partials, merge after join) gives the oracle's
+    groups, COUNTs exactly and F64 sums within 1e-12, for several thread
+    counts and morsel sizes; the as-of leg equals numpy searchsorted."""
+    L = O.oracle()
+    # config-1 shape (scaled): I64 key in [0, 1024), F64 values with NULLs
+    n = 50_000
+    key = rng.integers(0, 1024, n).astype(np.int64)
+    val = rng.uniform(0, 1000, n)
+    valid = rng.random(n) > 0.05
+    words = np.zeros((n + 63) // 64, dtype=np.uint64)
+    for i in np.flatnonzero(valid):
+        words[i >> 6] |= np.uint64(1) << np.uint64(i & 63)
+    want = {r[0]: r[1:] for r in O.aggregate([(key, None, I64), (val, valid, F64)], [0],
+                                             [(CS, 0), (SUM, 1), (CNT, 1)])}
+    for threads, morsel in ((1, 1024), (4, 1024), (7, 333)):
+        ok, oc, osum, onn = (np.empty(2048, np.int64), np.empty(2048, np.int64), np.empty(2048), np.empty(2048, np.int64))
+        g = L.orc_bl_groupby_i64_f64(key.ctypes.data, val.ctypes.data, words.ctypes.data, n, threads, morsel, 2048,
+                                     ok.ctypes.data, oc.ctypes.data, osum.ctypes.data, onn.ctypes.data)
+        assert g == len(want)
+        for i in range(g):
+            c, s, nn = want[int(ok[i])]
+            assert oc[i] == c and onn[i] == nn
+            assert abs(osum[i] - s) <= 1e-12 * abs(s) + 1e-12
+    # config-2 shape (scaled): duplicate build keys fan out, 10% probe misses
+    nb, npr = 3000, 40_000
+    bkey = rng.permutation(nb).astype(np.int64) % 2500  # 500 duplicated keys
+    bgrp = rng.integers(0, 100, nb).astype(np.int32)
+    pkey = np.where(rng.random(npr) < 0.9, rng.integers(0, 2500, npr), rng.integers(5000, 6000, npr)).astype(np.int64)
+    ppay = rng.uniform(0, 1000, npr)
+    wj = {r[0]: r[1:] for r in O.join_aggregate([(pkey, None, I64), (ppay, None, F64)],
+                                                [(bkey, None, I64), (bgrp, None, I32)], [0], [0], 0, [3],
+                                                [(SUM, 1), (CS, 0)])}
+    for threads in (1, 5):
+        og, osum, ocnt = np.empty(256, np.int32), np.empty(256), np.empty(256, np.int64)
+        g = L.orc_bl_join_groupby(pkey.ctypes.data, ppay.ctypes.data, npr, bkey.ctypes.data, bgrp.ctypes.data, nb,
+                                  threads, 1024, 256, og.ctypes.data, osum.ctypes.data, ocnt.ctypes.data)
+        assert g == len(wj)
+        for i in range(g):
+            s, c = wj[int(og[i])]
+            assert ocnt[i] == c
+            assert abs(osum[i] - s) <= 1e-12 * abs(s) + 1e-12
+    # as-of leg: upper_bound per probe over CSR segments of a sorted right side
+    nsym = 50
+    sym = np.sort(rng.integers(0, nsym, 5000))
+    ts = np.concatenate([np.sort(rng.choice(10**6, np.count_nonzero(sym == s), replace=False)) for s in range(nsym)])
+    ts = ts.astype(np.int64)
+    seg = np.searchsorted(sym, np.arange(nsym + 1)).astype(np.uint64)
+    psym = rng.integers(-1, nsym + 1, 20_000).astype(np.int32)
+    pts = rng.integers(-10, 10**6 + 10, 20_000).astype(np.int64)
+    out = np.empty(20_000, np.uint32)
+    L.orc_bl_asof_upper_bound(ts.ctypes.data, seg.ctypes.data, nsym, psym.ctypes.data, pts.ctypes.data, 20_000, 3,
+                              out.ctypes.data)
+    for i in range(20_000):
+        s = int(psym[i])
+        if s < 0 or s >= nsym:
+            assert out[i] == NO
+            continue
+        b, e = int(seg[s]), int(seg[s + 1])
+        u = b + int(np.searchsorted(ts[b:e], pts[i], side="right"))
+        assert out[i] == (NO if u == b else u - 1)
