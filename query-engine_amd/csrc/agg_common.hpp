@@ -193,6 +193,53 @@ __device__ inline void lds_update_vals(char* smem, uint32_t S, const AggProg& p,
     }
 }
 
+// A fold specialised for the programs a streaming probe mostly runs: COUNT(*)
+// (and COUNT of all-valid columns) plus at most two SUM/AVG accumulators over
+// all-valid value columns (no non-NULL counters, ncnt == 0). Its few fields
+// are kernel arguments held in scalar registers, so a row's fold is one LDS
+// atomic per field with no reload of the aggregate program.
+struct FastFold {
+    int32_t nacc;    // accumulators (0..2), the program's own
+    int32_t f64[2];  // accumulator i sums F64 (else a 64-bit integer)
+    int32_t col[2];  // value column (0 or 1) accumulator i reads
+};
+
+// Host: the fast fold of `p` when it has that shape (vcol: agg -> value column).
+inline bool make_fast_fold(const AggProg& p, const int32_t* vcol, FastFold& ff) {
+    ff = FastFold{};
+    if (p.ncnt != 0 || p.nacc > 2) return false;
+    for (int a = 0; a < p.n; ++a) {
+        const int k = p.kind[a];
+        if (k == K_COUNT_STAR || k == K_COUNT) continue;
+        if (k != K_SUM_F && k != K_SUM_I) return false;
+        if (!p.own_acc[a]) continue;
+        const int i = p.acc_of[a];
+        if (i < 0 || i > 1 || vcol[a] < 0 || vcol[a] > 1) return false;
+        ff.f64[i] = k == K_SUM_F ? 1 : 0;
+        ff.col[i] = vcol[a];
+    }
+    ff.nacc = p.nacc;
+    return true;
+}
+
+template <int NV>
+__device__ inline void lds_fold_fast(char* smem, uint32_t S, const FastFold& ff, uint32_t slot,
+                                     const uint64_t* bits) {
+    uint64_t* acc = reinterpret_cast<uint64_t*>(smem);
+    uint32_t* rows = reinterpret_cast<uint32_t*>(smem + size_t(8) * S * ff.nacc);
+    atomicAdd(&rows[slot], 1u);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        if (i >= ff.nacc) break;
+        const uint64_t v = (NV > 1 && ff.col[i]) ? bits[NV > 1 ? 1 : 0] : bits[0];
+        uint64_t* ap = &acc[size_t(i) * S + slot];
+        if (ff.f64[i])
+            atomicAdd(reinterpret_cast<double*>(ap), __longlong_as_double(static_cast<long long>(v)));
+        else
+            atomicAdd(reinterpret_cast<unsigned long long*>(ap), static_cast<unsigned long long>(v));
+    }
+}
+
 // Flush the block's LDS partials into partial[block] (compact layout).
 __device__ inline void lds_flush(const char* smem, uint32_t S, const AggProg& p, uint64_t* part) {
     const uint64_t* acc = reinterpret_cast<const uint64_t*>(smem);

@@ -520,8 +520,16 @@ __device__ inline void load_pair(const void* base, bool wide, uint64_t pr, uint6
 // Fused probe over a direct unique map with LDS partials. Each thread owns
 // PAIRS row pairs per tile; pair i of thread t is tile_base + i*blockDim + t, so
 // every load instruction of a wave reads one contiguous 1 KiB (8-byte column)
-// or 512 B (4-byte column) span. All key and value loads of a tile are issued
-// before the map lookups and LDS updates. NV = value columns read.
+// or 512 B (4-byte column) span. NV = value columns read.
+//
+// Software-pipelined: vector memory loads complete in issue order (vmcnt), so
+// a tile's map lookups are issued first, the NEXT tile's key and value loads
+// second, and only then does the wave wait -- for the lookups alone -- and
+// fold the current tile into LDS while the next tile's stream is in flight.
+// Every load is issued unconditionally (rows past the end re-read the last
+// pair, rows that cannot match look up entry 0; both are masked), so the
+// number of loads after the lookups is static and the compiler's wait before
+// the fold leaves the next tile's loads outstanding.
 //
 // Rows whose slot lies outside the LDS range are skipped (another launch covers
 // their range) or, with SPILL, written to the block's spill list: record i of
@@ -553,11 +561,31 @@ __device__ inline void spill_row(const Spill& spill, uint32_t* spill_n, const Ag
     }
 }
 
-template <typename KeyT, typename MapT, int PAIRS, int NV, bool SPILL>
+// The key and value words of one tile (pairs t0 + i*blockDim + threadIdx.x),
+// every load unconditional (pairs past the end re-read the last pair). Bit c
+// of WMASK: value column c is 8 bytes wide (else I32, sign-extended).
+template <typename KeyT, int PAIRS, int NV, int WMASK>
+__device__ inline void stream_tile_load(const KeyT* __restrict__ keys, const StreamCols& sc, uint64_t t0,
+                                        uint64_t npairs, uint64_t (&kw)[2 * PAIRS],
+                                        uint64_t (&vb)[NV > 0 ? NV : 1][2 * PAIRS]) {
+    constexpr bool kWideKey = sizeof(KeyT) == 8;
+#pragma unroll
+    for (int i = 0; i < PAIRS; ++i) {
+        uint64_t pr = t0 + uint64_t(i) * blockDim.x + threadIdx.x;
+        pr = pr < npairs ? pr : npairs - 1;
+        load_pair(keys, kWideKey, pr, kw[2 * i], kw[2 * i + 1]);
+#pragma unroll
+        for (int c = 0; c < NV; ++c) load_pair(sc.data[c], ((WMASK >> c) & 1) != 0, pr, vb[c][2 * i], vb[c][2 * i + 1]);
+    }
+}
+
+// FAST: the program has the fast fold's shape (lds_fold_fast), taken with its
+// few fields in scalar registers instead of the general program loop.
+template <typename KeyT, typename MapT, int PAIRS, int NV, int WMASK, bool SPILL, bool FAST>
 __global__ void __launch_bounds__(1024) join_agg_stream(const KeyT* __restrict__ keys, const uint64_t* key_valid,
                                                         uint64_t n, const MapT* __restrict__ gmap, int64_t kmin,
                                                         uint64_t range, uint32_t S, uint32_t slot_lo, AggProg p,
-                                                        StreamCols sc, uint64_t* part, Spill spill) {
+                                                        StreamCols sc, uint64_t* part, Spill spill, FastFold ff) {
     // S slots of LDS partials for slots [slot_lo, slot_lo + S) of the map's
     // slot space.
     extern __shared__ __align__(16) char smem[];
@@ -566,56 +594,79 @@ __global__ void __launch_bounds__(1024) join_agg_stream(const KeyT* __restrict__
     if (SPILL && threadIdx.x == 0) spill_n = 0;
     __syncthreads();
     constexpr MapT kEmpty = static_cast<MapT>(~MapT(0));
-    constexpr bool kWideKey = sizeof(KeyT) == 8;
+    constexpr int R = 2 * PAIRS;  // rows per thread per tile
     const uint64_t npairs = n >> 1;
     const uint64_t tile = uint64_t(blockDim.x) * PAIRS;
-    for (uint64_t t0 = blockIdx.x * tile; t0 < npairs; t0 += uint64_t(gridDim.x) * tile) {
-        uint64_t kw[2 * PAIRS];
-        uint64_t vb[NV > 0 ? NV : 1][2 * PAIRS];
+    const uint64_t step = uint64_t(gridDim.x) * tile;
+    uint64_t t0 = blockIdx.x * tile;
+    if (t0 < npairs) {
+        uint64_t kw[R];
+        uint64_t vb[NV > 0 ? NV : 1][R];
+        stream_tile_load<KeyT, PAIRS, NV, WMASK>(keys, sc, t0, npairs, kw, vb);
+        for (; t0 < npairs; t0 += step) {
+            // 1. This tile's validity words, then its map lookups, issued first.
+            uint32_t valid[R];
+            bool ok[R];
 #pragma unroll
-        for (int i = 0; i < PAIRS; ++i) {
-            const uint64_t pr = t0 + uint64_t(i) * blockDim.x + threadIdx.x;
-            kw[2 * i] = kw[2 * i + 1] = 0;
-            if (pr < npairs) {
-                load_pair(keys, kWideKey, pr, kw[2 * i], kw[2 * i + 1]);
-#pragma unroll
-                for (int c = 0; c < NV; ++c) load_pair(sc.data[c], sc.wide[c] != 0, pr, vb[c][2 * i], vb[c][2 * i + 1]);
-            }
-        }
-        MapT g[2 * PAIRS];
-#pragma unroll
-        for (int i = 0; i < PAIRS; ++i) {
-            const uint64_t pr = t0 + uint64_t(i) * blockDim.x + threadIdx.x;
-#pragma unroll
-            for (int e = 0; e < 2; ++e) {
-                const uint64_t r = 2 * pr + e;
-                const uint64_t off = kw[2 * i + e] - static_cast<uint64_t>(kmin);
-                bool ok = pr < npairs && off < range;
-                if (key_valid) ok = ok && ((key_valid[r >> 6] >> (r & 63)) & 1ull);
-                g[2 * i + e] = ok ? gmap[off] : kEmpty;
-            }
-        }
-#pragma unroll
-        for (int i = 0; i < PAIRS; ++i) {
-            const uint64_t pr = t0 + uint64_t(i) * blockDim.x + threadIdx.x;
-#pragma unroll
-            for (int e = 0; e < 2; ++e) {
-                const uint32_t sl = static_cast<uint32_t>(g[2 * i + e]) - slot_lo;
-                const bool in = g[2 * i + e] != kEmpty && sl < S;
-                if (!in && !(SPILL && g[2 * i + e] != kEmpty)) continue;
-                const uint64_t r = 2 * pr + e;
-                uint64_t bits[NV > 0 ? NV : 1];
-                uint32_t valid = 0;
+            for (int i = 0; i < PAIRS; ++i) {
+                const uint64_t pr = t0 + uint64_t(i) * blockDim.x + threadIdx.x;
+                const uint64_t w = (2 * pr) >> 6;  // both rows of a pair share a validity word
+                const uint32_t sh = static_cast<uint32_t>((2 * pr) & 63);
+                uint64_t kvw = ~0ull;
+                if (key_valid && pr < npairs) kvw = key_valid[w];
+                uint32_t v0 = 0, v1 = 0;
 #pragma unroll
                 for (int c = 0; c < NV; ++c) {
-                    bits[c] = vb[c][2 * i + e];
-                    const bool v = sc.validity[c] == nullptr || ((sc.validity[c][r >> 6] >> (r & 63)) & 1ull);
-                    valid |= static_cast<uint32_t>(v) << c;
+                    uint64_t vw = ~0ull;
+                    if (sc.validity[c] && pr < npairs) vw = sc.validity[c][w];
+                    v0 |= static_cast<uint32_t>((vw >> sh) & 1ull) << c;
+                    v1 |= static_cast<uint32_t>((vw >> (sh + 1)) & 1ull) << c;
                 }
-                if (!SPILL || in)
-                    lds_update_vals(smem, S, p, sl, sc.vcol, bits, valid);
-                else
-                    spill_row<NV>(spill, &spill_n, p, static_cast<uint32_t>(g[2 * i + e]), sc.vcol, bits, valid);
+                valid[2 * i] = v0;
+                valid[2 * i + 1] = v1;
+#pragma unroll
+                for (int e = 0; e < 2; ++e) {
+                    const uint64_t off = kw[2 * i + e] - static_cast<uint64_t>(kmin);
+                    ok[2 * i + e] = pr < npairs && off < range && ((kvw >> (sh + e)) & 1ull);
+                }
+            }
+            MapT g[R];
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const uint64_t off = kw[j] - static_cast<uint64_t>(kmin);
+                g[j] = gmap[ok[j] ? off : 0];
+            }
+            // 2. The next tile's stream, in flight while this tile folds.
+            uint64_t kn[R];
+            uint64_t vn[NV > 0 ? NV : 1][R];
+            stream_tile_load<KeyT, PAIRS, NV, WMASK>(keys, sc, t0 + step, npairs, kn, vn);
+            // Keep the fold below the prefetch: the scheduler would otherwise
+            // hoist the first row's fold and wait for its lookup first.
+            __builtin_amdgcn_sched_barrier(0);
+            // 3. Fold this tile into the LDS partials.
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const MapT gj = ok[j] ? g[j] : kEmpty;
+                const uint32_t sl = static_cast<uint32_t>(gj) - slot_lo;
+                const bool in = gj != kEmpty && sl < S;
+                if (!in && !(SPILL && gj != kEmpty)) continue;
+                uint64_t bits[NV > 0 ? NV : 1];
+#pragma unroll
+                for (int c = 0; c < NV; ++c) bits[c] = vb[c][j];
+                if (!SPILL || in) {
+                    if constexpr (FAST)
+                        lds_fold_fast<NV>(smem, S, ff, sl, bits);
+                    else
+                        lds_update_vals(smem, S, p, sl, sc.vcol, bits, valid[j]);
+                } else {
+                    spill_row<NV>(spill, &spill_n, p, static_cast<uint32_t>(gj), sc.vcol, bits, valid[j]);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                kw[j] = kn[j];
+#pragma unroll
+                for (int c = 0; c < NV; ++c) vb[c][j] = vn[c][j];
             }
         }
     }
@@ -1208,17 +1259,68 @@ void launch_spill_agg(qe_gpu_ctx* ctx, int nv, unsigned blocks, size_t lb, const
     QE_HIP(hipGetLastError());
 }
 
+template <typename KeyT, typename MapT, int PAIRS, int NV, int WMASK>
+void launch_stream_w(qe_gpu_ctx* ctx, unsigned blocks, size_t lb, const KeyT* keys, const uint64_t* kv, uint64_t n,
+                     const MapT* gm, int64_t kmin, uint64_t range, uint32_t S, uint32_t slot_lo, const AggProg& p,
+                     const StreamCols& sc, uint64_t* part, const Spill& spill) {
+    FastFold ff;
+    const bool fast = make_fast_fold(p, sc.vcol, ff);
+    if (spill.st) {
+        if (fast)
+            join_agg_stream<KeyT, MapT, PAIRS, NV, WMASK, true, true><<<blocks, 1024, lb, ctx->stream>>>(
+                keys, kv, n, gm, kmin, range, S, slot_lo, p, sc, part, spill, ff);
+        else
+            join_agg_stream<KeyT, MapT, PAIRS, NV, WMASK, true, false><<<blocks, 1024, lb, ctx->stream>>>(
+                keys, kv, n, gm, kmin, range, S, slot_lo, p, sc, part, spill, ff);
+    } else {
+        if (fast)
+            join_agg_stream<KeyT, MapT, PAIRS, NV, WMASK, false, true><<<blocks, 1024, lb, ctx->stream>>>(
+                keys, kv, n, gm, kmin, range, S, slot_lo, p, sc, part, spill, ff);
+        else
+            join_agg_stream<KeyT, MapT, PAIRS, NV, WMASK, false, false><<<blocks, 1024, lb, ctx->stream>>>(
+                keys, kv, n, gm, kmin, range, S, slot_lo, p, sc, part, spill, ff);
+    }
+    QE_HIP(hipGetLastError());
+}
+
+// Value widths as a template mask (bit c: column c is 8 bytes): no width
+// branch around the stream loads.
 template <typename KeyT, typename MapT, int PAIRS, int NV>
 void launch_stream_nv(qe_gpu_ctx* ctx, unsigned blocks, size_t lb, const KeyT* keys, const uint64_t* kv, uint64_t n,
                       const MapT* gm, int64_t kmin, uint64_t range, uint32_t S, uint32_t slot_lo, const AggProg& p,
                       const StreamCols& sc, uint64_t* part, const Spill& spill) {
-    if (spill.st)
-        join_agg_stream<KeyT, MapT, PAIRS, NV, true><<<blocks, 1024, lb, ctx->stream>>>(keys, kv, n, gm, kmin, range, S,
-                                                                                        slot_lo, p, sc, part, spill);
-    else
-        join_agg_stream<KeyT, MapT, PAIRS, NV, false><<<blocks, 1024, lb, ctx->stream>>>(
-            keys, kv, n, gm, kmin, range, S, slot_lo, p, sc, part, spill);
-    QE_HIP(hipGetLastError());
+    int wmask = 0;
+    for (int c = 0; c < NV; ++c) wmask |= (sc.wide[c] ? 1 : 0) << c;
+    if constexpr (NV == 0) {
+        launch_stream_w<KeyT, MapT, PAIRS, NV, 0>(ctx, blocks, lb, keys, kv, n, gm, kmin, range, S, slot_lo, p, sc,
+                                                  part, spill);
+    } else if constexpr (NV == 1) {
+        if (wmask)
+            launch_stream_w<KeyT, MapT, PAIRS, NV, 1>(ctx, blocks, lb, keys, kv, n, gm, kmin, range, S, slot_lo, p,
+                                                      sc, part, spill);
+        else
+            launch_stream_w<KeyT, MapT, PAIRS, NV, 0>(ctx, blocks, lb, keys, kv, n, gm, kmin, range, S, slot_lo, p,
+                                                      sc, part, spill);
+    } else {
+        switch (wmask) {
+            case 0:
+                launch_stream_w<KeyT, MapT, PAIRS, NV, 0>(ctx, blocks, lb, keys, kv, n, gm, kmin, range, S, slot_lo,
+                                                          p, sc, part, spill);
+                break;
+            case 1:
+                launch_stream_w<KeyT, MapT, PAIRS, NV, 1>(ctx, blocks, lb, keys, kv, n, gm, kmin, range, S, slot_lo,
+                                                          p, sc, part, spill);
+                break;
+            case 2:
+                launch_stream_w<KeyT, MapT, PAIRS, NV, 2>(ctx, blocks, lb, keys, kv, n, gm, kmin, range, S, slot_lo,
+                                                          p, sc, part, spill);
+                break;
+            default:
+                launch_stream_w<KeyT, MapT, PAIRS, NV, 3>(ctx, blocks, lb, keys, kv, n, gm, kmin, range, S, slot_lo,
+                                                          p, sc, part, spill);
+                break;
+        }
+    }
 }
 
 template <typename KeyT, typename MapT, int PAIRS>

@@ -312,7 +312,7 @@ void test_path_seams() {
         CHECK(aggs[i] == aggs[0]);
         CHECK(asofs[i] == asofs[0]);
     }
-    CHECK(joins[0].size() == 7 && aggs[0].size() == 2 && asofs[0].size() == 4);
+    CHECK(joins[0].size() == 14 && aggs[0].size() == 2 && asofs[0].size() == 4);  // 4x3 + 1 + one NULL-extended
     GpuHashTable t({Type::I32});
     OwnedColumn c = mk<int32_t>(Type::I32, {7, 7, 3});
     Column v = c.view();

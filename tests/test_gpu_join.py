@@ -220,6 +220,35 @@ def test_fused_direct_stream_shapes(ctx, qe, pairs, case):
         ctx.set_tuning(qe.TUNE_JOIN_PAIRS, 4)
 
 
+@pytest.mark.parametrize("case", range(5))
+def test_fused_stream_wide_maps(ctx, case):
+    """The streaming probe at 2^20+ rows over maps of 2^16..2^17 keys: skewed
+    and uniform keys with 10% misses, I32/I64 keys (negative offsets), one or
+    two value columns (F64, I64, narrow I32: every width specialisation of the
+    stream loads), an odd probe length, and (case 4) each probe tile's keys in
+    one 4096-key stretch of the map. Against the oracle."""
+    r = np.random.default_rng(1300 + case)
+    n_b = [1 << 16, 1 << 17, 50000, 1 << 16, 1 << 16][case]
+    n_p = [1 << 20, (1 << 20) + 3, 1 << 21, 1 << 20, 1 << 20][case]
+    kt = I32 if case % 2 else I64
+    kmin = [0, -5000, 123, 7, 0][case]
+    keys = r.permutation(n_b) + kmin
+    if case == 4:
+        tile = 1024 * 4 * 2  # rows per probe tile (4 pairs per thread)
+        t = np.arange(n_p) // tile
+        pkv = kmin + (t % 16) * 4096 + r.integers(0, 4096, n_p)
+    else:
+        s = [1.1, 0.0, 1.3, 1.05][case]
+        ranks = np.minimum(r.zipf(s, n_p) - 1, n_b - 1) if s > 0 else r.integers(0, n_b, n_p)
+        pkv = np.where(r.random(n_p) < 0.1, kmin + n_b + r.integers(0, 1000, n_p), kmin + ranks)
+    pcols = [col(pkv, kt), col(r.uniform(0, 1000, n_p), F64), col(r.integers(-2**40, 2**40, n_p), I64),
+             col(r.integers(-1000, 1000, n_p), I32)]
+    bcols = [col(keys, kt), col(r.integers(0, [10000, 3000, 500, 60000, 10000][case], n_b), I32)]
+    aggs = [[(SUM, 1), (CS, 0)], [(SUM, 1), (SUM, 2), (CS, 0)], [(AVG, 3), (CS, 0)],
+            [(SUM, 2), (AVG, 1), (CS, 0)], [(SUM, 1), (CS, 0)]][case]
+    join_agg_case(ctx, pcols, bcols, [0], [0], INNER, [5], aggs)
+
+
 def test_fused_duplicate_build_keys_list_path(ctx, rng):
     n_b, n_p = 5000, 50000
     pcols = [col(rng.integers(0, 2000, n_p), I64), col(rng.uniform(0, 10, n_p), F64, nulls=(3, 5, 7))]
