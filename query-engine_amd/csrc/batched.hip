@@ -136,52 +136,72 @@ __device__ inline void with_widths(const QueryDev& q, const F& f) {
     }
 }
 
-// One value's fold into LDS slot partials (layout acc u64[nacc][S] | cnt
-// u32[ncnt][S] | rows u32[S]) for a query whose aggregates all read the value
-// column: the same updates as lds_update_vals, with the shared accumulators and
-// the one non-NULL counter addressed directly.
-__device__ inline void lds_combine(int kind, uint64_t* ap, uint64_t v) {
-    switch (kind) {
-        case K_SUM_I:
-            atomicAdd(reinterpret_cast<unsigned long long*>(ap), static_cast<unsigned long long>(v));
-            break;
-        case K_SUM_F:
-            atomicAdd(reinterpret_cast<double*>(ap), __longlong_as_double(static_cast<long long>(v)));
-            break;
-        case K_MIN_I:
-            atomicMin(reinterpret_cast<long long*>(ap), static_cast<long long>(v));
-            break;
-        case K_MAX_I:
-            atomicMax(reinterpret_cast<long long*>(ap), static_cast<long long>(v));
-            break;
-        case K_MIN_F:
-            atomicMin(reinterpret_cast<unsigned long long*>(ap),
-                      static_cast<unsigned long long>(f64_order_key(__longlong_as_double(static_cast<long long>(v)))));
-            break;
-        case K_MAX_F:
-            atomicMax(reinterpret_cast<unsigned long long*>(ap),
-                      static_cast<unsigned long long>(f64_order_key(__longlong_as_double(static_cast<long long>(v)))));
-            break;
-        default:
-            break;
-    }
-}
-
+// One value's fold into LDS slot partials (the lds_bytes layout acc
+// u64[nacc][S] | cnt u32[ncnt][S] | rows u32[S] | fhi u32[nfs][S]) for a query
+// whose aggregates all read the value column: the same updates as
+// lds_update_vals (exact F64 sums included), with the shared accumulators and
+// the one non-NULL counter addressed directly. gslot_lo: the state slot of
+// LDS slot 0.
 __device__ inline void fold_value(char* smem, uint32_t S, const AggProg& p, uint32_t slot, uint64_t v,
-                                  uint32_t valid) {
-    uint64_t* acc = reinterpret_cast<uint64_t*>(smem);
+                                  uint32_t valid, uint64_t gslot_lo = 0) {
     uint32_t* cnt = reinterpret_cast<uint32_t*>(smem + size_t(8) * S * p.nacc);
     uint32_t* rows = cnt + size_t(S) * p.ncnt;
     atomicAdd(&rows[slot], 1u);
     if (!valid) return;
     if (p.ncnt) atomicAdd(&cnt[slot], 1u);
-    for (int a = 0; a < p.nacc; ++a) lds_combine(p.acc_kind[a], &acc[size_t(a) * S + slot], v);
+    for (int a = 0; a < p.nacc; ++a) lds_acc_value(smem, S, p, a, p.acc_kind[a], slot, gslot_lo + slot, v);
+}
+
+// LDS partial slot s (the lds_bytes layout of S slots) folded into state slot
+// g of the query's HBM state (SS slots) with order-free atomics: integer sums,
+// 128-bit F64 sums, MIN/MAX.
+__device__ inline void combine_atomic(int kind, uint64_t* dst, uint64_t v);
+__device__ inline void flush_slot_atomic(const char* smem, uint32_t S, const AggProg& p, uint32_t s, uint64_t* st,
+                                         uint64_t SS, uint64_t g) {
+    const uint64_t* acc = reinterpret_cast<const uint64_t*>(smem);
+    const uint32_t* cnt = reinterpret_cast<const uint32_t*>(smem + size_t(8) * S * p.nacc);
+    const uint32_t* rows = cnt + size_t(S) * p.ncnt;
+    const uint32_t* fhi = rows + S;
+    if (rows[s] == 0) return;
+    atomicAdd(reinterpret_cast<unsigned long long*>(&st[g]), static_cast<unsigned long long>(rows[s]));
+    for (int a = 0; a < p.nacc; ++a) {
+        const uint64_t v = acc[size_t(a) * S + s];
+        if (p.acc_kind[a] == K_SUM_F) {
+            const int j = p.fhi_of[a];
+            global_add_fixed(&st[(1 + size_t(a)) * SS + g], &st[state_fhi(SS, p, j) + g], v,
+                             static_cast<int64_t>(static_cast<int32_t>(fhi[size_t(j) * S + s])));
+        } else {
+            combine_atomic(p.acc_kind[a], &st[(1 + size_t(a)) * SS + g], v);
+        }
+    }
+    for (int c = 0; c < p.ncnt; ++c)
+        if (cnt[size_t(c) * S + s])
+            atomicAdd(reinterpret_cast<unsigned long long*>(&st[(1 + size_t(p.nacc) + c) * SS + g]),
+                      static_cast<unsigned long long>(cnt[size_t(c) * S + s]));
+}
+
+// The same when the block is the slot's only writer: plain stores.
+__device__ inline void flush_slot_store(const char* smem, uint32_t S, const AggProg& p, uint32_t s, uint64_t* st,
+                                        uint64_t SS, uint64_t g) {
+    const uint64_t* acc = reinterpret_cast<const uint64_t*>(smem);
+    const uint32_t* cnt = reinterpret_cast<const uint32_t*>(smem + size_t(8) * S * p.nacc);
+    const uint32_t* rows = cnt + size_t(S) * p.ncnt;
+    const uint32_t* fhi = rows + S;
+    st[g] = rows[s];
+    for (int a = 0; a < p.nacc; ++a) st[(1 + size_t(a)) * SS + g] = acc[size_t(a) * S + s];
+    for (int c = 0; c < p.ncnt; ++c) st[(1 + size_t(p.nacc) + c) * SS + g] = cnt[size_t(c) * S + s];
+    for (int j = 0; j < p.nfs; ++j)
+        st[state_fhi(SS, p, j) + g] = static_cast<uint64_t>(static_cast<int64_t>(static_cast<int32_t>(fhi[size_t(j) * S + s])));
 }
 
 // ---- LDS slot cache of the hash kind ------------------------------------------------------
 // Layout for C entries: tag u32[C] | rows u32[C] | acc u64[kBatchMaxAcc][C] | cnt u32[kBatchMaxAcc][C]
+// | fhi u32[kBatchMaxAcc][C] (high words of exact F64 sums).
 __host__ __device__ inline size_t cache_bytes(uint32_t C) {
-    return size_t(C) * (4 + 4 + 8 * kBatchMaxAcc + 4 * kBatchMaxAcc);
+    return size_t(C) * (4 + 4 + 8 * kBatchMaxAcc + 4 * kBatchMaxAcc + 4 * kBatchMaxAcc);
+}
+__device__ inline uint32_t* cache_fhi(char* smem, uint32_t C) {
+    return reinterpret_cast<uint32_t*>(smem + size_t(8) * C + size_t(12) * kBatchMaxAcc * C);
 }
 
 __device__ inline void cache_reset(char* smem, uint32_t C, const AggProg& p) {
@@ -194,6 +214,7 @@ __device__ inline void cache_reset(char* smem, uint32_t C, const AggProg& p) {
         rows[i] = 0;
         for (int a = 0; a < p.nacc; ++a) acc[size_t(a) * C + i] = acc_identity(p.acc_kind[a]);
         for (int c = 0; c < p.ncnt; ++c) cnt[size_t(c) * C + i] = 0;
+        for (int j = 0; j < p.nfs; ++j) cache_fhi(smem, C)[size_t(j) * C + i] = 0;
     }
 }
 
@@ -219,9 +240,18 @@ __device__ inline bool cache_update(char* smem, uint32_t C, const AggProg& p, ui
                 atomicAdd(reinterpret_cast<unsigned long long*>(ap),
                           static_cast<unsigned long long>(int_value(p.in[a], r)));
                 break;
-            case K_SUM_F:
-                atomicAdd(reinterpret_cast<double*>(ap), f64_value(p.in[a], r));
+            case K_SUM_F: {
+                const int i = p.acc_of[a];
+                uint64_t l;
+                int64_t h;
+                const uint64_t vbits = value_bits(p.in[a], r);
+                const uint32_t code = f64_fixed(vbits, p.fsc[i], p.fsc[kMaxAggs + i], l, h);
+                if (code == 0)
+                    lds_add_fixed(ap, cache_fhi(smem, C) + size_t(p.fhi_of[i]) * C + e, l, h);
+                else
+                    f64_special(p, p.fhi_of[i], s, code, vbits);
                 break;
+            }
             case K_MIN_I:
                 atomicMin(reinterpret_cast<long long*>(ap), static_cast<long long>(int_value(p.in[a], r)));
                 break;
@@ -248,12 +278,6 @@ __device__ inline void combine_atomic(int kind, uint64_t* dst, uint64_t v) {
         case K_SUM_I:
             atomicAdd(reinterpret_cast<unsigned long long*>(dst), static_cast<unsigned long long>(v));
             break;
-        case K_SUM_F: {
-            double d;
-            memcpy(&d, &v, 8);
-            unsafeAtomicAdd(reinterpret_cast<double*>(dst), d);
-            break;
-        }
         case K_MIN_I:
             atomicMin(reinterpret_cast<long long*>(dst), static_cast<long long>(v));
             break;
@@ -282,7 +306,15 @@ __device__ inline void cache_flush(char* smem, uint32_t C, const QueryDev& q) {
         const uint32_t s = tag[i];
         if (s == kEmptyTag) continue;
         atomicAdd(reinterpret_cast<unsigned long long*>(&q.st[s]), static_cast<unsigned long long>(rows[i]));
-        for (int a = 0; a < p.nacc; ++a) combine_atomic(p.acc_kind[a], &q.st[(1 + size_t(a)) * S + s], acc[size_t(a) * C + i]);
+        for (int a = 0; a < p.nacc; ++a) {
+            if (p.acc_kind[a] == K_SUM_F) {
+                const int j = p.fhi_of[a];
+                global_add_fixed(&q.st[(1 + size_t(a)) * S + s], &q.st[state_fhi(S, p, j) + s], acc[size_t(a) * C + i],
+                                 static_cast<int64_t>(static_cast<int32_t>(cache_fhi(smem, C)[size_t(j) * C + i])));
+            } else {
+                combine_atomic(p.acc_kind[a], &q.st[(1 + size_t(a)) * S + s], acc[size_t(a) * C + i]);
+            }
+        }
         for (int c = 0; c < p.ncnt; ++c)
             atomicAdd(reinterpret_cast<unsigned long long*>(&q.st[(1 + size_t(p.nacc) + c) * S + s]),
                       static_cast<unsigned long long>(cnt[size_t(c) * C + i]));
@@ -366,19 +398,7 @@ __device__ inline void small_tile(const QueryDev& q, const Tile& tile, char* sme
         }
     }
     __syncthreads();
-    const uint64_t* acc = reinterpret_cast<const uint64_t*>(smem);
-    const uint32_t* cnt = reinterpret_cast<const uint32_t*>(smem + size_t(8) * S * q.p.nacc);
-    const uint32_t* rows = cnt + size_t(S) * q.p.ncnt;
-    for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) {
-        if (rows[s] == 0) continue;
-        atomicAdd(reinterpret_cast<unsigned long long*>(&q.st[s]), static_cast<unsigned long long>(rows[s]));
-        for (int a = 0; a < q.p.nacc; ++a)
-            combine_atomic(q.p.acc_kind[a], &q.st[(1 + size_t(a)) * q.S + s], acc[size_t(a) * S + s]);
-        for (int c = 0; c < q.p.ncnt; ++c)
-            if (cnt[size_t(c) * S + s])
-                atomicAdd(reinterpret_cast<unsigned long long*>(&q.st[(1 + size_t(q.p.nacc) + c) * q.S + s]),
-                          static_cast<unsigned long long>(cnt[size_t(c) * S + s]));
-    }
+    for (uint32_t s = threadIdx.x; s < S; s += blockDim.x) flush_slot_atomic(smem, S, q.p, s, q.st, q.S, s);
 }
 
 // Staged scatter for queries with few partitions: each step of kStageRows rows
@@ -478,10 +498,16 @@ __device__ inline void scatter_tile(const QueryDev& q, const Tile& tile, char* s
 __global__ void batch_init_kernel(const QueryDev* qs, uint32_t nq, int all) {
     for (uint32_t qi = 0; qi < nq; ++qi) {
         const QueryDev& q = qs[qi];
-        if (q.kind == BK_PART && q.chunks == 1 && !all) continue;
         const uint64_t S = q.S;
+        if (q.kind == BK_PART && q.chunks == 1 && !all) {  // fully written; only the flags and outlier sums start at 0
+            for (uint64_t s = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; s < S;
+                 s += uint64_t(gridDim.x) * blockDim.x)
+                for (int j = 0; j < 2 * q.p.nfs; ++j) q.st[state_flags(S, q.p, 0) + size_t(j) * S + s] = 0;
+            continue;
+        }
         for (uint64_t s = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; s < S;
              s += uint64_t(gridDim.x) * blockDim.x) {
+            for (int j = 0; j < 3 * q.p.nfs; ++j) q.st[state_fhi(S, q.p, 0) + size_t(j) * S + s] = 0;
             if (q.kind == BK_HASH) q.t.state[s] = 0;
             q.st[s] = 0;
             for (int a = 0; a < q.p.nacc; ++a) q.st[(1 + size_t(a)) * S + s] = acc_identity(q.p.acc_kind[a]);
@@ -610,31 +636,14 @@ __global__ void __launch_bounds__(kPartAggThreads) batch_part_agg_kernel(const Q
 #pragma unroll
             for (int u = 0; u < kRowUnroll; ++u) {
                 const uint64_t k = k0 + uint64_t(u) * blockDim.x + threadIdx.x;
-                if (k < e) fold_value(smem, cnt, q.p, w[u] & 0x7FFFFFFFu, bits[u], w[u] >> 31);
+                if (k < e) fold_value(smem, cnt, q.p, w[u] & 0x7FFFFFFFu, bits[u], w[u] >> 31, lo);
             }
         }
         __syncthreads();
-        const uint64_t* acc = reinterpret_cast<const uint64_t*>(smem);
-        const uint32_t* cn = reinterpret_cast<const uint32_t*>(smem + size_t(8) * cnt * q.p.nacc);
-        const uint32_t* rows = cn + size_t(cnt) * q.p.ncnt;
         if (q.chunks == 1) {  // sole owner of these slots
-            for (uint32_t s = threadIdx.x; s < cnt; s += blockDim.x) {
-                q.st[lo + s] = rows[s];
-                for (int a = 0; a < q.p.nacc; ++a) q.st[(1 + size_t(a)) * q.S + lo + s] = acc[size_t(a) * cnt + s];
-                for (int c = 0; c < q.p.ncnt; ++c)
-                    q.st[(1 + size_t(q.p.nacc) + c) * q.S + lo + s] = cn[size_t(c) * cnt + s];
-            }
+            for (uint32_t s = threadIdx.x; s < cnt; s += blockDim.x) flush_slot_store(smem, cnt, q.p, s, q.st, q.S, lo + s);
         } else {
-            for (uint32_t s = threadIdx.x; s < cnt; s += blockDim.x) {
-                if (rows[s] == 0) continue;
-                atomicAdd(reinterpret_cast<unsigned long long*>(&q.st[lo + s]), static_cast<unsigned long long>(rows[s]));
-                for (int a = 0; a < q.p.nacc; ++a)
-                    combine_atomic(q.p.acc_kind[a], &q.st[(1 + size_t(a)) * q.S + lo + s], acc[size_t(a) * cnt + s]);
-                for (int c = 0; c < q.p.ncnt; ++c)
-                    if (cn[size_t(c) * cnt + s])
-                        atomicAdd(reinterpret_cast<unsigned long long*>(&q.st[(1 + size_t(q.p.nacc) + c) * q.S + lo + s]),
-                                  static_cast<unsigned long long>(cn[size_t(c) * cnt + s]));
-            }
+            for (uint32_t s = threadIdx.x; s < cnt; s += blockDim.x) flush_slot_atomic(smem, cnt, q.p, s, q.st, q.S, lo + s);
         }
         __syncthreads();
     }
@@ -778,6 +787,11 @@ static void build_batch_plan(qe_gpu_ctx* ctx, BatchPlan& bp, const qe_batch_quer
         Scratch st(ctx, state_words(d.S, d.p) * 8);
         d.st = st.as<uint64_t>();
         bp.bufs.push_back(std::move(st));
+        // Exact F64 sums: the plan runs many times over the same columns, so
+        // the scales come from every row, once, here.
+        Scratch fb;
+        prepare_f64(ctx, d.p, d.st, d.S, n, n, true, fb);
+        bp.bufs.push_back(std::move(fb));
         bp.hq[i] = d;
     }
     bp.dq = Scratch(ctx, nq * sizeof(QueryDev));

@@ -88,7 +88,139 @@ AggProg make_prog(const qe_agg_spec* aggs, uint32_t naggs, const ColView* inputs
             }
         }
     }
+    p.nfs = 0;
+    for (int i = 0; i < kMaxAggs; ++i) p.fhi_of[i] = -1;
+    for (int i = 0; i < p.nacc; ++i)
+        if (p.acc_kind[i] == K_SUM_F) p.fhi_of[i] = p.nfs++;
     return p;
+}
+
+// ---- exact F64 sums: the scales ------------------------------------------------------------
+// Columns summed in F64 and the accumulator each feeds.
+struct F64Cols {
+    ColView col[kMaxAggs];
+    int32_t acc[kMaxAggs];
+    int32_t n;
+};
+
+// Largest biased exponent of the finite, non-NULL values among rows 0, stride,
+// 2*stride, ... of column blockIdx.y: one word per (column, block).
+__global__ void f64_exp_kernel(F64Cols fc, uint64_t n, uint64_t stride, uint32_t* part) {
+    __shared__ uint32_t bm;
+    if (threadIdx.x == 0) bm = 0;
+    __syncthreads();
+    const ColView c = fc.col[blockIdx.y];
+    const uint64_t ns = stride ? (n + stride - 1) / stride : 0;
+    uint32_t mx = 0;
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < ns; i += uint64_t(gridDim.x) * blockDim.x) {
+        const uint64_t r = i * stride;
+        if (!is_valid(c, r)) continue;
+        const uint32_t be = static_cast<uint32_t>((static_cast<const uint64_t*>(c.data)[r] >> 52) & 0x7FF);
+        if (be != 0x7FF && be > mx) mx = be;
+    }
+    atomicMax(&bm, mx);
+    __syncthreads();
+    if (threadIdx.x == 0) part[size_t(blockIdx.y) * gridDim.x + blockIdx.x] = bm;
+}
+
+// Per column: the exponent bound e (|x| < 2^e: the largest exponent seen, +8
+// bits of headroom when sampled) and the scale s = 95 - e - L, where 2^L is
+// four times above the most rows one accumulator folds (a 96-bit LDS partial
+// or, for HBM atomics, the 128-bit state, which has 32 bits more).
+__global__ void f64_scale_kernel(const uint32_t* part, uint32_t nblk, F64Cols fc, int32_t headroom, int32_t L,
+                                 int32_t* fsc) {
+    const int j = threadIdx.x;
+    if (j >= fc.n) return;
+    uint32_t mx = 0;
+    for (uint32_t b = 0; b < nblk; ++b) mx = max(mx, part[size_t(j) * nblk + b]);
+    const int e = (mx == 0 ? -1022 : static_cast<int>(mx) - 1022) + headroom;
+    fsc[fc.acc[j]] = 95 - e - L;
+    fsc[kMaxAggs + fc.acc[j]] = e;
+}
+
+void prepare_f64(qe_gpu_ctx* ctx, AggProg& p, uint64_t* st, uint64_t S, uint64_t n_rows, uint64_t fold_rows,
+                 bool exact, Scratch& buf) {
+    if (p.nfs == 0) return;
+    F64Cols fc{};
+    for (int a = 0; a < p.n; ++a)
+        if (p.kind[a] == K_SUM_F && p.own_acc[a]) {
+            fc.col[fc.n] = p.in[a];
+            fc.acc[fc.n] = p.acc_of[a];
+            ++fc.n;
+        }
+    const uint32_t nblk = exact ? std::max<uint32_t>(1, std::min<uint32_t>(ctx->num_cus * 4, (n_rows + 255) / 256)) : 64;
+    // fsc | overflow word, outlier count | outliers | per-block exponents
+    buf = Scratch(ctx, size_t(2 * kMaxAggs) * 4 + 8 + size_t(kMaxOutliers) * 16 + size_t(fc.n) * nblk * 4);
+    int32_t* fsc = buf.as<int32_t>();
+    unsigned int* ovf = reinterpret_cast<unsigned int*>(fsc + 2 * kMaxAggs);
+    uint64_t* fout = reinterpret_cast<uint64_t*>(ovf + 2);
+    uint32_t* part = reinterpret_cast<uint32_t*>(fout + 2 * size_t(kMaxOutliers));
+    memset_async(ctx, ovf, 0, 8);
+    if (st) memset_async(ctx, st + state_flags(S, p, 0), 0, size_t(2 * p.nfs) * S * 8);  // flags, outlier sums
+    const uint64_t stride = exact ? 1 : std::max<uint64_t>(1, n_rows >> 16);
+    f64_exp_kernel<<<dim3(nblk, fc.n), 256, 0, ctx->stream>>>(fc, n_rows, n_rows ? stride : 0, part);
+    int L = 1;
+    while (L < 62 && (uint64_t(1) << L) <= fold_rows) ++L;
+    L += 2;
+    f64_scale_kernel<<<1, 64, 0, ctx->stream>>>(part, nblk, fc, exact ? 0 : 8, L, fsc);
+    QE_HIP(hipGetLastError());
+    p.fsc = fsc;
+    p.fflag = st ? st + state_flags(S, p, 0) : nullptr;
+    p.fflag_stride = S;
+    p.fovf = ovf;
+    p.fout = fout;
+    p.fout_n = ovf + 1;
+}
+
+// The outliers' sums per (F64 sum, slot): one block sorts the list by (sum,
+// slot, value bits) in LDS and each run is added in that order, so the sums
+// do not depend on the order the values arrived in. They land in the state's
+// osum words; the list is left empty.
+__global__ void __launch_bounds__(1024) outlier_fold_kernel(AggProg p, uint64_t* st, uint64_t S) {
+    __shared__ uint64_t key[kMaxOutliers], val[kMaxOutliers];
+    const uint32_t n = min(*p.fout_n, kMaxOutliers);
+    if (n == 0) return;
+    uint32_t m = 1;
+    while (m < n) m <<= 1;
+    for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+        key[i] = i < n ? p.fout[2 * i] : ~uint64_t(0);
+        val[i] = i < n ? p.fout[2 * i + 1] : 0;
+    }
+    __syncthreads();
+    for (uint32_t k = 2; k <= m; k <<= 1) {  // bitonic sort on (key, val)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t i = threadIdx.x; i < m; i += blockDim.x) {
+                const uint32_t l = i ^ j;
+                if (l <= i) continue;
+                const bool up = (i & k) == 0;
+                const bool gt = key[i] > key[l] || (key[i] == key[l] && val[i] > val[l]);
+                if (gt == up) {
+                    const uint64_t tk = key[i], tv = val[i];
+                    key[i] = key[l];
+                    val[i] = val[l];
+                    key[l] = tk;
+                    val[l] = tv;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        if (i > 0 && key[i] == key[i - 1]) continue;
+        double sum = 0.0;
+        for (uint32_t j = i; j < n && key[j] == key[i]; ++j) sum += __longlong_as_double(static_cast<long long>(val[j]));
+        const int fi = static_cast<int>(key[i] >> 48);
+        const uint64_t slot = key[i] & ((uint64_t(1) << 48) - 1);
+        st[state_osum(S, p, fi) + slot] = static_cast<uint64_t>(__double_as_longlong(sum));
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) *p.fout_n = 0;
+}
+
+void fold_outliers(qe_gpu_ctx* ctx, const AggProg& p, uint64_t* st, uint64_t S) {
+    if (p.nfs == 0 || p.fout_n == nullptr) return;
+    outlier_fold_kernel<<<1, 1024, 0, ctx->stream>>>(p, st, S);
+    QE_HIP(hipGetLastError());
 }
 
 __global__ void init_state_kernel(uint64_t* st, uint64_t S, AggProg p) {
@@ -97,6 +229,7 @@ __global__ void init_state_kernel(uint64_t* st, uint64_t S, AggProg p) {
         st[s] = 0;
         for (int i = 0; i < p.nacc; ++i) st[(1 + size_t(i)) * S + s] = acc_identity(p.acc_kind[i]);
         for (int c = 0; c < p.ncnt; ++c) st[(1 + size_t(p.nacc) + c) * S + s] = 0;
+        for (int j = 0; j < 3 * p.nfs; ++j) st[state_fhi(S, p, 0) + size_t(j) * S + s] = 0;  // fhi, flags, osum
     }
 }
 
@@ -109,15 +242,6 @@ __device__ inline uint64_t combine(int kind, uint64_t a, uint64_t b) {
     switch (kind) {
         case K_SUM_I:
             return a + b;
-        case K_SUM_F: {
-            double x, y;
-            memcpy(&x, &a, 8);
-            memcpy(&y, &b, 8);
-            x += y;
-            uint64_t r;
-            memcpy(&r, &x, 8);
-            return r;
-        }
         case K_MIN_I:
             return static_cast<int64_t>(a) < static_cast<int64_t>(b) ? a : b;
         case K_MAX_I:
@@ -132,30 +256,49 @@ __device__ inline uint64_t combine(int kind, uint64_t a, uint64_t b) {
 }
 
 // One thread per (field, slot) and chunk of `chunk` consecutive blocks: the
-// blocks of a chunk are folded in block order, then the chunk results in chunk
-// order, so the FP64 result is reproducible run to run. With `accumulate` the
-// result folds into what `out` holds instead of replacing it.
+// chunk's blocks fold in block order, then the chunks. Every sum is an integer
+// sum (F64 sums as 128-bit fixed point: the thread of the low word folds the
+// high word with its carries; the high-word field's own thread does nothing),
+// MIN/MAX are order-free, so the result does not depend on the order. With
+// `accumulate` the result folds into what `out` holds instead of replacing it.
 __global__ void merge_partials_kernel(const uint64_t* part, uint32_t nblocks, uint32_t chunk, uint64_t* out,
                                       uint64_t S, AggProg p, uint64_t out_S, uint64_t out_lo, int accumulate) {
-    const uint64_t W = state_words(S, p);
-    const uint64_t Wout = state_words(out_S, p);
+    const uint64_t W = merge_words(S, p);
+    const uint64_t Wout = merge_words(out_S, p);
     const uint32_t nchunks = nblocks ? (nblocks + chunk - 1) / chunk : 1;  // 0 blocks: identities
+    const uint64_t fhi0 = 1 + uint64_t(p.nacc) + p.ncnt;  // first high-word field
     for (uint64_t t = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; t < W * nchunks;
          t += uint64_t(gridDim.x) * blockDim.x) {
         const uint64_t i = t % W;
         const uint32_t c = static_cast<uint32_t>(t / W);
         const uint32_t b0 = c * chunk, b1 = min(nblocks, b0 + chunk);
         const uint64_t field = i / S;
-        const size_t o = size_t(c) * Wout + field * out_S + out_lo + (i - field * S);
-        uint64_t v;
+        const uint64_t s = i - field * S;
+        const size_t o = size_t(c) * Wout + field * out_S + out_lo + s;
+        if (field >= fhi0) continue;  // high words: folded with their low words
         if (field == 0 || field > uint64_t(p.nacc)) {  // rows or cnt: integer sums
-            v = accumulate ? out[o] : 0;
+            uint64_t v = accumulate ? out[o] : 0;
             for (uint32_t b = b0; b < b1; ++b) v += part[size_t(b) * W + i];
-        } else {
-            const int kind = p.acc_kind[field - 1];
-            v = accumulate ? out[o] : acc_identity(kind);
-            for (uint32_t b = b0; b < b1; ++b) v = combine(kind, v, part[size_t(b) * W + i]);
+            out[o] = v;
+            continue;
         }
+        const int ai = static_cast<int>(field - 1);
+        const int kind = p.acc_kind[ai];
+        if (kind == K_SUM_F) {
+            const uint64_t hf = fhi0 + p.fhi_of[ai];
+            const size_t oh = size_t(c) * Wout + hf * out_S + out_lo + s;
+            uint64_t lo = accumulate ? out[o] : 0, hi = accumulate ? out[oh] : 0;
+            for (uint32_t b = b0; b < b1; ++b) {
+                const uint64_t l = part[size_t(b) * W + i];
+                lo += l;
+                hi += part[size_t(b) * W + hf * S + s] + (lo < l ? 1 : 0);
+            }
+            out[o] = lo;
+            out[oh] = hi;
+            continue;
+        }
+        uint64_t v = accumulate ? out[o] : acc_identity(kind);
+        for (uint32_t b = b0; b < b1; ++b) v = combine(kind, v, part[size_t(b) * W + i]);
         out[o] = v;
     }
 }
@@ -163,7 +306,7 @@ __global__ void merge_partials_kernel(const uint64_t* part, uint32_t nblocks, ui
 void merge_partials(qe_gpu_ctx* ctx, const uint64_t* part, uint32_t nblocks, uint64_t* st,
                     uint64_t S, const AggProg& p, uint64_t out_S, uint64_t out_lo, bool accumulate) {
     const int acc = accumulate ? 1 : 0;
-    const uint64_t W = state_words(S, p);
+    const uint64_t W = merge_words(S, p);
     if (out_S == 0) out_S = S;
     constexpr uint32_t kChunk = 16;
     if (nblocks > 2 * kChunk) {
@@ -239,11 +382,16 @@ __device__ inline uint32_t emit_row(const uint64_t* st, uint64_t S, const AggPro
             out = cnt;
         } else if (cnt == 0) {
             valid = false;  // no non-NULL input: SUM/MIN/MAX/AVG are NULL
+        } else if (k == K_SUM_F) {
+            const int j = p.fhi_of[p.acc_of[a]];
+            double sum = fixed_sum_value(acc, static_cast<int64_t>(st[state_fhi(S, p, j) + s]), p.fsc[p.acc_of[a]],
+                                         st[state_flags(S, p, j) + s]);
+            const uint64_t ob = st[state_osum(S, p, j) + s];
+            if (ob) sum += __longlong_as_double(static_cast<long long>(ob));
+            if (f == QE_AGG_AVG) sum /= static_cast<double>(cnt);
+            memcpy(&out, &sum, 8);
         } else if (f == QE_AGG_AVG) {
-            double sum;
-            if (k == K_SUM_F) memcpy(&sum, &acc, 8);
-            else sum = static_cast<double>(static_cast<int64_t>(acc));
-            const double avg = sum / static_cast<double>(cnt);
+            const double avg = static_cast<double>(static_cast<int64_t>(acc)) / static_cast<double>(cnt);
             memcpy(&out, &avg, 8);
         } else if (k == K_MIN_F || k == K_MAX_F) {
             const double d = f64_from_order_key(acc);
@@ -358,11 +506,14 @@ static qe_dev_table* emit_groups_small(qe_gpu_ctx* ctx, const uint64_t* st, uint
         emit_rows_kernel<<<(S + 255) / 256, 256, 0, ctx->stream>>>(st, S, p, ke, slot_of.as<uint32_t>(), oc, vw,
                                                                    res.as<unsigned long long>());
         QE_HIP(hipGetLastError());
-        uint64_t* h = static_cast<uint64_t*>(pinned(ctx, size_t(2 + ncol) * 8));
+        uint64_t* h = static_cast<uint64_t*>(pinned(ctx, size_t(3 + ncol) * 8));
+        h[2 + ncol] = 0;
         QE_HIP(hipMemcpyAsync(h, res.p, size_t(1 + ncol) * 8, hipMemcpyDeviceToHost, ctx->stream));
         if (flag) QE_HIP(hipMemcpyAsync(h + 1 + ncol, flag, 4, hipMemcpyDeviceToHost, ctx->stream));
+        if (p.fovf) QE_HIP(hipMemcpyAsync(h + 2 + ncol, p.fovf, 4, hipMemcpyDeviceToHost, ctx->stream));
         QE_HIP(hipStreamSynchronize(ctx->stream));
         if (flag_out) *flag_out = flag ? *reinterpret_cast<const unsigned int*>(h + 1 + ncol) : 0u;
+        if (*reinterpret_cast<const unsigned int*>(h + 2 + ncol)) throw ScaleRetry();
         const uint64_t G = h[0];
         qe_dev_table* t = new_table(G);
         for (int c = 0; c < ncol; ++c) {
@@ -384,9 +535,11 @@ static qe_dev_table* emit_groups_small(qe_gpu_ctx* ctx, const uint64_t* st, uint
 
 qe_dev_table* emit_groups(qe_gpu_ctx* ctx, const uint64_t* st, uint64_t S, const AggProg& p,
                           const KeyEmit& ke, bool all_slots, const unsigned int* flag, unsigned int* flag_out) {
+    fold_outliers(ctx, p, const_cast<uint64_t*>(st), S);
     if (S > 0 && S <= kEmitSmall)
         return emit_groups_small(ctx, st, static_cast<uint32_t>(S), p, ke, all_slots, flag, flag_out);
     if (flag_out) *flag_out = flag ? read_scalar(ctx, flag) : 0u;
+    if (p.fovf && read_scalar(ctx, p.fovf)) throw ScaleRetry();
     Scratch flags(ctx, S * 4), offs(ctx, S * 8);
     group_flags_kernel<<<grid_for(S, 256, 8192), 256, 0, ctx->stream>>>(st, S, flags.as<uint32_t>(),
                                                                          all_slots ? 1 : 0);
@@ -468,9 +621,10 @@ __global__ void agg_global_kernel(uint64_t n, SlotFn slot_of, uint64_t S, AggPro
     }
 }
 
+// F64 scales (prepare_f64) are set here, where the rows one partial folds are known.
 template <class SlotFn>
-static void run_slots(qe_gpu_ctx* ctx, uint64_t n, SlotFn fn, uint32_t S, const AggProg& p,
-                      uint64_t* st, const char* timer) {
+static void run_slots(qe_gpu_ctx* ctx, uint64_t n, SlotFn fn, uint32_t S, AggProg& p, uint64_t* st,
+                      const char* timer, bool exact, Scratch& fbuf) {
     const size_t lb = lds_bytes(S, p);
     if (lb <= kLdsBudget) {
         const unsigned block = 512;
@@ -478,6 +632,9 @@ static void run_slots(qe_gpu_ctx* ctx, uint64_t n, SlotFn fn, uint32_t S, const 
         unsigned blocks = static_cast<unsigned>(std::min<uint64_t>(
             uint64_t(ctx->num_cus) * std::min(per_cu, 4u), (n + block - 1) / block));
         if (blocks == 0) blocks = 1;
+        // grid-stride rows: at most ceil(n / (blocks * 512)) per thread
+        prepare_f64(ctx, p, st, S, n, ((n + uint64_t(blocks) * block - 1) / (uint64_t(blocks) * block)) * block,
+                    exact, fbuf);
         Scratch part(ctx, size_t(blocks) * part_words(S, p) * 8);
         {
             TimedRegion tr(ctx, timer);
@@ -487,6 +644,7 @@ static void run_slots(qe_gpu_ctx* ctx, uint64_t n, SlotFn fn, uint32_t S, const 
         merge_partials(ctx, part.as<uint64_t>(), blocks, st, S, p);
     } else {
         init_state(ctx, st, S, p);
+        prepare_f64(ctx, p, st, S, n, n, exact, fbuf);
         TimedRegion tr(ctx, timer);
         agg_global_kernel<SlotFn><<<grid_for(n, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(n, fn, S, p, st);
         QE_HIP(hipGetLastError());
@@ -544,64 +702,71 @@ qe_gpu_status qe_gpu::aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* 
             ins[a] = view(cols[aggs[a].input_col]);
             types[a] = cols[aggs[a].input_col]->type;
         }
-        const AggProg p = make_prog(aggs, naggs, ins, types);
-
+        const AggProg p0 = make_prog(aggs, naggs, ins, types);
         KeyEmit ke{};
         ke.nkeys = static_cast<int>(nkeys);
         for (uint32_t k = 0; k < nkeys; ++k) ke.types[k] = cols[key_cols[k]]->type;
-
-        if (nkeys == 0) {
-            // Global aggregate: exactly one output row (ops/aggregate.h:20-41).
-            Scratch st(ctx, state_words(1, p) * 8);
-            run_slots(ctx, n, GlobalSlot{}, 1, p, st.as<uint64_t>(), "agg_global");
-            ke.mode = 0;
-            ke.null_slot = kNoSlot;
-            *out = emit_groups(ctx, st.as<uint64_t>(), 1, p, ke, true);
-            return QE_GPU_OK;
-        }
-
-        const qe_dev_col* k0 = cols[key_cols[0]];
-        if (nkeys == 1 && is_int_like(k0->type)) {
-            const MinMax mm = column_minmax(ctx, k0, n);
-            const uint64_t range = mm.valid ? static_cast<uint64_t>(mm.mx) - static_cast<uint64_t>(mm.mn) + 1 : 0;
-            if (range < kDirectMaxSlots && range <= 4 * n + 1024) {
-                const uint32_t S = static_cast<uint32_t>(range + 1);  // + the NULL slot
-                Scratch st(ctx, state_words(S, p) * 8);
-                DirectSlot fn{view(k0), mm.valid ? mm.mn : 0, S - 1};
-                run_slots(ctx, n, fn, S, p, st.as<uint64_t>(), "agg_direct");
+        // F64 sums scaled from a sample; a value past the sampled bound makes
+        // the emission throw ScaleRetry and the query runs again with bounds
+        // taken from every row.
+        auto run = [&](bool exact) -> qe_dev_table* {
+            AggProg p = p0;
+            Scratch fbuf;
+            if (nkeys == 0) {
+                // Global aggregate: exactly one output row (ops/aggregate.h:20-41).
+                Scratch st(ctx, state_words(1, p) * 8);
+                run_slots(ctx, n, GlobalSlot{}, 1, p, st.as<uint64_t>(), "agg_global", exact, fbuf);
                 ke.mode = 0;
-                ke.kmin = mm.valid ? mm.mn : 0;
-                ke.null_slot = S - 1;
-                *out = emit_groups(ctx, st.as<uint64_t>(), S, p, ke, false);
-                return QE_GPU_OK;
+                ke.null_slot = kNoSlot;
+                return emit_groups(ctx, st.as<uint64_t>(), 1, p, ke, true);
             }
+            const qe_dev_col* k0 = cols[key_cols[0]];
+            if (nkeys == 1 && is_int_like(k0->type)) {
+                const MinMax mm = column_minmax(ctx, k0, n);
+                const uint64_t range = mm.valid ? static_cast<uint64_t>(mm.mx) - static_cast<uint64_t>(mm.mn) + 1 : 0;
+                if (range < kDirectMaxSlots && range <= 4 * n + 1024) {
+                    const uint32_t S = static_cast<uint32_t>(range + 1);  // + the NULL slot
+                    Scratch st(ctx, state_words(S, p) * 8);
+                    DirectSlot fn{view(k0), mm.valid ? mm.mn : 0, S - 1};
+                    run_slots(ctx, n, fn, S, p, st.as<uint64_t>(), "agg_direct", exact, fbuf);
+                    ke.mode = 0;
+                    ke.kmin = mm.valid ? mm.mn : 0;
+                    ke.null_slot = S - 1;
+                    return emit_groups(ctx, st.as<uint64_t>(), S, p, ke, false);
+                }
+            }
+            // Hash path.
+            uint64_t cap = 1024;
+            while (cap < 2 * n) cap <<= 1;
+            check(cap <= (uint64_t(1) << 31), QE_GPU_UNSUPPORTED, "group-by input too large for one device table");
+            HashSlot fn{};
+            fn.hk.nkeys = static_cast<int>(nkeys);
+            for (uint32_t k = 0; k < nkeys; ++k) fn.hk.key[k] = view(cols[key_cols[k]]);
+            Scratch state(ctx, cap * 4), hash(ctx, cap * 8), words(ctx, cap * 8 * nkeys), nmask(ctx, cap * 8),
+                ovf(ctx, 8), st(ctx, state_words(cap, p) * 8);
+            memset_async(ctx, state.p, 0, cap * 4);
+            memset_async(ctx, ovf.p, 0, 8);
+            fn.t = HashTableView{state.as<uint32_t>(), hash.as<uint64_t>(), words.as<uint64_t>(),
+                                 nmask.as<uint64_t>(), cap, ovf.as<unsigned int>()};
+            init_state(ctx, st.as<uint64_t>(), cap, p);
+            prepare_f64(ctx, p, st.as<uint64_t>(), cap, n, n, exact, fbuf);
+            {
+                TimedRegion tr(ctx, "agg_hash");
+                agg_hash_kernel<<<grid_for(n, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(n, fn, cap, p,
+                                                                                               st.as<uint64_t>());
+                QE_HIP(hipGetLastError());
+            }
+            check(read_scalar(ctx, ovf.as<unsigned int>()) == 0, QE_GPU_OVERFLOW, "device group table overflow");
+            ke.mode = 1;
+            ke.words = words.as<uint64_t>();
+            ke.nullmask = nmask.as<uint64_t>();
+            return emit_groups(ctx, st.as<uint64_t>(), cap, p, ke, false);
+        };
+        try {
+            *out = run(false);
+        } catch (const ScaleRetry&) {
+            *out = run(true);
         }
-
-        // Hash path.
-        uint64_t cap = 1024;
-        while (cap < 2 * n) cap <<= 1;
-        check(cap <= (uint64_t(1) << 31), QE_GPU_UNSUPPORTED, "group-by input too large for one device table");
-        HashSlot fn{};
-        fn.hk.nkeys = static_cast<int>(nkeys);
-        for (uint32_t k = 0; k < nkeys; ++k) fn.hk.key[k] = view(cols[key_cols[k]]);
-        Scratch state(ctx, cap * 4), hash(ctx, cap * 8), words(ctx, cap * 8 * nkeys), nmask(ctx, cap * 8),
-            ovf(ctx, 8), st(ctx, state_words(cap, p) * 8);
-        memset_async(ctx, state.p, 0, cap * 4);
-        memset_async(ctx, ovf.p, 0, 8);
-        fn.t = HashTableView{state.as<uint32_t>(), hash.as<uint64_t>(), words.as<uint64_t>(),
-                             nmask.as<uint64_t>(), cap, ovf.as<unsigned int>()};
-        init_state(ctx, st.as<uint64_t>(), cap, p);
-        {
-            TimedRegion tr(ctx, "agg_hash");
-            agg_hash_kernel<<<grid_for(n, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(n, fn, cap, p,
-                                                                                           st.as<uint64_t>());
-            QE_HIP(hipGetLastError());
-        }
-        check(read_scalar(ctx, ovf.as<unsigned int>()) == 0, QE_GPU_OVERFLOW, "device group table overflow");
-        ke.mode = 1;
-        ke.words = words.as<uint64_t>();
-        ke.nullmask = nmask.as<uint64_t>();
-        *out = emit_groups(ctx, st.as<uint64_t>(), cap, p, ke, false);
     });
 }
 

@@ -466,7 +466,7 @@ __global__ void __launch_bounds__(1024) join_agg_direct_lds(const KeyT* __restri
             const uint32_t sl = static_cast<uint32_t>(g[j]) - slot_lo;
             if (g[j] == kEmpty || sl >= S) continue;
             for (int a = 0; a < p.n; ++a) vrow[a] = r0 + j;
-            lds_update(smem, S, p, sl, vrow);
+            lds_update(smem, S, p, sl, vrow, slot_lo);
         }
     }
     __syncthreads();
@@ -592,6 +592,7 @@ __global__ void __launch_bounds__(1024) join_agg_stream(const KeyT* __restrict__
     __shared__ uint32_t spill_n;
     lds_init(smem, S, p);
     if (SPILL && threadIdx.x == 0) spill_n = 0;
+    if constexpr (FAST) fast_fold_scales(ff, p);
     __syncthreads();
     constexpr MapT kEmpty = static_cast<MapT>(~MapT(0));
     constexpr int R = 2 * PAIRS;  // rows per thread per tile
@@ -655,9 +656,9 @@ __global__ void __launch_bounds__(1024) join_agg_stream(const KeyT* __restrict__
                 for (int c = 0; c < NV; ++c) bits[c] = vb[c][j];
                 if (!SPILL || in) {
                     if constexpr (FAST)
-                        lds_fold_fast<NV>(smem, S, ff, sl, bits);
+                        lds_fold_fast<NV>(smem, S, ff, p, sl, bits, slot_lo);
                     else
-                        lds_update_vals(smem, S, p, sl, sc.vcol, bits, valid[j]);
+                        lds_update_vals(smem, S, p, sl, sc.vcol, bits, valid[j], slot_lo);
                 } else {
                     spill_row<NV>(spill, &spill_n, p, static_cast<uint32_t>(gj), sc.vcol, bits, valid[j]);
                 }
@@ -688,7 +689,7 @@ __global__ void __launch_bounds__(1024) join_agg_stream(const KeyT* __restrict__
                 valid |= static_cast<uint32_t>(v) << c;
             }
             if (in)
-                lds_update_vals(smem, S, p, static_cast<uint32_t>(gg) - slot_lo, sc.vcol, bits, valid);
+                lds_update_vals(smem, S, p, static_cast<uint32_t>(gg) - slot_lo, sc.vcol, bits, valid, slot_lo);
             else
                 spill_row<NV>(spill, &spill_n, p, static_cast<uint32_t>(gg), sc.vcol, bits, valid);
         }
@@ -717,7 +718,7 @@ __global__ void __launch_bounds__(1024) spill_agg_kernel(const uint64_t* __restr
         uint64_t bits[NV > 0 ? NV : 1];
 #pragma unroll
         for (int c = 0; c < NV; ++c) bits[c] = rb[size_t(i) * (1 + NV) + 1 + c];
-        lds_update_vals(smem, S, p, sl, sc.vcol, bits, static_cast<uint32_t>(h >> 32));
+        lds_update_vals(smem, S, p, sl, sc.vcol, bits, static_cast<uint32_t>(h >> 32), slot_lo);
     }
     __syncthreads();
     lds_flush(smem, S, p, part);
@@ -1407,6 +1408,25 @@ qe_gpu_status qe_gpu_pairs_gather(qe_gpu_ctx* ctx, const qe_dev_pairs* p, uint64
 
 namespace qe_gpu {
 
+__global__ void max_run_kernel(const uint64_t* offs, uint64_t n, unsigned long long* out) {
+    unsigned long long m = 0;
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
+        m = max(m, static_cast<unsigned long long>(offs[i + 1] - offs[i]));
+    atomicMax(out, m);
+}
+
+// Longest build-row list of a non-unique join index (the largest fan-out).
+uint64_t max_list_len(qe_gpu_ctx* ctx, const JoinIndex& ix) {
+    const uint64_t ns = ix.nslots;
+    if (ix.unique || ns == 0) return 1;
+    Scratch m(ctx, 8);
+    memset_async(ctx, m.p, 0, 8);
+    max_run_kernel<<<grid_for(ns, 256, 1024), 256, 0, ctx->stream>>>(ix.offs.as<uint64_t>(), ns,
+                                                                    m.as<unsigned long long>());
+    QE_HIP(hipGetLastError());
+    return read_scalar(ctx, m.as<uint64_t>());
+}
+
 // Most LDS slot-range passes over the probe before HBM atomics are used.
 constexpr uint64_t kMaxSlotPasses = 32;
 // Fewest probe rows for which a sampled hot set may replace the passes, and the
@@ -1453,7 +1473,7 @@ qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* pro
             ins[a] = view(col_at(aggs[a].input_col));
             types[a] = col_at(aggs[a].input_col)->type;
         }
-        const AggProg p = make_prog(aggs, naggs, ins, types);
+        const AggProg p0 = make_prog(aggs, naggs, ins, types);
 
         if (!fast) {
             // General composition: pairs -> gather referenced columns -> aggregate.
@@ -1497,280 +1517,302 @@ qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* pro
         }
 
         // ---- fast path: pre-group the build side ---------------------------------
-        KeyEmit ke{};
-        ke.nkeys = static_cast<int>(ngk);
-        for (uint32_t k = 0; k < ngk; ++k) ke.types[k] = col_at(group_keys[k])->type;
-        Scratch gslot(ctx, std::max<uint64_t>(nb, 1) * 4);
-        DeviceHashTable gtab;
-        uint64_t S = 0;
-        const qe_dev_col* g0 = col_at(group_keys[0]);
-        bool gslot_ready = true;
-        // The per-row group slots of a direct group key, when a path needs the array.
-        auto ensure_gslot = [&]() {
-            if (gslot_ready) return;
-            gslot_direct_kernel<<<grid_for(nb, 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(
-                view(g0), nb, ke.kmin, static_cast<uint32_t>(ke.null_slot), gslot.as<uint32_t>());
-            QE_HIP(hipGetLastError());
-            gslot_ready = true;
-        };
-        bool gdirect = false;
-        const bool int_key = nkeys == 1 && (pk[0]->type == QE_I64 || pk[0]->type == QE_TS || pk[0]->type == QE_I32);
-        // Group-key and build-key ranges in one pass and one read-back.
-        const bool want_g = ngk == 1 && is_int_like(g0->type) && nb > 0;
-        const bool want_k = int_key && nb > 0 && np > 0;
-        MinMax g_mm{}, k_mm{};
-        {
-            const qe_dev_col* mc[2];
-            uint32_t nm = 0;
-            if (want_g) mc[nm++] = g0;
-            if (want_k) mc[nm++] = bk[0];
-            MinMax r[2];
-            if (nm) column_minmax_n(ctx, mc, nm, nb, r);
-            if (want_g) g_mm = r[0];
-            if (want_k) k_mm = r[want_g ? 1 : 0];
-        }
-        if (want_g) {
-            const MinMax mm = g_mm;
-            const uint64_t R = mm.valid ? static_cast<uint64_t>(mm.mx) - static_cast<uint64_t>(mm.mn) + 1 : 0;
-            if (R <= std::max<uint64_t>(2 * nb, 1024) && R < (uint64_t(1) << 30)) {
-                gdirect = true;
-                S = R + 1;
-                ke.mode = 0;
-                ke.kmin = mm.valid ? mm.mn : 0;
-                ke.null_slot = S - 1;
-                gslot_ready = false;  // computed only by the paths that read the array
+        // F64 sums scaled from a sample; a value past the sampled bound makes
+        // the emission throw ScaleRetry and the query runs again with bounds
+        // taken from every row.
+        auto run = [&](bool exact) -> qe_dev_table* {
+        AggProg p = p0;
+            KeyEmit ke{};
+            ke.nkeys = static_cast<int>(ngk);
+            for (uint32_t k = 0; k < ngk; ++k) ke.types[k] = col_at(group_keys[k])->type;
+            Scratch gslot(ctx, std::max<uint64_t>(nb, 1) * 4);
+            DeviceHashTable gtab;
+            uint64_t S = 0;
+            const qe_dev_col* g0 = col_at(group_keys[0]);
+            bool gslot_ready = true;
+            // The per-row group slots of a direct group key, when a path needs the array.
+            auto ensure_gslot = [&]() {
+                if (gslot_ready) return;
+                gslot_direct_kernel<<<grid_for(nb, 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(
+                    view(g0), nb, ke.kmin, static_cast<uint32_t>(ke.null_slot), gslot.as<uint32_t>());
+                QE_HIP(hipGetLastError());
+                gslot_ready = true;
+            };
+            bool gdirect = false;
+            const bool int_key = nkeys == 1 && (pk[0]->type == QE_I64 || pk[0]->type == QE_TS || pk[0]->type == QE_I32);
+            // Group-key and build-key ranges in one pass and one read-back.
+            const bool want_g = ngk == 1 && is_int_like(g0->type) && nb > 0;
+            const bool want_k = int_key && nb > 0 && np > 0;
+            MinMax g_mm{}, k_mm{};
+            {
+                const qe_dev_col* mc[2];
+                uint32_t nm = 0;
+                if (want_g) mc[nm++] = g0;
+                if (want_k) mc[nm++] = bk[0];
+                MinMax r[2];
+                if (nm) column_minmax_n(ctx, mc, nm, nb, r);
+                if (want_g) g_mm = r[0];
+                if (want_k) k_mm = r[want_g ? 1 : 0];
             }
-        }
-        if (!gdirect) {
-            HashKeys hk{};
-            hk.nkeys = static_cast<int>(ngk);
-            for (uint32_t k = 0; k < ngk; ++k) hk.key[k] = view(col_at(group_keys[k]));
-            gtab.create(ctx, static_cast<int>(ngk), std::max<uint64_t>(nb, 1));
-            S = gtab.v.cap;
-            if (nb)
-                gslot_hash_kernel<<<grid_for(nb, 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(hk, nb, gtab.v,
-                                                                                                 gslot.as<uint32_t>());
-            QE_HIP(hipGetLastError());
-            gtab.check_overflow(ctx);
-            ke.mode = 1;
-            ke.words = gtab.v.words;
-            ke.nullmask = gtab.v.nullmask;
-        }
+            if (want_g) {
+                const MinMax mm = g_mm;
+                const uint64_t R = mm.valid ? static_cast<uint64_t>(mm.mx) - static_cast<uint64_t>(mm.mn) + 1 : 0;
+                if (R <= std::max<uint64_t>(2 * nb, 1024) && R < (uint64_t(1) << 30)) {
+                    gdirect = true;
+                    S = R + 1;
+                    ke.mode = 0;
+                    ke.kmin = mm.valid ? mm.mn : 0;
+                    ke.null_slot = S - 1;
+                    gslot_ready = false;  // computed only by the paths that read the array
+                }
+            }
+            if (!gdirect) {
+                HashKeys hk{};
+                hk.nkeys = static_cast<int>(ngk);
+                for (uint32_t k = 0; k < ngk; ++k) hk.key[k] = view(col_at(group_keys[k]));
+                gtab.create(ctx, static_cast<int>(ngk), std::max<uint64_t>(nb, 1));
+                S = gtab.v.cap;
+                if (nb)
+                    gslot_hash_kernel<<<grid_for(nb, 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(hk, nb, gtab.v,
+                                                                                                     gslot.as<uint32_t>());
+                QE_HIP(hipGetLastError());
+                gtab.check_overflow(ctx);
+                ke.mode = 1;
+                ke.words = gtab.v.words;
+                ke.nullmask = gtab.v.nullmask;
+            }
 
-        Scratch st(ctx, state_words(S, p) * 8);
-        const size_t lb = lds_bytes(static_cast<uint32_t>(std::min<uint64_t>(S, UINT32_MAX)), p);
-        const bool use_lds = S < (uint64_t(1) << 20) && lb <= kLdsBudget;
-        // A slot space larger than LDS: LDS partials for one slot range per
-        // pass over the probe (each pass streams the probe columns again),
-        // instead of HBM atomics that serialise on the heavy groups.
-        const uint64_t pass_slots = std::max<uint64_t>(1, kLdsBudget / lds_bytes(1, p));
-        const uint64_t npasses = (S + pass_slots - 1) / pass_slots;
-        const bool multipass = !use_lds && npasses <= kMaxSlotPasses;
-        const unsigned lds_blocks = static_cast<unsigned>(
-            std::max<uint64_t>(1, std::min<uint64_t>(ctx->num_cus, (np + 1023) / 1024)));
-        const bool small = S < 0xFFFF;
-        const uint64_t* kv = pk[0]->all_valid ? nullptr : pk[0]->validity;
+            Scratch st(ctx, state_words(S, p) * 8);
+            const size_t lb = lds_bytes(static_cast<uint32_t>(std::min<uint64_t>(S, UINT32_MAX)), p);
+            const bool use_lds = S < (uint64_t(1) << 20) && lb <= kLdsBudget;
+            // A slot space larger than LDS: LDS partials for one slot range per
+            // pass over the probe (each pass streams the probe columns again),
+            // instead of HBM atomics that serialise on the heavy groups.
+            const uint64_t pass_slots = std::max<uint64_t>(1, kLdsBudget / lds_bytes(1, p));
+            const uint64_t npasses = (S + pass_slots - 1) / pass_slots;
+            const bool multipass = !use_lds && npasses <= kMaxSlotPasses;
+            const unsigned lds_blocks = static_cast<unsigned>(
+                std::max<uint64_t>(1, std::min<uint64_t>(ctx->num_cus, (np + 1023) / 1024)));
+            // Exact F64 sums: a block folds at most its share of the probe plus
+            // two tiles (and the odd last row) when every probe row is summed at
+            // most once (a unique build key); the row-list path below rescales
+            // for its fan-out.
+            Scratch fbuf;
+            const uint64_t fold_rows = (np + lds_blocks - 1) / lds_blocks + 2 * 8192 + 1;
+            prepare_f64(ctx, p, st.as<uint64_t>(), S, np, fold_rows, exact, fbuf);
+            const bool small = S < 0xFFFF;
+            const uint64_t* kv = pk[0]->all_valid ? nullptr : pk[0]->validity;
 
-        // Probe over a direct unique map (gmap[key - kmin] = group slot).
-        auto probe_direct = [&](const Scratch& gmap, int64_t kmin, uint64_t range) {
-            auto launch = [&](auto key_tag, auto map_tag) {
-                using KeyT = decltype(key_tag);
-                using MapT = decltype(map_tag);
-                const KeyT* keys = static_cast<const KeyT*>(pk[0]->data);
-                const MapT* gm = static_cast<const MapT*>(gmap.p);
-                StreamCols sc{};
-                int nv = 0;
-                const bool streamable = (use_lds || multipass) && stream_cols(p, keys, sizeof(KeyT), sc, nv);
-                // More slots than LDS holds: when a sample of the probe puts at
-                // least 7/8 of the rows in at most an LDS range of slots, and no
-                // slot outside them carries more than 1/4096 of the rows or
-                // about 2^16 rows, those slots are relabelled first (the map is
-                // rewritten through the relabelling) and one pass keeps them in
-                // LDS while the other rows go to per-block spill lists; the lists
-                // are then folded one LDS range at a time, and the relabelled
-                // state returns to slot order.
-                bool spill = false;
-                if (multipass && streamable && np >= kSpillMinRows) {
-                    const uint32_t S32 = static_cast<uint32_t>(S), W = static_cast<uint32_t>(pass_slots);
-                    Scratch perm(ctx, size_t(S) * 4);
-                    const HotSet hs = hot_set<KeyT, MapT>(ctx, keys, kv, np, gm, kmin, range, S32, W,
-                                                          perm.as<uint32_t>());
-                    // Spill lists sized for a quarter of each block's rows (the
-                    // sample put at most an eighth outside the LDS range).
-                    const uint64_t per_block = (np + lds_blocks - 1) / lds_blocks;
-                    const uint32_t cap = ctx->tune_join_spill_cap
-                                             ? ctx->tune_join_spill_cap
-                                             : static_cast<uint32_t>(std::min<uint64_t>(per_block / 4 + 1024, 1u << 30));
-                    const size_t rec_bytes = size_t(lds_blocks) * cap * (1 + nv) * 8;
-                    spill = hs.total >= 1024 && uint64_t(hs.total - hs.inside) * 8 <= hs.total &&
-                            (uint64_t(hs.cold_max) * 4096 <= hs.total ||
-                             uint64_t(hs.cold_max) * hs.stride <= (uint64_t(1) << 16)) &&
-                            rec_bytes <= kSpillMaxBytes;
-                    if (spill) {
-                        Scratch gm2(ctx, size_t(range) * sizeof(MapT));
-                        remap_slots_kernel<MapT><<<grid_for(range, 256, 4096), 256, 0, ctx->stream>>>(
-                            gm, range, perm.as<uint32_t>(), gm2.as<MapT>());
-                        QE_HIP(hipGetLastError());
-                        Scratch st2(ctx, state_words(S, p) * 8);
-                        Scratch part(ctx, size_t(lds_blocks) * part_words(W, p) * 8);
-                        Scratch rec(ctx, rec_bytes);
-                        Scratch rcount(ctx, size_t(lds_blocks) * 4);
-                        init_state(ctx, st2.as<uint64_t>(), S, p);
-                        Spill sp;
-                        sp.st = st2.as<uint64_t>();
-                        sp.S = S;
-                        sp.rec = rec.as<uint64_t>();
-                        sp.cap = cap;
-                        sp.count = rcount.as<uint32_t>();
-                        {
-                            TimedRegion tr(ctx, "join_agg_probe");
-                            launch_stream<KeyT, MapT>(ctx, ctx->tune_join_pairs, nv, lds_blocks, lds_bytes(W, p), keys,
-                                                      kv, np, gm2.as<MapT>(), kmin, range, W, 0, p, sc,
-                                                      part.as<uint64_t>(), sp);
-                        }
-                        merge_partials(ctx, part.as<uint64_t>(), lds_blocks, st2.as<uint64_t>(), W, p, S, 0, true);
-                        for (uint64_t lo = W; lo < S; lo += W) {
-                            const uint32_t Sp = static_cast<uint32_t>(std::min<uint64_t>(W, S - lo));
+            // Probe over a direct unique map (gmap[key - kmin] = group slot).
+            auto probe_direct = [&](const Scratch& gmap, int64_t kmin, uint64_t range) {
+                auto launch = [&](auto key_tag, auto map_tag) {
+                    using KeyT = decltype(key_tag);
+                    using MapT = decltype(map_tag);
+                    const KeyT* keys = static_cast<const KeyT*>(pk[0]->data);
+                    const MapT* gm = static_cast<const MapT*>(gmap.p);
+                    StreamCols sc{};
+                    int nv = 0;
+                    const bool streamable = (use_lds || multipass) && stream_cols(p, keys, sizeof(KeyT), sc, nv);
+                    // More slots than LDS holds: when a sample of the probe puts at
+                    // least 7/8 of the rows in at most an LDS range of slots, and no
+                    // slot outside them carries more than 1/4096 of the rows or
+                    // about 2^16 rows, those slots are relabelled first (the map is
+                    // rewritten through the relabelling) and one pass keeps them in
+                    // LDS while the other rows go to per-block spill lists; the lists
+                    // are then folded one LDS range at a time, and the relabelled
+                    // state returns to slot order.
+                    bool spill = false;
+                    if (multipass && streamable && np >= kSpillMinRows) {
+                        const uint32_t S32 = static_cast<uint32_t>(S), W = static_cast<uint32_t>(pass_slots);
+                        Scratch perm(ctx, size_t(S) * 4);
+                        const HotSet hs = hot_set<KeyT, MapT>(ctx, keys, kv, np, gm, kmin, range, S32, W,
+                                                              perm.as<uint32_t>());
+                        // Spill lists sized for a quarter of each block's rows (the
+                        // sample put at most an eighth outside the LDS range).
+                        const uint64_t per_block = (np + lds_blocks - 1) / lds_blocks;
+                        const uint32_t cap = ctx->tune_join_spill_cap
+                                                 ? ctx->tune_join_spill_cap
+                                                 : static_cast<uint32_t>(std::min<uint64_t>(per_block / 4 + 1024, 1u << 30));
+                        const size_t rec_bytes = size_t(lds_blocks) * cap * (1 + nv) * 8;
+                        spill = hs.total >= 1024 && uint64_t(hs.total - hs.inside) * 8 <= hs.total &&
+                                (uint64_t(hs.cold_max) * 4096 <= hs.total ||
+                                 uint64_t(hs.cold_max) * hs.stride <= (uint64_t(1) << 16)) &&
+                                rec_bytes <= kSpillMaxBytes;
+                        if (spill) {
+                            Scratch gm2(ctx, size_t(range) * sizeof(MapT));
+                            remap_slots_kernel<MapT><<<grid_for(range, 256, 4096), 256, 0, ctx->stream>>>(
+                                gm, range, perm.as<uint32_t>(), gm2.as<MapT>());
+                            QE_HIP(hipGetLastError());
+                            Scratch st2(ctx, state_words(S, p) * 8);
+                            Scratch part(ctx, size_t(lds_blocks) * part_words(W, p) * 8);
+                            Scratch rec(ctx, rec_bytes);
+                            Scratch rcount(ctx, size_t(lds_blocks) * 4);
+                            init_state(ctx, st2.as<uint64_t>(), S, p);
+                            // The relabelled state's own NaN/inf flags (moved with it).
+                            AggProg p2 = p;
+                            p2.fflag = st2.as<uint64_t>() + state_flags(S, p, 0);
+                            Spill sp;
+                            sp.st = st2.as<uint64_t>();
+                            sp.S = S;
+                            sp.rec = rec.as<uint64_t>();
+                            sp.cap = cap;
+                            sp.count = rcount.as<uint32_t>();
                             {
-                                TimedRegion tr(ctx, "join_agg_spill");
-                                launch_spill_agg(ctx, nv, lds_blocks, lds_bytes(Sp, p), rec.as<uint64_t>(),
-                                                 rcount.as<uint32_t>(), cap, static_cast<uint32_t>(lo), Sp, p, sc,
-                                                 part.as<uint64_t>());
+                                TimedRegion tr(ctx, "join_agg_probe");
+                                launch_stream<KeyT, MapT>(ctx, ctx->tune_join_pairs, nv, lds_blocks, lds_bytes(W, p), keys,
+                                                          kv, np, gm2.as<MapT>(), kmin, range, W, 0, p2, sc,
+                                                          part.as<uint64_t>(), sp);
                             }
-                            merge_partials(ctx, part.as<uint64_t>(), lds_blocks, st2.as<uint64_t>(), Sp, p, S, lo,
-                                           true);
+                            merge_partials(ctx, part.as<uint64_t>(), lds_blocks, st2.as<uint64_t>(), W, p, S, 0, true);
+                            for (uint64_t lo = W; lo < S; lo += W) {
+                                const uint32_t Sp = static_cast<uint32_t>(std::min<uint64_t>(W, S - lo));
+                                {
+                                    TimedRegion tr(ctx, "join_agg_spill");
+                                    launch_spill_agg(ctx, nv, lds_blocks, lds_bytes(Sp, p), rec.as<uint64_t>(),
+                                                     rcount.as<uint32_t>(), cap, static_cast<uint32_t>(lo), Sp, p2, sc,
+                                                     part.as<uint64_t>());
+                                }
+                                merge_partials(ctx, part.as<uint64_t>(), lds_blocks, st2.as<uint64_t>(), Sp, p, S, lo,
+                                               true);
+                            }
+                            fold_outliers(ctx, p2, st2.as<uint64_t>(), S);  // relabelled slots: before the move
+                            const uint32_t fields = static_cast<uint32_t>(state_words(1, p));
+                            unpermute_state_kernel<<<grid_for(S * fields, 256, 4096), 256, 0, ctx->stream>>>(
+                                st2.as<uint64_t>(), S, fields, perm.as<uint32_t>(), st.as<uint64_t>());
+                            QE_HIP(hipGetLastError());
                         }
-                        const uint32_t fields = static_cast<uint32_t>(1 + p.nacc + p.ncnt);
-                        unpermute_state_kernel<<<grid_for(S * fields, 256, 4096), 256, 0, ctx->stream>>>(
-                            st2.as<uint64_t>(), S, fields, perm.as<uint32_t>(), st.as<uint64_t>());
+                    }
+                    if (spill) return;
+                    if (use_lds || multipass) {
+                        const uint64_t per = use_lds ? S : pass_slots;
+                        Scratch part(ctx, size_t(lds_blocks) * part_words(static_cast<uint32_t>(per), p) * 8);
+                        for (uint64_t lo = 0; lo < S; lo += per) {
+                            const uint32_t Sp = static_cast<uint32_t>(std::min<uint64_t>(per, S - lo));
+                            const size_t lbp = lds_bytes(Sp, p);
+                            {
+                                TimedRegion tr(ctx, "join_agg_probe");
+                                if (streamable) {
+                                    launch_stream<KeyT, MapT>(ctx, ctx->tune_join_pairs, nv, lds_blocks, lbp, keys, kv, np,
+                                                              gm, kmin, range, Sp, static_cast<uint32_t>(lo), p, sc,
+                                                              part.as<uint64_t>());
+                                } else {
+                                    join_agg_direct_lds<KeyT, MapT><<<lds_blocks, 1024, lbp, ctx->stream>>>(
+                                        keys, kv, np, gm, kmin, range, Sp, static_cast<uint32_t>(lo), p,
+                                        part.as<uint64_t>());
+                                    QE_HIP(hipGetLastError());
+                                }
+                            }
+                            merge_partials(ctx, part.as<uint64_t>(), lds_blocks, st.as<uint64_t>(), Sp, p, S, lo);
+                        }
+                    } else {
+                        init_state(ctx, st.as<uint64_t>(), S, p);
+                        TimedRegion tr(ctx, "join_agg_probe");
+                        join_agg_direct_global<KeyT, MapT><<<grid_for(np, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(
+                            keys, kv, np, gm, kmin, range, S, p, st.as<uint64_t>());
                         QE_HIP(hipGetLastError());
+                    }
+                };
+                if (pk[0]->type == QE_I32) {
+                    if (small) launch(int32_t(0), uint16_t(0));
+                    else launch(int32_t(0), uint32_t(0));
+                } else {
+                    if (small) launch(int64_t(0), uint16_t(0));
+                    else launch(int64_t(0), uint32_t(0));
+                }
+            };
+
+            // Dense unique integer keys: partitioned map build, then the streaming probe.
+            bool done = false;
+            if (want_k) {
+                const MinMax mm = k_mm;
+                if (mm.valid == 0) {  // every build key NULL: nothing matches
+                    init_state(ctx, st.as<uint64_t>(), S, p);
+                    done = true;
+                } else {
+                    const uint64_t R = static_cast<uint64_t>(mm.mx) - static_cast<uint64_t>(mm.mn) + 1;
+                    if (R <= std::max<uint64_t>(4 * nb, uint64_t(1) << 16) && R < (uint64_t(1) << 31)) {
+                        Scratch gmap, dup;
+                        MapSlots ms;
+                        if (small && !gslot_ready) {  // the scatter reads the group column itself
+                            ms.g = g0;
+                            ms.gmin = ke.kmin;
+                            ms.null_slot = static_cast<uint32_t>(ke.null_slot);
+                        } else {
+                            ensure_gslot();
+                            ms.gslot = gslot.as<uint32_t>();
+                        }
+                        const MapBuild mbld = small ? build_direct_map<uint16_t>(ctx, bk[0], nb, mm.valid, mm.mn, R, ms,
+                                                                                 gmap, dup)
+                                                    : build_direct_map<uint32_t>(ctx, bk[0], nb, mm.valid, mm.mn, R, ms,
+                                                                                 gmap, dup);
+                        if (mbld == MapBuild::kOk) {
+                            probe_direct(gmap, mm.mn, R);
+                            // Repeated build keys make the map wrong: then the
+                            // groups are discarded and the row-list path runs. The
+                            // flag comes back with the emission's counts.
+                            unsigned int dupv = 0;
+                            qe_dev_table* t = emit_groups(ctx, st.as<uint64_t>(), S, p, ke, false,
+                                                          dup.as<unsigned int>(), &dupv);
+                            if (dupv == 0) return t;
+                            qe_gpu_table_free(ctx, t);
+                        }
                     }
                 }
-                if (spill) return;
-                if (use_lds || multipass) {
-                    const uint64_t per = use_lds ? S : pass_slots;
-                    Scratch part(ctx, size_t(lds_blocks) * part_words(static_cast<uint32_t>(per), p) * 8);
-                    for (uint64_t lo = 0; lo < S; lo += per) {
-                        const uint32_t Sp = static_cast<uint32_t>(std::min<uint64_t>(per, S - lo));
-                        const size_t lbp = lds_bytes(Sp, p);
-                        {
-                            TimedRegion tr(ctx, "join_agg_probe");
-                            if (streamable) {
-                                launch_stream<KeyT, MapT>(ctx, ctx->tune_join_pairs, nv, lds_blocks, lbp, keys, kv, np,
-                                                          gm, kmin, range, Sp, static_cast<uint32_t>(lo), p, sc,
-                                                          part.as<uint64_t>());
-                            } else {
-                                join_agg_direct_lds<KeyT, MapT><<<lds_blocks, 1024, lbp, ctx->stream>>>(
-                                    keys, kv, np, gm, kmin, range, Sp, static_cast<uint32_t>(lo), p,
-                                    part.as<uint64_t>());
-                                QE_HIP(hipGetLastError());
-                            }
-                        }
-                        merge_partials(ctx, part.as<uint64_t>(), lds_blocks, st.as<uint64_t>(), Sp, p, S, lo);
+            }
+
+            JoinIndex ix;
+            if (done) {
+            } else if (nb > 0) {
+                build_index(ctx, ix, bk.data(), nkeys, nb);
+            } else {
+                ix.empty = true;
+            }
+            if (!done) ensure_gslot();
+            if (done) {
+            } else if (ix.empty || np == 0) {
+                init_state(ctx, st.as<uint64_t>(), S, p);
+            } else if (ix.mode == 0 && ix.unique && int_key) {
+                // Direct unique index too wide for the partitioned build.
+                const size_t mb = ix.range * (small ? 2 : 4);
+                Scratch gmap(ctx, mb);
+                memset_async(ctx, gmap.p, 0xFF, mb);
+                if (small)
+                    gmap_fill_kernel<uint16_t><<<grid_for(nb, 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(
+                        view(bk[0]), nb, ix.kmin, gslot.as<uint32_t>(), gmap.as<uint16_t>());
+                else
+                    gmap_fill_kernel<uint32_t><<<grid_for(nb, 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(
+                        view(bk[0]), nb, ix.kmin, gslot.as<uint32_t>(), gmap.as<uint32_t>());
+                QE_HIP(hipGetLastError());
+                probe_direct(gmap, ix.kmin, ix.range);
+            } else {
+                if (!ix.unique)  // a probe row is summed once per build row of its key
+                    prepare_f64(ctx, p, st.as<uint64_t>(), S, np,
+                                fold_rows * std::max<uint64_t>(1, max_list_len(ctx, ix)), exact, fbuf);
+                ProbeAggLists L{ix.view(pk.data(), static_cast<int>(nkeys)), gslot.as<uint32_t>()};
+                if (use_lds) {
+                    Scratch part(ctx, size_t(lds_blocks) * part_words(static_cast<uint32_t>(S), p) * 8);
+                    {
+                        TimedRegion tr(ctx, "join_agg_probe");
+                        join_agg_lists_lds<<<lds_blocks, 1024, lb, ctx->stream>>>(L, np, static_cast<uint32_t>(S), p,
+                                                                                  part.as<uint64_t>());
+                        QE_HIP(hipGetLastError());
                     }
+                    merge_partials(ctx, part.as<uint64_t>(), lds_blocks, st.as<uint64_t>(), S, p);
                 } else {
                     init_state(ctx, st.as<uint64_t>(), S, p);
                     TimedRegion tr(ctx, "join_agg_probe");
-                    join_agg_direct_global<KeyT, MapT><<<grid_for(np, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(
-                        keys, kv, np, gm, kmin, range, S, p, st.as<uint64_t>());
+                    join_agg_lists_global<<<grid_for(np, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(
+                        L, np, S, p, st.as<uint64_t>());
                     QE_HIP(hipGetLastError());
                 }
-            };
-            if (pk[0]->type == QE_I32) {
-                if (small) launch(int32_t(0), uint16_t(0));
-                else launch(int32_t(0), uint32_t(0));
-            } else {
-                if (small) launch(int64_t(0), uint16_t(0));
-                else launch(int64_t(0), uint32_t(0));
             }
+            return emit_groups(ctx, st.as<uint64_t>(), S, p, ke, false);
         };
-
-        // Dense unique integer keys: partitioned map build, then the streaming probe.
-        bool done = false;
-        if (want_k) {
-            const MinMax mm = k_mm;
-            if (mm.valid == 0) {  // every build key NULL: nothing matches
-                init_state(ctx, st.as<uint64_t>(), S, p);
-                done = true;
-            } else {
-                const uint64_t R = static_cast<uint64_t>(mm.mx) - static_cast<uint64_t>(mm.mn) + 1;
-                if (R <= std::max<uint64_t>(4 * nb, uint64_t(1) << 16) && R < (uint64_t(1) << 31)) {
-                    Scratch gmap, dup;
-                    MapSlots ms;
-                    if (small && !gslot_ready) {  // the scatter reads the group column itself
-                        ms.g = g0;
-                        ms.gmin = ke.kmin;
-                        ms.null_slot = static_cast<uint32_t>(ke.null_slot);
-                    } else {
-                        ensure_gslot();
-                        ms.gslot = gslot.as<uint32_t>();
-                    }
-                    const MapBuild mbld = small ? build_direct_map<uint16_t>(ctx, bk[0], nb, mm.valid, mm.mn, R, ms,
-                                                                             gmap, dup)
-                                                : build_direct_map<uint32_t>(ctx, bk[0], nb, mm.valid, mm.mn, R, ms,
-                                                                             gmap, dup);
-                    if (mbld == MapBuild::kOk) {
-                        probe_direct(gmap, mm.mn, R);
-                        // Repeated build keys make the map wrong: then the
-                        // groups are discarded and the row-list path runs. The
-                        // flag comes back with the emission's counts.
-                        unsigned int dupv = 0;
-                        qe_dev_table* t = emit_groups(ctx, st.as<uint64_t>(), S, p, ke, false,
-                                                      dup.as<unsigned int>(), &dupv);
-                        if (dupv == 0) {
-                            *out = t;
-                            return QE_GPU_OK;
-                        }
-                        qe_gpu_table_free(ctx, t);
-                    }
-                }
-            }
+        try {
+            *out = run(false);
+        } catch (const ScaleRetry&) {
+            *out = run(true);
         }
-
-        JoinIndex ix;
-        if (done) {
-        } else if (nb > 0) {
-            build_index(ctx, ix, bk.data(), nkeys, nb);
-        } else {
-            ix.empty = true;
-        }
-        if (!done) ensure_gslot();
-        if (done) {
-        } else if (ix.empty || np == 0) {
-            init_state(ctx, st.as<uint64_t>(), S, p);
-        } else if (ix.mode == 0 && ix.unique && int_key) {
-            // Direct unique index too wide for the partitioned build.
-            const size_t mb = ix.range * (small ? 2 : 4);
-            Scratch gmap(ctx, mb);
-            memset_async(ctx, gmap.p, 0xFF, mb);
-            if (small)
-                gmap_fill_kernel<uint16_t><<<grid_for(nb, 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(
-                    view(bk[0]), nb, ix.kmin, gslot.as<uint32_t>(), gmap.as<uint16_t>());
-            else
-                gmap_fill_kernel<uint32_t><<<grid_for(nb, 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(
-                    view(bk[0]), nb, ix.kmin, gslot.as<uint32_t>(), gmap.as<uint32_t>());
-            QE_HIP(hipGetLastError());
-            probe_direct(gmap, ix.kmin, ix.range);
-        } else {
-            ProbeAggLists L{ix.view(pk.data(), static_cast<int>(nkeys)), gslot.as<uint32_t>()};
-            if (use_lds) {
-                Scratch part(ctx, size_t(lds_blocks) * part_words(static_cast<uint32_t>(S), p) * 8);
-                {
-                    TimedRegion tr(ctx, "join_agg_probe");
-                    join_agg_lists_lds<<<lds_blocks, 1024, lb, ctx->stream>>>(L, np, static_cast<uint32_t>(S), p,
-                                                                              part.as<uint64_t>());
-                    QE_HIP(hipGetLastError());
-                }
-                merge_partials(ctx, part.as<uint64_t>(), lds_blocks, st.as<uint64_t>(), S, p);
-            } else {
-                init_state(ctx, st.as<uint64_t>(), S, p);
-                TimedRegion tr(ctx, "join_agg_probe");
-                join_agg_lists_global<<<grid_for(np, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(
-                    L, np, S, p, st.as<uint64_t>());
-                QE_HIP(hipGetLastError());
-            }
-        }
-        *out = emit_groups(ctx, st.as<uint64_t>(), S, p, ke, false);
     });
 }
 
