@@ -10,13 +10,17 @@
 // and a 32-bit high word), 128-bit in HBM. Integer addition is associative,
 // so the order in which rows, waves and blocks add never changes a bit. The
 // scale s of each column is computed on the device before the aggregation:
-// s = 95 - e - L, where every |x| < 2^e (e = the sampled largest exponent + 8
-// bits of headroom) and 2^L exceeds the input row count, so no partial can
-// overflow 96 bits. A value at or beyond 2^e raises a flag that comes back
-// with the emission's counts and the query reruns with e from every row.
-// NaN and +/-inf set per-slot flags (the sum becomes NaN / +inf / -inf as the
-// row-order fold would: NaN if any NaN or both infinities). The finished sum
-// is the exact integer total scaled by 2^-s and rounded once to a double. Semantics follow ops/aggregate.cpp:118-155 (scatter_row),
+// s = 63 - e, where |x| < 2^e for the column's values (e = the largest
+// exponent of a 2^16-row sample + 2 bits of headroom), so |X| < 2^63 and a
+// 96-bit partial holds 2^32 terms; values at or past 2^e (outliers) go to a
+// per-query list summed in sorted order at emission, and a full list makes
+// the query rerun with e from every row. NaN and +/-inf set per-slot flags
+// (the sum becomes NaN / +inf / -inf as the row-order fold would: NaN if any
+// NaN or both infinities). The finished sum is the exact integer total scaled
+// by 2^-s, rounded once to a double (plus the slot's outlier sum, if any).
+// Terms whose magnitude is below 2^(e - 11) are rounded to the quantum 2^-s,
+// so a group made only of values much smaller than its column's largest keeps
+// an absolute error of at most (its rows) * 2^(e - 64). Semantics follow ops/aggregate.cpp:118-155 (scatter_row),
 // ops/agg_internal.h:34-81 (NaN-greatest order, identities) and :169-229
 // (finalisation), with the result types of ops/aggregate.cpp:78-97.
 #pragma once
@@ -498,15 +502,13 @@ __device__ inline void global_update_vals(uint64_t* st, uint64_t S, const AggPro
     }
 }
 
-// Host: the exact F64 sums of a query over n_rows input rows, of which one
-// accumulator (a block's LDS partial, or an HBM slot) folds at most
-// fold_rows. `prepare_f64` launches the scale computation for every
-// F64-summed column (a sample of 2^16 valid rows, or every row when `exact`),
-// clears the flag words of `st` (state of S slots) and points the program at
-// the device scales, those flags and the overflow word (all in `buf`). No host
-// round trip: the aggregation kernels read the scales on the device.
-void prepare_f64(qe_gpu_ctx* ctx, AggProg& p, uint64_t* st, uint64_t S, uint64_t n_rows, uint64_t fold_rows,
-                 bool exact, Scratch& buf);
+// Host: the exact F64 sums of a query over n_rows input rows. `prepare_f64`
+// launches the scale computation for every F64-summed column (a sample of
+// 2^16 valid rows, or every row when `exact`), clears the flag and outlier-sum
+// words of `st` (state of S slots) and points the program at the device
+// scales, those words, the outlier list and the overflow word (all in `buf`).
+// No host round trip: the aggregation kernels read the scales on the device.
+void prepare_f64(qe_gpu_ctx* ctx, AggProg& p, uint64_t* st, uint64_t S, uint64_t n_rows, bool exact, Scratch& buf);
 // Sum the outlier list into the state's osum words (emit_groups does this; a
 // path whose state is relabelled afterwards calls it before the relabelling).
 void fold_outliers(qe_gpu_ctx* ctx, const AggProg& p, uint64_t* st, uint64_t S);

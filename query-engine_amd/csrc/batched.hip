@@ -790,7 +790,7 @@ static void build_batch_plan(qe_gpu_ctx* ctx, BatchPlan& bp, const qe_batch_quer
         // Exact F64 sums: the plan runs many times over the same columns, so
         // the scales come from every row, once, here.
         Scratch fb;
-        prepare_f64(ctx, d.p, d.st, d.S, n, n, true, fb);
+        prepare_f64(ctx, d.p, d.st, d.S, n, true, fb);
         bp.bufs.push_back(std::move(fb));
         bp.hq[i] = d;
     }

@@ -123,23 +123,22 @@ __global__ void f64_exp_kernel(F64Cols fc, uint64_t n, uint64_t stride, uint32_t
     if (threadIdx.x == 0) part[size_t(blockIdx.y) * gridDim.x + blockIdx.x] = bm;
 }
 
-// Per column: the exponent bound e (|x| < 2^e: the largest exponent seen, +8
-// bits of headroom when sampled) and the scale s = 95 - e - L, where 2^L is
-// four times above the most rows one accumulator folds (a 96-bit LDS partial
-// or, for HBM atomics, the 128-bit state, which has 32 bits more).
-__global__ void f64_scale_kernel(const uint32_t* part, uint32_t nblk, F64Cols fc, int32_t headroom, int32_t L,
-                                 int32_t* fsc) {
+// Per column: the exponent bound e (|x| < 2^e: the largest exponent seen, +2
+// bits of headroom when sampled) and the scale s = 63 - e, so every term is
+// below 2^63: a 96-bit LDS partial holds 2^32 of them, and the 64-bit low
+// word wraps into the high word only now and then (an add whose carry or
+// borrow changes nothing skips the high word).
+__global__ void f64_scale_kernel(const uint32_t* part, uint32_t nblk, F64Cols fc, int32_t headroom, int32_t* fsc) {
     const int j = threadIdx.x;
     if (j >= fc.n) return;
     uint32_t mx = 0;
     for (uint32_t b = 0; b < nblk; ++b) mx = max(mx, part[size_t(j) * nblk + b]);
     const int e = (mx == 0 ? -1022 : static_cast<int>(mx) - 1022) + headroom;
-    fsc[fc.acc[j]] = 95 - e - L;
+    fsc[fc.acc[j]] = 63 - e;
     fsc[kMaxAggs + fc.acc[j]] = e;
 }
 
-void prepare_f64(qe_gpu_ctx* ctx, AggProg& p, uint64_t* st, uint64_t S, uint64_t n_rows, uint64_t fold_rows,
-                 bool exact, Scratch& buf) {
+void prepare_f64(qe_gpu_ctx* ctx, AggProg& p, uint64_t* st, uint64_t S, uint64_t n_rows, bool exact, Scratch& buf) {
     if (p.nfs == 0) return;
     F64Cols fc{};
     for (int a = 0; a < p.n; ++a)
@@ -159,10 +158,7 @@ void prepare_f64(qe_gpu_ctx* ctx, AggProg& p, uint64_t* st, uint64_t S, uint64_t
     if (st) memset_async(ctx, st + state_flags(S, p, 0), 0, size_t(2 * p.nfs) * S * 8);  // flags, outlier sums
     const uint64_t stride = exact ? 1 : std::max<uint64_t>(1, n_rows >> 16);
     f64_exp_kernel<<<dim3(nblk, fc.n), 256, 0, ctx->stream>>>(fc, n_rows, n_rows ? stride : 0, part);
-    int L = 1;
-    while (L < 62 && (uint64_t(1) << L) <= fold_rows) ++L;
-    L += 2;
-    f64_scale_kernel<<<1, 64, 0, ctx->stream>>>(part, nblk, fc, exact ? 0 : 8, L, fsc);
+    f64_scale_kernel<<<1, 64, 0, ctx->stream>>>(part, nblk, fc, exact ? 0 : 2, fsc);
     QE_HIP(hipGetLastError());
     p.fsc = fsc;
     p.fflag = st ? st + state_flags(S, p, 0) : nullptr;
@@ -621,7 +617,6 @@ __global__ void agg_global_kernel(uint64_t n, SlotFn slot_of, uint64_t S, AggPro
     }
 }
 
-// F64 scales (prepare_f64) are set here, where the rows one partial folds are known.
 template <class SlotFn>
 static void run_slots(qe_gpu_ctx* ctx, uint64_t n, SlotFn fn, uint32_t S, AggProg& p, uint64_t* st,
                       const char* timer, bool exact, Scratch& fbuf) {
@@ -632,9 +627,7 @@ static void run_slots(qe_gpu_ctx* ctx, uint64_t n, SlotFn fn, uint32_t S, AggPro
         unsigned blocks = static_cast<unsigned>(std::min<uint64_t>(
             uint64_t(ctx->num_cus) * std::min(per_cu, 4u), (n + block - 1) / block));
         if (blocks == 0) blocks = 1;
-        // grid-stride rows: at most ceil(n / (blocks * 512)) per thread
-        prepare_f64(ctx, p, st, S, n, ((n + uint64_t(blocks) * block - 1) / (uint64_t(blocks) * block)) * block,
-                    exact, fbuf);
+        prepare_f64(ctx, p, st, S, n, exact, fbuf);
         Scratch part(ctx, size_t(blocks) * part_words(S, p) * 8);
         {
             TimedRegion tr(ctx, timer);
@@ -644,7 +637,7 @@ static void run_slots(qe_gpu_ctx* ctx, uint64_t n, SlotFn fn, uint32_t S, AggPro
         merge_partials(ctx, part.as<uint64_t>(), blocks, st, S, p);
     } else {
         init_state(ctx, st, S, p);
-        prepare_f64(ctx, p, st, S, n, n, exact, fbuf);
+        prepare_f64(ctx, p, st, S, n, exact, fbuf);
         TimedRegion tr(ctx, timer);
         agg_global_kernel<SlotFn><<<grid_for(n, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(n, fn, S, p, st);
         QE_HIP(hipGetLastError());
@@ -749,7 +742,7 @@ qe_gpu_status qe_gpu::aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* 
             fn.t = HashTableView{state.as<uint32_t>(), hash.as<uint64_t>(), words.as<uint64_t>(),
                                  nmask.as<uint64_t>(), cap, ovf.as<unsigned int>()};
             init_state(ctx, st.as<uint64_t>(), cap, p);
-            prepare_f64(ctx, p, st.as<uint64_t>(), cap, n, n, exact, fbuf);
+            prepare_f64(ctx, p, st.as<uint64_t>(), cap, n, exact, fbuf);
             {
                 TimedRegion tr(ctx, "agg_hash");
                 agg_hash_kernel<<<grid_for(n, 256, ctx->num_cus * 16), 256, 0, ctx->stream>>>(n, fn, cap, p,

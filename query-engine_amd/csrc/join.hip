@@ -1408,25 +1408,6 @@ qe_gpu_status qe_gpu_pairs_gather(qe_gpu_ctx* ctx, const qe_dev_pairs* p, uint64
 
 namespace qe_gpu {
 
-__global__ void max_run_kernel(const uint64_t* offs, uint64_t n, unsigned long long* out) {
-    unsigned long long m = 0;
-    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n; i += uint64_t(gridDim.x) * blockDim.x)
-        m = max(m, static_cast<unsigned long long>(offs[i + 1] - offs[i]));
-    atomicMax(out, m);
-}
-
-// Longest build-row list of a non-unique join index (the largest fan-out).
-uint64_t max_list_len(qe_gpu_ctx* ctx, const JoinIndex& ix) {
-    const uint64_t ns = ix.nslots;
-    if (ix.unique || ns == 0) return 1;
-    Scratch m(ctx, 8);
-    memset_async(ctx, m.p, 0, 8);
-    max_run_kernel<<<grid_for(ns, 256, 1024), 256, 0, ctx->stream>>>(ix.offs.as<uint64_t>(), ns,
-                                                                    m.as<unsigned long long>());
-    QE_HIP(hipGetLastError());
-    return read_scalar(ctx, m.as<uint64_t>());
-}
-
 // Most LDS slot-range passes over the probe before HBM atomics are used.
 constexpr uint64_t kMaxSlotPasses = 32;
 // Fewest probe rows for which a sampled hot set may replace the passes, and the
@@ -1593,13 +1574,8 @@ qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* pro
             const bool multipass = !use_lds && npasses <= kMaxSlotPasses;
             const unsigned lds_blocks = static_cast<unsigned>(
                 std::max<uint64_t>(1, std::min<uint64_t>(ctx->num_cus, (np + 1023) / 1024)));
-            // Exact F64 sums: a block folds at most its share of the probe plus
-            // two tiles (and the odd last row) when every probe row is summed at
-            // most once (a unique build key); the row-list path below rescales
-            // for its fan-out.
-            Scratch fbuf;
-            const uint64_t fold_rows = (np + lds_blocks - 1) / lds_blocks + 2 * 8192 + 1;
-            prepare_f64(ctx, p, st.as<uint64_t>(), S, np, fold_rows, exact, fbuf);
+            Scratch fbuf;  // exact F64 sums: scales, flags, outliers
+            prepare_f64(ctx, p, st.as<uint64_t>(), S, np, exact, fbuf);
             const bool small = S < 0xFFFF;
             const uint64_t* kv = pk[0]->all_valid ? nullptr : pk[0]->validity;
 
@@ -1785,9 +1761,6 @@ qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* pro
                 QE_HIP(hipGetLastError());
                 probe_direct(gmap, ix.kmin, ix.range);
             } else {
-                if (!ix.unique)  // a probe row is summed once per build row of its key
-                    prepare_f64(ctx, p, st.as<uint64_t>(), S, np,
-                                fold_rows * std::max<uint64_t>(1, max_list_len(ctx, ix)), exact, fbuf);
                 ProbeAggLists L{ix.view(pk.data(), static_cast<int>(nkeys)), gslot.as<uint32_t>()};
                 if (use_lds) {
                     Scratch part(ctx, size_t(lds_blocks) * part_words(static_cast<uint32_t>(S), p) * 8);
