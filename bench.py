@@ -146,16 +146,26 @@ def cpu_baseline_join(qe, ctx, cols, args):
     t.free()
     pk_s.free()
     pp_s.free()
-    gpu = {int(k): (float(s), int(c)) for k, s, c in zip(gv[0][0], gv[1][0], gv[2][0])}
+    # Parity on the sample: the oracle's join -> aggregate with Neumaier sums;
+    # keys and COUNTs exact, SUM within 1e-12 relative.
+    from _compare import diff, table_rows
+    t4 = time.perf_counter()
+    want = O.join_aggregate([(pkn, None, O.I64), (ppn, None, O.F64)], [(bk, None, O.I64), (bg, None, O.I32)],
+                            [0], [0], 0, [3], [(O.SUM, 1), (O.COUNT_STAR, 0)], sum_mode=1)
+    t5 = time.perf_counter()
+    mismatch = diff(table_rows(gv), want, 1e-12, 1e-12, False)
     cpu = {int(og[i]): (float(osum[i]), int(ocnt[i])) for i in range(g)}
-    ok = gpu.keys() == cpu.keys() and all(
-        gpu[k][1] == cpu[k][1] and abs(gpu[k][0] - cpu[k][0]) <= 1e-9 * max(1.0, abs(cpu[k][0])) for k in cpu)
+    port_vs_oracle = all(cpu.get(r[0], (None, -1))[1] == r[2] for r in want) and len(cpu) == len(want)
     res = {
         "value": sample / (t1 - t0), "unit": "probe rows/s", "cores": threads, "kind": "port",
         "sample": f"full 2^{args.build_log2}-row build + first {sample} probe rows; build single-threaded "
                   f"then morsel-parallel probe (1024-row morsels) with per-thread partial aggregates "
                   f"(oracle/qe_baseline.cpp)",
-        "seconds": t1 - t0, "gpu_parity_on_sample": bool(ok),
+        "seconds": t1 - t0, "gpu_parity_on_sample": mismatch is None,
+        "parity_check": "GPU vs oracle (single-thread join -> aggregate, Neumaier sums) on the sample: "
+                        "keys and COUNTs exact, SUM within 1e-12 rel",
+        "parity_mismatch": mismatch, "oracle_single_thread_s": t5 - t4,
+        "port_counts_equal_oracle": bool(port_vs_oracle),
     }
     # The reference's own operators (compiled from its sources into oracle/_ref
     # where this tree was built), single-threaded: Aggregate(HashJoin(Scan, Scan)).
@@ -247,10 +257,52 @@ def cpu_baseline_asof(qe, ctx, idx, ls, lt, gpu_match, st, args):
                               threads, out.ctypes.data)
     t1 = time.perf_counter()
     mapped = np.where(out == qe.NO_ROW, qe.NO_ROW, rows[np.minimum(out, max(n - 1, 0))])
-    return {"value": sample / (t1 - t0), "unit": "probe rows/s", "cores": threads, "kind": "port",
-            "sample": f"first {sample} probes against the full sorted right side; std::upper_bound per probe "
-                      f"(oracle/qe_baseline.cpp)",
-            "gpu_parity_on_sample": bool((mapped == gpu_match[:sample]).all())}
+    res = {"value": sample / (t1 - t0), "unit": "probe rows/s", "cores": threads, "kind": "port",
+           "sample": f"first {sample} probes against the full sorted right side; std::upper_bound per probe "
+                     f"(oracle/qe_baseline.cpp)",
+           "gpu_parity_on_sample": bool((mapped == gpu_match[:sample]).all())}
+    res["single_thread_asof_join"] = cpu_asof_single_thread(qe, ctx, O, args)
+    return res
+
+
+def cpu_asof_single_thread(qe, ctx, O, args):
+    """The whole as-of join single-threaded on a sample of the same generator
+    (right 2^22 rows, left 2^20, 5000 Zipf(1.1) symbols): the oracle's
+    restatement and, where this tree was built with it, the reference's
+    AsofJoin (sort of both sides + cursor merge, tsx/asof.cpp:234-308, 2048-row
+    batches), each timed over the operator run alone. The GPU binary probe on
+    the same sample is checked against the reference's pairs."""
+    rn, ln = (1 << 22, 1 << 20) if not args.quick else (1 << 18, 1 << 16)
+    (rs, rt, rp), (ls, lt) = ctx.gen_asof(20260616, rn, ln, 5000, 1.1)
+    idx = ctx.asof_build([rs], rt)
+    gpu = idx.probe([ls], lt, mode=qe.ASOF_BINARY)
+    idx.free()
+    cols = [c.numpy()[0] for c in (rs, rt, rp, ls, lt)]
+    for c in (rs, rt, rp, ls, lt):
+        c.free()
+    bc = [(cols[0], None, O.I32), (cols[1], None, O.I64), (cols[2], None, O.F64)]
+    pc = [(cols[3], None, O.I32), (cols[4], None, O.I64)]
+    t0 = time.perf_counter()
+    p, b = O.asof_pairs(pc, bc, [0], [0], 1, 1, join_type=0)
+    t1 = time.perf_counter()
+    want = np.full(ln, qe.NO_ROW, dtype=np.uint32)
+    want[p] = b
+    res = {"sample": f"gen_asof(seed 20260616) at right {rn} / left {ln} rows, inner as-of on sym, ts <=",
+           "cores": 1, "unit": "left rows/s", "port_value": ln / (t1 - t0),
+           "gpu_equals_port": bool((want == gpu).all())}
+    if O.ref_available():
+        R = O.ref()
+        bcs, pcs = O.Cols(bc), O.Cols(pc)
+        t2 = time.perf_counter()
+        h = R.ref_asof_run(pcs.arr, 2, pcs.n, bcs.arr, 3, bcs.n, O._u32([0]), O._u32([0]), 1, 1, 1, 0, 0, 0, 2048)
+        t3 = time.perf_counter()
+        rows = O._ref_rows(h)  # frees h; each row is probe cols, probe row id, build cols, build row id
+        rwant = np.full(ln, qe.NO_ROW, dtype=np.uint32)
+        rwant[np.array([r[2] for r in rows], dtype=np.int64)] = [r[6] for r in rows]
+        res.update({"value": ln / (t3 - t2), "kind": "reference", "reference_seconds": t3 - t2,
+                    "gpu_equals_reference": bool((rwant == gpu).all()),
+                    "restatement_vs_reference": (t1 - t0) / (t3 - t2)})
+    return res
 
 
 def batched_workload(qe, ctx, args):
@@ -297,16 +349,23 @@ def batched_workload(qe, ctx, args):
         res["speedup_persistent_vs_per_batch"] = res["per_batch_s"] / res["persistent_s"]
         res["speedup_graph_vs_per_batch"] = res["per_batch_s"] / res["graph_s"]
     res["rows_per_s_graph"] = total_rows / res["graph_s"]
-    # parity: the three modes produce identical groups on one query sample
+    # Parity between the modes: every query's groups, keys and aggregates
+    # (COUNT, SUM, AVG) identical in bits once rows are ordered by key.
     outs = {m: plan.run(m, tuning=tune) for m in (qe.BATCH_PERSISTENT, qe.BATCH_GRAPH)}
     same = True
     for a, b in zip(outs[qe.BATCH_PERSISTENT], outs[qe.BATCH_GRAPH]):
-        ka = a.numpy()
-        kb = b.numpy()
-        same &= a.rows == b.rows and np.array_equal(np.sort(ka[0][0]), np.sort(kb[0][0]))
+        ca, cb = a.numpy(), b.numpy()
+        same &= a.rows == b.rows and len(ca) == len(cb)
+        if same:
+            oa, ob = np.argsort(ca[0][0], kind="stable"), np.argsort(cb[0][0], kind="stable")
+            for (va, ma), (vb, mb) in zip(ca, cb):
+                same &= np.array_equal(va[oa].view(np.uint8), vb[ob].view(np.uint8))
+                same &= (ma is None and mb is None) or (ma is not None and mb is not None
+                                                        and np.array_equal(ma[oa], mb[ob]))
         a.free()
         b.free()
     res["graph_equals_persistent"] = bool(same)
+    res["graph_equals_persistent_check"] = "all result columns, bitwise, rows ordered by key"
     plan.free()
     for k, v, *_ in queries:
         k.free()
@@ -341,6 +400,9 @@ def copy_ceiling(qe, ctx, nbytes=1 << 30, reps=5):
 
 
 def groupby_1m(qe, ctx, args):
+    """Plain group-by: 2^20 rows, I64 key uniform in [0,1024), F64 value in
+    [0,1000) with 5% NULLs; COUNT(*), SUM, AVG per key. The GPU query is timed
+    end to end (result on the host); beside it the CPU paths on the same rows."""
     n = 1 << 20
     k, v = ctx.gen_groupby(20260615, n, 1024, 0.05)
     aggs = [(qe.COUNT_STAR, 0), (qe.SUM, 1), (qe.AVG, 1)]
@@ -353,9 +415,74 @@ def groupby_1m(qe, ctx, args):
         ctx.aggregate([k, v], [0], aggs).free()
     ctx.sync()
     s = (time.perf_counter() - t) / reps
+    res = {"rows": n, "ms_per_query": s * 1e3, "rows_per_s": n / s}
+    if not args.skip_cpu:
+        tab = ctx.aggregate([k, v], [0], aggs)
+        gv = tab.numpy()
+        tab.free()
+        res["cpu_baseline"] = cpu_baseline_groupby(qe, k.numpy(), v.numpy(), gv)
     k.free()
     v.free()
-    return {"rows": n, "ms_per_query": s * 1e3, "rows_per_s": n / s}
+    return res
+
+
+def _best_of(fn, reps):
+    best = 1e30
+    for _ in range(reps):
+        t = time.perf_counter()
+        fn()
+        best = min(best, time.perf_counter() - t)
+    return best
+
+
+def cpu_baseline_groupby(qe, kcol, vcol, gpu_cols):
+    """The plain group-by on the host cores, whole workload (2^20 rows), best
+    of 5: the port single-threaded and morsel-parallel (oracle/qe_baseline.cpp,
+    1024-row morsels, per-thread hash maps merged at the end), the oracle's
+    restatement (ops/aggregate.cpp order, Neumaier sums) and, where this tree
+    was built with it, the reference's own Aggregate over 2048-row batches
+    (compiled from its sources into oracle/_ref). restatement_vs_reference is
+    the restatement's time over the reference's."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import _oracle as O
+    from _compare import diff, table_rows
+    L = O.oracle()
+    key, _ = kcol
+    val, valid = vcol
+    n = len(key)
+    words = qe.pack_validity(valid) if valid is not None else None
+    wptr = words.ctypes.data if words is not None else None
+    maxg = 1 << 12
+    ok_, oc, osum, onn = (np.empty(maxg, np.int64), np.empty(maxg, np.int64), np.empty(maxg, np.float64),
+                          np.empty(maxg, np.int64))
+    threads = int(L.orc_bl_hw_threads())
+
+    def port(th):
+        return lambda: L.orc_bl_groupby_i64_f64(key.ctypes.data, val.ctypes.data, wptr, n, th, 1024, maxg,
+                                                ok_.ctypes.data, oc.ctypes.data, osum.ctypes.data, onn.ctypes.data)
+
+    cols = [(key, None, O.I64), (val, valid, O.F64)]
+    oaggs = [(O.COUNT_STAR, 0), (O.SUM, 1), (O.AVG, 1)]
+    res = {"unit": "rows/s", "kind": "port", "cores": threads,
+           "sample": f"the whole workload ({n} rows), best of 5"}
+    t1 = _best_of(port(1), 5)
+    tm = _best_of(port(threads), 5)
+    res["value"] = n / tm
+    res["port_single_thread_rows_per_s"] = n / t1
+    res["port_morsel_rows_per_s"] = n / tm
+    want = []
+    to = _best_of(lambda: want.append(O.aggregate(cols, [0], oaggs, sum_mode=1)), 5)
+    res["oracle_restatement_single_thread_rows_per_s"] = n / to
+    mismatch = diff(table_rows(gpu_cols), want[-1], 1e-12, 1e-12, False)
+    res["gpu_parity"] = mismatch is None
+    res["parity_mismatch"] = mismatch
+    if O.ref_available():
+        tr = _best_of(lambda: O.ref_aggregate(cols, [0], oaggs), 5)
+        res["reference_single_thread"] = {"value": n / tr, "unit": "rows/s", "cores": 1, "kind": "reference",
+                                          "sample": f"{n} rows through the reference's Aggregate (2048-row "
+                                                    f"batches), best of 5"}
+        res["restatement_vs_reference"] = to / tr
+    return res
 
 
 def filter_workload(qe, ctx, args):
@@ -577,6 +704,60 @@ def tick_workload(qe, ctx, args):
     return res
 
 
+def read_ceiling(qe, ctx, nbytes=1 << 30, reps=5):
+    """Measured read-only HBM ceiling: 16-byte nontemporal loads, four in
+    flight per thread, nothing written."""
+    src, sink = qe.DevBuffer(ctx, nbytes), qe.DevBuffer(ctx, 1024)
+    best = 1e30
+    for _ in range(reps + 1):
+        best = min(best, ctx.stream_read(src, sink, nbytes))
+    src.free()
+    sink.free()
+    return {"bytes_read": nbytes, "best_ms": best, "gbs": nbytes / (best / 1e3) / 1e9}
+
+
+def _read(path):
+    try:
+        return Path(path).read_text()
+    except OSError:
+        return ""
+
+
+def result_tags(qe, ctx, args):
+    """The machine state that travels with the numbers, after the reference's
+    bench/bench_report.h:22-46 (CPU model, cores, RAM, governors, kernel, load,
+    build provenance), plus the GPU and ROCm."""
+    import glob
+    import hashlib
+    import platform
+    cpu_model = next((ln.split(":", 1)[1].strip() for ln in _read("/proc/cpuinfo").splitlines()
+                      if ln.startswith("model name")), platform.processor() or "n/a")
+    mem = next((int(ln.split()[1]) * 1024 for ln in _read("/proc/meminfo").splitlines()
+                if ln.startswith("MemTotal:")), 0)
+    govs = sorted({_read(g).strip() for g in glob.glob("/sys/devices/system/cpu/cpu*/cpufreq/scaling_governor")})
+    src = hashlib.sha256()
+    for f in sorted((ROOT / "query-engine_amd" / "csrc").glob("*")) + [ROOT / "bench.py"]:
+        if f.is_file():
+            src.update(f.read_bytes())
+    try:
+        load1 = os.getloadavg()[0]
+    except OSError:
+        load1 = None
+    return {
+        "seeds": {"join_groupby": 20260615, "groupby_1m": 20260615, "asof": 20260616, "batched": 20260617,
+                  "filter": 20260617, "tick_pipeline": 20260618, "tick_cpu_sample": 20260619},
+        "host": {"cpu_model": cpu_model, "logical_cpus": os.cpu_count(),
+                 "usable_cpus": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None,
+                 "ram_bytes": mem, "governors": ",".join(g for g in govs if g) or "n/a",
+                 "kernel": platform.release(), "loadavg1": load1},
+        "gpu": ctx.device_info(),
+        "rocm": _read("/opt/rocm/.info/version").strip() or "n/a",
+        "build": {"compiler": "hipcc --offload-arch=gfx950 -O3 -std=c++17, packed-fp32 ops off",
+                  "source_sha16": src.hexdigest()[:16]},
+        "args": {k: v for k, v in vars(args).items()},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -637,7 +818,8 @@ def main():
         print(json.dumps({args.only: fn(qe, ctx, args)}), flush=True)
         return
     ceiling = copy_ceiling(qe, ctx)
-    log(f"stream copy ceiling {ceiling['gbs']:.0f} GB/s")
+    rceil = read_ceiling(qe, ctx)
+    log(f"stream copy ceiling {ceiling['gbs']:.0f} GB/s, read-only {rceil['gbs']:.0f} GB/s")
     res, cols = join_groupby_workload(qe, ctx, args)
     log(f"join+groupby: {res['ms_per_step']:.3f} ms/step, kernel {res['kernel_ms']:.3f} ms")
     cpu = None
@@ -693,6 +875,8 @@ def main():
             "kernel": res["kernel"], "kernel_ms": res["kernel_ms"],
             "algorithmic_bytes_per_launch": res["probe_bytes"],
             "measured_copy_ceiling_gbs": ceiling["gbs"],
+            "measured_read_ceiling_gbs": rceil["gbs"],
+            "frac_of_measured_read_ceiling": (kach / rceil["gbs"]) if kach else None,
         },
         "step_latency_ms": {"p50": res["step_ms_p50"], "p99": res["step_ms_p99"], "reps": res["step_reps"]},
         "step_roofline": {"achieved": res["step_achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -700,6 +884,7 @@ def main():
                           "algorithmic_bytes_per_step": res["step_bytes"]},
         "cpu_baseline": cpu,
         "secondary": secondary,
+        "tags": result_tags(qe, ctx, args),
     }
     print(json.dumps(line), flush=True)
     ctx.close()

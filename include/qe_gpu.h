@@ -86,6 +86,23 @@ typedef struct qe_dev_asof qe_dev_asof;   /* sorted as-of build side        */
 qe_gpu_status qe_gpu_ctx_create(int device, qe_gpu_ctx** out);
 void qe_gpu_ctx_destroy(qe_gpu_ctx* ctx);
 const char* qe_gpu_last_error(const qe_gpu_ctx* ctx);
+/* The context's device as the HIP runtime reports it, for result tags
+ * (bench/bench_report.h:22-46 records the host the same way). */
+typedef struct {
+    char name[256];
+    char arch[64];               /* e.g. "gfx950:sramecc+:xnack-"            */
+    int32_t device;
+    int32_t compute_units;
+    uint64_t hbm_bytes;
+    uint64_t lds_bytes_per_block;
+    uint64_t l2_bytes;
+    int32_t clock_khz;
+    int32_t memory_clock_khz;
+    int32_t memory_bus_bits;
+    int32_t hip_runtime_version;
+    int32_t hip_driver_version;
+} qe_gpu_device_info_t;
+qe_gpu_status qe_gpu_device_info(qe_gpu_ctx* ctx, qe_gpu_device_info_t* out);
 qe_gpu_status qe_gpu_sync(qe_gpu_ctx* ctx);
 /* Device memory limit for this context (0 = no limit): bytes in live device
  * buffers may not exceed it (cached free blocks are released first when a new
@@ -392,6 +409,11 @@ qe_gpu_status qe_gpu_memcpy_h2d(qe_gpu_ctx* ctx, void* dst, const void* src, uin
  * kernel time in ms (HIP events on the context stream). No counterpart in the
  * reference (measurement support). */
 qe_gpu_status qe_gpu_stream_copy(qe_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes, double* ms);
+/* Read-only streaming kernel over src (bytes % 16 == 0): the measured ceiling
+ * for kernels that mostly read. sink is a 1 KiB device buffer it practically
+ * never writes. Returns the kernel time in ms. No counterpart in the reference
+ * (measurement support). */
+qe_gpu_status qe_gpu_stream_read(qe_gpu_ctx* ctx, const void* src, uint64_t bytes, void* sink, double* ms);
 
 /* ---- synthetic workload generators (bench / tests; seeded, counter-based) ---------*/
 /* Config shapes from the benchmark definitions; every value is a pure

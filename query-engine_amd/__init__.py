@@ -152,6 +152,8 @@ _SIGS = {
     "qe_gpu_memcpy_d2h": (C.c_int, [P, P, P, C.c_uint64]),
     "qe_gpu_memcpy_h2d": (C.c_int, [P, P, P, C.c_uint64]),
     "qe_gpu_stream_copy": (C.c_int, [P, P, P, C.c_uint64, C.POINTER(C.c_double)]),
+    "qe_gpu_stream_read": (C.c_int, [P, P, C.c_uint64, P, C.POINTER(C.c_double)]),
+    "qe_gpu_device_info": (C.c_int, [P, P]),
     "qe_gpu_gen_groupby": (C.c_int, [P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, P, P]),
     "qe_gpu_gen_join_build": (C.c_int, [P, C.c_uint64, C.c_uint64, C.c_uint32, P, P]),
     "qe_gpu_gen_join_probe": (C.c_int, [P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, C.c_double, P, P]),
@@ -428,6 +430,20 @@ class Context:
         self._check(lib().qe_gpu_stream_copy(self.h, dst.p, src.p, nbytes, C.byref(ms)))
         return ms.value
 
+    def device_info(self) -> dict:
+        """The context's device (name, gfx arch, CUs, HBM bytes, clocks, HIP versions)."""
+        d = DeviceInfo()
+        self._check(lib().qe_gpu_device_info(self.h, C.byref(d)))
+        out = {f: getattr(d, f) for f, _ in DeviceInfo._fields_}
+        out["name"], out["arch"] = d.name.decode(), d.arch.decode()
+        return out
+
+    def stream_read(self, src: "DevBuffer", sink: "DevBuffer", nbytes: int) -> float:
+        """A read-only stream over src (the measured read ceiling); returns its kernel ms."""
+        ms = C.c_double()
+        self._check(lib().qe_gpu_stream_read(self.h, src.p, nbytes, sink.p, C.byref(ms)))
+        return ms.value
+
     def sort_perm_into(self, keys, out: "DevBuffer", desc=None, nulls_first=None):
         """Like sort_perm, with the permutation left in a device buffer."""
         d = None if desc is None else (C.c_int32 * len(keys))(*desc)
@@ -678,6 +694,14 @@ class HashTable:
         if self.h:
             lib().qe_gpu_ht_free(self.ctx.h, self.h)
         self.h = None
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 256), ("arch", C.c_char * 64), ("device", C.c_int32),
+                ("compute_units", C.c_int32), ("hbm_bytes", C.c_uint64), ("lds_bytes_per_block", C.c_uint64),
+                ("l2_bytes", C.c_uint64), ("clock_khz", C.c_int32), ("memory_clock_khz", C.c_int32),
+                ("memory_bus_bits", C.c_int32), ("hip_runtime_version", C.c_int32),
+                ("hip_driver_version", C.c_int32)]
 
 
 class AsofIndex:
