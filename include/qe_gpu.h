@@ -227,6 +227,18 @@ qe_gpu_status qe_gpu_filter(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, uint
 qe_gpu_status qe_gpu_filter_select(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, uint32_t ncols,
                                    const qe_pred_term* terms, uint32_t nterms, uint32_t* sel_out,
                                    uint64_t* count_out);
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (serving loops and harnesses that forbid hipMalloc): the same
+ * selection as qe_gpu_filter_select, identical output, with no device
+ * allocation, no free and no allocating library call. Its working memory is a
+ * static 16.4 MiB device array per GPU (held until the process ends), so it
+ * takes at most 2^27 rows (QE_GPU_UNSUPPORTED beyond). Every call on one GPU
+ * shares that array: calls that overlap on one GPU, from any context or
+ * thread, race. It synchronises the context stream (the count is read back),
+ * so it cannot be captured into a HIP graph. */
+qe_gpu_status qe_gpu_filter_select_noalloc(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, uint32_t ncols,
+                                           const qe_pred_term* terms, uint32_t nterms, uint32_t* sel_out,
+                                           uint64_t* count_out);
 
 /* ---- hash join ----------------------------------------------------------------*/
 /* Replaces qe::HashJoin(probe, build, probe_keys, build_keys, type)

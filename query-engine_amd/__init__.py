@@ -89,6 +89,7 @@ _SIGS = {
     "qe_gpu_set_timing": (C.c_int, [P, C.c_int]),
     "qe_gpu_filter": (C.c_int, [P, P, C.c_uint32, P, C.c_uint32, C.POINTER(P)]),
     "qe_gpu_filter_select": (C.c_int, [P, P, C.c_uint32, P, C.c_uint32, P, C.POINTER(C.c_uint64)]),
+    "qe_gpu_filter_select_noalloc": (C.c_int, [P, P, C.c_uint32, P, C.c_uint32, P, C.POINTER(C.c_uint64)]),
     "qe_gpu_set_tuning": (C.c_int, [P, C.c_int32, C.c_int64]),
     "qe_gpu_xcd_of_blocks": (C.c_int, [P, C.c_uint32, P]),
     "qe_gpu_timing": (C.c_int, [P, C.c_char_p, C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
@@ -540,6 +541,15 @@ class Context:
             return buf.numpy(np.uint32, cnt.value)
         finally:
             buf.free()
+
+    def filter_select_noalloc_into(self, cols, terms, out: "DevBuffer") -> int:
+        """The same selection through the entry point that allocates nothing:
+        indices land in `out` (device u32[len]); returns their count."""
+        cnt = C.c_uint64()
+        self._check(lib().qe_gpu_filter_select_noalloc(self.h, _cols(cols), len(cols),
+                                                       C.cast(self._terms(terms), P), len(terms), out.p,
+                                                       C.byref(cnt)))
+        return cnt.value
 
     def hash_join(self, probe_keys, build_keys, join_type=INNER) -> Pairs:
         out = P()

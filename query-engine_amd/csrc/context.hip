@@ -538,6 +538,7 @@ qe_gpu_status qe_gpu_ctx_create(int device, qe_gpu_ctx** out) {
     }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
+    qe_gpu::noalloc_setup(device);
     *out = ctx;
     return QE_GPU_OK;
 }

@@ -29,6 +29,8 @@
 // output.
 #include "internal.hpp"
 
+#include <mutex>
+
 using namespace qe_gpu;
 
 namespace qe_gpu {
@@ -425,6 +427,67 @@ uint64_t select_pairs(qe_gpu_ctx* ctx, const uint32_t* match, uint64_t n, bool k
     return f.total;
 }
 
+// ---- selection with no allocation ---------------------------------------------
+// qe_gpu_filter_select_noalloc keeps its bitmap, tile counts, tile offsets and
+// scan partials in one file-scope device array (below) instead of allocator
+// blocks. Layout for at most kNoAllocMaxRows rows (256-byte aligned parts):
+//   [0, 16 MiB)                 passing-row bitmap, one bit per row
+//   [16 MiB, +128 KiB)          per-tile passing counts (u32)
+//   [16 MiB + 128 KiB, +256 KiB) per-tile output offsets (u64)
+//   [16 MiB + 384 KiB, +4 KiB)   scan partials (u64)
+// Every word is written before it is read (the mask kernel writes every bitmap
+// word and every tile count, the scan writes every partial and offset), so no
+// part needs clearing between calls.
+constexpr uint64_t kNoAllocMaxRows = uint64_t(1) << 27;
+constexpr uint64_t kNoAllocTiles = kNoAllocMaxRows / kPredTile;
+constexpr uint64_t kNoAllocMaskOff = 0;
+constexpr uint64_t kNoAllocCntOff = kNoAllocMaskOff + kNoAllocMaxRows / 8;
+constexpr uint64_t kNoAllocTileOff = kNoAllocCntOff + kNoAllocTiles * 4;
+constexpr uint64_t kNoAllocSumsOff = kNoAllocTileOff + kNoAllocTiles * 8;
+constexpr uint64_t kNoAllocSumsBytes = 4096;
+constexpr uint64_t kNoAllocBytes = kNoAllocSumsOff + kNoAllocSumsBytes;
+static_assert(kNoAllocMaxRows % kPredTile == 0 && kNoAllocCntOff % 256 == 0 && kNoAllocTileOff % 256 == 0 &&
+                  kNoAllocSumsOff % 256 == 0,
+              "scratch parts stay 256-byte aligned");
+
+}  // namespace qe_gpu
+
+// The no-allocation path's device memory: one copy per GPU, held by the loaded
+// code object until the process ends. Host code never uses this name as an
+// address (on the host it is a stand-in, not device memory): it uses the
+// per-device address that qe_gpu::noalloc_setup resolves.
+static __device__ __align__(256) uint8_t scratch[qe_gpu::kNoAllocBytes];
+
+namespace qe_gpu {
+namespace {
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kNoAllocMaxDevices];
+int g_noalloc_state[kNoAllocMaxDevices];  // 0 not yet resolved, 1 resolved, -1 failed
+}  // namespace
+
+void noalloc_setup(int device) {
+    if (device < 0 || device >= kNoAllocMaxDevices) return;
+    std::lock_guard<std::mutex> g(g_noalloc_mu);
+    if (g_noalloc_state[device] != 0) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && p != nullptr) {
+        g_noalloc_ptr[device] = p;
+        g_noalloc_state[device] = 1;
+    } else {
+        (void)hipGetLastError();
+        g_noalloc_state[device] = -1;
+    }
+}
+
+static uint8_t* noalloc_scratch(int device) {
+    std::lock_guard<std::mutex> g(g_noalloc_mu);
+    check(device >= 0 && device < kNoAllocMaxDevices && g_noalloc_state[device] == 1, QE_GPU_HIP_ERROR,
+          "the no-allocation scratch array has no device address on this GPU");
+    return static_cast<uint8_t*>(g_noalloc_ptr[device]);
+}
+
 }  // namespace qe_gpu
 
 extern "C" {
@@ -494,6 +557,44 @@ qe_gpu_status qe_gpu_filter_select(qe_gpu_ctx* ctx, const qe_dev_col* const* col
         if (f.total > 0) launch_compact(ctx, f, n, CompactCols{}, sel_out, nullptr);
         *count_out = f.total;
         QE_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+
+qe_gpu_status qe_gpu_filter_select_noalloc(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, uint32_t ncols,
+                                           const qe_pred_term* terms, uint32_t nterms, uint32_t* sel_out,
+                                           uint64_t* count_out) {
+    QE_ENTRY(ctx, {
+        invalid(cols != nullptr && ncols >= 1 && count_out != nullptr, "filter needs >=1 column");
+        const uint64_t n = common_len(cols, ncols);
+        check(n <= kNoAllocMaxRows, QE_GPU_UNSUPPORTED,
+              "filter_select_noalloc takes at most 2^27 rows (the size of its static scratch); "
+              "use qe_gpu_filter_select for more");
+        invalid(n == 0 || sel_out != nullptr, "null selection output");
+        const Pred P = make_pred(cols, ncols, terms, nterms);
+        uint8_t* base = noalloc_scratch(ctx->device);
+        *count_out = 0;
+        if (n == 0) return QE_GPU_OK;
+        const uint64_t tiles = (n + kPredTile - 1) / kPredTile;
+        uint64_t* mask = reinterpret_cast<uint64_t*>(base + kNoAllocMaskOff);
+        uint32_t* cnt = reinterpret_cast<uint32_t*>(base + kNoAllocCntOff);
+        uint64_t* off = reinterpret_cast<uint64_t*>(base + kNoAllocTileOff);
+        uint64_t* sums = reinterpret_cast<uint64_t*>(base + kNoAllocSumsOff);
+        const uint64_t nsums = scan_sums_words(tiles);
+        check(nsums * 8 <= kNoAllocSumsBytes, QE_GPU_UNSUPPORTED, "scan partials exceed their scratch part");
+        pred_mask_kernel<<<tiles, kPredBlock, 0, ctx->stream>>>(P, n, mask, cnt);
+        QE_HIP(hipGetLastError());
+        exclusive_scan_u32_dev(ctx, cnt, off, tiles, sums);
+        uint64_t total = 0;  // sums[nsums - 1] holds the scan's total
+        QE_HIP(hipMemcpyAsync(&total, sums + nsums - 1, 8, hipMemcpyDeviceToHost, ctx->stream));
+        QE_HIP(hipStreamSynchronize(ctx->stream));
+        if (total > 0) {
+            compact_kernel<<<tiles, kPredBlock, 0, ctx->stream>>>(mask, n, off, total, CompactCols{}, sel_out,
+                                                                  nullptr);
+            QE_HIP(hipGetLastError());
+        }
+        QE_HIP(hipStreamSynchronize(ctx->stream));
+        *count_out = total;
     });
 }
 

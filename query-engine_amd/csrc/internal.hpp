@@ -415,6 +415,10 @@ std::vector<qe_dev_col*> gather_cols(qe_gpu_ctx* ctx, const qe_dev_col* const* s
 // capture): `sums` holds scan_sums_words(n) words.
 uint64_t scan_sums_words(uint64_t n);
 void exclusive_scan_u32_dev(qe_gpu_ctx* ctx, const uint32_t* in, uint64_t* out, uint64_t n, uint64_t* sums);
+// Resolves, once per device and with that device current, the device address
+// of the no-allocation filter's static scratch array (called at context
+// creation).
+void noalloc_setup(int device);
 // Same for u64 inputs (in-place allowed).
 uint64_t exclusive_scan_u64(qe_gpu_ctx* ctx, const uint64_t* in, uint64_t* out, uint64_t n);
 
