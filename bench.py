@@ -96,6 +96,13 @@ def join_groupby_workload(qe, ctx, args):
         step()
         reps.append((time.perf_counter() - ta) * 1e3)
     kernel_ms = kms / max(launches, 1)
+    # The floor of the probe: its map lookups alone (the probe keys streamed,
+    # one 16-bit entry gathered per key from a map of the build's key range),
+    # best of 6.
+    gmap, sink = qe.DevBuffer(ctx, 2 * build_n), qe.DevBuffer(ctx, 1024)
+    gather_ms = min(ctx.map_gather(pk, 0, build_n, gmap, sink) for _ in range(6))
+    gmap.free()
+    sink.free()
     probe_bytes = probe_n * 16
     step_bytes = probe_n * 16 + build_n * 12 + groups * 20
     res = {
@@ -111,6 +118,8 @@ def join_groupby_workload(qe, ctx, args):
         "value": probe_n / (ms_step / 1e3),
         "step_achieved_gbs": step_bytes / (ms_step / 1e3) / 1e9,
         "kernel_achieved_gbs": probe_bytes / (kernel_ms / 1e3) / 1e9 if kernel_ms > 0 else None,
+        "map_gather_floor_ms": gather_ms,
+        "kernel_over_gather_floor": kernel_ms / gather_ms if gather_ms > 0 else None,
     }
     cols = {"pk": pk, "pp": pp, "bk": bk, "bg": bg}
     return res, cols
@@ -877,6 +886,8 @@ def main():
             "measured_copy_ceiling_gbs": ceiling["gbs"],
             "measured_read_ceiling_gbs": rceil["gbs"],
             "frac_of_measured_read_ceiling": (kach / rceil["gbs"]) if kach else None,
+            "map_gather_floor_ms": res["map_gather_floor_ms"],
+            "kernel_over_gather_floor": res["kernel_over_gather_floor"],
         },
         "step_latency_ms": {"p50": res["step_ms_p50"], "p99": res["step_ms_p99"], "reps": res["step_reps"]},
         "step_roofline": {"achieved": res["step_achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -426,6 +426,14 @@ qe_gpu_status qe_gpu_stream_copy(qe_gpu_ctx* ctx, void* dst, const void* src, ui
  * never writes. Returns the kernel time in ms. No counterpart in the reference
  * (measurement support). */
 qe_gpu_status qe_gpu_stream_read(qe_gpu_ctx* ctx, const void* src, uint64_t bytes, void* sink, double* ms);
+/* The fused join probe's map lookups alone: reads an I64 key column (data
+ * 16-byte aligned; validity ignored) and gathers map[clamp(key - kmin, 0, range - 1)] from a 16-bit map,
+ * with no payload and no aggregation. Over the probe's keys and a map of the
+ * probe's size its time is the floor of that probe. sink: a 1 KiB device
+ * buffer it practically never writes. No counterpart in the reference
+ * (measurement support). */
+qe_gpu_status qe_gpu_map_gather(qe_gpu_ctx* ctx, const qe_dev_col* key, int64_t kmin, uint64_t range,
+                                const uint16_t* map, void* sink, double* ms);
 
 /* ---- synthetic workload generators (bench / tests; seeded, counter-based) ---------*/
 /* Config shapes from the benchmark definitions; every value is a pure

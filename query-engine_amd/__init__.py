@@ -154,6 +154,7 @@ _SIGS = {
     "qe_gpu_memcpy_h2d": (C.c_int, [P, P, P, C.c_uint64]),
     "qe_gpu_stream_copy": (C.c_int, [P, P, P, C.c_uint64, C.POINTER(C.c_double)]),
     "qe_gpu_stream_read": (C.c_int, [P, P, C.c_uint64, P, C.POINTER(C.c_double)]),
+    "qe_gpu_map_gather": (C.c_int, [P, P, C.c_int64, C.c_uint64, P, P, C.POINTER(C.c_double)]),
     "qe_gpu_device_info": (C.c_int, [P, P]),
     "qe_gpu_gen_groupby": (C.c_int, [P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, P, P]),
     "qe_gpu_gen_join_build": (C.c_int, [P, C.c_uint64, C.c_uint64, C.c_uint32, P, P]),
@@ -429,6 +430,12 @@ class Context:
         """A plain streaming copy (the measured HBM ceiling); returns its kernel ms."""
         ms = C.c_double()
         self._check(lib().qe_gpu_stream_copy(self.h, dst.p, src.p, nbytes, C.byref(ms)))
+        return ms.value
+
+    def map_gather(self, keys: "DevCol", kmin: int, range_: int, map_: "DevBuffer", sink: "DevBuffer") -> float:
+        """The join probe's map lookups alone over an I64 key column; returns kernel ms."""
+        ms = C.c_double()
+        self._check(lib().qe_gpu_map_gather(self.h, keys.h, kmin, range_, map_.p, sink.p, C.byref(ms)))
         return ms.value
 
     def device_info(self) -> dict:

@@ -862,6 +862,68 @@ qe_gpu_status qe_gpu_stream_read(qe_gpu_ctx* ctx, const void* src, uint64_t byte
     });
 }
 
+// The map lookups of the fused join probe alone: 16-byte nontemporal loads of
+// key pairs (four pairs in flight per thread), each key's offset clamped into
+// [0, range) and its 16-bit map entry gathered, the entries xor-folded. No
+// payload stream and no aggregation: the time this takes over a probe column
+// and a map of the probe's size is the floor the fused probe cannot beat.
+__global__ void __launch_bounds__(256) map_gather_kernel(const int64_t* __restrict__ keys, uint64_t n, int64_t kmin,
+                                                         uint64_t range, const uint16_t* __restrict__ map,
+                                                         u32x4* __restrict__ sink) {
+    typedef long long i64x2 __attribute__((ext_vector_type(2)));
+    const i64x2* kp = reinterpret_cast<const i64x2*>(keys);
+    const uint64_t np = n / 2;
+    const uint64_t G = uint64_t(gridDim.x) * blockDim.x;
+    uint32_t acc = 0;
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < np; i += 4 * G) {
+        i64x2 k[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint64_t r = i + j * G;
+            k[j] = __builtin_nontemporal_load(kp + (r < np ? r : np - 1));
+        }
+        uint16_t v[8];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const uint64_t o0 = static_cast<uint64_t>(k[j].x - kmin), o1 = static_cast<uint64_t>(k[j].y - kmin);
+            v[2 * j] = map[o0 < range ? o0 : range - 1];
+            v[2 * j + 1] = map[o1 < range ? o1 : range - 1];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc = (acc << 1 | acc >> 31) ^ v[j];
+    }
+    if (n & 1) {
+        const uint64_t o = static_cast<uint64_t>(keys[n - 1] - kmin);
+        acc ^= map[o < range ? o : range - 1];
+    }
+    if (acc == 0xFFFFFFFFu) sink[threadIdx.x & 63] = u32x4{acc, acc, acc, acc};
+}
+
+qe_gpu_status qe_gpu_map_gather(qe_gpu_ctx* ctx, const qe_dev_col* key, int64_t kmin, uint64_t range,
+                                const uint16_t* map, void* sink, double* ms) {
+    QE_ENTRY(ctx, {
+        invalid(key && map && sink && range > 0, "map gather needs a key column, a map and a sink");
+        invalid(key->type == QE_I64 && reinterpret_cast<uintptr_t>(key->data) % 16 == 0,
+                "map gather reads 16-byte aligned I64 keys");
+        const int64_t* keys = static_cast<const int64_t*>(key->data);
+        const uint64_t n = key->len;
+        hipEvent_t a, b;
+        QE_HIP(hipEventCreate(&a));
+        QE_HIP(hipEventCreate(&b));
+        QE_HIP(hipEventRecord(a, ctx->stream));
+        map_gather_kernel<<<grid_for(std::max<uint64_t>(n / 2, 1), 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(
+            keys, n, kmin, range, map, static_cast<u32x4*>(sink));
+        QE_HIP(hipGetLastError());
+        QE_HIP(hipEventRecord(b, ctx->stream));
+        QE_HIP(hipEventSynchronize(b));
+        float t = 0;
+        QE_HIP(hipEventElapsedTime(&t, a, b));
+        hipEventDestroy(a);
+        hipEventDestroy(b);
+        if (ms) *ms = t;
+    });
+}
+
 qe_gpu_status qe_gpu_memcpy_h2d(qe_gpu_ctx* ctx, void* dst, const void* src, uint64_t bytes) {
     QE_ENTRY(ctx, {
         if (bytes) QE_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));

@@ -419,13 +419,22 @@ __global__ void emit_kernel(const uint64_t* st, uint64_t S, AggProg p, KeyEmit k
 // wave writing its 64 rows' validity word per column straight from a ballot
 // and counting NULLs into out[1 + c]; the host reads the counts back together.
 constexpr uint32_t kEmitSmall = 32768;
+// out[0] = the group count; out[1 .. nzero) are cleared for the emission's
+// NULL counts; out[nzero], out[nzero + 1] receive *flag and *ovf (0 for a null
+// pointer) so that one copy brings every scalar back.
 __global__ void __launch_bounds__(1024) group_rows_kernel(const uint64_t* st, uint32_t S, int all_slots,
-                                                          uint32_t* slot_of, unsigned long long* out) {
+                                                          uint32_t* slot_of, unsigned long long* out, uint32_t nzero,
+                                                          const unsigned int* flag, const unsigned int* ovf) {
     __shared__ uint32_t wcnt[kMaxWaves];
     __shared__ uint32_t base_s;
     const unsigned lane = lane_id(), w = threadIdx.x / kWave;
     const uint64_t below = (uint64_t(1) << lane) - 1;
     if (threadIdx.x == 0) base_s = 0;
+    for (uint32_t i = 1 + threadIdx.x; i < nzero; i += blockDim.x) out[i] = 0;
+    if (threadIdx.x == 0 && nzero) {
+        out[nzero] = flag ? *flag : 0u;
+        out[nzero + 1] = ovf ? *ovf : 0u;
+    }
     __syncthreads();
     for (uint32_t c0 = 0; c0 < S; c0 += 1024) {
         const uint32_t s = c0 + threadIdx.x;
@@ -495,21 +504,19 @@ static qe_dev_table* emit_groups_small(qe_gpu_ctx* ctx, const uint64_t* st, uint
             oc.type[c] = type;
             vw.w[c] = cols.back()->validity;
         }
-        Scratch res(ctx, size_t(1 + ncol) * 8), slot_of(ctx, size_t(S) * 4);
-        memset_async(ctx, res.p, 0, size_t(1 + ncol) * 8);
+        // res: group count | NULL count per column | *flag | *fovf
+        Scratch res(ctx, size_t(3 + ncol) * 8), slot_of(ctx, size_t(S) * 4);
         group_rows_kernel<<<1, 1024, 0, ctx->stream>>>(st, S, all_slots ? 1 : 0, slot_of.as<uint32_t>(),
-                                                       res.as<unsigned long long>());
+                                                       res.as<unsigned long long>(), static_cast<uint32_t>(1 + ncol),
+                                                       flag, p.fovf);
         emit_rows_kernel<<<(S + 255) / 256, 256, 0, ctx->stream>>>(st, S, p, ke, slot_of.as<uint32_t>(), oc, vw,
                                                                    res.as<unsigned long long>());
         QE_HIP(hipGetLastError());
         uint64_t* h = static_cast<uint64_t*>(pinned(ctx, size_t(3 + ncol) * 8));
-        h[2 + ncol] = 0;
-        QE_HIP(hipMemcpyAsync(h, res.p, size_t(1 + ncol) * 8, hipMemcpyDeviceToHost, ctx->stream));
-        if (flag) QE_HIP(hipMemcpyAsync(h + 1 + ncol, flag, 4, hipMemcpyDeviceToHost, ctx->stream));
-        if (p.fovf) QE_HIP(hipMemcpyAsync(h + 2 + ncol, p.fovf, 4, hipMemcpyDeviceToHost, ctx->stream));
+        QE_HIP(hipMemcpyAsync(h, res.p, size_t(3 + ncol) * 8, hipMemcpyDeviceToHost, ctx->stream));
         QE_HIP(hipStreamSynchronize(ctx->stream));
-        if (flag_out) *flag_out = flag ? *reinterpret_cast<const unsigned int*>(h + 1 + ncol) : 0u;
-        if (*reinterpret_cast<const unsigned int*>(h + 2 + ncol)) throw ScaleRetry();
+        if (flag_out) *flag_out = static_cast<unsigned int>(h[1 + ncol]);
+        if (h[2 + ncol]) throw ScaleRetry();
         const uint64_t G = h[0];
         qe_dev_table* t = new_table(G);
         for (int c = 0; c < ncol; ++c) {

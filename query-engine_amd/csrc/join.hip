@@ -991,18 +991,27 @@ __global__ void __launch_bounds__(1024) map_part_scatter(ColView key, uint64_t n
 // Scatter for 16-bit maps (2^pb <= 2^16 keys per partition): the tile is sorted
 // by partition in LDS first, then each partition's run is written contiguously.
 // Entry = (key offset within the partition) << 16 | group slot.
+// Two layouts of the entries:
+//  * offs != nullptr: partition-major runs at the offsets of a scan of the
+//    per-(partition, tile) counts (any key distribution);
+//  * gcur != nullptr: a fixed bucket of 2^pb entries per partition (as many
+//    as the partition has distinct keys), each block claiming its runs with
+//    one atomic per partition on gcur; no count pass and no scan. A bucket
+//    that would overflow can only hold repeated keys: the excess is dropped
+//    and the build kernel reports duplicates from gcur.
 template <int KW, class Slots>
 __global__ void __launch_bounds__(1024) map_part_scatter16(ColView key, uint64_t n, int64_t kmin, uint32_t pb,
                                                           uint32_t nparts, uint32_t ntiles, const uint64_t* offs,
-                                                          Slots slots, uint32_t* ent) {
+                                                          uint32_t* gcur, Slots slots, uint32_t* ent) {
     extern __shared__ __align__(16) char msm[];
     uint64_t* gbase = reinterpret_cast<uint64_t*>(msm);                 // [nparts]
     uint32_t* stage = reinterpret_cast<uint32_t*>(gbase + nparts);      // [kMapTile]
     uint16_t* spart = reinterpret_cast<uint16_t*>(stage + kMapTile);    // [kMapTile]
     uint32_t* cnt = reinterpret_cast<uint32_t*>(spart + kMapTile);      // [nparts] counts, then local bases
     uint32_t* ws = cnt + nparts;                                        // [kMaxWaves + 1]
+    const bool fixed = gcur != nullptr;
     for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x) {
-        gbase[i] = offs[size_t(i) * ntiles + blockIdx.x];
+        gbase[i] = fixed ? 0 : offs[size_t(i) * ntiles + blockIdx.x];
         cnt[i] = 0;
     }
     __syncthreads();
@@ -1035,6 +1044,12 @@ __global__ void __launch_bounds__(1024) map_part_scatter16(ColView key, uint64_t
         }
     }
     __syncthreads();
+    if (fixed) {  // claim this tile's run in each partition's bucket
+        for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x) {
+            const uint32_t c = cnt[i];
+            if (c) gbase[i] = atomicAdd(gcur + i, c);
+        }
+    }
     const uint32_t total = block_exclusive_scan(cnt, nparts, ws);
 #pragma unroll
     for (int k = 0; k < kRpt; ++k) {
@@ -1044,16 +1059,29 @@ __global__ void __launch_bounds__(1024) map_part_scatter16(ColView key, uint64_t
         spart[at] = static_cast<uint16_t>(part[k]);
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
-        const uint32_t pp = spart[i];
-        ent[gbase[pp] + (i - cnt[pp])] = stage[i];
+    if (fixed) {
+        const uint64_t cap = uint64_t(1) << pb;
+        for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
+            const uint32_t pp = spart[i];
+            const uint64_t within = gbase[pp] + (i - cnt[pp]);
+            if (within < cap) ent[(uint64_t(pp) << pb) + within] = stage[i];
+        }
+    } else {
+        for (uint32_t i = threadIdx.x; i < total; i += blockDim.x) {
+            const uint32_t pp = spart[i];
+            ent[gbase[pp] + (i - cnt[pp])] = stage[i];
+        }
     }
 }
 
+// One block per partition: its entries (a run of the scan layout, or its
+// fixed bucket when gcur != nullptr) land in an LDS copy of the partition's
+// map range, which is then written out whole. A key seen twice (or a bucket
+// that overflowed) sets *dup.
 template <typename MapT, typename EntT>
-__global__ void __launch_bounds__(1024) map_part_build(const EntT* ent, const uint64_t* offs, uint32_t ntiles,
-                                                       uint32_t nparts, uint64_t total, uint32_t pb, uint64_t range,
-                                                       MapT* gmap, unsigned int* dup) {
+__global__ void __launch_bounds__(1024) map_part_build(const EntT* ent, const uint64_t* offs, const uint32_t* gcur,
+                                                       uint32_t ntiles, uint32_t nparts, uint64_t total, uint32_t pb,
+                                                       uint64_t range, MapT* gmap, unsigned int* dup) {
     extern __shared__ __align__(16) char msm[];
     const uint32_t P = 1u << pb;
     MapT* m = reinterpret_cast<MapT*>(msm);
@@ -1062,9 +1090,17 @@ __global__ void __launch_bounds__(1024) map_part_build(const EntT* ent, const ui
     for (uint32_t i = threadIdx.x; i < P / 32; i += blockDim.x) bm[i] = 0;
     __syncthreads();
     const uint32_t part = blockIdx.x;
-    const uint64_t b = offs[size_t(part) * ntiles];
-    const uint64_t e = (part + 1 < nparts) ? offs[size_t(part + 1) * ntiles] : total;
+    uint64_t b, e;
     bool d = false;
+    if (gcur) {
+        const uint64_t c = gcur[part];
+        b = uint64_t(part) << pb;
+        e = b + min<uint64_t>(c, P);
+        d = c > P;
+    } else {
+        b = offs[size_t(part) * ntiles];
+        e = (part + 1 < nparts) ? offs[size_t(part + 1) * ntiles] : total;
+    }
     for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
         const uint64_t v = ent[i];
         constexpr int kShift = sizeof(EntT) == 4 ? 16 : 32;  // offset | slot packing of the scatter
@@ -1093,26 +1129,29 @@ struct MapSlots {
 
 template <int KW>
 static void launch_scatter16(qe_gpu_ctx* ctx, uint32_t ntiles, size_t lds, const ColView& kv, uint64_t nb,
-                             int64_t kmin, uint32_t pb, uint32_t nparts, const uint64_t* offs, const MapSlots& ms,
-                             uint32_t* ent) {
+                             int64_t kmin, uint32_t pb, uint32_t nparts, const uint64_t* offs, uint32_t* gcur,
+                             const MapSlots& ms, uint32_t* ent) {
     if (ms.gslot) {
         map_part_scatter16<KW, SlotArray><<<ntiles, 1024, lds, ctx->stream>>>(kv, nb, kmin, pb, nparts, ntiles, offs,
-                                                                             SlotArray{ms.gslot}, ent);
+                                                                             gcur, SlotArray{ms.gslot}, ent);
         return;
     }
     const uint64_t* gv = ms.g->all_valid ? nullptr : ms.g->validity;
     switch (type_width(ms.g->type)) {
         case 1:
             map_part_scatter16<KW, SlotDirect<1>><<<ntiles, 1024, lds, ctx->stream>>>(
-                kv, nb, kmin, pb, nparts, ntiles, offs, SlotDirect<1>{ms.g->data, gv, ms.gmin, ms.null_slot}, ent);
+                kv, nb, kmin, pb, nparts, ntiles, offs, gcur, SlotDirect<1>{ms.g->data, gv, ms.gmin, ms.null_slot},
+                ent);
             break;
         case 4:
             map_part_scatter16<KW, SlotDirect<4>><<<ntiles, 1024, lds, ctx->stream>>>(
-                kv, nb, kmin, pb, nparts, ntiles, offs, SlotDirect<4>{ms.g->data, gv, ms.gmin, ms.null_slot}, ent);
+                kv, nb, kmin, pb, nparts, ntiles, offs, gcur, SlotDirect<4>{ms.g->data, gv, ms.gmin, ms.null_slot},
+                ent);
             break;
         default:
             map_part_scatter16<KW, SlotDirect<8>><<<ntiles, 1024, lds, ctx->stream>>>(
-                kv, nb, kmin, pb, nparts, ntiles, offs, SlotDirect<8>{ms.g->data, gv, ms.gmin, ms.null_slot}, ent);
+                kv, nb, kmin, pb, nparts, ntiles, offs, gcur, SlotDirect<8>{ms.g->data, gv, ms.gmin, ms.null_slot},
+                ent);
     }
 }
 
@@ -1131,10 +1170,37 @@ MapBuild build_direct_map(qe_gpu_ctx* ctx, const qe_dev_col* key, uint64_t nb, u
     const uint32_t nparts = static_cast<uint32_t>(np64);
     const uint32_t ntiles = static_cast<uint32_t>((nb + kMapTile - 1) / kMapTile);
     const uint64_t nh = uint64_t(nparts) * ntiles;
-    Scratch hist(ctx, nh * 4), offs(ctx, nh * 8), sums(ctx, scan_sums_words(nh) * 8);
     dup = Scratch(ctx, 4);
     const ColView kv = view(key);
     const int kw = static_cast<int>(type_width(key->type));
+    constexpr bool k16 = sizeof(MapT) == 2;  // 4-byte entries, LDS-sorted scatter
+    const size_t build_lds = (size_t(1) << pb) * sizeof(MapT) + ((size_t(1) << pb) / 32) * 4;
+    if (k16 && range <= 2 * total) {
+        // Dense keys: fixed buckets of 2^pb entries per partition (at most 2x
+        // the entries of a scanned layout), so no count pass and no scan.
+        Scratch gcur(ctx, size_t(nparts) * 4);
+        Scratch ent(ctx, (size_t(nparts) << pb) * 4);
+        memset_async(ctx, gcur.p, 0, size_t(nparts) * 4);
+        const size_t lds = size_t(nparts) * 12 + size_t(kMapTile) * 6 + (kMaxWaves + 1) * 4;
+        if (kw == 8)
+            launch_scatter16<8>(ctx, ntiles, lds, kv, nb, kmin, pb, nparts, nullptr, gcur.as<uint32_t>(), ms,
+                                 ent.as<uint32_t>());
+        else if (kw == 4)
+            launch_scatter16<4>(ctx, ntiles, lds, kv, nb, kmin, pb, nparts, nullptr, gcur.as<uint32_t>(), ms,
+                                 ent.as<uint32_t>());
+        else
+            launch_scatter16<1>(ctx, ntiles, lds, kv, nb, kmin, pb, nparts, nullptr, gcur.as<uint32_t>(), ms,
+                                 ent.as<uint32_t>());
+        QE_HIP(hipGetLastError());
+        gmap = Scratch(ctx, range * sizeof(MapT));
+        memset_async(ctx, dup.p, 0, 4);
+        map_part_build<MapT, uint32_t><<<nparts, 1024, build_lds, ctx->stream>>>(
+            ent.as<uint32_t>(), nullptr, gcur.as<uint32_t>(), ntiles, nparts, total, pb, range, gmap.as<MapT>(),
+            dup.as<unsigned int>());
+        QE_HIP(hipGetLastError());
+        return MapBuild::kOk;
+    }
+    Scratch hist(ctx, nh * 4), offs(ctx, nh * 8), sums(ctx, scan_sums_words(nh) * 8);
     if (kw == 8)
         map_part_hist<8><<<ntiles, 1024, nparts * 4, ctx->stream>>>(kv, nb, kmin, pb, nparts, ntiles,
                                                                     hist.as<uint32_t>());
@@ -1146,18 +1212,17 @@ MapBuild build_direct_map(qe_gpu_ctx* ctx, const qe_dev_col* key, uint64_t nb, u
                                                                     hist.as<uint32_t>());
     QE_HIP(hipGetLastError());
     exclusive_scan_u32_dev(ctx, hist.as<uint32_t>(), offs.as<uint64_t>(), nh, sums.as<uint64_t>());
-    constexpr bool k16 = sizeof(MapT) == 2;  // 4-byte entries, LDS-sorted scatter
     Scratch ent(ctx, std::max<uint64_t>(total, 1) * (k16 ? 4 : 8));
     if (k16) {
         const size_t lds = size_t(nparts) * 12 + size_t(kMapTile) * 6 + (kMaxWaves + 1) * 4;
         if (kw == 8)
-            launch_scatter16<8>(ctx, ntiles, lds, kv, nb, kmin, pb, nparts, offs.as<uint64_t>(), ms,
+            launch_scatter16<8>(ctx, ntiles, lds, kv, nb, kmin, pb, nparts, offs.as<uint64_t>(), nullptr, ms,
                                  ent.as<uint32_t>());
         else if (kw == 4)
-            launch_scatter16<4>(ctx, ntiles, lds, kv, nb, kmin, pb, nparts, offs.as<uint64_t>(), ms,
+            launch_scatter16<4>(ctx, ntiles, lds, kv, nb, kmin, pb, nparts, offs.as<uint64_t>(), nullptr, ms,
                                  ent.as<uint32_t>());
         else
-            launch_scatter16<1>(ctx, ntiles, lds, kv, nb, kmin, pb, nparts, offs.as<uint64_t>(), ms,
+            launch_scatter16<1>(ctx, ntiles, lds, kv, nb, kmin, pb, nparts, offs.as<uint64_t>(), nullptr, ms,
                                  ent.as<uint32_t>());
     } else {
         invalid(ms.gslot != nullptr, "32-bit map build needs the slot array");
@@ -1168,14 +1233,13 @@ MapBuild build_direct_map(qe_gpu_ctx* ctx, const qe_dev_col* key, uint64_t nb, u
     QE_HIP(hipGetLastError());
     gmap = Scratch(ctx, range * sizeof(MapT));
     memset_async(ctx, dup.p, 0, 4);
-    const size_t lds = (size_t(1) << pb) * sizeof(MapT) + ((size_t(1) << pb) / 32) * 4;
     if (k16)
-        map_part_build<MapT, uint32_t><<<nparts, 1024, lds, ctx->stream>>>(
-            ent.as<uint32_t>(), offs.as<uint64_t>(), ntiles, nparts, total, pb, range, gmap.as<MapT>(),
+        map_part_build<MapT, uint32_t><<<nparts, 1024, build_lds, ctx->stream>>>(
+            ent.as<uint32_t>(), offs.as<uint64_t>(), nullptr, ntiles, nparts, total, pb, range, gmap.as<MapT>(),
             dup.as<unsigned int>());
     else
-        map_part_build<MapT, uint64_t><<<nparts, 1024, lds, ctx->stream>>>(
-            ent.as<uint64_t>(), offs.as<uint64_t>(), ntiles, nparts, total, pb, range, gmap.as<MapT>(),
+        map_part_build<MapT, uint64_t><<<nparts, 1024, build_lds, ctx->stream>>>(
+            ent.as<uint64_t>(), offs.as<uint64_t>(), nullptr, ntiles, nparts, total, pb, range, gmap.as<MapT>(),
             dup.as<unsigned int>());
     QE_HIP(hipGetLastError());
     return MapBuild::kOk;
