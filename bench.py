@@ -213,9 +213,13 @@ def asof_workload(qe, ctx, args):
     res = {"right_rows": rn, "left_rows": ln, "build_sort_s": t1 - t0, "train_s": t2 - t1,
            "piece_rows": args.piece_rows, "train_steps": args.train_steps, "bucket_knots": args.bucket_knots}
     results = {}
+    srt = qe.ASOF_OUT_SORTED
     for name, mode, timer in (("binary", qe.ASOF_BINARY, "asof_probe_binary"),
                               ("learned", qe.ASOF_LEARNED, "asof_probe_learned"),
-                              ("learned_no_mlp", qe.ASOF_LEARNED_NO_MLP, "asof_probe_learned_no_mlp")):
+                              ("learned_no_mlp", qe.ASOF_LEARNED_NO_MLP, "asof_probe_learned_no_mlp"),
+                              ("binary_sorted", qe.ASOF_BINARY | srt, "asof_probe_binary"),
+                              ("learned_sorted", qe.ASOF_LEARNED | srt, "asof_probe_learned"),
+                              ("learned_no_mlp_sorted", qe.ASOF_LEARNED_NO_MLP | srt, "asof_probe_learned_no_mlp")):
         for _ in range(2):
             idx.probe_into([ls], lt, out, mode=mode)
         ctx.set_timing(True)
@@ -229,13 +233,18 @@ def asof_workload(qe, ctx, args):
         res[f"{name}_probes_per_s"] = ln / (ms / max(n, 1) / 1e3)
     st = idx.stats()
     res["learned_equals_binary"] = bool((results["binary"] == results["learned"]).all()
-                                        and (results["binary"] == results["learned_no_mlp"]).all())
+                                        and (results["binary"] == results["learned_no_mlp"]).all()
+                                        and (results["binary_sorted"] == results["learned_sorted"]).all()
+                                        and (results["binary_sorted"] == results["learned_no_mlp_sorted"]).all())
     res["speedup_learned_vs_binary"] = res["binary_ms"] / res["learned_ms"]
     res["speedup_mlp_vs_no_mlp"] = res["learned_no_mlp_ms"] / res["learned_ms"]
+    res["speedup_learned_vs_binary_sorted_output"] = res["binary_sorted_ms"] / res["learned_sorted_ms"]
+    res["speedup_mlp_vs_no_mlp_sorted_output"] = res["learned_no_mlp_sorted_ms"] / res["learned_sorted_ms"]
     res["matched_frac"] = float((results["binary"] != qe.NO_ROW).mean())
     probe_bytes = ln * 16  # left sym 4 + ts 8 + match index 4
     res["learned_achieved_gbs"] = probe_bytes / (res["learned_ms"] / 1e3) / 1e9
     res["binary_achieved_gbs"] = probe_bytes / (res["binary_ms"] / 1e3) / 1e9
+    res["learned_sorted_achieved_gbs"] = probe_bytes / (res["learned_sorted_ms"] / 1e3) / 1e9
     res["stats"] = {k: (float(v) if isinstance(v, float) else int(v)) for k, v in st.items()}
     if not args.skip_cpu:
         res["cpu_baseline"] = cpu_baseline_asof(qe, ctx, idx, ls, lt, results["binary"], st, args)
@@ -651,7 +660,9 @@ def tick_workload(qe, ctx, args):
         if timed_stages:
             t = stamp("filter", t)
         m = f.rows
-        idx.probe_into([f.col(0)], f.col(1), match, mode=mode)
+        # Only which trades matched is used downstream (the gather takes
+        # trade columns), so the probe reports sorted positions.
+        idx.probe_into([f.col(0)], f.col(1), match, mode=mode | qe.ASOF_OUT_SORTED)
         pairs = ctx.asof_pairs(match, m, qe.INNER)
         if timed_stages:
             t = stamp("asof", t)
@@ -844,7 +855,8 @@ def main():
         secondary["asof"] = asof_workload(qe, ctx, args)
         a = secondary["asof"]
         log(f"as-of: binary {a['binary_ms']:.3f} ms, learned {a['learned_ms']:.3f} ms "
-            f"(x{a['speedup_learned_vs_binary']:.2f}), identical={a['learned_equals_binary']}")
+            f"(x{a['speedup_learned_vs_binary']:.2f}), identical={a['learned_equals_binary']}; sorted-position "
+            f"output: binary {a['binary_sorted_ms']:.3f} ms, learned {a['learned_sorted_ms']:.3f} ms")
         secondary["filter"] = filter_workload(qe, ctx, args)
         f = secondary["filter"]
         log(f"filter: {f['ms_per_step']:.3f} ms/step ({f['rows_per_s']:.3e} rows/s), mask {f['pred_mask_gbs']:.0f} GB/s, "

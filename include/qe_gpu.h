@@ -69,6 +69,14 @@ typedef enum qe_null_policy { QE_NULL_EQUAL = 0, QE_NULL_NEVER_MATCH = 1 } qe_nu
  * outward from the piece's predicted row, a verified knot-range search with a
  * whole-partition fallback for buckets of more than seven knots). */
 typedef enum qe_asof_mode { QE_ASOF_BINARY = 0, QE_ASOF_LEARNED = 1, QE_ASOF_LEARNED_NO_MLP = 2 } qe_asof_mode;
+/* Or'ed into the mode: report each match as its position in the index's
+ * sorted build order (qe_gpu_asof_sorted, qe_gpu_asof_take_sorted) instead of
+ * the original build row, which spares the probe one dependent random read.
+ * Enough for a caller that only needs which probes matched (an inner as-of
+ * feeding probe-side columns only), or that reads build columns it took into
+ * sorted order once with qe_gpu_asof_take_sorted. No reference counterpart:
+ * the reference's AsofJoin emits build rows. */
+#define QE_ASOF_OUT_SORTED 0x100
 
 /* Sentinel for "no match" in match-index outputs: kNoGroup / kNullBuildRow
  * (ops/hashtable.h:58, ops/join_internal.h:31). */
@@ -342,15 +350,26 @@ typedef struct qe_asof_stats {
     uint64_t pieces;           /* spline pieces                                  */
     uint64_t max_piece_error;  /* max recorded |predicted - actual| position     */
     uint64_t router_buckets;   /* radix buckets addressed by the router          */
-    double mean_bucket_knots;  /* mean knots searched per probe bucket (learned router: MLP where it helps) */
+    double mean_bucket_knots;  /* mean knots searched per probe bucket (learned router as probed) */
     double mean_bucket_knots_linear; /* same with the linear (no-MLP) router everywhere */
     uint64_t fallbacks;        /* probes whose piece was not in its bucket's knot range and was
                                   searched over the whole partition (last probe) */
     double final_loss;         /* mean router training loss over trained partitions */
     uint64_t mlp_candidates;   /* partitions with a trained router               */
     uint64_t mlp_partitions;   /* partitions whose learned probe uses the MLP    */
+    double mean_bucket_knots_mlp;  /* same with the trained MLP router in every trained partition */
+    /* The candidate router (MLP where it searches fewer knots) and the linear
+     * one timed on a sample of up to 2^21 build rows, in ms; the MLP is kept
+     * only if at least 2% faster (0 when no partition was a candidate). */
+    double sample_probe_ms_mlp;
+    double sample_probe_ms_linear;
 } qe_asof_stats;
 qe_gpu_status qe_gpu_asof_stats(qe_gpu_ctx* ctx, const qe_dev_asof* idx, qe_asof_stats* out);
+/* Build-side columns (of the build side's length) gathered into the index's
+ * sorted order: row i of the result is the build row at sorted position i,
+ * the rows a QE_ASOF_OUT_SORTED match addresses. */
+qe_gpu_status qe_gpu_asof_take_sorted(qe_gpu_ctx* ctx, const qe_dev_asof* idx, const qe_dev_col* const* cols,
+                                      uint32_t ncols, qe_dev_table** out);
 /* Sorted build side for inspection: sorted timestamps, original row of each
  * sorted position, and CSR partition offsets (partitions+1). Any may be NULL. */
 qe_gpu_status qe_gpu_asof_sorted(qe_gpu_ctx* ctx, const qe_dev_asof* idx, int64_t* ts, uint32_t* rows, uint64_t* offsets);

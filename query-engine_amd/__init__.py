@@ -27,6 +27,7 @@ NP_DTYPE = {I32: np.int32, I64: np.int64, F64: np.float64, BOOL: np.uint8, TS: n
 COUNT_STAR, COUNT, SUM, MIN, MAX, AVG = range(6)
 INNER, LEFT = 0, 1
 ASOF_BINARY, ASOF_LEARNED, ASOF_LEARNED_NO_MLP = 0, 1, 2
+ASOF_OUT_SORTED = 0x100  # or into a mode: matches as sorted positions of the index
 NO_ROW = 0xFFFFFFFF
 
 
@@ -76,7 +77,9 @@ class AsofStats(C.Structure):
                 ("max_piece_error", C.c_uint64), ("router_buckets", C.c_uint64),
                 ("mean_bucket_knots", C.c_double), ("mean_bucket_knots_linear", C.c_double),
                 ("fallbacks", C.c_uint64), ("final_loss", C.c_double),
-                ("mlp_candidates", C.c_uint64), ("mlp_partitions", C.c_uint64)]
+                ("mlp_candidates", C.c_uint64), ("mlp_partitions", C.c_uint64),
+                ("mean_bucket_knots_mlp", C.c_double), ("sample_probe_ms_mlp", C.c_double),
+                ("sample_probe_ms_linear", C.c_double)]
 
 
 P = C.c_void_p
@@ -138,6 +141,7 @@ _SIGS = {
     "qe_gpu_asof_train": (C.c_int, [P, P, C.c_uint32, C.c_uint32, C.c_float]),
     "qe_gpu_asof_probe": (C.c_int, [P, P, C.POINTER(P), C.c_uint32, P, C.c_int32, C.c_int64, C.c_int32, P]),
     "qe_gpu_asof_pairs": (C.c_int, [P, P, C.c_uint64, C.c_int32, C.POINTER(P)]),
+    "qe_gpu_asof_take_sorted": (C.c_int, [P, P, P, C.c_uint32, C.POINTER(P)]),
     "qe_gpu_asof_stats": (C.c_int, [P, P, C.POINTER(AsofStats)]),
     "qe_gpu_asof_sorted": (C.c_int, [P, P, P, P, P]),
     "qe_gpu_asof_free": (None, [P, P]),
@@ -741,6 +745,12 @@ class AsofIndex:
             return out.numpy(np.uint32, n)
         finally:
             out.free()
+
+    def take_sorted(self, cols) -> "Table":
+        """Build-side columns gathered into the index's sorted order."""
+        out = P()
+        self.ctx._check(lib().qe_gpu_asof_take_sorted(self.ctx.h, self.h, _cols(cols), len(cols), C.byref(out)))
+        return Table(self.ctx, out)
 
     def stats(self) -> dict:
         s = AsofStats()

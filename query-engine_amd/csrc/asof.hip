@@ -69,6 +69,7 @@ struct qe_dev_asof {
     Scratch part_of_slot;     // hash mode: slot -> dense partition id
     uint64_t nparts = 0;
     uint64_t n = 0;           // matchable build rows
+    uint64_t nrows_in = 0;    // build rows given (the original row ids index these)
     Scratch ts, row, offs;    // sorted timestamps, original rows, CSR (nparts+1)
     // learned index
     bool trained = false;
@@ -77,6 +78,9 @@ struct qe_dev_asof {
     Scratch meta, knots, tab_mlp, tab_lin, params, grid;
     Scratch bkt_mlp, bkt_lin;  // per router-table entry: the bucket's knots inline (BucketLine)
     double final_loss = 0, mean_knots_mlp = 0, mean_knots_lin = 0;
+    double mean_knots_cand = 0;             // the MLP router in every trained partition
+    double sample_ms_mlp = 0, sample_ms_lin = 0;  // router timing sample (0: not timed)
+    bool mlp_rejected = false;              // the timing kept the linear router
     uint64_t max_err = 0, mlp_candidates = 0, mlp_parts = 0;
     Scratch fallback;  // u64 counter of the last probe
     uint64_t last_fallbacks = 0;
@@ -213,6 +217,11 @@ struct DirectKeyPart {  // one integer key, partition = key - kmin
     }
 };
 
+struct GivenPart {  // partition ids given per probe (the router timing sample)
+    const uint32_t* ids;
+    __device__ uint32_t part(uint64_t r) const { return ids[r]; }
+};
+
 struct ProbeArgs {
     ColView time;
     uint64_t n;
@@ -220,6 +229,7 @@ struct ProbeArgs {
     const uint32_t* row;
     const uint64_t* offs;
     int has_tol;
+    int sorted_out;  // report the sorted position m instead of the original row row[m]
     uint64_t tol;
     uint32_t* out;
 };
@@ -228,7 +238,7 @@ __device__ inline void finish_match(const ProbeArgs& a, uint64_t j, int64_t t, u
     uint32_t res = kNoRow;
     if (found) {
         const uint64_t span = static_cast<uint64_t>(t) - static_cast<uint64_t>(a.ts[m]);
-        if (!a.has_tol || span <= a.tol) res = a.row[m];
+        if (!a.has_tol || span <= a.tol) res = a.sorted_out ? static_cast<uint32_t>(m) : a.row[m];
     }
     a.out[j] = res;
 }
@@ -486,7 +496,9 @@ __global__ void router_stats_kernel(const PartMeta* meta, const uint64_t* kb, ui
     if (lane_id() == 0) atomicAdd(total, local);
 }
 
-// Keep the MLP router where it searches fewer knots than the linear one.
+// Candidate learned router: the MLP where it searches fewer knots than the
+// linear one. Whether the candidate is kept is decided by timing it against
+// the linear router on a sample of the index's own rows (qe_gpu_asof_train).
 __global__ void router_choose_kernel(PartMeta* meta, uint64_t nparts, const double* occ_lin, const double* occ_mlp,
                                      unsigned long long* chosen) {
     uint64_t c = 0;
@@ -630,6 +642,32 @@ __global__ void asof_learned_kernel(PK pk, ProbeArgs a, LearnedArgs L) {
     }
     for (int off = kWave / 2; off > 0; off >>= 1) fb += __shfl_down(fb, off);
     if (lane_id() == 0 && fb) atomicAdd(L.fallbacks, static_cast<unsigned long long>(fb));
+}
+
+// Probes for timing the routers: n_s rows of the index drawn by a hashed
+// stride (unsorted, like real probes), each probing its own time, so every
+// probe matches and the sample follows the rows' distribution over partitions
+// and time.
+__global__ void router_sample_kernel(const uint64_t* offs, uint64_t nparts, const int64_t* ts, uint64_t n,
+                                     uint64_t n_s, uint32_t* ids, int64_t* t) {
+    for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < n_s;
+         i += uint64_t(gridDim.x) * blockDim.x) {
+        const uint64_t q = mix64(i * 0x9E3779B97F4A7C15ull + 0x51E57A11ull) % n;
+        uint64_t lo = 0, hi = nparts;  // offs[lo] <= q < offs[hi]
+        while (hi - lo > 1) {
+            const uint64_t mid = (lo + hi) / 2;
+            if (offs[mid] <= q) lo = mid;
+            else hi = mid;
+        }
+        ids[i] = static_cast<uint32_t>(lo);
+        t[i] = ts[q];
+    }
+}
+
+__global__ void clear_mlp_kernel(PartMeta* meta, uint64_t nparts) {
+    for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < nparts;
+         p += uint64_t(gridDim.x) * blockDim.x)
+        meta[p].mlp = 0;
 }
 
 // ---- MLP training (MFMA) ---------------------------------------------------------------
@@ -951,6 +989,7 @@ static qe_gpu_status asof_build_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* 
         check_rows32(n);
         auto* idx = new qe_dev_asof();
         try {
+            idx->nrows_in = n;
             idx->nkeys = static_cast<int>(nkeys);
             for (uint32_t j = 0; j < nkeys; ++j) idx->key_types[j] = build_keys[j]->type;
             if (nkeys == 0) {
@@ -1058,12 +1097,83 @@ qe_gpu_status qe_gpu_asof_build(qe_gpu_ctx* ctx, const qe_dev_col* const* build_
     });
 }
 
+}  // extern "C"
+
+namespace qe_gpu {
+
+// Times the candidate learned router (the MLP where it searches fewer knots)
+// against the linear router on a sample of the index's rows, best of three
+// launches each, and keeps the MLP only if it is at least 2% faster; else the
+// learned probe uses the linear router in every partition. Both timings stay
+// in the index's statistics.
+static void time_routers(qe_gpu_ctx* ctx, qe_dev_asof* idx, uint64_t P, uint64_t nk, uint64_t nt) {
+    const uint64_t ns = std::min<uint64_t>(idx->n, uint64_t(1) << 21);
+    if (ns == 0) return;
+    Scratch ids(ctx, ns * 4), tv(ctx, ns * 8), out(ctx, ns * 4), fb(ctx, 8);
+    router_sample_kernel<<<grid_for(ns, 256, ctx->num_cus * 8), 256, 0, ctx->stream>>>(
+        idx->offs.as<uint64_t>(), P, idx->ts.as<int64_t>(), idx->n, ns, ids.as<uint32_t>(), tv.as<int64_t>());
+    QE_HIP(hipGetLastError());
+    memset_async(ctx, fb.p, 0, 8);
+    ProbeArgs a{};
+    a.time = ColView{tv.p, nullptr, QE_I64, 0};
+    a.n = ns;
+    a.ts = idx->ts.as<int64_t>();
+    a.row = idx->row.as<uint32_t>();
+    a.offs = idx->offs.as<uint64_t>();
+    a.out = out.as<uint32_t>();
+    const GivenPart gp{ids.as<uint32_t>()};
+    const LearnedArgs Lm{idx->meta.as<PartMeta>(), idx->knots.as<Knot>(), idx->tab_mlp.as<uint32_t>(),
+                         idx->bkt_mlp.as<BucketLine>(), idx->grid.as<float>(), 1, idx->B,
+                         fb.as<unsigned long long>()};
+    LearnedArgs Ll = Lm;
+    Ll.tab = idx->tab_lin.as<uint32_t>();
+    Ll.bkt = idx->bkt_lin.as<BucketLine>();
+    Ll.use_mlp = 0;
+    const unsigned g = grid_for(ns, 256, ctx->num_cus * 32);
+    hipEvent_t e0, e1;
+    QE_HIP(hipEventCreate(&e0));
+    QE_HIP(hipEventCreate(&e1));
+    float best[2] = {1e30f, 1e30f};
+    for (int rep = 0; rep < 4; ++rep) {
+        for (int w = 0; w < 2; ++w) {
+            QE_HIP(hipEventRecord(e0, ctx->stream));
+            asof_learned_kernel<<<g, 256, 0, ctx->stream>>>(gp, a, w == 0 ? Lm : Ll);
+            QE_HIP(hipGetLastError());
+            QE_HIP(hipEventRecord(e1, ctx->stream));
+            QE_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            QE_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (rep > 0) best[w] = std::min(best[w], ms);  // the first round warms the caches
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    idx->sample_ms_mlp = best[0];
+    idx->sample_ms_lin = best[1];
+    if (best[0] > 0.98f * best[1]) {
+        clear_mlp_kernel<<<grid_for(P, 256, 4096), 256, 0, ctx->stream>>>(idx->meta.as<PartMeta>(), P);
+        QE_HIP(hipGetLastError());
+        QE_HIP(hipMemcpyAsync(idx->tab_mlp.p, idx->tab_lin.p, std::max<uint64_t>(nt, 1) * 4, hipMemcpyDeviceToDevice,
+                              ctx->stream));
+        QE_HIP(hipMemcpyAsync(idx->bkt_mlp.p, idx->bkt_lin.p, std::max<uint64_t>(nt, 1) * sizeof(BucketLine),
+                              hipMemcpyDeviceToDevice, ctx->stream));
+        idx->mlp_rejected = true;
+    }
+    (void)nk;
+}
+
+}  // namespace qe_gpu
+
+extern "C" {
+
 qe_gpu_status qe_gpu_asof_train(qe_gpu_ctx* ctx, qe_dev_asof* idx, uint32_t piece_rows, uint32_t steps, float lr) {
     QE_ENTRY(ctx, {
         invalid(idx != nullptr, "null index");
         invalid(piece_rows >= 2 && piece_rows <= 4096, "piece_rows must be in [2, 4096]");
         const uint64_t P = idx->nparts;
         idx->B = piece_rows;
+        idx->mlp_rejected = false;
+        idx->sample_ms_mlp = idx->sample_ms_lin = 0;
         Scratch K(ctx, P * 4), R1(ctx, P * 4), kb(ctx, (P + 1) * 8), tb(ctx, (P + 1) * 8);
         const uint32_t kpb = static_cast<uint32_t>(ctx->tune_asof_bucket_knots);
         idx->kpb = kpb;
@@ -1127,7 +1237,7 @@ qe_gpu_status qe_gpu_asof_train(qe_gpu_ctx* ctx, qe_dev_asof* idx, uint32_t piec
                                                              idx->tab_lin.as<uint32_t>(), occ_lin, sums.as<double>());
             router_choose_kernel<<<grid_for(P, 256, 4096), 256, 0, ctx->stream>>>(meta, P, occ_lin, occ_mlp,
                                                                                    chosen.as<unsigned long long>());
-            // Final learned router: the MLP only where it searches fewer knots.
+            // Candidate learned router: the MLP only where it searches fewer knots.
             router_table_kernel<<<gk, 256, 0, ctx->stream>>>(meta, kb.as<uint64_t>(), P, nk, knots, grid, 1,
                                                              idx->tab_mlp.as<uint32_t>());
             router_stats_kernel<<<gk, 256, 0, ctx->stream>>>(meta, kb.as<uint64_t>(), P, nk, knots, grid, 1,
@@ -1140,6 +1250,7 @@ qe_gpu_status qe_gpu_asof_train(qe_gpu_ctx* ctx, qe_dev_asof* idx, uint32_t piec
             bucket_lines_kernel<<<gt, 256, 0, ctx->stream>>>(meta, tb.as<uint64_t>(), P, nt, knots,
                                                              idx->tab_lin.as<uint32_t>(), idx->bkt_lin.as<BucketLine>());
             QE_HIP(hipGetLastError());
+            if (read_scalar(ctx, chosen.as<uint64_t>()) > 0) time_routers(ctx, idx, P, nk, nt);
         }
         std::vector<float> hl(std::max<uint64_t>(P, 1));
         std::vector<PartMeta> hm(std::max<uint64_t>(P, 1));
@@ -1162,6 +1273,11 @@ qe_gpu_status qe_gpu_asof_train(qe_gpu_ctx* ctx, qe_dev_asof* idx, uint32_t piec
         idx->final_loss = cand ? ls / cand : 0.0;
         idx->mean_knots_lin = nk ? hs[0] / nk : 0;
         idx->mean_knots_mlp = nk ? hs[1] / nk : 0;
+        idx->mean_knots_cand = nk ? hs[2] / nk : 0;
+        if (idx->mlp_rejected) {  // the learned probe routes linearly everywhere
+            idx->mlp_parts = 0;
+            idx->mean_knots_mlp = idx->mean_knots_lin;
+        }
         idx->fallback = Scratch(ctx, 8);
         idx->trained = true;
     });
@@ -1182,6 +1298,8 @@ static qe_gpu_status asof_probe_codes(qe_gpu_ctx* ctx, const qe_dev_asof* idx, c
             invalid(probe_keys[j] != nullptr && probe_keys[j]->len == probe_time->len, "bad probe key column");
             invalid(probe_keys[j]->type == idx->key_types[j], "AsofJoin: key column types must match positionally");
         }
+        const int sorted_out = (mode & QE_ASOF_OUT_SORTED) ? 1 : 0;
+        mode &= ~QE_ASOF_OUT_SORTED;
         invalid(mode >= QE_ASOF_BINARY && mode <= QE_ASOF_LEARNED_NO_MLP, "unknown as-of mode");
         check(mode == QE_ASOF_BINARY || idx->trained, QE_GPU_INVALID_ARGUMENT,
               "learned as-of probe needs qe_gpu_asof_train first");
@@ -1196,6 +1314,7 @@ static qe_gpu_status asof_probe_codes(qe_gpu_ctx* ctx, const qe_dev_asof* idx, c
         a.row = idx->row.as<uint32_t>();
         a.offs = idx->offs.as<uint64_t>();
         a.has_tol = has_tolerance ? 1 : 0;
+        a.sorted_out = sorted_out;
         a.tol = static_cast<uint64_t>(tolerance);
         a.out = match_out;
         const unsigned g = grid_for(n, 256, ctx->num_cus * 32);
@@ -1282,6 +1401,28 @@ qe_gpu_status qe_gpu_asof_pairs(qe_gpu_ctx* ctx, const uint32_t* match, uint64_t
     });
 }
 
+qe_gpu_status qe_gpu_asof_take_sorted(qe_gpu_ctx* ctx, const qe_dev_asof* idx, const qe_dev_col* const* cols,
+                                      uint32_t ncols, qe_dev_table** out) {
+    QE_ENTRY(ctx, {
+        invalid(idx != nullptr && out != nullptr && cols != nullptr && ncols >= 1, "null argument");
+        const uint64_t n = common_len(cols, ncols);
+        invalid(n == idx->nrows_in, "take_sorted needs columns of the build side's length");
+        const uint64_t m = idx->n;
+        qe_dev_table* t = new_table(m);
+        try {
+            std::vector<const uint32_t*> idxs(ncols, idx->row.as<uint32_t>());
+            if (m) t->cols = gather_cols(ctx, cols, idxs.data(), ncols, m);
+            else
+                for (uint32_t c = 0; c < ncols; ++c) t->cols.push_back(new_col(ctx, cols[c]->type, 0, false));
+        } catch (...) {
+            qe_gpu_table_free(ctx, t);
+            throw;
+        }
+        QE_HIP(hipStreamSynchronize(ctx->stream));
+        *out = t;
+    });
+}
+
 qe_gpu_status qe_gpu_asof_stats(qe_gpu_ctx* ctx, const qe_dev_asof* idx, qe_asof_stats* out) {
     QE_ENTRY(ctx, {
         invalid(idx != nullptr && out != nullptr, "null argument");
@@ -1296,6 +1437,9 @@ qe_gpu_status qe_gpu_asof_stats(qe_gpu_ctx* ctx, const qe_dev_asof* idx, qe_asof
         out->final_loss = idx->final_loss;
         out->mlp_candidates = idx->mlp_candidates;
         out->mlp_partitions = idx->mlp_parts;
+        out->mean_bucket_knots_mlp = idx->mean_knots_cand;
+        out->sample_probe_ms_mlp = idx->sample_ms_mlp;
+        out->sample_probe_ms_linear = idx->sample_ms_lin;
     });
 }
 

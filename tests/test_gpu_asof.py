@@ -242,8 +242,17 @@ def test_router_mlp_gain_on_session_shape(ctx, qe):
         hi = np.searchsorted(ssym, lsy[j], side="right")
         k = np.searchsorted(sts[lo:hi], lts[j], side="right")
         assert b[j] == (NO if k == 0 else order[lo + k - 1])
-    assert st["mlp_partitions"] > 0
-    assert st["mean_bucket_knots"] < 0.8 * st["mean_bucket_knots_linear"], st
+    # What the trained network gains: routed through it, a probe's bucket holds
+    # far fewer knots than under the linear router.
+    assert st["mean_bucket_knots_mlp"] < 0.8 * st["mean_bucket_knots_linear"], st
+    # The probe keeps the MLP only where the timing says it is faster.
+    assert st["sample_probe_ms_mlp"] > 0 and st["sample_probe_ms_linear"] > 0, st
+    if st["mlp_partitions"] > 0:
+        assert st["sample_probe_ms_mlp"] <= 0.98 * st["sample_probe_ms_linear"], st
+        assert st["mean_bucket_knots"] < st["mean_bucket_knots_linear"], st
+    else:
+        assert st["sample_probe_ms_mlp"] > 0.98 * st["sample_probe_ms_linear"], st
+        assert st["mean_bucket_knots"] == st["mean_bucket_knots_linear"], st
 
 
 @pytest.mark.parametrize("kpb", [2, 8, 64])
@@ -271,6 +280,70 @@ def test_coarse_router_buckets_stay_exact(ctx, qe, kpb):
         assert st["fallbacks"] == 0
         assert st["router_buckets"] < st["pieces"]
         assert st["mean_bucket_knots"] <= st["mean_bucket_knots_linear"]
+
+
+def test_sorted_position_output_and_take_sorted(ctx, qe):
+    # QE_ASOF_OUT_SORTED: every method reports the sorted position whose
+    # original row is the plain output's match, misses and tolerance drops in
+    # the same places; take_sorted lays build columns out in that order.
+    (rs, rt, rp), (ls, lt) = ctx.gen_asof(31, 1 << 21, 1 << 19, 300, 1.1)
+    try:
+        idx = ctx.asof_build([rs], rt)
+        try:
+            idx.train(64, 200, 0.003)
+            st = idx.stats()
+            sts, rows, offs = idx.sorted(st["build_rows"], st["partitions"])
+            got = {}
+            for mode in (qe.ASOF_BINARY, qe.ASOF_LEARNED, qe.ASOF_LEARNED_NO_MLP):
+                for tol in (None, 700):
+                    got[(mode, tol)] = (idx.probe([ls], lt, tolerance=tol, mode=mode),
+                                        idx.probe([ls], lt, tolerance=tol, mode=mode | qe.ASOF_OUT_SORTED))
+            t = idx.take_sorted([rp, rt])
+            tv = t.numpy()
+            t.free()
+        finally:
+            idx.free()
+        rpn, _ = rp.numpy()
+        rtn, _ = rt.numpy()
+    finally:
+        for c in (rs, rt, rp, ls, lt):
+            c.free()
+    for (mode, tol), (orig, pos) in got.items():
+        hit = pos != NO
+        assert np.array_equal(hit, orig != NO), (mode, tol)
+        assert np.array_equal(rows[pos[hit]], orig[hit]), (mode, tol)
+        assert (pos[hit] < len(rows)).all()
+    # Tolerance drops some matches and keeps others.
+    assert 0 < (got[(qe.ASOF_BINARY, 700)][0] != NO).sum() < (got[(qe.ASOF_BINARY, None)][0] != NO).sum()
+    assert np.array_equal(tv[0][0], rpn[rows]) and np.array_equal(tv[1][0], sts)
+    assert np.array_equal(tv[1][0], rtn[rows])
+
+
+def test_take_sorted_skips_unmatchable_rows(ctx, qe):
+    # Rows with a NULL key or time are not in the index: take_sorted returns
+    # the matchable rows only, in (key, time) order.
+    valid_k = np.array([1, 0, 1, 1, 1], dtype=bool)
+    valid_t = np.array([1, 1, 1, 1, 0], dtype=bool)
+    keys = ctx.upload(np.array([1, 5, 1, 2, 2], dtype=np.int32), I32, valid_k)
+    ts = ctx.upload(np.array([10, 5, 20, 7, 3], dtype=np.int64), I64, valid_t)
+    pay = ctx.upload(np.array([100, 101, 102, 103, 104], dtype=np.int64))
+    pk = ctx.upload(np.array([1, 2], dtype=np.int32), I32)
+    pt = ctx.upload(np.array([15, 9], dtype=np.int64), I64)
+    try:
+        idx = ctx.asof_build([keys], ts)
+        try:
+            t = idx.take_sorted([pay])
+            vals = t.numpy()[0][0].tolist()
+            t.free()
+            pos = idx.probe([pk], pt, mode=qe.ASOF_BINARY | qe.ASOF_OUT_SORTED)
+            orig = idx.probe([pk], pt, mode=qe.ASOF_BINARY)
+        finally:
+            idx.free()
+    finally:
+        for c in (keys, ts, pay, pk, pt):
+            c.free()
+    assert vals == [100, 102, 103]
+    assert [vals[p] for p in pos] == [100, 103] and orig.tolist() == [0, 3]
 
 
 def test_mlp_forward_matches_cpu(ctx):

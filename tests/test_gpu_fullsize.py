@@ -13,9 +13,9 @@
   pin it uniquely as the (sym, ts, row) order: CSR offsets, rows a
   permutation, (sym, ts) of rows[i] equal to the sorted arrays, ts
   non-decreasing and rows increasing among equal ts inside each partition.
-  Then the binary, learned and linear-router probes are compared with a
-  multi-threaded CPU upper_bound over the verified sorted arrays on every
-  probe.
+  Then the binary, learned and linear-router probes, each with original-row
+  and sorted-position output, are compared with a multi-threaded CPU
+  upper_bound over the verified sorted arrays on every probe.
 """
 import time
 
@@ -85,6 +85,7 @@ def test_asof_full_size_every_probe(ctx, qe):
             for name, mode in (("binary", qe.ASOF_BINARY), ("learned", qe.ASOF_LEARNED),
                                ("learned_no_mlp", qe.ASOF_LEARNED_NO_MLP)):
                 res[name] = idx.probe([ls], lt, mode=mode)
+                res[name + "_sorted"] = idx.probe([ls], lt, mode=mode | qe.ASOF_OUT_SORTED)
             st = idx.stats()
             nparts = int(st["partitions"])
             sts, rows, offs = idx.sorted(right_n, nparts)
@@ -141,5 +142,6 @@ def test_asof_full_size_every_probe(ctx, qe):
     want[hit] = rows[cpu[hit]]
     assert 0.5 * left_n < hit.sum() <= left_n
     for name, got in res.items():
-        bad = np.flatnonzero(got != want)
+        # Sorted-position outputs are the CPU search's own indices.
+        bad = np.flatnonzero(got != (cpu if name.endswith("_sorted") else want))
         assert bad.size == 0, f"{name}: {bad.size} probes differ, first at {bad[:5].tolist()}"
