@@ -641,7 +641,9 @@ def tick_workload(qe, ctx, args):
     idx = ctx.asof_build([qs], qt)
     ctx.sync()
     t1 = time.perf_counter()
+    ctx.set_tuning(qe.TUNE_ASOF_BUCKET_KNOTS, args.bucket_knots)
     idx.train(args.piece_rows, args.train_steps, args.train_lr)
+    ctx.set_tuning(qe.TUNE_ASOF_BUCKET_KNOTS, 1)
     ctx.sync()
     t2 = time.perf_counter()
     match = qe.DevBuffer(ctx, 4 * nt)
@@ -795,10 +797,11 @@ def main():
     ap.add_argument("--asof-right", type=int, default=400 << 20)
     ap.add_argument("--asof-left", type=int, default=100 << 20)
     ap.add_argument("--asof-reps", type=int, default=5)
-    ap.add_argument("--piece-rows", type=int, default=64)
+    ap.add_argument("--piece-rows", type=int, default=16,
+                    help="rows per spline piece (pieces of 16 rows fill one 128-byte line of timestamps)")
     ap.add_argument("--train-steps", type=int, default=300)
     ap.add_argument("--train-lr", type=float, default=0.003)
-    ap.add_argument("--bucket-knots", type=int, default=1, help="spline knots per learned-router bucket")
+    ap.add_argument("--bucket-knots", type=int, default=2, help="spline knots per learned-router bucket")
     ap.add_argument("--asof-sessions", type=int, default=0,
                     help="0: bursty exponential gaps (the tick shape); >0: trading days with U-shaped activity")
     ap.add_argument("--cpu-asof-log2", type=int, default=22)

@@ -334,14 +334,32 @@ __device__ inline uint32_t bucket_of(const PartMeta& pm, const float* W, bool ml
     return static_cast<uint32_t>(b);
 }
 
-// Knots K = ceil(n / B) and buckets R = ceil(K / kpb) per partition.
+// Pieces cut a partition's sorted rows at the multiples of B of the whole
+// sorted array, so a piece of at most 16 rows lies inside one 128-byte line of
+// timestamps (the array starts 256-byte aligned) and its search reads that
+// line alone. Piece j of a partition of n rows starting at seg covers
+// [max(seg, (seg / B + j) B), min(seg + n, (seg / B + j + 1) B)); the first and
+// last may be short.
+__device__ inline uint32_t pieces_of(uint64_t seg, uint64_t n, uint32_t B) {
+    return n ? static_cast<uint32_t>((seg + n + B - 1) / B - seg / B) : 0u;
+}
+__device__ inline uint64_t piece_start(uint64_t seg, uint32_t j, uint32_t B) {
+    const uint64_t s = (seg / B + j) * B;
+    return s > seg ? s : seg;
+}
+__device__ inline uint32_t piece_count(uint64_t seg, uint64_t n, uint32_t j, uint32_t B) {
+    const uint64_t e = min(seg + n, (seg / B + j + 1) * B);
+    return static_cast<uint32_t>(e - piece_start(seg, j, B));
+}
+
+// Knots K (pieces, as above) and buckets R = ceil(K / kpb) per partition.
 __device__ inline uint32_t buckets_for(uint32_t k, uint32_t kpb) { return (k + kpb - 1) / kpb; }
 __global__ void meta_counts_kernel(const uint64_t* offs, uint64_t nparts, uint32_t B, uint32_t kpb, uint32_t* K,
                                    uint32_t* R1) {
     for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < nparts;
          p += uint64_t(gridDim.x) * blockDim.x) {
         const uint64_t n = offs[p + 1] - offs[p];
-        const uint32_t k = static_cast<uint32_t>((n + B - 1) / B);
+        const uint32_t k = pieces_of(offs[p], n, B);
         K[p] = k;
         R1[p] = k ? buckets_for(k, kpb) + 1 : 0;  // R buckets -> R+1 table entries
     }
@@ -361,7 +379,7 @@ __global__ void meta_fill_kernel(const uint64_t* offs, uint64_t nparts, uint32_t
         m.mlp = 0;
         if (m.K) {
             m.t0 = ts[m.seg];
-            const int64_t tl = ts[m.seg + uint64_t(m.K - 1) * B];
+            const int64_t tl = ts[piece_start(m.seg, m.K - 1, B)];
             const uint64_t span = static_cast<uint64_t>(tl) - static_cast<uint64_t>(m.t0);
             m.inv_span = span ? 1.0 / static_cast<double>(span) : 0.0;
             if (!span) m.mlp_ok = 0;
@@ -390,8 +408,8 @@ __global__ void pieces_kernel(const PartMeta* meta, const uint64_t* kb, uint64_t
         const uint32_t p = part_of_knot(kb, nparts + 1, g);
         const PartMeta pm = meta[p];
         const uint32_t j = static_cast<uint32_t>(g - pm.kb);
-        const uint64_t pos0 = pm.seg + uint64_t(j) * B;
-        const uint32_t cnt = min(B, pm.n - j * B);
+        const uint64_t pos0 = piece_start(pm.seg, j, B);
+        const uint32_t cnt = piece_count(pm.seg, pm.n, j, B);
         const int64_t t0 = ts[pos0], tl = ts[pos0 + cnt - 1];
         const uint64_t span = static_cast<uint64_t>(tl) - static_cast<uint64_t>(t0);
         const float s = (cnt > 1 && span) ? static_cast<float>(static_cast<double>(cnt - 1) / static_cast<double>(span))
@@ -633,8 +651,8 @@ __global__ void asof_learned_kernel(PK pk, ProbeArgs a, LearnedArgs L) {
         }
         // 2. inside the piece, search outward from the spline's prediction; the
         //    result is exact wherever the prediction lands.
-        const uint64_t base = pm.seg + uint64_t(piece) * L.B;
-        const uint32_t cnt = min(L.B, pm.n - piece * L.B);
+        const uint64_t base = piece_start(pm.seg, piece, L.B);
+        const uint32_t cnt = piece_count(pm.seg, pm.n, piece, L.B);
         const uint32_t pred = piece_pred(tau, slope, cnt, t);
         const uint32_t c = gallop_count(a.ts, base, cnt, pred, t);
         // tau[piece] <= t guarantees c >= 1.
