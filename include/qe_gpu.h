@@ -116,6 +116,9 @@ qe_gpu_status qe_gpu_sync(qe_gpu_ctx* ctx);
  * buffers may not exceed it (cached free blocks are released first when a new
  * allocation would pass it). Lets a test force the OUT_OF_MEMORY path. */
 qe_gpu_status qe_gpu_set_mem_limit(qe_gpu_ctx* ctx, uint64_t bytes);
+/* Bytes in the context's live device buffers and bytes it holds from the
+ * device (live plus cached free blocks). */
+qe_gpu_status qe_gpu_mem_usage(qe_gpu_ctx* ctx, uint64_t* live_bytes, uint64_t* held_bytes);
 /* Kernel timing: when enabled, the dominant kernel of each call is bracketed by
  * HIP events on the context stream; qe_gpu_timing reads the accumulated time
  * (ms) and launch count per kernel name, then clears it. */

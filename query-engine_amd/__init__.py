@@ -89,6 +89,7 @@ _SIGS = {
     "qe_gpu_last_error": (C.c_char_p, [P]),
     "qe_gpu_sync": (C.c_int, [P]),
     "qe_gpu_set_mem_limit": (C.c_int, [P, C.c_uint64]),
+    "qe_gpu_mem_usage": (C.c_int, [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     "qe_gpu_set_timing": (C.c_int, [P, C.c_int]),
     "qe_gpu_filter": (C.c_int, [P, P, C.c_uint32, P, C.c_uint32, C.POINTER(P)]),
     "qe_gpu_filter_select": (C.c_int, [P, P, C.c_uint32, P, C.c_uint32, P, C.POINTER(C.c_uint64)]),
@@ -512,6 +513,12 @@ class Context:
 
     def set_mem_limit(self, nbytes: int):
         self._check(lib().qe_gpu_set_mem_limit(self.h, nbytes))
+
+    def mem_usage(self) -> tuple[int, int]:
+        """(bytes in live device buffers, bytes held from the device incl. cached blocks)."""
+        live, held = C.c_uint64(), C.c_uint64()
+        self._check(lib().qe_gpu_mem_usage(self.h, C.byref(live), C.byref(held)))
+        return live.value, held.value
 
     # ---- operators ---------------------------------------------------------
     def aggregate(self, cols, keys, aggs) -> Table:

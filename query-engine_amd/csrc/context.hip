@@ -55,6 +55,17 @@ void* dev_alloc(qe_gpu_ctx* ctx, size_t bytes) {
     return p;
 }
 
+bool device_room(qe_gpu_ctx* ctx, uint64_t bytes) {
+    if (ctx->mem_limit && ctx->live_bytes + bytes > ctx->mem_limit) return false;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    const uint64_t cached = ctx->mem_in_use - ctx->live_bytes;  // released on demand by dev_alloc
+    return uint64_t(free_b) + cached >= bytes + (uint64_t(256) << 20);
+}
+
 void dev_free(qe_gpu_ctx* ctx, void* p) {
     if (!p) return;
     auto it = ctx->live_blocks.find(p);
@@ -583,6 +594,13 @@ const char* qe_gpu_last_error(const qe_gpu_ctx* ctx) {
 
 qe_gpu_status qe_gpu_sync(qe_gpu_ctx* ctx) {
     QE_ENTRY(ctx, QE_HIP(hipStreamSynchronize(ctx->stream)));
+}
+
+qe_gpu_status qe_gpu_mem_usage(qe_gpu_ctx* ctx, uint64_t* live_bytes, uint64_t* held_bytes) {
+    QE_ENTRY(ctx, {
+        if (live_bytes) *live_bytes = ctx->live_bytes;
+        if (held_bytes) *held_bytes = ctx->mem_in_use;
+    });
 }
 
 qe_gpu_status qe_gpu_set_mem_limit(qe_gpu_ctx* ctx, uint64_t bytes) {

@@ -130,6 +130,10 @@ namespace qe_gpu {
 // ---- caching allocator (per context, stream-ordered use only) ---------------
 void* dev_alloc(qe_gpu_ctx* ctx, size_t bytes);
 void dev_free(qe_gpu_ctx* ctx, void* p);
+// Whether `bytes` more of device memory can be taken now: within the context's
+// limit, and free on the device (counting this context's cached blocks) with
+// 256 MiB to spare. For optional working sets that have a leaner fallback.
+bool device_room(qe_gpu_ctx* ctx, uint64_t bytes);
 
 // RAII scratch buffer.
 struct Scratch {

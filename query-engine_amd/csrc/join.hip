@@ -1674,19 +1674,37 @@ qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* pro
                                                  ? ctx->tune_join_spill_cap
                                                  : static_cast<uint32_t>(std::min<uint64_t>(per_block / 4 + 1024, 1u << 30));
                         const size_t rec_bytes = size_t(lds_blocks) * cap * (1 + nv) * 8;
+                        // Its working set (relabelled map and state, partials, spill
+                        // lists) must fit next to what is live, else the passes run.
+                        const size_t spill_bytes = size_t(range) * sizeof(MapT) + state_words(S, p) * 8 +
+                                                   size_t(lds_blocks) * part_words(W, p) * 8 + rec_bytes +
+                                                   size_t(lds_blocks) * 4;
                         spill = hs.total >= 1024 && uint64_t(hs.total - hs.inside) * 8 <= hs.total &&
                                 (uint64_t(hs.cold_max) * 4096 <= hs.total ||
                                  uint64_t(hs.cold_max) * hs.stride <= (uint64_t(1) << 16)) &&
-                                rec_bytes <= kSpillMaxBytes;
+                                rec_bytes <= kSpillMaxBytes && device_room(ctx, spill_bytes);
+                        Scratch gm2, st2, part, rec, rcount;
                         if (spill) {
-                            Scratch gm2(ctx, size_t(range) * sizeof(MapT));
+                            try {  // every allocation before any launch: a failure leaves no trace
+                                gm2 = Scratch(ctx, size_t(range) * sizeof(MapT));
+                                st2 = Scratch(ctx, state_words(S, p) * 8);
+                                part = Scratch(ctx, size_t(lds_blocks) * part_words(W, p) * 8);
+                                rec = Scratch(ctx, rec_bytes);
+                                rcount = Scratch(ctx, size_t(lds_blocks) * 4);
+                            } catch (const Error& e) {
+                                if (e.status != QE_GPU_OUT_OF_MEMORY) throw;
+                                spill = false;
+                                gm2 = Scratch();
+                                st2 = Scratch();
+                                part = Scratch();
+                                rec = Scratch();
+                                rcount = Scratch();
+                            }
+                        }
+                        if (spill) {
                             remap_slots_kernel<MapT><<<grid_for(range, 256, 4096), 256, 0, ctx->stream>>>(
                                 gm, range, perm.as<uint32_t>(), gm2.as<MapT>());
                             QE_HIP(hipGetLastError());
-                            Scratch st2(ctx, state_words(S, p) * 8);
-                            Scratch part(ctx, size_t(lds_blocks) * part_words(W, p) * 8);
-                            Scratch rec(ctx, rec_bytes);
-                            Scratch rcount(ctx, size_t(lds_blocks) * 4);
                             init_state(ctx, st2.as<uint64_t>(), S, p);
                             // The relabelled state's own NaN/inf flags (moved with it).
                             AggProg p2 = p;

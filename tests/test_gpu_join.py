@@ -314,18 +314,10 @@ def test_fused_many_groups_slot_range_passes(ctx, case):
     join_agg_case(ctx, pcols, bcols, [0], [0], INNER, [5], aggs)
 
 
-@pytest.mark.parametrize("case", range(4))
-def test_fused_many_groups_hot_set_one_pass(ctx, qe, case):
-    # More groups than LDS holds, but the probe puts nearly all rows on a few
-    # thousand groups: a sample of 2^16 probe rows finds them, they are
-    # relabelled to the first slots, and one probe pass keeps them in LDS while
-    # the other rows fold with HBM atomics; the state returns to slot order.
-    # Hot groups form one range of group values (case 0) or are scattered
-    # (1, 2); case 3 spreads the rows over all groups and keeps one pass per
-    # slot range. Case 2 caps the spill lists at 64 records per block, so most
-    # spilled rows take the HBM-atomic fold. I32 and I64 keys; odd probe
-    # lengths (tail row); NULL groups; the streamed probe (one or two value
-    # columns).
+def hot_set_shape(case):
+    """Build and probe columns of the hot-set shape (60,000 groups; 90% of 2^22
+    probe rows on 4000 hot groups) for case 0-2, or spread over all groups for
+    case 3, with the aggregates that case uses."""
     r = np.random.default_rng(1100 + case)
     kt = I32 if case % 2 else I64
     n_b, groups = 60000, 60000
@@ -348,6 +340,22 @@ def test_fused_many_groups_hot_set_one_pass(ctx, qe, case):
                                       tuple(np.nonzero(r.random(n_p) < 0.02)[0])),
              col(r.integers(-1000, 1000, n_p), I32), col(r.integers(-2**40, 2**40, n_p), I64)]
     aggs = [(CS, 0), (AVG, 1)] if case in (0, 3) else [(CS, 0), (SUM, 1), (MIN, 1), (MAX, 3), (SUM, 3), (CNT, 1)]
+    return pcols, bcols, aggs
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_fused_many_groups_hot_set_one_pass(ctx, qe, case):
+    # More groups than LDS holds, but the probe puts nearly all rows on a few
+    # thousand groups: a sample of 2^16 probe rows finds them, they are
+    # relabelled to the first slots, and one probe pass keeps them in LDS while
+    # the other rows fold with HBM atomics; the state returns to slot order.
+    # Hot groups form one range of group values (case 0) or are scattered
+    # (1, 2); case 3 spreads the rows over all groups and keeps one pass per
+    # slot range. Case 2 caps the spill lists at 64 records per block, so most
+    # spilled rows take the HBM-atomic fold. I32 and I64 keys; odd probe
+    # lengths (tail row); NULL groups; the streamed probe (one or two value
+    # columns).
+    pcols, bcols, aggs = hot_set_shape(case)
     ctx.set_timing(True)
     if case == 2:
         ctx.set_tuning(qe.TUNE_JOIN_SPILL_CAP, 64)
@@ -366,3 +374,33 @@ def test_fused_many_groups_hot_set_one_pass(ctx, qe, case):
         assert probes > 1 and spill_passes == 0
     else:
         assert probes == 1 and spill_passes >= 1
+
+
+def test_hot_set_without_room_for_spill_lists_runs_the_passes(ctx, qe):
+    # The hot-set shape with spill lists of 2^20 records per block (about 6 GiB)
+    # under a device memory limit that leaves 1 GiB beyond the inputs: the
+    # lists do not fit, so the query runs one probe pass per LDS slot range
+    # instead of failing, and returns the oracle's groups.
+    pcols, bcols, aggs = hot_set_shape(1)
+    P, B = up(ctx, pcols), up(ctx, bcols)
+    live, _ = ctx.mem_usage()
+    ctx.set_timing(True)
+    ctx.set_tuning(qe.TUNE_JOIN_SPILL_CAP, 1 << 20)
+    ctx.set_mem_limit(live + (1 << 30))
+    try:
+        for name in ("join_agg_hot_set", "join_agg_probe", "join_agg_spill"):
+            ctx.timing(name)
+        t = ctx.join_aggregate(P, B, [0], [0], INNER, [5], aggs)
+        got = table_rows(t.numpy())
+        t.free()
+        _, sampled = ctx.timing("join_agg_hot_set")
+        _, probes = ctx.timing("join_agg_probe")
+        _, spill_passes = ctx.timing("join_agg_spill")
+    finally:
+        ctx.set_mem_limit(0)
+        ctx.set_timing(False)
+        ctx.set_tuning(qe.TUNE_JOIN_SPILL_CAP, 0)
+        for d in P + B:
+            d.free()
+    assert sampled == 1 and probes > 1 and spill_passes == 0
+    assert_same(got, O.join_aggregate(pcols, bcols, [0], [0], INNER, [5], aggs))

@@ -71,3 +71,50 @@ def test_tick_pipeline_matches_oracle(ctx, qe, nsym, nq, nt):
         for c in (qs, qt, qp, ts_, tt, tp, sym_id, sym_name):
             c.free()
         d.free()
+
+
+def test_tick_shape_groupby_by_name_takes_the_hot_set_path(ctx, qe):
+    # The pipeline's last join -> group-by at the tick shape: 2^22 + 3 trades
+    # whose symbols follow Zipf(1.1) over 50,000, joined to a 50,000-row symbol
+    # table with dictionary-encoded names (3-12 characters), GROUP BY name
+    # COUNT(*), AVG(px). 50,000 groups do not fit LDS, but the sampled hot set
+    # (about 92% of the rows on the top LDS range of names) does: one probe pass
+    # keeps it in LDS and the other rows are folded from the spill lists. Keys
+    # and counts exact, AVG within 1e-12 of the oracle's Neumaier sums.
+    nsym, nt = 50000, (1 << 22) + 3
+    r = np.random.default_rng(4242)
+    (qs, qt, qp), (ts_, tt) = ctx.gen_asof(4242, 1 << 16, nt, nsym, 1.1)
+    tp = ctx.gen_uniform(4242, 5, qe.F64, nt, 0.0, 200.0)
+    names = sorted({"".join(r.choice(list("ABCDEFGHIJ"), int(r.integers(3, 13)))) for _ in range(3 * nsym)})[:nsym]
+    d = ctx.dict(list(r.permutation(names)))
+    strings = list(d.strings)
+    code = {x: i for i, x in enumerate(strings)}
+    order = r.permutation(nsym)
+    name_codes = np.array([code[names[i]] for i in order], dtype=np.int32)
+    sym_id = ctx.upload(np.arange(nsym, dtype=np.int32), qe.I32)
+    sym_name = ctx.upload(name_codes, qe.STR, dict=d)
+    ctx.set_timing(True)
+    try:
+        for name in ("join_agg_hot_set", "join_agg_spill"):
+            ctx.timing(name)
+        agg = ctx.join_aggregate([ts_, tp], [sym_id, sym_name], [0], [0], qe.INNER, [3],
+                                 [(qe.COUNT_STAR, 0), (qe.AVG, 1)])
+        a, av = agg.col(2).numpy()
+        got = list(zip(agg.col(0).strings(), agg.col(1).numpy()[0].tolist(),
+                       [None if (av is not None and not av[i]) else x for i, x in enumerate(a.tolist())]))
+        agg.free()
+        _, sampled = ctx.timing("join_agg_hot_set")
+        _, spill_passes = ctx.timing("join_agg_spill")
+        tsym, _ = ts_.numpy()
+        tpx, _ = tp.numpy()
+    finally:
+        ctx.set_timing(False)
+        for c in (qs, qt, qp, ts_, tt, tp, sym_id, sym_name):
+            c.free()
+        d.free()
+    assert sampled == 1 and spill_passes >= 1
+    assert len(got) > 40000 and sum(c for _, c, _ in got) == nt
+    sym_tab = [(np.arange(nsym, dtype=np.int32), None, O.I32), (name_codes, None, O.STR, strings)]
+    want = O.join_aggregate([(tsym, None, O.I32), (tpx, None, O.F64)], sym_tab, [0], [0], 0, [3],
+                            [(O.COUNT_STAR, 0), (O.AVG, 1)])
+    assert_same(got, want)
