@@ -47,6 +47,35 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def load_tuning(args):
+    """The persisted autotuner choices: {workload signature: entry}."""
+    try:
+        return json.loads(Path(args.tuning_file).read_text())
+    except (OSError, ValueError):
+        return {}
+
+
+def save_tuning(args, key, entry):
+    data = load_tuning(args)
+    data[key] = entry
+    path = Path(args.tuning_file)
+    path.parent.mkdir(parents=True, exist_ok=True)
+    path.write_text(json.dumps(data, indent=1, sort_keys=True) + "\n")
+
+
+def tuning_for(args, key, tune_fn):
+    """The entry for `key`: from the tuning file when present (unless
+    --retune), else from the autotuner, which is then written to the file."""
+    entry = None if args.retune else load_tuning(args).get(key)
+    if entry is not None:
+        return dict(entry, source="file", file=str(args.tuning_file))
+    t0 = time.perf_counter()
+    entry = tune_fn()
+    entry["autotune_s"] = time.perf_counter() - t0
+    save_tuning(args, key, entry)
+    return dict(entry, source="autotune", file=str(args.tuning_file))
+
+
 def join_groupby_workload(qe, ctx, args):
     build_n = 1 << args.build_log2
     probe_n = 1 << args.probe_log2
@@ -54,7 +83,15 @@ def join_groupby_workload(qe, ctx, args):
     log(f"generating join workload: build {build_n} probe {probe_n}")
     (pk, pp), (bk, bg) = ctx.gen_join(20260615, build_n, probe_n, groups, args.zipf, args.miss)
     aggs = [(qe.SUM, 1), (qe.COUNT_STAR, 0)]
-    ctx.set_tuning(qe.TUNE_JOIN_PAIRS, args.join_pairs)
+    # Launch shape: the persisted autotuner choice for this workload on this
+    # GPU, or a fresh autotune (timed on this query) written back to the file.
+    key = (f"join_groupby/{ctx.device_info()['arch'].split(':')[0]}/build2^{args.build_log2}/"
+           f"probe2^{args.probe_log2}/zipf{args.zipf:g}/miss{args.miss:g}/groups{groups}/sum,count_star")
+    tuning = tuning_for(args, key, lambda: ctx.join_autotune([pk, pp], [bk, bg], [0], [0], qe.INNER, [3], aggs))
+    pairs = args.join_pairs or tuning["pairs"]
+    ctx.set_tuning(qe.TUNE_JOIN_PAIRS, pairs)
+    ctx.set_tuning(qe.TUNE_JOIN_BLOCKS_PER_CU, tuning["blocks_per_cu"])
+    tuning["applied"] = {"pairs": pairs, "blocks_per_cu": tuning["blocks_per_cu"]}
 
     # The C ABI call a host engine makes per query; its arguments are
     # marshalled once, so the step times the engine, not Python list handling.
@@ -120,6 +157,7 @@ def join_groupby_workload(qe, ctx, args):
         "kernel_achieved_gbs": probe_bytes / (kernel_ms / 1e3) / 1e9 if kernel_ms > 0 else None,
         "map_gather_floor_ms": gather_ms,
         "kernel_over_gather_floor": kernel_ms / gather_ms if gather_ms > 0 else None,
+        "tuning": tuning,
     }
     cols = {"pk": pk, "pp": pp, "bk": bk, "bg": bg}
     return res, cols
@@ -346,10 +384,15 @@ def batched_workload(qe, ctx, args):
         total_rows += n
     plan = ctx.batch(queries)
     res = {"queries": nq, "morsels": nq * per, "rows": total_rows}
-    t0 = time.perf_counter()
-    tune = plan.autotune()
-    res["autotune_s"] = time.perf_counter() - t0
-    res["tuning"] = {"tile_rows": tune[0], "blocks_per_cu": tune[1], "lds_slots": tune[2]}
+    key = (f"batched_groupby/{ctx.device_info()['arch'].split(':')[0]}/queries{nq}/morsels{nq * per}/"
+           f"max2^{args.batch_max_log2}")
+
+    def autotune():
+        t = plan.autotune()
+        return {"tile_rows": t[0], "blocks_per_cu": t[1], "lds_slots": t[2]}
+
+    res["tuning"] = tuning_for(args, key, autotune)
+    tune = (res["tuning"]["tile_rows"], res["tuning"]["blocks_per_cu"], res["tuning"]["lds_slots"])
 
     def timed(mode, reps):
         plan.run(mode, tuning=tune, collect=False)
@@ -792,8 +835,11 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--zipf", type=float, default=1.0, help="Zipf exponent of the probe keys")
     ap.add_argument("--miss", type=float, default=0.1, help="fraction of probe keys outside the build domain")
-    ap.add_argument("--join-pairs", type=int, default=4, choices=[2, 4],
-                    help="row pairs per thread per tile in the streaming probe")
+    ap.add_argument("--tuning-file", default=str(ROOT / "query-engine_amd" / "tuning" / "autotune.json"),
+                    help="persisted autotuner choices (read, and written after a fresh autotune)")
+    ap.add_argument("--retune", action="store_true", help="autotune even where the tuning file has an entry")
+    ap.add_argument("--join-pairs", type=int, default=0, choices=[0, 2, 4],
+                    help="row pairs per thread per tile in the streaming probe (0: the tuned value)")
     ap.add_argument("--asof-right", type=int, default=400 << 20)
     ap.add_argument("--asof-left", type=int, default=100 << 20)
     ap.add_argument("--asof-reps", type=int, default=5)
@@ -891,7 +937,7 @@ def main():
                         "[0,10000)), probe 2^%d rows (Zipf(%g) keys, %g%% misses, F64 payload), SUM/COUNT per "
                         "group" % (args.build_log2, args.probe_log2, args.zipf, 100 * args.miss),
             "build_rows": res["build_rows"], "probe_rows": res["probe_rows"], "groups": 10000,
-            "parallelism": "single GPU",
+            "parallelism": "single GPU", "tuning": res["tuning"],
         },
         "roofline": {
             "bound": "hbm", "achieved": kach, "peak": HBM_PEAK_GBS, "unit": "GB/s",

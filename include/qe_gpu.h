@@ -132,11 +132,14 @@ qe_gpu_status qe_gpu_timing(qe_gpu_ctx* ctx, const char* kernel, double* ms, uin
  * qe_gpu_asof_train: larger values shrink the radix table.
  * QE_TUNE_JOIN_SPILL_CAP: records per block in the spill lists of a fused
  * join -> group-by whose groups exceed LDS (0 = a quarter of the block's rows;
- * rows past the cap fold with HBM atomics). */
+ * rows past the cap fold with HBM atomics). QE_TUNE_JOIN_BLOCKS_PER_CU:
+ * 1024-thread blocks per CU of the fused join's streaming probe, 1..8
+ * (default 1), capped by what the LDS partials leave room for. */
 typedef enum qe_tune_key {
     QE_TUNE_JOIN_PAIRS = 0,
     QE_TUNE_ASOF_BUCKET_KNOTS = 1,
-    QE_TUNE_JOIN_SPILL_CAP = 2
+    QE_TUNE_JOIN_SPILL_CAP = 2,
+    QE_TUNE_JOIN_BLOCKS_PER_CU = 3
 } qe_tune_key;
 qe_gpu_status qe_gpu_set_tuning(qe_gpu_ctx* ctx, int32_t key, int64_t value);
 /* Inspection: the XCD (die) id that ran each of nblocks one-wave blocks of a
@@ -283,6 +286,24 @@ qe_gpu_status qe_gpu_join_aggregate(qe_gpu_ctx* ctx,
                                     const uint32_t* probe_keys, const uint32_t* build_keys, uint32_t nkeys,
                                     int32_t join_type, const uint32_t* group_keys, uint32_t ngroup_keys,
                                     const qe_agg_spec* aggs, uint32_t naggs, qe_dev_table** out);
+/* Autotuner for the fused join -> group-by: runs this query once per launch
+ * shape (row pairs per thread 2/4 x blocks per CU 1/2/4), times the streaming
+ * probe with HIP events (best of two runs), leaves the fastest shape set on
+ * the context (QE_TUNE_JOIN_PAIRS, QE_TUNE_JOIN_BLOCKS_PER_CU) and reports it.
+ * The caller persists the choice (the benchmark keeps it in a tuning file).
+ * No counterpart in the reference. */
+typedef struct qe_join_tuning {
+    int32_t pairs;
+    int32_t blocks_per_cu;
+    double probe_ms;       /* probe kernel time of the chosen shape       */
+    double default_ms;     /* the same with the default shape (4 pairs, 1 block per CU) */
+    int32_t candidates;    /* shapes timed                                */
+} qe_join_tuning;
+qe_gpu_status qe_gpu_join_autotune(qe_gpu_ctx* ctx, const qe_dev_col* const* probe_cols, uint32_t nprobe,
+                                   const qe_dev_col* const* build_cols, uint32_t nbuild, const uint32_t* probe_keys,
+                                   const uint32_t* build_keys, uint32_t nkeys, int32_t join_type,
+                                   const uint32_t* group_keys, uint32_t ngk, const qe_agg_spec* aggs, uint32_t naggs,
+                                   qe_join_tuning* out);
 
 /* ---- hash table -----------------------------------------------------------------*/
 /* Replaces qe::HashTable (ops/hashtable.h:108-190): key -> dense group id in

@@ -160,6 +160,8 @@ _SIGS = {
     "qe_gpu_stream_copy": (C.c_int, [P, P, P, C.c_uint64, C.POINTER(C.c_double)]),
     "qe_gpu_stream_read": (C.c_int, [P, P, C.c_uint64, P, C.POINTER(C.c_double)]),
     "qe_gpu_map_gather": (C.c_int, [P, P, C.c_int64, C.c_uint64, P, P, C.POINTER(C.c_double)]),
+    "qe_gpu_join_autotune": (C.c_int, [P, P, C.c_uint32, P, C.c_uint32, P, P, C.c_uint32, C.c_int32, P, C.c_uint32,
+                                       P, C.c_uint32, P]),
     "qe_gpu_device_info": (C.c_int, [P, P]),
     "qe_gpu_gen_groupby": (C.c_int, [P, C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, P, P]),
     "qe_gpu_gen_join_build": (C.c_int, [P, C.c_uint64, C.c_uint64, C.c_uint32, P, P]),
@@ -584,6 +586,16 @@ class Context:
             len(aggs), C.byref(out)))
         return Table(self, out)
 
+    def join_autotune(self, probe_cols, build_cols, probe_keys, build_keys, join_type, group_keys, aggs) -> dict:
+        """Time the fused join -> group-by's launch shapes on this query; the
+        fastest is left set on the context and returned."""
+        t = JoinTuning()
+        self._check(lib().qe_gpu_join_autotune(self.h, _cols(probe_cols), len(probe_cols), _cols(build_cols),
+                                               len(build_cols), _u32(probe_keys), _u32(build_keys), len(probe_keys),
+                                               join_type, _u32(group_keys), len(group_keys), _aggs(aggs), len(aggs),
+                                               C.byref(t)))
+        return {f: getattr(t, f) for f, _ in JoinTuning._fields_}
+
     def prepare_join_aggregate(self, probe_cols, build_cols, probe_keys, build_keys, join_type, group_keys,
                                aggs):
         """join_aggregate with its arguments marshalled once, for repeated calls
@@ -724,6 +736,11 @@ class HashTable:
         self.h = None
 
 
+class JoinTuning(C.Structure):
+    _fields_ = [("pairs", C.c_int32), ("blocks_per_cu", C.c_int32), ("probe_ms", C.c_double),
+                ("default_ms", C.c_double), ("candidates", C.c_int32)]
+
+
 class DeviceInfo(C.Structure):
     _fields_ = [("name", C.c_char * 256), ("arch", C.c_char * 64), ("device", C.c_int32),
                 ("compute_units", C.c_int32), ("hbm_bytes", C.c_uint64), ("lds_bytes_per_block", C.c_uint64),
@@ -787,7 +804,7 @@ class BatchTuning(C.Structure):
     _fields_ = [("tile_rows", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("lds_slots", C.c_uint32)]
 
 
-TUNE_JOIN_PAIRS, TUNE_ASOF_BUCKET_KNOTS, TUNE_JOIN_SPILL_CAP = 0, 1, 2
+TUNE_JOIN_PAIRS, TUNE_ASOF_BUCKET_KNOTS, TUNE_JOIN_SPILL_CAP, TUNE_JOIN_BLOCKS_PER_CU = 0, 1, 2, 3
 CMP_LT, CMP_LE, CMP_GT, CMP_GE, CMP_EQ, CMP_NE = range(6)
 BATCH_PERSISTENT, BATCH_PER_BATCH, BATCH_GRAPH = 0, 1, 2
 

@@ -637,6 +637,10 @@ qe_gpu_status qe_gpu_set_tuning(qe_gpu_ctx* ctx, int32_t key, int64_t value) {
                 invalid(value >= 0 && value <= (int64_t(1) << 30), "QE_TUNE_JOIN_SPILL_CAP must be in [0, 2^30]");
                 ctx->tune_join_spill_cap = static_cast<uint32_t>(value);
                 break;
+            case QE_TUNE_JOIN_BLOCKS_PER_CU:
+                invalid(value >= 1 && value <= 8, "QE_TUNE_JOIN_BLOCKS_PER_CU must be in [1, 8]");
+                ctx->tune_join_blocks_per_cu = static_cast<int>(value);
+                break;
             default:
                 invalid(false, "unknown tuning key");
         }

@@ -120,6 +120,7 @@ struct qe_gpu_ctx {
     int tune_join_pairs = 4;  // row pairs per thread per tile in the streaming probe
     int tune_asof_bucket_knots = 1;  // knots per learned-router bucket (set at train time)
     uint32_t tune_join_spill_cap = 0;  // spill-list records per block (0 = sized from the probe)
+    int tune_join_blocks_per_cu = 1;   // streaming-probe blocks per CU (as many as LDS allows)
     // Small pinned host staging buffer for scalars read back after kernels.
     void* pinned = nullptr;
     size_t pinned_bytes = 0;

@@ -1636,8 +1636,13 @@ qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* pro
             const uint64_t pass_slots = std::max<uint64_t>(1, kLdsBudget / lds_bytes(1, p));
             const uint64_t npasses = (S + pass_slots - 1) / pass_slots;
             const bool multipass = !use_lds && npasses <= kMaxSlotPasses;
+            // Blocks per CU as tuned, as far as the LDS partials leave room.
+            const uint64_t lds_per_block = lds_bytes(static_cast<uint32_t>(std::min<uint64_t>(S, pass_slots)), p);
+            const uint64_t bpc = std::max<uint64_t>(
+                1, std::min<uint64_t>(ctx->tune_join_blocks_per_cu,
+                                      lds_per_block ? (kLdsBudget + 256) / lds_per_block : 8));
             const unsigned lds_blocks = static_cast<unsigned>(
-                std::max<uint64_t>(1, std::min<uint64_t>(ctx->num_cus, (np + 1023) / 1024)));
+                std::max<uint64_t>(1, std::min<uint64_t>(uint64_t(ctx->num_cus) * bpc, (np + 1023) / 1024)));
             Scratch fbuf;  // exact F64 sums: scales, flags, outliers
             prepare_f64(ctx, p, st.as<uint64_t>(), S, np, exact, fbuf);
             const bool small = S < 0xFFFF;
@@ -1972,5 +1977,57 @@ extern "C" qe_gpu_status qe_gpu_join_aggregate(qe_gpu_ctx* ctx, const qe_dev_col
             }
         }
         *out = t;
+    });
+}
+
+extern "C" qe_gpu_status qe_gpu_join_autotune(qe_gpu_ctx* ctx, const qe_dev_col* const* probe_cols, uint32_t nprobe,
+                                              const qe_dev_col* const* build_cols, uint32_t nbuild,
+                                              const uint32_t* probe_keys, const uint32_t* build_keys, uint32_t nkeys,
+                                              int32_t join_type, const uint32_t* group_keys, uint32_t ngk,
+                                              const qe_agg_spec* aggs, uint32_t naggs, qe_join_tuning* out) {
+    QE_ENTRY(ctx, {
+        invalid(out != nullptr, "null output");
+        const bool timing = ctx->timing;
+        const int pairs0 = ctx->tune_join_pairs, bpc0 = ctx->tune_join_blocks_per_cu;
+        ctx->timing = true;
+        qe_join_tuning best{pairs0, bpc0, 1e30, 0.0, 0};
+        try {
+            for (int pairs : {4, 2}) {
+                for (int bpc : {1, 2, 4}) {
+                    ctx->tune_join_pairs = pairs;
+                    ctx->tune_join_blocks_per_cu = bpc;
+                    double t = 1e30;
+                    for (int rep = 0; rep < 2; ++rep) {
+                        ctx->timings.erase("join_agg_probe");
+                        qe_dev_table* tab = nullptr;
+                        const qe_gpu_status st = qe_gpu_join_aggregate(ctx, probe_cols, nprobe, build_cols, nbuild,
+                                                                       probe_keys, build_keys, nkeys, join_type,
+                                                                       group_keys, ngk, aggs, naggs, &tab);
+                        if (st != QE_GPU_OK) throw Error{st, ctx->last_error};
+                        qe_gpu_table_free(ctx, tab);
+                        const auto it = ctx->timings.find("join_agg_probe");
+                        const double ms = it == ctx->timings.end() ? 0.0 : it->second.ms;
+                        t = std::min(t, ms);
+                    }
+                    ++best.candidates;
+                    if (pairs == 4 && bpc == 1) best.default_ms = t;
+                    if (t < best.probe_ms) {
+                        best.probe_ms = t;
+                        best.pairs = pairs;
+                        best.blocks_per_cu = bpc;
+                    }
+                }
+            }
+        } catch (...) {
+            ctx->timing = timing;
+            ctx->tune_join_pairs = pairs0;
+            ctx->tune_join_blocks_per_cu = bpc0;
+            throw;
+        }
+        ctx->timing = timing;
+        ctx->timings.erase("join_agg_probe");
+        ctx->tune_join_pairs = best.pairs;
+        ctx->tune_join_blocks_per_cu = best.blocks_per_cu;
+        *out = best;
     });
 }

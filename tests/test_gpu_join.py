@@ -404,3 +404,35 @@ def test_hot_set_without_room_for_spill_lists_runs_the_passes(ctx, qe):
             d.free()
     assert sampled == 1 and probes > 1 and spill_passes == 0
     assert_same(got, O.join_aggregate(pcols, bcols, [0], [0], INNER, [5], aggs))
+
+
+def test_join_autotune_keeps_results(ctx, qe):
+    # The autotuner times six launch shapes on this query and leaves the
+    # fastest set; every shape returns the same groups (bit-identical sums).
+    (pk, pp), (bk, bg) = ctx.gen_join(11, 1 << 16, 1 << 20, 3000)
+    aggs = [(SUM, 1), (CS, 0)]
+    try:
+        before = ctx.join_aggregate([pk, pp], [bk, bg], [0], [0], INNER, [3], aggs)
+        a = before.numpy()
+        before.free()
+        t = ctx.join_autotune([pk, pp], [bk, bg], [0], [0], INNER, [3], aggs)
+        assert t["candidates"] == 6 and t["pairs"] in (2, 4) and t["blocks_per_cu"] in (1, 2, 4)
+        assert 0 < t["probe_ms"] <= t["default_ms"]
+        outs = []
+        for pairs, bpc in ((t["pairs"], t["blocks_per_cu"]), (2, 4), (4, 2)):
+            ctx.set_tuning(qe.TUNE_JOIN_PAIRS, pairs)
+            ctx.set_tuning(qe.TUNE_JOIN_BLOCKS_PER_CU, bpc)
+            r = ctx.join_aggregate([pk, pp], [bk, bg], [0], [0], INNER, [3], aggs)
+            outs.append(r.numpy())
+            r.free()
+        with pytest.raises(qe.InvalidArgument):
+            ctx.set_tuning(qe.TUNE_JOIN_BLOCKS_PER_CU, 9)
+    finally:
+        ctx.set_tuning(qe.TUNE_JOIN_PAIRS, 4)
+        ctx.set_tuning(qe.TUNE_JOIN_BLOCKS_PER_CU, 1)
+        for d in (pk, pp, bk, bg):
+            d.free()
+    for o in outs:
+        assert len(o) == len(a)
+        for (va, _), (vb, _) in zip(a, o):
+            assert np.array_equal(np.sort(va.view(np.uint64)), np.sort(vb.view(np.uint64)))
