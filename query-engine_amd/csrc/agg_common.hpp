@@ -124,8 +124,9 @@ __host__ __device__ inline size_t merge_words(uint64_t S, const AggProg& p) {
 __host__ __device__ inline size_t part_words(uint32_t S, const AggProg& p) { return merge_words(S, p); }
 
 // ---- exact F64 sums -----------------------------------------------------------------------
-// x as round(x * 2^s) in 128-bit two's complement (lo, hi). Returns 0, or a
-// flag: 1 NaN, 2 +inf, 4 -inf, 8 |x| >= 2^e (the scale does not cover x).
+// x as round(x * 2^s) in 128-bit two's complement (lo, hi; hi is 0 or -1).
+// Returns 0, or a flag: 1 NaN, 2 +inf, 4 -inf, 8 |x| >= 2^e (the scale does
+// not cover x).
 __device__ inline uint32_t f64_fixed(uint64_t bits, int s, int e, uint64_t& lo, int64_t& hi) {
     lo = 0;
     hi = 0;
@@ -137,26 +138,20 @@ __device__ inline uint32_t f64_fixed(uint64_t bits, int s, int e, uint64_t& lo, 
     if (m == 0) return 0;
     if (bexp - 1023 >= e) return 8u;
     const int k = (bexp ? bexp - 1075 : -1074) + s;  // |x| = m * 2^(k - s)
-    uint64_t l, h = 0;
-    if (k >= 64) {
-        l = 0;
-        h = m << (k - 64);
-    } else if (k > 0) {
-        l = m << k;
-        h = m >> (64 - k);
-    } else if (k == 0) {
-        l = m;
-    } else if (k >= -53) {
-        l = (m + (uint64_t(1) << (-k - 1))) >> (-k);  // half a quantum rounds away from zero
-    } else {
-        return 0;  // below half a quantum
-    }
-    if (neg) {
-        l = ~l + 1;
-        h = ~h + (l == 0 ? 1 : 0);
-    }
+    // With s = 63 - e and |x| < 2^e, |x| 2^s < 2^63: k <= 10 (11 for a subnormal,
+    // whose m < 2^52) and the value fits the low word. A scale that breaks
+    // this treats x as outside it.
+    if (k > (bexp ? 10 : 11)) return 8u;
+    // Both shifts computed and one selected (no divergent branches): m 2^k for
+    // k >= 0, else m 2^k with half a quantum rounding away from zero (0 below
+    // half a quantum, k < -53).
+    const int r = k < 0 ? (k < -63 ? 63 : -k) : 1;
+    const uint64_t up = m << (k < 0 ? 0 : k);
+    const uint64_t dn = (m + (uint64_t(1) << (r - 1))) >> r;
+    uint64_t l = k >= 0 ? up : (k < -53 ? 0 : dn);
+    if (neg) l = ~l + 1;
     lo = l;
-    hi = static_cast<int64_t>(h);
+    hi = (neg && l != 0) ? -1 : 0;
     return 0;
 }
 
