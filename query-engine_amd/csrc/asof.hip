@@ -653,8 +653,9 @@ __global__ void asof_learned_kernel(PK pk, ProbeArgs a, LearnedArgs L) {
         //    result is exact wherever the prediction lands.
         const uint64_t base = piece_start(pm.seg, piece, L.B);
         const uint32_t cnt = piece_count(pm.seg, pm.n, piece, L.B);
-        const uint32_t pred = piece_pred(tau, slope, cnt, t);
-        const uint32_t c = gallop_count(a.ts, base, cnt, pred, t);
+        // A one-row piece is its knot: tau <= t already places the match there,
+        // with no read of the timestamps.
+        const uint32_t c = cnt == 1 ? 1u : gallop_count(a.ts, base, cnt, piece_pred(tau, slope, cnt, t), t);
         // tau[piece] <= t guarantees c >= 1.
         finish_match(a, j, t, base + c - 1, c > 0);
     }
@@ -1187,7 +1188,7 @@ extern "C" {
 qe_gpu_status qe_gpu_asof_train(qe_gpu_ctx* ctx, qe_dev_asof* idx, uint32_t piece_rows, uint32_t steps, float lr) {
     QE_ENTRY(ctx, {
         invalid(idx != nullptr, "null index");
-        invalid(piece_rows >= 2 && piece_rows <= 4096, "piece_rows must be in [2, 4096]");
+        invalid(piece_rows >= 1 && piece_rows <= 4096, "piece_rows must be in [1, 4096]");
         const uint64_t P = idx->nparts;
         idx->B = piece_rows;
         idx->mlp_rejected = false;

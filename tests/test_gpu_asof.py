@@ -118,7 +118,7 @@ def test_random_unsorted_inputs(ctx, qe, case):
     check_all(ctx, qe, pcols, bcols, [0], [0], 1, 1, tol=tol, piece_rows=int(r.choice([4, 16, 64])))
 
 
-@pytest.mark.parametrize("piece_rows", [4, 16, 64])
+@pytest.mark.parametrize("piece_rows", [1, 2, 4, 16, 64])
 def test_duplicate_time_runs_across_pieces(ctx, qe, piece_rows):
     # Runs of equal build timestamps longer than a piece (several knots share
     # one tau, and a piece's rows may all be equal), rows in shuffled order,
