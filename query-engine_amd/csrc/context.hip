@@ -253,14 +253,22 @@ TimedRegion::TimedRegion(qe_gpu_ctx* c, const char* n) : ctx(c), name(n) {
 TimedRegion::~TimedRegion() {
     if (!ctx->timing || !a) return;
     hipEventRecord(b, ctx->stream);
-    hipEventSynchronize(b);
-    float ms = 0;
-    hipEventElapsedTime(&ms, a, b);
-    auto& t = ctx->timings[name];
-    t.ms += ms;
-    t.launches += 1;
-    hipEventDestroy(a);
-    hipEventDestroy(b);
+    ctx->pending_regions.push_back({name, a, b});
+    if (ctx->pending_regions.size() >= 4096) resolve_timings(ctx);
+}
+
+void resolve_timings(qe_gpu_ctx* ctx) {
+    for (auto& r : ctx->pending_regions) {
+        hipEventSynchronize(r.b);
+        float ms = 0;
+        hipEventElapsedTime(&ms, r.a, r.b);
+        auto& t = ctx->timings[r.name];
+        t.ms += ms;
+        t.launches += 1;
+        hipEventDestroy(r.a);
+        hipEventDestroy(r.b);
+    }
+    ctx->pending_regions.clear();
 }
 
 uint64_t common_len(const qe_dev_col* const* cols, uint32_t n) {
@@ -557,6 +565,10 @@ qe_gpu_status qe_gpu_ctx_create(int device, qe_gpu_ctx** out) {
 void qe_gpu_ctx_destroy(qe_gpu_ctx* ctx) {
     if (!ctx) return;
     hipStreamSynchronize(ctx->stream);
+    for (auto& r : ctx->pending_regions) {
+        hipEventDestroy(r.a);
+        hipEventDestroy(r.b);
+    }
     for (auto& kv : ctx->free_blocks) hipFree(kv.second);
     for (auto& kv : ctx->live_blocks) hipFree(kv.first);
     if (ctx->pinned) hipHostFree(ctx->pinned);
@@ -650,6 +662,7 @@ qe_gpu_status qe_gpu_set_tuning(qe_gpu_ctx* ctx, int32_t key, int64_t value) {
 qe_gpu_status qe_gpu_timing(qe_gpu_ctx* ctx, const char* kernel, double* ms, uint64_t* launches) {
     QE_ENTRY(ctx, {
         invalid(kernel != nullptr, "null kernel name");
+        resolve_timings(ctx);
         auto it = ctx->timings.find(kernel);
         if (ms) *ms = (it == ctx->timings.end()) ? 0.0 : it->second.ms;
         if (launches) *launches = (it == ctx->timings.end()) ? 0 : it->second.launches;

@@ -115,6 +115,13 @@ struct qe_gpu_ctx {
         uint64_t launches = 0;
     };
     std::map<std::string, Timing> timings;
+    // Timed regions whose end event may still be pending (resolved when the
+    // timings are read, so a region adds no synchronisation of its own).
+    struct PendingRegion {
+        const char* name;
+        hipEvent_t a, b;
+    };
+    std::vector<PendingRegion> pending_regions;
     int num_cus = 256;
     // Tuning knobs (qe_gpu_set_tuning).
     int tune_join_pairs = 4;  // row pairs per thread per tile in the streaming probe
@@ -177,7 +184,11 @@ T read_scalar(qe_gpu_ctx* ctx, const T* dev) {
     return *h;
 }
 
-// Kernel timing bracket (events on the ctx stream).
+// Kernel timing bracket (events on the ctx stream); the elapsed time is
+// taken when the timings are read (resolve_timings), so a region does not
+// synchronise the stream.
+void resolve_timings(qe_gpu_ctx* ctx);
+
 struct TimedRegion {
     qe_gpu_ctx* ctx;
     const char* name;

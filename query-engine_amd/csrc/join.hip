@@ -1998,6 +1998,7 @@ extern "C" qe_gpu_status qe_gpu_join_autotune(qe_gpu_ctx* ctx, const qe_dev_col*
                     ctx->tune_join_blocks_per_cu = bpc;
                     double t = 1e30;
                     for (int rep = 0; rep < 2; ++rep) {
+                        resolve_timings(ctx);
                         ctx->timings.erase("join_agg_probe");
                         qe_dev_table* tab = nullptr;
                         const qe_gpu_status st = qe_gpu_join_aggregate(ctx, probe_cols, nprobe, build_cols, nbuild,
@@ -2005,6 +2006,7 @@ extern "C" qe_gpu_status qe_gpu_join_autotune(qe_gpu_ctx* ctx, const qe_dev_col*
                                                                        group_keys, ngk, aggs, naggs, &tab);
                         if (st != QE_GPU_OK) throw Error{st, ctx->last_error};
                         qe_gpu_table_free(ctx, tab);
+                        resolve_timings(ctx);
                         const auto it = ctx->timings.find("join_agg_probe");
                         const double ms = it == ctx->timings.end() ? 0.0 : it->second.ms;
                         t = std::min(t, ms);
@@ -2025,6 +2027,7 @@ extern "C" qe_gpu_status qe_gpu_join_autotune(qe_gpu_ctx* ctx, const qe_dev_col*
             throw;
         }
         ctx->timing = timing;
+        resolve_timings(ctx);
         ctx->timings.erase("join_agg_probe");
         ctx->tune_join_pairs = best.pairs;
         ctx->tune_join_blocks_per_cu = best.blocks_per_cu;
