@@ -13,6 +13,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
+import _device_cases as DC
 import _oracle as O
 
 sys.path.insert(0, str(Path(__file__).resolve().parent.parent / "oracle"))
@@ -102,20 +103,9 @@ def test_empty_sides(ctx, qe):
 
 @pytest.mark.parametrize("case", range(10))
 def test_random_unsorted_inputs(ctx, qe, case):
-    r = np.random.default_rng(4000 + case)
-    nb, np_ = int(r.choice([1, 100, 5000, 60000])), int(r.choice([1, 100, 5000, 30000]))
-    nsym = int(r.choice([1, 7, 300]))
-    bsym = r.integers(0, nsym, nb)
-    bts = r.permutation(nb * 5)[:nb] * 3 - 1000  # distinct, unsorted, some negative
-    psym = r.integers(0, nsym + 2, np_)
-    pts = r.integers(-2000, nb * 15 + 10, np_)
-    bn = tuple(np.nonzero(r.random(nb) < 0.02)[0])
-    pn = tuple(np.nonzero(r.random(np_) < 0.02)[0])
-    tcol = int(r.choice([TS, I64]))
-    bcols = [col(bsym, I32, bn), col(bts, tcol)]
-    pcols = [col(psym, I32), col(pts, tcol, pn)]
-    tol = None if case % 3 else int(r.integers(0, 40))
-    check_all(ctx, qe, pcols, bcols, [0], [0], 1, 1, tol=tol, piece_rows=int(r.choice([4, 16, 64])))
+    # Inputs shared with the device models' mutation catalog.
+    pcols, bcols, tol, piece_rows = DC.asof_random_case(case)
+    check_all(ctx, qe, pcols, bcols, [0], [0], 1, 1, tol=tol, piece_rows=piece_rows)
 
 
 @pytest.mark.parametrize("piece_rows", [1, 2, 4, 16, 64])
@@ -125,14 +115,21 @@ def test_duplicate_time_runs_across_pieces(ctx, qe, piece_rows):
     # probes at, between, before and after the runs: the search inside a piece
     # starts from the spline's prediction and must end on the run's last row
     # in the stable (key, time) order.
-    r = np.random.default_rng(808 + piece_rows)
-    nb, np_ = 30000, 20000
-    bsym = r.integers(0, 3, nb)
-    bts = r.integers(0, 100, nb) * 10
-    psym = r.integers(0, 4, np_)
-    pts = r.integers(-20, 1020, np_)
-    check_all(ctx, qe, [col(psym, I32), col(pts, TS)], [col(bsym, I32), col(bts, TS)], [0], [0], 1, 1,
-              piece_rows=piece_rows)
+    pcols, bcols = DC.asof_dup_runs_case(piece_rows)
+    check_all(ctx, qe, pcols, bcols, [0], [0], 1, 1, piece_rows=piece_rows)
+
+
+@pytest.mark.parametrize("piece_rows", [4, 16, 64])
+def test_knots_at_router_bucket_edges(ctx, qe, piece_rows):
+    # Knots a few hundred thousand microseconds past the linear router's
+    # bucket edges over spans near 2^42 us: within the float32 resolution of
+    # the normalised time, so a probe routed with other arithmetic than the
+    # table's lands one bucket low and takes the wrong piece without checking
+    # its successor. Probes at every knot, one microsecond either side and at
+    # every row: the learned probe (MLP and linear routers) must equal the
+    # binary search and the oracle on all of them.
+    pcols, bcols = DC.asof_bucket_edge_case(piece_rows)
+    check_all(ctx, qe, pcols, bcols, [0], [0], 1, 1, piece_rows=piece_rows)
 
 
 @pytest.mark.parametrize("n", [1, 63, 4095, 4096, 4097, 100_003])

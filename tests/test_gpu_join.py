@@ -9,6 +9,7 @@ are bit-exact; the fused aggregate is bit-exact for keys and COUNTs and within
 import numpy as np
 import pytest
 
+import _device_cases as DC
 import _oracle as O
 from _compare import assert_same, table_rows
 
@@ -195,23 +196,14 @@ def test_fused_direct_stream_shapes(ctx, qe, pairs, case):
     # Direct unique build map with the streaming probe: odd and tiny probe
     # lengths (tail row), I32 / I64 keys, NULL probe keys, narrow and nullable
     # value columns, 0..3 distinct value columns (3 takes the per-row kernel).
-    r = np.random.default_rng(900 + case)
-    kt = I32 if case % 2 else I64
-    n_b = [1, 64, 3000, 70000][case % 4]
-    n_p = [1, 3, 4097, 100001, 65536, 262143, 2, 5][case]
-    keys = r.permutation(2 * n_b)[:n_b] - n_b  # unique, negative and positive, with gaps
-    bcols = [col(keys, kt), col(r.integers(0, 37, n_b), I32)]
-    knull = tuple(np.nonzero(r.random(n_p) < 0.03)[0]) if case % 3 == 0 else ()
-    vnull = tuple(np.nonzero(r.random(n_p) < 0.05)[0]) if case % 2 else ()
-    pcols = [col(r.integers(-n_b - 5, n_b + 5, n_p), kt, knull),
-             col(r.integers(-1000, 1000, n_p), I32, vnull),
-             col(r.normal(size=n_p) * 100, F64, vnull),
-             col(r.integers(-2**40, 2**40, n_p), I64)]
+    # Inputs shared with the device models' mutation catalog.
+    pcols, bcols = DC.direct_stream_case(case)
     aggsets = [[(CS, 0)],
                [(SUM, 2), (CS, 0)],
                [(SUM, 1), (MIN, 1), (MAX, 1), (CNT, 1), (AVG, 1)],
                [(SUM, 2), (MAX, 2), (MIN, 3), (SUM, 3), (CS, 0)],
-               [(SUM, 1), (SUM, 2), (SUM, 3)]]
+               [(SUM, 1), (SUM, 2), (SUM, 3)],
+               [(SUM, 3), (CS, 0)]]
     ctx.set_tuning(qe.TUNE_JOIN_PAIRS, pairs)
     try:
         for aggs in aggsets:
@@ -226,26 +218,9 @@ def test_fused_stream_wide_maps(ctx, case):
     and uniform keys with 10% misses, I32/I64 keys (negative offsets), one or
     two value columns (F64, I64, narrow I32: every width specialisation of the
     stream loads), an odd probe length, and (case 4) each probe tile's keys in
-    one 4096-key stretch of the map. Against the oracle."""
-    r = np.random.default_rng(1300 + case)
-    n_b = [1 << 16, 1 << 17, 50000, 1 << 16, 1 << 16][case]
-    n_p = [1 << 20, (1 << 20) + 3, 1 << 21, 1 << 20, 1 << 20][case]
-    kt = I32 if case % 2 else I64
-    kmin = [0, -5000, 123, 7, 0][case]
-    keys = r.permutation(n_b) + kmin
-    if case == 4:
-        tile = 1024 * 4 * 2  # rows per probe tile (4 pairs per thread)
-        t = np.arange(n_p) // tile
-        pkv = kmin + (t % 16) * 4096 + r.integers(0, 4096, n_p)
-    else:
-        s = [1.1, 0.0, 1.3, 1.05][case]
-        ranks = np.minimum(r.zipf(s, n_p) - 1, n_b - 1) if s > 0 else r.integers(0, n_b, n_p)
-        pkv = np.where(r.random(n_p) < 0.1, kmin + n_b + r.integers(0, 1000, n_p), kmin + ranks)
-    pcols = [col(pkv, kt), col(r.uniform(0, 1000, n_p), F64), col(r.integers(-2**40, 2**40, n_p), I64),
-             col(r.integers(-1000, 1000, n_p), I32)]
-    bcols = [col(keys, kt), col(r.integers(0, [10000, 3000, 500, 60000, 10000][case], n_b), I32)]
-    aggs = [[(SUM, 1), (CS, 0)], [(SUM, 1), (SUM, 2), (CS, 0)], [(AVG, 3), (CS, 0)],
-            [(SUM, 2), (AVG, 1), (CS, 0)], [(SUM, 1), (CS, 0)]][case]
+    one 4096-key stretch of the map. Against the oracle; inputs shared with the
+    device models' mutation catalog."""
+    pcols, bcols, aggs = DC.wide_map_case(case)
     join_agg_case(ctx, pcols, bcols, [0], [0], INNER, [5], aggs)
 
 
