@@ -576,7 +576,10 @@ AggResult* aggregate_impl(const orc_col* cols, uint64_t nrows, const uint32_t* k
                 }
                 continue;
             }
-            const double dsum = (sum_mode == 1) ? c.d + c.comp : c.d;
+            // A non-finite running sum is already the row-order fold's answer
+            // (+/-inf, or NaN from a NaN or from both infinities); the
+            // compensation term is then NaN (inf - inf) and is dropped.
+            const double dsum = (sum_mode == 1 && std::isfinite(c.d)) ? c.d + c.comp : c.d;
             if (f == Avg)
                 res->out_d[a][g] = (isf[a] ? dsum : static_cast<double>(c.i)) /
                                    static_cast<double>(c.cnt);

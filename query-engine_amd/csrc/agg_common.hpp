@@ -504,6 +504,11 @@ __device__ inline void global_update_vals(uint64_t* st, uint64_t S, const AggPro
 // scales, those words, the outlier list and the overflow word (all in `buf`).
 // No host round trip: the aggregation kernels read the scales on the device.
 void prepare_f64(qe_gpu_ctx* ctx, AggProg& p, uint64_t* st, uint64_t S, uint64_t n_rows, bool exact, Scratch& buf);
+// Its two halves: the column scales (sample, exponent bound; needs only the
+// input rows, so it can be queued before the slot count is known) and the
+// state's NaN/inf flags and outlier sums (needs the state).
+void prepare_f64_scales(qe_gpu_ctx* ctx, AggProg& p, uint64_t n_rows, bool exact, Scratch& buf);
+void prepare_f64_state(qe_gpu_ctx* ctx, AggProg& p, uint64_t* st, uint64_t S);
 // Sum the outlier list into the state's osum words (emit_groups does this; a
 // path whose state is relabelled afterwards calls it before the relabelling).
 void fold_outliers(qe_gpu_ctx* ctx, const AggProg& p, uint64_t* st, uint64_t S);

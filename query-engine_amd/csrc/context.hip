@@ -511,11 +511,15 @@ MinMax column_minmax(qe_gpu_ctx* ctx, const qe_dev_col* c, uint64_t n) {
 }
 
 void column_minmax_n(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, uint32_t ncol, uint64_t n, MinMax* out) {
+    column_minmax_finish(ctx, column_minmax_start(ctx, cols, ncol, n), out);
+}
+
+MinMaxReadback column_minmax_start(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, uint32_t ncol, uint64_t n) {
     check(ncol >= 1 && ncol <= kMaxMinMaxCols, QE_GPU_UNSUPPORTED, "min/max of too many columns at once");
-    if (n == 0) {
-        for (uint32_t c = 0; c < ncol; ++c) out[c] = MinMax{INT64_MAX, INT64_MIN, 0};
-        return;
-    }
+    MinMaxReadback rb;
+    rb.ncol = ncol;
+    rb.n = n;
+    if (n == 0) return rb;
     // About 8-16 rows per thread: one or two batches of eight independent loads.
     const uint32_t blocks = static_cast<uint32_t>(grid_for((n + 7) / 8, 256, ctx->num_cus * 16));
     Scratch part(ctx, size_t(blocks) * 3 * 8 * ncol), s(ctx, size_t(3) * 8 * ncol);
@@ -527,10 +531,20 @@ void column_minmax_n(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, uint32_t nc
     minmax_kernel<<<dim3(blocks, ncol), 256, 0, ctx->stream>>>(C, n, part.as<int64_t>());
     minmax_final<<<ncol, 256, 0, ctx->stream>>>(part.as<int64_t>(), blocks, s.as<int64_t>());
     QE_HIP(hipGetLastError());
-    int64_t* h = static_cast<int64_t*>(pinned(ctx, size_t(24) * ncol));
-    QE_HIP(hipMemcpyAsync(h, s.p, size_t(24) * ncol, hipMemcpyDeviceToHost, ctx->stream));
-    QE_HIP(hipStreamSynchronize(ctx->stream));
-    for (uint32_t c = 0; c < ncol; ++c) out[c] = MinMax{h[3 * c], h[3 * c + 1], static_cast<uint64_t>(h[3 * c + 2])};
+    rb.h = static_cast<int64_t*>(pinned(ctx, size_t(24) * ncol));
+    QE_HIP(hipMemcpyAsync(rb.h, s.p, size_t(24) * ncol, hipMemcpyDeviceToHost, ctx->stream));
+    QE_HIP(hipEventRecord(ctx->readback_event, ctx->stream));
+    return rb;
+}
+
+void column_minmax_finish(qe_gpu_ctx* ctx, const MinMaxReadback& rb, MinMax* out) {
+    if (rb.n == 0) {
+        for (uint32_t c = 0; c < rb.ncol; ++c) out[c] = MinMax{INT64_MAX, INT64_MIN, 0};
+        return;
+    }
+    QE_HIP(hipEventSynchronize(ctx->readback_event));
+    const int64_t* h = rb.h;
+    for (uint32_t c = 0; c < rb.ncol; ++c) out[c] = MinMax{h[3 * c], h[3 * c + 1], static_cast<uint64_t>(h[3 * c + 2])};
 }
 
 }  // namespace qe_gpu
@@ -555,6 +569,11 @@ qe_gpu_status qe_gpu_ctx_create(int device, qe_gpu_ctx** out) {
         delete ctx;
         return QE_GPU_HIP_ERROR;
     }
+    if (hipEventCreateWithFlags(&ctx->readback_event, hipEventDisableTiming) != hipSuccess) {
+        hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return QE_GPU_HIP_ERROR;
+    }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->num_cus = prop.multiProcessorCount;
     qe_gpu::noalloc_setup(device);
@@ -572,6 +591,7 @@ void qe_gpu_ctx_destroy(qe_gpu_ctx* ctx) {
     for (auto& kv : ctx->free_blocks) hipFree(kv.second);
     for (auto& kv : ctx->live_blocks) hipFree(kv.first);
     if (ctx->pinned) hipHostFree(ctx->pinned);
+    hipEventDestroy(ctx->readback_event);
     hipStreamDestroy(ctx->stream);
     delete ctx;
 }

@@ -104,10 +104,12 @@ struct F64Cols {
 };
 
 // Largest biased exponent of the finite, non-NULL values among rows 0, stride,
-// 2*stride, ... of column blockIdx.y: one word per (column, block).
-__global__ void f64_exp_kernel(F64Cols fc, uint64_t n, uint64_t stride, uint32_t* part) {
+// 2*stride, ... of column blockIdx.y: one word per (column, block). The first
+// block also clears the overflow word and the outlier count (ovf[0..1]).
+__global__ void f64_exp_kernel(F64Cols fc, uint64_t n, uint64_t stride, uint32_t* part, unsigned int* ovf) {
     __shared__ uint32_t bm;
     if (threadIdx.x == 0) bm = 0;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 2) ovf[threadIdx.x] = 0;
     __syncthreads();
     const ColView c = fc.col[blockIdx.y];
     const uint64_t ns = stride ? (n + stride - 1) / stride : 0;
@@ -139,6 +141,18 @@ __global__ void f64_scale_kernel(const uint32_t* part, uint32_t nblk, F64Cols fc
 }
 
 void prepare_f64(qe_gpu_ctx* ctx, AggProg& p, uint64_t* st, uint64_t S, uint64_t n_rows, bool exact, Scratch& buf) {
+    prepare_f64_scales(ctx, p, n_rows, exact, buf);
+    prepare_f64_state(ctx, p, st, S);
+}
+
+void prepare_f64_state(qe_gpu_ctx* ctx, AggProg& p, uint64_t* st, uint64_t S) {
+    if (p.nfs == 0) return;
+    if (st) memset_async(ctx, st + state_flags(S, p, 0), 0, size_t(2 * p.nfs) * S * 8);  // flags, outlier sums
+    p.fflag = st ? st + state_flags(S, p, 0) : nullptr;
+    p.fflag_stride = S;
+}
+
+void prepare_f64_scales(qe_gpu_ctx* ctx, AggProg& p, uint64_t n_rows, bool exact, Scratch& buf) {
     if (p.nfs == 0) return;
     F64Cols fc{};
     for (int a = 0; a < p.n; ++a)
@@ -154,15 +168,11 @@ void prepare_f64(qe_gpu_ctx* ctx, AggProg& p, uint64_t* st, uint64_t S, uint64_t
     unsigned int* ovf = reinterpret_cast<unsigned int*>(fsc + 2 * kMaxAggs);
     uint64_t* fout = reinterpret_cast<uint64_t*>(ovf + 2);
     uint32_t* part = reinterpret_cast<uint32_t*>(fout + 2 * size_t(kMaxOutliers));
-    memset_async(ctx, ovf, 0, 8);
-    if (st) memset_async(ctx, st + state_flags(S, p, 0), 0, size_t(2 * p.nfs) * S * 8);  // flags, outlier sums
     const uint64_t stride = exact ? 1 : std::max<uint64_t>(1, n_rows >> 16);
-    f64_exp_kernel<<<dim3(nblk, fc.n), 256, 0, ctx->stream>>>(fc, n_rows, n_rows ? stride : 0, part);
+    f64_exp_kernel<<<dim3(nblk, fc.n), 256, 0, ctx->stream>>>(fc, n_rows, n_rows ? stride : 0, part, ovf);
     f64_scale_kernel<<<1, 64, 0, ctx->stream>>>(part, nblk, fc, exact ? 0 : 2, fsc);
     QE_HIP(hipGetLastError());
     p.fsc = fsc;
-    p.fflag = st ? st + state_flags(S, p, 0) : nullptr;
-    p.fflag_stride = S;
     p.fovf = ovf;
     p.fout = fout;
     p.fout_n = ovf + 1;

@@ -131,6 +131,9 @@ struct qe_gpu_ctx {
     // Small pinned host staging buffer for scalars read back after kernels.
     void* pinned = nullptr;
     size_t pinned_bytes = 0;
+    // Recorded after a read-back copy that later work on the stream need not
+    // delay (column_minmax_start / _finish).
+    hipEvent_t readback_event = nullptr;
 };
 
 namespace qe_gpu {
@@ -446,6 +449,17 @@ struct MinMax {
 MinMax column_minmax(qe_gpu_ctx* ctx, const qe_dev_col* c, uint64_t n);
 // Several columns of length n, one read-back.
 void column_minmax_n(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, uint32_t ncol, uint64_t n, MinMax* out);
+// The same in two steps, so the caller can queue device work that does not
+// depend on the ranges between them: start() queues the kernels and the copy
+// to pinned memory, finish() waits for the copy alone (an event recorded
+// after it) and fills out.
+struct MinMaxReadback {
+    uint32_t ncol = 0;
+    uint64_t n = 0;
+    int64_t* h = nullptr;
+};
+MinMaxReadback column_minmax_start(qe_gpu_ctx* ctx, const qe_dev_col* const* cols, uint32_t ncol, uint64_t n);
+void column_minmax_finish(qe_gpu_ctx* ctx, const MinMaxReadback& rb, MinMax* out);
 
 // Stable LSD radix sort of (u64 key, u32 value) pairs on bits [begin_bit,
 // end_bit), in place (sort.hip).

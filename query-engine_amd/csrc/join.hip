@@ -1170,36 +1170,35 @@ MapBuild build_direct_map(qe_gpu_ctx* ctx, const qe_dev_col* key, uint64_t nb, u
     const uint32_t nparts = static_cast<uint32_t>(np64);
     const uint32_t ntiles = static_cast<uint32_t>((nb + kMapTile - 1) / kMapTile);
     const uint64_t nh = uint64_t(nparts) * ntiles;
-    dup = Scratch(ctx, 4);
     const ColView kv = view(key);
     const int kw = static_cast<int>(type_width(key->type));
     constexpr bool k16 = sizeof(MapT) == 2;  // 4-byte entries, LDS-sorted scatter
     const size_t build_lds = (size_t(1) << pb) * sizeof(MapT) + ((size_t(1) << pb) / 32) * 4;
     if (k16 && range <= 2 * total) {
         // Dense keys: fixed buckets of 2^pb entries per partition (at most 2x
-        // the entries of a scanned layout), so no count pass and no scan.
-        Scratch gcur(ctx, size_t(nparts) * 4);
+        // the entries of a scanned layout), so no count pass and no scan. The
+        // bucket cursors follow the duplicate flag in its allocation: one
+        // clear for both.
+        dup = Scratch(ctx, (size_t(nparts) + 1) * 4);
+        uint32_t* gcur = dup.as<uint32_t>() + 1;
         Scratch ent(ctx, (size_t(nparts) << pb) * 4);
-        memset_async(ctx, gcur.p, 0, size_t(nparts) * 4);
+        memset_async(ctx, dup.p, 0, (size_t(nparts) + 1) * 4);
         const size_t lds = size_t(nparts) * 12 + size_t(kMapTile) * 6 + (kMaxWaves + 1) * 4;
         if (kw == 8)
-            launch_scatter16<8>(ctx, ntiles, lds, kv, nb, kmin, pb, nparts, nullptr, gcur.as<uint32_t>(), ms,
-                                 ent.as<uint32_t>());
+            launch_scatter16<8>(ctx, ntiles, lds, kv, nb, kmin, pb, nparts, nullptr, gcur, ms, ent.as<uint32_t>());
         else if (kw == 4)
-            launch_scatter16<4>(ctx, ntiles, lds, kv, nb, kmin, pb, nparts, nullptr, gcur.as<uint32_t>(), ms,
-                                 ent.as<uint32_t>());
+            launch_scatter16<4>(ctx, ntiles, lds, kv, nb, kmin, pb, nparts, nullptr, gcur, ms, ent.as<uint32_t>());
         else
-            launch_scatter16<1>(ctx, ntiles, lds, kv, nb, kmin, pb, nparts, nullptr, gcur.as<uint32_t>(), ms,
-                                 ent.as<uint32_t>());
+            launch_scatter16<1>(ctx, ntiles, lds, kv, nb, kmin, pb, nparts, nullptr, gcur, ms, ent.as<uint32_t>());
         QE_HIP(hipGetLastError());
         gmap = Scratch(ctx, range * sizeof(MapT));
-        memset_async(ctx, dup.p, 0, 4);
         map_part_build<MapT, uint32_t><<<nparts, 1024, build_lds, ctx->stream>>>(
-            ent.as<uint32_t>(), nullptr, gcur.as<uint32_t>(), ntiles, nparts, total, pb, range, gmap.as<MapT>(),
+            ent.as<uint32_t>(), nullptr, gcur, ntiles, nparts, total, pb, range, gmap.as<MapT>(),
             dup.as<unsigned int>());
         QE_HIP(hipGetLastError());
         return MapBuild::kOk;
     }
+    dup = Scratch(ctx, 4);
     Scratch hist(ctx, nh * 4), offs(ctx, nh * 8), sums(ctx, scan_sums_words(nh) * 8);
     if (kw == 8)
         map_part_hist<8><<<ntiles, 1024, nparts * 4, ctx->stream>>>(kv, nb, kmin, pb, nparts, ntiles,
@@ -1589,13 +1588,19 @@ qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* pro
             const bool want_g = ngk == 1 && is_int_like(g0->type) && nb > 0;
             const bool want_k = int_key && nb > 0 && np > 0;
             MinMax g_mm{}, k_mm{};
+            Scratch fbuf;  // exact F64 sums: scales, flags, outliers
             {
                 const qe_dev_col* mc[2];
                 uint32_t nm = 0;
                 if (want_g) mc[nm++] = g0;
                 if (want_k) mc[nm++] = bk[0];
                 MinMax r[2];
-                if (nm) column_minmax_n(ctx, mc, nm, nb, r);
+                MinMaxReadback rb;
+                if (nm) rb = column_minmax_start(ctx, mc, nm, nb);
+                // The F64 scales need only the probe's values: queued here,
+                // they run while the host waits for the ranges.
+                prepare_f64_scales(ctx, p, np, exact, fbuf);
+                if (nm) column_minmax_finish(ctx, rb, r);
                 if (want_g) g_mm = r[0];
                 if (want_k) k_mm = r[want_g ? 1 : 0];
             }
@@ -1643,13 +1648,22 @@ qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* pro
                                       lds_per_block ? (kLdsBudget + 256) / lds_per_block : 8));
             const unsigned lds_blocks = static_cast<unsigned>(
                 std::max<uint64_t>(1, std::min<uint64_t>(uint64_t(ctx->num_cus) * bpc, (np + 1023) / 1024)));
-            Scratch fbuf;  // exact F64 sums: scales, flags, outliers
-            prepare_f64(ctx, p, st.as<uint64_t>(), S, np, exact, fbuf);
+            // The state's F64 NaN/inf flags and outlier sums cleared: before
+            // the first kernel that updates the state or the emission, on every
+            // path, and as late as each path allows (the dense path clears them
+            // after the map build's launches, so the build starts sooner).
+            bool f64_state_ready = false;
+            auto ensure_f64_state = [&]() {
+                if (f64_state_ready) return;
+                prepare_f64_state(ctx, p, st.as<uint64_t>(), S);
+                f64_state_ready = true;
+            };
             const bool small = S < 0xFFFF;
             const uint64_t* kv = pk[0]->all_valid ? nullptr : pk[0]->validity;
 
             // Probe over a direct unique map (gmap[key - kmin] = group slot).
             auto probe_direct = [&](const Scratch& gmap, int64_t kmin, uint64_t range) {
+                ensure_f64_state();
                 auto launch = [&](auto key_tag, auto map_tag) {
                     using KeyT = decltype(key_tag);
                     using MapT = decltype(map_tag);
@@ -1789,6 +1803,7 @@ qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* pro
             if (want_k) {
                 const MinMax mm = k_mm;
                 if (mm.valid == 0) {  // every build key NULL: nothing matches
+                    ensure_f64_state();
                     init_state(ctx, st.as<uint64_t>(), S, p);
                     done = true;
                 } else {
@@ -1818,11 +1833,16 @@ qe_gpu_status join_aggregate_codes(qe_gpu_ctx* ctx, const qe_dev_col* const* pro
                                                           dup.as<unsigned int>(), &dupv);
                             if (dupv == 0) return t;
                             qe_gpu_table_free(ctx, t);
+                            // The row-list path starts over: an empty outlier
+                            // list and cleared flags.
+                            prepare_f64_scales(ctx, p, np, exact, fbuf);
+                            f64_state_ready = false;
                         }
                     }
                 }
             }
 
+            ensure_f64_state();
             JoinIndex ix;
             if (done) {
             } else if (nb > 0) {

@@ -224,6 +224,28 @@ def test_fused_stream_wide_maps(ctx, case):
     join_agg_case(ctx, pcols, bcols, [0], [0], INNER, [5], aggs)
 
 
+def test_dense_map_duplicate_falls_back_with_clean_f64_state(ctx):
+    # Dense build keys with one repeated key: the direct map is built and
+    # probed, the duplicate is found with the emission, and the query runs
+    # again through the row lists. F64 values include outliers on rows the
+    # scale sample skips (every 16th row is sampled at 2^20 rows), a NaN and
+    # an infinity: the second run must start from an empty outlier list and
+    # cleared flags, so nothing is counted twice.
+    r = np.random.default_rng(2024)
+    n_b, n_p = 50000, 1 << 20
+    keys = r.permutation(n_b)
+    keys[7] = keys[8]  # one repeated key
+    pk = r.integers(0, n_b + 100, n_p)
+    pv = r.uniform(0, 1, n_p)
+    out_rows = np.arange(1, n_p, 16)[::4000]  # never sampled
+    pv[out_rows] = r.uniform(1e6, 1e7, len(out_rows)) * np.where(r.random(len(out_rows)) < 0.5, -1, 1)
+    pv[5] = np.nan
+    pv[17] = np.inf
+    pcols = [col(pk, I64), col(pv, F64)]
+    bcols = [col(keys, I64), col(r.integers(0, 300, n_b), I32)]
+    join_agg_case(ctx, pcols, bcols, [0], [0], INNER, [3], [(SUM, 1), (CS, 0), (AVG, 1)])
+
+
 def test_fused_duplicate_build_keys_list_path(ctx, rng):
     n_b, n_p = 5000, 50000
     pcols = [col(rng.integers(0, 2000, n_p), I64), col(rng.uniform(0, 10, n_p), F64, nulls=(3, 5, 7))]

@@ -238,6 +238,22 @@ def test_agg_all_null_group_empty_and_nan():
     assert math.isnan(d[4][1]) and math.isnan(d[4][2]) and d[4][3] == 1
 
 
+def test_agg_sum_nonfinite_both_modes():
+    # The row-order fold (ops/aggregate.cpp:128-131) with IEEE addition: a lone
+    # +inf or -inf stays, both infinities or any NaN give NaN. The compensated
+    # mode must agree (its correction term is NaN once the sum is infinite),
+    # and stays exact where the naive fold cancels (1e16 + 1 - 1e16).
+    inf, nan = float("inf"), float("nan")
+    g = col([1, 1, 2, 2, 3, 3, 4, 4, 5, 5, 5], I32)
+    v = col([inf, 1.0, -inf, 2.0, inf, -inf, nan, 3.0, 1e16, 1.0, -1e16], F64)
+    for mode in (0, 1):
+        d = {r[0]: r for r in O.aggregate([g, v], [0], [(SUM, 1), (AVG, 1)], sum_mode=mode)}
+        assert d[1][1] == inf and d[1][2] == inf, mode
+        assert d[2][1] == -inf and d[2][2] == -inf, mode
+        assert math.isnan(d[3][1]) and math.isnan(d[4][1]), mode
+        assert d[5][1] == (1.0 if mode == 1 else 0.0), mode
+
+
 def _py_groupby(cols, keys, aggs):
     groups = {}
     order = []
