@@ -435,35 +435,58 @@ constexpr uint32_t kEmitSmall = 32768;
 __global__ void __launch_bounds__(1024) group_rows_kernel(const uint64_t* st, uint32_t S, int all_slots,
                                                           uint32_t* slot_of, unsigned long long* out, uint32_t nzero,
                                                           const unsigned int* flag, const unsigned int* ovf) {
-    __shared__ uint32_t wcnt[kMaxWaves];
-    __shared__ uint32_t base_s;
-    const unsigned lane = lane_id(), w = threadIdx.x / kWave;
-    const uint64_t below = (uint64_t(1) << lane) - 1;
-    if (threadIdx.x == 0) base_s = 0;
-    for (uint32_t i = 1 + threadIdx.x; i < nzero; i += blockDim.x) out[i] = 0;
-    if (threadIdx.x == 0 && nzero) {
+    __shared__ uint32_t ws[kMaxWaves + 1];
+    const uint32_t t = threadIdx.x, lane = lane_id(), w = t / kWave, nw = blockDim.x / kWave;
+    for (uint32_t i = 1 + t; i < nzero; i += blockDim.x) out[i] = 0;
+    if (t == 0 && nzero) {
         out[nzero] = flag ? *flag : 0u;
         out[nzero + 1] = ovf ? *ovf : 0u;
     }
-    __syncthreads();
-    for (uint32_t c0 = 0; c0 < S; c0 += 1024) {
-        const uint32_t s = c0 + threadIdx.x;
-        const bool f = s < S && (all_slots || st[s] > 0);
-        const uint64_t m = __ballot(f);
-        if (lane == 0) wcnt[w] = static_cast<uint32_t>(__popcll(m));
-        __syncthreads();
-        uint32_t off = base_s;
-        for (unsigned k = 0; k < w; ++k) off += wcnt[k];
-        if (f) slot_of[off + static_cast<uint32_t>(__popcll(m & below))] = s;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t t = 0;
-            for (unsigned k = 0; k < blockDim.x / kWave; ++k) t += wcnt[k];
-            base_s += t;
+    // Thread t owns the k consecutive slots from t * k (k <= 32 for S <=
+    // kEmitSmall): their row counters are read eight at a time, the owned
+    // groups counted, and one block scan of the counts places every thread's
+    // groups, in slot order.
+    const uint32_t k = (S + blockDim.x - 1) / blockDim.x;
+    const uint32_t s0 = t * k;
+    uint32_t bits = 0;
+    for (uint32_t j0 = 0; j0 < k; j0 += 8) {
+        uint64_t r[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const uint32_t s = s0 + j0 + u;
+            r[u] = st[j0 + u < k && s < S ? s : 0];
         }
-        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const uint32_t s = s0 + j0 + u;
+            if (j0 + u < k && s < S && (all_slots || r[u] > 0)) bits |= 1u << (j0 + u);
+        }
     }
-    if (threadIdx.x == 0) out[0] = base_s;
+    const uint32_t cnt = static_cast<uint32_t>(__popc(bits));
+    uint32_t x = cnt;  // inclusive wave scan of the counts
+    for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off);
+        if (lane >= static_cast<unsigned>(off)) x += y;
+    }
+    if (lane == kWave - 1) ws[w] = x;
+    __syncthreads();
+    if (t == 0) {
+        uint32_t run = 0;
+        for (uint32_t q = 0; q < nw; ++q) {
+            const uint32_t c = ws[q];
+            ws[q] = run;
+            run += c;
+        }
+        ws[kMaxWaves] = run;
+    }
+    __syncthreads();
+    uint32_t at = ws[w] + x - cnt;
+    while (bits) {
+        const int j = __ffs(bits) - 1;
+        bits &= bits - 1;
+        slot_of[at++] = s0 + j;
+    }
+    if (t == 0) out[0] = ws[kMaxWaves];
 }
 
 // Output validity words of each column (kernel argument: no upload).

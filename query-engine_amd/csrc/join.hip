@@ -1101,13 +1101,26 @@ __global__ void __launch_bounds__(1024) map_part_build(const EntT* ent, const ui
         b = offs[size_t(part) * ntiles];
         e = (part + 1 < nparts) ? offs[size_t(part + 1) * ntiles] : total;
     }
-    for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) {
-        const uint64_t v = ent[i];
-        constexpr int kShift = sizeof(EntT) == 4 ? 16 : 32;  // offset | slot packing of the scatter
-        const uint32_t o = static_cast<uint32_t>(v >> kShift);
-        const uint32_t bit = 1u << (o & 31);
-        if (atomicOr(&bm[o >> 5], bit) & bit) d = true;
-        m[o] = static_cast<MapT>(static_cast<uint32_t>(v & ((uint64_t(1) << kShift) - 1)));
+    // Eight entries per thread in flight (unconditional loads at a clamped
+    // index through a global pointer), then their LDS updates.
+    constexpr int kShift = sizeof(EntT) == 4 ? 16 : 32;  // offset | slot packing of the scatter
+    constexpr int U = 8;
+    const gptr<EntT> eg = gcast<EntT>(ent);
+    for (uint64_t i0 = b + threadIdx.x; i0 < e; i0 += uint64_t(U) * blockDim.x) {
+        uint64_t v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t i = i0 + uint64_t(u) * blockDim.x;
+            v[u] = eg[i < e ? i : e - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (i0 + uint64_t(u) * blockDim.x >= e) break;
+            const uint32_t o = static_cast<uint32_t>(v[u] >> kShift);
+            const uint32_t bit = 1u << (o & 31);
+            if (atomicOr(&bm[o >> 5], bit) & bit) d = true;
+            m[o] = static_cast<MapT>(static_cast<uint32_t>(v[u] & ((uint64_t(1) << kShift) - 1)));
+        }
     }
     if (d) atomicOr(dup, 1u);
     __syncthreads();
