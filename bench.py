@@ -243,13 +243,16 @@ def asof_workload(qe, ctx, args):
     ctx.sync()
     t1 = time.perf_counter()
     ctx.set_tuning(qe.TUNE_ASOF_BUCKET_KNOTS, args.bucket_knots)
+    ctx.set_tuning(qe.TUNE_ASOF_DATA_LINES, args.data_lines)
     idx.train(args.piece_rows, args.train_steps, args.train_lr)
     ctx.set_tuning(qe.TUNE_ASOF_BUCKET_KNOTS, 1)
+    ctx.set_tuning(qe.TUNE_ASOF_DATA_LINES, 1)
     ctx.sync()
     t2 = time.perf_counter()
     out = qe.DevBuffer(ctx, 4 * ln)
     res = {"right_rows": rn, "left_rows": ln, "build_sort_s": t1 - t0, "train_s": t2 - t1,
-           "piece_rows": args.piece_rows, "train_steps": args.train_steps, "bucket_knots": args.bucket_knots}
+           "piece_rows": args.piece_rows, "train_steps": args.train_steps, "bucket_knots": args.bucket_knots,
+           "data_lines": args.data_lines}
     results = {}
     srt = qe.ASOF_OUT_SORTED
     for name, mode, timer in (("binary", qe.ASOF_BINARY, "asof_probe_binary"),
@@ -685,8 +688,10 @@ def tick_workload(qe, ctx, args):
     ctx.sync()
     t1 = time.perf_counter()
     ctx.set_tuning(qe.TUNE_ASOF_BUCKET_KNOTS, args.bucket_knots)
-    idx.train(args.piece_rows, args.train_steps, args.train_lr)
+    ctx.set_tuning(qe.TUNE_ASOF_DATA_LINES, args.data_lines)
+    idx.train(args.tick_piece_rows, args.train_steps, args.train_lr)
     ctx.set_tuning(qe.TUNE_ASOF_BUCKET_KNOTS, 1)
+    ctx.set_tuning(qe.TUNE_ASOF_DATA_LINES, 1)
     ctx.sync()
     t2 = time.perf_counter()
     match = qe.DevBuffer(ctx, 4 * nt)
@@ -753,9 +758,11 @@ def tick_workload(qe, ctx, args):
     ref[3].free()
     step(qe.ASOF_LEARNED, timed_stages=True)[3].free()
     cpu = None if args.skip_cpu else tick_cpu_baseline(qe, ctx, args, d, sym_id, sym_name, names, order, code)
+    st = idx.stats()
     res = {"quotes": nq, "trades": nt, "symbols": nsym, "passing_filter": m, "asof_matched": matched,
            "groups": ng, "ms_per_step": wall * 1e3, "trade_rows_per_s": nt / wall,
-           "index_build_s": t1 - t0, "index_train_s": t2 - t1,
+           "index_build_s": t1 - t0, "index_train_s": t2 - t1, "piece_rows": args.tick_piece_rows,
+           "router_buckets": int(st["router_buckets"]), "data_line_buckets": int(st["data_line_buckets"]),
            "stage_ms": {k: v * 1e3 for k, v in stages.items()},
            "counts_sorted_desc": bool(np.all(np.diff(counts.astype(np.int64)) <= 0)),
            "top5": top, "learned_equals_binary": bool(same), "cpu_baseline": cpu}
@@ -843,11 +850,15 @@ def main():
     ap.add_argument("--asof-right", type=int, default=400 << 20)
     ap.add_argument("--asof-left", type=int, default=100 << 20)
     ap.add_argument("--asof-reps", type=int, default=5)
-    ap.add_argument("--piece-rows", type=int, default=16,
-                    help="rows per spline piece (pieces of 16 rows fill one 128-byte line of timestamps)")
+    ap.add_argument("--piece-rows", type=int, default=32,
+                    help="rows per spline piece (with one knot per router bucket, the rows a bucket holds on average)")
+    ap.add_argument("--tick-piece-rows", type=int, default=64,
+                    help="rows per spline piece of the tick pipeline's 2^31-row quote index")
     ap.add_argument("--train-steps", type=int, default=300)
     ap.add_argument("--train-lr", type=float, default=0.003)
-    ap.add_argument("--bucket-knots", type=int, default=2, help="spline knots per learned-router bucket")
+    ap.add_argument("--bucket-knots", type=int, default=1, help="spline knots per learned-router bucket")
+    ap.add_argument("--data-lines", type=int, default=1, choices=[0, 1],
+                    help="learned index: bucket rows inline where they fit (1) or knot lines only (0)")
     ap.add_argument("--asof-sessions", type=int, default=0,
                     help="0: bursty exponential gaps (the tick shape); >0: trading days with U-shaped activity")
     ap.add_argument("--cpu-asof-log2", type=int, default=22)

@@ -134,12 +134,18 @@ qe_gpu_status qe_gpu_timing(qe_gpu_ctx* ctx, const char* kernel, double* ms, uin
  * join -> group-by whose groups exceed LDS (0 = a quarter of the block's rows;
  * rows past the cap fold with HBM atomics). QE_TUNE_JOIN_BLOCKS_PER_CU:
  * 1024-thread blocks per CU of the fused join's streaming probe, 1..8
- * (default 1), capped by what the LDS partials leave room for. */
+ * (default 1), capped by what the LDS partials leave room for.
+ * QE_TUNE_ASOF_DATA_LINES: 1 (default) stores the sorted times of a router
+ * bucket of at most 28 rows spanning less than 2^32 time units, or 56 rows
+ * spanning less than 2^16, inline in the bucket's line, so a learned probe
+ * there reads that one line; 0 keeps knot records in every line. Read by
+ * qe_gpu_asof_train. */
 typedef enum qe_tune_key {
     QE_TUNE_JOIN_PAIRS = 0,
     QE_TUNE_ASOF_BUCKET_KNOTS = 1,
     QE_TUNE_JOIN_SPILL_CAP = 2,
-    QE_TUNE_JOIN_BLOCKS_PER_CU = 3
+    QE_TUNE_JOIN_BLOCKS_PER_CU = 3,
+    QE_TUNE_ASOF_DATA_LINES = 4
 } qe_tune_key;
 qe_gpu_status qe_gpu_set_tuning(qe_gpu_ctx* ctx, int32_t key, int64_t value);
 /* Inspection: the XCD (die) id that ran each of nblocks one-wave blocks of a
@@ -387,6 +393,7 @@ typedef struct qe_asof_stats {
      * only if at least 2% faster (0 when no partition was a candidate). */
     double sample_probe_ms_mlp;
     double sample_probe_ms_linear;
+    uint64_t data_line_buckets;  /* router buckets whose rows sit inline in their line (learned router as probed) */
 } qe_asof_stats;
 qe_gpu_status qe_gpu_asof_stats(qe_gpu_ctx* ctx, const qe_dev_asof* idx, qe_asof_stats* out);
 /* Build-side columns (of the build side's length) gathered into the index's

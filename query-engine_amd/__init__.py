@@ -79,7 +79,7 @@ class AsofStats(C.Structure):
                 ("fallbacks", C.c_uint64), ("final_loss", C.c_double),
                 ("mlp_candidates", C.c_uint64), ("mlp_partitions", C.c_uint64),
                 ("mean_bucket_knots_mlp", C.c_double), ("sample_probe_ms_mlp", C.c_double),
-                ("sample_probe_ms_linear", C.c_double)]
+                ("sample_probe_ms_linear", C.c_double), ("data_line_buckets", C.c_uint64)]
 
 
 P = C.c_void_p
@@ -805,6 +805,7 @@ class BatchTuning(C.Structure):
 
 
 TUNE_JOIN_PAIRS, TUNE_ASOF_BUCKET_KNOTS, TUNE_JOIN_SPILL_CAP, TUNE_JOIN_BLOCKS_PER_CU = 0, 1, 2, 3
+TUNE_ASOF_DATA_LINES = 4
 CMP_LT, CMP_LE, CMP_GT, CMP_GE, CMP_EQ, CMP_NE = range(6)
 BATCH_PERSISTENT, BATCH_PER_BATCH, BATCH_GRAPH = 0, 1, 2
 

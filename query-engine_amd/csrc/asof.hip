@@ -82,6 +82,7 @@ struct qe_dev_asof {
     double sample_ms_mlp = 0, sample_ms_lin = 0;  // router timing sample (0: not timed)
     bool mlp_rejected = false;              // the timing kept the linear router
     uint64_t max_err = 0, mlp_candidates = 0, mlp_parts = 0;
+    uint64_t data_lines = 0;  // buckets of the learned table (as probed) holding their rows inline
     Scratch fallback;  // u64 counter of the last probe
     uint64_t last_fallbacks = 0;
 };
@@ -297,6 +298,33 @@ struct BucketLine {
     SlopePiece sp[kBucketRecs];
 };
 
+// A bucket whose rows are few and close in time keeps the rows themselves in
+// its line instead of knots: the first row's time, the bucket's first sorted
+// position r0, and each row's time minus the first (ascending; unused entries
+// all ones), as 28 32-bit offsets or, for up to 56 rows within 2^16 - 1 time
+// units, 56 16-bit ones (kData16). The last word carries kDataLine [| kData16]
+// | row count, in the place of a knot line's record-7 piece, which never has
+// those bits while the index has fewer than 2^29 knots. The router is monotone
+// in t and built the buckets, so every row before r0 lies in an earlier bucket
+// and has time <= t: the match of a time routed here is sorted position
+// r0 + c - 1, c the count of the bucket's rows with time <= t, with no further
+// read.
+constexpr int kDataRows = 28;
+constexpr int kDataRows16 = 56;
+constexpr uint32_t kDataLine = 0x40000000u;
+constexpr uint32_t kData16 = 0x20000000u;
+constexpr uint32_t kDataCount = 0xFFFFu;
+constexpr uint32_t kDataPad = 0xFFFFFFFFu;
+constexpr uint64_t kDataMaxSpan = 0xFFFFFFFEull;
+constexpr uint64_t kDataMaxSpan16 = 0xFFFEull;
+struct DataLine {
+    int64_t base;
+    uint32_t r0;
+    uint32_t d[kDataRows];  // or uint16_t[kDataRows16] with kData16
+    uint32_t flag;          // kDataLine [| kData16] | rows
+};
+static_assert(sizeof(BucketLine) == 128 && sizeof(DataLine) == 128, "one 128-byte line per bucket");
+
 // Position inside a piece predicted for t >= tau, in [0, cnt). Non-decreasing
 // in t: exact unsigned difference, monotone conversion and fmaf, clamps.
 __device__ inline uint32_t piece_pred(int64_t tau, float slope, uint32_t cnt, int64_t t) {
@@ -454,16 +482,70 @@ __global__ void router_table_kernel(const PartMeta* meta, const uint64_t* kb, ui
     }
 }
 
+// First sorted row of the partition whose bucket is >= b, given k = tab[b]
+// (the first knot whose bucket is >= b): knot k - 1 routes below b, so the row
+// lies after that knot's row and at or before knot k's (the partition's end
+// when k = K); a binary search of at most one piece.
+__device__ inline uint64_t first_row_from_bucket(const PartMeta& pm, const float* W, bool mlp, const int64_t* ts,
+                                                 uint32_t B, uint32_t k, uint32_t b) {
+    if (k == 0) return pm.seg;
+    uint64_t lo = piece_start(pm.seg, k - 1, B) + 1;
+    uint64_t hi = k < pm.K ? piece_start(pm.seg, k, B) : pm.seg + pm.n;
+    while (lo < hi) {
+        const uint64_t mid = lo + (hi - lo) / 2;
+        if (bucket_of(pm, W, mlp, ts[mid]) >= b) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
 // Inline bucket lines from the router table (entries tb .. tb + R - 1 of each
-// partition; entry tb + R is the table's end marker and gets no line).
+// partition; entry tb + R is the table's end marker and gets no line): a data
+// line where data_lines is set and the bucket's rows fit one, else its knots.
+// which: the router the table was built with (router_mlp). ndata counts the
+// data lines.
 __global__ void bucket_lines_kernel(const PartMeta* meta, const uint64_t* tbs, uint64_t nparts, uint64_t nt,
-                                    const Knot* knots, const uint32_t* tab, BucketLine* out) {
+                                    const Knot* knots, const uint32_t* tab, const int64_t* ts, const float* grid,
+                                    int which, uint32_t B, int data_lines, BucketLine* out,
+                                    unsigned long long* ndata) {
+    uint32_t nd = 0;
     for (uint64_t i = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; i < nt;
          i += uint64_t(gridDim.x) * blockDim.x) {
         const uint32_t p = part_of_knot(tbs, nparts + 1, i);
         const PartMeta pm = meta[p];
-        if (i - pm.tb >= pm.R) continue;
+        const uint32_t b = static_cast<uint32_t>(i - pm.tb);
+        if (b >= pm.R) continue;
         const uint32_t lo = tab[i], hi = tab[i + 1];
+        if (data_lines) {
+            const float* W = grid + size_t(p) * (kGrid + 1);
+            const bool mlp = router_mlp(pm, which);
+            const uint64_t r0 = first_row_from_bucket(pm, W, mlp, ts, B, lo, b);
+            const uint64_t r1 = first_row_from_bucket(pm, W, mlp, ts, B, hi, b + 1);
+            const uint64_t cnt = r1 - r0;
+            const uint64_t span = cnt ? static_cast<uint64_t>(ts[r1 - 1]) - static_cast<uint64_t>(ts[r0]) : 0;
+            const bool w32 = cnt <= kDataRows && span <= kDataMaxSpan;
+            if (w32 || (cnt <= kDataRows16 && span <= kDataMaxSpan16)) {
+                DataLine D;
+                D.base = cnt ? ts[r0] : 0;
+                D.r0 = static_cast<uint32_t>(r0);
+                auto off = [&](uint32_t k) {
+                    return static_cast<uint32_t>(static_cast<uint64_t>(ts[r0 + k]) - static_cast<uint64_t>(D.base));
+                };
+                for (uint32_t k = 0; k < kDataRows; ++k) {
+                    if (w32) {
+                        D.d[k] = k < cnt ? off(k) : kDataPad;
+                    } else {  // rows 2k and 2k + 1 in the low and high halves
+                        const uint32_t lo = 2 * k < cnt ? off(2 * k) : 0xFFFFu;
+                        const uint32_t hi = 2 * k + 1 < cnt ? off(2 * k + 1) : 0xFFFFu;
+                        D.d[k] = lo | (hi << 16);
+                    }
+                }
+                D.flag = kDataLine | (w32 ? 0u : kData16) | static_cast<uint32_t>(cnt);
+                *reinterpret_cast<DataLine*>(out + i) = D;
+                ++nd;
+                continue;
+            }
+        }
         const Knot* kp = knots + pm.kb;
         BucketLine L;
         SlopePiece* sp = L.sp;
@@ -477,6 +559,8 @@ __global__ void bucket_lines_kernel(const PartMeta* meta, const uint64_t* tbs, u
         if (hi - lo > kBucketRecs - 1) sp[kBucketRecs - 1].piece |= kBucketMore;
         out[i] = L;
     }
+    for (int off = kWave / 2; off > 0; off >>= 1) nd += __shfl_down(nd, off);
+    if (lane_id() == 0 && nd) atomicAdd(ndata, static_cast<unsigned long long>(nd));
 }
 
 // Knots a probe searches, summed per partition and overall (probes assumed to
@@ -605,21 +689,70 @@ __global__ void asof_learned_kernel(PK pk, ProbeArgs a, LearnedArgs L) {
         //    with tau <= t.
         const bool mlp = L.use_mlp && pm.mlp;
         const uint32_t b = bucket_of(pm, L.grid + size_t(p) * (kGrid + 1), mlp, t);
-        const BucketLine* bl = L.bkt + (pm.tb + b);
-        typedef int64_t i64x2 __attribute__((ext_vector_type(2)));
-        const i64x2* tv = reinterpret_cast<const i64x2*>(bl->tau);
+        // The whole 128-byte line in eight 16-byte loads.
+        const uint4* lp = reinterpret_cast<const uint4*>(L.bkt + (pm.tb + b));
+        uint32_t w[32];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint4 v = lp[k];
+            w[4 * k] = v.x;
+            w[4 * k + 1] = v.y;
+            w[4 * k + 2] = v.z;
+            w[4 * k + 3] = v.w;
+        }
+        if (w[31] & kDataLine) {
+            // Data line (DataLine): count the bucket's rows with time <= t.
+            const int64_t base = static_cast<int64_t>((static_cast<uint64_t>(w[1]) << 32) | w[0]);
+            uint32_t c = 0, dsel = 0;
+            if (t >= base) {
+                const uint64_t dt64 = static_cast<uint64_t>(t) - static_cast<uint64_t>(base);
+                if (!(w[31] & kData16)) {
+                    const uint32_t dt = dt64 < kDataMaxSpan ? static_cast<uint32_t>(dt64)
+                                                            : static_cast<uint32_t>(kDataMaxSpan);
+#pragma unroll
+                    for (int k = 0; k < kDataRows; ++k) {
+                        const bool le = w[3 + k] <= dt;  // ascending; padding never <= dt
+                        c += le ? 1u : 0u;
+                        dsel = le ? w[3 + k] : dsel;
+                    }
+                } else {
+                    const uint32_t dt = dt64 < kDataMaxSpan16 ? static_cast<uint32_t>(dt64)
+                                                              : static_cast<uint32_t>(kDataMaxSpan16);
+#pragma unroll
+                    for (int k = 0; k < kDataRows; ++k) {
+                        const uint32_t lo = w[3 + k] & 0xFFFFu, hi = w[3 + k] >> 16;
+                        const bool le0 = lo <= dt, le1 = hi <= dt;
+                        c += (le0 ? 1u : 0u) + (le1 ? 1u : 0u);
+                        dsel = le1 ? hi : (le0 ? lo : dsel);
+                    }
+                }
+            }
+            // t >= the partition's first time, so r0 + c >= 1 (an empty bucket
+            // or a time before its first row matches the row before r0).
+            const uint64_t m = static_cast<uint64_t>(w[2]) + c - 1;
+            uint32_t res = a.sorted_out ? static_cast<uint32_t>(m) : kNoRow;
+            if (a.has_tol) {
+                const int64_t tm = c ? static_cast<int64_t>(static_cast<uint64_t>(base) + dsel) : a.ts[m];
+                if (static_cast<uint64_t>(t) - static_cast<uint64_t>(tm) > a.tol) {
+                    a.out[j] = kNoRow;
+                    continue;
+                }
+            }
+            if (!a.sorted_out) res = a.row[m];
+            a.out[j] = res;
+            continue;
+        }
         int64_t rt[kBucketRecs];
 #pragma unroll
-        for (int k = 0; k < kBucketRecs / 2; ++k) {
-            const i64x2 v = tv[k];
-            rt[2 * k] = v.x;
-            rt[2 * k + 1] = v.y;
-        }
+        for (int k = 0; k < kBucketRecs; ++k)
+            rt[k] = static_cast<int64_t>((static_cast<uint64_t>(w[2 * k + 1]) << 32) | w[2 * k]);
         uint32_t c0 = 0;
 #pragma unroll
         for (int k = 0; k < kBucketRecs; ++k) c0 += rt[k] <= t ? 1u : 0u;
         SlopePiece sp{0.0f, kBucketMore};
-        if (c0 > 0) sp = bl->sp[c0 - 1];
+#pragma unroll
+        for (int k = 0; k < kBucketRecs; ++k)
+            if (static_cast<uint32_t>(k) + 1 == c0) sp = SlopePiece{__uint_as_float(w[16 + 2 * k]), w[17 + 2 * k]};
         uint32_t piece;
         int64_t tau;
         float slope;
@@ -1219,7 +1352,8 @@ qe_gpu_status qe_gpu_asof_train(qe_gpu_ctx* ctx, qe_dev_asof* idx, uint32_t piec
         idx->params = Scratch(ctx, std::max<uint64_t>(P, 1) * kParams * 4);
         idx->grid = Scratch(ctx, std::max<uint64_t>(P, 1) * (kGrid + 1) * 4);
         Scratch losses(ctx, std::max<uint64_t>(P, 1) * 4), maxe(ctx, 8), sums(ctx, 24),
-            occ(ctx, std::max<uint64_t>(P, 1) * 16), chosen(ctx, 8);
+            occ(ctx, std::max<uint64_t>(P, 1) * 16), chosen(ctx, 8), ndl(ctx, 16);
+        memset_async(ctx, ndl.p, 0, 16);
         memset_async(ctx, losses.p, 0, std::max<uint64_t>(P, 1) * 4);
         memset_async(ctx, maxe.p, 0, 8);
         memset_async(ctx, sums.p, 0, 24);
@@ -1264,10 +1398,14 @@ qe_gpu_status qe_gpu_asof_train(qe_gpu_ctx* ctx, qe_dev_asof* idx, uint32_t piec
                                                              sums.as<double>() + 1);
             QE_HIP(hipGetLastError());
             const unsigned gt = grid_for(nt, 256, ctx->num_cus * 16);
-            bucket_lines_kernel<<<gt, 256, 0, ctx->stream>>>(meta, tb.as<uint64_t>(), P, nt, knots,
-                                                             idx->tab_mlp.as<uint32_t>(), idx->bkt_mlp.as<BucketLine>());
-            bucket_lines_kernel<<<gt, 256, 0, ctx->stream>>>(meta, tb.as<uint64_t>(), P, nt, knots,
-                                                             idx->tab_lin.as<uint32_t>(), idx->bkt_lin.as<BucketLine>());
+            // Data lines need the kDataLine and kData16 bits clear in every knot line's last word.
+            const int dl = ctx->tune_asof_data_lines && nk < (uint64_t(1) << 29) ? 1 : 0;
+            bucket_lines_kernel<<<gt, 256, 0, ctx->stream>>>(
+                meta, tb.as<uint64_t>(), P, nt, knots, idx->tab_mlp.as<uint32_t>(), idx->ts.as<int64_t>(), grid, 1,
+                piece_rows, dl, idx->bkt_mlp.as<BucketLine>(), ndl.as<unsigned long long>());
+            bucket_lines_kernel<<<gt, 256, 0, ctx->stream>>>(
+                meta, tb.as<uint64_t>(), P, nt, knots, idx->tab_lin.as<uint32_t>(), idx->ts.as<int64_t>(), grid, 0,
+                piece_rows, dl, idx->bkt_lin.as<BucketLine>(), ndl.as<unsigned long long>() + 1);
             QE_HIP(hipGetLastError());
             if (read_scalar(ctx, chosen.as<uint64_t>()) > 0) time_routers(ctx, idx, P, nk, nt);
         }
@@ -1279,7 +1417,10 @@ qe_gpu_status qe_gpu_asof_train(qe_gpu_ctx* ctx, qe_dev_asof* idx, uint32_t piec
             QE_HIP(hipMemcpyAsync(hm.data(), idx->meta.p, P * sizeof(PartMeta), hipMemcpyDeviceToHost, ctx->stream));
         }
         QE_HIP(hipMemcpyAsync(hs, sums.p, 24, hipMemcpyDeviceToHost, ctx->stream));
+        uint64_t hd[2] = {0, 0};
+        QE_HIP(hipMemcpyAsync(hd, ndl.p, 16, hipMemcpyDeviceToHost, ctx->stream));
         idx->max_err = read_scalar(ctx, maxe.as<uint64_t>());
+        idx->data_lines = idx->mlp_rejected ? hd[1] : hd[0];
         idx->mlp_parts = read_scalar(ctx, chosen.as<uint64_t>());
         double ls = 0;
         uint64_t cand = 0;
@@ -1459,6 +1600,7 @@ qe_gpu_status qe_gpu_asof_stats(qe_gpu_ctx* ctx, const qe_dev_asof* idx, qe_asof
         out->mean_bucket_knots_mlp = idx->mean_knots_cand;
         out->sample_probe_ms_mlp = idx->sample_ms_mlp;
         out->sample_probe_ms_linear = idx->sample_ms_lin;
+        out->data_line_buckets = idx->data_lines;
     });
 }
 

@@ -673,6 +673,10 @@ qe_gpu_status qe_gpu_set_tuning(qe_gpu_ctx* ctx, int32_t key, int64_t value) {
                 invalid(value >= 1 && value <= 8, "QE_TUNE_JOIN_BLOCKS_PER_CU must be in [1, 8]");
                 ctx->tune_join_blocks_per_cu = static_cast<int>(value);
                 break;
+            case QE_TUNE_ASOF_DATA_LINES:
+                invalid(value == 0 || value == 1, "QE_TUNE_ASOF_DATA_LINES must be 0 or 1");
+                ctx->tune_asof_data_lines = static_cast<int>(value);
+                break;
             default:
                 invalid(false, "unknown tuning key");
         }

@@ -126,6 +126,7 @@ struct qe_gpu_ctx {
     // Tuning knobs (qe_gpu_set_tuning).
     int tune_join_pairs = 4;  // row pairs per thread per tile in the streaming probe
     int tune_asof_bucket_knots = 1;  // knots per learned-router bucket (set at train time)
+    int tune_asof_data_lines = 1;    // learned index: bucket rows inline where they fit (set at train time)
     uint32_t tune_join_spill_cap = 0;  // spill-list records per block (0 = sized from the probe)
     int tune_join_blocks_per_cu = 1;   // streaming-probe blocks per CU (as many as LDS allows)
     // Small pinned host staging buffer for scalars read back after kernels.

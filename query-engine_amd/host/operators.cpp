@@ -532,7 +532,15 @@ void GpuAsofJoin::open() {
         QE_CALL(qe_gpu_asof_build(ctx, bk.data(), static_cast<uint32_t>(bk.size()), bcols_->cols.at(build_time_),
                                   &built));
         std::shared_ptr<void> owner(built, [](void* p) { qe_gpu_asof_free(device_context(), static_cast<qe_dev_asof*>(p)); });
-        if (method_ == tsx::AsofMethod::Learned) QE_CALL(qe_gpu_asof_train(ctx, built, 64, 300, 0.003f));
+        // One knot per router bucket (the context default), so most buckets
+        // hold their rows inline in one 128-byte line: pieces of 32 rows, or
+        // 64 above 2^30 rows, where the probe's random lines would otherwise
+        // span more than the GPU's address translation covers.
+        if (method_ == tsx::AsofMethod::Learned) {
+            qe_asof_stats st{};
+            QE_CALL(qe_gpu_asof_stats(ctx, built, &st));
+            QE_CALL(qe_gpu_asof_train(ctx, built, st.build_rows > (uint64_t(1) << 30) ? 64 : 32, 300, 0.003f));
+        }
         slot = std::move(owner);
     }
     qe_dev_asof* idx = static_cast<qe_dev_asof*>(slot.get());

@@ -135,3 +135,66 @@ def asof_bucket_edge_case(piece_rows):
     bsym, bts = np.concatenate(bsym)[order], np.concatenate(bts)[order]
     psym, pts = np.concatenate(psym), np.concatenate(pts)
     return [col(psym, I32), col(pts, TS)], [col(bsym, I32), col(bts, TS)]
+
+
+def asof_bucket_fill_case(piece_rows):
+    """Router buckets with chosen row counts: per symbol K pieces of B rows
+    (the segment starts at a multiple of B, so the linear router has R = K
+    buckets of equal width w over [first time, first time of the last piece])
+    and every bucket but the last filled with a set number of rows strictly
+    inside it: counts at a data line's capacities (27-30 and 55-57 rows),
+    empty and one-row buckets, fillers, each with its rows within 2^15 time
+    units (16-bit offsets fit), within 2^20 (32-bit offsets), or spread over
+    most of a 2^34-unit bucket (wider than 32-bit offsets); the last bucket
+    holds the last piece. Probes at every row, one unit either side, just
+    inside every bucket edge, before the first row and past the last, and for
+    an absent symbol. Returns (probe, build)."""
+    r = np.random.default_rng(2600 + piece_rows)
+    B = piece_rows
+    K = {4: 256, 16: 64, 64: 32}[B]
+    w = 1 << 34
+    edge = [27, 28, 29, 30, 55, 56, 57, 0, 1]
+    widths = {0: 1 << 15, 1: 1 << 20, 2: w - (w >> 5)}
+    bsym, bts, psym, pts = [], [], [], []
+    for s in range(3):
+        occ = [(c, wide) for c in edge for wide in (0, 1, 2)]
+        rest = (K - 1) - len(occ)
+        need = (K - 1) * B - sum(c for c, _ in occ)
+        fill = r.multinomial(need, np.ones(rest) / rest) if rest > 0 else []
+        occ += [(int(c), int(r.integers(0, 3))) for c in fill]
+        occ = [occ[i] for i in r.permutation(len(occ))]
+        t0 = int(r.integers(-(1 << 40), 1 << 40))
+        tl = t0 + K * w  # the last piece's first row: x = 1, the last bucket
+        ts = []
+        for b, (c, wide) in enumerate(occ):
+            lo = t0 + b * w + (w >> 6)
+            span = widths[wide]
+            ts.extend(sorted(int(v) for v in lo + r.integers(0, span, c)))
+        if not ts or ts[0] != t0:
+            # row 0 is the partition's first time (x = 0): put it at t0 in bucket 0
+            first = next(i for i, (c, _) in enumerate(occ) if c)
+            del ts[0]
+            ts.insert(0, t0)
+            if first:  # keep the counts: move one row of bucket `first` to bucket 0
+                occ[0] = (occ[0][0] + 1, occ[0][1])
+                occ[first] = (occ[first][0] - 1, occ[first][1])
+        ts.extend([tl] + sorted(int(v) for v in tl + 1 + r.integers(0, 1 << 20, B - 1)))
+        ts = np.asarray(ts, dtype=np.int64)
+        assert len(ts) == K * B
+        bsym.append(np.full(len(ts), s))
+        bts.append(ts)
+        edges = t0 + np.arange(K + 1, dtype=np.int64) * w
+        for d in (-1, 0, 1):
+            psym.append(np.full(len(ts), s))
+            pts.append(ts + d)
+        for d in (1, (w >> 7)):
+            psym.append(np.full(K + 1, s))
+            pts.append(edges + d)
+        psym.append(np.array([s, s]))
+        pts.append(np.array([t0 - 1, int(ts[-1]) + (1 << 30)]))
+    psym.append(np.array([7, 7]))
+    pts.append(np.array([0, 1 << 41]))
+    order = r.permutation(sum(len(t) for t in bts))
+    bsym, bts = np.concatenate(bsym)[order], np.concatenate(bts)[order]
+    psym, pts = np.concatenate(psym), np.concatenate(pts)
+    return [col(psym, I32), col(pts, TS)], [col(bsym, I32), col(bts, TS)]

@@ -30,14 +30,26 @@ asof_learned -- asof.hip: matchable build rows sorted by (partition, time,
   the records and the outward (galloping) search from the spline's
   prediction, with the knot-range search and its verification past the 7th
   record.
+  Buckets of at most 28 rows spanning less than 2^32 time units, or 56 rows
+  spanning less than 2^16, hold the rows inline (data lines: the first time,
+  the first sorted position r0 and each row's offset from the first time, in
+  32 or 16 bits); the match is r0 + c - 1 for c rows at or before the probe's
+  time.
   Mutants:
-    "le_strict"  -- `<` for `<=` when the records place the probe's time;
+    "le_strict"  -- `<` for `<=` when the records or a data line's offsets
+                    place the probe's time;
     "router_f32" -- probes normalise time in float32 arithmetic while the
                     router table was built with the double-precision form, so a
                     time near a bucket edge can route one bucket low and take
                     the last record there without checking its successor;
     "window_err" -- the search inside a piece is a window of the piece's
-                    recorded error around the prediction, not bracketed.
+                    recorded error around the prediction, not bracketed;
+    "dl_span"    -- data lines are built without the span check, so offsets
+                    past their width (32 or 16 bits) wrap;
+    "dl_cap"     -- a bucket of one row more than a data line holds (28 or
+                    56) is still written as one, its last row dropped;
+    "dl_below"   -- a probe before its data line's first time is not held at
+                    zero rows: its offset wraps and counts every row.
 """
 import numpy as np
 
@@ -155,7 +167,8 @@ def stream_grid_blocks(n, num_cus=256, blocks_per_cu=1):
 
 # ---- learned as-of probe (linear router) -----------------------------------------------
 BUCKET_RECS = 8
-ASOF_MUTANTS = ("le_strict", "router_f32", "window_err")
+ASOF_MUTANTS = ("le_strict", "router_f32", "window_err", "dl_span", "dl_cap", "dl_below")
+DATA_ROWS, DATA_ROWS16 = 28, 56
 I64_MIN, I64_MAX = -(1 << 63), (1 << 63) - 1
 
 
@@ -178,10 +191,13 @@ def piece_pred(tau, slope, cnt, t):
 class LearnedIndex:
     """The learned index of one set of partitions (sorted timestamps per
     partition, laid out back to back like the device's sorted array), linear
-    router, B rows per piece, one knot per router bucket."""
+    router, B rows per piece, one knot per router bucket, data lines where a
+    bucket's rows fit (data_lines)."""
 
-    def __init__(self, seg_ts, B):
+    def __init__(self, seg_ts, B, data_lines=True, mutant=None):
         self.B = B
+        self.data_lines = data_lines
+        self.build_mutant = mutant
         self.parts = []
         seg = 0
         for ts in seg_ts:
@@ -207,6 +223,7 @@ class LearnedIndex:
                 tab = self._table(meta)
                 meta["tab"] = tab
                 meta["lines"] = [self._line(meta, b) for b in range(K)]
+                meta["dlines"] = [self._dline(meta, b) if data_lines else None for b in range(K)]
             self.parts.append(meta)
             seg += n
 
@@ -248,14 +265,56 @@ class LearnedIndex:
         more = hi - lo > BUCKET_RECS - 1
         return recs, more
 
+    def _first_row(self, meta, k, b):
+        """First row (partition-local) whose bucket is >= b, given k = tab[b]:
+        a binary search between knot k - 1's row and knot k's."""
+        if k == 0:
+            return 0
+        lo = meta["starts"][k - 1] + 1
+        hi = meta["starts"][k] if k < meta["K"] else meta["n"]
+        ts = meta["ts"]
+        while lo < hi:
+            mid = lo + (hi - lo) // 2
+            if self._bucket(meta, int(ts[mid])) >= b:
+                hi = mid
+            else:
+                lo = mid + 1
+        return lo
+
+    def _dline(self, meta, b):
+        """(base, r0, offsets, max offset) when bucket b's rows fit a data line
+        (32-bit offsets first, else 16-bit ones), else None."""
+        r0 = self._first_row(meta, meta["tab"][b], b)
+        r1 = self._first_row(meta, meta["tab"][b + 1], b + 1)
+        cnt = r1 - r0
+        ts = meta["ts"]
+        extra = 1 if self.build_mutant == "dl_cap" else 0
+        base = int(ts[r0]) if cnt else 0
+        span = ((int(ts[r1 - 1]) - base) & 0xFFFFFFFFFFFFFFFF) if cnt else 0
+        for rows, top in ((DATA_ROWS, 0xFFFFFFFE), (DATA_ROWS16, 0xFFFE)):
+            if cnt > rows + extra or (span > top and self.build_mutant != "dl_span"):
+                continue
+            n = min(cnt, rows)
+            offs = [((int(ts[r0 + k]) - base) & 0xFFFFFFFFFFFFFFFF) & (top + 1) for k in range(n)]  # all-ones mask
+            return base, r0, offs, top
+        return None
+
     def probe(self, p, t, mutant=None):
         """Sorted position (within the partition) of the match, or -1."""
         meta = self.parts[p]
         if meta["K"] == 0 or t < meta["t0"]:
             return -1
         b = self._bucket(meta, t, f32=mutant == "router_f32")
-        recs, more = meta["lines"][b]
         le = (lambda a, x: a < x) if mutant == "le_strict" else (lambda a, x: a <= x)
+        dl = meta["dlines"][b]
+        if dl is not None:
+            base, r0, offs, top = dl
+            c = 0
+            if t >= base or mutant == "dl_below":
+                dt = min((t - base) & 0xFFFFFFFFFFFFFFFF, top)
+                c = sum(1 for o in offs if le(o, dt))
+            return r0 + c - 1
+        recs, more = meta["lines"][b]
         c0 = sum(1 for tau, _ in recs if le(tau, t))
         kn = meta["knots"]
         if c0 > 0 and not (more and c0 == BUCKET_RECS):
@@ -293,7 +352,7 @@ class LearnedIndex:
         return lo
 
 
-def asof_learned(bsym, bts, psym, pts, B, mutant=None, tol=None):
+def asof_learned(bsym, bts, psym, pts, B, mutant=None, tol=None, data_lines=True):
     """Per-probe matched build row (NO_ROW for none) through the learned index
     model: one partition per integer key value, NULL-free inputs."""
     bsym, bts = np.asarray(bsym, dtype=np.int64), np.asarray(bts, dtype=np.int64)
@@ -303,7 +362,7 @@ def asof_learned(bsym, bts, psym, pts, B, mutant=None, tol=None):
     sym_s, ts_s = bsym[order], bts[order]
     bounds = np.searchsorted(sym_s, np.arange(kmin, kmin + nparts + 1))
     segs = [ts_s[bounds[i]:bounds[i + 1]] for i in range(nparts)]
-    idx = LearnedIndex(segs, B)
+    idx = LearnedIndex(segs, B, data_lines, mutant if mutant in ("dl_span", "dl_cap") else None)
     out = np.full(len(pts), NO_ROW, dtype=np.uint32)
     for j, (s, t) in enumerate(zip(np.asarray(psym, dtype=np.int64), np.asarray(pts, dtype=np.int64))):
         p = int(s) - kmin

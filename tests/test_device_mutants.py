@@ -26,7 +26,7 @@ from _compare import diff
 
 JOIN_CASES = [("direct%d" % c, c) for c in range(8)] + [("wide%d" % c, c) for c in (0, 1, 2, 4)]
 ASOF_CASES = ([("random%d" % c, c) for c in range(10)] + [("dup_runs_B%d" % b, b) for b in (1, 2, 4, 16, 64)]
-              + [("bucket_edge_B%d" % b, b) for b in (4, 16, 64)])
+              + [("bucket_edge_B%d" % b, b) for b in (4, 16, 64)] + [("bucket_fill_B%d" % b, b) for b in (4, 16, 64)])
 
 
 def join_inputs(name):
@@ -64,7 +64,9 @@ def asof_inputs(name):
         p, b, tol, B = DC.asof_random_case(int(name[6:]))
         return p, b, tol, B
     B = int(name.rsplit("B", 1)[1])
-    p, b = DC.asof_dup_runs_case(B) if name.startswith("dup") else DC.asof_bucket_edge_case(B)
+    build = {"dup": DC.asof_dup_runs_case, "bucket_edge": DC.asof_bucket_edge_case,
+             "bucket_fill": DC.asof_bucket_fill_case}[name.rsplit("_B", 1)[0].replace("_runs", "")]
+    p, b = build(B)
     return p, b, None, B
 
 
@@ -81,7 +83,7 @@ def asof_case(name):
     return _ASOF[name]
 
 
-def model_asof(p, b, tol, B, mutant):
+def model_asof(p, b, tol, B, mutant, data_lines=True):
     """The learned-probe model over the valid rows (NULL keys or times never
     match, as in the kernels)."""
     bv = np.ones(len(b[0][0]), dtype=bool)
@@ -97,7 +99,8 @@ def model_asof(p, b, tol, B, mutant):
     if len(keep) == 0:
         return out
     no_sym = int(b[0][0][keep].min()) - 1  # NULL probes: a key with no partition
-    m = DM.asof_learned(b[0][0][keep], b[1][0][keep], np.where(pv, p[0][0], no_sym), p[1][0], B, mutant, tol)
+    m = DM.asof_learned(b[0][0][keep], b[1][0][keep], np.where(pv, p[0][0], no_sym), p[1][0], B, mutant, tol,
+                        data_lines)
     hit = m != DM.NO_ROW
     out[hit] = keep[m[hit]]
     return out
@@ -121,9 +124,10 @@ def test_join_mutant_is_flagged(mutant):
 @pytest.mark.parametrize("name", [n for n, _ in ASOF_CASES])
 def test_asof_model_equals_oracle(name):
     p, b, tol, B, want = asof_case(name)
-    got = model_asof(p, b, tol, B, None)
-    bad = np.nonzero(got != want)[0]
-    assert len(bad) == 0, (bad[:5], got[bad[:5]], want[bad[:5]])
+    for data_lines in (True, False):
+        got = model_asof(p, b, tol, B, None, data_lines)
+        bad = np.nonzero(got != want)[0]
+        assert len(bad) == 0, (data_lines, bad[:5], got[bad[:5]], want[bad[:5]])
 
 
 @pytest.mark.parametrize("mutant", DM.ASOF_MUTANTS)
@@ -134,8 +138,10 @@ def test_asof_mutant_is_flagged(mutant):
         if (model_asof(p, b, tol, B, mutant) != want).any():
             flagged.append(name)
     assert flagged, f"no GPU as-of case exposes {mutant}"
-    if mutant == "router_f32":  # only probes at knots placed on bucket edges expose it
-        assert all(n.startswith("bucket_edge") for n in flagged)
+    if mutant == "router_f32":  # only probes placed at bucket edges expose it
+        assert all(n.startswith(("bucket_edge", "bucket_fill")) for n in flagged), flagged
+    if mutant.startswith("dl_"):  # data-line defects need the buckets filled to and past its edges
+        assert any(n.startswith("bucket_fill") for n in flagged), flagged
 
 
 def test_models_follow_kernel_layout():

@@ -58,11 +58,22 @@ def gpu_matches(ctx, qe, pcols, bcols, pk, bk, pt, bt, tol=None, modes=(0,), tra
 
 
 def check_all(ctx, qe, pcols, bcols, pk, bk, pt, bt, tol=None, piece_rows=8):
+    # Every method, with bucket rows inline where they fit (the default) and
+    # with knot records in every bucket line.
     want = oracle_matches(pcols, bcols, pk, bk, pt, bt, tol)
-    res, st = gpu_matches(ctx, qe, pcols, bcols, pk, bk, pt, bt, tol, modes=(0, 1, 2), train=(piece_rows, 50, 0.01))
-    for m, got in zip((0, 1, 2), res):
-        assert got.tolist() == want.tolist(), f"mode {m}"
-    return want, st
+    stats = []
+    for data_lines in (1, 0):
+        ctx.set_tuning(qe.TUNE_ASOF_DATA_LINES, data_lines)
+        try:
+            res, st = gpu_matches(ctx, qe, pcols, bcols, pk, bk, pt, bt, tol, modes=(0, 1, 2),
+                                  train=(piece_rows, 50, 0.01))
+        finally:
+            ctx.set_tuning(qe.TUNE_ASOF_DATA_LINES, 1)
+        for m, got in zip((0, 1, 2), res):
+            assert got.tolist() == want.tolist(), f"mode {m}, data lines {data_lines}"
+        stats.append(st)
+    assert stats[1]["data_line_buckets"] == 0
+    return want, stats[0]
 
 
 def test_literal_gap_tie_nearest(ctx, qe):
@@ -130,6 +141,47 @@ def test_knots_at_router_bucket_edges(ctx, qe, piece_rows):
     # binary search and the oracle on all of them.
     pcols, bcols = DC.asof_bucket_edge_case(piece_rows)
     check_all(ctx, qe, pcols, bcols, [0], [0], 1, 1, piece_rows=piece_rows)
+
+
+@pytest.mark.parametrize("piece_rows", [4, 16, 64])
+def test_router_buckets_filled_to_the_data_line_edge(ctx, qe, piece_rows):
+    # Buckets of exactly 27-30 rows (a data line holds 28), empty and one-row
+    # buckets, and buckets whose rows span more than 32-bit offsets: each
+    # bucket is a data line or a knot line, and probes at every row, either
+    # side of it, just inside every bucket edge (before the bucket's first row)
+    # and past the ends must match the oracle, with and without a tolerance.
+    pcols, bcols = DC.asof_bucket_fill_case(piece_rows)
+    for tol in (None, 1 << 19, 1 << 33):
+        _, st = check_all(ctx, qe, pcols, bcols, [0], [0], 1, 1, tol=tol, piece_rows=piece_rows)
+        assert 0 < st["data_line_buckets"] < st["router_buckets"], st
+
+
+def test_data_lines_every_output_mode(ctx, qe):
+    # The data-line probe's three answers: the sorted position, the original
+    # row, and the tolerance check on the matched time (taken from the line,
+    # or from the sorted times for a match before the bucket's first row).
+    (rs, rt, rp), (ls, lt) = ctx.gen_asof(41, 1 << 21, 1 << 19, 300, 1.1)
+    try:
+        idx = ctx.asof_build([rs], rt)
+        try:
+            idx.train(16, 100, 0.003)
+            st = idx.stats()
+            rows = idx.sorted(st["build_rows"], st["partitions"])[1]
+            for tol in (None, 0, 150, 700, 1 << 40):
+                want = idx.probe([ls], lt, tolerance=tol, mode=qe.ASOF_BINARY)
+                for mode in (qe.ASOF_LEARNED, qe.ASOF_LEARNED_NO_MLP):
+                    assert np.array_equal(idx.probe([ls], lt, tolerance=tol, mode=mode), want), (mode, tol)
+                    pos = idx.probe([ls], lt, tolerance=tol, mode=mode | qe.ASOF_OUT_SORTED)
+                    hit = pos != NO
+                    assert np.array_equal(hit, want != NO), (mode, tol)
+                    assert np.array_equal(rows[pos[hit]], want[hit]), (mode, tol)
+        finally:
+            idx.free()
+    finally:
+        for c in (rs, rt, rp, ls, lt):
+            c.free()
+    # Most buckets of this shape hold their rows inline.
+    assert st["data_line_buckets"] > 0.5 * st["router_buckets"], st
 
 
 @pytest.mark.parametrize("n", [1, 63, 4095, 4096, 4097, 100_003])
