@@ -710,6 +710,14 @@ def tick_workload(qe, ctx, args):
         if timed_stages:
             t = stamp("filter", t)
         m = f.rows
+        if args.tick_cluster:
+            # Probes grouped by the index region they read (the as-of join's
+            # probe-side ordering); the later stages are order-insensitive.
+            c = idx.cluster([f.col(0)], f.col(1), [f.col(0), f.col(1), f.col(2)])
+            f.free()
+            f = c
+            if timed_stages:
+                t = stamp("asof_cluster", t)
         # Only which trades matched is used downstream (the gather takes
         # trade columns), so the probe reports sorted positions.
         idx.probe_into([f.col(0)], f.col(1), match, mode=mode | qe.ASOF_OUT_SORTED)
@@ -762,6 +770,7 @@ def tick_workload(qe, ctx, args):
     res = {"quotes": nq, "trades": nt, "symbols": nsym, "passing_filter": m, "asof_matched": matched,
            "groups": ng, "ms_per_step": wall * 1e3, "trade_rows_per_s": nt / wall,
            "index_build_s": t1 - t0, "index_train_s": t2 - t1, "piece_rows": args.tick_piece_rows,
+           "probe_clustering": bool(args.tick_cluster),
            "router_buckets": int(st["router_buckets"]), "data_line_buckets": int(st["data_line_buckets"]),
            "stage_ms": {k: v * 1e3 for k, v in stages.items()},
            "counts_sorted_desc": bool(np.all(np.diff(counts.astype(np.int64)) <= 0)),
@@ -854,6 +863,8 @@ def main():
                     help="rows per spline piece (with one knot per router bucket, the rows a bucket holds on average)")
     ap.add_argument("--tick-piece-rows", type=int, default=64,
                     help="rows per spline piece of the tick pipeline's 2^31-row quote index")
+    ap.add_argument("--tick-cluster", type=int, default=1, choices=[0, 1],
+                    help="tick pipeline: group the filtered trades by the index region their probes read")
     ap.add_argument("--train-steps", type=int, default=300)
     ap.add_argument("--train-lr", type=float, default=0.003)
     ap.add_argument("--bucket-knots", type=int, default=1, help="spline knots per learned-router bucket")

@@ -401,6 +401,24 @@ qe_gpu_status qe_gpu_asof_stats(qe_gpu_ctx* ctx, const qe_dev_asof* idx, qe_asof
  * the rows a QE_ASOF_OUT_SORTED match addresses. */
 qe_gpu_status qe_gpu_asof_take_sorted(qe_gpu_ctx* ctx, const qe_dev_asof* idx, const qe_dev_col* const* cols,
                                       uint32_t ncols, qe_dev_table** out);
+/* Probe-side ordering before the probe. It replaces the Sort that
+ * tsx::AsofJoin wraps around its probe child (tsx/asof.cpp:161-164), which
+ * orders the probes by (keys..., time) so that neighbouring probes read
+ * neighbouring build rows.
+ * Here one partitioning pass reorders the rows of `cols` (each of
+ * probe_time's length; validity and dictionaries kept) into 256 groups by the
+ * index region each row's probe will read:
+ *   - the learned router's bucket line, for a trained index;
+ *   - else the partition's first sorted row.
+ * Within a group, rows keep their input order, so the result is
+ * deterministic. Rows that probe nothing (NULL key or time, no partition)
+ * join the first group.
+ * Probing the returned key and time columns gives the same match for each
+ * row as probing the input. Each group's random reads stay within 1/256 of
+ * the index. */
+qe_gpu_status qe_gpu_asof_cluster(qe_gpu_ctx* ctx, const qe_dev_asof* idx, const qe_dev_col* const* probe_keys,
+                                  uint32_t nkeys, const qe_dev_col* probe_time, const qe_dev_col* const* cols,
+                                  uint32_t ncols, qe_dev_table** out);
 /* Sorted build side for inspection: sorted timestamps, original row of each
  * sorted position, and CSR partition offsets (partitions+1). Any may be NULL. */
 qe_gpu_status qe_gpu_asof_sorted(qe_gpu_ctx* ctx, const qe_dev_asof* idx, int64_t* ts, uint32_t* rows, uint64_t* offsets);

@@ -143,6 +143,7 @@ _SIGS = {
     "qe_gpu_asof_probe": (C.c_int, [P, P, C.POINTER(P), C.c_uint32, P, C.c_int32, C.c_int64, C.c_int32, P]),
     "qe_gpu_asof_pairs": (C.c_int, [P, P, C.c_uint64, C.c_int32, C.POINTER(P)]),
     "qe_gpu_asof_take_sorted": (C.c_int, [P, P, P, C.c_uint32, C.POINTER(P)]),
+    "qe_gpu_asof_cluster": (C.c_int, [P, P, P, C.c_uint32, P, P, C.c_uint32, C.POINTER(P)]),
     "qe_gpu_asof_stats": (C.c_int, [P, P, C.POINTER(AsofStats)]),
     "qe_gpu_asof_sorted": (C.c_int, [P, P, P, P, P]),
     "qe_gpu_asof_free": (None, [P, P]),
@@ -769,6 +770,14 @@ class AsofIndex:
             return out.numpy(np.uint32, n)
         finally:
             out.free()
+
+    def cluster(self, probe_keys, probe_time, cols) -> "Table":
+        """`cols` (probe-side columns) reordered into groups by the index
+        region their probes read (qe_gpu_asof_cluster)."""
+        out = P()
+        self.ctx._check(lib().qe_gpu_asof_cluster(self.ctx.h, self.h, _cols(probe_keys), len(probe_keys),
+                                                  probe_time.h, _cols(cols), len(cols), C.byref(out)))
+        return Table(self.ctx, out)
 
     def take_sorted(self, cols) -> "Table":
         """Build-side columns gathered into the index's sorted order."""

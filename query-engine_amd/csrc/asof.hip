@@ -27,6 +27,7 @@
 // (reduction over the batch) and the back-propagated activation gradient.
 #include "agg_common.hpp"
 #include "hash_common.hpp"
+#include "tile_rank.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -796,6 +797,149 @@ __global__ void asof_learned_kernel(PK pk, ProbeArgs a, LearnedArgs L) {
     if (lane_id() == 0 && fb) atomicAdd(L.fallbacks, static_cast<unsigned long long>(fb));
 }
 
+// ---- probe-side clustering --------------------------------------------------------------
+// The reference's AsofJoin sorts its probe child by (key, time) before merging
+// (tsx/asof.cpp:163-166). Here the probe table is instead partitioned in one
+// pass into kClGroups groups by the index region its probes will read: the
+// learned router's line address (the table entry of the probe's bucket), or,
+// untrained, the partition's first sorted position. Lines are laid out by
+// partition and then bucket, so a group's probes read one contiguous 1/256 of
+// the index, which keeps their random line reads within what the GPU's address
+// translation covers. Rows that probe nothing (NULL key or time, no
+// partition) go to group 0. Within a group rows keep their input order.
+constexpr unsigned kClItems = 16;
+constexpr unsigned kClTile = kRankBlock * kClItems;  // 4096 rows
+constexpr unsigned kClGroups = kRankDigits;
+constexpr int kClMaxCols = 8;
+
+struct ClusterArgs {
+    ColView time;
+    uint64_t n;
+    const PartMeta* meta;  // nullptr: untrained index
+    const float* grid;
+    const uint64_t* offs;  // partition offsets into the sorted rows
+    uint64_t nsorted;      // sorted rows
+    uint64_t ntab;         // router-table entries (line addresses < ntab)
+    uint8_t* grp;
+    uint32_t* hist;        // hist[g * ntiles + tile]
+    uint64_t ntiles;
+};
+
+template <class PK>
+__global__ void __launch_bounds__(kRankBlock) cluster_group_kernel(PK pk, ClusterArgs a) {
+    __shared__ uint32_t h[kClGroups];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t base = uint64_t(blockIdx.x) * kClTile;
+    // Every row's partition and time first, so their loads are in flight together.
+    uint32_t part[kClItems];
+    int64_t tv[kClItems];
+#pragma unroll
+    for (unsigned i = 0; i < kClItems; ++i) {
+        const uint64_t j = base + uint64_t(i) * kRankBlock + threadIdx.x;
+        const bool in = j < a.n && is_valid(a.time, j);
+        part[i] = in ? pk.part(j) : kNoRow;
+        tv[i] = in ? time_value(a.time, j) : 0;
+    }
+#pragma unroll
+    for (unsigned i = 0; i < kClItems; ++i) {
+        const uint64_t j = base + uint64_t(i) * kRankBlock + threadIdx.x;
+        if (j >= a.n) continue;
+        uint64_t g = 0;
+        const uint32_t p = part[i];
+        if (p != kNoRow) {
+            if (a.meta) {
+                const PartMeta pm = a.meta[p];
+                uint64_t line = pm.tb;
+                if (pm.K != 0 && tv[i] >= pm.t0)
+                    line += bucket_of(pm, a.grid + size_t(p) * (kGrid + 1), pm.mlp != 0, tv[i]);
+                g = line * kClGroups / a.ntab;
+            } else {
+                g = a.nsorted ? a.offs[p] * kClGroups / a.nsorted : 0;
+            }
+            g = g < kClGroups ? g : kClGroups - 1;
+        }
+        a.grp[j] = static_cast<uint8_t>(g);
+        atomicAdd(&h[g], 1u);
+    }
+    __syncthreads();
+    a.hist[uint64_t(threadIdx.x) * a.ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+struct ClusterCol {
+    const void* src;
+    const uint64_t* svalid;  // nullptr: all valid
+    void* dst;
+    uint64_t* dvalid;        // preset to all ones; NULL rows cleared here
+    int32_t width;           // 1, 4 or 8 bytes
+};
+struct ClusterCols {
+    int32_t n;
+    ClusterCol c[kClMaxCols];
+};
+
+__device__ inline uint64_t load_width(const void* p, int width, uint64_t r) {
+    if (width == 8) return static_cast<const uint64_t*>(p)[r];
+    if (width == 4) return static_cast<const uint32_t*>(p)[r];
+    return static_cast<const uint8_t*>(p)[r];
+}
+__device__ inline void store_width(void* p, int width, uint64_t r, uint64_t v) {
+    if (width == 8) static_cast<uint64_t*>(p)[r] = v;
+    else if (width == 4) static_cast<uint32_t*>(p)[r] = static_cast<uint32_t>(v);
+    else static_cast<uint8_t*>(p)[r] = static_cast<uint8_t>(v);
+}
+
+// Each tile ranks its rows stably by group, stages every column in LDS in
+// group order and writes each group's run contiguously at its offset.
+__global__ void __launch_bounds__(kRankBlock) cluster_scatter_kernel(const uint8_t* grp, uint64_t n,
+                                                                     const uint64_t* offs, uint64_t ntiles,
+                                                                     ClusterCols C) {
+    __shared__ TileRankLds L;
+    __shared__ uint64_t sval[kClTile];
+    __shared__ uint16_t srank[kClTile];
+    __shared__ uint8_t sgrp[kClTile];
+    __shared__ uint32_t snull[kClTile / 32];  // NULL flag of each staged row (bitmask)
+    __shared__ uint64_t goff[kClGroups];      // this tile's output offset of each group
+    const uint64_t tile = die_tile(ntiles);
+    if (tile >= ntiles) return;
+    const uint64_t base = tile * kClTile;
+    goff[threadIdx.x] = offs[uint64_t(threadIdx.x) * ntiles + tile];  // read before the ranking's barriers
+    uint32_t d[kClItems];
+    uint32_t at[kClItems];
+    const uint32_t live_n = static_cast<uint32_t>(n - base < kClTile ? n - base : kClTile);
+    auto live = [&](unsigned i) { return tile_row<kClItems>(i) < live_n; };
+    for (unsigned i = 0; i < kClItems; ++i) d[i] = live(i) ? grp[base + tile_row<kClItems>(i)] : 0u;
+    auto digit = [&](unsigned i) { return d[i]; };
+    tile_digit_ranks<kClItems>(L, digit, live, at);
+    for (unsigned i = 0; i < kClItems; ++i) {
+        if (!live(i)) continue;
+        sgrp[at[i]] = static_cast<uint8_t>(d[i]);
+        srank[at[i]] = static_cast<uint16_t>(at[i] - L.start[d[i]]);
+    }
+    for (int c = 0; c < C.n; ++c) {
+        const ClusterCol& col = C.c[c];
+        __syncthreads();  // the previous column's staging has been written out
+        if (col.dvalid) {
+            if (threadIdx.x < kClTile / 32) snull[threadIdx.x] = 0;
+            __syncthreads();
+        }
+        for (unsigned i = 0; i < kClItems; ++i) {
+            if (!live(i)) continue;
+            const uint64_t j = base + tile_row<kClItems>(i);
+            sval[at[i]] = load_width(col.src, col.width, j);
+            if (col.dvalid && col.svalid && !((col.svalid[j >> 6] >> (j & 63)) & 1ull))
+                atomicOr(&snull[at[i] >> 5], 1u << (at[i] & 31));
+        }
+        __syncthreads();
+        for (uint32_t s = threadIdx.x; s < live_n; s += kRankBlock) {
+            const uint64_t dst = goff[sgrp[s]] + srank[s];
+            store_width(col.dst, col.width, dst, sval[s]);
+            if (col.dvalid && ((snull[s >> 5] >> (s & 31)) & 1u))
+                atomicAnd(reinterpret_cast<unsigned long long*>(&col.dvalid[dst >> 6]), ~(1ull << (dst & 63)));
+        }
+    }
+}
+
 // Probes for timing the routers: n_s rows of the index drawn by a hashed
 // stride (unsorted, like real probes), each probing its own time, so every
 // probe matches and the sample follows the rows' distribution over partitions
@@ -1513,10 +1657,30 @@ static qe_gpu_status asof_probe_codes(qe_gpu_ctx* ctx, const qe_dev_asof* idx, c
 
 }  // namespace qe_gpu
 
+namespace qe_gpu {
+
+// Probe key columns in the index's key space: STR probe keys map onto the
+// build dictionary's values (strings.hip), a value the build side lacks
+// getting an id no partition has.
+static std::vector<const qe_dev_col*> probe_key_cols(qe_gpu_ctx* ctx, const qe_dev_asof* idx,
+                                                     const qe_dev_col* const* probe_keys, uint32_t nkeys,
+                                                     OwnedCols& own) {
+    std::vector<const qe_dev_col*> pk(probe_keys, probe_keys + nkeys);
+    for (uint32_t j = 0; j < nkeys; ++j) {
+        if (pk[j] == nullptr || pk[j]->type != QE_STR || idx->key_types[j] != QE_STR) continue;
+        const qe_gpu_dict* pd = need_dict(pk[j]);
+        const qe_gpu_dict* bd = idx->key_dicts[j];
+        invalid(bd != nullptr, "as-of index has no dictionary for a STR key");
+        if (pd == bd && bd->unique) continue;
+        pk[j] = own.keep(remap_col(ctx, pk[j], pd == bd ? bd->canon : probe_ids(bd, pd), QE_STR, nullptr));
+    }
+    return pk;
+}
+
+}  // namespace qe_gpu
+
 extern "C" {
 
-// STR probe keys map onto the build dictionary's values (strings.hip): a value
-// the build side lacks gets an id no partition has.
 qe_gpu_status qe_gpu_asof_probe(qe_gpu_ctx* ctx, const qe_dev_asof* idx, const qe_dev_col* const* probe_keys,
                                 uint32_t nkeys, const qe_dev_col* probe_time, int32_t has_tolerance,
                                 int64_t tolerance, int32_t mode, uint32_t* match_out) {
@@ -1526,15 +1690,7 @@ qe_gpu_status qe_gpu_asof_probe(qe_gpu_ctx* ctx, const qe_dev_asof* idx, const q
         invalid(probe_time != nullptr, "null argument");
         check_rows32(probe_time->len);
         OwnedCols own(ctx);
-        std::vector<const qe_dev_col*> pk(probe_keys, probe_keys + nkeys);
-        for (uint32_t j = 0; j < nkeys; ++j) {
-            if (pk[j] == nullptr || pk[j]->type != QE_STR || idx->key_types[j] != QE_STR) continue;
-            const qe_gpu_dict* pd = need_dict(pk[j]);
-            const qe_gpu_dict* bd = idx->key_dicts[j];
-            invalid(bd != nullptr, "as-of index has no dictionary for a STR key");
-            if (pd == bd && bd->unique) continue;
-            pk[j] = own.keep(remap_col(ctx, pk[j], pd == bd ? bd->canon : probe_ids(bd, pd), QE_STR, nullptr));
-        }
+        const std::vector<const qe_dev_col*> pk = probe_key_cols(ctx, idx, probe_keys, nkeys, own);
         const qe_gpu_status st = asof_probe_codes(ctx, idx, pk.data(), nkeys, probe_time, has_tolerance, tolerance,
                                                   mode, match_out);
         if (st != QE_GPU_OK) throw Error{st, ctx->last_error};
@@ -1558,6 +1714,83 @@ qe_gpu_status qe_gpu_asof_pairs(qe_gpu_ctx* ctx, const uint32_t* match, uint64_t
             throw;
         }
         *out = pr;
+    });
+}
+
+qe_gpu_status qe_gpu_asof_cluster(qe_gpu_ctx* ctx, const qe_dev_asof* idx, const qe_dev_col* const* probe_keys,
+                                  uint32_t nkeys, const qe_dev_col* probe_time, const qe_dev_col* const* cols,
+                                  uint32_t ncols, qe_dev_table** out) {
+    QE_ENTRY(ctx, {
+        invalid(idx != nullptr && probe_time != nullptr && out != nullptr && cols != nullptr && ncols >= 1 &&
+                    (probe_keys != nullptr || nkeys == 0),
+                "null argument");
+        invalid(static_cast<int>(nkeys) == idx->nkeys, "as-of needs equal-length key lists");
+        invalid(time_type_ok(probe_time->type), "AsofJoin: time columns must be I32/I64/TS");
+        const uint64_t n = probe_time->len;
+        for (uint32_t j = 0; j < nkeys; ++j) {
+            invalid(probe_keys[j] != nullptr && probe_keys[j]->len == n, "bad probe key column");
+            invalid(probe_keys[j]->type == idx->key_types[j], "AsofJoin: key column types must match positionally");
+        }
+        for (uint32_t c = 0; c < ncols; ++c)
+            invalid(cols[c] != nullptr && cols[c]->len == n, "clustered columns need the probe time's length");
+        check_rows32(n);
+        OwnedCols own(ctx);
+        const std::vector<const qe_dev_col*> pk = probe_key_cols(ctx, idx, probe_keys, nkeys, own);
+        qe_dev_table* t = new_table(n);
+        try {
+            for (uint32_t c = 0; c < ncols; ++c) {
+                const bool nullable = !cols[c]->all_valid && cols[c]->validity != nullptr && n > 0;
+                t->cols.push_back(new_col(ctx, cols[c]->type, n, nullable));
+                t->cols.back()->dict = cols[c]->dict;
+                // All ones, the all-valid padding convention past n included; the
+                // scatter clears the NULL rows' bits.
+                if (nullable) memset_async(ctx, t->cols.back()->validity, 0xFF, words_for(n) * 8);
+            }
+            if (n) {
+                const uint64_t ntiles = (n + kClTile - 1) / kClTile;
+                Scratch grp(ctx, n), hist(ctx, ntiles * kClGroups * 4), offs(ctx, ntiles * kClGroups * 8);
+                ClusterArgs a{};
+                a.time = view(probe_time);
+                a.n = n;
+                a.meta = idx->trained && idx->nknots ? idx->meta.as<PartMeta>() : nullptr;
+                a.grid = idx->grid.as<float>();
+                a.offs = idx->offs.as<uint64_t>();
+                a.nsorted = idx->n;
+                a.ntab = std::max<uint64_t>(idx->ntab, 1);
+                a.grp = grp.as<uint8_t>();
+                a.hist = hist.as<uint32_t>();
+                a.ntiles = ntiles;
+                if (idx->mode == 0) {
+                    cluster_group_kernel<<<ntiles, kRankBlock, 0, ctx->stream>>>(ZeroKeyPart{}, a);
+                } else if (idx->mode == 1) {
+                    cluster_group_kernel<<<ntiles, kRankBlock, 0, ctx->stream>>>(
+                        DirectKeyPart{view(pk[0]), idx->kmin, idx->range}, a);
+                } else {
+                    cluster_group_kernel<<<ntiles, kRankBlock, 0, ctx->stream>>>(part_keys(idx, pk.data()), a);
+                }
+                QE_HIP(hipGetLastError());
+                exclusive_scan_u32(ctx, hist.as<uint32_t>(), offs.as<uint64_t>(), ntiles * kClGroups);
+                for (uint32_t c0 = 0; c0 < ncols; c0 += kClMaxCols) {
+                    ClusterCols C{};
+                    C.n = static_cast<int32_t>(std::min<uint32_t>(kClMaxCols, ncols - c0));
+                    for (int i = 0; i < C.n; ++i) {
+                        const qe_dev_col* src = cols[c0 + i];
+                        qe_dev_col* dst = t->cols[c0 + i];
+                        C.c[i] = ClusterCol{src->data, dst->validity ? src->validity : nullptr, dst->data,
+                                            dst->validity, static_cast<int32_t>(type_width(src->type))};
+                    }
+                    TimedRegion tr(ctx, "asof_cluster_scatter");
+                    cluster_scatter_kernel<<<tiles_per_die(ntiles) * kTileDies, kRankBlock, 0, ctx->stream>>>(
+                        grp.as<uint8_t>(), n, offs.as<uint64_t>(), ntiles, C);
+                    QE_HIP(hipGetLastError());
+                }
+            }
+        } catch (...) {
+            qe_gpu_table_free(ctx, t);
+            throw;
+        }
+        QE_HIP(hipStreamSynchronize(ctx->stream));
+        *out = t;
     });
 }
 

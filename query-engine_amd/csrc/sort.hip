@@ -13,6 +13,7 @@
 // Stability across tiles comes from the digit-major scan; within a tile from
 // the (iteration, wave, lane) position order used everywhere below.
 #include "internal.hpp"
+#include "tile_rank.hpp"
 
 #include <algorithm>
 
@@ -20,8 +21,7 @@ using namespace qe_gpu;
 
 namespace qe_gpu {
 
-constexpr unsigned kSortBlock = 256;
-constexpr unsigned kSortWaves = kSortBlock / kWave;
+constexpr unsigned kSortBlock = kRankBlock;
 constexpr unsigned kSortItems = 16;
 constexpr unsigned kSortTile = kSortBlock * kSortItems;  // 4096 keys
 constexpr unsigned kRadix = 256;
@@ -47,90 +47,40 @@ __global__ void __launch_bounds__(kSortBlock) radix_scatter_kernel(const uint64_
                                                                    uint64_t n, unsigned shift,
                                                                    const uint64_t* offs, uint64_t ntiles,
                                                                    uint64_t* keys_out, uint32_t* vals_out) {
-    __shared__ uint32_t wcnt[kSortWaves][kRadix];  // per-wave digit counts of one item row
-    __shared__ uint32_t run[kRadix];               // running digit counts within the tile
-    __shared__ uint32_t tile_start[kRadix];        // digit start within the tile (staging order)
+    __shared__ TileRankLds L;
     __shared__ uint64_t skeys[kSortTile];
     __shared__ uint32_t svals[kSortTile];
-    __shared__ uint16_t srank[kSortTile];          // rank of each key within its digit in the tile
+    __shared__ uint16_t srank[kSortTile];  // rank of each key within its digit in the tile
+    __shared__ uint64_t doff[kRadix];      // this tile's output offset of each digit
 
-    const uint64_t base = uint64_t(blockIdx.x) * kSortTile;
-    const unsigned lane = lane_id();
-    const unsigned w = threadIdx.x / kWave;
-    run[threadIdx.x] = 0;
+    const uint64_t tile = die_tile(ntiles);
+    if (tile >= ntiles) return;
+    const uint64_t base = tile * kSortTile;
+    doff[threadIdx.x] = offs[uint64_t(threadIdx.x) * ntiles + tile];  // read before the ranking's barriers
     uint64_t k[kSortItems];
     uint32_t v[kSortItems];
-    uint16_t rank[kSortItems];
+    uint32_t pos[kSortItems];
+    auto live = [&](unsigned i) { return base + tile_row<kSortItems>(i) < n; };
     for (unsigned i = 0; i < kSortItems; ++i) {
-        const uint64_t p = base + tile_pos(i);
-        const bool live = p < n;
-        k[i] = live ? keys[p] : ~0ull;
-        v[i] = live ? vals[p] : 0u;
+        const uint64_t p = base + tile_row<kSortItems>(i);
+        k[i] = live(i) ? keys[p] : ~0ull;
+        v[i] = live(i) ? vals[p] : 0u;
     }
-    __syncthreads();
+    auto digit = [&](unsigned i) { return static_cast<uint32_t>((k[i] >> shift) & 0xFF); };
+    tile_digit_ranks<kSortItems>(L, digit, live, pos);
     for (unsigned i = 0; i < kSortItems; ++i) {
-        const uint64_t p = base + tile_pos(i);
-        const bool live = p < n;
-        const unsigned d = static_cast<unsigned>((k[i] >> shift) & 0xFF);
-        // Lanes of this wave holding the same digit (dead lanes match nobody).
-        uint64_t peers = __ballot(live);
-        for (unsigned b = 0; b < 8; ++b) {
-            const uint64_t m = __ballot(live && ((d >> b) & 1u));
-            peers &= ((d >> b) & 1u) ? m : ~m;
-        }
-        const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-        const unsigned in_wave = __popcll(peers & lt);
-        for (unsigned j = lane; j < kRadix; j += kWave) wcnt[w][j] = 0;
-        __syncthreads();
-        if (live && in_wave == 0) wcnt[w][d] = __popcll(peers);
-        __syncthreads();
-        unsigned before = 0;
-        if (live) {
-            before = run[d];
-            for (unsigned ww = 0; ww < w; ++ww) before += wcnt[ww][d];
-        }
-        rank[i] = static_cast<uint16_t>(before + in_wave);
-        __syncthreads();
-        {  // advance the running counts by this item row's totals
-            const unsigned dd = threadIdx.x;
-            unsigned tot = 0;
-            for (unsigned ww = 0; ww < kSortWaves; ++ww) tot += wcnt[ww][dd];
-            run[dd] += tot;
-        }
-        __syncthreads();
-    }
-    // run[] now holds the tile's digit counts: exclusive scan -> staging starts.
-    {
-        const unsigned d = threadIdx.x;
-        const unsigned c = run[d];
-        unsigned incl = c;
-        for (int off = 1; off < kWave; off <<= 1) {
-            const unsigned t = __shfl_up(incl, off);
-            if (lane >= off) incl += t;
-        }
-        __shared__ unsigned wsum[kSortWaves];
-        if (lane == kWave - 1) wsum[w] = incl;
-        __syncthreads();
-        unsigned wb = 0;
-        for (unsigned ww = 0; ww < w; ++ww) wb += wsum[ww];
-        tile_start[d] = wb + incl - c;
-    }
-    __syncthreads();
-    for (unsigned i = 0; i < kSortItems; ++i) {
-        const uint64_t p = base + tile_pos(i);
-        if (p >= n) continue;
-        const unsigned d = static_cast<unsigned>((k[i] >> shift) & 0xFF);
-        const unsigned s = tile_start[d] + rank[i];
+        if (!live(i)) continue;
+        const unsigned s = pos[i];
         skeys[s] = k[i];
         svals[s] = v[i];
-        srank[s] = rank[i];
+        srank[s] = static_cast<uint16_t>(s - L.start[digit(i)]);
     }
     __syncthreads();
     const uint64_t live_n = (n - base < kSortTile) ? (n - base) : kSortTile;
     for (unsigned s = threadIdx.x; s < live_n; s += kSortBlock) {
         const uint64_t key = skeys[s];
-        const unsigned d = static_cast<unsigned>((key >> shift) & 0xFF);
-        const uint64_t dst = offs[uint64_t(d) * ntiles + blockIdx.x] + srank[s];
+        const unsigned dg = static_cast<unsigned>((key >> shift) & 0xFF);
+        const uint64_t dst = doff[dg] + srank[s];
         keys_out[dst] = key;
         vals_out[dst] = svals[s];
     }
@@ -152,8 +102,8 @@ void radix_sort_pairs(qe_gpu_ctx* ctx, uint64_t* keys, uint32_t* vals, uint64_t 
         radix_hist_kernel<<<ntiles, kSortBlock, 0, ctx->stream>>>(kin, n, shift, hist.as<uint32_t>(), ntiles);
         QE_HIP(hipGetLastError());
         exclusive_scan_u32(ctx, hist.as<uint32_t>(), offs.as<uint64_t>(), ntiles * kRadix);
-        radix_scatter_kernel<<<ntiles, kSortBlock, 0, ctx->stream>>>(kin, vin, n, shift, offs.as<uint64_t>(), ntiles,
-                                                                     kout, vout);
+        radix_scatter_kernel<<<tiles_per_die(ntiles) * kTileDies, kSortBlock, 0, ctx->stream>>>(
+            kin, vin, n, shift, offs.as<uint64_t>(), ntiles, kout, vout);
         QE_HIP(hipGetLastError());
         std::swap(kin, kout);
         std::swap(vin, vout);

@@ -544,6 +544,23 @@ void GpuAsofJoin::open() {
         slot = std::move(owner);
     }
     qe_dev_asof* idx = static_cast<qe_dev_asof*>(slot.get());
+    // A learned probe over an index whose bucket lines outgrow what the GPU's
+    // address translation covers (about 2 GiB) first groups its probe rows by
+    // the index region they read (qe_gpu_asof_cluster): the join then emits in
+    // that order, input order within a group.
+    if (method_ == tsx::AsofMethod::Learned && pcols_->rows >= (size_t(1) << 22)) {
+        qe_asof_stats st{};
+        QE_CALL(qe_gpu_asof_stats(ctx, idx, &st));
+        if (st.router_buckets * 128 >= (uint64_t(1) << 31)) {
+            qe_dev_table* t = nullptr;
+            QE_CALL(qe_gpu_asof_cluster(ctx, idx, pk.data(), static_cast<uint32_t>(pk.size()),
+                                        pcols_->cols.at(probe_time_), pcols_->cols.data(),
+                                        static_cast<uint32_t>(pcols_->cols.size()), &t));
+            pcols_ = table_columns(t, {pcols_});
+            pk.clear();
+            for (uint32_t k : probe_keys_) pk.push_back(pcols_->cols.at(k));
+        }
+    }
     const size_t n = pcols_->rows;
     void* match = nullptr;
     try {

@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 import _device_cases as DC
+import _device_models as DM
 import _oracle as O
 
 sys.path.insert(0, str(Path(__file__).resolve().parent.parent / "oracle"))
@@ -182,6 +183,139 @@ def test_data_lines_every_output_mode(ctx, qe):
             c.free()
     # Most buckets of this shape hold their rows inline.
     assert st["data_line_buckets"] > 0.5 * st["router_buckets"], st
+
+
+def _cluster_case(seed):
+    """Build side of 40 symbols (ids 3..42, some absent) with up to 1500 rows
+    each (under 64 pieces of 32 rows, so no router network is trained and the
+    linear router is exact to model), and probes with NULL keys and times,
+    absent symbols and times before and after each symbol's rows."""
+    r = np.random.default_rng(seed)
+    syms = np.array([s for s in range(3, 43) if s % 7 != 0])
+    bsym = np.concatenate([np.full(int(r.integers(1, 1500)), s) for s in syms])
+    bts = r.integers(0, 10**7, len(bsym)) * 3
+    n = 30011
+    psym = r.integers(0, 46, n)
+    pts = r.integers(-10**6, 3 * 10**7 + 10**6, n)
+    pkn = tuple(np.nonzero(r.random(n) < 0.02)[0])
+    ptn = tuple(np.nonzero(r.random(n) < 0.02)[0])
+    return ([col(bsym, I32), col(bts, I64)],
+            [col(psym, I32, pkn), col(pts, I64, ptn), col(r.normal(size=n), O.F64, tuple(np.nonzero(r.random(n) < 0.1)[0])),
+             col(r.integers(0, 2, n), O.BOOL), col(np.arange(n), I64)])
+
+
+def _expected_groups(bcols, pcols, B, trained):
+    """Each probe's group under the index layout: router-table entries laid out
+    by partition (R = K entries + 1 per non-empty partition), the line of a
+    probe its partition's first entry plus its linear-router bucket (the
+    device arithmetic, modelled in tests/_device_models.py); untrained, the
+    partition's first sorted row."""
+    bsym, bts = bcols[0][0].astype(np.int64), bcols[1][0]
+    kmin, kmax = int(bsym.min()), int(bsym.max())
+    segs = [np.sort(bts[bsym == k]) for k in range(kmin, kmax + 1)]
+    idx = DM.LearnedIndex(segs, B)
+    tb, seg = [], []
+    acc = acc_rows = 0
+    for m in idx.parts:
+        tb.append(acc)
+        seg.append(acc_rows)
+        acc += m["K"] + 1 if m["K"] else 0
+        acc_rows += m["n"]
+    ntab, nsorted = max(acc, 1), acc_rows
+    psym, pts = pcols[0][0].astype(np.int64), pcols[1][0]
+    pv = np.ones(len(psym), bool)
+    for c in (pcols[0], pcols[1]):
+        if c[1] is not None:
+            pv &= c[1]
+    g = np.zeros(len(psym), np.int64)
+    for j in range(len(psym)):
+        p = int(psym[j]) - kmin
+        if not pv[j] or p < 0 or p > kmax - kmin:
+            continue
+        if trained:
+            m = idx.parts[p]
+            line = tb[p]
+            if m["K"] and int(pts[j]) >= m["t0"]:
+                line += idx._bucket(m, int(pts[j]))
+            g[j] = min(line * 256 // ntab, 255)
+        else:
+            g[j] = min(seg[p] * 256 // nsorted, 255)
+    return g
+
+
+@pytest.mark.parametrize("trained", [True, False])
+def test_cluster_groups_probes_by_index_region(ctx, qe, trained):
+    # qe_gpu_asof_cluster: the probe rows reordered by the index region their
+    # probe reads (stable within a group), every column carried with its
+    # validity, and each row's match unchanged by the reordering.
+    bcols, pcols = _cluster_case(11 if trained else 12)
+    B = [ctx.upload(v, t, valid) for v, valid, t in bcols]
+    P = [ctx.upload(v, t, valid) for v, valid, t in pcols]
+    try:
+        idx = ctx.asof_build([B[0]], B[1])
+        try:
+            if trained:
+                idx.train(32, 50, 0.01)
+                assert idx.stats()["mlp_candidates"] == 0
+            modes = (qe.ASOF_BINARY, qe.ASOF_LEARNED) if trained else (qe.ASOF_BINARY,)
+            orig = [idx.probe([P[0]], P[1], mode=m) for m in modes]
+            cl = idx.cluster([P[0]], P[1], P)
+            cl2 = idx.cluster([P[0]], P[1], P)
+            try:
+                got = [c.numpy() for c in (cl.col(i) for i in range(len(P)))]
+                again = cl2.col(4).numpy()[0]
+                moved = [idx.probe([cl.col(0)], cl.col(1), mode=m) for m in modes]
+            finally:
+                cl.free()
+                cl2.free()
+        finally:
+            idx.free()
+    finally:
+        for d in P + B:
+            d.free()
+    rid = got[4][0]
+    want = np.argsort(_expected_groups(bcols, pcols, 32, trained), kind="stable")
+    assert np.array_equal(rid, want)
+    assert np.array_equal(again, rid)  # deterministic
+    for (vals, valid, _), (gv, gvalid) in zip(pcols, got):
+        assert np.array_equal(gv, vals[rid])
+        want_valid = np.ones(len(rid), bool) if valid is None else valid[rid]
+        assert np.array_equal(np.ones(len(rid), bool) if gvalid is None else gvalid, want_valid)
+    for o, m in zip(orig, moved):
+        assert np.array_equal(m, o[rid])
+
+
+def test_cluster_empty_and_tiny(ctx, qe):
+    b = [ctx.upload(np.array([1, 1, 2], np.int32), I32), ctx.upload(np.array([5, 9, 7], np.int64), I64)]
+    try:
+        idx = ctx.asof_build([b[0]], b[1])
+        try:
+            for n in (0, 1, 5):
+                pk = ctx.upload(np.array([1, 2, 9, 1, 2][:n], np.int32), I32)
+                pt = ctx.upload(np.array([6, 8, 1, 10, 3][:n], np.int64), I64)
+                try:
+                    cl = idx.cluster([pk], pt, [pk, pt])
+                    try:
+                        assert cl.rows == n
+                        ks, ts = cl.col(0).numpy()[0], cl.col(1).numpy()[0]
+                        assert sorted(zip(ks.tolist(), ts.tolist())) == sorted(
+                            zip([1, 2, 9, 1, 2][:n], [6, 8, 1, 10, 3][:n]))
+                    finally:
+                        cl.free()
+                finally:
+                    pk.free()
+                    pt.free()
+            with pytest.raises(qe.InvalidArgument):
+                short = ctx.upload(np.array([1], np.int32), I32)
+                try:
+                    idx.cluster([b[0]], b[1], [short])
+                finally:
+                    short.free()
+        finally:
+            idx.free()
+    finally:
+        for d in b:
+            d.free()
 
 
 @pytest.mark.parametrize("n", [1, 63, 4095, 4096, 4097, 100_003])
