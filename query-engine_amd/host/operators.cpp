@@ -3,6 +3,7 @@
 #include "qe_expr.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <mutex>
 #include <new>
@@ -514,6 +515,14 @@ GpuAsofJoin::GpuAsofJoin(std::unique_ptr<Operator> probe, std::unique_ptr<Operat
 
 Schema GpuAsofJoin::output_schema() const { return concat(probe_schema_, build_schema_); }
 
+static std::atomic<size_t> g_cluster_min_rows{size_t(1) << 22};
+static std::atomic<uint64_t> g_cluster_min_line_bytes{uint64_t(1) << 31};
+
+void GpuAsofJoin::set_cluster_threshold(size_t min_rows, uint64_t min_line_bytes) {
+    g_cluster_min_rows = min_rows;
+    g_cluster_min_line_bytes = min_line_bytes;
+}
+
 void GpuAsofJoin::open() {
     pcols_ = device_input(*probe_);
     bcols_ = device_input(*build_);
@@ -548,10 +557,10 @@ void GpuAsofJoin::open() {
     // address translation covers (about 2 GiB) first groups its probe rows by
     // the index region they read (qe_gpu_asof_cluster): the join then emits in
     // that order, input order within a group.
-    if (method_ == tsx::AsofMethod::Learned && pcols_->rows >= (size_t(1) << 22)) {
+    if (method_ == tsx::AsofMethod::Learned && pcols_->rows >= g_cluster_min_rows.load()) {
         qe_asof_stats st{};
         QE_CALL(qe_gpu_asof_stats(ctx, idx, &st));
-        if (st.router_buckets * 128 >= (uint64_t(1) << 31)) {
+        if (st.router_buckets * 128 >= g_cluster_min_line_bytes.load()) {
             qe_dev_table* t = nullptr;
             QE_CALL(qe_gpu_asof_cluster(ctx, idx, pk.data(), static_cast<uint32_t>(pk.size()),
                                         pcols_->cols.at(probe_time_), pcols_->cols.data(),

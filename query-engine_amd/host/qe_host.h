@@ -302,6 +302,11 @@ class GpuHashJoin : public Operator, public DeviceSource {
         hash_path_ = hash_path;
         gather_path_ = gather_path;
     }
+    // A learned probe of at least min_rows rows over at least min_line_bytes
+    // of router bucket lines first groups its rows by the index region they
+    // read (qe_gpu_asof_cluster). Defaults: 2^22 rows, 2 GiB. Process-wide,
+    // read at open(); no counterpart in the reference.
+    static void set_cluster_threshold(size_t min_rows, uint64_t min_line_bytes);
     void open() override;
     std::optional<Batch> next() override;
     void close() override;
@@ -359,6 +364,11 @@ class GpuAsofJoin : public Operator, public DeviceSource {
         hash_path_ = hash_path;
         gather_path_ = gather_path;
     }
+    // A learned probe of at least min_rows rows over at least min_line_bytes
+    // of router bucket lines first groups its rows by the index region they
+    // read (qe_gpu_asof_cluster). Defaults: 2^22 rows, 2 GiB. Process-wide,
+    // read at open(); no counterpart in the reference.
+    static void set_cluster_threshold(size_t min_rows, uint64_t min_line_bytes);
     void open() override;
     std::optional<Batch> next() override;
     void close() override;

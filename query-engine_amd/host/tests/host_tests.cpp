@@ -187,7 +187,10 @@ void test_asof_plan() {
         c.push_back(mk<int64_t>(Type::TS, {5, 10, 25, 15, 99, 50}));
         return c;
     }());
-    for (auto method : {tsx::AsofMethod::Binary, tsx::AsofMethod::Learned}) {
+    // Learned twice: as configured, and with the probe clustering forced on.
+    for (int pass = 0; pass < 3; ++pass) {
+        const auto method = pass == 0 ? tsx::AsofMethod::Binary : tsx::AsofMethod::Learned;
+        if (pass == 2) GpuAsofJoin::set_cluster_threshold(1, 0);
         plan::LowerOptions opt;
         opt.asof_method = method;
         auto op = plan::scan(probe)
@@ -201,6 +204,7 @@ void test_asof_plan() {
         std::sort(pay.begin(), pay.end());
         CHECK((pay == std::vector<double>{100, 150, 200, 300}));
     }
+    GpuAsofJoin::set_cluster_threshold(size_t(1) << 22, uint64_t(1) << 31);
 }
 
 void test_exceptions() {
@@ -662,7 +666,10 @@ void test_tick_pipeline_plan() {
         return c;
     }());
     using namespace qe::expr;
-    for (auto method : {tsx::AsofMethod::Binary, tsx::AsofMethod::Learned}) {
+    // Binary, learned, and learned with the probe clustering forced on.
+    for (int pass = 0; pass < 3; ++pass) {
+        const auto method = pass == 0 ? tsx::AsofMethod::Binary : tsx::AsofMethod::Learned;
+        GpuAsofJoin::set_cluster_threshold(pass == 2 ? 1 : size_t(1) << 22, pass == 2 ? 0 : uint64_t(1) << 31);
         plan::LowerOptions opt;
         opt.asof_method = method;
         auto op = plan::scan(trades)
@@ -694,6 +701,7 @@ void test_tick_pipeline_plan() {
         CHECK(rows.size() == 3 && rows[1] == (std::vector<std::string>{"AAA", "1|150"}) &&
               rows[2] == (std::vector<std::string>{"CCC", "1|102"}));
     }
+    GpuAsofJoin::set_cluster_threshold(size_t(1) << 22, uint64_t(1) << 31);
 }
 
 int main() {

@@ -285,6 +285,64 @@ def test_cluster_groups_probes_by_index_region(ctx, qe, trained):
         assert np.array_equal(m, o[rid])
 
 
+@pytest.mark.parametrize("shape", ["zero_keys", "hashed_i64", "str_keys"])
+def test_cluster_other_key_shapes(ctx, qe, shape):
+    # The clustering's other partition functors: no keys (one partition),
+    # hashed I64 keys (range too wide for direct partitions) and STR keys
+    # (probe dictionary remapped onto the build's). The result is a
+    # permutation of the probe rows, carrying every column, and each row's
+    # match is unchanged.
+    r = np.random.default_rng(77)
+    nb, n = 50000, 60001
+    if shape == "zero_keys":
+        bk, pk = [], []
+    elif shape == "hashed_i64":
+        keys = r.integers(-2**62, 2**62, 40)
+        bk = [col(keys[r.integers(0, 40, nb)], I64)]
+        pk = [col(np.where(r.random(n) < 0.9, keys[r.integers(0, 40, n)], 12345), I64)]
+    else:
+        names = [f"sym{i}" for i in range(30)]
+        bk = [("strs", [names[i] for i in r.integers(0, 30, nb)])]
+        pk = [("strs", [names[i] if i < 30 else "none" for i in r.integers(0, 32, n)])]
+    bts, pts = r.integers(0, 10**8, nb), r.integers(-10**6, 10**8 + 10**6, n)
+
+    def up(c):
+        if isinstance(c[0], str):  # a dictionary per side, in first-seen order
+            sc = ctx.upload_strings(c[1])
+            return sc, sc.dict
+        return ctx.upload(c[0], c[2], c[1]), None
+
+    B = [up(c) for c in bk]
+    P = [up(c) for c in pk]
+    bt, pt = ctx.upload(bts, I64), ctx.upload(pts, I64)
+    rid = ctx.upload(np.arange(n), I64)
+    try:
+        idx = ctx.asof_build([b for b, _ in B], bt)
+        try:
+            idx.train(32, 50, 0.01)
+            want = idx.probe([p for p, _ in P], pt, mode=qe.ASOF_LEARNED)
+            cl = idx.cluster([p for p, _ in P], pt, [pt, rid] + [p for p, _ in P])
+            try:
+                perm = cl.col(1).numpy()[0]
+                assert np.array_equal(np.sort(perm), np.arange(n))
+                assert np.array_equal(cl.col(0).numpy()[0], pts[perm])
+                keys = [cl.col(2 + i) for i in range(len(P))]
+                got = idx.probe(keys, cl.col(0), mode=qe.ASOF_LEARNED)
+                assert np.array_equal(got, want[perm])
+            finally:
+                cl.free()
+        finally:
+            idx.free()
+    finally:
+        for d, dd in B + P:
+            d.free()
+        for d in (bt, pt, rid):
+            d.free()
+        for _, dd in B + P:
+            if dd is not None:
+                dd.free()
+
+
 def test_cluster_empty_and_tiny(ctx, qe):
     b = [ctx.upload(np.array([1, 1, 2], np.int32), I32), ctx.upload(np.array([5, 9, 7], np.int64), I64)]
     try:
