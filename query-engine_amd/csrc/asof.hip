@@ -906,9 +906,15 @@ __global__ void __launch_bounds__(kRankBlock) cluster_scatter_kernel(const uint8
     goff[threadIdx.x] = offs[uint64_t(threadIdx.x) * ntiles + tile];  // read before the ranking's barriers
     uint32_t d[kClItems];
     uint32_t at[kClItems];
+    uint64_t v[kClItems];  // the next column's values, loaded ahead of its staging
     const uint32_t live_n = static_cast<uint32_t>(n - base < kClTile ? n - base : kClTile);
     auto live = [&](unsigned i) { return tile_row<kClItems>(i) < live_n; };
+    auto load_col = [&](int c) {
+        for (unsigned i = 0; i < kClItems; ++i)
+            v[i] = live(i) ? load_width(C.c[c].src, C.c[c].width, base + tile_row<kClItems>(i)) : 0ull;
+    };
     for (unsigned i = 0; i < kClItems; ++i) d[i] = live(i) ? grp[base + tile_row<kClItems>(i)] : 0u;
+    load_col(0);  // in flight during the ranking
     auto digit = [&](unsigned i) { return d[i]; };
     tile_digit_ranks<kClItems>(L, digit, live, at);
     for (unsigned i = 0; i < kClItems; ++i) {
@@ -926,10 +932,11 @@ __global__ void __launch_bounds__(kRankBlock) cluster_scatter_kernel(const uint8
         for (unsigned i = 0; i < kClItems; ++i) {
             if (!live(i)) continue;
             const uint64_t j = base + tile_row<kClItems>(i);
-            sval[at[i]] = load_width(col.src, col.width, j);
+            sval[at[i]] = v[i];
             if (col.dvalid && col.svalid && !((col.svalid[j >> 6] >> (j & 63)) & 1ull))
                 atomicOr(&snull[at[i] >> 5], 1u << (at[i] & 31));
         }
+        if (c + 1 < C.n) load_col(c + 1);  // in flight during this column's write-out
         __syncthreads();
         for (uint32_t s = threadIdx.x; s < live_n; s += kRankBlock) {
             const uint64_t dst = goff[sgrp[s]] + srank[s];
