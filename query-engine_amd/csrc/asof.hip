@@ -77,6 +77,7 @@ struct qe_dev_asof {
     uint32_t B = 0, kpb = 1;  // rows per piece, knots per router bucket
     uint64_t nknots = 0, ntab = 0;
     Scratch meta, knots, tab_mlp, tab_lin, params, grid;
+    Scratch route;  // RouteRec per partition, from meta (rebuilt when the routers change)
     Scratch bkt_mlp, bkt_lin;  // per router-table entry: the bucket's knots inline (BucketLine)
     double final_loss = 0, mean_knots_mlp = 0, mean_knots_lin = 0;
     double mean_knots_cand = 0;             // the MLP router in every trained partition
@@ -337,15 +338,18 @@ __device__ inline uint32_t piece_pred(int64_t tau, float slope, uint32_t cnt, in
     return static_cast<uint32_t>(p);
 }
 
-// Normalised time in [0, 1] for t >= pm.t0.
-__device__ inline float norm_x(const PartMeta& pm, int64_t t) {
+// Normalised time in [0, 1] for t >= pm.t0. M: PartMeta or RouteRec (the
+// fields t0, inv_span, R).
+template <class M>
+__device__ inline float norm_x(const M& pm, int64_t t) {
     double x = static_cast<double>(static_cast<uint64_t>(t) - static_cast<uint64_t>(pm.t0)) * pm.inv_span;
     x = x > 1.0 ? 1.0 : x;
     return static_cast<float>(x);
 }
 
 // Router output in [0, 1]: x itself (linear) or the interpolated warp grid W.
-__device__ inline float route_y(const PartMeta& pm, const float* W, bool mlp, int64_t t) {
+template <class M>
+__device__ inline float route_y(const M& pm, const float* W, bool mlp, int64_t t) {
     const float x = norm_x(pm, t);
     if (!mlp) return x;
     const float s = x * static_cast<float>(kGrid);
@@ -356,7 +360,8 @@ __device__ inline float route_y(const PartMeta& pm, const float* W, bool mlp, in
     return y < W[g + 1] ? y : W[g + 1];  // never past the next cell's start
 }
 
-__device__ inline uint32_t bucket_of(const PartMeta& pm, const float* W, bool mlp, int64_t t) {
+template <class M>
+__device__ inline uint32_t bucket_of(const M& pm, const float* W, bool mlp, int64_t t) {
     float b = floorf(route_y(pm, W, mlp, t) * static_cast<float>(pm.R));
     if (!(b >= 0.0f)) b = 0.0f;  // also maps NaN to 0
     if (b > static_cast<float>(pm.R - 1)) b = static_cast<float>(pm.R - 1);
@@ -615,9 +620,31 @@ __global__ void router_choose_kernel(PartMeta* meta, uint64_t nparts, const doub
     if (lane_id() == 0 && c) atomicAdd(chosen, static_cast<unsigned long long>(c));
 }
 
+// The fields of PartMeta that route a time to its table entry, in 32 bytes
+// (two 16-byte loads per probe row instead of four).
+struct RouteRec {
+    int64_t t0;
+    double inv_span;
+    uint64_t tb;
+    uint32_t R;
+    uint32_t flags;  // kRouteKnots: the partition has knots; kRouteMlp: the MLP router
+};
+constexpr uint32_t kRouteKnots = 1, kRouteMlp = 2;
+static_assert(sizeof(RouteRec) == 32, "two 16-byte loads");
+
+__global__ void route_recs_kernel(const PartMeta* meta, uint64_t nparts, RouteRec* out) {
+    for (uint64_t p = blockIdx.x * uint64_t(blockDim.x) + threadIdx.x; p < nparts;
+         p += uint64_t(gridDim.x) * blockDim.x) {
+        const PartMeta pm = meta[p];
+        out[p] = RouteRec{pm.t0, pm.inv_span, pm.tb, pm.R,
+                          (pm.K != 0 ? kRouteKnots : 0u) | (pm.mlp != 0 ? kRouteMlp : 0u)};
+    }
+}
+
 // ---- learned probe ---------------------------------------------------------------------
 struct LearnedArgs {
-    const PartMeta* meta;
+    const RouteRec* route;  // per partition: what routes a time to its bucket line
+    const PartMeta* meta;   // the rest, read on the knot-line path only
     const Knot* knots;
     const uint32_t* tab;
     const BucketLine* bkt;
@@ -679,8 +706,8 @@ __global__ void asof_learned_kernel(PK pk, ProbeArgs a, LearnedArgs L) {
             continue;
         }
         const int64_t t = time_value(a.time, j);
-        const PartMeta pm = L.meta[p];
-        if (pm.K == 0 || t < pm.t0) {  // before the partition's first row
+        const RouteRec rr = L.route[p];
+        if (!(rr.flags & kRouteKnots) || t < rr.t0) {  // before the partition's first row
             a.out[j] = kNoRow;
             continue;
         }
@@ -688,10 +715,10 @@ __global__ void asof_learned_kernel(PK pk, ProbeArgs a, LearnedArgs L) {
         //    The router is monotone in t and built the buckets, so the piece is
         //    the last of the bucket's records (its knots and the predecessor)
         //    with tau <= t.
-        const bool mlp = L.use_mlp && pm.mlp;
-        const uint32_t b = bucket_of(pm, L.grid + size_t(p) * (kGrid + 1), mlp, t);
+        const bool mlp = L.use_mlp && (rr.flags & kRouteMlp);
+        const uint32_t b = bucket_of(rr, L.grid + size_t(p) * (kGrid + 1), mlp, t);
         // The whole 128-byte line in eight 16-byte loads.
-        const uint4* lp = reinterpret_cast<const uint4*>(L.bkt + (pm.tb + b));
+        const uint4* lp = reinterpret_cast<const uint4*>(L.bkt + (rr.tb + b));
         uint32_t w[32];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
@@ -743,6 +770,7 @@ __global__ void asof_learned_kernel(PK pk, ProbeArgs a, LearnedArgs L) {
             a.out[j] = res;
             continue;
         }
+        const PartMeta pm = L.meta[p];
         int64_t rt[kBucketRecs];
 #pragma unroll
         for (int k = 0; k < kBucketRecs; ++k)
@@ -815,7 +843,7 @@ constexpr int kClMaxCols = 8;
 struct ClusterArgs {
     ColView time;
     uint64_t n;
-    const PartMeta* meta;  // nullptr: untrained index
+    const RouteRec* route;  // nullptr: untrained index
     const float* grid;
     const uint64_t* offs;  // partition offsets into the sorted rows
     uint64_t nsorted;      // sorted rows
@@ -848,11 +876,11 @@ __global__ void __launch_bounds__(kRankBlock) cluster_group_kernel(PK pk, Cluste
         uint64_t g = 0;
         const uint32_t p = part[i];
         if (p != kNoRow) {
-            if (a.meta) {
-                const PartMeta pm = a.meta[p];
-                uint64_t line = pm.tb;
-                if (pm.K != 0 && tv[i] >= pm.t0)
-                    line += bucket_of(pm, a.grid + size_t(p) * (kGrid + 1), pm.mlp != 0, tv[i]);
+            if (a.route) {
+                const RouteRec rr = a.route[p];
+                uint64_t line = rr.tb;
+                if ((rr.flags & kRouteKnots) && tv[i] >= rr.t0)
+                    line += bucket_of(rr, a.grid + size_t(p) * (kGrid + 1), (rr.flags & kRouteMlp) != 0, tv[i]);
                 g = line * kClGroups / a.ntab;
             } else {
                 g = a.nsorted ? a.offs[p] * kClGroups / a.nsorted : 0;
@@ -1404,6 +1432,15 @@ qe_gpu_status qe_gpu_asof_build(qe_gpu_ctx* ctx, const qe_dev_col* const* build_
 
 namespace qe_gpu {
 
+// The per-partition route records the learned probe and the clustering read.
+static void build_routes(qe_gpu_ctx* ctx, qe_dev_asof* idx, uint64_t P) {
+    idx->route = Scratch(ctx, std::max<uint64_t>(P, 1) * sizeof(RouteRec));
+    if (P == 0) return;
+    route_recs_kernel<<<grid_for(P, 256, 4096), 256, 0, ctx->stream>>>(idx->meta.as<PartMeta>(), P,
+                                                                      idx->route.as<RouteRec>());
+    QE_HIP(hipGetLastError());
+}
+
 // Times the candidate learned router (the MLP where it searches fewer knots)
 // against the linear router on a sample of the index's rows, best of three
 // launches each, and keeps the MLP only if it is at least 2% faster; else the
@@ -1425,7 +1462,8 @@ static void time_routers(qe_gpu_ctx* ctx, qe_dev_asof* idx, uint64_t P, uint64_t
     a.offs = idx->offs.as<uint64_t>();
     a.out = out.as<uint32_t>();
     const GivenPart gp{ids.as<uint32_t>()};
-    const LearnedArgs Lm{idx->meta.as<PartMeta>(), idx->knots.as<Knot>(), idx->tab_mlp.as<uint32_t>(),
+    const LearnedArgs Lm{idx->route.as<RouteRec>(), idx->meta.as<PartMeta>(), idx->knots.as<Knot>(),
+                         idx->tab_mlp.as<uint32_t>(),
                          idx->bkt_mlp.as<BucketLine>(), idx->grid.as<float>(), 1, idx->B,
                          fb.as<unsigned long long>()};
     LearnedArgs Ll = Lm;
@@ -1558,7 +1596,13 @@ qe_gpu_status qe_gpu_asof_train(qe_gpu_ctx* ctx, qe_dev_asof* idx, uint32_t piec
                 meta, tb.as<uint64_t>(), P, nt, knots, idx->tab_lin.as<uint32_t>(), idx->ts.as<int64_t>(), grid, 0,
                 piece_rows, dl, idx->bkt_lin.as<BucketLine>(), ndl.as<unsigned long long>() + 1);
             QE_HIP(hipGetLastError());
-            if (read_scalar(ctx, chosen.as<uint64_t>()) > 0) time_routers(ctx, idx, P, nk, nt);
+            build_routes(ctx, idx, P);
+            if (read_scalar(ctx, chosen.as<uint64_t>()) > 0) {
+                time_routers(ctx, idx, P, nk, nt);
+                build_routes(ctx, idx, P);  // the timing may have dropped the MLP routers
+            }
+        } else {
+            build_routes(ctx, idx, P);
         }
         std::vector<float> hl(std::max<uint64_t>(P, 1));
         std::vector<PartMeta> hm(std::max<uint64_t>(P, 1));
@@ -1647,7 +1691,7 @@ static qe_gpu_status asof_probe_codes(qe_gpu_ctx* ctx, const qe_dev_asof* idx, c
             auto* mi = const_cast<qe_dev_asof*>(idx);
             memset_async(ctx, mi->fallback.p, 0, 8);
             const bool lm = mode == QE_ASOF_LEARNED;
-            LearnedArgs L{idx->meta.as<PartMeta>(), idx->knots.as<Knot>(),
+            LearnedArgs L{idx->route.as<RouteRec>(), idx->meta.as<PartMeta>(), idx->knots.as<Knot>(),
                           lm ? idx->tab_mlp.as<uint32_t>() : idx->tab_lin.as<uint32_t>(),
                           lm ? idx->bkt_mlp.as<BucketLine>() : idx->bkt_lin.as<BucketLine>(), idx->grid.as<float>(),
                           lm ? 1 : 0, idx->B, mi->fallback.as<unsigned long long>()};
@@ -1759,7 +1803,7 @@ qe_gpu_status qe_gpu_asof_cluster(qe_gpu_ctx* ctx, const qe_dev_asof* idx, const
                 ClusterArgs a{};
                 a.time = view(probe_time);
                 a.n = n;
-                a.meta = idx->trained && idx->nknots ? idx->meta.as<PartMeta>() : nullptr;
+                if (idx->trained && idx->nknots) a.route = idx->route.as<RouteRec>();
                 a.grid = idx->grid.as<float>();
                 a.offs = idx->offs.as<uint64_t>();
                 a.nsorted = idx->n;
