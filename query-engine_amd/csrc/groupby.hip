@@ -664,8 +664,12 @@ static void run_slots(qe_gpu_ctx* ctx, uint64_t n, SlotFn fn, uint32_t S, AggPro
     if (lb <= kLdsBudget) {
         const unsigned block = 512;
         const unsigned per_cu = std::max<unsigned>(1, static_cast<unsigned>((160 * 1024) / std::max<size_t>(lb, 1)));
+        // At least kMinRowsPerBlock rows per block: each block writes a full
+        // set of partials (S slots) for the merge, which on a small input
+        // would cost more than its rows.
+        constexpr uint64_t kMinRowsPerBlock = 8192;
         unsigned blocks = static_cast<unsigned>(std::min<uint64_t>(
-            uint64_t(ctx->num_cus) * std::min(per_cu, 4u), (n + block - 1) / block));
+            uint64_t(ctx->num_cus) * std::min(per_cu, 4u), (n + kMinRowsPerBlock - 1) / kMinRowsPerBlock));
         if (blocks == 0) blocks = 1;
         prepare_f64(ctx, p, st, S, n, exact, fbuf);
         Scratch part(ctx, size_t(blocks) * part_words(S, p) * 8);
