@@ -80,12 +80,23 @@ def test_asof_full_size_every_probe(ctx, qe):
     try:
         idx = ctx.asof_build([rs], rt)
         try:
-            idx.train(64, 300, 0.003)
+            idx.train(32, 300, 0.003)  # the bench's pieces: most buckets are data lines
             res = {}
             for name, mode in (("binary", qe.ASOF_BINARY), ("learned", qe.ASOF_LEARNED),
                                ("learned_no_mlp", qe.ASOF_LEARNED_NO_MLP)):
                 res[name] = idx.probe([ls], lt, mode=mode)
                 res[name + "_sorted"] = idx.probe([ls], lt, mode=mode | qe.ASOF_OUT_SORTED)
+            # The probes clustered by index region, each carrying its row id.
+            rid = ctx.upload(np.arange(left_n, dtype=np.int64))
+            try:
+                cl = idx.cluster([ls], lt, [ls, lt, rid])
+                try:
+                    clustered = idx.probe([cl.col(0)], cl.col(1), mode=qe.ASOF_LEARNED | qe.ASOF_OUT_SORTED)
+                    perm = cl.col(2).numpy()[0]
+                finally:
+                    cl.free()
+            finally:
+                rid.free()
             st = idx.stats()
             nparts = int(st["partitions"])
             sts, rows, offs = idx.sorted(right_n, nparts)
@@ -98,8 +109,10 @@ def test_asof_full_size_every_probe(ctx, qe):
     finally:
         for c in (rs, rt, rp, ls, lt):
             c.free()
-    _say(f"as-of: {nparts} partitions, fallbacks {st['fallbacks']}")
+    _say(f"as-of: {nparts} partitions, fallbacks {st['fallbacks']}, data lines {st['data_line_buckets']} of "
+         f"{st['router_buckets']} buckets")
     assert st["fallbacks"] == 0
+    assert st["data_line_buckets"] > 0.5 * st["router_buckets"]
     smin = int(rsy.min())
     assert nparts == int(rsy.max()) - smin + 1  # one direct partition per symbol value
 
@@ -145,3 +158,7 @@ def test_asof_full_size_every_probe(ctx, qe):
         # Sorted-position outputs are the CPU search's own indices.
         bad = np.flatnonzero(got != (cpu if name.endswith("_sorted") else want))
         assert bad.size == 0, f"{name}: {bad.size} probes differ, first at {bad[:5].tolist()}"
+    # Clustered: a permutation of the probes, each with its own row's match.
+    assert np.array_equal(np.bincount(perm, minlength=left_n), np.ones(left_n, dtype=np.int64))
+    bad = np.flatnonzero(clustered != cpu[perm])
+    assert bad.size == 0, f"clustered: {bad.size} probes differ, first at {bad[:5].tolist()}"
