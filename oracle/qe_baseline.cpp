@@ -29,7 +29,7 @@
 extern "C" {
 
 // Dense single-key I64 group-by of F64 values with validity:
-// COUNT(*), SUM(v), AVG(v) (the config-1 query shape).
+// COUNT(*), SUM(v), AVG(v) (the 1M-row group-by's query shape).
 // out arrays are sized by the caller to max_groups; returns #groups or -1.
 int64_t orc_bl_groupby_i64_f64(const int64_t* key, const double* val,
                                const uint64_t* val_validity, uint64_t n,
@@ -86,7 +86,7 @@ int64_t orc_bl_groupby_i64_f64(const int64_t* key, const double* val,
 }
 
 // Inner join on a unique-or-duplicate I64 build key, then GROUP BY a build-side
-// I32 column: SUM(probe F64 payload), COUNT(*) (the config-2 query shape).
+// I32 column: SUM(probe F64 payload), COUNT(*) (the join -> group-by workload's query shape).
 // Build: open-addressing table key -> build row list head (shared read-only).
 int64_t orc_bl_join_groupby(const int64_t* pkey, const double* ppay, uint64_t np,
                             const int64_t* bkey, const int32_t* bgrp, uint64_t nb,

@@ -4,8 +4,8 @@ Every finite summed F64 value is taken as an integer at a per-column scale and
 added with integer atomics, so the result cannot depend on the order in which
 rows, waves or blocks add (the reference folds in row order,
 ops/aggregate.cpp:128-131). Checked here:
-  * the same query three times gives the same bits (fused join -> group-by at
-    the config-2 shape scaled to 2^22 probe rows, the group-by, the batched
+  * the same query three times gives the same bits (the fused join ->
+    group-by workload scaled to 2^22 probe rows, the group-by, the batched
     group-by), and every run is within 1e-12 of the Neumaier oracle;
   * NaN and infinities follow the row-order fold (NaN if any NaN or both
     infinities, else the infinity), also when a finite sum overflows;

@@ -904,7 +904,7 @@ def test_morsel_baseline_equals_oracle(rng):
     groups, COUNTs exactly and F64 sums within 1e-12, for several thread
     counts and morsel sizes; the as-of leg equals numpy searchsorted."""
     L = O.oracle()
-    # config-1 shape (scaled): I64 key in [0, 1024), F64 values with NULLs
+    # the 1M-row group-by's shape (scaled): I64 key in [0, 1024), F64 values with NULLs
     n = 50_000
     key = rng.integers(0, 1024, n).astype(np.int64)
     val = rng.uniform(0, 1000, n)
@@ -923,7 +923,7 @@ def test_morsel_baseline_equals_oracle(rng):
             c, s, nn = want[int(ok[i])]
             assert oc[i] == c and onn[i] == nn
             assert abs(osum[i] - s) <= 1e-12 * abs(s) + 1e-12
-    # config-2 shape (scaled): duplicate build keys fan out, 10% probe misses
+    # the join -> group-by workload's shape (scaled): duplicate build keys fan out, 10% probe misses
     nb, npr = 3000, 40_000
     bkey = rng.permutation(nb).astype(np.int64) % 2500  # 500 duplicated keys
     bgrp = rng.integers(0, 100, nb).astype(np.int32)
