@@ -209,13 +209,13 @@ __device__ inline uint64_t upper_bound_count(const int64_t* ts, uint64_t lo, uin
 struct ZeroKeyPart {
     __device__ uint32_t part(uint64_t) const { return 0; }
 };
-struct DirectKeyPart {  // one integer key, partition = key - kmin
+struct DirectKeyPart {  // one integer key, partition = key - kmin (probe keys: streamed reads)
     ColView key;
     int64_t kmin;
     uint64_t range;
     __device__ uint32_t part(uint64_t r) const {
         if (!is_valid(key, r)) return kNoRow;
-        const uint64_t off = static_cast<uint64_t>(int_value(key, r) - kmin);
+        const uint64_t off = static_cast<uint64_t>(int_value_stream(key, r) - kmin);
         return off < range ? static_cast<uint32_t>(off) : kNoRow;
     }
 };
@@ -243,7 +243,7 @@ __device__ inline void finish_match(const ProbeArgs& a, uint64_t j, int64_t t, u
         const uint64_t span = static_cast<uint64_t>(t) - static_cast<uint64_t>(a.ts[m]);
         if (!a.has_tol || span <= a.tol) res = a.sorted_out ? static_cast<uint32_t>(m) : a.row[m];
     }
-    a.out[j] = res;
+    __builtin_nontemporal_store(res, &a.out[j]);
 }
 
 template <class PK>
@@ -252,7 +252,7 @@ __global__ void asof_binary_kernel(PK pk, ProbeArgs a) {
          j += uint64_t(gridDim.x) * blockDim.x) {
         const uint32_t p = is_valid(a.time, j) ? pk.part(j) : kNoRow;
         if (p == kNoRow) {
-            a.out[j] = kNoRow;
+            __builtin_nontemporal_store(kNoRow, &a.out[j]);
             continue;
         }
         const int64_t t = time_value(a.time, j);
@@ -702,13 +702,13 @@ __global__ void asof_learned_kernel(PK pk, ProbeArgs a, LearnedArgs L) {
          j += uint64_t(gridDim.x) * blockDim.x) {
         const uint32_t p = is_valid(a.time, j) ? pk.part(j) : kNoRow;
         if (p == kNoRow) {
-            a.out[j] = kNoRow;
+            __builtin_nontemporal_store(kNoRow, &a.out[j]);
             continue;
         }
-        const int64_t t = time_value(a.time, j);
+        const int64_t t = int_value_stream(a.time, j);
         const RouteRec rr = L.route[p];
         if (!(rr.flags & kRouteKnots) || t < rr.t0) {  // before the partition's first row
-            a.out[j] = kNoRow;
+            __builtin_nontemporal_store(kNoRow, &a.out[j]);
             continue;
         }
         // 1. route to a bucket and locate the piece (tau[piece] <= t < tau[piece+1]).
@@ -762,12 +762,12 @@ __global__ void asof_learned_kernel(PK pk, ProbeArgs a, LearnedArgs L) {
             if (a.has_tol) {
                 const int64_t tm = c ? static_cast<int64_t>(static_cast<uint64_t>(base) + dsel) : a.ts[m];
                 if (static_cast<uint64_t>(t) - static_cast<uint64_t>(tm) > a.tol) {
-                    a.out[j] = kNoRow;
+                    __builtin_nontemporal_store(kNoRow, &a.out[j]);
                     continue;
                 }
             }
             if (!a.sorted_out) res = a.row[m];
-            a.out[j] = res;
+            __builtin_nontemporal_store(res, &a.out[j]);
             continue;
         }
         const PartMeta pm = L.meta[p];

@@ -267,6 +267,19 @@ __device__ inline int64_t int_value(const ColView& c, uint64_t r) {
             return static_cast<const int64_t*>(c.data)[r];
     }
 }
+// int_value for a column read once, in order (a probe side): non-temporal
+// loads, which leave the caches to data that is reused.
+__device__ inline int64_t int_value_stream(const ColView& c, uint64_t r) {
+    switch (c.type) {
+        case QE_I32:
+        case QE_STR:
+            return __builtin_nontemporal_load(static_cast<const int32_t*>(c.data) + r);
+        case QE_BOOL:
+            return __builtin_nontemporal_load(static_cast<const uint8_t*>(c.data) + r);
+        default:
+            return __builtin_nontemporal_load(static_cast<const int64_t*>(c.data) + r);
+    }
+}
 __device__ inline double f64_value(const ColView& c, uint64_t r) {
     return static_cast<const double*>(c.data)[r];
 }
