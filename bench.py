@@ -387,6 +387,11 @@ def batched_workload(qe, ctx, args):
         total_rows += n
     plan = ctx.batch(queries)
     res = {"queries": nq, "morsels": nq * per, "rows": total_rows}
+    strat = {}
+    for (name, _), (_k, _v, offs, *_rest) in zip(plan.strategies(), queries):
+        q_, r_ = strat.get(name, (0, 0))
+        strat[name] = (q_ + 1, r_ + int(offs[-1]))
+    res["strategies"] = {name: {"queries": q_, "rows": r_} for name, (q_, r_) in sorted(strat.items())}
     key = (f"batched_groupby/{ctx.device_info()['arch'].split(':')[0]}/queries{nq}/morsels{nq * per}/"
            f"max2^{args.batch_max_log2}")
 

@@ -473,6 +473,19 @@ qe_gpu_status qe_gpu_batch_run(qe_gpu_ctx* ctx, qe_batch_plan* plan, const qe_ba
                                const qe_batch_tuning* tuning, qe_dev_table** outs);
 /* Times tile size x blocks per CU x LDS cache on this batch; returns the fastest. */
 qe_gpu_status qe_gpu_batch_autotune(qe_gpu_ctx* ctx, qe_batch_plan* plan, qe_batch_tuning* out);
+/* The strategy prepare chose for each query (kinds[nq]: QE_BATCH_SMALL, _PART,
+ * _PART_PASS or _HASH) and its partition count (nparts[nq], 0 unless
+ * partitioned; may be NULL). Host-side only, no device work. No reference
+ * counterpart: the reference's per-query driver has one strategy (a hash map
+ * per thread, ops/aggregate.cpp); this reports which of the device's the
+ * batch uses. */
+typedef enum qe_batch_kind {
+    QE_BATCH_SMALL = 0,     /* dense LDS table per tile                                  */
+    QE_BATCH_PART = 1,      /* dense range: histogram, scatter by partition, LDS per part */
+    QE_BATCH_HASH = 2,      /* device hash table                                          */
+    QE_BATCH_PART_PASS = 3  /* dense range of <= 2 partitions, folded from the columns    */
+} qe_batch_kind;
+qe_gpu_status qe_gpu_batch_strategies(qe_gpu_ctx* ctx, const qe_batch_plan* plan, int32_t* kinds, uint32_t* nparts);
 void qe_gpu_batch_free(qe_gpu_ctx* ctx, qe_batch_plan* plan);
 /* Uniform values in [lo, hi) (integers floored) with Bernoulli(null_frac)
  * NULLs; a pure function of (seed, stream, row). */

@@ -152,6 +152,7 @@ _SIGS = {
     "qe_gpu_batch_prepare": (C.c_int, [P, P, C.c_uint32, C.POINTER(P)]),
     "qe_gpu_batch_run": (C.c_int, [P, P, P, C.c_int32, P, P]),
     "qe_gpu_batch_autotune": (C.c_int, [P, P, P]),
+    "qe_gpu_batch_strategies": (C.c_int, [P, P, P, P]),
     "qe_gpu_batch_free": (None, [P, P]),
     "qe_gpu_gen_uniform": (C.c_int, [P, C.c_uint64, C.c_uint64, P, C.c_double, C.c_double, C.c_double]),
     "qe_gpu_malloc": (C.c_int, [P, C.c_uint64, C.POINTER(P)]),
@@ -817,6 +818,7 @@ TUNE_JOIN_PAIRS, TUNE_ASOF_BUCKET_KNOTS, TUNE_JOIN_SPILL_CAP, TUNE_JOIN_BLOCKS_P
 TUNE_ASOF_DATA_LINES = 4
 CMP_LT, CMP_LE, CMP_GT, CMP_GE, CMP_EQ, CMP_NE = range(6)
 BATCH_PERSISTENT, BATCH_PER_BATCH, BATCH_GRAPH = 0, 1, 2
+BATCH_KINDS = {0: "small", 1: "part", 2: "hash", 3: "part_pass"}
 
 
 class BatchPlan:
@@ -848,6 +850,13 @@ class BatchPlan:
         t = BatchTuning()
         self.ctx._check(lib().qe_gpu_batch_autotune(self.ctx.h, self.h, C.byref(t)))
         return (t.tile_rows, t.blocks_per_cu, t.lds_slots)
+
+    def strategies(self):
+        """Per query: (strategy name, partitions), the strategy prepare chose."""
+        kinds = (C.c_int32 * self.nq)()
+        nparts = (C.c_uint32 * self.nq)()
+        self.ctx._check(lib().qe_gpu_batch_strategies(self.ctx.h, self.h, kinds, nparts))
+        return [(BATCH_KINDS[kinds[i]], nparts[i]) for i in range(self.nq)]
 
     def free(self):
         if self.h:

@@ -14,7 +14,10 @@
 //            a scatter pass), then each partition is aggregated in LDS by one
 //            block (its slots' final state written with plain stores) or, when
 //            the query's partitions are large, by `chunks` blocks that each take
-//            an equal slice of it and fold their tables in with atomics;
+//            an equal slice of it and fold their tables in with atomics; a
+//            query of at most kPassMaxParts partitions skips the histogram
+//            and scatter: its blocks each take a slice of the input rows and
+//            one partition, and fold that partition's rows of the slice;
 //  * hash  : anything else (sparse ranges, aggregates over the key); rows probe
 //            the shared device hash table, with an optional LDS slot cache.
 // Each phase pulls work from one device queue shared by all queries; every
@@ -44,6 +47,13 @@ constexpr uint32_t kMaxParts = 8192;     // partitions of one query
 constexpr uint32_t kPartAggThreads = 1024;
 constexpr uint64_t kChunkEntries = 1 << 16;  // expected entries per partition-aggregation block
 constexpr uint32_t kSmallTileFactor = 4;     // small-kind tiles are this many times longer
+// Partitioned queries of at most kPassMaxParts partitions skip the histogram
+// and scatter: each partition's blocks read the key and value columns (every
+// row, once per partition) and fold the partition's rows. Per row that is
+// kPassMaxParts x (key + value) bytes, against key (histogram) + key + value
+// (scatter) + 12 written + 12 read back through the entries.
+constexpr uint32_t kPassMaxParts = 2;
+constexpr uint64_t kPassChunkRows = uint64_t(1) << 19;  // input rows per aggregation block of a pass query
 
 enum BatchKind : int32_t { BK_SMALL = 0, BK_PART = 1, BK_HASH = 2 };
 
@@ -65,6 +75,7 @@ struct QueryDev {
     uint32_t chunks;  // part: blocks per partition (slices of equal length)
     int32_t on_value; // every aggregate reads the value column (or is COUNT(*))
     int32_t kw, vw;   // key / value element width in bytes (1, 4 or 8)
+    int32_t pass;     // part: aggregated straight from the key and value columns, no scatter
 };
 
 struct Tile {
@@ -407,7 +418,7 @@ __device__ inline void small_tile(const QueryDev& q, const Tile& tile, char* sme
 // order, so consecutive lanes store consecutive entries of one partition's
 // range instead of one entry in each of up to 64 ranges.
 // LDS: gbase u64[np] | sval u64[S] | skey u32[S] | cnt u32[np] | ws u32[kMaxWaves + 1] | spart u16[S].
-constexpr int kStageUnroll = 2 * kRowUnroll;               // rows per thread per step
+constexpr int kStageUnroll = 4 * kRowUnroll;               // rows per thread per step
 constexpr uint32_t kStageRows = kStageUnroll * 256;        // rows per step (256-thread blocks)
 constexpr uint32_t kStageMaxParts = 1024;                  // partitions of a staged query (block scan: 4 per thread)
 
@@ -600,6 +611,30 @@ __global__ void __launch_bounds__(256) batch_tiles_kernel(const QueryDev* __rest
     }
 }
 
+// Pass queries: one block folds the rows [b, e) of the input whose slot falls
+// in partition `part` (LDS slots [0, cnt) hold state slots lo + s).
+struct PassRows {
+    const QueryDev& q;
+    uint32_t part;
+    uint64_t lo;
+    uint32_t cnt;
+    uint64_t b, e;
+    char* smem;
+    template <int KW, int VW>
+    __device__ void run() const {
+        const uint32_t pbits = q.pbits, mask = (1u << pbits) - 1;
+        for (uint64_t r0 = b; r0 < e; r0 += kRowUnroll * uint64_t(blockDim.x)) {
+            uint32_t slot[kRowUnroll], valid[kRowUnroll];
+            uint64_t bits[kRowUnroll];
+            load_rows<KW, VW, true>(q, r0, e, slot, bits, valid);
+#pragma unroll
+            for (int u = 0; u < kRowUnroll; ++u)
+                if (slot[u] != kNoSlot && (slot[u] >> pbits) == part)
+                    fold_value(smem, cnt, q.p, slot[u] & mask, bits[u], valid[u], lo);
+        }
+    }
+};
+
 // Part queries, pass 2: one block per (query, partition) aggregates the
 // partition's entries in LDS and writes its slots' final state.
 __global__ void __launch_bounds__(kPartAggThreads) batch_part_agg_kernel(const QueryDev* __restrict__ qs,
@@ -620,6 +655,18 @@ __global__ void __launch_bounds__(kPartAggThreads) batch_part_agg_kernel(const Q
         const uint32_t cnt = static_cast<uint32_t>(min<uint64_t>(uint64_t(1) << q.pbits, q.S - lo));
         lds_init(smem, cnt, q.p);
         __syncthreads();
+        if (q.pass) {
+            const uint64_t b = q.rows * it.chunk / q.chunks, e = q.rows * (it.chunk + 1) / q.chunks;
+            with_widths(q, PassRows{q, it.part, lo, cnt, b, e, smem});
+            __syncthreads();
+            if (q.chunks == 1) {
+                for (uint32_t s = threadIdx.x; s < cnt; s += blockDim.x) flush_slot_store(smem, cnt, q.p, s, q.st, q.S, lo + s);
+            } else {
+                for (uint32_t s = threadIdx.x; s < cnt; s += blockDim.x) flush_slot_atomic(smem, cnt, q.p, s, q.st, q.S, lo + s);
+            }
+            __syncthreads();
+            continue;
+        }
         const uint64_t pb = offs[q.hbase + uint64_t(it.part) * q.ntiles];
         const uint64_t pe = (it.part + 1 < q.nparts) ? offs[q.hbase + uint64_t(it.part + 1) * q.ntiles] : q.ebase + q.rows;
         const uint64_t b = pb + (pe - pb) * it.chunk / q.chunks, e = pb + (pe - pb) * (it.chunk + 1) / q.chunks;
@@ -755,7 +802,12 @@ static void build_batch_plan(qe_gpu_ctx* ctx, BatchPlan& bp, const qe_batch_quer
                     uint32_t pb = 16;
                     while (pb > 8 && lds_bytes(1u << pb, d.p) > kPartLds) --pb;
                     const uint64_t np64 = (S + (uint64_t(1) << pb) - 1) >> pb;
-                    if (lds_bytes(1u << pb, d.p) <= kPartLds && np64 <= kMaxParts) {
+                    if (lds_bytes(1u << pb, d.p) <= kPartLds && np64 <= kPassMaxParts) {
+                        d.kind = BK_PART;
+                        d.pbits = pb;
+                        d.nparts = static_cast<uint32_t>(np64);
+                        d.pass = 1;
+                    } else if (lds_bytes(1u << pb, d.p) <= kPartLds && np64 <= kMaxParts) {
                         d.kind = BK_PART;
                         d.pbits = pb;
                         d.nparts = static_cast<uint32_t>(np64);
@@ -815,7 +867,7 @@ static void make_layout(qe_gpu_ctx* ctx, BatchPlan& bp, uint32_t tile_rows) {
     for (size_t k = 0;; ++k) {
         bool any = false;
         for (uint32_t q = 0; q < bp.nq; ++q)
-            if (k < per[q].size()) {
+            if (k < per[q].size() && !bp.hq[q].pass) {
                 kt[bp.hq[q].kind].push_back(per[q][k]);
                 if (bp.hq[q].kind == BK_PART) ht.push_back(per[q][k]);
                 any = true;
@@ -823,10 +875,17 @@ static void make_layout(qe_gpu_ctx* ctx, BatchPlan& bp, uint32_t tile_rows) {
         if (!any) break;
     }
     uint64_t hbase = 0, ebase = 0;
-    std::vector<std::vector<PartItem>> qitems(bp.nq);
+    std::vector<std::vector<PartItem>> qitems(bp.nq), pitems(bp.nq);
     for (uint32_t q = 0; q < bp.nq; ++q) {
         QueryDev& d = bp.hq[q];
         if (d.kind != BK_PART) continue;
+        if (d.pass) {  // row slices of the input; a slice's partitions adjacent, so later passes find it in cache
+            d.ntiles = 0;
+            d.chunks = static_cast<uint32_t>(std::max<uint64_t>(1, (d.rows + kPassChunkRows - 1) / kPassChunkRows));
+            for (uint32_t c = 0; c < d.chunks; ++c)
+                for (uint32_t p = 0; p < d.nparts; ++p) pitems[q].push_back(PartItem{q, p, c, 0});
+            continue;
+        }
         d.ntiles = static_cast<uint32_t>(per[q].size());
         d.hbase = hbase;
         d.ebase = ebase;
@@ -838,16 +897,18 @@ static void make_layout(qe_gpu_ctx* ctx, BatchPlan& bp, uint32_t tile_rows) {
         for (uint32_t p = 0; p < d.nparts; ++p)
             for (uint32_t c = 0; c < d.chunks; ++c) qitems[q].push_back(PartItem{q, p, c, 0});
     }
+    // Pass items first: they are the longest, and run while the short items fill in behind.
     std::vector<PartItem> it;
-    for (size_t k = 0;; ++k) {
-        bool any = false;
-        for (uint32_t q = 0; q < bp.nq; ++q)
-            if (k < qitems[q].size()) {
-                it.push_back(qitems[q][k]);
-                any = true;
-            }
-        if (!any) break;
-    }
+    for (auto* lists : {&pitems, &qitems})
+        for (size_t k = 0;; ++k) {
+            bool any = false;
+            for (uint32_t q = 0; q < bp.nq; ++q)
+                if (k < (*lists)[q].size()) {
+                    it.push_back((*lists)[q][k]);
+                    any = true;
+                }
+            if (!any) break;
+        }
     for (int k = 0; k < 3; ++k) bp.n_ktiles[k] = kt[k].size();
     bp.n_htiles = ht.size();
     bp.n_items = it.size();
@@ -1032,6 +1093,20 @@ qe_gpu_status qe_gpu_batch_autotune(qe_gpu_ctx* ctx, qe_batch_plan* plan, qe_bat
                     }
                 }
         *out = bt;
+    });
+}
+
+qe_gpu_status qe_gpu_batch_strategies(qe_gpu_ctx* ctx, const qe_batch_plan* plan, int32_t* kinds, uint32_t* nparts) {
+    QE_ENTRY(ctx, {
+        invalid(plan != nullptr && kinds != nullptr, "null argument");
+        for (uint32_t q = 0; q < plan->bp.nq; ++q) {
+            const QueryDev& d = plan->bp.hq[q];
+            kinds[q] = d.kind == BK_PART && d.pass ? QE_BATCH_PART_PASS
+                       : d.kind == BK_PART         ? QE_BATCH_PART
+                       : d.kind == BK_SMALL        ? QE_BATCH_SMALL
+                                                   : QE_BATCH_HASH;
+            if (nparts) nparts[q] = d.kind == BK_PART ? d.nparts : 0;
+        }
     });
 }
 

@@ -200,3 +200,49 @@ def test_partitioned_staged_and_direct_scatter(ctx, qe):
         for k, v, *_ in queries:
             k.free()
             v.free()
+
+
+def test_partitioned_pass_without_scatter(ctx, qe):
+    # Dense ranges of one or two LDS partitions (4096 slots each at these
+    # aggregates) are folded straight from the key and value columns, one
+    # block per (row slice, partition): no histogram, no scatter. Slices of
+    # 2^19 rows: the 1.5M-row queries take 3 slices whose tables fold in with
+    # atomics, the short ones one slice written with plain stores. A
+    # three-partition query beside them keeps the scatter. Nullable keys and
+    # values, I32 and I64 keys, F64 and I64 values; every mode, two tile shapes.
+    r = np.random.default_rng(95)
+    cases = [  # rows, key span, key type, value type, expected strategy
+        (1_500_000, 7000, I32, F64, ("part_pass", 2)),
+        (300_000, 3000, I64, I64, ("part_pass", 1)),
+        (1_100_000, 2500, I32, F64, ("part_pass", 1)),
+        (200_000, 6000, I64, F64, ("part_pass", 2)),
+        (400_000, 11000, I64, F64, ("part", 3)),
+    ]
+    queries, hostcols = [], []
+    for n, span, kt, vt, _ in cases:
+        sizes = np.diff(np.sort(np.concatenate([[0, n], r.integers(0, n, 4)])))
+        offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
+        kv = r.integers(0, span, n).astype(O.NP[kt])
+        kvalid = r.random(n) >= 0.005
+        vv = r.normal(size=n) * 50 if vt == F64 else r.integers(-(1 << 40), 1 << 40, n).astype(np.int64)
+        vvalid = r.random(n) >= 0.02
+        aggs = [(CS, 0), (SUM, 1), (AVG, 1), (MAX, 1)]
+        k = ctx.upload(kv, kt, kvalid)
+        v = ctx.upload(vv, vt, vvalid)
+        queries.append((k, v, offs, aggs, 0))
+        hostcols.append(([(kv, kvalid, kt), (vv, vvalid, vt)], aggs))
+    expect = [O.aggregate(cols, [0], aggs) for cols, aggs in hostcols]
+    plan = ctx.batch(queries)
+    try:
+        assert plan.strategies() == [c[4] for c in cases]
+        for mode in (qe.BATCH_PERSISTENT, qe.BATCH_GRAPH, qe.BATCH_PER_BATCH):
+            for tuning in ((4096, 2, 0), (65536, 4, 0)):
+                for t, e in zip(plan.run(mode, tuning=tuning), expect):
+                    got = table_rows(t.numpy())
+                    t.free()
+                    assert_same(got, e)
+    finally:
+        plan.free()
+        for k, v, *_ in queries:
+            k.free()
+            v.free()
