@@ -51,8 +51,13 @@ constexpr uint32_t kSmallTileFactor = 4;     // small-kind tiles are this many t
 // and scatter: each partition's blocks read the key and value columns (every
 // row, once per partition) and fold the partition's rows. Per row that is
 // kPassMaxParts x (key + value) bytes, against key (histogram) + key + value
-// (scatter) + 12 written + 12 read back through the entries.
+// (scatter) + 10 written + 10 read back through the entries.
 constexpr uint32_t kPassMaxParts = 2;
+// A partitioned entry's key is 16 bits: the slot within its partition (LDS
+// holds at least 12 bytes per slot, so a 128 KiB partition has at most 2^13
+// slots) | value validity << 15.
+constexpr uint32_t kEntryValid = 15;
+constexpr uint32_t kMaxPartBits = 15;
 constexpr uint64_t kPassChunkRows = uint64_t(1) << 19;  // input rows per aggregation block of a pass query
 
 enum BatchKind : int32_t { BK_SMALL = 0, BK_PART = 1, BK_HASH = 2 };
@@ -371,7 +376,7 @@ struct ScatterRows {
     const Tile& tile;
     const uint64_t* base;
     uint32_t* cur;
-    uint32_t* ekey;
+    uint16_t* ekey;
     uint64_t* eval;
     template <int KW, int VW>
     __device__ void run() const {
@@ -386,7 +391,7 @@ struct ScatterRows {
                 if (slot[u] == kNoSlot) continue;
                 const uint32_t part = slot[u] >> pbits;
                 const uint64_t pos = base[part] + atomicAdd(&cur[part], 1u);
-                ekey[pos] = (slot[u] & mask) | (valid[u] << 31);
+                ekey[pos] = static_cast<uint16_t>((slot[u] & mask) | (valid[u] << kEntryValid));
                 eval[pos] = bits[u];
             }
         }
@@ -417,28 +422,28 @@ __device__ inline void small_tile(const QueryDev& q, const Tile& tile, char* sme
 // LDS atomic, partition offsets from a block scan), then written out in LDS
 // order, so consecutive lanes store consecutive entries of one partition's
 // range instead of one entry in each of up to 64 ranges.
-// LDS: gbase u64[np] | sval u64[S] | skey u32[S] | cnt u32[np] | ws u32[kMaxWaves + 1] | spart u16[S].
+// LDS: gbase u64[np] | sval u64[S] | skey u16[S] | cnt u32[np] | ws u32[kMaxWaves + 1] | spart u16[S].
 constexpr int kStageUnroll = 4 * kRowUnroll;               // rows per thread per step
 constexpr uint32_t kStageRows = kStageUnroll * 256;        // rows per step (256-thread blocks)
 constexpr uint32_t kStageMaxParts = 1024;                  // partitions of a staged query (block scan: 4 per thread)
 
 __host__ __device__ inline size_t stage_lds_bytes(uint32_t np) {
-    return size_t(np) * 12 + size_t(kStageRows) * 14 + 4 * (kMaxWaves + 1);
+    return size_t(np) * 12 + size_t(kStageRows) * 12 + 4 * (kMaxWaves + 1);
 }
 
 struct StagedScatterRows {
     const QueryDev& q;
     const Tile& tile;
     char* smem;
-    uint32_t* ekey;
+    uint16_t* ekey;
     uint64_t* eval;
     template <int KW, int VW>
     __device__ void run() const {
         const uint32_t np = q.nparts, pbits = q.pbits, mask = (1u << pbits) - 1;
         uint64_t* gbase = reinterpret_cast<uint64_t*>(smem);
         uint64_t* sval = gbase + np;
-        uint32_t* skey = reinterpret_cast<uint32_t*>(sval + kStageRows);
-        uint32_t* cnt = skey + kStageRows;
+        uint16_t* skey = reinterpret_cast<uint16_t*>(sval + kStageRows);
+        uint32_t* cnt = reinterpret_cast<uint32_t*>(skey + kStageRows);
         uint32_t* ws = cnt + np;
         uint16_t* spart = reinterpret_cast<uint16_t*>(ws + kMaxWaves + 1);
         for (uint64_t r0 = tile.begin; r0 < tile.end; r0 += kStageRows) {
@@ -455,7 +460,7 @@ struct StagedScatterRows {
                 if (slot[u] == kNoSlot) continue;
                 const uint32_t part = slot[u] >> pbits;
                 const uint32_t at = cnt[part] + rank[u];
-                skey[at] = (slot[u] & mask) | (valid[u] << 31);
+                skey[at] = static_cast<uint16_t>((slot[u] & mask) | (valid[u] << kEntryValid));
                 sval[at] = bits[u];
                 spart[at] = static_cast<uint16_t>(part);
             }
@@ -488,11 +493,11 @@ struct StagedScatterRows {
 
 // Part kind, scatter (staged or direct) after this tile's partition bases are read.
 __device__ inline void scatter_tile(const QueryDev& q, const Tile& tile, char* smem, const uint64_t* offs,
-                                    uint32_t* ekey, uint64_t* eval) {
+                                    uint16_t* ekey, uint64_t* eval) {
     const uint32_t np = q.nparts;
     const bool staged = np <= kStageMaxParts;
     uint64_t* base = reinterpret_cast<uint64_t*>(smem);
-    uint32_t* cur = staged ? reinterpret_cast<uint32_t*>(smem + size_t(np) * 8 + size_t(kStageRows) * 12)
+    uint32_t* cur = staged ? reinterpret_cast<uint32_t*>(smem + size_t(np) * 8 + size_t(kStageRows) * 10)
                            : reinterpret_cast<uint32_t*>(base + np);
     for (uint32_t i = threadIdx.x; i < np; i += blockDim.x) {
         base[i] = offs[q.hbase + uint64_t(i) * q.ntiles + tile.qt];
@@ -582,7 +587,7 @@ template <int KIND>
 __global__ void __launch_bounds__(256) batch_tiles_kernel(const QueryDev* __restrict__ qs,
                                                           const Tile* __restrict__ tiles, uint64_t ntiles,
                                                           unsigned long long* next, uint32_t C, const uint64_t* offs,
-                                                          uint32_t* ekey, uint64_t* eval) {
+                                                          uint16_t* ekey, uint64_t* eval) {
     extern __shared__ __align__(16) char smem[];
     __shared__ uint64_t cur;
     while (true) {
@@ -640,7 +645,7 @@ struct PassRows {
 __global__ void __launch_bounds__(kPartAggThreads) batch_part_agg_kernel(const QueryDev* __restrict__ qs,
                                                                          const PartItem* __restrict__ items,
                                                                          uint64_t nitems, unsigned long long* next,
-                                                                         const uint64_t* offs, const uint32_t* ekey,
+                                                                         const uint64_t* offs, const uint16_t* ekey,
                                                                          const uint64_t* eval) {
     extern __shared__ __align__(16) char smem[];
     __shared__ uint64_t cur;
@@ -671,7 +676,7 @@ __global__ void __launch_bounds__(kPartAggThreads) batch_part_agg_kernel(const Q
         const uint64_t pe = (it.part + 1 < q.nparts) ? offs[q.hbase + uint64_t(it.part + 1) * q.ntiles] : q.ebase + q.rows;
         const uint64_t b = pb + (pe - pb) * it.chunk / q.chunks, e = pb + (pe - pb) * (it.chunk + 1) / q.chunks;
         for (uint64_t k0 = b; k0 < e; k0 += kRowUnroll * uint64_t(blockDim.x)) {
-            uint32_t w[kRowUnroll];
+            uint16_t w[kRowUnroll];
             uint64_t bits[kRowUnroll];
 #pragma unroll
             for (int u = 0; u < kRowUnroll; ++u) {
@@ -683,7 +688,7 @@ __global__ void __launch_bounds__(kPartAggThreads) batch_part_agg_kernel(const Q
 #pragma unroll
             for (int u = 0; u < kRowUnroll; ++u) {
                 const uint64_t k = k0 + uint64_t(u) * blockDim.x + threadIdx.x;
-                if (k < e) fold_value(smem, cnt, q.p, w[u] & 0x7FFFFFFFu, bits[u], w[u] >> 31, lo);
+                if (k < e) fold_value(smem, cnt, q.p, w[u] & ((1u << kEntryValid) - 1), bits[u], w[u] >> kEntryValid, lo);
             }
         }
         __syncthreads();
@@ -799,7 +804,7 @@ static void build_batch_plan(qe_gpu_ctx* ctx, BatchPlan& bp, const qe_batch_quer
                     d.kind = BK_SMALL;
                     bp.small_lds = std::max(bp.small_lds, lds_bytes(static_cast<uint32_t>(S), d.p));
                 } else if (on_value && n < (uint64_t(1) << 32)) {  // LDS row counters are 32-bit
-                    uint32_t pb = 16;
+                    uint32_t pb = kMaxPartBits;
                     while (pb > 8 && lds_bytes(1u << pb, d.p) > kPartLds) --pb;
                     const uint64_t np64 = (S + (uint64_t(1) << pb) - 1) >> pb;
                     if (lds_bytes(1u << pb, d.p) <= kPartLds && np64 <= kPassMaxParts) {
@@ -920,7 +925,7 @@ static void make_layout(qe_gpu_ctx* ctx, BatchPlan& bp, uint32_t tile_rows) {
     bp.hist = Scratch(ctx, std::max<uint64_t>(hbase, 1) * 4);
     bp.offs = Scratch(ctx, std::max<uint64_t>(hbase, 1) * 8);
     bp.sums = Scratch(ctx, scan_sums_words(std::max<uint64_t>(hbase, 1)) * 8);
-    bp.ekey = Scratch(ctx, std::max<uint64_t>(ebase, 1) * 4);
+    bp.ekey = Scratch(ctx, std::max<uint64_t>(ebase, 1) * 2);
     bp.eval = Scratch(ctx, std::max<uint64_t>(ebase, 1) * 8);
     for (int k = 0; k < 3; ++k)
         if (!kt[k].empty())
@@ -953,18 +958,18 @@ static void enqueue_persistent(qe_gpu_ctx* ctx, BatchPlan& bp, uint32_t blocks, 
     if (bp.n_ktiles[BK_SMALL])
         batch_tiles_kernel<BK_SMALL><<<blocks, 256, std::max<size_t>(bp.small_lds, 16), ctx->stream>>>(
             bp.dq.as<QueryDev>(), bp.ktiles[BK_SMALL].as<Tile>(), bp.n_ktiles[BK_SMALL], next + 1, C, offs,
-            bp.ekey.as<uint32_t>(), bp.eval.as<uint64_t>());
+            bp.ekey.as<uint16_t>(), bp.eval.as<uint64_t>());
     if (bp.n_ktiles[BK_PART])
         batch_tiles_kernel<BK_PART><<<blocks, 256, bp.part_lds, ctx->stream>>>(
             bp.dq.as<QueryDev>(), bp.ktiles[BK_PART].as<Tile>(), bp.n_ktiles[BK_PART], next + 2, C, offs,
-            bp.ekey.as<uint32_t>(), bp.eval.as<uint64_t>());
+            bp.ekey.as<uint16_t>(), bp.eval.as<uint64_t>());
     if (bp.n_ktiles[BK_HASH])
         batch_tiles_kernel<BK_HASH><<<blocks, 256, std::max<size_t>(cache_bytes(C), 16), ctx->stream>>>(
             bp.dq.as<QueryDev>(), bp.ktiles[BK_HASH].as<Tile>(), bp.n_ktiles[BK_HASH], next + 3, C, offs,
-            bp.ekey.as<uint32_t>(), bp.eval.as<uint64_t>());
+            bp.ekey.as<uint16_t>(), bp.eval.as<uint64_t>());
     if (bp.n_items)
         batch_part_agg_kernel<<<ctx->num_cus, kPartAggThreads, kPartLds, ctx->stream>>>(
-            bp.dq.as<QueryDev>(), bp.items.as<PartItem>(), bp.n_items, next + 4, offs, bp.ekey.as<uint32_t>(),
+            bp.dq.as<QueryDev>(), bp.items.as<PartItem>(), bp.n_items, next + 4, offs, bp.ekey.as<uint16_t>(),
             bp.eval.as<uint64_t>());
 }
 
