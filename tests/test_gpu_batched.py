@@ -209,7 +209,8 @@ def test_partitioned_pass_without_scatter(ctx, qe):
     # 2^19 rows: the 1.5M-row queries take 3 slices whose tables fold in with
     # atomics, the short ones one slice written with plain stores. A
     # three-partition query beside them keeps the scatter. Nullable keys and
-    # values, I32 and I64 keys, F64 and I64 values; every mode, two tile shapes.
+    # values; I32, I64 and TS keys; F64, I64, I32 and BOOL values (every value
+    # width of the row loads); every mode, two tile shapes.
     r = np.random.default_rng(95)
     cases = [  # rows, key span, key type, value type, expected strategy
         (1_500_000, 7000, I32, F64, ("part_pass", 2)),
@@ -217,6 +218,8 @@ def test_partitioned_pass_without_scatter(ctx, qe):
         (1_100_000, 2500, I32, F64, ("part_pass", 1)),
         (200_000, 6000, I64, F64, ("part_pass", 2)),
         (400_000, 11000, I64, F64, ("part", 3)),
+        (600_000, 3000, O.TS, O.BOOL, ("part_pass", 1)),
+        (900_000, 7000, I32, I32, ("part_pass", 2)),
     ]
     queries, hostcols = [], []
     for n, span, kt, vt, _ in cases:
@@ -224,7 +227,12 @@ def test_partitioned_pass_without_scatter(ctx, qe):
         offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.uint64)
         kv = r.integers(0, span, n).astype(O.NP[kt])
         kvalid = r.random(n) >= 0.005
-        vv = r.normal(size=n) * 50 if vt == F64 else r.integers(-(1 << 40), 1 << 40, n).astype(np.int64)
+        if vt == F64:
+            vv = r.normal(size=n) * 50
+        elif vt == O.BOOL:
+            vv = r.integers(0, 2, n).astype(np.uint8)
+        else:
+            vv = r.integers(-(1 << 30), 1 << 30, n).astype(O.NP[vt]) * (1 << 10 if vt == I64 else 1)
         vvalid = r.random(n) >= 0.02
         aggs = [(CS, 0), (SUM, 1), (AVG, 1), (MAX, 1)]
         k = ctx.upload(kv, kt, kvalid)
