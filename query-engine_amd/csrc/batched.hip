@@ -454,7 +454,7 @@ struct StagedScatterRows {
             for (int u = 0; u < int(kStageUnroll); ++u)
                 if (slot[u] != kNoSlot) rank[u] = atomicAdd(&cnt[slot[u] >> pbits], 1u);
             __syncthreads();
-            const uint32_t total = block_exclusive_scan(cnt, np, ws);
+            const uint32_t total = block_exclusive_scan<kStageMaxParts / 256>(cnt, np, ws);
 #pragma unroll
             for (int u = 0; u < int(kStageUnroll); ++u) {
                 if (slot[u] == kNoSlot) continue;

@@ -339,14 +339,15 @@ __device__ inline uint64_t elem_bits(const void* base, uint64_t r) {
 }
 
 
-// In-place exclusive scan of a[0, n) (n <= 4 * blockDim.x) by one block;
+// In-place exclusive scan of a[0, n) (n <= PER * blockDim.x) by one block;
 // returns the total. `ws` holds kMaxWaves + 1 words of scratch.
+template <int PER = 4>
 __device__ inline uint32_t block_exclusive_scan(uint32_t* a, uint32_t n, uint32_t* ws) {
     const uint32_t t = threadIdx.x, lane = lane_id(), w = t / kWave, nw = blockDim.x / kWave;
-    uint32_t v[4], sum = 0;
+    uint32_t v[PER], sum = 0;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t i = 4 * t + k;
+    for (int k = 0; k < PER; ++k) {
+        const uint32_t i = PER * t + k;
         v[k] = i < n ? a[i] : 0u;
         sum += v[k];
     }
@@ -369,8 +370,8 @@ __device__ inline uint32_t block_exclusive_scan(uint32_t* a, uint32_t n, uint32_
     __syncthreads();
     uint32_t base = ws[w] + x - sum;
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const uint32_t i = 4 * t + k;
+    for (int k = 0; k < PER; ++k) {
+        const uint32_t i = PER * t + k;
         if (i < n) a[i] = base;
         base += v[k];
     }
